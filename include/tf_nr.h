@@ -1,0 +1,161 @@
+/*
+ * tf_nr.h -- C ABI of the MI355X implicit Newton-Raphson step for growing prism tissues.
+ *
+ * One handle owns one or more independent tissues ("segments") resident in GPU memory.
+ * Every vector crossing this boundary uses the reference's interleaved DoF order
+ * [x0, y0, z0, x1, y1, z1, ...]; element-local arrays use the reference's node-major
+ * order 3a+i.  Host pointers are read or written synchronously inside the call and
+ * never retained.  Every entry point returns TF_OK (0) or a tf_status code and never
+ * exits the process; tf_last_error() describes the last failure.
+ *
+ * Reference call sites each entry point stands in for are cited with paths relative
+ * to TissueFolding/TissueFolding/SourceCode/ of the reference tree.
+ */
+#ifndef TF_NR_H
+#define TF_NR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tf_handle tf_handle;
+
+typedef enum tf_status {
+    TF_OK = 0,
+    TF_NOT_CONVERGED = 1, /* NR hit its iteration cap (the reference prints and continues) */
+    TF_KRYLOV_STALL = 2,  /* Krylov breakdown or iteration cap without reaching tolerance    */
+    TF_NEG_JACOBIAN = 3,  /* det Fe <= 0 in some element (inverted prism)                    */
+    TF_NAN = 4,           /* NaN/Inf in the residual or the update                           */
+    TF_OOM = 5,           /* device allocation failed                                        */
+    TF_HIP_ERROR = 6,     /* any other HIP runtime error                                     */
+    TF_INVALID = 7        /* invalid argument                                                */
+} tf_status;
+
+/* Mesh and material description, copied at tf_create.
+ * Replaces the state the reference keeps in Node / Prism / ReferenceShapeBase objects:
+ * Node::Position, FixedPos, externalViscosity (Node.h:19-38), ShapeBase::NodeIds,
+ * ReferenceShape->Positions, lambda/mu/internalViscosity (ShapeBase.h:178-187),
+ * exposed apical/basal faces (Prism.cpp:2674-2711). */
+typedef struct tf_mesh_desc {
+    int32_t n_nodes;
+    int32_t n_elements;
+    const double* node_pos;       /* [3*n_nodes]  initial positions                            */
+    const int32_t* conn;          /* [6*n_elements] node ids, 0-2 basal, 3-5 apical            */
+    const double* ref_pos;        /* [18*n_elements] element reference positions, or NULL =     */
+                                  /*   gathered from node_pos                                  */
+    const double* lambda;         /* [n_elements] Lame lambda (stiffness multiplier applied)    */
+    const double* mu;             /* [n_elements] Lame mu                                       */
+    const double* eta_int;        /* [n_elements] internal viscosity, or NULL = 0               */
+    const double* eta_ext;        /* [3*n_nodes] external viscosity per DoF, or NULL = 0        */
+    const uint8_t* fixed;         /* [3*n_nodes] 1 = fixed DoF, or NULL = none                  */
+    const uint8_t* exposed;       /* [n_elements] bit0 basal face, bit1 apical face, or NULL    */
+    const double* growth_rate;    /* [3*n_elements] growth rate per second (x,y,z), or NULL     */
+    int32_t n_segments;           /* independent tissues in this handle (>= 1)                  */
+    const int32_t* node_seg_ptr;  /* [n_segments+1] contiguous node ranges, NULL if 1 segment   */
+    const int32_t* elem_seg_ptr;  /* [n_segments+1] contiguous element ranges, NULL if 1 seg.   */
+} tf_mesh_desc;
+
+enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2 };
+enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
+
+typedef struct tf_solver_opts {
+    int32_t krylov;            /* TF_KRYLOV_*                                                  */
+    int32_t krylov_max_iter;   /* per linear solve                                             */
+    double krylov_rtol;        /* ||r|| <= max(rtol*||g||, atol)                               */
+    double krylov_atol;
+    int32_t gmres_restart;     /* restart length for TF_KRYLOV_GMRES                           */
+    int32_t iters_per_sync;    /* Krylov iterations between host convergence checks            */
+    double nr_tol;             /* ||dU||_2 <= nr_tol (absolute, per tissue) -- reference 1e-8  */
+    int32_t nr_max_iter;       /* reference: 20 (stops when the count exceeds it)              */
+    int32_t tangent_precision; /* TF_PREC_*: arithmetic of the element tangent contraction     */
+    int32_t matrix_precision;  /* TF_PREC_FP64 or TF_PREC_FP32 storage of the Jacobian values  */
+    int32_t growth_rotation;   /* 1: rotate growth by the element's mean rotation (1 deg rule) */
+    int32_t use_graph;         /* 1: replay Krylov iteration chunks from a captured hipGraph   */
+} tf_solver_opts;
+
+typedef struct tf_step_stats {
+    int32_t nr_iterations;       /* max over tissues                                           */
+    int32_t krylov_iterations;   /* summed over NR iterations (max over tissues)               */
+    int32_t converged;           /* 1 when every tissue met nr_tol                             */
+    int32_t rotated_elements;    /* growth increments rotated this step                        */
+    double last_du_norm;         /* max over tissues of the last ||dU||_2                      */
+    double ms_grow, ms_assemble, ms_solve, ms_total;
+} tf_step_stats;
+
+/* ---- lifetime --------------------------------------------------------------------------- */
+/* Set-up: NewtonRaphsonSolver::NewtonRaphsonSolver (NewtonRaphsonSolver.cpp:16-36) plus the
+ * element construction of Simulation::initiateMesh (Simulation.cpp:2947-3009, Prism.cpp:9-269)
+ * and the shape-function set-up (Prism.cpp:705-728). */
+int tf_create(const tf_mesh_desc* desc, tf_handle** out);
+/* NewtonRaphsonSolver::~NewtonRaphsonSolver (NewtonRaphsonSolver.cpp:38-70). */
+void tf_destroy(tf_handle* h);
+/* Last error text of h (or of the last failed tf_create when h is NULL). */
+const char* tf_last_error(const tf_handle* h);
+void tf_default_options(tf_solver_opts* opts);
+int tf_set_options(tf_handle* h, const tf_solver_opts* opts);
+int tf_get_options(const tf_handle* h, tf_solver_opts* opts);
+int tf_get_sizes(const tf_handle* h, int32_t* n_nodes, int32_t* n_elements, int32_t* n_blocks,
+                 int32_t* n_segments);
+
+/* ---- state ------------------------------------------------------------------------------- */
+/* Node positions (Node::Position / NewtonRaphsonSolver::uk). */
+int tf_set_positions(tf_handle* h, const double* x);
+int tf_get_positions(tf_handle* h, double* x);
+/* Growth and material state (ShapeBase::Fg / InvFg, ShapeBase.cpp:1215-1251; lambda, mu). */
+int tf_set_growth_rates(tf_handle* h, const double* rate);
+int tf_set_fg(tf_handle* h, const double* Fg);
+int tf_get_fg(tf_handle* h, double* Fg, double* InvFg);
+int tf_set_material(tf_handle* h, const double* lambda, const double* mu, const double* eta_int);
+
+/* ---- per-step set-up ----------------------------------------------------------------------- */
+/* Simulation::updateGrowthRotationMatrices + calculateGrowth + ShapeBase::growShapeByFg
+ * (Simulation.cpp:5876-5909, :9651-9799; ShapeBase.cpp:396-421, :1215-1251, :1656-1811). */
+int tf_grow(tf_handle* h, double dt, int32_t* n_rotated);
+/* Simulation::updateNodeViscositySurfaces (Simulation.cpp:7563-7581). */
+int tf_update_viscosity_surfaces(tf_handle* h);
+int tf_get_viscosity_surfaces(tf_handle* h, double* Sa);
+
+/* ---- Newton-Raphson pieces (Simulation::updateStepNR, Simulation.cpp:6449-6590) ---------------- */
+/* constructUnMatrix + initialteUkMatrix (NewtonRaphsonSolver.cpp:135-145). */
+int tf_begin_step(tf_handle* h);
+/* calculateDisplacementMatrix, calculateForcesAndJacobianMatrixNR, writeForcesTogeAndgvInternal,
+ * writeImplicitElementalKToJacobian, calculateExternalViscousForcesForNR,
+ * addImplicitKViscousExternalToJacobian, calculateSumOfInternalForces, calcutateFixedK
+ * (NewtonRaphsonSolver.cpp:148-152, :237-348, :363-367): residual g and Jacobian M on device. */
+int tf_assemble(tf_handle* h, double dt);
+/* gSum (NewtonRaphsonSolver.h:35). */
+int tf_get_residual(tf_handle* h, double* g);
+/* Jacobian in natural orientation M = d(internal force)/du, which is the transpose of the
+ * reference's K (the reference solves K^T dU = g, NewtonRaphsonSolver.cpp:636-643).
+ * 3x3 block CSR: row_ptr [n_nodes+1], col [n_blocks], val [9*n_blocks] row-major in-block. */
+int tf_get_jacobian_bsr(tf_handle* h, int32_t* row_ptr, int32_t* col, double* val);
+/* Element-local arrays of one prism at the current uk/un, in the reference's layout:
+ * ElementalElasticSystemForces, ElementalInternalViscousSystemForces (ShapeBase.cpp:2198-2226),
+ * TriPointKe and TriPointKv as stored by the reference (ShapeBase.cpp:2619-2705). */
+int tf_element_dump(tf_handle* h, int32_t e, double dt, double* fe, double* fv, double* Ke, double* Kv);
+/* solveForDeltaU (NewtonRaphsonSolver.cpp:382-404): M dU = g by preconditioned Krylov. */
+int tf_solve(tf_handle* h, double* dU, int32_t* krylov_iters, double* rel_residual);
+/* checkConvergenceViaDeltaU + updateUkInIteration (NewtonRaphsonSolver.cpp:727-761). */
+int tf_update_uk(tf_handle* h, double* du_norm, int32_t* converged);
+/* The whole NR loop of one step (Simulation::updateStepNR). */
+int tf_nr_step(tf_handle* h, double dt, tf_step_stats* stats);
+/* Growth + viscosity surfaces + NR loop (the NR-adjacent part of Simulation::runOneStep,
+ * Simulation.cpp:5820-5974). */
+int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats);
+/* Per-tissue results of the last NR step: iterations, last ||dU||, converged flag. */
+int tf_get_segment_stats(tf_handle* h, int32_t* nr_iters, double* du_norm, int32_t* converged);
+
+/* ---- measurement ------------------------------------------------------------------------------ */
+/* Average device time (ms) per launch of the named kernel over the calls since the last reset,
+ * recorded with HIP events on the stream the kernel runs on.  name: "assemble_elements",
+ * "assemble_gather", "spmv". */
+int tf_kernel_timing(tf_handle* h, const char* name, int32_t enable, double* avg_ms, int64_t* launches);
+/* Algorithmic bytes of one launch of the named kernel (unique minimum traffic). */
+int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TF_NR_H */

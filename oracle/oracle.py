@@ -1,0 +1,304 @@
+"""CPU oracle for the TissueFolding Newton-Raphson path -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``oracle/_build/libtforacle.so`` (built from ``oracle/tf_oracle.c`` by ``oracle/Makefile``)
+and drives it in the reference's order (``Simulation::updateStepNR``, Simulation.cpp:6449-6590;
+``Simulation::runOneStep``, Simulation.cpp:5820-5974).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module, and only as the checker or as the timed CPU
+baseline: the product path never calls it.
+
+Linear solves follow the reference's transposed direct solve (NewtonRaphsonSolver.cpp:636-643):
+``dense``  -- dense LU with partial pivoting in C (small systems),
+``pardiso``-- Intel MKL PARDISO through dlopen (mtype 11, one refinement step, transposed solve),
+``superlu``-- SciPy's SuperLU on the CSR matrix (fallback when MKL is absent).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libtforacle.so")
+_lib = None
+
+dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+bp = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class _Model(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_elem", C.c_int), ("conn", C.c_void_p), ("Xref", C.c_void_p),
+                ("lam", C.c_void_p), ("mu", C.c_void_p), ("eta", C.c_void_p), ("eta_ext", C.c_void_p),
+                ("fixed", C.c_void_p), ("InvFg", C.c_void_p), ("Sa", C.c_void_p)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.tfo_element.argtypes = [dp, dp, C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                  dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tfo_element_energy.argtypes = [dp, dp, dp, C.c_double, C.c_double]
+        L.tfo_element_energy.restype = C.c_double
+        L.tfo_reference_geometry.argtypes = [dp, dp, dp]
+        L.tfo_shape_derivs.argtypes = [C.c_int, dp]
+        L.tfo_face_area.argtypes = [dp, dp, dp]
+        L.tfo_face_area.restype = C.c_double
+        L.tfo_viscosity_surfaces.argtypes = [C.c_int, C.c_int, ip, bp, dp, dp]
+        L.tfo_elements.argtypes = [C.POINTER(_Model), dp, dp, C.c_double, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tfo_residual.argtypes = [C.POINTER(_Model), dp, dp, C.c_double, dp, dp, dp]
+        L.tfo_dense_K.argtypes = [C.POINTER(_Model), C.c_double, dp, dp, dp]
+        L.tfo_csr_K.argtypes = [C.POINTER(_Model), C.c_double, dp, dp, lp, ip, dp]
+        L.tfo_dense_to_csr.argtypes = [C.c_int, dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tfo_dense_to_csr.restype = C.c_int64
+        L.tfo_dense_solve_transposed.argtypes = [C.c_int, dp, dp, dp]
+        L.tfo_pardiso_available.argtypes = [C.c_char_p]
+        L.tfo_pardiso_solve_transposed.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_int]
+        L.tfo_num_threads.restype = C.c_int
+        L.tfo_polar_rotation.argtypes = [dp, dp]
+        L.tfo_grow_all.argtypes = [C.c_int, ip, dp, dp, dp, C.c_double, C.c_int, dp, dp]
+        L.tfo_grow_all.restype = C.c_int
+        L.tfo_mean_F.argtypes = [dp, dp, dp]
+        L.tfo_inv3.argtypes = [dp, dp]
+        L.tfo_det3.argtypes = [dp]
+        L.tfo_det3.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# --------------------------------------------------------------------------------------------
+# element level
+# --------------------------------------------------------------------------------------------
+def element(X, x, InvFg, lam, mu, eta=0.0, dt=60.0, v=None, fixed_local=None, want_K=True):
+    """One prism in the reference's element-local layout: returns dict fe, fv, Ke, Kv, TriPointF."""
+    L = lib()
+    X, x, InvFg = _c(X).ravel(), _c(x).ravel(), _c(InvFg).ravel()
+    fe, fv = np.zeros(18), np.zeros(18)
+    Ke, Kv, TF = np.zeros(324), np.zeros(324), np.zeros(9)
+    vv = None if v is None else _c(v).ravel()
+    fl = None if fixed_local is None else _c(fixed_local, np.uint8).ravel()
+    L.tfo_element(X, x, None if vv is None else vv.ctypes.data, InvFg, lam, mu, eta, dt,
+                  None if fl is None else fl.ctypes.data, fe, fv,
+                  Ke.ctypes.data if want_K else None, Kv.ctypes.data if want_K else None, TF.ctypes.data)
+    return dict(fe=fe, fv=fv, Ke=Ke.reshape(18, 18), Kv=Kv.reshape(18, 18), TriPointF=TF.reshape(3, 3))
+
+
+def element_energy(X, x, InvFg, lam, mu):
+    return lib().tfo_element_energy(_c(X).ravel(), _c(x).ravel(), _c(InvFg).ravel(), lam, mu)
+
+
+def reference_geometry(X):
+    IJ0, D0 = np.zeros(54), np.zeros(6)
+    lib().tfo_reference_geometry(_c(X).ravel(), IJ0, D0)
+    return IJ0.reshape(6, 3, 3), D0
+
+
+def shape_derivs(g):
+    d = np.zeros(18)
+    lib().tfo_shape_derivs(g, d)
+    return d.reshape(3, 6)
+
+
+def polar_rotation(F):
+    R = np.zeros(9)
+    lib().tfo_polar_rotation(_c(F).ravel(), R)
+    return R.reshape(3, 3)
+
+
+def face_area(p0, p1, p2):
+    return lib().tfo_face_area(_c(p0), _c(p1), _c(p2))
+
+
+# --------------------------------------------------------------------------------------------
+# global level
+# --------------------------------------------------------------------------------------------
+def node_pattern(n_nodes, conn):
+    """Node-pair couplings of a prism mesh: (row_ptr (N+1,), cols) sorted, incl. diagonal."""
+    E = conn.shape[0]
+    a = np.repeat(conn, 6, axis=1).ravel()
+    b = np.tile(conn, (1, 6)).ravel()
+    key = np.unique(a.astype(np.int64) * n_nodes + b)
+    rows = key // n_nodes
+    cols = (key % n_nodes).astype(np.int32)
+    row_ptr = np.zeros(n_nodes + 1, np.int64)
+    np.add.at(row_ptr, rows + 1, 1)
+    return np.cumsum(row_ptr), cols
+
+
+def scalar_csr_pattern(n_nodes, conn):
+    """Scalar (3N) CSR pattern expanded from the node pattern, columns sorted."""
+    rp, cols = node_pattern(n_nodes, conn)
+    deg = np.diff(rp)
+    n = 3 * n_nodes
+    row_ptr = np.zeros(n + 1, np.int64)
+    row_ptr[1:] = np.cumsum(np.repeat(3 * deg, 3))
+    # scalar row 3r+i holds columns 3c+j for every node c coupled to r, j = 0..2
+    node_of_row = np.repeat(np.arange(n_nodes), 3)
+    lens = deg[node_of_row]
+    local = np.arange(row_ptr[-1]) - np.repeat(row_ptr[:-1], 3 * lens)
+    idx = np.repeat(rp[node_of_row], 3 * lens) + local // 3
+    col = (3 * cols[idx] + local % 3).astype(np.int32)
+    return row_ptr, col
+
+
+class OracleTissue:
+    """NR state of one tissue on the CPU, in the reference's order and orientation."""
+
+    def __init__(self, tissue, solver="auto", allow_rotation=True):
+        m = tissue.mesh
+        self.t = tissue
+        self.N, self.E = m.n_nodes, m.n_elements
+        self.conn = _c(m.conn, np.int32)
+        self.Xref = _c(m.xref)
+        self.lam, self.mu = _c(tissue.lam), _c(tissue.mu)
+        self.eta = _c(tissue.eta_int)
+        self.eta_ext = _c(tissue.eta_ext)
+        self.fixed = _c(tissue.fixed, np.uint8)
+        self.exposed = _c(tissue.exposed, np.uint8)
+        self.rates = _c(tissue.rates)
+        self.pos = _c(m.pos).ravel().copy()
+        self.Fg = np.tile(np.eye(3).ravel(), self.E)
+        self.InvFg = self.Fg.copy()
+        self.Sa = np.zeros(self.N)
+        self.allow_rotation = allow_rotation
+        self.solver = solver
+        self.last_rotations = 0
+        self._csr = None
+        self.stats = []
+
+    def _model(self):
+        mm = _Model()
+        mm.n_nodes, mm.n_elem = self.N, self.E
+        for name, arr in (("conn", self.conn), ("Xref", self.Xref), ("lam", self.lam), ("mu", self.mu),
+                          ("eta", self.eta), ("eta_ext", self.eta_ext), ("fixed", self.fixed),
+                          ("InvFg", self.InvFg), ("Sa", self.Sa)):
+            setattr(mm, name, arr.ctypes.data)
+        return mm
+
+    # -- per-step updates ------------------------------------------------------------------
+    def grow(self, dt):
+        self.last_rotations = lib().tfo_grow_all(self.E, self.conn, self.Xref, self.pos, self.rates, dt,
+                                                 1 if self.allow_rotation else 0, self.Fg, self.InvFg)
+        return self.last_rotations
+
+    def update_viscosity_surfaces(self):
+        lib().tfo_viscosity_surfaces(self.N, self.E, self.conn, self.exposed, self.pos, self.Sa)
+
+    # -- NR pieces ---------------------------------------------------------------------------
+    def elements(self, uk, un, dt, want_K=True):
+        fe, fv = np.zeros(18 * self.E), np.zeros(18 * self.E)
+        Ke = np.zeros(324 * self.E) if want_K else None
+        Kv = np.zeros(324 * self.E) if want_K else None
+        mm = self._model()
+        lib().tfo_elements(C.byref(mm), _c(uk), _c(un), dt, fe, fv, None if Ke is None else Ke.ctypes.data,
+                           None if Kv is None else Kv.ctypes.data, None)
+        return fe, fv, Ke, Kv
+
+    def residual(self, uk, un, dt, fe, fv):
+        g = np.zeros(3 * self.N)
+        mm = self._model()
+        lib().tfo_residual(C.byref(mm), _c(uk), _c(un), dt, fe, fv, g)
+        return g
+
+    def dense_K(self, dt, Ke, Kv):
+        n = 3 * self.N
+        K = np.zeros((n, n))
+        mm = self._model()
+        lib().tfo_dense_K(C.byref(mm), dt, Ke, Kv, K)
+        return K
+
+    def csr_K(self, dt, Ke, Kv):
+        if self._csr is None:
+            self._csr = scalar_csr_pattern(self.N, self.conn)
+        rp, col = self._csr
+        val = np.zeros(len(col))
+        mm = self._model()
+        lib().tfo_csr_K(C.byref(mm), dt, Ke, Kv, rp, col, val)
+        return rp, col, val
+
+    def assemble(self, uk, un, dt, dense=None):
+        """(K_ref, g): K in the reference orientation (dense ndarray or scipy CSR)."""
+        fe, fv, Ke, Kv = self.elements(uk, un, dt)
+        g = self.residual(uk, un, dt, fe, fv)
+        if dense is None:
+            dense = 3 * self.N <= 3000
+        if dense:
+            return self.dense_K(dt, Ke, Kv), g
+        import scipy.sparse as sp
+        rp, col, val = self.csr_K(dt, Ke, Kv)
+        return sp.csr_matrix((val, col, rp), shape=(3 * self.N, 3 * self.N)), g
+
+    def solve(self, K, g):
+        """Solve K^T dU = g (the reference's transposed solve)."""
+        import scipy.sparse as sp
+        n = len(g)
+        if isinstance(K, np.ndarray):
+            x = np.zeros(n)
+            Kc = K.copy()
+            rc = lib().tfo_dense_solve_transposed(n, Kc, _c(g), x)
+            if rc:
+                raise RuntimeError("singular Jacobian in dense oracle solve")
+            return x
+        kind = self.solver
+        if kind in ("auto", "pardiso") and lib().tfo_pardiso_available(None):
+            Kc = K.tocsr()
+            Kc.sort_indices()
+            x = np.zeros(n)
+            rc = lib().tfo_pardiso_solve_transposed(n, _c(Kc.indptr, np.int32), _c(Kc.indices, np.int32),
+                                                    _c(Kc.data), _c(g), x, lib().tfo_num_threads())
+            if rc:
+                raise RuntimeError(f"PARDISO error {rc}")
+            return x
+        if kind == "pardiso":
+            raise RuntimeError("MKL PARDISO not available")
+        import scipy.sparse.linalg as sla
+        return sla.spsolve(sp.csc_matrix(K.T), g)
+
+    def solver_kind(self):
+        if 3 * self.N <= 3000:
+            return "dense-lu"
+        if self.solver in ("auto", "pardiso") and lib().tfo_pardiso_available(None):
+            return "mkl-pardiso"
+        return "superlu"
+
+    def nr_step(self, dt, tol=1e-8, max_iter=20, dense=None):
+        """Simulation::updateStepNR: returns (iterations, last ||dU||)."""
+        un = self.pos.copy()
+        uk = un.copy()
+        it = 0
+        conv = False
+        d = np.inf
+        while not conv:
+            K, g = self.assemble(uk, un, dt, dense)
+            dU = self.solve(K, g)
+            d = float(np.linalg.norm(dU))
+            conv = d <= tol
+            uk += dU
+            it += 1
+            if not conv and it > max_iter:
+                conv = True
+        self.pos = uk
+        return it, d
+
+    def timestep(self, dt, tol=1e-8, max_iter=20, dense=None):
+        """Growth (with the rotation rule), viscosity surfaces, then the NR loop."""
+        self.grow(dt)
+        self.update_viscosity_surfaces()
+        t0 = time.perf_counter()
+        it, d = self.nr_step(dt, tol, max_iter, dense)
+        self.stats.append(dict(nr_iters=it, dU=d, seconds=time.perf_counter() - t0, rotations=self.last_rotations))
+        return it, d

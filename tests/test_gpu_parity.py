@@ -1,0 +1,189 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle (FP64).
+
+Bars: element forces and tangents and the global residual/Jacobian within 1e-10 relative
+(relative to the largest entry of the compared array), Newton updates within 1e-8, trajectories
+within 1e-8 relative, the config-1 exact answer within 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from tissuefolding_paperversion_amd import configs
+from tissuefolding_paperversion_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(tissue, **opts):
+    from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
+    return NewtonRaphsonSolver(tissue, opts or None)
+
+
+def _perturbed_state(tissue, seed, amp=0.04, growth=0.03):
+    rng = np.random.default_rng(seed)
+    X = tissue.mesh.pos.ravel().copy()
+    x = X + amp * rng.standard_normal(X.shape) * 5.0
+    Fg = np.tile(np.eye(3), (tissue.n_elements, 1, 1)) + growth * rng.standard_normal((tissue.n_elements, 3, 3))
+    return X, x, Fg
+
+
+def _small_viscous_disc(eta_int=40.0, eta_ext=80.0):
+    return configs.disc_tissue(radius=22.0, h=5.5, z_levels=(0.0, 4.0, 9.0), seed=11, eta_ext=eta_ext,
+                               eta_int=eta_int, name="small viscous disc")
+
+
+@pytest.mark.parametrize("eta_int", [0.0, 40.0])
+def test_element_dump_matches_oracle(oracle, eta_int):
+    t = _small_viscous_disc(eta_int=eta_int)
+    X, x, Fg = _perturbed_state(t, 1)
+    s = _solver(t)
+    s.positions = X
+    s.constructUnMatrix()
+    s.positions = x
+    s.set_fg(Fg)
+    dt = 60.0
+    Finv = np.linalg.inv(Fg)
+    v = (x - X) / dt
+    for e in [0, 7, t.n_elements // 2, t.n_elements - 1]:
+        c = t.mesh.conn[e]
+        got = s.element_dump(e, dt)
+        fl = t.fixed.reshape(-1, 3)[c].ravel()
+        ref = oracle.element(t.mesh.xref[e], x.reshape(-1, 3)[c].ravel(), Finv[e], t.lam[e], t.mu[e], t.eta_int[e], dt,
+                             v=v.reshape(-1, 3)[c].ravel(), fixed_local=fl)
+        assert rel_err(got["fe"], ref["fe"]) < 1e-10
+        assert rel_err(got["Ke"], ref["Ke"]) < 1e-10
+        if eta_int:
+            assert rel_err(got["fv"], ref["fv"]) < 1e-10
+            assert rel_err(got["Kv"], ref["Kv"]) < 1e-10
+        else:
+            assert np.all(got["Kv"] == 0) and np.all(got["fv"] == 0)
+
+
+@pytest.mark.parametrize("eta_int", [0.0, 40.0])
+def test_global_assembly_matches_oracle(oracle, eta_int):
+    t = _small_viscous_disc(eta_int=eta_int)
+    X, x, Fg = _perturbed_state(t, 2)
+    s = _solver(t)
+    s.positions = X
+    s.updateNodeViscositySurfaces()
+    s.constructUnMatrix()
+    s.positions = x
+    s.set_fg(Fg)
+    dt = 60.0
+    assert s.calculateForcesAndJacobianMatrixNR(dt) == L.TF_OK
+    g = s.gSum
+    M = s.jacobian_csr().toarray()
+    o = oracle.OracleTissue(t)
+    o.InvFg[:] = np.linalg.inv(Fg).ravel()
+    o.pos = X.copy()
+    o.update_viscosity_surfaces()
+    assert rel_err(s.viscosity_surfaces(), o.Sa) < 1e-13
+    K, go = o.assemble(x, X, dt, dense=True)
+    assert rel_err(g, go) < 1e-10
+    # the reference solves K^T dU = g: the natural-orientation Jacobian is K^T
+    assert rel_err(M, K.T) < 1e-10
+
+
+@pytest.mark.parametrize("krylov", [L.KRYLOV_BICGSTAB, L.KRYLOV_CG])
+def test_linear_solve_matches_direct(oracle, krylov):
+    t = _small_viscous_disc(eta_int=0.0)
+    X, x, Fg = _perturbed_state(t, 3, amp=0.01, growth=0.01)
+    s = _solver(t, krylov=krylov, krylov_rtol=1e-13)
+    s.positions = X
+    s.updateNodeViscositySurfaces()
+    s.constructUnMatrix()
+    s.positions = x
+    s.set_fg(Fg)
+    s.calculateForcesAndJacobianMatrixNR(60.0)
+    g = s.gSum
+    M = s.jacobian_csr()
+    du = s.solveForDeltaU()
+    assert s.last_krylov["status"] == L.TF_OK
+    assert np.linalg.norm(M @ du - g) <= 1e-11 * np.linalg.norm(g)
+    ref = np.linalg.solve(M.toarray(), g)
+    assert rel_err(du, ref) < 1e-8
+
+
+@pytest.mark.parametrize("krylov", [L.KRYLOV_CG, L.KRYLOV_BICGSTAB])
+def test_config1_known_answer(oracle, krylov):
+    """Uniform in-plane growth of the pinned disc: x(t) = exp(r t) X in plane, z unchanged."""
+    t = configs.config1()
+    s = _solver(t, krylov=krylov)
+    X = t.mesh.pos.copy()
+    r = t.rates[0, 0]
+    o = oracle.OracleTissue(t)
+    for step in range(1, 6):
+        st = s.runOneStep(t.dt)
+        it_o, _ = o.timestep(t.dt)
+        assert st["converged"] == 1
+        assert st["nr_iterations"] == it_o
+        exact = X.copy()
+        exact[:, :2] *= np.exp(r * t.dt * step)
+        got = s.positions.reshape(-1, 3)
+        assert rel_err(got, exact) < 1e-10
+        assert rel_err(got, o.pos.reshape(-1, 3)) < 1e-10
+
+
+def test_viscous_trajectory_matches_oracle(oracle):
+    """Heterogeneous growth, internal and external viscosity: trajectory vs the oracle."""
+    t = configs.wing_disc_tissue(a=30.0, b=20.0, h=4.0, z_levels=(0.0, 2.0, 6.0, 10.0), seed=5, growth_seed=6,
+                                 name="small ellipse")
+    t.eta_int[:] = 30.0
+    # stronger growth so that several NR iterations are needed
+    t.rates *= 20.0
+    s = _solver(t)
+    o = oracle.OracleTissue(t)
+    for step in range(3):
+        st = s.runOneStep(t.dt)
+        it_o, _ = o.timestep(t.dt)
+        assert st["converged"] == 1
+        assert abs(st["nr_iterations"] - it_o) <= 1
+        ref = o.pos
+        disp = np.max(np.abs(ref - t.mesh.pos.ravel()))
+        assert disp > 1e-3
+        assert rel_err(s.positions, ref) < 1e-8
+
+
+def test_growth_rotation_matches_oracle(oracle):
+    """A rigidly rotated tissue triggers the 1-degree growth-rotation rule; Fg must match."""
+    t = configs.wing_disc_tissue(a=30.0, b=20.0, h=4.0, z_levels=(0.0, 2.0, 6.0, 10.0), seed=5, growth_seed=6)
+    th = np.deg2rad(4.0)
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    xr = (t.mesh.pos @ R.T).ravel()
+    s = _solver(t)
+    s.positions = xr
+    n_rot = s.grow(t.dt)
+    o = oracle.OracleTissue(t)
+    o.pos = xr.copy()
+    n_rot_o = o.grow(t.dt)
+    assert n_rot == n_rot_o == t.n_elements
+    Fg, Fi = s.get_fg()
+    assert rel_err(Fg.ravel(), o.Fg) < 1e-13
+    assert rel_err(Fi.ravel(), o.InvFg) < 1e-13
+
+
+def test_segments_equal_single_handles():
+    """Independent tissues in one handle give the same bits as one handle each."""
+    ts = [configs.disc_tissue(radius=r, h=5.0, seed=20 + k, name=f"d{k}") for k, r in enumerate((12.0, 18.0, 15.0))]
+    ts[1].eta_ext[:] = 50.0
+    batch = _solver(ts)
+    for _ in range(2):
+        batch.runOneStep(60.0)
+    xb = batch.positions
+    off = 0
+    for t in ts:
+        s = _solver(t)
+        for _ in range(2):
+            s.runOneStep(60.0)
+        xs = s.positions
+        assert np.array_equal(xs, xb[off:off + 3 * t.n_nodes])
+        off += 3 * t.n_nodes
+
+
+def test_determinism():
+    t = configs.config1()
+    a, b = _solver(t), _solver(t)
+    for _ in range(2):
+        a.runOneStep(60.0)
+        b.runOneStep(60.0)
+    assert np.array_equal(a.positions, b.positions)

@@ -1,0 +1,125 @@
+"""ctypes binding of ``libtfnr.so`` (the C ABI declared in include/tf_nr.h).
+
+The library is built in-tree by ``tissuefolding_paperversion_amd.build``.  Loading fails loudly
+when it is missing: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtfnr.so")
+
+TF_OK, TF_NOT_CONVERGED, TF_KRYLOV_STALL, TF_NEG_JACOBIAN, TF_NAN, TF_OOM, TF_HIP_ERROR, TF_INVALID = range(8)
+STATUS_NAMES = {0: "TF_OK", 1: "TF_NOT_CONVERGED", 2: "TF_KRYLOV_STALL", 3: "TF_NEG_JACOBIAN", 4: "TF_NAN",
+                5: "TF_OOM", 6: "TF_HIP_ERROR", 7: "TF_INVALID"}
+KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_GMRES = -1, 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+
+# Every symbol include/tf_nr.h declares (checked by the CPU test suite).
+EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf_set_options", "tf_get_options",
+           "tf_get_sizes", "tf_set_positions", "tf_get_positions", "tf_set_growth_rates", "tf_set_fg", "tf_get_fg",
+           "tf_set_material", "tf_grow", "tf_update_viscosity_surfaces", "tf_get_viscosity_surfaces",
+           "tf_begin_step", "tf_assemble", "tf_get_residual", "tf_get_jacobian_bsr", "tf_element_dump", "tf_solve",
+           "tf_update_uk", "tf_nr_step", "tf_timestep", "tf_get_segment_stats", "tf_kernel_timing",
+           "tf_kernel_bytes"]
+
+
+class TfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("n_elements", C.c_int32), ("node_pos", C.c_void_p), ("conn", C.c_void_p),
+                ("ref_pos", C.c_void_p), ("lam", C.c_void_p), ("mu", C.c_void_p), ("eta_int", C.c_void_p),
+                ("eta_ext", C.c_void_p), ("fixed", C.c_void_p), ("exposed", C.c_void_p), ("growth_rate", C.c_void_p),
+                ("n_segments", C.c_int32), ("node_seg_ptr", C.c_void_p), ("elem_seg_ptr", C.c_void_p)]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("krylov", C.c_int32), ("krylov_max_iter", C.c_int32), ("krylov_rtol", C.c_double),
+                ("krylov_atol", C.c_double), ("gmres_restart", C.c_int32), ("iters_per_sync", C.c_int32),
+                ("nr_tol", C.c_double), ("nr_max_iter", C.c_int32), ("tangent_precision", C.c_int32),
+                ("matrix_precision", C.c_int32), ("growth_rotation", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("nr_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("converged", C.c_int32),
+                ("rotated_elements", C.c_int32), ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
+                ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libtfnr.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m tissuefolding_paperversion_amd.build` "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(p)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "tf_create": ([C.POINTER(MeshDesc), C.POINTER(vp)], i32),
+        "tf_destroy": ([vp], None),
+        "tf_last_error": ([vp], C.c_char_p),
+        "tf_default_options": ([C.POINTER(SolverOpts)], None),
+        "tf_set_options": ([vp, C.POINTER(SolverOpts)], i32),
+        "tf_get_options": ([vp, C.POINTER(SolverOpts)], i32),
+        "tf_get_sizes": ([vp, vp, vp, vp, vp], i32),
+        "tf_set_positions": ([vp, vp], i32),
+        "tf_get_positions": ([vp, vp], i32),
+        "tf_set_growth_rates": ([vp, vp], i32),
+        "tf_set_fg": ([vp, vp], i32),
+        "tf_get_fg": ([vp, vp, vp], i32),
+        "tf_set_material": ([vp, vp, vp, vp], i32),
+        "tf_grow": ([vp, dbl, vp], i32),
+        "tf_update_viscosity_surfaces": ([vp], i32),
+        "tf_get_viscosity_surfaces": ([vp, vp], i32),
+        "tf_begin_step": ([vp], i32),
+        "tf_assemble": ([vp, dbl], i32),
+        "tf_get_residual": ([vp, vp], i32),
+        "tf_get_jacobian_bsr": ([vp, vp, vp, vp], i32),
+        "tf_element_dump": ([vp, i32, dbl, vp, vp, vp, vp], i32),
+        "tf_solve": ([vp, vp, vp, vp], i32),
+        "tf_update_uk": ([vp, vp, vp], i32),
+        "tf_nr_step": ([vp, dbl, C.POINTER(StepStats)], i32),
+        "tf_timestep": ([vp, dbl, C.POINTER(StepStats)], i32),
+        "tf_get_segment_stats": ([vp, vp, vp, vp], i32),
+        "tf_kernel_timing": ([vp, C.c_char_p, i32, vp, vp], i32),
+        "tf_kernel_bytes": ([vp, C.c_char_p, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)

@@ -1,0 +1,158 @@
+"""Physical set-up of a tissue for the Newton-Raphson path, and the named workloads.
+
+The set-up rules follow the reference:
+
+* Young's modulus by element placement, Poisson ratio clamped to [0, 0.5], Lame parameters
+  (``Prism::setElasticProperties``, Prism.cpp:491-516);
+* internal viscosity by placement (``ShapeBase::setViscosity``, ShapeBase.cpp:893-902);
+* external (nodal) viscosity by node placement, circumference nodes of the mid-layer take the
+  minimum of the apical and basal values (``Node::setExternalViscosity``, Node.cpp:68-116);
+* growth rates are given per hour and converted to per second
+  (``ModelInputObject.cpp:412-420``); the per-step increment is ``exp(rate * dt)`` along x, y, z
+  (``ShapeBase::calculateFgFromRates``, ShapeBase.cpp:396-421).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .mesh import APICAL, BASAL, MIDLINE, LATERAL, TissueMesh, disc_mesh, ellipse_mesh
+
+
+def lame(E, nu):
+    nu = np.clip(np.asarray(nu, float), 0.0, 0.5)
+    E = np.asarray(E, float)
+    lam = E * nu / (1 + nu) / (1 - 2.0 * nu)
+    mu = E / 2.0 / (1 + nu)
+    return lam, mu
+
+
+@dataclasses.dataclass
+class Tissue:
+    """Everything the NR path needs about one tissue (arrays in the reference's DoF order)."""
+    mesh: TissueMesh
+    lam: np.ndarray        # (E,)
+    mu: np.ndarray         # (E,)
+    eta_int: np.ndarray    # (E,)
+    eta_ext: np.ndarray    # (3N,)
+    fixed: np.ndarray      # (3N,) uint8
+    rates: np.ndarray      # (E, 3) growth rate per second
+    exposed: np.ndarray    # (E,) uint8 exposed-face bits
+    dt: float = 60.0
+    n_steps: int = 1
+    name: str = ""
+    E_young: np.ndarray | None = None   # (E,) Young's modulus (for per-layer gradients)
+    nu: np.ndarray | None = None
+
+    @property
+    def n_nodes(self):
+        return self.mesh.n_nodes
+
+    @property
+    def n_elements(self):
+        return self.mesh.n_elements
+
+
+def assign_material(mesh: TissueMesh, E_apical, E_basal, E_mid, nu):
+    pl, _ = mesh.element_placement()
+    E = np.full(mesh.n_elements, float(E_mid))
+    E[pl == BASAL] = E_basal
+    E[pl == APICAL] = E_apical
+    lam, mu = lame(E, np.full(mesh.n_elements, nu))
+    return E, lam, mu
+
+
+def assign_external_viscosity(mesh: TissueMesh, apical: float, basal: float, extend_to_inner: bool = False):
+    """Node::setExternalViscosity (Node.cpp:68-116) for every node, same value on x, y, z."""
+    v = np.zeros(mesh.n_nodes)
+    p = mesh.node_placement
+    v[p == BASAL] = basal
+    v[p == APICAL] = apical
+    mid = (p != BASAL) & (p != APICAL)
+    v[mid & mesh.at_circumference] = min(apical, basal)
+    if extend_to_inner:
+        inner = mid & ~mesh.at_circumference & ((p == MIDLINE) | (p == LATERAL))
+        v[inner] = min(apical, basal)
+    return np.repeat(v, 3)
+
+
+def per_hour(rate):
+    return np.asarray(rate, float) * (1.0 / 3600.0)
+
+
+def _nearest(pos, target):
+    return int(np.argmin(np.linalg.norm(pos - np.asarray(target), axis=1)))
+
+
+def disc_tissue(radius=50.0, h=5.85, z_levels=(0.0, 5.0, 10.0), seed=1, E=1000.0, nu=0.3,
+                growth_per_hour=(0.01, 0.01, 0.0), eta_ext=0.0, eta_int=0.0, dt=60.0, n_steps=60,
+                name="disc") -> Tissue:
+    """Plain FP64 disc: basal z fixed, centre x/y pinned, rim node at (+r, 0) pinned in y.
+
+    The extra rim y-pin removes the free in-plane rotation of an un-damped disc; with it the
+    stress-free scaling field x(t) = exp(r t) X (in plane) satisfies every boundary condition,
+    which gives this workload an exact answer.
+    """
+    mesh = disc_mesh(radius, h, z_levels, seed)
+    Ey, lam, mu = assign_material(mesh, E, E, E, nu)
+    fixed = np.zeros(3 * mesh.n_nodes, np.uint8)
+    basal = np.where(mesh.node_placement == BASAL)[0]
+    fixed[3 * basal + 2] = 1
+    for z in z_levels[:1]:
+        c = _nearest(mesh.pos, (0.0, 0.0, z))
+        fixed[3 * c + 0] = 1
+        fixed[3 * c + 1] = 1
+        r = _nearest(mesh.pos, (radius, 0.0, z))
+        fixed[3 * r + 1] = 1
+    rates = np.tile(per_hour(growth_per_hour), (mesh.n_elements, 1))
+    return Tissue(mesh, lam, mu, np.full(mesh.n_elements, float(eta_int)),
+                  assign_external_viscosity(mesh, eta_ext, eta_ext), fixed, rates, mesh.exposed_faces(),
+                  dt, n_steps, name, Ey, np.full(mesh.n_elements, nu))
+
+
+def gaussian_bump_rates(mesh: TissueMesh, a: float, b: float, n_bumps: int = 8, seed: int = 3):
+    """Per-element growth rates (per second) from a sum of seeded anisotropic Gaussian bumps,
+    evaluated at each prism's reference centroid.  Each bump has its own centre inside the
+    outline, widths in x and y, and peak rates in x and y drawn from 0.5-3 %/hr; z rate is 0."""
+    rng = np.random.default_rng(seed)
+    r = np.sqrt(rng.uniform(0, 1, n_bumps)) * 0.8
+    th = rng.uniform(0, 2 * math.pi, n_bumps)
+    cx, cy = a * r * np.cos(th), b * r * np.sin(th)
+    sx = rng.uniform(0.15, 0.35, n_bumps) * a
+    sy = rng.uniform(0.15, 0.35, n_bumps) * b
+    px = rng.uniform(0.5, 3.0, n_bumps) * 0.01
+    py = rng.uniform(0.5, 3.0, n_bumps) * 0.01
+    cen = mesh.xref.reshape(-1, 6, 3).mean(axis=1)
+    rates = np.zeros((mesh.n_elements, 3))
+    for k in range(n_bumps):
+        g = np.exp(-0.5 * ((cen[:, 0] - cx[k]) / sx[k]) ** 2 - 0.5 * ((cen[:, 1] - cy[k]) / sy[k]) ** 2)
+        rates[:, 0] += px[k] * g
+        rates[:, 1] += py[k] * g
+    return per_hour(rates)
+
+
+def wing_disc_tissue(a=200.0, b=120.0, h=3.66, z_levels=(0.0, 2.0, 7.0, 12.0), seed=2, growth_seed=3,
+                     E_apical=1000.0, E_mid=1000.0, E_basal=5000.0, nu=0.3, eta_ext=100.0, dt=60.0,
+                     n_steps=120, name="wing-disc ellipse") -> Tissue:
+    """Ellipse of three prism layers (apical, columnar, basal ECM) with a Gaussian-bump growth
+    map.  External viscosity on apical and basal surfaces (and mid-layer rim nodes) removes the
+    rigid-body modes, so no DoF is fixed."""
+    mesh = ellipse_mesh(a, b, h, z_levels, seed)
+    Ey, lam, mu = assign_material(mesh, E_apical, E_basal, E_mid, nu)
+    fixed = np.zeros(3 * mesh.n_nodes, np.uint8)
+    rates = gaussian_bump_rates(mesh, a, b, 8, growth_seed)
+    return Tissue(mesh, lam, mu, np.zeros(mesh.n_elements), assign_external_viscosity(mesh, eta_ext, eta_ext),
+                  fixed, rates, mesh.exposed_faces(), dt, n_steps, name, Ey, np.full(mesh.n_elements, nu))
+
+
+def config1() -> Tissue:
+    """Plain FP64 baseline disc: r = 50 um, ~500 triangles x 2 layers of 5 um, E = 1000 Pa,
+    nu = 0.3, 1 %/hr in-plane growth, no viscosity, dt = 60 s, 60 steps."""
+    return disc_tissue(name="disc r=50um, 2 layers (~1000 prisms)")
+
+
+def config2(h: float = 3.66) -> Tissue:
+    """Wing-disc-like ellipse 200 x 120 um, ~13k triangles x 3 layers (~40k prisms)."""
+    return wing_disc_tissue(h=h, name="wing-disc ellipse 200x120um, 3 layers (~40k prisms)")

@@ -1,0 +1,860 @@
+// tf_api.hip -- C ABI of the MI355X Newton-Raphson step: set-up, state transfer, NR driver.
+#include "tf_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+static thread_local std::string g_create_error;
+
+namespace {
+
+template <typename T>
+T* dev_alloc(size_t n)
+{
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw tf_hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    return (T*)p;
+}
+
+// Uploads complete before returning: callers often pass temporaries.
+template <typename T>
+void h2d(tf_handle* h, T* dst, const T* src, size_t n)
+{
+    if (!n) return;
+    TF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+}
+template <typename T>
+void d2h(tf_handle* h, T* dst, const T* src, size_t n)
+{
+    if (n) TF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+}
+
+// SoA transposition: out[k*E + e] = in[e*K + k]
+template <typename T>
+std::vector<T> to_soa(const T* in, int E, int K)
+{
+    std::vector<T> out((size_t)E * K);
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < K; ++k) out[(size_t)k * E + e] = in[(size_t)e * K + k];
+    return out;
+}
+template <typename T>
+std::vector<T> from_soa(const T* in, int E, int K)
+{
+    std::vector<T> out((size_t)E * K);
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < K; ++k) out[(size_t)e * K + k] = in[(size_t)k * E + e];
+    return out;
+}
+
+int fail(tf_handle* h, int code, const std::string& msg)
+{
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define TF_GUARD_BEGIN try {
+#define TF_GUARD_END(h)                                                                             \
+    }                                                                                               \
+    catch (const tf_hip_error& ex) {                                                                \
+        return fail(h, ex.code == hipErrorOutOfMemory ? TF_OOM : TF_HIP_ERROR, ex.what);            \
+    }                                                                                               \
+    catch (const std::exception& ex) { return fail(h, TF_INVALID, ex.what()); }
+
+void free_all(tf_handle* h)
+{
+    void* ptrs[] = {h->conn, h->xref, h->fg, h->invfg, h->lam, h->mu, h->eta, h->rate, h->exposed, h->elem_seg,
+                    h->uk, h->un, h->g, h->du, h->eta_ext, h->fixed, h->sa, h->face_area, h->kelem, h->felem,
+                    h->status, h->row_ptr, h->col, h->blk_row, h->diag_blk, h->val, h->val32, h->dinv, h->blk_ptr,
+                    h->blk_src, h->node_ptr, h->node_src, h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr,
+                    h->seg_node_ptr, h->seg_counter, h->partial, h->seg, h->seg_norm, h->kr, h->kr0, h->kp, h->kph,
+                    h->kv, h->ks, h->ksh, h->kt, h->kx};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->h_seg) (void)hipHostFree(h->h_seg);
+    if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
+    if (h->krylov_graph) (void)hipGraphExecDestroy(h->krylov_graph);
+    for (KernelTimer* t : {&h->t_elem, &h->t_gather, &h->t_spmv})
+        for (hipEvent_t e : t->pending) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+__global__ void k_seg_begin(SegScalars* seg, int S)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) { seg[s].active = 1; seg[s].converged = 0; seg[s].stalled = 0; }
+}
+__global__ void k_seg_nr_check(SegScalars* seg, const double* norm, double tol, int S)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S && seg[s].active && norm[s] <= tol) seg[s].active = 0;
+}
+__global__ void k_seg_deactivate(SegScalars* seg, int S)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) seg[s].active = 0;
+}
+__global__ void k_zero_int(int32_t* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// timers
+void timer_begin(tf_handle* h, KernelTimer& t, hipEvent_t* ev)
+{
+    if (!t.enabled) return;
+    TF_HIP(hipEventCreate(&ev[0]));
+    TF_HIP(hipEventCreate(&ev[1]));
+    TF_HIP(hipEventRecord(ev[0], h->stream));
+}
+void timer_end(tf_handle* h, KernelTimer& t, hipEvent_t* ev)
+{
+    if (!t.enabled) return;
+    TF_HIP(hipEventRecord(ev[1], h->stream));
+    t.pending.push_back(ev[0]);
+    t.pending.push_back(ev[1]);
+}
+void timer_collect(KernelTimer& t)
+{
+    if (t.pending.empty()) return;
+    TF_HIP(hipEventSynchronize(t.pending.back()));
+    for (size_t i = 0; i + 1 < t.pending.size(); i += 2) {
+        float ms = 0.f;
+        TF_HIP(hipEventElapsedTime(&ms, t.pending[i], t.pending[i + 1]));
+        t.total_ms += ms;
+        t.launches += 1;
+        (void)hipEventDestroy(t.pending[i]);
+        (void)hipEventDestroy(t.pending[i + 1]);
+    }
+    t.pending.clear();
+}
+
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+void tf_default_options(tf_solver_opts* o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->krylov = -1;  // automatic: CG when the Jacobian is symmetric (no internal viscosity), else BiCGSTAB
+    o->krylov_max_iter = 20000;
+    o->krylov_rtol = 1e-12;
+    o->krylov_atol = 0.0;
+    o->gmres_restart = 50;
+    o->iters_per_sync = 32;
+    o->nr_tol = 1e-8;
+    o->nr_max_iter = 20;
+    o->tangent_precision = TF_PREC_FP64;
+    o->matrix_precision = TF_PREC_FP64;
+    o->growth_rotation = 1;
+    o->use_graph = 1;
+}
+
+const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int tf_create(const tf_mesh_desc* d, tf_handle** out)
+{
+    if (!d || !out) return fail(nullptr, TF_INVALID, "null argument");
+    *out = nullptr;
+    if (d->n_nodes <= 0 || d->n_elements <= 0 || !d->node_pos || !d->conn || !d->lambda || !d->mu)
+        return fail(nullptr, TF_INVALID, "mesh needs n_nodes > 0, n_elements > 0, node_pos, conn, lambda, mu");
+    const int N = d->n_nodes, E = d->n_elements;
+    const int S = d->n_segments > 0 ? d->n_segments : 1;
+    if (S > 1 && (!d->node_seg_ptr || !d->elem_seg_ptr))
+        return fail(nullptr, TF_INVALID, "n_segments > 1 needs node_seg_ptr and elem_seg_ptr");
+    tf_handle* h = new tf_handle();
+    try {
+        tf_default_options(&h->opts);
+        h->N = N; h->E = E; h->S = S;
+        TF_HIP(hipGetDevice(&h->device));
+        TF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->h_conn.assign(d->conn, d->conn + (size_t)6 * E);
+        h->h_node_seg_ptr.resize(S + 1);
+        h->h_elem_seg_ptr.resize(S + 1);
+        if (S == 1) {
+            h->h_node_seg_ptr = {0, N};
+            h->h_elem_seg_ptr = {0, E};
+        } else {
+            std::copy(d->node_seg_ptr, d->node_seg_ptr + S + 1, h->h_node_seg_ptr.begin());
+            std::copy(d->elem_seg_ptr, d->elem_seg_ptr + S + 1, h->h_elem_seg_ptr.begin());
+            if (h->h_node_seg_ptr[0] != 0 || h->h_node_seg_ptr[S] != N || h->h_elem_seg_ptr[0] != 0 ||
+                h->h_elem_seg_ptr[S] != E)
+                throw std::runtime_error("segment pointers must cover [0, n) exactly");
+            for (int s = 0; s < S; ++s)
+                if (h->h_node_seg_ptr[s + 1] <= h->h_node_seg_ptr[s] || h->h_elem_seg_ptr[s + 1] <= h->h_elem_seg_ptr[s])
+                    throw std::runtime_error("every segment needs at least one node and one element");
+        }
+        // ---- validate connectivity and segment membership ----
+        std::vector<int32_t> eseg(E);
+        for (int s = 0; s < S; ++s)
+            for (int e = h->h_elem_seg_ptr[s]; e < h->h_elem_seg_ptr[s + 1]; ++e) {
+                eseg[e] = s;
+                for (int a = 0; a < 6; ++a) {
+                    const int id = h->h_conn[6 * e + a];
+                    if (id < h->h_node_seg_ptr[s] || id >= h->h_node_seg_ptr[s + 1])
+                        throw std::runtime_error("element " + std::to_string(e) + " references node " +
+                                                 std::to_string(id) + " outside its tissue");
+                }
+                for (int a = 0; a < 6; ++a)
+                    for (int b = a + 1; b < 6; ++b)
+                        if (h->h_conn[6 * e + a] == h->h_conn[6 * e + b])
+                            throw std::runtime_error("element " + std::to_string(e) + " repeats a node");
+            }
+        // ---- node -> element corners ----
+        std::vector<int32_t> node_ptr(N + 1, 0), node_src((size_t)6 * E);
+        for (int k = 0; k < 6 * E; ++k) node_ptr[h->h_conn[k] + 1]++;
+        for (int n = 0; n < N; ++n) node_ptr[n + 1] += node_ptr[n];
+        {
+            std::vector<int32_t> fill(node_ptr.begin(), node_ptr.end() - 1);
+            for (int k = 0; k < 6 * E; ++k) node_src[fill[h->h_conn[k]]++] = k;  // k = e*6 + a, ascending
+        }
+        // ---- BSR pattern: row A couples to every node of every element containing A ----
+        std::vector<int32_t> row_ptr(N + 1, 0), col;
+        col.reserve((size_t)N * 20);
+        std::vector<int32_t> tmp;
+        for (int A = 0; A < N; ++A) {
+            tmp.clear();
+            for (int k = node_ptr[A]; k < node_ptr[A + 1]; ++k) {
+                const int e = node_src[k] / 6;
+                for (int b = 0; b < 6; ++b) tmp.push_back(h->h_conn[6 * e + b]);
+            }
+            if (tmp.empty()) tmp.push_back(A);  // isolated node keeps its diagonal
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            col.insert(col.end(), tmp.begin(), tmp.end());
+            row_ptr[A + 1] = (int32_t)col.size();
+        }
+        const int nb = (int)col.size();
+        h->nb = nb;
+        std::vector<int32_t> blk_row(nb), diag_blk(N, -1);
+        for (int A = 0; A < N; ++A)
+            for (int j = row_ptr[A]; j < row_ptr[A + 1]; ++j) {
+                blk_row[j] = A;
+                if (col[j] == A) diag_blk[A] = j;
+            }
+        // ---- element block -> BSR block, then contributors per block (ascending e*36+ab) ----
+        std::vector<int32_t> eblk((size_t)36 * E);
+        std::vector<int32_t> blk_ptr(nb + 1, 0);
+        for (int e = 0; e < E; ++e)
+            for (int a = 0; a < 6; ++a) {
+                const int A = h->h_conn[6 * e + a];
+                const int32_t* c0 = col.data() + row_ptr[A];
+                const int32_t* c1 = col.data() + row_ptr[A + 1];
+                for (int b = 0; b < 6; ++b) {
+                    const int B = h->h_conn[6 * e + b];
+                    const int j = (int)(std::lower_bound(c0, c1, B) - col.data());
+                    eblk[(size_t)36 * e + 6 * a + b] = j;
+                    blk_ptr[j + 1]++;
+                }
+            }
+        for (int j = 0; j < nb; ++j) blk_ptr[j + 1] += blk_ptr[j];
+        std::vector<int32_t> blk_src((size_t)36 * E);
+        {
+            std::vector<int32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
+            for (size_t k = 0; k < (size_t)36 * E; ++k) blk_src[fill[eblk[k]]++] = (int32_t)k;
+        }
+        // ---- row tiles, never crossing a tissue ----
+        std::vector<int32_t> tr0, tr1, tseg, seg_tile_ptr(S + 1, 0);
+        for (int s = 0; s < S; ++s) {
+            int r = h->h_node_seg_ptr[s];
+            const int rend = h->h_node_seg_ptr[s + 1];
+            while (r < rend) {
+                int r1 = r, blocks = 0;
+                while (r1 < rend && r1 - r < TF_TILE_MAX_ROWS && blocks + (row_ptr[r1 + 1] - row_ptr[r1]) <= TF_TILE_MAX_BLOCKS) {
+                    blocks += row_ptr[r1 + 1] - row_ptr[r1];
+                    ++r1;
+                }
+                if (r1 == r) throw std::runtime_error("a block row exceeds the tile capacity");
+                tr0.push_back(r); tr1.push_back(r1); tseg.push_back(s);
+                r = r1;
+            }
+            seg_tile_ptr[s + 1] = (int32_t)tr0.size();
+        }
+        h->T = (int)tr0.size();
+        h->h_row_ptr = row_ptr;
+        h->h_col = col;
+        h->h_seg_tile_ptr = seg_tile_ptr;
+
+        // ---- device allocations ----
+        h->conn = dev_alloc<int32_t>((size_t)6 * E);
+        h->xref = dev_alloc<double>((size_t)18 * E);
+        h->fg = dev_alloc<double>((size_t)9 * E);
+        h->invfg = dev_alloc<double>((size_t)9 * E);
+        h->lam = dev_alloc<double>(E);
+        h->mu = dev_alloc<double>(E);
+        h->eta = dev_alloc<double>(E);
+        h->rate = dev_alloc<double>((size_t)3 * E);
+        h->exposed = dev_alloc<uint8_t>(E);
+        h->elem_seg = dev_alloc<int32_t>(E);
+        h->uk = dev_alloc<double>((size_t)3 * N);
+        h->un = dev_alloc<double>((size_t)3 * N);
+        h->g = dev_alloc<double>((size_t)3 * N);
+        h->du = dev_alloc<double>((size_t)3 * N);
+        h->eta_ext = dev_alloc<double>((size_t)3 * N);
+        h->fixed = dev_alloc<uint8_t>((size_t)3 * N);
+        h->sa = dev_alloc<double>(N);
+        h->face_area = dev_alloc<double>((size_t)2 * E);
+        h->kelem = dev_alloc<double>((size_t)324 * E);
+        h->felem = dev_alloc<double>((size_t)18 * E);
+        h->status = dev_alloc<int32_t>(4);
+        h->row_ptr = dev_alloc<int32_t>(N + 1);
+        h->col = dev_alloc<int32_t>(nb);
+        h->blk_row = dev_alloc<int32_t>(nb);
+        h->diag_blk = dev_alloc<int32_t>(N);
+        h->val = dev_alloc<double>((size_t)9 * nb);
+        h->val32 = dev_alloc<float>((size_t)9 * nb);
+        h->dinv = dev_alloc<double>((size_t)9 * N);
+        h->blk_ptr = dev_alloc<int32_t>(nb + 1);
+        h->blk_src = dev_alloc<int32_t>((size_t)36 * E);
+        h->node_ptr = dev_alloc<int32_t>(N + 1);
+        h->node_src = dev_alloc<int32_t>((size_t)6 * E);
+        h->tile_r0 = dev_alloc<int32_t>(h->T);
+        h->tile_r1 = dev_alloc<int32_t>(h->T);
+        h->tile_seg = dev_alloc<int32_t>(h->T);
+        h->seg_tile_ptr = dev_alloc<int32_t>(S + 1);
+        h->seg_node_ptr = dev_alloc<int32_t>(S + 1);
+        h->seg_counter = dev_alloc<int32_t>(S);
+        h->partial = dev_alloc<double>((size_t)h->T * TF_RED_SLOTS);
+        h->seg = dev_alloc<SegScalars>(S);
+        h->seg_norm = dev_alloc<double>(S);
+        TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
+        TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
+        for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx})
+            *v = dev_alloc<double>((size_t)3 * N);
+
+        // ---- uploads ----
+        std::vector<int32_t> conn_soa = to_soa(d->conn, E, 6);
+        h2d(h, h->conn, conn_soa.data(), conn_soa.size());
+        std::vector<double> xr((size_t)18 * E);
+        if (d->ref_pos) std::copy(d->ref_pos, d->ref_pos + xr.size(), xr.begin());
+        else
+            for (int e = 0; e < E; ++e)
+                for (int a = 0; a < 6; ++a)
+                    for (int i = 0; i < 3; ++i) xr[(size_t)18 * e + 3 * a + i] = d->node_pos[3 * h->h_conn[6 * e + a] + i];
+        std::vector<double> xr_soa = to_soa(xr.data(), E, 18);
+        h2d(h, h->xref, xr_soa.data(), xr_soa.size());
+        std::vector<double> eye((size_t)9 * E, 0.0);
+        for (int e = 0; e < E; ++e) eye[e] = eye[4 * (size_t)E + e] = eye[8 * (size_t)E + e] = 1.0;
+        h2d(h, h->fg, eye.data(), eye.size());
+        h2d(h, h->invfg, eye.data(), eye.size());
+        h2d(h, h->lam, d->lambda, E);
+        h2d(h, h->mu, d->mu, E);
+        std::vector<double> zeroE(E, 0.0), zero3E((size_t)3 * E, 0.0), zero3N((size_t)3 * N, 0.0);
+        h2d(h, h->eta, d->eta_int ? d->eta_int : zeroE.data(), E);
+        if (d->eta_int)
+            for (int e = 0; e < E; ++e) h->any_viscous |= (d->eta_int[e] != 0.0);
+        if (d->growth_rate) {
+            std::vector<double> r = to_soa(d->growth_rate, E, 3);
+            h2d(h, h->rate, r.data(), r.size());
+        } else h2d(h, h->rate, zero3E.data(), zero3E.size());
+        std::vector<uint8_t> ex(E, 0), fx((size_t)3 * N, 0);
+        h2d(h, h->exposed, d->exposed ? d->exposed : ex.data(), E);
+        h2d(h, h->elem_seg, eseg.data(), E);
+        h2d(h, h->uk, d->node_pos, (size_t)3 * N);
+        h2d(h, h->un, d->node_pos, (size_t)3 * N);
+        h2d(h, h->eta_ext, d->eta_ext ? d->eta_ext : zero3N.data(), (size_t)3 * N);
+        h2d(h, h->fixed, d->fixed ? d->fixed : fx.data(), (size_t)3 * N);
+        h2d(h, h->g, zero3N.data(), zero3N.size());
+        h2d(h, h->du, zero3N.data(), zero3N.size());
+        std::vector<double> zN(N, 0.0);
+        h2d(h, h->sa, zN.data(), N);
+        h2d(h, h->row_ptr, row_ptr.data(), N + 1);
+        h2d(h, h->col, col.data(), nb);
+        h2d(h, h->blk_row, blk_row.data(), nb);
+        h2d(h, h->diag_blk, diag_blk.data(), N);
+        h2d(h, h->blk_ptr, blk_ptr.data(), nb + 1);
+        h2d(h, h->blk_src, blk_src.data(), blk_src.size());
+        h2d(h, h->node_ptr, node_ptr.data(), N + 1);
+        h2d(h, h->node_src, node_src.data(), node_src.size());
+        h2d(h, h->tile_r0, tr0.data(), h->T);
+        h2d(h, h->tile_r1, tr1.data(), h->T);
+        h2d(h, h->tile_seg, tseg.data(), h->T);
+        h2d(h, h->seg_tile_ptr, seg_tile_ptr.data(), S + 1);
+        h2d(h, h->seg_node_ptr, h->h_node_seg_ptr.data(), S + 1);
+        std::vector<int32_t> zS(S, 0);
+        h2d(h, h->seg_counter, zS.data(), S);
+        std::vector<SegScalars> s0(S);
+        std::memset(s0.data(), 0, sizeof(SegScalars) * S);
+        h2d(h, h->seg, s0.data(), S);
+        std::vector<int32_t> z4(4, 0);
+        h2d(h, h->status, z4.data(), 4);
+        TF_HIP(hipStreamSynchronize(h->stream));
+        h->seg_nr_iters.assign(S, 0);
+        h->seg_converged.assign(S, 0);
+        h->seg_du.assign(S, 0.0);
+    } catch (const tf_hip_error& ex) {
+        free_all(h);
+        delete h;
+        return fail(nullptr, ex.code == hipErrorOutOfMemory ? TF_OOM : TF_HIP_ERROR, ex.what);
+    } catch (const std::exception& ex) {
+        free_all(h);
+        delete h;
+        return fail(nullptr, TF_INVALID, ex.what());
+    }
+    *out = h;
+    return TF_OK;
+}
+
+void tf_destroy(tf_handle* h)
+{
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+}
+
+int tf_set_options(tf_handle* h, const tf_solver_opts* o)
+{
+    if (!h || !o) return TF_INVALID;
+    if (o->krylov < -1 || o->krylov > TF_KRYLOV_GMRES || o->krylov_max_iter <= 0 || o->nr_max_iter < 0 ||
+        !(o->krylov_rtol >= 0) || !(o->nr_tol >= 0) ||
+        (o->matrix_precision != TF_PREC_FP64 && o->matrix_precision != TF_PREC_FP32) ||
+        o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16)
+        return fail(h, TF_INVALID, "invalid solver options");
+    if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
+    h->opts = *o;
+    if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+    return TF_OK;
+}
+
+int tf_get_options(const tf_handle* h, tf_solver_opts* o)
+{
+    if (!h || !o) return TF_INVALID;
+    *o = h->opts;
+    return TF_OK;
+}
+
+int tf_get_sizes(const tf_handle* h, int32_t* n_nodes, int32_t* n_elements, int32_t* n_blocks, int32_t* n_segments)
+{
+    if (!h) return TF_INVALID;
+    if (n_nodes) *n_nodes = h->N;
+    if (n_elements) *n_elements = h->E;
+    if (n_blocks) *n_blocks = h->nb;
+    if (n_segments) *n_segments = h->S;
+    return TF_OK;
+}
+
+int tf_set_positions(tf_handle* h, const double* x)
+{
+    if (!h || !x) return TF_INVALID;
+    TF_GUARD_BEGIN
+    h2d(h, h->uk, x, (size_t)3 * h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_positions(tf_handle* h, double* x)
+{
+    if (!h || !x) return TF_INVALID;
+    TF_GUARD_BEGIN
+    d2h(h, x, h->uk, (size_t)3 * h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_growth_rates(tf_handle* h, const double* rate)
+{
+    if (!h || !rate) return TF_INVALID;
+    TF_GUARD_BEGIN
+    std::vector<double> r = to_soa(rate, h->E, 3);
+    h2d(h, h->rate, r.data(), r.size());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_fg(tf_handle* h, const double* Fg)
+{
+    if (!h || !Fg) return TF_INVALID;
+    TF_GUARD_BEGIN
+    std::vector<double> inv((size_t)9 * h->E);
+    for (int e = 0; e < h->E; ++e) {
+        const double* M = Fg + 9 * (size_t)e;
+        double* I = inv.data() + 9 * (size_t)e;
+        double c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8], c02 = M[3] * M[7] - M[4] * M[6];
+        double det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+        if (!(std::fabs(det) > 0)) throw std::runtime_error("singular Fg");
+        double id = 1.0 / det;
+        I[0] = c00 * id; I[1] = (M[2] * M[7] - M[1] * M[8]) * id; I[2] = (M[1] * M[5] - M[2] * M[4]) * id;
+        I[3] = c01 * id; I[4] = (M[0] * M[8] - M[2] * M[6]) * id; I[5] = (M[2] * M[3] - M[0] * M[5]) * id;
+        I[6] = c02 * id; I[7] = (M[1] * M[6] - M[0] * M[7]) * id; I[8] = (M[0] * M[4] - M[1] * M[3]) * id;
+    }
+    std::vector<double> a = to_soa(Fg, h->E, 9), b = to_soa(inv.data(), h->E, 9);
+    h2d(h, h->fg, a.data(), a.size());
+    h2d(h, h->invfg, b.data(), b.size());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_fg(tf_handle* h, double* Fg, double* InvFg)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    std::vector<double> a((size_t)9 * h->E), b((size_t)9 * h->E);
+    d2h(h, a.data(), h->fg, a.size());
+    d2h(h, b.data(), h->invfg, b.size());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (Fg) { auto r = from_soa(a.data(), h->E, 9); std::copy(r.begin(), r.end(), Fg); }
+    if (InvFg) { auto r = from_soa(b.data(), h->E, 9); std::copy(r.begin(), r.end(), InvFg); }
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_material(tf_handle* h, const double* lambda, const double* mu, const double* eta_int)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (lambda) h2d(h, h->lam, lambda, h->E);
+    if (mu) h2d(h, h->mu, mu, h->E);
+    if (eta_int) {
+        h2d(h, h->eta, eta_int, h->E);
+        h->any_viscous = 0;
+        for (int e = 0; e < h->E; ++e) h->any_viscous |= (eta_int[e] != 0.0);
+    }
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_grow(tf_handle* h, double dt, int32_t* n_rotated)
+{
+    if (!h || !(dt > 0)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, h->stream, h->status + 1, 1);
+    launch_grow(h, dt);
+    int32_t st[4];
+    d2h(h, st, h->status, 4);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (n_rotated) *n_rotated = st[1];
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_update_viscosity_surfaces(tf_handle* h)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    launch_surfaces(h);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_viscosity_surfaces(tf_handle* h, double* Sa)
+{
+    if (!h || !Sa) return TF_INVALID;
+    TF_GUARD_BEGIN
+    d2h(h, Sa, h->sa, h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_begin_step(tf_handle* h)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
+    hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    TF_HIP(hipGetLastError());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
+    std::fill(h->seg_converged.begin(), h->seg_converged.end(), 0);
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+static int assemble_impl(tf_handle* h, double dt)
+{
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, h->stream, h->status, 1);
+    launch_assemble_elements(h, dt);
+    launch_assemble_gather(h, dt);
+    launch_residual(h, dt);
+    return TF_OK;
+}
+
+int tf_assemble(tf_handle* h, double dt)
+{
+    if (!h || !(dt > 0)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    assemble_impl(h, dt);
+    int32_t st[4];
+    d2h(h, st, h->status, 4);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    timer_collect(h->t_elem);
+    timer_collect(h->t_gather);
+    if (st[0] > 0) return fail(h, TF_NEG_JACOBIAN, std::to_string(st[0]) + " Gauss points with det Fe <= 0");
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_residual(tf_handle* h, double* g)
+{
+    if (!h || !g) return TF_INVALID;
+    TF_GUARD_BEGIN
+    d2h(h, g, h->g, (size_t)3 * h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_jacobian_bsr(tf_handle* h, int32_t* row_ptr, int32_t* col, double* val)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (row_ptr) std::copy(h->h_row_ptr.begin(), h->h_row_ptr.end(), row_ptr);
+    if (col) std::copy(h->h_col.begin(), h->h_col.end(), col);
+    if (val) {
+        std::vector<double> v((size_t)9 * h->nb);
+        d2h(h, v.data(), h->val, v.size());
+        TF_HIP(hipStreamSynchronize(h->stream));
+        for (int j = 0; j < h->nb; ++j)
+            for (int q = 0; q < 9; ++q) val[(size_t)9 * j + q] = v[(size_t)q * h->nb + j];
+    }
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_element_dump(tf_handle* h, int32_t e, double dt, double* fe, double* fv, double* Ke, double* Kv)
+{
+    if (!h || e < 0 || e >= h->E || !(dt > 0)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    double* d_out = dev_alloc<double>(36 + 648);
+    launch_element_dump(h, e, dt, d_out);
+    std::vector<double> o(36 + 648);
+    d2h(h, o.data(), d_out, o.size());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    (void)hipFree(d_out);
+    if (fe) std::copy(o.begin(), o.begin() + 18, fe);
+    if (fv) std::copy(o.begin() + 18, o.begin() + 36, fv);
+    if (Ke) std::copy(o.begin() + 36, o.begin() + 360, Ke);
+    if (Kv) std::copy(o.begin() + 360, o.end(), Kv);
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+static int solve_impl(tf_handle* h)
+{
+    tf_solver_opts saved = h->opts;
+    if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG;
+    int st = krylov_solve(h);
+    h->opts.krylov = saved.krylov;
+    return st;
+}
+
+int tf_solve(tf_handle* h, double* dU, int32_t* krylov_iters, double* rel_residual)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    int st = solve_impl(h);
+    if (dU) d2h(h, dU, h->du, (size_t)3 * h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (krylov_iters) *krylov_iters = h->last_krylov_iters;
+    if (rel_residual) {
+        double worst = 0.0;
+        for (int s = 0; s < h->S; ++s)
+            if (h->h_seg[s].bnorm2 > 0) worst = std::max(worst, std::sqrt(h->h_seg[s].rnorm2 / h->h_seg[s].bnorm2));
+        *rel_residual = worst;
+    }
+    if (st != TF_OK) return fail(h, st, "Krylov solver did not reach its tolerance");
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+static int update_uk_impl(tf_handle* h, int iter_index)
+{
+    launch_seg_norm(h, h->du);
+    launch_update_uk(h);
+    hipLaunchKernelGGL(k_seg_nr_check, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->seg_norm,
+                       h->opts.nr_tol, h->S);
+    TF_HIP(hipGetLastError());
+    d2h(h, h->h_seg_norm, h->seg_norm, h->S);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    int st = TF_OK;
+    for (int s = 0; s < h->S; ++s) {
+        if (h->seg_converged[s]) continue;
+        const double dn = h->h_seg_norm[s];
+        h->seg_du[s] = dn;
+        h->seg_nr_iters[s] = iter_index + 1;
+        if (!std::isfinite(dn)) st = TF_NAN;
+        else if (dn <= h->opts.nr_tol) h->seg_converged[s] = 1;
+    }
+    return st;
+}
+
+int tf_update_uk(tf_handle* h, double* du_norm, int32_t* converged)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    int it = 0;
+    for (int s = 0; s < h->S; ++s) it = std::max(it, h->seg_nr_iters[s]);
+    int st = update_uk_impl(h, it);
+    double worst = 0.0;
+    int all = 1;
+    for (int s = 0; s < h->S; ++s) { worst = std::max(worst, h->seg_du[s]); all &= h->seg_converged[s]; }
+    if (du_norm) *du_norm = worst;
+    if (converged) *converged = all;
+    if (st != TF_OK) return fail(h, st, "NaN in the Newton update");
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
+    hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
+    std::fill(h->seg_converged.begin(), h->seg_converged.end(), 0);
+    int status = TF_OK;
+    int krylov_total = 0;
+    double ms_asm = 0, ms_sol = 0;
+    int it = 0;
+    while (true) {
+        auto ta = std::chrono::steady_clock::now();
+        assemble_impl(h, dt);
+        int32_t st[4];
+        d2h(h, st, h->status, 4);
+        TF_HIP(hipStreamSynchronize(h->stream));
+        ms_asm += ms_since(ta);
+        if (st[0] > 0) {
+            status = TF_NEG_JACOBIAN;
+            h->err = std::to_string(st[0]) + " Gauss points with det Fe <= 0";
+            break;
+        }
+        auto ts = std::chrono::steady_clock::now();
+        int kst = solve_impl(h);
+        krylov_total += h->last_krylov_iters;
+        int ust = update_uk_impl(h, it);
+        ms_sol += ms_since(ts);
+        if (ust == TF_NAN) { status = TF_NAN; h->err = "NaN in the Newton update"; break; }
+        if (kst != TF_OK && status == TF_OK) status = TF_OK;  // inexact step: NR keeps iterating
+        ++it;
+        bool all = true;
+        for (int s = 0; s < h->S; ++s) all &= (bool)h->seg_converged[s];
+        if (all) break;
+        if (it > h->opts.nr_max_iter) {
+            status = TF_NOT_CONVERGED;
+            h->err = "Newton-Raphson did not converge";
+            hipLaunchKernelGGL(k_seg_deactivate, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+            break;
+        }
+    }
+    timer_collect(h->t_elem);
+    timer_collect(h->t_gather);
+    if (stats) {
+        int maxit = 0, allc = 1;
+        double worst = 0.0;
+        for (int s = 0; s < h->S; ++s) {
+            maxit = std::max(maxit, h->seg_nr_iters[s]);
+            allc &= h->seg_converged[s];
+            worst = std::max(worst, h->seg_du[s]);
+        }
+        stats->nr_iterations = maxit;
+        stats->krylov_iterations = krylov_total;
+        stats->converged = allc;
+        stats->last_du_norm = worst;
+        stats->ms_assemble = ms_asm;
+        stats->ms_solve = ms_sol;
+        stats->ms_total = ms_since(t0);
+    }
+    return status;
+}
+
+int tf_nr_step(tf_handle* h, double dt, tf_step_stats* stats)
+{
+    if (!h || !(dt > 0)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return nr_step_impl(h, dt, stats);
+    TF_GUARD_END(h)
+}
+
+int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats)
+{
+    if (!h || !(dt > 0)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, h->stream, h->status + 1, 1);
+    launch_grow(h, dt);
+    launch_surfaces(h);
+    int32_t st4[4];
+    d2h(h, st4, h->status, 4);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    double ms_grow = ms_since(t0);
+    int st = nr_step_impl(h, dt, stats);
+    if (stats) {
+        stats->rotated_elements = st4[1];
+        stats->ms_grow = ms_grow;
+        stats->ms_total = ms_since(t0);
+    }
+    return st;
+    TF_GUARD_END(h)
+}
+
+int tf_get_segment_stats(tf_handle* h, int32_t* nr_iters, double* du_norm, int32_t* converged)
+{
+    if (!h) return TF_INVALID;
+    for (int s = 0; s < h->S; ++s) {
+        if (nr_iters) nr_iters[s] = h->seg_nr_iters[s];
+        if (du_norm) du_norm[s] = h->seg_du[s];
+        if (converged) converged[s] = h->seg_converged[s];
+    }
+    return TF_OK;
+}
+
+int tf_kernel_timing(tf_handle* h, const char* name, int32_t enable, double* avg_ms, int64_t* launches)
+{
+    if (!h || !name) return TF_INVALID;
+    KernelTimer* t = nullptr;
+    if (!std::strcmp(name, "assemble_elements")) t = &h->t_elem;
+    else if (!std::strcmp(name, "assemble_gather")) t = &h->t_gather;
+    else if (!std::strcmp(name, "spmv")) t = &h->t_spmv;
+    if (!t) return fail(h, TF_INVALID, std::string("unknown kernel ") + name);
+    TF_GUARD_BEGIN
+    timer_collect(*t);
+    if (avg_ms) *avg_ms = t->launches ? t->total_ms / t->launches : 0.0;
+    if (launches) *launches = t->launches;
+    if (enable >= 0) {
+        t->enabled = enable != 0;
+        t->total_ms = 0.0;
+        t->launches = 0;
+    }
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
+{
+    if (!h || !name || !bytes) return TF_INVALID;
+    const double E = h->E, N = h->N, nb = h->nb;
+    const double vb = h->opts.matrix_precision == TF_PREC_FP32 ? 36.0 : 72.0;
+    if (!std::strcmp(name, "spmv")) *bytes = nb * (vb + 4.0) + (N + 1) * 4.0 + 2.0 * 3.0 * N * 8.0;
+    else if (!std::strcmp(name, "assemble_elements"))
+        *bytes = E * (24.0 + 144.0 + 72.0 + 16.0 + 144.0 + (h->any_viscous ? 152.0 : 0.0) + 324.0 * 8.0 + 18.0 * 8.0);
+    else if (!std::strcmp(name, "assemble_gather"))
+        *bytes = E * 324.0 * 8.0 + E * 36.0 * 4.0 + nb * 4.0 * 3 + nb * 72.0 + N * 72.0 + N * (8.0 + 24.0 + 3.0);
+    else return fail(h, TF_INVALID, std::string("unknown kernel ") + name);
+    return TF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,512 @@
+// tf_assemble.hip -- Jacobian / residual assembly, growth and viscosity-surface kernels (gfx950, FP64).
+//
+// Assembly is two deterministic passes, no atomics on values:
+//   k_elements : one workgroup = 32 prisms.  Phase A: one lane per (prism, Gauss point) evaluates
+//                the kinematics and stresses and stages them in LDS.  Phase B: one lane per
+//                (prism, node pair a-b) contracts the six staged Gauss points into the 3x3 block
+//                M_ab and stores it (36 blocks per prism, contiguous).  Phase C: one lane per
+//                (prism, node) sums the nodal force.
+//   k_gather   : one lane per 3x3 block of the BSR Jacobian sums its contributing element
+//                blocks in a fixed (element, pair) order, adds the nodal external-viscosity
+//                diagonal, applies fixed DoFs (unit row/column) and inverts diagonal blocks for
+//                the block-Jacobi preconditioner.
+//   k_residual : one lane per node sums its element corners and the external viscous force.
+#include "tf_internal.hpp"
+#include "tf_element.hpp"
+
+namespace {
+
+constexpr int EPB = 32;   // prisms per workgroup
+constexpr int WG = 256;
+
+template <bool VISC>
+__global__ __launch_bounds__(WG) void k_elements(const int32_t* __restrict__ conn, const double* __restrict__ xref,
+                                                 const double* __restrict__ invfg, const double* __restrict__ lam,
+                                                 const double* __restrict__ mu, const double* __restrict__ eta,
+                                                 const double* __restrict__ uk, const double* __restrict__ un,
+                                                 double inv_dt, int E, double* __restrict__ kelem,
+                                                 double* __restrict__ felem, int32_t* __restrict__ status)
+{
+    __shared__ tf::GPElastic sE[6 * EPB];
+    __shared__ tf::GPViscous sV[VISC ? 6 * EPB : 1];
+    const int tid = threadIdx.x;
+    const int e0 = blockIdx.x * EPB;
+
+    // ---- phase A: (prism, Gauss point) ----
+    if (tid < 6 * EPB) {
+        const int el = tid % EPB, g = tid / EPB;
+        const int e = e0 + el;
+        if (e < E) {
+            double X[18], x[18], Fi[9], v[18];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                const int id = conn[a * E + e];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    X[3 * a + i] = xref[(3 * a + i) * E + e];
+                    x[3 * a + i] = uk[3 * id + i];
+                    if (VISC) v[3 * a + i] = (uk[3 * id + i] - un[3 * id + i]) * inv_dt;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Fi[q] = invfg[q * E + e];
+            const double et = VISC ? eta[e] : 0.0;
+            tf::GPViscous vv;
+            const double Je = tf::gp_eval(g, X, x, Fi, lam[e], mu[e], et, v, sE[tid], VISC ? &vv : nullptr);
+            if (VISC) {
+                if (et == 0.0) {
+                    vv.ceta = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) vv.sv[q] = 0.0;
+                }
+                sV[tid] = vv;
+            }
+            if (!(Je > 0.0)) atomicAdd(&status[0], 1);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase B: (prism, node pair) ----
+    for (int t = tid; t < 36 * EPB; t += WG) {
+        const int el = t / 36, ab = t % 36;
+        const int e = e0 + el;
+        if (e >= E) break;
+        const int a = ab / 6, b = ab % 6;
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int g = 0; g < 6; ++g) {
+            tf::block_elastic(sE[g * EPB + el], a, b, acc);
+            if (VISC) tf::block_viscous(sE[g * EPB + el], sV[g * EPB + el], a, b, inv_dt, acc);
+        }
+        double* out = kelem + (size_t)e * 324 + ab * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out[q] = acc[q];
+    }
+
+    // ---- phase C: (prism, node) forces, negated as the reference stores them ----
+    if (tid < 6 * EPB) {
+        const int el = tid / 6, a = tid % 6;
+        const int e = e0 + el;
+        if (e < E) {
+            double f[3] = {0, 0, 0};
+#pragma unroll
+            for (int g = 0; g < 6; ++g) {
+                double r[3];
+                tf::voigt_mv(sE[g * EPB + el].sig, sE[g * EPB + el].gn + 3 * a, r);
+                f[0] += r[0]; f[1] += r[1]; f[2] += r[2];
+                if (VISC) {
+                    tf::voigt_mv(sV[g * EPB + el].sv, sE[g * EPB + el].gn + 3 * a, r);
+                    f[0] += r[0]; f[1] += r[1]; f[2] += r[2];
+                }
+            }
+            double* out = felem + (size_t)e * 18 + 3 * a;
+            out[0] = -f[0]; out[1] = -f[1]; out[2] = -f[2];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_ptr, const int32_t* __restrict__ blk_src,
+                                                const int32_t* __restrict__ blk_row, const int32_t* __restrict__ col,
+                                                const double* __restrict__ kelem, const double* __restrict__ sa,
+                                                const double* __restrict__ eta_ext, const uint8_t* __restrict__ fixed,
+                                                double inv_dt, int nb, double* __restrict__ val,
+                                                float* __restrict__ val32, double* __restrict__ dinv)
+{
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= nb) return;
+    const int A = blk_row[blk], B = col[blk];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int k1 = blk_ptr[blk + 1];
+    for (int k = blk_ptr[blk]; k < k1; ++k) {
+        const double* p = kelem + (size_t)blk_src[k] * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] += p[q];
+    }
+    if (A == B) {
+        const double s = sa[A] * inv_dt;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[4 * i] += s * eta_ext[3 * A + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (fixed[3 * A + i] || fixed[3 * B + j]) acc[3 * i + j] = (A == B && i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) val[(size_t)q * nb + blk] = acc[q];
+    if (val32) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) val32[(size_t)q * nb + blk] = (float)acc[q];
+    }
+    if (A == B) {
+        double inv[9];
+        tf::inv3(acc, inv);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dinv[9 * A + q] = inv[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_residual(const int32_t* __restrict__ node_ptr, const int32_t* __restrict__ node_src,
+                                                  const double* __restrict__ felem, const double* __restrict__ uk,
+                                                  const double* __restrict__ un, const double* __restrict__ sa,
+                                                  const double* __restrict__ eta_ext, const uint8_t* __restrict__ fixed,
+                                                  double inv_dt, int N, double* __restrict__ g)
+{
+    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A >= N) return;
+    double acc[3] = {0, 0, 0};
+    const int k1 = node_ptr[A + 1];
+    for (int k = node_ptr[A]; k < k1; ++k) {
+        const double* p = felem + (size_t)node_src[k] * 3;
+        acc[0] += p[0]; acc[1] += p[1]; acc[2] += p[2];
+    }
+    const double s = sa[A];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int d = 3 * A + i;
+        const double vel = (uk[d] - un[d]) * inv_dt;
+        const double gext = (s * eta_ext[d] * vel) * -1.0;
+        g[d] = fixed[d] ? 0.0 : acc[i] + gext;
+    }
+}
+
+// Element-local arrays of one prism in the reference layout (parity dumps).
+__global__ void k_element_dump(const int32_t* conn, const double* xref, const double* invfg, const double* lam,
+                               const double* mu, const double* eta, const double* uk, const double* un,
+                               const uint8_t* fixed, double inv_dt, int E, int e, double* out)
+{
+    // out: fe[18], fv[18], Ke[324] (TriPointKe), Kv[324] (TriPointKv, stored transposed)
+    const int t = threadIdx.x;  // 36 lanes: one node pair each
+    __shared__ tf::GPElastic sE[6];
+    __shared__ tf::GPViscous sV[6];
+    if (t < 6) {
+        double X[18], x[18], Fi[9], v[18];
+        for (int a = 0; a < 6; ++a) {
+            const int id = conn[a * E + e];
+            for (int i = 0; i < 3; ++i) {
+                X[3 * a + i] = xref[(3 * a + i) * E + e];
+                x[3 * a + i] = uk[3 * id + i];
+                v[3 * a + i] = (uk[3 * id + i] - un[3 * id + i]) * inv_dt;
+            }
+        }
+        for (int q = 0; q < 9; ++q) Fi[q] = invfg[q * E + e];
+        tf::gp_eval(t, X, x, Fi, lam[e], mu[e], eta[e], v, sE[t], &sV[t]);
+    }
+    __syncthreads();
+    if (t < 36) {
+        const int a = t / 6, b = t % 6;
+        double ke[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, kv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int g = 0; g < 6; ++g) {
+            tf::block_elastic(sE[g], a, b, ke);
+            if (eta[e] != 0.0) tf::block_viscous(sE[g], sV[g], a, b, inv_dt, kv);
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                out[36 + (3 * a + i) * 18 + 3 * b + j] = ke[3 * i + j];
+                out[36 + 324 + (3 * b + j) * 18 + 3 * a + i] = kv[3 * i + j];
+            }
+    }
+    if (t < 6) {
+        const int a = t;
+        const int id = conn[a * E + e];
+        double f[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
+        for (int g = 0; g < 6; ++g) {
+            double r[3];
+            tf::voigt_mv(sE[g].sig, sE[g].gn + 3 * a, r);
+            for (int i = 0; i < 3; ++i) f[i] += r[i];
+            if (eta[e] != 0.0) {
+                tf::voigt_mv(sV[g].sv, sE[g].gn + 3 * a, r);
+                for (int i = 0; i < 3; ++i) fv[i] += r[i];
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            const bool fx = fixed[3 * id + i] != 0;
+            out[3 * a + i] = fx ? 0.0 : -f[i];
+            out[18 + 3 * a + i] = fx ? 0.0 : -fv[i];
+        }
+    }
+}
+
+// Rotation factor of the polar decomposition by scaled Newton iteration (Higham):
+// R <- (gamma R + R^-T / gamma) / 2.  For det F > 0 this is U V^T of the SVD F = U S V^T.
+__device__ inline void polar_rotation(const double* F, double* R)
+{
+    for (int q = 0; q < 9; ++q) R[q] = F[q];
+    bool scale = true;
+    for (int it = 0; it < 60; ++it) {
+        double Ri[9];
+        tf::inv3(R, Ri);
+        double n1 = 0.0, n2 = 0.0;
+        for (int q = 0; q < 9; ++q) { n1 += R[q] * R[q]; n2 += Ri[q] * Ri[q]; }
+        const double gam = scale ? sqrt(sqrt(n2 / n1)) : 1.0;
+        double diff = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double nv = 0.5 * (gam * R[3 * i + j] + Ri[3 * j + i] / gam);
+                const double d = nv - R[3 * i + j];
+                diff += d * d;
+                R[3 * i + j] = nv;
+            }
+        if (diff < 1e-4) scale = false;
+        if (diff < 1e-34) break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, const double* __restrict__ xref,
+                                              const double* __restrict__ pos, const double* __restrict__ rate,
+                                              double dt, int allow_rotation, int E, double* __restrict__ fg,
+                                              double* __restrict__ invfg, int32_t* __restrict__ status)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double inc[9] = {exp(rate[e] * dt), 0, 0, 0, exp(rate[E + e] * dt), 0, 0, 0, exp(rate[2 * E + e] * dt)};
+    if (allow_rotation) {
+        // mean F at the current positions (ShapeBase.cpp:1676-1689)
+        double X[18], x[18];
+        for (int a = 0; a < 6; ++a) {
+            const int id = conn[a * E + e];
+            for (int i = 0; i < 3; ++i) {
+                X[3 * a + i] = xref[(3 * a + i) * E + e];
+                x[3 * a + i] = pos[3 * id + i];
+            }
+        }
+        double Fm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int g = 0; g < 6; ++g) {
+            double G0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, IJ0[9], F[9];
+            for (int a = 0; a < 6; ++a) {
+                double d[3];
+                tf::shape_deriv(g, a, d);
+                for (int i = 0; i < 3; ++i)
+                    for (int k = 0; k < 3; ++k) {
+                        G0[3 * i + k] += X[3 * a + i] * d[k];
+                        G[3 * i + k] += x[3 * a + i] * d[k];
+                    }
+            }
+            tf::inv3(G0, IJ0);
+            tf::mm3(G, IJ0, F);
+            for (int q = 0; q < 9; ++q) Fm[q] += F[q] * (1.0 / 6.0);
+        }
+        double R[9];
+        polar_rotation(Fm, R);
+        bool big = false;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                if (i != j && (R[3 * i + j] > 0.017 || R[3 * i + j] < -0.017)) big = true;
+        if (big) {
+            const double tz = atan2(R[1], R[0]);
+            if (tz > 0.017 || tz < -0.017) {
+                // increment <- GRM^T inc GRM with GRM = Rz(tz)^T (ShapeBase.cpp:1216-1222, 1697-1712)
+                const double c = cos(tz), s = sin(tz);
+                const double Rz[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+                const double RzT[9] = {c, s, 0, -s, c, 0, 0, 0, 1};
+                double T[9];
+                tf::mm3(Rz, inc, T);
+                tf::mm3(T, RzT, inc);
+                atomicAdd(&status[1], 1);
+            }
+        }
+    }
+    double F0[9], F1[9], Fi[9];
+    for (int q = 0; q < 9; ++q) F0[q] = fg[q * E + e];
+    tf::mm3(inc, F0, F1);
+    tf::inv3(F1, Fi);
+    for (int q = 0; q < 9; ++q) {
+        fg[q * E + e] = F1[q];
+        invfg[q * E + e] = Fi[q];
+    }
+}
+
+// exposed-face areas (Prism.cpp:2424-2503)
+__device__ inline double face_area(const double* P0, const double* P1, const double* P2)
+{
+    const double Threshold = 1e-5;
+    double s1 = 0.0, s2 = 0.0, ct = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const double a = P1[i] - P0[i], b = P2[i] - P0[i];
+        ct += a * b; s1 += a * a; s2 += b * b;
+    }
+    double area = 0.0;
+    if (s1 > Threshold && s2 > Threshold) {
+        s1 = sqrt(s1); s2 = sqrt(s2);
+        ct /= (s1 * s2);
+        const double st2 = 1 - ct * ct;
+        const double st = (st2 > Threshold) ? sqrt(st2) : 0.0;
+        area = s1 * s2 * st / 2.0;
+    }
+    return area;
+}
+
+__global__ __launch_bounds__(256) void k_face_areas(const int32_t* __restrict__ conn, const uint8_t* __restrict__ exposed,
+                                                    const double* __restrict__ pos, int E, double* __restrict__ area)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double P[6][3];
+    for (int a = 0; a < 6; ++a) {
+        const int id = conn[a * E + e];
+        for (int i = 0; i < 3; ++i) P[a][i] = pos[3 * id + i];
+    }
+    const uint8_t x = exposed[e];
+    area[e] = (x & 1) ? face_area(P[0], P[1], P[2]) : 0.0;
+    area[E + e] = (x & 2) ? face_area(P[3], P[4], P[5]) : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_node_surfaces(const int32_t* __restrict__ node_ptr, const int32_t* __restrict__ node_src,
+                                                       const uint8_t* __restrict__ exposed, const double* __restrict__ area,
+                                                       int E, int N, double* __restrict__ sa)
+{
+    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A >= N) return;
+    double s = 0.0;
+    for (int k = node_ptr[A]; k < node_ptr[A + 1]; ++k) {
+        const int src = node_src[k];
+        const int e = src / 6, a = src % 6;
+        if (a >= 3) { if (exposed[e] & 2) s += area[E + e] / 3; }
+        else if (exposed[e] & 1) s += area[e] / 3;
+    }
+    sa[A] = s;
+}
+
+__global__ void k_copy(double* dst, const double* src, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+__global__ void k_fill(double* dst, double v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+// per-segment ||x||^2 over rows of each tile, last tile of a segment finishes (deterministic order)
+__global__ __launch_bounds__(256) void k_seg_norm(const double* __restrict__ x, const int32_t* __restrict__ tile_r0,
+                                                  const int32_t* __restrict__ tile_r1, const int32_t* __restrict__ tile_seg,
+                                                  const int32_t* __restrict__ seg_tile_ptr, int32_t* __restrict__ counter,
+                                                  double* __restrict__ partial, double* __restrict__ out)
+{
+    __shared__ double red[256];
+    __shared__ int last;
+    const int t = blockIdx.x;
+    const int d0 = 3 * tile_r0[t], d1 = 3 * tile_r1[t];
+    double s = 0.0;
+    for (int d = d0 + threadIdx.x; d < d1; d += blockDim.x) s += x[d] * x[d];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const int sg = tile_seg[t];
+    if (threadIdx.x == 0) {
+        partial[t * TF_RED_SLOTS] = red[0];
+        __threadfence();
+        const int ntiles = seg_tile_ptr[sg + 1] - seg_tile_ptr[sg];
+        last = (atomicAdd(&counter[sg], 1) == ntiles - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double tot = 0.0;
+        for (int k = seg_tile_ptr[sg]; k < seg_tile_ptr[sg + 1]; ++k)
+            tot += __hip_atomic_load(&partial[k * TF_RED_SLOTS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out[sg] = sqrt(tot);
+        counter[sg] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_update_uk(double* __restrict__ uk, const double* __restrict__ du,
+                                                   const int32_t* __restrict__ tile_r0, const int32_t* __restrict__ tile_r1,
+                                                   const int32_t* __restrict__ tile_seg, const SegScalars* __restrict__ seg)
+{
+    const int t = blockIdx.x;
+    if (!seg[tile_seg[t]].active) return;
+    const int d0 = 3 * tile_r0[t], d1 = 3 * tile_r1[t];
+    for (int d = d0 + threadIdx.x; d < d1; d += blockDim.x) uk[d] = uk[d] + du[d];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+void launch_assemble_elements(tf_handle* h, double dt)
+{
+    const int grid = (h->E + EPB - 1) / EPB;
+    hipEvent_t ev[2];
+    timer_begin(h, h->t_elem, ev);
+    if (h->any_viscous)
+        hipLaunchKernelGGL(k_elements<true>, dim3(grid), dim3(WG), 0, h->stream, h->conn, h->xref, h->invfg, h->lam,
+                           h->mu, h->eta, h->uk, h->un, 1.0 / dt, h->E, h->kelem, h->felem, h->status);
+    else
+        hipLaunchKernelGGL(k_elements<false>, dim3(grid), dim3(WG), 0, h->stream, h->conn, h->xref, h->invfg, h->lam,
+                           h->mu, h->eta, h->uk, h->un, 1.0 / dt, h->E, h->kelem, h->felem, h->status);
+    TF_HIP(hipGetLastError());
+    timer_end(h, h->t_elem, ev);
+}
+
+void launch_assemble_gather(tf_handle* h, double dt)
+{
+    hipEvent_t ev[2];
+    timer_begin(h, h->t_gather, ev);
+    hipLaunchKernelGGL(k_gather, dim3((h->nb + 255) / 256), dim3(256), 0, h->stream, h->blk_ptr, h->blk_src, h->blk_row,
+                       h->col, h->kelem, h->sa, h->eta_ext, h->fixed, 1.0 / dt, h->nb, h->val,
+                       h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr, h->dinv);
+    TF_HIP(hipGetLastError());
+    timer_end(h, h->t_gather, ev);
+}
+
+void launch_residual(tf_handle* h, double dt)
+{
+    hipLaunchKernelGGL(k_residual, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src, h->felem,
+                       h->uk, h->un, h->sa, h->eta_ext, h->fixed, 1.0 / dt, h->N, h->g);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_element_dump(tf_handle* h, int32_t e, double dt, double* d_out)
+{
+    hipLaunchKernelGGL(k_element_dump, dim3(1), dim3(64), 0, h->stream, h->conn, h->xref, h->invfg, h->lam, h->mu, h->eta,
+                       h->uk, h->un, h->fixed, 1.0 / dt, h->E, e, d_out);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_grow(tf_handle* h, double dt)
+{
+    hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->xref, h->uk, h->rate, dt,
+                       h->opts.growth_rotation, h->E, h->fg, h->invfg, h->status);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_surfaces(tf_handle* h)
+{
+    hipLaunchKernelGGL(k_face_areas, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->exposed, h->uk, h->E,
+                       h->face_area);
+    hipLaunchKernelGGL(k_node_surfaces, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src,
+                       h->exposed, h->face_area, h->E, h->N, h->sa);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_copy(tf_handle* h, double* dst, const double* src, int64_t n)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, dst, src, n);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_fill(tf_handle* h, double* dst, double v, int64_t n)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, dst, v, n);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_seg_norm(tf_handle* h, const double* x)
+{
+    hipLaunchKernelGGL(k_seg_norm, dim3(h->T), dim3(256), 0, h->stream, x, h->tile_r0, h->tile_r1, h->tile_seg,
+                       h->seg_tile_ptr, h->seg_counter, h->partial, h->seg_norm);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_update_uk(tf_handle* h)
+{
+    hipLaunchKernelGGL(k_update_uk, dim3(h->T), dim3(256), 0, h->stream, h->uk, h->du, h->tile_r0, h->tile_r1,
+                       h->tile_seg, h->seg);
+    TF_HIP(hipGetLastError());
+}

@@ -1,0 +1,152 @@
+// tf_internal.hpp -- device-resident state of a tf_handle and the launchers the C ABI drives.
+//
+// HBM layout (structure of arrays wherever a kernel walks elements or blocks in lanes):
+//   element data   : conn[a*E + e] (int32), xref[k*E + e] (k = 3a+i), invfg/fg[q*E + e],
+//                    lam/mu/eta[e], rate[c*E + e]
+//   element output : kelem[e*324 + ab*9 + q] (36 node-pair blocks of 3x3, natural orientation),
+//                    felem[e*18 + 3a + i] (negated elastic + internal viscous force)
+//   node data      : uk/un/g/du [3N] interleaved (the reference's DoF order), sa[N], eta_ext[3N]
+//   Jacobian (BSR3): row_ptr[N+1], col[nb], blk_row[nb], val[q*nb + blk] (component-major)
+//   scatter maps   : blk_ptr[nb+1] / blk_src[36E] (contributing element blocks of every BSR
+//                    block, sorted), node_ptr[N+1] / node_src[6E] (element corners of every node)
+//   row tiles      : tile_r0/tile_r1/tile_seg -- runs of block rows of one tissue with at most
+//                    TF_TILE_MAX_BLOCKS blocks, the unit of every row kernel and reduction
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/tf_nr.h"
+
+#define TF_TILE_MAX_BLOCKS 1024
+#define TF_TILE_MAX_ROWS 64
+#define TF_RED_SLOTS 4       // partial sums per tile per reduction kernel
+
+// Per-tissue Krylov / NR scalars, one struct per segment, resident on the device.
+struct SegScalars {
+    double rho, rho_prev, alpha, omega, beta;
+    double bnorm2;           // ||b||^2
+    double rnorm2;           // ||r||^2 (latest)
+    double tol2;             // squared absolute stopping threshold
+    double tmp0, tmp1, tmp2, tmp3;
+    int32_t converged;       // Krylov converged (or nothing to do)
+    int32_t stalled;         // breakdown / cap
+    int32_t iters;           // Krylov iterations done
+    int32_t active;          // NR still iterating on this segment
+};
+
+struct KernelTimer {
+    bool enabled = false;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+    std::vector<hipEvent_t> pending;  // start/stop pairs not yet read back
+};
+
+struct tf_handle {
+    // sizes
+    int32_t N = 0, E = 0, nb = 0, S = 1, T = 0;
+    // options
+    tf_solver_opts opts;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // host copies needed for set-up and export
+    std::vector<int32_t> h_row_ptr, h_col, h_node_seg_ptr, h_elem_seg_ptr, h_seg_tile_ptr;
+    std::vector<int32_t> h_conn;  // [E*6]
+
+    // element data
+    int32_t *conn = nullptr;      // [6][E]
+    double *xref = nullptr;       // [18][E]
+    double *fg = nullptr, *invfg = nullptr;  // [9][E]
+    double *lam = nullptr, *mu = nullptr, *eta = nullptr;  // [E]
+    double *rate = nullptr;       // [3][E]
+    uint8_t *exposed = nullptr;   // [E]
+    int32_t *elem_seg = nullptr;  // [E]
+    int any_viscous = 0;
+
+    // node data
+    double *uk = nullptr, *un = nullptr, *g = nullptr, *du = nullptr;  // [3N]
+    double *eta_ext = nullptr;    // [3N]
+    uint8_t *fixed = nullptr;     // [3N]
+    double *sa = nullptr;         // [N]
+    double *face_area = nullptr;  // [2][E]
+
+    // assembly output / scratch
+    double *kelem = nullptr;      // [E*324]
+    double *felem = nullptr;      // [E*18]
+    int32_t *status = nullptr;    // [4] device flags: 0 = non-positive det Fe count, 1 = rotated count
+
+    // Jacobian
+    int32_t *row_ptr = nullptr, *col = nullptr, *blk_row = nullptr, *diag_blk = nullptr;
+    double *val = nullptr;        // [9][nb]
+    float *val32 = nullptr;       // [9][nb] (FP32 storage mode)
+    double *dinv = nullptr;       // [N][9]
+    int32_t *blk_ptr = nullptr, *blk_src = nullptr;
+    int32_t *node_ptr = nullptr, *node_src = nullptr;
+
+    // tiles and segments
+    int32_t *tile_r0 = nullptr, *tile_r1 = nullptr, *tile_seg = nullptr;
+    int32_t *seg_tile_ptr = nullptr;  // [S+1]
+    int32_t *seg_node_ptr = nullptr;  // [S+1]
+    int32_t *seg_counter = nullptr;   // [S] tickets for last-tile reductions
+    double *partial = nullptr;        // [T][TF_RED_SLOTS]
+    SegScalars *seg = nullptr;        // [S]
+    SegScalars *h_seg = nullptr;      // pinned host mirror
+    double *seg_norm = nullptr;       // [S] per-segment ||dU||
+    double *h_seg_norm = nullptr;     // pinned
+
+    // Krylov work vectors [3N]
+    double *kr = nullptr, *kr0 = nullptr, *kp = nullptr, *kph = nullptr, *kv = nullptr, *ks = nullptr,
+           *ksh = nullptr, *kt = nullptr, *kx = nullptr;
+
+    // per-segment NR bookkeeping on the host
+    std::vector<int32_t> seg_nr_iters, seg_converged;
+    std::vector<double> seg_du;
+    int32_t last_krylov_iters = 0;
+
+    // graphs
+    hipGraphExec_t krylov_graph = nullptr;
+    int krylov_graph_iters = 0;
+    int krylov_graph_kind = -1;
+
+    // timing
+    KernelTimer t_elem, t_gather, t_spmv;
+};
+
+// ---- launchers (tf_assemble.hip) ------------------------------------------------------------
+void launch_assemble_elements(tf_handle* h, double dt);
+void launch_assemble_gather(tf_handle* h, double dt);
+void launch_residual(tf_handle* h, double dt);
+void launch_element_dump(tf_handle* h, int32_t e, double dt, double* d_out);
+void launch_grow(tf_handle* h, double dt);
+void launch_surfaces(tf_handle* h);
+void launch_copy(tf_handle* h, double* dst, const double* src, int64_t n);
+void launch_fill(tf_handle* h, double* dst, double v, int64_t n);
+void launch_seg_norm(tf_handle* h, const double* x);   // per-segment ||x||_2 into h->seg_norm
+void launch_update_uk(tf_handle* h);                    // uk += du on active segments
+// ---- Krylov (tf_krylov.hip) -----------------------------------------------------------------
+int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
+void launch_spmv(tf_handle* h, const double* x, double* y);
+
+// error helpers
+#define TF_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) throw tf_hip_error(e_, #call, __FILE__, __LINE__);     \
+    } while (0)
+
+struct tf_hip_error {
+    hipError_t code;
+    std::string what;
+    tf_hip_error(hipError_t c, const char* call, const char* file, int line) : code(c) {
+        what = std::string(hipGetErrorString(c)) + " in " + call + " at " + file + ":" + std::to_string(line);
+    }
+};
+
+// timing helpers (record events around a launch on the handle's stream)
+void timer_begin(tf_handle* h, KernelTimer& t, hipEvent_t* ev);
+void timer_end(tf_handle* h, KernelTimer& t, hipEvent_t* ev);
+void timer_collect(KernelTimer& t);

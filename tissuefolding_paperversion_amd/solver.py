@@ -1,0 +1,222 @@
+"""Host-side facade over the C ABI whose methods mirror the reference's ``NewtonRaphsonSolver``.
+
+The reference drives one NR step from ``Simulation::updateStepNR`` (Simulation.cpp:6449-6590)
+through these calls, in this order::
+
+    setMatricesToZeroAtTheBeginningOfIteration / constructUnMatrix / initialteUkMatrix
+    loop: calculateDisplacementMatrix, calculateForcesAndJacobianMatrixNR,
+          writeForcesTogeAndgvInternal, writeImplicitElementalKToJacobian,
+          calculateExternalViscousForcesForNR, addImplicitKViscousExternalToJacobian,
+          calculateSumOfInternalForces, addExernalForces, calcutateFixedK, solveForDeltaU,
+          checkConvergenceViaDeltaU, updateUkInIteration
+
+:class:`NewtonRaphsonSolver` keeps those names; the element-side calls are merged into
+``calculateForcesAndJacobianMatrixNR`` (one device pass builds g and the Jacobian), and the whole
+loop is also available as one call (:meth:`updateStepNR`).  All state lives on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .configs import Tissue
+
+
+class NewtonRaphsonSolver:
+    def __init__(self, tissues, opts: dict | None = None):
+        """``tissues``: one :class:`Tissue` or a list of them (independent segments of one handle)."""
+        if isinstance(tissues, Tissue):
+            tissues = [tissues]
+        self.tissues = list(tissues)
+        self._lib = L.load()
+        node_off = np.cumsum([0] + [t.n_nodes for t in self.tissues])
+        elem_off = np.cumsum([0] + [t.n_elements for t in self.tissues])
+        self.node_seg_ptr = L.i32(node_off)
+        self.elem_seg_ptr = L.i32(elem_off)
+        cat = lambda f: np.concatenate([f(t) for t in self.tissues])  # noqa: E731
+        self._keep = dict(
+            pos=L.f64(cat(lambda t: t.mesh.pos.ravel())),
+            conn=L.i32(np.concatenate([t.mesh.conn.ravel() + node_off[k] for k, t in enumerate(self.tissues)])),
+            xref=L.f64(cat(lambda t: t.mesh.xref.ravel())),
+            lam=L.f64(cat(lambda t: t.lam)), mu=L.f64(cat(lambda t: t.mu)), eta=L.f64(cat(lambda t: t.eta_int)),
+            eta_ext=L.f64(cat(lambda t: t.eta_ext)), fixed=L.u8(cat(lambda t: t.fixed)),
+            exposed=L.u8(cat(lambda t: t.exposed)), rate=L.f64(cat(lambda t: t.rates.ravel())),
+        )
+        k = self._keep
+        d = L.MeshDesc()
+        d.n_nodes, d.n_elements = int(node_off[-1]), int(elem_off[-1])
+        d.node_pos, d.conn, d.ref_pos = L.ptr(k["pos"]), L.ptr(k["conn"]), L.ptr(k["xref"])
+        d.lam, d.mu, d.eta_int = L.ptr(k["lam"]), L.ptr(k["mu"]), L.ptr(k["eta"])
+        d.eta_ext, d.fixed, d.exposed, d.growth_rate = L.ptr(k["eta_ext"]), L.ptr(k["fixed"]), L.ptr(k["exposed"]), L.ptr(k["rate"])
+        d.n_segments = len(self.tissues)
+        d.node_seg_ptr = L.ptr(self.node_seg_ptr) if len(self.tissues) > 1 else None
+        d.elem_seg_ptr = L.ptr(self.elem_seg_ptr) if len(self.tissues) > 1 else None
+        h = C.c_void_p()
+        rc = self._lib.tf_create(C.byref(d), C.byref(h))
+        if rc != L.TF_OK:
+            raise L.TfError(rc, self._lib.tf_last_error(None).decode())
+        self.h = h
+        self.N, self.E = d.n_nodes, d.n_elements
+        nb = C.c_int32()
+        self._lib.tf_get_sizes(self.h, None, None, C.byref(nb), None)
+        self.nb = nb.value
+        if opts:
+            self.set_options(**opts)
+
+    # ---- lifetime / options --------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.tf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, allow=()):
+        if rc != L.TF_OK and rc not in allow:
+            raise L.TfError(rc, self._lib.tf_last_error(self.h).decode())
+        return rc
+
+    def options(self) -> dict:
+        o = L.SolverOpts()
+        self._check(self._lib.tf_get_options(self.h, C.byref(o)))
+        return {k: getattr(o, k) for k, _ in o._fields_}
+
+    def set_options(self, **kw):
+        o = L.SolverOpts()
+        self._check(self._lib.tf_get_options(self.h, C.byref(o)))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self._check(self._lib.tf_set_options(self.h, C.byref(o)))
+
+    # ---- state ------------------------------------------------------------------------------
+    @property
+    def positions(self) -> np.ndarray:
+        x = np.zeros(3 * self.N)
+        self._check(self._lib.tf_get_positions(self.h, L.ptr(x)))
+        return x
+
+    @positions.setter
+    def positions(self, x):
+        x = L.f64(x).ravel()
+        self._check(self._lib.tf_set_positions(self.h, L.ptr(x)))
+
+    def get_fg(self):
+        a, b = np.zeros(9 * self.E), np.zeros(9 * self.E)
+        self._check(self._lib.tf_get_fg(self.h, L.ptr(a), L.ptr(b)))
+        return a.reshape(-1, 3, 3), b.reshape(-1, 3, 3)
+
+    def set_fg(self, Fg):
+        Fg = L.f64(Fg).ravel()
+        self._check(self._lib.tf_set_fg(self.h, L.ptr(Fg)))
+
+    def set_growth_rates(self, rates):
+        r = L.f64(rates).ravel()
+        self._check(self._lib.tf_set_growth_rates(self.h, L.ptr(r)))
+
+    def set_material(self, lam=None, mu=None, eta=None):
+        a = None if lam is None else L.f64(lam)
+        b = None if mu is None else L.f64(mu)
+        c = None if eta is None else L.f64(eta)
+        self._check(self._lib.tf_set_material(self.h, L.ptr(a), L.ptr(b), L.ptr(c)))
+
+    # ---- per step -----------------------------------------------------------------------------
+    def grow(self, dt) -> int:
+        n = C.c_int32()
+        self._check(self._lib.tf_grow(self.h, dt, C.byref(n)))
+        return n.value
+
+    def updateNodeViscositySurfaces(self):
+        self._check(self._lib.tf_update_viscosity_surfaces(self.h))
+
+    def viscosity_surfaces(self):
+        s = np.zeros(self.N)
+        self._check(self._lib.tf_get_viscosity_surfaces(self.h, L.ptr(s)))
+        return s
+
+    # ---- NewtonRaphsonSolver call sequence -----------------------------------------------------
+    def constructUnMatrix(self):
+        """constructUnMatrix + initialteUkMatrix: un <- x, uk <- un."""
+        self._check(self._lib.tf_begin_step(self.h))
+
+    initialteUkMatrix = constructUnMatrix
+
+    def calculateForcesAndJacobianMatrixNR(self, dt):
+        """Element forces/tangents, their scatter, external viscosity, fixed DoFs (device)."""
+        return self._check(self._lib.tf_assemble(self.h, dt), allow=(L.TF_NEG_JACOBIAN,))
+
+    @property
+    def gSum(self) -> np.ndarray:
+        g = np.zeros(3 * self.N)
+        self._check(self._lib.tf_get_residual(self.h, L.ptr(g)))
+        return g
+
+    def jacobian_bsr(self):
+        rp = np.zeros(self.N + 1, np.int32)
+        col = np.zeros(self.nb, np.int32)
+        val = np.zeros(9 * self.nb)
+        self._check(self._lib.tf_get_jacobian_bsr(self.h, L.ptr(rp), L.ptr(col), L.ptr(val)))
+        return rp, col, val.reshape(-1, 3, 3)
+
+    def jacobian_csr(self):
+        """Natural-orientation Jacobian M as a scipy CSR matrix (3N x 3N)."""
+        import scipy.sparse as sp
+        rp, col, val = self.jacobian_bsr()
+        return sp.bsr_matrix((val, col, rp), shape=(3 * self.N, 3 * self.N)).tocsr()
+
+    def element_dump(self, e, dt):
+        fe, fv, Ke, Kv = np.zeros(18), np.zeros(18), np.zeros(324), np.zeros(324)
+        self._check(self._lib.tf_element_dump(self.h, int(e), dt, L.ptr(fe), L.ptr(fv), L.ptr(Ke), L.ptr(Kv)))
+        return dict(fe=fe, fv=fv, Ke=Ke.reshape(18, 18), Kv=Kv.reshape(18, 18))
+
+    def solveForDeltaU(self):
+        du = np.zeros(3 * self.N)
+        it = C.c_int32()
+        rr = C.c_double()
+        rc = self._check(self._lib.tf_solve(self.h, L.ptr(du), C.byref(it), C.byref(rr)), allow=(L.TF_KRYLOV_STALL,))
+        self.last_krylov = dict(iterations=it.value, rel_residual=rr.value, status=rc)
+        return du
+
+    def updateUkInIteration(self):
+        """checkConvergenceViaDeltaU + updateUkInIteration: returns (||dU||, converged)."""
+        d = C.c_double()
+        c = C.c_int32()
+        self._check(self._lib.tf_update_uk(self.h, C.byref(d), C.byref(c)))
+        return d.value, bool(c.value)
+
+    def updateStepNR(self, dt):
+        st = L.StepStats()
+        rc = self._check(self._lib.tf_nr_step(self.h, dt, C.byref(st)), allow=(L.TF_NOT_CONVERGED,))
+        out = st.as_dict()
+        out["status"] = rc
+        return out
+
+    def runOneStep(self, dt):
+        """Growth, viscosity surfaces and the NR loop of one timestep."""
+        st = L.StepStats()
+        rc = self._check(self._lib.tf_timestep(self.h, dt, C.byref(st)), allow=(L.TF_NOT_CONVERGED,))
+        out = st.as_dict()
+        out["status"] = rc
+        return out
+
+    def segment_stats(self):
+        S = len(self.tissues)
+        it, dn, cv = np.zeros(S, np.int32), np.zeros(S), np.zeros(S, np.int32)
+        self._check(self._lib.tf_get_segment_stats(self.h, L.ptr(it), L.ptr(dn), L.ptr(cv)))
+        return it, dn, cv
+
+    # ---- measurement ------------------------------------------------------------------------------
+    def kernel_timing(self, name, enable=-1):
+        avg, n = C.c_double(), C.c_int64()
+        self._check(self._lib.tf_kernel_timing(self.h, name.encode(), enable, C.byref(avg), C.byref(n)))
+        return avg.value, n.value
+
+    def kernel_bytes(self, name):
+        b = C.c_double()
+        self._check(self._lib.tf_kernel_bytes(self.h, name.encode(), C.byref(b)))
+        return b.value
