@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Benchmark: implicit NR timesteps/s of the wing-disc ellipse (~40k prisms) on one MI355X.
+
+One step = one implicit timestep: growth update (with the growth-rotation rule), nodal viscosity
+surfaces, and the full Newton-Raphson loop until ||dU||_2 <= 1e-8 (assembly of forces and the
+Jacobian, fixed DoFs, block-Jacobi preconditioned Krylov solve, update).  Inputs are resident in
+HBM before the timed region.  Prints one JSON line.
+
+The dominant kernel (the BSR SpMV inside the Krylov solver) is timed with HIP events on the
+stream it runs on; its algorithmic bytes per launch are nb*(72+4) + (N+1)*4 + 2*3N*8.
+The CPU baseline is the FP64 CPU oracle (same element math, OpenMP element loop, CSR assembly
+and an MKL PARDISO direct transposed solve) timed on the host cores over one full timestep.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="config2", choices=["config1", "config2"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--krylov", type=int, default=-1)
+    p.add_argument("--matrix-fp32", action="store_true")
+    p.add_argument("--no-graph", action="store_true")
+    return p.parse_args()
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary, scaled by the
+    copy-kernel calibration recorded in the same file (None if absent)."""
+    path = os.path.join(ROOT, "profiles", f"spmv_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt, re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(tissue, dt):
+    """One full timestep of the CPU oracle (growth, surfaces, NR loop with PARDISO)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the CPU oracle, timed as the baseline only
+
+    o = O.OracleTissue(tissue, solver="auto")
+    cores = O.lib().tfo_num_threads()
+    t0 = time.perf_counter()
+    it, _ = o.timestep(dt)
+    sec = time.perf_counter() - t0
+    return {"value": 1.0 / sec, "unit": "timesteps/s", "cores": int(cores), "kind": "port",
+            "sample": f"one full implicit timestep (growth + surfaces + {it} NR iterations, literal 729-term "
+                      f"element tangent, CSR assembly, {o.solver_kind()} transposed solve) of the same workload",
+            "seconds": sec, "nr_iterations": it}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is described "
+              "in DESIGN.md and is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    from tissuefolding_paperversion_amd import configs
+    from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
+
+    tissue = configs.config2() if a.workload == "config2" else configs.config1()
+    dt = tissue.dt
+    opts = {"krylov": a.krylov}
+    if a.matrix_fp32:
+        opts["matrix_precision"] = 1
+    if a.no_graph:
+        opts["use_graph"] = 0
+    s = NewtonRaphsonSolver(tissue, opts)
+    for _ in range(a.warmup):
+        s.runOneStep(dt)
+    s.kernel_timing("spmv", enable=1)
+    s.kernel_timing("assemble_elements", enable=1)
+    s.kernel_timing("assemble_gather", enable=1)
+    s.positions  # synchronises the stream
+    t0 = time.perf_counter()
+    stats = [s.runOneStep(dt) for _ in range(a.steps)]
+    s.positions
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    spmv_ms, spmv_n = s.kernel_timing("spmv", enable=0)
+    el_ms, el_n = s.kernel_timing("assemble_elements", enable=0)
+    ga_ms, ga_n = s.kernel_timing("assemble_gather", enable=0)
+    spmv_bytes = s.kernel_bytes("spmv")
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
+    nr = [st["nr_iterations"] for st in stats]
+    kr = [st["krylov_iterations"] for st in stats]
+    out = {
+        "metric": "NR timesteps/s at 40k prisms, plain FP64 vs mixed (disp err <=1e-8 rel)",
+        "value": a.steps / sec,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * sec / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Delaunay ellipse mesh and seeded Gaussian-bump growth map)",
+        "config": {"workload": f"{tissue.name}: {tissue.n_elements} prisms, {tissue.n_nodes} nodes, "
+                               f"{3 * tissue.n_nodes} DoF, dt=60 s, plain FP64",
+                   "prisms": tissue.n_elements, "nodes": tissue.n_nodes, "bsr_blocks": s.nb,
+                   "krylov": "auto (CG, BiCGSTAB fallback)" if a.krylov < 0 else str(a.krylov),
+                   "matrix_values": "fp32" if a.matrix_fp32 else "fp64"},
+        "nr_iterations_per_step": float(np.mean(nr)),
+        "krylov_iterations_per_nr_iteration": float(np.sum(kr) / max(1, np.sum(nr))),
+        "converged_steps": int(sum(st["converged"] for st in stats)),
+        "roofline": {"kernel": "spmv (BSR3 FP64, CSR-stream tiles, fused dot products)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(a.workload), "avg_launch_ms": spmv_ms, "launches": spmv_n,
+                     "algorithmic_bytes_per_launch": spmv_bytes},
+        "kernels": {"assemble_elements_ms": el_ms, "assemble_gather_ms": ga_ms, "spmv_ms": spmv_ms},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(configs.config2() if a.workload == "config2" else configs.config1(), dt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,95 @@
+"""Global-level pins of the CPU oracle: brute-force assembly on the reference's own one- and
+two-prism meshes, dense/CSR/direct-solver agreement, fixed DoFs, and the exact answer of the
+config-1 disc under uniform growth."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from tissuefolding_paperversion_amd import configs, mesh as M
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _tissue_from_mesh(path, eta_int=0.0, eta_ext=0.0, E=1000.0, nu=0.3):
+    m = M.read_mesh(path)
+    Ey, lam, mu = configs.assign_material(m, E, E, E, nu)
+    fixed = np.zeros(3 * m.n_nodes, np.uint8)
+    return configs.Tissue(m, lam, mu, np.full(m.n_elements, eta_int), configs.assign_external_viscosity(m, eta_ext, eta_ext),
+                          fixed, np.zeros((m.n_elements, 3)), m.exposed_faces())
+
+
+@pytest.mark.parametrize("name", ["OnePrism.mesh", "TwoPrisms.mesh", "DoublePrism.mesh", "Wind48hr_02.mesh"])
+def test_brute_force_scatter(oracle, name):
+    """Dense K and g equal a direct loop over elements and local DoFs (reference orientation)."""
+    t = _tissue_from_mesh(os.path.join(GOLD, name), eta_int=10.0, eta_ext=5.0)
+    rng = np.random.default_rng(0)
+    o = oracle.OracleTissue(t)
+    o.update_viscosity_surfaces()
+    X = t.mesh.pos.ravel().copy()
+    x = X + 0.05 * rng.standard_normal(X.shape)
+    dt = 60.0
+    K, g = o.assemble(x, X, dt, dense=True)
+    n = 3 * t.n_nodes
+    Kb, gb = np.zeros((n, n)), np.zeros(n)
+    v = (x - X) / dt
+    for e in range(t.n_elements):
+        c = t.mesh.conn[e]
+        dofs = (3 * c[:, None] + np.arange(3)).ravel()
+        out = oracle.element(t.mesh.xref[e], x[dofs], np.eye(3), t.lam[e], t.mu[e], t.eta_int[e], dt, v=v[dofs])
+        Kb[np.ix_(dofs, dofs)] += out["Ke"] + out["Kv"]
+        gb[dofs] += out["fe"] + out["fv"]
+    Sa = o.Sa
+    for i in range(t.n_nodes):
+        for j in range(3):
+            d = 3 * i + j
+            Kb[d, d] += Sa[i] * t.eta_ext[d] / dt
+            gb[d] -= Sa[i] * t.eta_ext[d] * v[d]
+    assert rel_err(K, Kb) < 1e-13
+    assert rel_err(g, gb) < 1e-13
+
+
+def test_fixed_dofs_unit_rows(oracle):
+    t = configs.disc_tissue(radius=10.0, h=4.0)
+    o = oracle.OracleTissue(t)
+    X = t.mesh.pos.ravel().copy()
+    x = X + 0.01
+    K, g = o.assemble(x, X, 60.0, dense=True)
+    fx = np.where(t.fixed)[0]
+    assert len(fx) > 0
+    for d in fx:
+        e = np.zeros(len(g))
+        e[d] = 1.0
+        assert np.array_equal(K[d], e) and np.array_equal(K[:, d], e)
+        assert g[d] == 0.0
+
+
+def test_dense_csr_and_solvers_agree(oracle):
+    t = configs.disc_tissue(radius=18.0, h=4.5, eta_ext=30.0)
+    o = oracle.OracleTissue(t)
+    o.grow(60.0)
+    o.update_viscosity_surfaces()
+    X = o.pos.copy()
+    Kd, gd = o.assemble(X, X, 60.0, dense=True)
+    Ks, gs = o.assemble(X, X, 60.0, dense=False)
+    assert np.array_equal(Ks.toarray(), Kd)
+    assert np.array_equal(gs, gd)
+    xd = o.solve(Kd, gd)
+    xs = o.solve(Ks, gs)
+    assert rel_err(xs, xd) < 1e-10
+    # transposed solve: K^T x = g
+    assert np.linalg.norm(Kd.T @ xd - gd) < 1e-10 * np.linalg.norm(gd)
+
+
+def test_config1_exact_answer(oracle):
+    t = configs.config1()
+    o = oracle.OracleTissue(t)
+    X = t.mesh.pos.copy()
+    r = t.rates[0, 0]
+    for step in range(1, 3):
+        it, d = o.timestep(t.dt)
+        assert d <= 1e-8
+        exact = X.copy()
+        exact[:, :2] *= np.exp(r * t.dt * step)
+        assert rel_err(o.pos.reshape(-1, 3), exact) < 1e-12

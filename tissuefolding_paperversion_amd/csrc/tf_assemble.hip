@@ -378,42 +378,6 @@ __global__ void k_fill(double* dst, double v, int64_t n)
     if (i < n) dst[i] = v;
 }
 
-// per-segment ||x||^2 over rows of each tile, last tile of a segment finishes (deterministic order)
-__global__ __launch_bounds__(256) void k_seg_norm(const double* __restrict__ x, const int32_t* __restrict__ tile_r0,
-                                                  const int32_t* __restrict__ tile_r1, const int32_t* __restrict__ tile_seg,
-                                                  const int32_t* __restrict__ seg_tile_ptr, int32_t* __restrict__ counter,
-                                                  double* __restrict__ partial, double* __restrict__ out)
-{
-    __shared__ double red[256];
-    __shared__ int last;
-    const int t = blockIdx.x;
-    const int d0 = 3 * tile_r0[t], d1 = 3 * tile_r1[t];
-    double s = 0.0;
-    for (int d = d0 + threadIdx.x; d < d1; d += blockDim.x) s += x[d] * x[d];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    const int sg = tile_seg[t];
-    if (threadIdx.x == 0) {
-        partial[t * TF_RED_SLOTS] = red[0];
-        __threadfence();
-        const int ntiles = seg_tile_ptr[sg + 1] - seg_tile_ptr[sg];
-        last = (atomicAdd(&counter[sg], 1) == ntiles - 1);
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double tot = 0.0;
-        for (int k = seg_tile_ptr[sg]; k < seg_tile_ptr[sg + 1]; ++k)
-            tot += __hip_atomic_load(&partial[k * TF_RED_SLOTS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        out[sg] = sqrt(tot);
-        counter[sg] = 0;
-    }
-}
-
 __global__ __launch_bounds__(256) void k_update_uk(double* __restrict__ uk, const double* __restrict__ du,
                                                    const int32_t* __restrict__ tile_r0, const int32_t* __restrict__ tile_r1,
                                                    const int32_t* __restrict__ tile_seg, const SegScalars* __restrict__ seg)
@@ -494,13 +458,6 @@ void launch_fill(tf_handle* h, double* dst, double v, int64_t n)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, dst, v, n);
-    TF_HIP(hipGetLastError());
-}
-
-void launch_seg_norm(tf_handle* h, const double* x)
-{
-    hipLaunchKernelGGL(k_seg_norm, dim3(h->T), dim3(256), 0, h->stream, x, h->tile_r0, h->tile_r1, h->tile_seg,
-                       h->seg_tile_ptr, h->seg_counter, h->partial, h->seg_norm);
     TF_HIP(hipGetLastError());
 }
 

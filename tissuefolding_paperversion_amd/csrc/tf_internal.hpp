@@ -20,21 +20,28 @@
 
 #include "../../include/tf_nr.h"
 
-#define TF_TILE_MAX_BLOCKS 1024
+#define TF_TILE_MAX_BLOCKS 512
 #define TF_TILE_MAX_ROWS 64
-#define TF_RED_SLOTS 4       // partial sums per tile per reduction kernel
+#define TF_RED_SLOTS 16      // partial-sum slots per tile; partial[slot * T + tile]
 
-// Per-tissue Krylov / NR scalars, one struct per segment, resident on the device.
+// Per-tissue Krylov / NR scalars, one struct per segment, resident on the device.  Within a
+// Krylov iteration the scalars (alpha, beta, omega, ...) are not stored: every tile re-derives
+// them from the tiles' partial sums (see tf_krylov.hip).  These fields are written by the first
+// tile of a tissue only and read by later kernels.
 struct SegScalars {
-    double rho, rho_prev, alpha, omega, beta;
     double bnorm2;           // ||b||^2
-    double rnorm2;           // ||r||^2 (latest)
+    double rnorm2;           // ||r||^2 at the last convergence test
     double tol2;             // squared absolute stopping threshold
-    double tmp0, tmp1, tmp2, tmp3;
+    double pad0;
     int32_t converged;       // Krylov converged (or nothing to do)
-    int32_t stalled;         // breakdown / cap
+    int32_t stalled;         // breakdown / iteration cap
     int32_t iters;           // Krylov iterations done
     int32_t active;          // NR still iterating on this segment
+};
+
+// partial-sum slots
+enum {
+    SL_RHO0 = 0, SL_RHO1, SL_RR0, SL_RR1, SL_RV, SL_TS, SL_TT, SL_BB, SL_PQ, SL_RZ0, SL_RZ1, SL_NORM, SL_COUNT
 };
 
 struct KernelTimer {

@@ -4,9 +4,13 @@
 // 3x3 blocks).  One 256-lane workgroup per tile.  The SpMV is CSR-stream: lanes stream the
 // tile's blocks contiguously (component-major values, so every load instruction is coalesced),
 // park the 3-vector products in LDS, then one lane per (row, component) sums its row.
-// Dot products are fused into the kernel that produces their operands; every tile writes its
-// partial sums and the last tile of a tissue to arrive (atomic ticket) sums the partials in
-// tile order and updates that tissue's scalars -- deterministic, no host round trip.
+//
+// Reductions carry no atomics and no device-wide fences: the kernel producing a dot product's
+// operands writes one partial per tile (partial[slot*T + tile]); every tile of the next kernel
+// that needs the scalar re-sums its tissue's partials with a fixed lane/tree order.  All tiles
+// of a tissue therefore derive bit-identical scalars (alpha, beta, omega, convergence decision)
+// and kernel ordering alone makes the partials visible.  The first tile of a tissue records the
+// outcome (converged / stalled / iteration count) for the host.
 //
 // Preconditioner: 3x3 nodal block Jacobi (diagonal blocks inverted during assembly).
 //   BiCGSTAB (van der Vorst), right preconditioned, 5 kernels / iteration, 2 SpMV.
@@ -20,18 +24,17 @@
 namespace {
 
 constexpr int WG = 256;
+constexpr int NW = WG / 64;
 
 struct TileArgs {
     const int32_t* tile_r0;
     const int32_t* tile_r1;
     const int32_t* tile_seg;
     const int32_t* seg_tile_ptr;
-    int32_t* counter;
     double* partial;
     SegScalars* seg;
+    int T;
 };
-
-__device__ inline bool seg_done(const SegScalars& s) { return s.converged || s.stalled; }
 
 __device__ inline double wave_sum(double v)
 {
@@ -40,64 +43,74 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Block-wide sum of two values (all lanes of the block must call). Result valid in lane 0.
-__device__ inline void block_sum2(double& a, double& b)
+// Tile partials: reduce v[q] over the workgroup and store partial[slot[q]*T + t].
+template <int K>
+__device__ inline void tile_write(const TileArgs& ta, int t, const int (&slot)[K], double (&v)[K])
 {
-    __shared__ double ra[WG / 64], rb[WG / 64];
-    a = wave_sum(a);
-    b = wave_sum(b);
+    __shared__ double red[K][NW];
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { ra[w] = a; rb[w] = b; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double sa = 0.0, sb = 0.0;
-        for (int k = 0; k < WG / 64; ++k) { sa += ra[k]; sb += rb[k]; }
-        a = sa; b = sb;
-    }
-}
-
-// Write this tile's partials; returns true (in lane 0) when this tile is the last of its tissue,
-// in which case tot[] holds the tissue totals summed in tile order.
-__device__ inline bool tile_reduce(const TileArgs& ta, int t, double a, double b, double* tot)
-{
-    __shared__ int last;
-    block_sum2(a, b);
-    const int sg = ta.tile_seg[t];
-    if (threadIdx.x == 0) {
-        ta.partial[t * TF_RED_SLOTS + 0] = a;
-        ta.partial[t * TF_RED_SLOTS + 1] = b;
-        __threadfence();
-        const int nt = ta.seg_tile_ptr[sg + 1] - ta.seg_tile_ptr[sg];
-        last = (atomicAdd(&ta.counter[sg], 1) == nt - 1);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        v[q] = wave_sum(v[q]);
+        if (l == 0) red[q][w] = v[q];
     }
     __syncthreads();
-    if (threadIdx.x == 0 && last) {
-        __threadfence();
-        double s0 = 0.0, s1 = 0.0;
-        for (int k = ta.seg_tile_ptr[sg]; k < ta.seg_tile_ptr[sg + 1]; ++k) {
-            s0 += __hip_atomic_load(&ta.partial[k * TF_RED_SLOTS + 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s1 += __hip_atomic_load(&ta.partial[k * TF_RED_SLOTS + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) s += red[q][k];
+            ta.partial[(size_t)slot[q] * ta.T + t] = s;
         }
-        tot[0] = s0;
-        tot[1] = s1;
-        ta.counter[sg] = 0;
-        return true;
     }
-    return false;
 }
 
-__device__ inline void dinv_apply(const double* __restrict__ dinv, const double* __restrict__ r, int d, double& z)
+// Tissue totals of K partial slots, identical in every lane and every tile of the tissue.
+template <int K>
+__device__ inline void seg_sums(const TileArgs& ta, int sg, const int (&slot)[K], double (&out)[K])
+{
+    __shared__ double red[K][NW];
+    const int t0 = ta.seg_tile_ptr[sg], t1 = ta.seg_tile_ptr[sg + 1];
+    double s[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) s[q] = 0.0;
+    for (int k = t0 + threadIdx.x; k < t1; k += WG)
+#pragma unroll
+        for (int q = 0; q < K; ++q) s[q] += ta.partial[(size_t)slot[q] * ta.T + k];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        s[q] = wave_sum(s[q]);
+        if (l == 0) red[q][w] = s[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) a += red[q][k];
+        out[q] = a;
+    }
+    __syncthreads();
+}
+
+__device__ inline bool first_tile(const TileArgs& ta, int t, int sg) { return t == ta.seg_tile_ptr[sg]; }
+
+__device__ inline bool seg_done(const SegScalars& s) { return s.converged || s.stalled; }
+
+__device__ inline double dinv_row(const double* __restrict__ dinv, const double* __restrict__ r, int d)
 {
     const int A = d / 3, i = d % 3;
     const double* D = dinv + 9 * A + 3 * i;
     const double* rr = r + 3 * A;
-    z = D[0] * rr[0] + D[1] * rr[1] + D[2] * rr[2];
+    return D[0] * rr[0] + D[1] * rr[1] + D[2] * rr[2];
 }
 
-// CSR-stream product of one tile: prod[3*(j-b0)+c] = (A_j x_col(j))_c, x given by xin(col, c).
+// CSR-stream products of one tile: prod[3*(j-b0)+c] = (A_j x_col(j))_c, x given by xin(col, c).
 template <typename VT, typename XF>
-__device__ inline void tile_products(const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                                     const VT* __restrict__ val, int nb, int b0, int b1, XF xin, double* prod)
+__device__ inline void tile_products(const int32_t* __restrict__ col, const VT* __restrict__ val, int nb, int b0, int b1,
+                                     XF xin, double* prod)
 {
     for (int j = b0 + threadIdx.x; j < b1; j += WG) {
         const int c = col[j];
@@ -122,6 +135,8 @@ __device__ inline double row_sum(const int32_t* __restrict__ row_ptr, int r, int
     return s;
 }
 
+#define XLOAD(vec) [&](int c, double& a0, double& a1, double& a2) { a0 = vec[3 * c]; a1 = vec[3 * c + 1]; a2 = vec[3 * c + 2]; }
+
 // ---------------------------------------------------------------------------------------------
 // plain y = M x
 template <typename VT>
@@ -134,19 +149,53 @@ __global__ __launch_bounds__(WG) void k_spmv(const int32_t* __restrict__ row_ptr
     const int t = blockIdx.x;
     const int r0 = tile_r0[t], r1 = tile_r1[t];
     const int b0 = row_ptr[r0], b1 = row_ptr[r1];
-    tile_products<VT>(row_ptr, col, val, nb, b0, b1,
-                      [&](int c, double& a0, double& a1, double& a2) { a0 = x[3 * c]; a1 = x[3 * c + 1]; a2 = x[3 * c + 2]; },
-                      prod);
+    tile_products<VT>(col, val, nb, b0, b1, XLOAD(x), prod);
     for (int task = threadIdx.x; task < 3 * (r1 - r0); task += WG) {
         const int r = r0 + task / 3, comp = task % 3;
         y[3 * r + comp] = row_sum(row_ptr, r, comp, b0, prod);
     }
 }
 
+// one workgroup per tissue: ||b||^2 -> tolerance, initial state of the tissue's solve
+__global__ __launch_bounds__(WG) void k_seg_init(TileArgs ta, double rtol2, double atol2)
+{
+    const int sg = blockIdx.x;
+    const int sl[1] = {SL_BB};
+    double tot[1];
+    seg_sums<1>(ta, sg, sl, tot);
+    if (threadIdx.x == 0) {
+        SegScalars& S = ta.seg[sg];
+        S.bnorm2 = tot[0];
+        S.rnorm2 = tot[0];
+        S.tol2 = fmax(rtol2 * tot[0], atol2);
+        S.iters = 0;
+        S.stalled = 0;
+        S.converged = (!S.active || tot[0] <= S.tol2) ? 1 : 0;
+    }
+}
+
+// Convergence / cap test shared by the first kernel of every iteration.  Returns true when the
+// tissue stops (all tiles of the tissue take the same decision).
+__device__ inline bool check_stop(const TileArgs& ta, int t, int sg, int slot_rr, int maxit)
+{
+    const int sl[1] = {slot_rr};
+    double rr[1];
+    seg_sums<1>(ta, sg, sl, rr);
+    SegScalars& S = ta.seg[sg];
+    const bool conv = rr[0] <= S.tol2;
+    const bool cap = !conv && (S.iters >= maxit || !isfinite(rr[0]));
+    if ((conv || cap) && first_tile(ta, t, sg) && threadIdx.x == 0) {
+        S.rnorm2 = rr[0];
+        if (conv) S.converged = 1;
+        else S.stalled = 1;
+    }
+    return conv || cap;
+}
+
 // ---------------------------------------------------------------------------------------------
 // BiCGSTAB
 __global__ __launch_bounds__(WG) void k_bicg_init(TileArgs ta, const double* __restrict__ b, double* x, double* r, double* r0,
-                                                  double* p, double* v, double rtol2, double atol2)
+                                                  double* p, double* v)
 {
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
@@ -158,39 +207,41 @@ __global__ __launch_bounds__(WG) void k_bicg_init(TileArgs ta, const double* __r
         x[d] = 0.0; r[d] = bv; r0[d] = bv; p[d] = 0.0; v[d] = 0.0;
         s += bv * bv;
     }
-    double tot[2];
-    if (tile_reduce(ta, t, s, 0.0, tot)) {
-        SegScalars& S = ta.seg[sg];
-        S.bnorm2 = tot[0];
-        S.rho = tot[0];
-        S.rho_prev = 1.0; S.alpha = 1.0; S.omega = 1.0;
-        S.beta = tot[0];
-        S.rnorm2 = tot[0];
-        S.tol2 = fmax(rtol2 * tot[0], atol2);
-        S.iters = 0;
-        S.stalled = 0;
-        S.converged = (!active || tot[0] <= S.tol2) ? 1 : 0;
-    }
+    const int sl[3] = {SL_BB, SL_RR0, SL_RHO0};
+    double vv[3] = {s, s, s};
+    tile_write<3>(ta, t, sl, vv);
 }
 
+template <int P>
 __global__ __launch_bounds__(WG) void k_bicg_p(TileArgs ta, const double* __restrict__ dinv, const double* __restrict__ r,
-                                               double* __restrict__ p, const double* __restrict__ v, double* __restrict__ ph)
+                                               double* __restrict__ p, const double* __restrict__ v, double* __restrict__ ph,
+                                               int maxit)
 {
     const int t = blockIdx.x;
-    const SegScalars& S = ta.seg[ta.tile_seg[t]];
+    const int sg = ta.tile_seg[t];
+    SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    const double beta = S.beta, omega = S.omega;
+    if (check_stop(ta, t, sg, SL_RR0 + P, maxit)) return;
+    double beta = 0.0, omega = 0.0;
+    if (S.iters > 0) {
+        const int sl[5] = {SL_RHO0 + P, SL_RHO0 + (1 - P), SL_RV, SL_TS, SL_TT};
+        double q[5];
+        seg_sums<5>(ta, sg, sl, q);
+        const double alpha_prev = q[1] / q[2];
+        omega = q[3] / q[4];
+        beta = (q[0] / q[1]) * (alpha_prev / omega);
+        if (!isfinite(beta) || q[0] == 0.0 || omega == 0.0) {
+            if (first_tile(ta, t, sg) && threadIdx.x == 0) S.stalled = 1;
+            return;
+        }
+    }
     const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
     for (int d = d0 + threadIdx.x; d < d1; d += WG) p[d] = r[d] + beta * (p[d] - omega * v[d]);
     __syncthreads();
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
-        double z;
-        dinv_apply(dinv, p, d, z);
-        ph[d] = z;
-    }
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) ph[d] = dinv_row(dinv, p, d);
 }
 
-// y = M x with fused dot products; MODE 0: (r0, y) -> alpha; MODE 1: (y, s), (y, y) -> omega
+// y = M x with fused dot products; MODE 0: (w, y) -> SL_RV; MODE 1: (y, w) -> SL_TS, (y, y) -> SL_TT
 template <typename VT, int MODE>
 __global__ __launch_bounds__(WG) void k_bicg_spmv(TileArgs ta, const int32_t* __restrict__ row_ptr,
                                                   const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
@@ -203,56 +254,63 @@ __global__ __launch_bounds__(WG) void k_bicg_spmv(TileArgs ta, const int32_t* __
     if (seg_done(ta.seg[sg])) return;
     const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
     const int b0 = row_ptr[r0], b1 = row_ptr[r1];
-    tile_products<VT>(row_ptr, col, val, nb, b0, b1,
-                      [&](int c, double& a0, double& a1, double& a2) { a0 = x[3 * c]; a1 = x[3 * c + 1]; a2 = x[3 * c + 2]; },
-                      prod);
+    tile_products<VT>(col, val, nb, b0, b1, XLOAD(x), prod);
     double s0 = 0.0, s1 = 0.0;
     for (int task = threadIdx.x; task < 3 * (r1 - r0); task += WG) {
         const int r = r0 + task / 3, comp = task % 3;
         const int d = 3 * r + comp;
         const double yv = row_sum(row_ptr, r, comp, b0, prod);
         y[d] = yv;
-        if (MODE == 0) s0 += w[d] * yv;
-        else { s0 += yv * w[d]; s1 += yv * yv; }
+        s0 += w[d] * yv;
+        if (MODE == 1) s1 += yv * yv;
     }
-    double tot[2];
-    if (tile_reduce(ta, t, s0, s1, tot)) {
-        SegScalars& S = ta.seg[sg];
-        if (MODE == 0) {
-            if (!(tot[0] != 0.0) || !isfinite(tot[0])) S.stalled = 1;
-            else S.alpha = S.rho / tot[0];
-        } else {
-            S.omega = (tot[1] > 0.0) ? tot[0] / tot[1] : 0.0;
-        }
+    if (MODE == 0) {
+        const int sl[1] = {SL_RV};
+        double vv[1] = {s0};
+        tile_write<1>(ta, t, sl, vv);
+    } else {
+        const int sl[2] = {SL_TS, SL_TT};
+        double vv[2] = {s0, s1};
+        tile_write<2>(ta, t, sl, vv);
     }
 }
 
+template <int P>
 __global__ __launch_bounds__(WG) void k_bicg_s(TileArgs ta, const double* __restrict__ dinv, const double* __restrict__ r,
                                                const double* __restrict__ v, double* __restrict__ s, double* __restrict__ sh)
 {
     const int t = blockIdx.x;
-    const SegScalars& S = ta.seg[ta.tile_seg[t]];
+    const int sg = ta.tile_seg[t];
+    SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    const double alpha = S.alpha;
+    const int sl[2] = {SL_RHO0 + P, SL_RV};
+    double q[2];
+    seg_sums<2>(ta, sg, sl, q);
+    const double alpha = q[0] / q[1];
+    if (!(q[1] != 0.0) || !isfinite(alpha)) {
+        if (first_tile(ta, t, sg) && threadIdx.x == 0) S.stalled = 1;
+        return;
+    }
     const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
     for (int d = d0 + threadIdx.x; d < d1; d += WG) s[d] = r[d] - alpha * v[d];
     __syncthreads();
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
-        double z;
-        dinv_apply(dinv, s, d, z);
-        sh[d] = z;
-    }
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) sh[d] = dinv_row(dinv, s, d);
 }
 
+template <int P>
 __global__ __launch_bounds__(WG) void k_bicg_x(TileArgs ta, const double* __restrict__ ph, const double* __restrict__ sh,
                                                const double* __restrict__ s, const double* __restrict__ tv,
-                                               const double* __restrict__ r0, double* __restrict__ x, double* __restrict__ r,
-                                               int maxit)
+                                               const double* __restrict__ r0, double* __restrict__ x, double* __restrict__ r)
 {
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
-    if (seg_done(ta.seg[sg])) return;
-    const double alpha = ta.seg[sg].alpha, omega = ta.seg[sg].omega;
+    SegScalars& S = ta.seg[sg];
+    if (seg_done(S)) return;
+    const int sl[4] = {SL_RHO0 + P, SL_RV, SL_TS, SL_TT};
+    double q[4];
+    seg_sums<4>(ta, sg, sl, q);
+    const double alpha = q[0] / q[1];
+    const double omega = (q[3] > 0.0) ? q[2] / q[3] : 0.0;
     const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
     double a = 0.0, b = 0.0;
     for (int d = d0 + threadIdx.x; d < d1; d += WG) {
@@ -262,24 +320,16 @@ __global__ __launch_bounds__(WG) void k_bicg_x(TileArgs ta, const double* __rest
         a += r0[d] * rn;
         b += rn * rn;
     }
-    double tot[2];
-    if (tile_reduce(ta, t, a, b, tot)) {
-        SegScalars& S = ta.seg[sg];
-        S.rho_prev = S.rho;
-        S.rho = tot[0];
-        S.rnorm2 = tot[1];
-        S.iters += 1;
-        if (!isfinite(tot[1])) S.stalled = 1;
-        else if (tot[1] <= S.tol2) S.converged = 1;
-        else if (S.iters >= maxit || tot[0] == 0.0 || S.omega == 0.0) S.stalled = 1;
-        else S.beta = (S.rho / S.rho_prev) * (S.alpha / S.omega);
-    }
+    const int so[2] = {SL_RHO0 + (1 - P), SL_RR0 + (1 - P)};
+    double vv[2] = {a, b};
+    tile_write<2>(ta, t, so, vv);
+    if (first_tile(ta, t, sg) && threadIdx.x == 0) S.iters += 1;
 }
 
 // ---------------------------------------------------------------------------------------------
 // CG
 __global__ __launch_bounds__(WG) void k_cg_init(TileArgs ta, const double* __restrict__ dinv, const double* __restrict__ b,
-                                                double* x, double* r, double* z, double* p_old, double rtol2, double atol2)
+                                                double* x, double* r, double* z, double* p_old)
 {
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
@@ -291,41 +341,39 @@ __global__ __launch_bounds__(WG) void k_cg_init(TileArgs ta, const double* __res
     __syncthreads();
     double a = 0.0, bb = 0.0;
     for (int d = d0 + threadIdx.x; d < d1; d += WG) {
-        double zz;
-        dinv_apply(dinv, r, d, zz);
+        const double zz = dinv_row(dinv, r, d);
         z[d] = zz;
         a += r[d] * zz;
         bb += r[d] * r[d];
     }
-    double tot[2];
-    if (tile_reduce(ta, t, a, bb, tot)) {
-        SegScalars& S = ta.seg[sg];
-        S.rho = tot[0];
-        S.bnorm2 = tot[1];
-        S.rnorm2 = tot[1];
-        S.beta = 0.0;
-        S.tol2 = fmax(rtol2 * tot[1], atol2);
-        S.iters = 0;
-        S.stalled = 0;
-        S.converged = (!active || tot[1] <= S.tol2) ? 1 : 0;
-    }
+    const int sl[3] = {SL_RZ0, SL_RR0, SL_BB};
+    double vv[3] = {a, bb, bb};
+    tile_write<3>(ta, t, sl, vv);
 }
 
-// p = z + beta p_old (own rows written to p_new, neighbours formed on the fly), q = M p, (p, q) -> alpha
-template <typename VT>
+// p = z + beta p_old (own rows written to p_new, neighbours formed on the fly), q = M p, (p, q)
+template <typename VT, int P>
 __global__ __launch_bounds__(WG) void k_cg_q(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, int nb, const double* __restrict__ z,
                                              const double* __restrict__ p_old, double* __restrict__ p_new,
-                                             double* __restrict__ q)
+                                             double* __restrict__ q, int maxit)
 {
     __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
-    if (seg_done(ta.seg[sg])) return;
-    const double beta = ta.seg[sg].beta;
+    SegScalars& S = ta.seg[sg];
+    if (seg_done(S)) return;
+    if (check_stop(ta, t, sg, SL_RR0 + P, maxit)) return;
+    double beta = 0.0;
+    if (S.iters > 0) {
+        const int sl[2] = {SL_RZ0 + P, SL_RZ0 + (1 - P)};
+        double rz[2];
+        seg_sums<2>(ta, sg, sl, rz);
+        beta = rz[0] / rz[1];
+    }
     const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
     const int b0 = row_ptr[r0], b1 = row_ptr[r1];
-    tile_products<VT>(row_ptr, col, val, nb, b0, b1,
+    tile_products<VT>(col, val, nb, b0, b1,
                       [&](int c, double& a0, double& a1, double& a2) {
                           a0 = z[3 * c] + beta * p_old[3 * c];
                           a1 = z[3 * c + 1] + beta * p_old[3 * c + 1];
@@ -342,22 +390,28 @@ __global__ __launch_bounds__(WG) void k_cg_q(TileArgs ta, const int32_t* __restr
         p_new[d] = pv;
         s0 += pv * qv;
     }
-    double tot[2];
-    if (tile_reduce(ta, t, s0, 0.0, tot)) {
-        SegScalars& S = ta.seg[sg];
-        if (!(tot[0] > 0.0) || !isfinite(tot[0])) S.stalled = 1;
-        else S.alpha = S.rho / tot[0];
-    }
+    const int sl[1] = {SL_PQ};
+    double vv[1] = {s0};
+    tile_write<1>(ta, t, sl, vv);
 }
 
+template <int P>
 __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restrict__ dinv, const double* __restrict__ p,
                                              const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
-                                             double* __restrict__ z, int maxit)
+                                             double* __restrict__ z)
 {
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
-    if (seg_done(ta.seg[sg])) return;
-    const double alpha = ta.seg[sg].alpha;
+    SegScalars& S = ta.seg[sg];
+    if (seg_done(S)) return;
+    const int sl[2] = {SL_RZ0 + P, SL_PQ};
+    double sc[2];
+    seg_sums<2>(ta, sg, sl, sc);
+    if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite
+        if (first_tile(ta, t, sg) && threadIdx.x == 0) S.stalled = 1;
+        return;
+    }
+    const double alpha = sc[0] / sc[1];
     const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
     for (int d = d0 + threadIdx.x; d < d1; d += WG) {
         x[d] = x[d] + alpha * p[d];
@@ -366,80 +420,90 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
     __syncthreads();
     double a = 0.0, b = 0.0;
     for (int d = d0 + threadIdx.x; d < d1; d += WG) {
-        double zz;
-        dinv_apply(dinv, r, d, zz);
+        const double zz = dinv_row(dinv, r, d);
         z[d] = zz;
         a += r[d] * zz;
         b += r[d] * r[d];
     }
-    double tot[2];
-    if (tile_reduce(ta, t, a, b, tot)) {
-        SegScalars& S = ta.seg[sg];
-        S.iters += 1;
-        S.rnorm2 = tot[1];
-        if (!isfinite(tot[1])) S.stalled = 1;
-        else if (tot[1] <= S.tol2) S.converged = 1;
-        else if (S.iters >= maxit) S.stalled = 1;
-        else {
-            S.beta = tot[0] / S.rho;
-            S.rho = tot[0];
-        }
-    }
+    const int so[2] = {SL_RZ0 + (1 - P), SL_RR0 + (1 - P)};
+    double vv[2] = {a, b};
+    tile_write<2>(ta, t, so, vv);
+    if (first_tile(ta, t, sg) && threadIdx.x == 0) S.iters += 1;
 }
 
-__global__ void k_mark_active(SegScalars* seg, int S)
+// ---------------------------------------------------------------------------------------------
+// per-tissue ||x||_2
+__global__ __launch_bounds__(WG) void k_tile_norm(TileArgs ta, const double* __restrict__ x)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S) { seg[s].converged = 0; seg[s].stalled = 0; }
+    const int t = blockIdx.x;
+    const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
+    double s = 0.0;
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) s += x[d] * x[d];
+    const int sl[1] = {SL_NORM};
+    double vv[1] = {s};
+    tile_write<1>(ta, t, sl, vv);
+}
+__global__ __launch_bounds__(WG) void k_seg_norm_fin(TileArgs ta, double* __restrict__ out)
+{
+    const int sg = blockIdx.x;
+    const int sl[1] = {SL_NORM};
+    double tot[1];
+    seg_sums<1>(ta, sg, sl, tot);
+    if (threadIdx.x == 0) out[sg] = sqrt(tot[0]);
 }
 
 TileArgs tile_args(tf_handle* h)
 {
-    return TileArgs{h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->seg_counter, h->partial, h->seg};
+    return TileArgs{h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->partial, h->seg, h->T};
 }
 
-template <typename VT>
+template <typename VT, int P>
 void enqueue_bicg_iter(tf_handle* h, const VT* val)
 {
     TileArgs ta = tile_args(h);
     const dim3 grid(h->T), blk(WG);
-    hipLaunchKernelGGL(k_bicg_p, grid, blk, 0, h->stream, ta, h->dinv, h->kr, h->kp, h->kv, h->kph);
+    hipLaunchKernelGGL(k_bicg_p<P>, grid, blk, 0, h->stream, ta, h->dinv, h->kr, h->kp, h->kv, h->kph,
+                       h->opts.krylov_max_iter);
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
     hipLaunchKernelGGL((k_bicg_spmv<VT, 0>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->kph, h->kv,
                        h->kr0);
     timer_end(h, h->t_spmv, ev);
-    hipLaunchKernelGGL(k_bicg_s, grid, blk, 0, h->stream, ta, h->dinv, h->kr, h->kv, h->ks, h->ksh);
+    hipLaunchKernelGGL(k_bicg_s<P>, grid, blk, 0, h->stream, ta, h->dinv, h->kr, h->kv, h->ks, h->ksh);
     timer_begin(h, h->t_spmv, ev);
     hipLaunchKernelGGL((k_bicg_spmv<VT, 1>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ksh, h->kt,
                        h->ks);
     timer_end(h, h->t_spmv, ev);
-    hipLaunchKernelGGL(k_bicg_x, grid, blk, 0, h->stream, ta, h->kph, h->ksh, h->ks, h->kt, h->kr0, h->kx, h->kr,
-                       h->opts.krylov_max_iter);
+    hipLaunchKernelGGL(k_bicg_x<P>, grid, blk, 0, h->stream, ta, h->kph, h->ksh, h->ks, h->kt, h->kr0, h->kx, h->kr);
 }
 
 // CG uses kp / kph as the ping-pong direction buffers, kv as q, ks as z.
-template <typename VT>
-void enqueue_cg_iter(tf_handle* h, const VT* val, int parity)
+template <typename VT, int P>
+void enqueue_cg_iter(tf_handle* h, const VT* val)
 {
     TileArgs ta = tile_args(h);
     const dim3 grid(h->T), blk(WG);
-    double* p_old = parity ? h->kph : h->kp;
-    double* p_new = parity ? h->kp : h->kph;
+    double* p_old = P ? h->kph : h->kp;
+    double* p_new = P ? h->kp : h->kph;
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_cg_q<VT>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ks, p_old, p_new, h->kv);
+    hipLaunchKernelGGL((k_cg_q<VT, P>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ks, p_old, p_new,
+                       h->kv, h->opts.krylov_max_iter);
     timer_end(h, h->t_spmv, ev);
-    hipLaunchKernelGGL(k_cg_x, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks,
-                       h->opts.krylov_max_iter);
+    hipLaunchKernelGGL(k_cg_x<P>, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks);
 }
 
 template <typename VT>
 void enqueue_chunk(tf_handle* h, const VT* val, int kind, int iters)
 {
-    for (int i = 0; i < iters; ++i) {
-        if (kind == TF_KRYLOV_CG) enqueue_cg_iter<VT>(h, val, i & 1);
-        else enqueue_bicg_iter<VT>(h, val);
+    for (int i = 0; i < iters; i += 2) {
+        if (kind == TF_KRYLOV_CG) {
+            enqueue_cg_iter<VT, 0>(h, val);
+            enqueue_cg_iter<VT, 1>(h, val);
+        } else {
+            enqueue_bicg_iter<VT, 0>(h, val);
+            enqueue_bicg_iter<VT, 1>(h, val);
+        }
     }
 }
 
@@ -460,20 +524,21 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const double rtol2 = h->opts.krylov_rtol * h->opts.krylov_rtol;
     const double atol2 = h->opts.krylov_atol * h->opts.krylov_atol;
     if (kind == TF_KRYLOV_CG)
-        hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp, rtol2, atol2);
+        hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
     else
-        hipLaunchKernelGGL(k_bicg_init, grid, blk, 0, h->stream, ta, h->g, h->kx, h->kr, h->kr0, h->kp, h->kv, rtol2,
-                           atol2);
+        hipLaunchKernelGGL(k_bicg_init, grid, blk, 0, h->stream, ta, h->g, h->kx, h->kr, h->kr0, h->kp, h->kv);
+    hipLaunchKernelGGL(k_seg_init, dim3(h->S), blk, 0, h->stream, ta, rtol2, atol2);
     TF_HIP(hipGetLastError());
     int chunk = std::max(2, h->opts.iters_per_sync);
-    chunk += chunk & 1;  // CG ping-pong needs an even chunk
+    chunk += chunk & 1;  // iterations come in parity pairs
     const bool timing = h->t_spmv.enabled;
     const bool graph = h->opts.use_graph && !timing;
     int done_iters = 0;
     while (true) {
         if (graph) {
-            if (!h->krylov_graph || h->krylov_graph_iters != chunk || h->krylov_graph_kind != kind * 4 + (int)sizeof(VT)) {
-                if (h->krylov_graph) { hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+            const int key = kind * 4 + (int)sizeof(VT);
+            if (!h->krylov_graph || h->krylov_graph_iters != chunk || h->krylov_graph_kind != key) {
+                if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
                 hipGraph_t gr;
                 TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
                 enqueue_chunk<VT>(h, val, kind, chunk);
@@ -481,7 +546,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
                 TF_HIP(hipGraphInstantiate(&h->krylov_graph, gr, nullptr, nullptr, 0));
                 TF_HIP(hipGraphDestroy(gr));
                 h->krylov_graph_iters = chunk;
-                h->krylov_graph_kind = kind * 4 + (int)sizeof(VT);
+                h->krylov_graph_kind = key;
             }
             TF_HIP(hipGraphLaunch(h->krylov_graph, h->stream));
         } else {
@@ -490,7 +555,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         }
         done_iters += chunk;
         if (all_done(h)) break;
-        if (done_iters > h->opts.krylov_max_iter + 2 * chunk) break;  // scalars mark the stall
+        if (done_iters > h->opts.krylov_max_iter + 2 * chunk) break;  // the cap marks the tissue stalled
     }
     timer_collect(h->t_spmv);
     int st = TF_OK;
@@ -510,13 +575,11 @@ int krylov_solve(tf_handle* h)
 {
     int kind = h->opts.krylov;
     const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
-    auto run = [&](int k) {
-        return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k);
-    };
+    auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     int st = run(kind);
     if (st == TF_KRYLOV_STALL && kind == TF_KRYLOV_CG) {
-        // not positive definite (or too slow): fall back to BiCGSTAB from zero
-        int cg_iters = h->last_krylov_iters;
+        // not positive definite (or too slow): BiCGSTAB from zero
+        const int cg_iters = h->last_krylov_iters;
         st = run(TF_KRYLOV_BICGSTAB);
         h->last_krylov_iters += cg_iters;
     }
@@ -532,5 +595,13 @@ void launch_spmv(tf_handle* h, const double* x, double* y)
     else
         hipLaunchKernelGGL(k_spmv<double>, dim3(h->T), dim3(WG), 0, h->stream, h->row_ptr, h->col, h->val, h->nb,
                            h->tile_r0, h->tile_r1, x, y);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_seg_norm(tf_handle* h, const double* x)
+{
+    TileArgs ta = tile_args(h);
+    hipLaunchKernelGGL(k_tile_norm, dim3(h->T), dim3(WG), 0, h->stream, ta, x);
+    hipLaunchKernelGGL(k_seg_norm_fin, dim3(h->S), dim3(WG), 0, h->stream, ta, h->seg_norm);
     TF_HIP(hipGetLastError());
 }

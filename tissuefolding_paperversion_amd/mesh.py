@@ -107,7 +107,7 @@ def write_mesh(mesh: TissueMesh, path: str) -> None:
         f.write(f"{mesh.n_nodes}\n")
         for i in range(mesh.n_nodes):
             p = mesh.pos[i]
-            f.write(f"{p[0]!r} {p[1]!r} {p[2]!r} {int(mesh.node_placement[i])} {int(mesh.node_type[i])} "
+            f.write(f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r} {int(mesh.node_placement[i])} {int(mesh.node_type[i])} "
                     f"{int(mesh.at_circumference[i])}\n")
         f.write(f"{mesh.n_elements}\n")
         for e in range(mesh.n_elements):
