@@ -1,0 +1,141 @@
+// tf_nr.hpp -- header-only C++ facade over the C ABI in tf_nr.h.
+//
+// tf::NewtonRaphsonSolver keeps the reference's NewtonRaphsonSolver method names
+// (TissueFolding/TissueFolding/SourceCode/NewtonRaphsonSolver.h:26-88) so that the driver loop of
+// Simulation::updateStepNR (Simulation.cpp:6449-6590) reads the same against the GPU path:
+//
+//     solver.constructUnMatrix();                 // un <- x, uk <- un
+//     while (!converged) {
+//         solver.calculateForcesAndJacobianMatrixNR(dt);   // forces, Ke/Kv, scatter, BCs
+//         solver.solveForDeltaU();
+//         converged = solver.checkConvergenceViaDeltaU();  // and updateUkInIteration
+//     }
+//
+// Element-side entry points of ShapeBase/Prism (calculateForces, calculateImplicitKElastic,
+// calculateImplicitKViscous) are folded into calculateForcesAndJacobianMatrixNR, which runs them
+// for every prism on the GPU; elementDump() returns one prism's arrays in the reference layout.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tf_nr.h"
+
+namespace tf {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code(code) {}
+    int code;
+};
+
+class NewtonRaphsonSolver {
+public:
+    explicit NewtonRaphsonSolver(const tf_mesh_desc& desc)
+    {
+        check(tf_create(&desc, &h_), nullptr);
+        int32_t n = 0, e = 0, nb = 0, s = 0;
+        tf_get_sizes(h_, &n, &e, &nb, &s);
+        nNodes_ = n;
+        nElements_ = e;
+    }
+    ~NewtonRaphsonSolver() { tf_destroy(h_); }
+    NewtonRaphsonSolver(const NewtonRaphsonSolver&) = delete;
+    NewtonRaphsonSolver& operator=(const NewtonRaphsonSolver&) = delete;
+
+    tf_solver_opts options() const
+    {
+        tf_solver_opts o;
+        tf_get_options(h_, &o);
+        return o;
+    }
+    void setOptions(const tf_solver_opts& o) { check(tf_set_options(h_, &o), h_); }
+
+    // ---- per step (Simulation::runOneStep) ----
+    int growShapeByFg(double dt)
+    {
+        int32_t n = 0;
+        check(tf_grow(h_, dt, &n), h_);
+        return n;
+    }
+    void updateNodeViscositySurfaces() { check(tf_update_viscosity_surfaces(h_), h_); }
+
+    // ---- NewtonRaphsonSolver call sequence ----
+    void constructUnMatrix() { check(tf_begin_step(h_), h_); }
+    void initialteUkMatrix() {}  // folded into constructUnMatrix (uk <- un)
+    int calculateForcesAndJacobianMatrixNR(double dt)
+    {
+        const int rc = tf_assemble(h_, dt);
+        if (rc != TF_OK && rc != TF_NEG_JACOBIAN) check(rc, h_);
+        return rc;
+    }
+    int solveForDeltaU(std::vector<double>* deltaU = nullptr)
+    {
+        if (deltaU) deltaU->resize(3 * (size_t)nNodes_);
+        int32_t it = 0;
+        double rr = 0.0;
+        const int rc = tf_solve(h_, deltaU ? deltaU->data() : nullptr, &it, &rr);
+        if (rc != TF_OK && rc != TF_KRYLOV_STALL) check(rc, h_);
+        lastKrylovIterations = it;
+        return rc;
+    }
+    bool checkConvergenceViaDeltaU()
+    {
+        int32_t conv = 0;
+        check(tf_update_uk(h_, &lastDeltaUNorm, &conv), h_);
+        return conv != 0;
+    }
+    std::vector<double> gSum()
+    {
+        std::vector<double> g(3 * (size_t)nNodes_);
+        check(tf_get_residual(h_, g.data()), h_);
+        return g;
+    }
+    // whole updateStepNR on the device
+    tf_step_stats updateStepNR(double dt)
+    {
+        tf_step_stats st;
+        const int rc = tf_nr_step(h_, dt, &st);
+        if (rc != TF_OK && rc != TF_NOT_CONVERGED) check(rc, h_);
+        return st;
+    }
+    // growth + surfaces + updateStepNR
+    tf_step_stats runOneStep(double dt)
+    {
+        tf_step_stats st;
+        const int rc = tf_timestep(h_, dt, &st);
+        if (rc != TF_OK && rc != TF_NOT_CONVERGED) check(rc, h_);
+        return st;
+    }
+
+    // ---- state ----
+    std::vector<double> positions()
+    {
+        std::vector<double> x(3 * (size_t)nNodes_);
+        check(tf_get_positions(h_, x.data()), h_);
+        return x;
+    }
+    void setPositions(const std::vector<double>& x) { check(tf_set_positions(h_, x.data()), h_); }
+    void elementDump(int e, double dt, double fe[18], double fv[18], double Ke[324], double Kv[324])
+    {
+        check(tf_element_dump(h_, e, dt, fe, fv, Ke, Kv), h_);
+    }
+
+    int nNodes() const { return nNodes_; }
+    int nElements() const { return nElements_; }
+    tf_handle* handle() { return h_; }
+
+    int lastKrylovIterations = 0;
+    double lastDeltaUNorm = 0.0;
+
+private:
+    static void check(int rc, const tf_handle* h)
+    {
+        if (rc != TF_OK) throw Error(rc, tf_last_error(h));
+    }
+    tf_handle* h_ = nullptr;
+    int nNodes_ = 0, nElements_ = 0;
+};
+
+}  // namespace tf

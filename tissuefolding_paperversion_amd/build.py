@@ -36,6 +36,16 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libtfnr.so")
     os.replace(tmp, LIB)
+    # C++ host driver over the facade header (plain g++, links the library above)
+    drv = os.path.join(LIBDIR, "tf_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "tf_driver.cpp"), "-L" + LIBDIR, "-ltfnr",
+           "-Wl,-rpath,$ORIGIN", "-o", drv]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building tf_driver")
     return LIB
 
 
