@@ -57,7 +57,9 @@ typedef struct tf_mesh_desc {
     const int32_t* elem_seg_ptr;  /* [n_segments+1] contiguous element ranges, NULL if 1 seg.   */
 } tf_mesh_desc;
 
-enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2 };
+/* TF_KRYLOV_CG1: CG with one fused kernel per iteration (single-reduction recurrences of
+ * Chronopoulos and Gear); TF_KRYLOV_CG: textbook PCG, two kernels per iteration. */
+enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_CG1 = 3 };
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
 
 typedef struct tf_solver_opts {

@@ -84,7 +84,7 @@ def test_global_assembly_matches_oracle(oracle, eta_int):
     assert rel_err(M, K.T) < 1e-10
 
 
-@pytest.mark.parametrize("krylov", [L.KRYLOV_BICGSTAB, L.KRYLOV_CG])
+@pytest.mark.parametrize("krylov", [L.KRYLOV_BICGSTAB, L.KRYLOV_CG, L.KRYLOV_CG1])
 def test_linear_solve_matches_direct(oracle, krylov):
     t = _small_viscous_disc(eta_int=0.0)
     X, x, Fg = _perturbed_state(t, 3, amp=0.01, growth=0.01)
@@ -104,7 +104,7 @@ def test_linear_solve_matches_direct(oracle, krylov):
     assert rel_err(du, ref) < 1e-8
 
 
-@pytest.mark.parametrize("krylov", [L.KRYLOV_CG, L.KRYLOV_BICGSTAB])
+@pytest.mark.parametrize("krylov", [L.KRYLOV_CG1, L.KRYLOV_CG, L.KRYLOV_BICGSTAB])
 def test_config1_known_answer(oracle, krylov):
     """Uniform in-plane growth of the pinned disc: x(t) = exp(r t) X in plane, z unchanged."""
     t = configs.config1()
@@ -180,9 +180,10 @@ def test_segments_equal_single_handles():
         off += 3 * t.n_nodes
 
 
-def test_determinism():
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_determinism(use_graph):
     t = configs.config1()
-    a, b = _solver(t), _solver(t)
+    a, b = _solver(t, use_graph=use_graph), _solver(t, use_graph=1)
     for _ in range(2):
         a.runOneStep(60.0)
         b.runOneStep(60.0)

@@ -33,6 +33,7 @@ struct SegScalars {
     double rnorm2;           // ||r||^2 at the last convergence test
     double tol2;             // squared absolute stopping threshold
     double pad0;
+    double alpha_hist[2];    // single-kernel CG: alpha of the last two iterations (by parity)
     int32_t converged;       // Krylov converged (or nothing to do)
     int32_t stalled;         // breakdown / iteration cap
     int32_t iters;           // Krylov iterations done
@@ -41,7 +42,8 @@ struct SegScalars {
 
 // partial-sum slots
 enum {
-    SL_RHO0 = 0, SL_RHO1, SL_RR0, SL_RR1, SL_RV, SL_TS, SL_TT, SL_BB, SL_PQ, SL_RZ0, SL_RZ1, SL_NORM, SL_COUNT
+    SL_RHO0 = 0, SL_RHO1, SL_RR0, SL_RR1, SL_RV, SL_TS, SL_TT, SL_BB, SL_PQ, SL_RZ0, SL_RZ1, SL_NORM, SL_DL0, SL_DL1,
+    SL_COUNT
 };
 
 struct KernelTimer {
@@ -49,6 +51,9 @@ struct KernelTimer {
     double total_ms = 0.0;
     int64_t launches = 0;
     std::vector<hipEvent_t> pending;  // start/stop pairs not yet read back
+    // graph replays: events bracketing a whole replayed chunk
+    hipEvent_t sample[2] = {nullptr, nullptr};
+    bool capturing = false;           // inside a stream capture: no per-launch events
 };
 
 struct tf_handle {

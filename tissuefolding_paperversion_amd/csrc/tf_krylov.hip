@@ -432,6 +432,156 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 }
 
 // ---------------------------------------------------------------------------------------------
+// Single-kernel CG (the single-reduction recurrences of Chronopoulos and Gear):
+//   x_{k} = x_{k-1} + a_{k-1} p_{k-1},   r_k = r_{k-1} - a_{k-1} s_{k-1},   u_k = D^-1 r_k,
+//   w_k = M u_k,  g_k = (r_k, u_k),  d_k = (w_k, u_k),
+//   b_k = g_k / g_{k-1},  a_k = g_k / (d_k - b_k g_k / a_{k-1}),
+//   p_k = u_k + b_k p_{k-1},  s_k = w_k + b_k s_{k-1}  (s_k = M p_k).
+// Kernel k forms a_{k-1}, b_{k-1} from the partials of kernel k-1, updates its own rows, and
+// rebuilds r_k and u_k of neighbouring block columns on the fly from r_{k-1}, w_{k-1}, s_{k-2}
+// (ping-pong buffers, so no tile overwrites what another tile still reads) before the SpMV.
+__device__ inline void cg1_update(double w, double s_old, double r_old, double beta, double alpha, double& s_new,
+                                  double& r_new)
+{
+    s_new = w + beta * s_old;
+    r_new = r_old - alpha * s_new;
+}
+__device__ inline double dinv_apply3(const double* __restrict__ D, double r0, double r1, double r2)
+{
+    return D[0] * r0 + D[1] * r1 + D[2] * r2;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_cg1_init(TileArgs ta, const int32_t* __restrict__ row_ptr,
+                                                 const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
+                                                 const double* __restrict__ dinv, const double* __restrict__ b,
+                                                 double* __restrict__ x, double* __restrict__ r0, double* __restrict__ w0,
+                                                 double* __restrict__ s_m1, double* __restrict__ u, double* __restrict__ p)
+{
+    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    const bool active = ta.seg[sg].active;
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+        const double bv = active ? b[d] : 0.0;
+        x[d] = 0.0; r0[d] = bv; s_m1[d] = 0.0; p[d] = 0.0;
+    }
+    __syncthreads();
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) u[d] = dinv_row(dinv, r0, d);
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    tile_products<VT>(col, val, nb, b0, b1,
+                          [&](int c, double& a0, double& a1, double& a2) {
+                              const double q0 = active ? b[3 * c] : 0.0, q1 = active ? b[3 * c + 1] : 0.0,
+                                           q2 = active ? b[3 * c + 2] : 0.0;
+                              const double* D = dinv + 9 * c;
+                              a0 = dinv_apply3(D, q0, q1, q2);
+                              a1 = dinv_apply3(D + 3, q0, q1, q2);
+                              a2 = dinv_apply3(D + 6, q0, q1, q2);
+                          },
+                          prod);
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int task = threadIdx.x; task < 3 * (rr1 - rr0); task += WG) {
+        const int r = rr0 + task / 3, comp = task % 3;
+        const int d = 3 * r + comp;
+        const double wv = row_sum(row_ptr, r, comp, b0, prod);
+        w0[d] = wv;
+        g += r0[d] * u[d];
+        dl += wv * u[d];
+        rr += r0[d] * r0[d];
+    }
+    const int sl[4] = {SL_RHO0, SL_DL0, SL_RR0, SL_BB};
+    double vv[4] = {g, dl, rr, rr};
+    tile_write<4>(ta, t, sl, vv);
+}
+
+template <typename VT, int P>
+__global__ __launch_bounds__(WG) void k_cg1(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                                            const VT* __restrict__ val, int nb, const double* __restrict__ dinv,
+                                            double* __restrict__ x, const double* __restrict__ r_old,
+                                            double* __restrict__ r_new, const double* __restrict__ w_old,
+                                            double* __restrict__ w_new, const double* __restrict__ s_old,
+                                            double* __restrict__ s_new, double* __restrict__ u, double* __restrict__ p,
+                                            int maxit)
+{
+    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    SegScalars& S = ta.seg[sg];
+    if (seg_done(S)) return;
+    // kernel parity P = k & 1: partials of iteration k-1 sit in slots of parity 1-P
+    const int sl[4] = {SL_RHO0 + (1 - P), SL_DL0 + (1 - P), SL_RR0 + (1 - P), SL_RHO0 + P};
+    double q[4];
+    seg_sums<4>(ta, sg, sl, q);
+    const bool lead = first_tile(ta, t, sg) && threadIdx.x == 0;
+    {
+        const bool conv = q[2] <= S.tol2;
+        const bool cap = !conv && (S.iters >= maxit || !isfinite(q[2]));
+        if (conv || cap) {
+            if (lead) {
+                S.rnorm2 = q[2];
+                if (conv) S.converged = 1;
+                else S.stalled = 1;
+            }
+            return;
+        }
+    }
+    const bool first = S.iters == 0;
+    const double beta = first ? 0.0 : q[0] / q[3];
+    const double den = first ? q[1] : q[1] - beta * q[0] / S.alpha_hist[P];
+    const double alpha = q[0] / den;
+    if (!(den > 0.0) || !isfinite(alpha)) {  // M not positive definite along this direction
+        if (lead) S.stalled = 1;
+        return;
+    }
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+        double sn, rn;
+        cg1_update(w_old[d], s_old[d], r_old[d], beta, alpha, sn, rn);
+        const double pn = u[d] + beta * p[d];
+        s_new[d] = sn;
+        r_new[d] = rn;
+        p[d] = pn;
+        x[d] = x[d] + alpha * pn;
+    }
+    __syncthreads();
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) u[d] = dinv_row(dinv, r_new, d);
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    tile_products<VT>(col, val, nb, b0, b1,
+                      [&](int c, double& a0, double& a1, double& a2) {
+                          double sn, q0, q1, q2;
+                          cg1_update(w_old[3 * c], s_old[3 * c], r_old[3 * c], beta, alpha, sn, q0);
+                          cg1_update(w_old[3 * c + 1], s_old[3 * c + 1], r_old[3 * c + 1], beta, alpha, sn, q1);
+                          cg1_update(w_old[3 * c + 2], s_old[3 * c + 2], r_old[3 * c + 2], beta, alpha, sn, q2);
+                          const double* D = dinv + 9 * c;
+                          a0 = dinv_apply3(D, q0, q1, q2);
+                          a1 = dinv_apply3(D + 3, q0, q1, q2);
+                          a2 = dinv_apply3(D + 6, q0, q1, q2);
+                      },
+                      prod);
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int task = threadIdx.x; task < 3 * (rr1 - rr0); task += WG) {
+        const int r = rr0 + task / 3, comp = task % 3;
+        const int d = 3 * r + comp;
+        const double wv = row_sum(row_ptr, r, comp, b0, prod);
+        w_new[d] = wv;
+        const double uv = u[d], rv = r_new[d];
+        g += rv * uv;
+        dl += wv * uv;
+        rr += rv * rv;
+    }
+    const int so[3] = {SL_RHO0 + P, SL_DL0 + P, SL_RR0 + P};
+    double vv[3] = {g, dl, rr};
+    tile_write<3>(ta, t, so, vv);
+    if (lead) {
+        S.alpha_hist[1 - P] = alpha;
+        S.iters += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // per-tissue ||x||_2
 __global__ __launch_bounds__(WG) void k_tile_norm(TileArgs ta, const double* __restrict__ x)
 {
@@ -493,11 +643,30 @@ void enqueue_cg_iter(tf_handle* h, const VT* val)
     hipLaunchKernelGGL(k_cg_x<P>, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks);
 }
 
+// Single-kernel CG buffers: r (kr, kr0), w (kv, kt), s (ks, ksh) ping-pong; u = kph, p = kp, x = kx.
+// Kernel k has parity P = k & 1 and reads the buffers of parity 1-P (r, w) and P (s).
+template <typename VT, int P>
+void enqueue_cg1_iter(tf_handle* h, const VT* val)
+{
+    TileArgs ta = tile_args(h);
+    double* R[2] = {h->kr, h->kr0};
+    double* W[2] = {h->kv, h->kt};
+    double* Sv[2] = {h->ks, h->ksh};
+    hipEvent_t ev[2];
+    timer_begin(h, h->t_spmv, ev);
+    hipLaunchKernelGGL((k_cg1<VT, P>), dim3(h->T), dim3(WG), 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv,
+                       h->kx, R[1 - P], R[P], W[1 - P], W[P], Sv[P], Sv[1 - P], h->kph, h->kp, h->opts.krylov_max_iter);
+    timer_end(h, h->t_spmv, ev);
+}
+
 template <typename VT>
 void enqueue_chunk(tf_handle* h, const VT* val, int kind, int iters)
 {
     for (int i = 0; i < iters; i += 2) {
-        if (kind == TF_KRYLOV_CG) {
+        if (kind == TF_KRYLOV_CG1) {
+            enqueue_cg1_iter<VT, 1>(h, val);  // chunks start at an odd kernel (kernel 0 is the init)
+            enqueue_cg1_iter<VT, 0>(h, val);
+        } else if (kind == TF_KRYLOV_CG) {
             enqueue_cg_iter<VT, 0>(h, val);
             enqueue_cg_iter<VT, 1>(h, val);
         } else {
@@ -523,7 +692,10 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const dim3 grid(h->T), blk(WG);
     const double rtol2 = h->opts.krylov_rtol * h->opts.krylov_rtol;
     const double atol2 = h->opts.krylov_atol * h->opts.krylov_atol;
-    if (kind == TF_KRYLOV_CG)
+    if (kind == TF_KRYLOV_CG1)
+        hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv, h->g,
+                           h->kx, h->kr, h->kv, h->ksh, h->kph, h->kp);
+    else if (kind == TF_KRYLOV_CG)
         hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
     else
         hipLaunchKernelGGL(k_bicg_init, grid, blk, 0, h->stream, ta, h->g, h->kx, h->kr, h->kr0, h->kp, h->kv);
@@ -532,29 +704,46 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     int chunk = std::max(2, h->opts.iters_per_sync);
     chunk += chunk & 1;  // iterations come in parity pairs
     const bool timing = h->t_spmv.enabled;
-    const bool graph = h->opts.use_graph && !timing;
+    // Graph replay.  While timing, only the single-kernel CG keeps the graph: a replayed chunk is
+    // then exactly `chunk` launches of the dominant kernel, timed by events around the replay.
+    const bool graph = h->opts.use_graph != 0 && (!timing || kind == TF_KRYLOV_CG1);
     int done_iters = 0;
+    if (graph && timing && !h->t_spmv.sample[0]) {
+        TF_HIP(hipEventCreate(&h->t_spmv.sample[0]));
+        TF_HIP(hipEventCreate(&h->t_spmv.sample[1]));
+    }
     while (true) {
         if (graph) {
-            const int key = kind * 4 + (int)sizeof(VT);
+            const int key = kind * 16 + (int)sizeof(VT) * 2;
             if (!h->krylov_graph || h->krylov_graph_iters != chunk || h->krylov_graph_kind != key) {
                 if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
                 hipGraph_t gr;
                 TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                h->t_spmv.capturing = true;
                 enqueue_chunk<VT>(h, val, kind, chunk);
+                h->t_spmv.capturing = false;
                 TF_HIP(hipStreamEndCapture(h->stream, &gr));
                 TF_HIP(hipGraphInstantiate(&h->krylov_graph, gr, nullptr, nullptr, 0));
                 TF_HIP(hipGraphDestroy(gr));
                 h->krylov_graph_iters = chunk;
                 h->krylov_graph_kind = key;
             }
+            if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[0], h->stream));
             TF_HIP(hipGraphLaunch(h->krylov_graph, h->stream));
+            if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[1], h->stream));
         } else {
             enqueue_chunk<VT>(h, val, kind, chunk);
             TF_HIP(hipGetLastError());
         }
         done_iters += chunk;
-        if (all_done(h)) break;
+        const bool fin = all_done(h);
+        if (graph && timing) {
+            float ms = 0.f;
+            TF_HIP(hipEventElapsedTime(&ms, h->t_spmv.sample[0], h->t_spmv.sample[1]));
+            h->t_spmv.total_ms += ms;
+            h->t_spmv.launches += chunk;
+        }
+        if (fin) break;
         if (done_iters > h->opts.krylov_max_iter + 2 * chunk) break;  // the cap marks the tissue stalled
     }
     timer_collect(h->t_spmv);
@@ -577,7 +766,7 @@ int krylov_solve(tf_handle* h)
     const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
     auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     int st = run(kind);
-    if (st == TF_KRYLOV_STALL && kind == TF_KRYLOV_CG) {
+    if (st == TF_KRYLOV_STALL && (kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1)) {
         // not positive definite (or too slow): BiCGSTAB from zero
         const int cg_iters = h->last_krylov_iters;
         st = run(TF_KRYLOV_BICGSTAB);
