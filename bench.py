@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="config2", choices=["config1", "config2"])
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     p.add_argument("--krylov", type=int, default=-1)
     p.add_argument("--matrix-fp32", action="store_true")
     p.add_argument("--no-graph", action="store_true")
@@ -78,6 +79,28 @@ def cpu_baseline(tissue, dt):
             "seconds": sec, "nr_iterations": it}
 
 
+def elastic_forces(s):
+    """FP64 nodal forces at the current positions (un = uk, so no viscous term)."""
+    s.constructUnMatrix()
+    s.calculateForcesAndJacobianMatrixNR(60.0)
+    return s.gSum
+
+
+def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing):
+    s = NewtonRaphsonSolver(tissue, opts)
+    dt = tissue.dt
+    for _ in range(warmup):
+        s.runOneStep(dt)
+    for k in ("spmv", "assemble_elements", "assemble_gather"):
+        s.kernel_timing(k, enable=timing)
+    s.positions  # synchronises the stream
+    t0 = time.perf_counter()
+    stats = [s.runOneStep(dt) for _ in range(steps)]
+    x = s.positions
+    sec = time.perf_counter() - t0
+    return s, stats, sec, x
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -94,19 +117,8 @@ def main():
         opts["matrix_precision"] = 1
     if a.no_graph:
         opts["use_graph"] = 0
-    s = NewtonRaphsonSolver(tissue, opts)
-    for _ in range(a.warmup):
-        s.runOneStep(dt)
     timing = 0 if a.no_kernel_timing else 1
-    s.kernel_timing("spmv", enable=timing)
-    s.kernel_timing("assemble_elements", enable=timing)
-    s.kernel_timing("assemble_gather", enable=timing)
-    s.positions  # synchronises the stream
-    t0 = time.perf_counter()
-    stats = [s.runOneStep(dt) for _ in range(a.steps)]
-    s.positions
-    t1 = time.perf_counter()
-    sec = t1 - t0
+    s, stats, sec, x_plain = run_timed(NewtonRaphsonSolver, tissue, opts, a.warmup, a.steps, timing)
     spmv_ms, spmv_n = s.kernel_timing("spmv", enable=0)
     el_ms, el_n = s.kernel_timing("assemble_elements", enable=0)
     ga_ms, ga_n = s.kernel_timing("assemble_gather", enable=0)
@@ -141,6 +153,24 @@ def main():
                      "algorithmic_bytes_per_launch": spmv_bytes},
         "kernels": {"assemble_elements_ms": el_ms, "assemble_gather_ms": ga_ms, "spmv_ms": spmv_ms},
     }
+    if not a.no_mixed:
+        # the same workload with MFMA element tangents and FP32 Jacobian values (FP64 residual,
+        # update, Krylov vectors and dots): throughput, and errors against the plain FP64 run
+        f_plain = elastic_forces(s)
+        fscale = np.max(np.abs(f_plain))
+        mixed = {}
+        for name, tp in (("fp32_mfma", 1), ("bf16_mfma", 2)):
+            mopts = dict(opts, tangent_precision=tp, matrix_precision=1)
+            sm, mstats, msec, x_mixed = run_timed(NewtonRaphsonSolver, tissue, mopts, a.warmup, a.steps, 0)
+            disp_err = float(np.max(np.abs(x_mixed - x_plain)) / np.max(np.abs(x_plain)))
+            f_err = float(np.max(np.abs(elastic_forces(sm) - f_plain)) / fscale)
+            mixed[name] = {"value": a.steps / msec, "ms_per_step": 1e3 * msec / a.steps,
+                           "disp_err_rel_after_all_steps": disp_err, "force_err_rel": f_err,
+                           "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in mstats])),
+                           "converged_steps": int(sum(st["converged"] for st in mstats)),
+                           "speedup_vs_fp64": (a.steps / msec) / (a.steps / sec)}
+            sm.close()
+        out["mixed"] = mixed
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(configs.config2() if a.workload == "config2" else configs.config1(), dt)
     print(json.dumps(out), flush=True)

@@ -19,7 +19,90 @@ namespace {
 constexpr int EPB = 32;   // prisms per workgroup
 constexpr int WG = 256;
 
-template <bool VISC>
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Mixed-precision element tangent on the matrix cores.  With the 54-long contraction index
+// kk = (g, i', j) (Gauss point g, displacement-gradient component du_i'/dx_j), the prism tangent
+// (both KIntegral1 and KIntegral2) is one product  K = X^T Y  where
+//   X[kk][(a,i)] = d_{i i'} gN_a,j                                    (gradient operator G_g)
+//   Y[kk][(b,k)] = clam d_{i'j} gN_b,k + cmu (d_{i'k} gN_b,j + d_{jk} gN_b,i') + d_{i'k} (sig gN_b)_j
+// (Y = A_g G_g with A_g the Gauss point's 9x9 tangent; clam, cmu, sig carry w J D0 as staged).
+// The 18x18 result is one 32x32 MFMA tile: M = N = 18 padded to 32, K = 54 padded to 56 (FP32,
+// 28 steps of v_mfma_f32_32x32x2_f32) or 64 (BF16, 4 steps of v_mfma_f32_32x32x16_bf16).
+// Operands are formed in FP64 from the staged Gauss-point data and rounded to FP32 / BF16;
+// accumulation is FP32.
+__device__ inline double mx_X(const tf::GPElastic& E, int kk_ij, int m)
+{
+    if (m >= 18) return 0.0;
+    const int ip = kk_ij / 3, j = kk_ij % 3;
+    return (m % 3 == ip) ? E.gn[3 * (m / 3) + j] : 0.0;
+}
+__device__ inline double mx_Y(const tf::GPElastic& E, int kk_ij, int n)
+{
+    if (n >= 18) return 0.0;
+    const int ip = kk_ij / 3, j = kk_ij % 3;
+    const int b = n / 3, k = n % 3;
+    const double* B = E.gn + 3 * b;
+    double sB[3];
+    tf::voigt_mv(E.sig, B, sB);
+    double y = 0.0;
+    if (ip == j) y += E.clam * B[k];
+    if (ip == k) y += E.cmu * B[j] + sB[j];
+    if (j == k) y += E.cmu * B[ip];
+    return y;
+}
+
+// one wavefront: tangent of prism el into kelem (FP32 accumulate, stored as FP64)
+template <int TP>
+__device__ inline void mfma_tangent(const tf::GPElastic* sE, int el, int e, double* __restrict__ kelem)
+{
+    const int l = threadIdx.x % 64;
+    const int row = l % 32;
+    floatx16 c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0.f;
+    if (TP == TF_PREC_FP32) {
+        for (int st = 0; st < 27; ++st) {
+            const int kk = 2 * st + l / 32;       // 0..53
+            const int g = kk / 9, ij = kk % 9;
+            const tf::GPElastic& E = sE[g * EPB + el];
+            const float av = (float)mx_X(E, ij, row);
+            const float bv = (float)mx_Y(E, ij, row);
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, c, 0, 0, 0);
+        }
+    } else {
+        for (int st = 0; st < 4; ++st) {
+            bf16x8 av, bv;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int kk = 16 * st + 8 * (l / 32) + t;
+                double x = 0.0, y = 0.0;
+                if (kk < 54) {
+                    const int g = kk / 9, ij = kk % 9;
+                    const tf::GPElastic& E = sE[g * EPB + el];
+                    x = mx_X(E, ij, row);
+                    y = mx_Y(E, ij, row);
+                }
+                av[t] = (__bf16)(float)x;
+                bv[t] = (__bf16)(float)y;
+            }
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+        }
+    }
+    // accumulator layout: register r of lane l holds D[8*(r/4) + 4*(l/32) + r%4][l%32]
+    const int col = l % 32;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r / 4) + 4 * (l / 32) + (r % 4);
+        if (m < 18 && col < 18) {
+            const int a = m / 3, i = m % 3, b = col / 3, k = col % 3;
+            kelem[(size_t)e * 324 + (6 * a + b) * 9 + 3 * i + k] = (double)c[r];
+        }
+    }
+}
+
+template <bool VISC, int TP>
 __global__ __launch_bounds__(WG) void k_elements(const int32_t* __restrict__ conn, const double* __restrict__ xref,
                                                  const double* __restrict__ invfg, const double* __restrict__ lam,
                                                  const double* __restrict__ mu, const double* __restrict__ eta,
@@ -67,20 +150,42 @@ __global__ __launch_bounds__(WG) void k_elements(const int32_t* __restrict__ con
     __syncthreads();
 
     // ---- phase B: (prism, node pair) ----
-    for (int t = tid; t < 36 * EPB; t += WG) {
-        const int el = t / 36, ab = t % 36;
-        const int e = e0 + el;
-        if (e >= E) break;
-        const int a = ab / 6, b = ab % 6;
-        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (TP == TF_PREC_FP64) {
+        for (int t = tid; t < 36 * EPB; t += WG) {
+            const int el = t / 36, ab = t % 36;
+            const int e = e0 + el;
+            if (e >= E) break;
+            const int a = ab / 6, b = ab % 6;
+            double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int g = 0; g < 6; ++g) {
-            tf::block_elastic(sE[g * EPB + el], a, b, acc);
-            if (VISC) tf::block_viscous(sE[g * EPB + el], sV[g * EPB + el], a, b, inv_dt, acc);
+            for (int g = 0; g < 6; ++g) {
+                tf::block_elastic(sE[g * EPB + el], a, b, acc);
+                if (VISC) tf::block_viscous(sE[g * EPB + el], sV[g * EPB + el], a, b, inv_dt, acc);
+            }
+            double* out = kelem + (size_t)e * 324 + ab * 9;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) out[q] = acc[q];
         }
-        double* out = kelem + (size_t)e * 324 + ab * 9;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) out[q] = acc[q];
+    } else {
+        // matrix cores: one wavefront per prism for the elastic tangent
+        for (int el = tid / 64; el < EPB; el += WG / 64) {
+            const int e = e0 + el;
+            if (e >= E) break;
+            mfma_tangent<TP>(sE, el, e, kelem);
+        }
+        if (VISC) {  // the viscous blocks stay FP64 and are added afterwards
+            __syncthreads();
+            for (int t = tid; t < 36 * EPB; t += WG) {
+                const int el = t / 36, ab = t % 36;
+                const int e = e0 + el;
+                if (e >= E) break;
+                const int a = ab / 6, b = ab % 6;
+                double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int g = 0; g < 6; ++g) tf::block_viscous(sE[g * EPB + el], sV[g * EPB + el], a, b, inv_dt, acc);
+                double* out = kelem + (size_t)e * 324 + ab * 9;
+                for (int q = 0; q < 9; ++q) out[q] += acc[q];
+            }
+        }
     }
 
     // ---- phase C: (prism, node) forces, negated as the reference stores them ----
@@ -391,17 +496,28 @@ __global__ __launch_bounds__(256) void k_update_uk(double* __restrict__ uk, cons
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
+template <bool VISC, int TP>
+static void launch_elements_t(tf_handle* h, double dt, int grid)
+{
+    hipLaunchKernelGGL((k_elements<VISC, TP>), dim3(grid), dim3(WG), 0, h->stream, h->conn, h->xref, h->invfg, h->lam,
+                       h->mu, h->eta, h->uk, h->un, 1.0 / dt, h->E, h->kelem, h->felem, h->status);
+}
+
 void launch_assemble_elements(tf_handle* h, double dt)
 {
     const int grid = (h->E + EPB - 1) / EPB;
     hipEvent_t ev[2];
     timer_begin(h, h->t_elem, ev);
-    if (h->any_viscous)
-        hipLaunchKernelGGL(k_elements<true>, dim3(grid), dim3(WG), 0, h->stream, h->conn, h->xref, h->invfg, h->lam,
-                           h->mu, h->eta, h->uk, h->un, 1.0 / dt, h->E, h->kelem, h->felem, h->status);
-    else
-        hipLaunchKernelGGL(k_elements<false>, dim3(grid), dim3(WG), 0, h->stream, h->conn, h->xref, h->invfg, h->lam,
-                           h->mu, h->eta, h->uk, h->un, 1.0 / dt, h->E, h->kelem, h->felem, h->status);
+    const int tp = h->opts.tangent_precision;
+    if (h->any_viscous) {
+        if (tp == TF_PREC_FP32) launch_elements_t<true, TF_PREC_FP32>(h, dt, grid);
+        else if (tp == TF_PREC_BF16) launch_elements_t<true, TF_PREC_BF16>(h, dt, grid);
+        else launch_elements_t<true, TF_PREC_FP64>(h, dt, grid);
+    } else {
+        if (tp == TF_PREC_FP32) launch_elements_t<false, TF_PREC_FP32>(h, dt, grid);
+        else if (tp == TF_PREC_BF16) launch_elements_t<false, TF_PREC_BF16>(h, dt, grid);
+        else launch_elements_t<false, TF_PREC_FP64>(h, dt, grid);
+    }
     TF_HIP(hipGetLastError());
     timer_end(h, h->t_elem, ev);
 }
