@@ -86,11 +86,13 @@ def elastic_forces(s):
     return s.gSum
 
 
-def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing):
+def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing, tune=True):
     s = NewtonRaphsonSolver(tissue, opts)
     dt = tissue.dt
     for _ in range(warmup):
         s.runOneStep(dt)
+    if tune:  # set-up, outside the timed region: pick the Krylov tiling for this mesh
+        s.autotune(dt, n_iters=64)
     for k in ("spmv", "assemble_elements", "assemble_gather"):
         s.kernel_timing(k, enable=timing)
     s.positions  # synchronises the stream
@@ -143,10 +145,12 @@ def main():
                                f"{3 * tissue.n_nodes} DoF, dt=60 s, plain FP64",
                    "prisms": tissue.n_elements, "nodes": tissue.n_nodes, "bsr_blocks": s.nb,
                    "krylov": "auto (single-kernel CG, BiCGSTAB fallback)" if a.krylov < 0 else str(a.krylov),
-                   "matrix_values": "fp32" if a.matrix_fp32 else "fp64"},
+                   "matrix_values": "fp32" if a.matrix_fp32 else "fp64",
+                   "tiling": dict(zip(("workgroup", "tile_blocks", "tiles"), s.tiling()))},
         "nr_iterations_per_step": float(np.mean(nr)),
         "krylov_iterations_per_nr_iteration": float(np.sum(kr) / max(1, np.sum(nr))),
         "converged_steps": int(sum(st["converged"] for st in stats)),
+        "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in stats)),
         "roofline": {"kernel": KERNEL_NAMES.get(a.krylov, KERNEL_NAMES[-1]), "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(a.workload), "avg_launch_ms": spmv_ms, "launches": spmv_n,
@@ -168,6 +172,7 @@ def main():
                            "disp_err_rel_after_all_steps": disp_err, "force_err_rel": f_err,
                            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in mstats])),
                            "converged_steps": int(sum(st["converged"] for st in mstats)),
+                           "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in mstats)),
                            "speedup_vs_fp64": (a.steps / msec) / (a.steps / sec)}
             sm.close()
         out["mixed"] = mixed

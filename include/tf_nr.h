@@ -82,6 +82,8 @@ typedef struct tf_step_stats {
     int32_t krylov_iterations;   /* summed over NR iterations (max over tissues)               */
     int32_t converged;           /* 1 when every tissue met nr_tol                             */
     int32_t rotated_elements;    /* growth increments rotated this step                        */
+    int32_t krylov_fallbacks;    /* linear solves that fell back from CG to BiCGSTAB           */
+    int32_t pad;
     double last_du_norm;         /* max over tissues of the last ||dU||_2                      */
     double ms_grow, ms_assemble, ms_solve, ms_total;
 } tf_step_stats;
@@ -148,6 +150,16 @@ int tf_nr_step(tf_handle* h, double dt, tf_step_stats* stats);
 int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats);
 /* Per-tissue results of the last NR step: iterations, last ||dU||, converged flag. */
 int tf_get_segment_stats(tf_handle* h, int32_t* nr_iters, double* du_norm, int32_t* converged);
+
+/* ---- tuning (no reference counterpart: schedule choices of this implementation) ----------------- */
+/* Row tiling of the Krylov kernels: workgroup size wg in {256, 512, 1024} and tile capacity
+ * wg * {1, 2, 4} <= 2048 blocks. */
+int tf_set_tiling(tf_handle* h, int32_t wg, int32_t tile_cap);
+int tf_get_tiling(const tf_handle* h, int32_t* wg, int32_t* tile_cap, int32_t* n_tiles);
+/* Times n_iters single-kernel CG iterations on the Jacobian at the current positions for every
+ * supported tiling, keeps the fastest; per-variant results (up to 16) are written to the arrays. */
+int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, int32_t* wg, int32_t* tile_cap,
+                double* ms_per_iter);
 
 /* ---- measurement ------------------------------------------------------------------------------ */
 /* Average device time (ms) per launch of the named kernel over the calls since the last reset,

@@ -188,3 +188,28 @@ def test_determinism(use_graph):
         a.runOneStep(60.0)
         b.runOneStep(60.0)
     assert np.array_equal(a.positions, b.positions)
+
+
+def test_tilings_agree_and_autotune():
+    """Every supported tiling of the Krylov kernels gives the same trajectory (to rounding);
+    the autotuner times them all and keeps one."""
+    t = configs.wing_disc_tissue(a=40.0, b=25.0, h=4.0, z_levels=(0.0, 2.0, 6.0, 10.0), seed=7, growth_seed=8)
+    t.rates *= 15.0
+    ref = None
+    for wg, cap in [(256, 256), (256, 1024), (512, 1024), (1024, 2048)]:
+        s = _solver(t)
+        s.set_tiling(wg, cap)
+        assert s.tiling()[:2] == (wg, cap)
+        for _ in range(2):
+            assert s.runOneStep(t.dt)["converged"] == 1
+        x = s.positions
+        if ref is None:
+            ref = x
+        else:
+            assert rel_err(x, ref) < 1e-10
+    s = _solver(t)
+    table = s.autotune(t.dt, n_iters=16)
+    assert len(table) >= 6 and all(v["ms_per_iter"] > 0 for v in table)
+    best = min(table, key=lambda v: v["ms_per_iter"])
+    assert s.tiling()[:2] == (best["wg"], best["tile_cap"])
+    assert s.runOneStep(t.dt)["converged"] == 1

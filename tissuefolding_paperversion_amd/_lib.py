@@ -25,7 +25,7 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_set_material", "tf_grow", "tf_update_viscosity_surfaces", "tf_get_viscosity_surfaces",
            "tf_begin_step", "tf_assemble", "tf_get_residual", "tf_get_jacobian_bsr", "tf_element_dump", "tf_solve",
            "tf_update_uk", "tf_nr_step", "tf_timestep", "tf_get_segment_stats", "tf_kernel_timing",
-           "tf_kernel_bytes"]
+           "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune"]
 
 
 class TfError(RuntimeError):
@@ -50,7 +50,8 @@ class SolverOpts(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("nr_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("converged", C.c_int32),
-                ("rotated_elements", C.c_int32), ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
+                ("rotated_elements", C.c_int32), ("krylov_fallbacks", C.c_int32), ("pad", C.c_int32),
+                ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double)]
 
     def as_dict(self):
@@ -100,6 +101,9 @@ def load(path: str | None = None):
         "tf_get_segment_stats": ([vp, vp, vp, vp], i32),
         "tf_kernel_timing": ([vp, C.c_char_p, i32, vp, vp], i32),
         "tf_kernel_bytes": ([vp, C.c_char_p, vp], i32),
+        "tf_set_tiling": ([vp, i32, i32], i32),
+        "tf_get_tiling": ([vp, vp, vp, vp], i32),
+        "tf_autotune": ([vp, dbl, i32, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -144,6 +144,50 @@ void timer_collect(KernelTimer& t)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Row tiles: runs of block rows of one tissue with at most tile_cap blocks and wg/3 rows (so a
+// tile's DoFs never outnumber the lanes of a workgroup).  Rebuilt when the tiling is retuned.
+void build_tiles(tf_handle* h, int wg, int tile_cap)
+{
+    if (!(wg == 256 || wg == 512 || wg == 1024) || tile_cap % wg || tile_cap > TF_TILE_CAP_MAX || tile_cap / wg > 4)
+        throw std::runtime_error("unsupported tiling: wg in {256, 512, 1024}, tile_cap = wg * {1, 2, 4} <= 2048");
+    const std::vector<int32_t>& row_ptr = h->h_row_ptr;
+    std::vector<int32_t> tr0, tr1, tseg, seg_tile_ptr(h->S + 1, 0);
+    const int max_rows = wg / 3;
+    for (int s = 0; s < h->S; ++s) {
+        int r = h->h_node_seg_ptr[s];
+        const int rend = h->h_node_seg_ptr[s + 1];
+        while (r < rend) {
+            int r1 = r, blocks = 0;
+            while (r1 < rend && r1 - r < max_rows && blocks + (row_ptr[r1 + 1] - row_ptr[r1]) <= tile_cap) {
+                blocks += row_ptr[r1 + 1] - row_ptr[r1];
+                ++r1;
+            }
+            if (r1 == r) throw std::runtime_error("a block row exceeds the tile capacity");
+            tr0.push_back(r); tr1.push_back(r1); tseg.push_back(s);
+            r = r1;
+        }
+        seg_tile_ptr[s + 1] = (int32_t)tr0.size();
+    }
+    for (int32_t** p : {&h->tile_r0, &h->tile_r1, &h->tile_seg, &h->seg_tile_ptr})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (h->partial) { (void)hipFree(h->partial); h->partial = nullptr; }
+    h->T = (int)tr0.size();
+    h->tile_r0 = dev_alloc<int32_t>(h->T);
+    h->tile_r1 = dev_alloc<int32_t>(h->T);
+    h->tile_seg = dev_alloc<int32_t>(h->T);
+    h->seg_tile_ptr = dev_alloc<int32_t>(h->S + 1);
+    h->partial = dev_alloc<double>((size_t)h->T * TF_RED_SLOTS);
+    h2d(h, h->tile_r0, tr0.data(), h->T);
+    h2d(h, h->tile_r1, tr1.data(), h->T);
+    h2d(h, h->tile_seg, tseg.data(), h->T);
+    h2d(h, h->seg_tile_ptr, seg_tile_ptr.data(), h->S + 1);
+    h->h_seg_tile_ptr = seg_tile_ptr;
+    h->wg = wg;
+    h->tile_cap = tile_cap;
+    if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+}
+
+// ---------------------------------------------------------------------------------------------
 extern "C" {
 
 void tf_default_options(tf_solver_opts* o)
@@ -266,27 +310,8 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
             std::vector<int32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
             for (size_t k = 0; k < (size_t)36 * E; ++k) blk_src[fill[eblk[k]]++] = (int32_t)k;
         }
-        // ---- row tiles, never crossing a tissue ----
-        std::vector<int32_t> tr0, tr1, tseg, seg_tile_ptr(S + 1, 0);
-        for (int s = 0; s < S; ++s) {
-            int r = h->h_node_seg_ptr[s];
-            const int rend = h->h_node_seg_ptr[s + 1];
-            while (r < rend) {
-                int r1 = r, blocks = 0;
-                while (r1 < rend && r1 - r < TF_TILE_MAX_ROWS && blocks + (row_ptr[r1 + 1] - row_ptr[r1]) <= TF_TILE_MAX_BLOCKS) {
-                    blocks += row_ptr[r1 + 1] - row_ptr[r1];
-                    ++r1;
-                }
-                if (r1 == r) throw std::runtime_error("a block row exceeds the tile capacity");
-                tr0.push_back(r); tr1.push_back(r1); tseg.push_back(s);
-                r = r1;
-            }
-            seg_tile_ptr[s + 1] = (int32_t)tr0.size();
-        }
-        h->T = (int)tr0.size();
         h->h_row_ptr = row_ptr;
         h->h_col = col;
-        h->h_seg_tile_ptr = seg_tile_ptr;
 
         // ---- device allocations ----
         h->conn = dev_alloc<int32_t>((size_t)6 * E);
@@ -321,13 +346,8 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->blk_src = dev_alloc<int32_t>((size_t)36 * E);
         h->node_ptr = dev_alloc<int32_t>(N + 1);
         h->node_src = dev_alloc<int32_t>((size_t)6 * E);
-        h->tile_r0 = dev_alloc<int32_t>(h->T);
-        h->tile_r1 = dev_alloc<int32_t>(h->T);
-        h->tile_seg = dev_alloc<int32_t>(h->T);
-        h->seg_tile_ptr = dev_alloc<int32_t>(S + 1);
         h->seg_node_ptr = dev_alloc<int32_t>(S + 1);
         h->seg_counter = dev_alloc<int32_t>(S);
-        h->partial = dev_alloc<double>((size_t)h->T * TF_RED_SLOTS);
         h->seg = dev_alloc<SegScalars>(S);
         h->seg_norm = dev_alloc<double>(S);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
@@ -379,10 +399,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h2d(h, h->blk_src, blk_src.data(), blk_src.size());
         h2d(h, h->node_ptr, node_ptr.data(), N + 1);
         h2d(h, h->node_src, node_src.data(), node_src.size());
-        h2d(h, h->tile_r0, tr0.data(), h->T);
-        h2d(h, h->tile_r1, tr1.data(), h->T);
-        h2d(h, h->tile_seg, tseg.data(), h->T);
-        h2d(h, h->seg_tile_ptr, seg_tile_ptr.data(), S + 1);
+        build_tiles(h, 256, 512);
         h2d(h, h->seg_node_ptr, h->h_node_seg_ptr.data(), S + 1);
         std::vector<int32_t> zS(S, 0);
         h2d(h, h->seg_counter, zS.data(), S);
@@ -729,6 +746,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     std::fill(h->seg_converged.begin(), h->seg_converged.end(), 0);
     int status = TF_OK;
     int krylov_total = 0;
+    h->fallbacks = 0;
     double ms_asm = 0, ms_sol = 0;
     int it = 0;
     while (true) {
@@ -773,6 +791,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         }
         stats->nr_iterations = maxit;
         stats->krylov_iterations = krylov_total;
+        stats->krylov_fallbacks = h->fallbacks;
         stats->converged = allc;
         stats->last_du_norm = worst;
         stats->ms_assemble = ms_asm;
@@ -811,6 +830,53 @@ int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats)
         stats->ms_total = ms_since(t0);
     }
     return st;
+    TF_GUARD_END(h)
+}
+
+int tf_set_tiling(tf_handle* h, int32_t wg, int32_t tile_cap)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    build_tiles(h, wg, tile_cap);
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_tiling(const tf_handle* h, int32_t* wg, int32_t* tile_cap, int32_t* n_tiles)
+{
+    if (!h) return TF_INVALID;
+    if (wg) *wg = h->wg;
+    if (tile_cap) *tile_cap = h->tile_cap;
+    if (n_tiles) *n_tiles = h->T;
+    return TF_OK;
+}
+
+int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, int32_t* wg_out, int32_t* cap_out,
+                double* ms_out)
+{
+    if (!h || !(dt > 0) || n_iters < 2) return TF_INVALID;
+    TF_GUARD_BEGIN
+    static const int variants[][2] = {{256, 256}, {256, 512}, {256, 1024}, {512, 512},
+                                      {512, 1024}, {512, 2048}, {1024, 1024}, {1024, 2048}};
+    const int nv = (int)(sizeof(variants) / sizeof(variants[0]));
+    // a representative system: the Jacobian at the current positions
+    launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
+    hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    assemble_impl(h, dt);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    double best = 1e300;
+    int bi = 0;
+    for (int v = 0; v < nv; ++v) {
+        build_tiles(h, variants[v][0], variants[v][1]);
+        const double ms = krylov_time_cg1(h, n_iters);
+        if (wg_out) wg_out[v] = variants[v][0];
+        if (cap_out) cap_out[v] = variants[v][1];
+        if (ms_out) ms_out[v] = ms;
+        if (ms > 0.0 && ms < best) { best = ms; bi = v; }
+    }
+    build_tiles(h, variants[bi][0], variants[bi][1]);
+    if (n_variants) *n_variants = nv;
+    return TF_OK;
     TF_GUARD_END(h)
 }
 

@@ -10,7 +10,7 @@
 //   scatter maps   : blk_ptr[nb+1] / blk_src[36E] (contributing element blocks of every BSR
 //                    block, sorted), node_ptr[N+1] / node_src[6E] (element corners of every node)
 //   row tiles      : tile_r0/tile_r1/tile_seg -- runs of block rows of one tissue with at most
-//                    TF_TILE_MAX_BLOCKS blocks, the unit of every row kernel and reduction
+//                    tile_cap blocks and wg/3 rows, the unit of every row kernel and reduction
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,23 +20,29 @@
 
 #include "../../include/tf_nr.h"
 
-#define TF_TILE_MAX_BLOCKS 512
-#define TF_TILE_MAX_ROWS 64
+#define TF_TILE_CAP_MAX 2048   // largest tile capacity (blocks); 48 KB of LDS products
 #define TF_RED_SLOTS 16      // partial-sum slots per tile; partial[slot * T + tile]
 
 // Per-tissue Krylov / NR scalars, one struct per segment, resident on the device.  Within a
-// Krylov iteration the scalars (alpha, beta, omega, ...) are not stored: every tile re-derives
-// them from the tiles' partial sums (see tf_krylov.hip).  These fields are written by the first
-// tile of a tissue only and read by later kernels.
+// Krylov iteration the scalars (alpha, beta, omega, ...) are re-derived by every tile from the
+// tiles' partial sums (see tf_krylov.hip).  The fields below are written by the first tile of a
+// tissue only; a kernel never reads a field that it (or another tile of the same launch) writes,
+// except the monotone converged/stalled flags, whose value every tile derives identically.
+// Fields the single-kernel CG carries from launch to launch are indexed by launch parity P:
+// launch P reads slot P and writes slot 1-P.
 struct SegScalars {
     double bnorm2;           // ||b||^2
     double rnorm2;           // ||r||^2 at the last convergence test
     double tol2;             // squared absolute stopping threshold
     double pad0;
-    double alpha_hist[2];    // single-kernel CG: alpha of the last two iterations (by parity)
+    double alpha_hist[2];    // single-kernel CG: alpha of the previous iteration
+    double gam_hist[2];      // single-kernel CG: gamma = (r, D^-1 r) of the previous iteration
+    double best_rr_h[2];     // single-kernel CG: smallest ||r||^2 seen (stagnation test)
+    int32_t iters_h[2];      // single-kernel CG: iterations done
+    int32_t best_it_h[2];    // single-kernel CG: iteration at which best_rr_h was last halved
     int32_t converged;       // Krylov converged (or nothing to do)
     int32_t stalled;         // breakdown / iteration cap
-    int32_t iters;           // Krylov iterations done
+    int32_t iters;           // Krylov iterations done (reported to the host)
     int32_t active;          // NR still iterating on this segment
 };
 
@@ -59,6 +65,7 @@ struct KernelTimer {
 struct tf_handle {
     // sizes
     int32_t N = 0, E = 0, nb = 0, S = 1, T = 0;
+    int32_t wg = 256, tile_cap = 512;  // single-kernel CG workgroup size and tile capacity (blocks)
     // options
     tf_solver_opts opts;
     std::string err;
@@ -118,6 +125,7 @@ struct tf_handle {
     std::vector<int32_t> seg_nr_iters, seg_converged;
     std::vector<double> seg_du;
     int32_t last_krylov_iters = 0;
+    int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
 
     // graphs
     hipGraphExec_t krylov_graph = nullptr;
@@ -139,8 +147,10 @@ void launch_copy(tf_handle* h, double* dst, const double* src, int64_t n);
 void launch_fill(tf_handle* h, double* dst, double v, int64_t n);
 void launch_seg_norm(tf_handle* h, const double* x);   // per-segment ||x||_2 into h->seg_norm
 void launch_update_uk(tf_handle* h);                    // uk += du on active segments
+void build_tiles(tf_handle* h, int wg, int tile_cap);  // (re)partitions rows into tiles (tf_api.hip)
 // ---- Krylov (tf_krylov.hip) -----------------------------------------------------------------
 int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
+double krylov_time_cg1(tf_handle* h, int n_iters);  // ms per single-kernel CG iteration
 void launch_spmv(tf_handle* h, const double* x, double* y);
 
 // error helpers

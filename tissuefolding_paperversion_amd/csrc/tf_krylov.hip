@@ -1,7 +1,8 @@
 // tf_krylov.hip -- FP64 preconditioned Krylov solvers for M du = g on the BSR3 Jacobian (gfx950).
 //
-// Work unit: a row tile (a run of block rows of one tissue holding at most TF_TILE_MAX_BLOCKS
-// 3x3 blocks).  One 256-lane workgroup per tile.  The SpMV is CSR-stream: lanes stream the
+// Work unit: a row tile (a run of block rows of one tissue holding at most tile_cap 3x3 blocks
+// and at most wg/3 block rows).  One workgroup per tile (256 lanes; the single-kernel CG uses
+// the tunable wg).  The SpMV is CSR-stream: lanes stream the
 // tile's blocks contiguously (component-major values, so every load instruction is coalesced),
 // park the 3-vector products in LDS, then one lane per (row, component) sums its row.
 //
@@ -23,8 +24,7 @@
 
 namespace {
 
-constexpr int WG = 256;
-constexpr int NW = WG / 64;
+constexpr int WG = 256;  // workgroup of every tile kernel except the single-kernel CG (tunable)
 
 struct TileArgs {
     const int32_t* tile_r0;
@@ -44,9 +44,10 @@ __device__ inline double wave_sum(double v)
 }
 
 // Tile partials: reduce v[q] over the workgroup and store partial[slot[q]*T + t].
-template <int K>
+template <int K, int WGT = WG>
 __device__ inline void tile_write(const TileArgs& ta, int t, const int (&slot)[K], double (&v)[K])
 {
+    constexpr int NW = WGT / 64;
     __shared__ double red[K][NW];
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 #pragma unroll
@@ -67,15 +68,16 @@ __device__ inline void tile_write(const TileArgs& ta, int t, const int (&slot)[K
 }
 
 // Tissue totals of K partial slots, identical in every lane and every tile of the tissue.
-template <int K>
+template <int K, int WGT = WG>
 __device__ inline void seg_sums(const TileArgs& ta, int sg, const int (&slot)[K], double (&out)[K])
 {
+    constexpr int NW = WGT / 64;
     __shared__ double red[K][NW];
     const int t0 = ta.seg_tile_ptr[sg], t1 = ta.seg_tile_ptr[sg + 1];
     double s[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) s[q] = 0.0;
-    for (int k = t0 + threadIdx.x; k < t1; k += WG)
+    for (int k = t0 + threadIdx.x; k < t1; k += WGT)
 #pragma unroll
         for (int q = 0; q < K; ++q) s[q] += ta.partial[(size_t)slot[q] * ta.T + k];
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(WG) void k_spmv(const int32_t* __restrict__ row_ptr
                                              const int32_t* __restrict__ tile_r1, const double* __restrict__ x,
                                              double* __restrict__ y)
 {
-    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
     const int t = blockIdx.x;
     const int r0 = tile_r0[t], r1 = tile_r1[t];
     const int b0 = row_ptr[r0], b1 = row_ptr[r1];
@@ -170,6 +172,13 @@ __global__ __launch_bounds__(WG) void k_seg_init(TileArgs ta, double rtol2, doub
         S.tol2 = fmax(rtol2 * tot[0], atol2);
         S.iters = 0;
         S.stalled = 0;
+        for (int q = 0; q < 2; ++q) {
+            S.alpha_hist[q] = 0.0;
+            S.gam_hist[q] = 0.0;
+            S.best_rr_h[q] = tot[0];
+            S.iters_h[q] = 0;
+            S.best_it_h[q] = 0;
+        }
         S.converged = (!S.active || tot[0] <= S.tol2) ? 1 : 0;
     }
 }
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(WG) void k_bicg_spmv(TileArgs ta, const int32_t* __
                                                   const double* __restrict__ x, double* __restrict__ y,
                                                   const double* __restrict__ w)
 {
-    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
     if (seg_done(ta.seg[sg])) return;
@@ -358,7 +367,7 @@ __global__ __launch_bounds__(WG) void k_cg_q(TileArgs ta, const int32_t* __restr
                                              const double* __restrict__ p_old, double* __restrict__ p_new,
                                              double* __restrict__ q, int maxit)
 {
-    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(WG) void k_cg1_init(TileArgs ta, const int32_t* __r
                                                  double* __restrict__ x, double* __restrict__ r0, double* __restrict__ w0,
                                                  double* __restrict__ s_m1, double* __restrict__ u, double* __restrict__ p)
 {
-    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
     const bool active = ta.seg[sg].active;
@@ -496,88 +505,152 @@ __global__ __launch_bounds__(WG) void k_cg1_init(TileArgs ta, const int32_t* __r
     tile_write<4>(ta, t, sl, vv);
 }
 
-template <typename VT, int P>
-__global__ __launch_bounds__(WG) void k_cg1(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                                            const VT* __restrict__ val, int nb, const double* __restrict__ dinv,
-                                            double* __restrict__ x, const double* __restrict__ r_old,
-                                            double* __restrict__ r_new, const double* __restrict__ w_old,
-                                            double* __restrict__ w_new, const double* __restrict__ s_old,
-                                            double* __restrict__ s_new, double* __restrict__ u, double* __restrict__ p,
-                                            int maxit)
+// One iteration per launch.  Everything that does not depend on this iteration's scalars (the
+// tile's matrix blocks, the neighbour columns' r, w, s and block inverses, the own rows' vectors)
+// is loaded into registers first, so the loads are in flight while the workgroup re-sums the
+// partials; then the scalars are formed, own rows updated and the products parked in LDS.
+// WGT lanes per workgroup, BPT matrix blocks per lane (tile capacity WGT*BPT), at most one own
+// DoF per lane (tiles hold at most WGT/3 block rows).
+template <typename VT, int P, int WGT, int BPT>
+__global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                                             const VT* __restrict__ val, int nb, const double* __restrict__ dinv,
+                                             double* __restrict__ x, const double* __restrict__ r_old,
+                                             double* __restrict__ r_new, const double* __restrict__ w_old,
+                                             double* __restrict__ w_new, const double* __restrict__ s_old,
+                                             double* __restrict__ s_new, double* __restrict__ u, double* __restrict__ p,
+                                             int maxit, int tile_cap)
 {
-    __shared__ double prod[3 * TF_TILE_MAX_BLOCKS];
+    extern __shared__ double prod[];
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    // kernel parity P = k & 1: partials of iteration k-1 sit in slots of parity 1-P
-    const int sl[4] = {SL_RHO0 + (1 - P), SL_DL0 + (1 - P), SL_RR0 + (1 - P), SL_RHO0 + P};
-    double q[4];
-    seg_sums<4>(ta, sg, sl, q);
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    // ---- prefetch ----
+    double mv[BPT][9], cw[BPT][3], cs[BPT][3], cr[BPT][3], cd[BPT][9];
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        const int j = b0 + threadIdx.x + q * WGT;
+        if (j < b1) {
+            const int c = col[j];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) mv[q][k] = val[(size_t)k * nb + j];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                cw[q][k] = w_old[3 * c + k];
+                cs[q][k] = s_old[3 * c + k];
+                cr[q][k] = r_old[3 * c + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) cd[q][k] = dinv[9 * c + k];
+        }
+    }
+    const int d = d0 + threadIdx.x;
+    const bool own = d < d1;
+    double ow = 0.0, os = 0.0, orr = 0.0, ou = 0.0, op = 0.0, ox = 0.0, od[3] = {0.0, 0.0, 0.0};
+    int rb0 = 0, rb1 = 0;
+    if (own) {
+        ow = w_old[d]; os = s_old[d]; orr = r_old[d]; ou = u[d]; op = p[d]; ox = x[d];
+        const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
+        od[0] = D[0]; od[1] = D[1]; od[2] = D[2];
+        rb0 = row_ptr[d / 3]; rb1 = row_ptr[d / 3 + 1];
+    }
+    double* rloc = prod + 3 * (size_t)tile_cap;  // the tile's new residual, exchanged in LDS
+    // ---- scalars of iteration k-1: partials of launch k-1 (slots of parity 1-P) and the
+    //      history launch k-1 left in the parity-P slots of the tissue's scalars ----
+    const int sl[3] = {SL_RHO0 + (1 - P), SL_DL0 + (1 - P), SL_RR0 + (1 - P)};
+    double q3[3];
+    seg_sums<3, WGT>(ta, sg, sl, q3);
+    const double gk = q3[0], dk = q3[1], rr = q3[2];
+    const int it = S.iters_h[P];
+    const double best = S.best_rr_h[P];
+    const int best_it = S.best_it_h[P];
     const bool lead = first_tile(ta, t, sg) && threadIdx.x == 0;
     {
-        const bool conv = q[2] <= S.tol2;
-        const bool cap = !conv && (S.iters >= maxit || !isfinite(q[2]));
+        // The single-reduction recurrences can level off just above a very tight tolerance:
+        // without a halving of ||r|| for 256 iterations the solve stops, as converged when the
+        // best residual came within 10x of the tolerance, else as stalled (BiCGSTAB takes over).
+        const bool stagnant = it - best_it > 256;
+        const bool conv = rr <= S.tol2 || (stagnant && best <= 100.0 * S.tol2);
+        const bool cap = !conv && (it >= maxit || !isfinite(rr) || stagnant);
         if (conv || cap) {
             if (lead) {
-                S.rnorm2 = q[2];
+                S.rnorm2 = rr;
+                S.iters = it;
                 if (conv) S.converged = 1;
                 else S.stalled = 1;
             }
             return;
         }
     }
-    const bool first = S.iters == 0;
-    const double beta = first ? 0.0 : q[0] / q[3];
-    const double den = first ? q[1] : q[1] - beta * q[0] / S.alpha_hist[P];
-    const double alpha = q[0] / den;
+    const bool first = it == 0;
+    const double beta = first ? 0.0 : gk / S.gam_hist[P];
+    const double den = first ? dk : dk - beta * gk / S.alpha_hist[P];
+    const double alpha = gk / den;
     if (!(den > 0.0) || !isfinite(alpha)) {  // M not positive definite along this direction
-        if (lead) S.stalled = 1;
+        if (lead) {
+            S.iters = it;
+            S.stalled = 1;
+        }
         return;
     }
-    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
-    const int d0 = 3 * rr0, d1 = 3 * rr1;
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+    // ---- own rows ----
+    if (own) {
         double sn, rn;
-        cg1_update(w_old[d], s_old[d], r_old[d], beta, alpha, sn, rn);
-        const double pn = u[d] + beta * p[d];
+        cg1_update(ow, os, orr, beta, alpha, sn, rn);
+        const double pn = ou + beta * op;
         s_new[d] = sn;
         r_new[d] = rn;
+        rloc[d - d0] = rn;
         p[d] = pn;
-        x[d] = x[d] + alpha * pn;
+        x[d] = ox + alpha * pn;
+    }
+    // ---- products with the neighbours' u = D^-1 r (rebuilt from the prefetched columns) ----
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        const int j = b0 + threadIdx.x + q * WGT;
+        if (j < b1) {
+            double sn, q0, q1, q2;
+            cg1_update(cw[q][0], cs[q][0], cr[q][0], beta, alpha, sn, q0);
+            cg1_update(cw[q][1], cs[q][1], cr[q][1], beta, alpha, sn, q1);
+            cg1_update(cw[q][2], cs[q][2], cr[q][2], beta, alpha, sn, q2);
+            const double a0 = dinv_apply3(cd[q], q0, q1, q2);
+            const double a1 = dinv_apply3(cd[q] + 3, q0, q1, q2);
+            const double a2 = dinv_apply3(cd[q] + 6, q0, q1, q2);
+            double* pr = prod + 3 * (j - b0);
+            pr[0] = mv[q][0] * a0 + mv[q][1] * a1 + mv[q][2] * a2;
+            pr[1] = mv[q][3] * a0 + mv[q][4] * a1 + mv[q][5] * a2;
+            pr[2] = mv[q][6] * a0 + mv[q][7] * a1 + mv[q][8] * a2;
+        }
     }
     __syncthreads();
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) u[d] = dinv_row(dinv, r_new, d);
-    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
-    tile_products<VT>(col, val, nb, b0, b1,
-                      [&](int c, double& a0, double& a1, double& a2) {
-                          double sn, q0, q1, q2;
-                          cg1_update(w_old[3 * c], s_old[3 * c], r_old[3 * c], beta, alpha, sn, q0);
-                          cg1_update(w_old[3 * c + 1], s_old[3 * c + 1], r_old[3 * c + 1], beta, alpha, sn, q1);
-                          cg1_update(w_old[3 * c + 2], s_old[3 * c + 2], r_old[3 * c + 2], beta, alpha, sn, q2);
-                          const double* D = dinv + 9 * c;
-                          a0 = dinv_apply3(D, q0, q1, q2);
-                          a1 = dinv_apply3(D + 3, q0, q1, q2);
-                          a2 = dinv_apply3(D + 6, q0, q1, q2);
-                      },
-                      prod);
-    double g = 0.0, dl = 0.0, rr = 0.0;
-    for (int task = threadIdx.x; task < 3 * (rr1 - rr0); task += WG) {
-        const int r = rr0 + task / 3, comp = task % 3;
-        const int d = 3 * r + comp;
-        const double wv = row_sum(row_ptr, r, comp, b0, prod);
+    double g = 0.0, dl = 0.0, rrn = 0.0;
+    if (own) {
+        const int nd = d - d0 - d % 3;  // first DoF of this node within the tile
+        const double rv = rloc[d - d0];
+        const double un = od[0] * rloc[nd] + od[1] * rloc[nd + 1] + od[2] * rloc[nd + 2];
+        u[d] = un;
+        const int comp = d % 3;
+        double wv = 0.0;
+        for (int j = rb0; j < rb1; ++j) wv += prod[3 * (j - b0) + comp];
         w_new[d] = wv;
-        const double uv = u[d], rv = r_new[d];
-        g += rv * uv;
-        dl += wv * uv;
-        rr += rv * rv;
+        g = rv * un;
+        dl = wv * un;
+        rrn = rv * rv;
     }
     const int so[3] = {SL_RHO0 + P, SL_DL0 + P, SL_RR0 + P};
-    double vv[3] = {g, dl, rr};
-    tile_write<3>(ta, t, so, vv);
+    double vv[3] = {g, dl, rrn};
+    tile_write<3, WGT>(ta, t, so, vv);
     if (lead) {
         S.alpha_hist[1 - P] = alpha;
-        S.iters += 1;
+        S.gam_hist[1 - P] = gk;
+        S.iters_h[1 - P] = it + 1;
+        S.iters = it + 1;
+        const bool better = rr < 0.5 * best;
+        S.best_rr_h[1 - P] = better ? rr : best;
+        S.best_it_h[1 - P] = better ? it : best_it;
     }
 }
 
@@ -616,12 +689,13 @@ void enqueue_bicg_iter(tf_handle* h, const VT* val)
                        h->opts.krylov_max_iter);
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_bicg_spmv<VT, 0>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->kph, h->kv,
+    const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
+    hipLaunchKernelGGL((k_bicg_spmv<VT, 0>), grid, blk, lds, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->kph, h->kv,
                        h->kr0);
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL(k_bicg_s<P>, grid, blk, 0, h->stream, ta, h->dinv, h->kr, h->kv, h->ks, h->ksh);
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_bicg_spmv<VT, 1>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ksh, h->kt,
+    hipLaunchKernelGGL((k_bicg_spmv<VT, 1>), grid, blk, lds, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ksh, h->kt,
                        h->ks);
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL(k_bicg_x<P>, grid, blk, 0, h->stream, ta, h->kph, h->ksh, h->ks, h->kt, h->kr0, h->kx, h->kr);
@@ -637,7 +711,7 @@ void enqueue_cg_iter(tf_handle* h, const VT* val)
     double* p_new = P ? h->kp : h->kph;
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_cg_q<VT, P>), grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ks, p_old, p_new,
+    hipLaunchKernelGGL((k_cg_q<VT, P>), grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ks, p_old, p_new,
                        h->kv, h->opts.krylov_max_iter);
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL(k_cg_x<P>, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks);
@@ -654,8 +728,20 @@ void enqueue_cg1_iter(tf_handle* h, const VT* val)
     double* Sv[2] = {h->ks, h->ksh};
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_cg1<VT, P>), dim3(h->T), dim3(WG), 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv,
-                       h->kx, R[1 - P], R[P], W[1 - P], W[P], Sv[P], Sv[1 - P], h->kph, h->kp, h->opts.krylov_max_iter);
+    const size_t lds = sizeof(double) * (3 * (size_t)h->tile_cap + h->wg);
+#define TF_CG1(WGT, BPT)                                                                                          \
+    hipLaunchKernelGGL((k_cg1<VT, P, WGT, BPT>), dim3(h->T), dim3(WGT), lds, h->stream, ta, h->row_ptr, h->col, val, \
+                       h->nb, h->dinv, h->kx, R[1 - P], R[P], W[1 - P], W[P], Sv[P], Sv[1 - P], h->kph, h->kp,        \
+                       h->opts.krylov_max_iter, h->tile_cap)
+    const int bpt = h->tile_cap / h->wg;
+    if (h->wg == 512) {
+        if (bpt == 1) TF_CG1(512, 1); else if (bpt == 2) TF_CG1(512, 2); else TF_CG1(512, 4);
+    } else if (h->wg == 1024) {
+        if (bpt == 1) TF_CG1(1024, 1); else TF_CG1(1024, 2);
+    } else {
+        if (bpt == 1) TF_CG1(256, 1); else if (bpt == 2) TF_CG1(256, 2); else TF_CG1(256, 4);
+    }
+#undef TF_CG1
     timer_end(h, h->t_spmv, ev);
 }
 
@@ -693,7 +779,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const double rtol2 = h->opts.krylov_rtol * h->opts.krylov_rtol;
     const double atol2 = h->opts.krylov_atol * h->opts.krylov_atol;
     if (kind == TF_KRYLOV_CG1)
-        hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, 0, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv, h->g,
+        hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv, h->g,
                            h->kx, h->kr, h->kv, h->ksh, h->kph, h->kp);
     else if (kind == TF_KRYLOV_CG)
         hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
@@ -759,6 +845,32 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
 
 }  // namespace
 
+// Average device time (ms) of one single-kernel CG iteration over n_iters iterations on the
+// current Jacobian, under the current tiling (tolerance 0; no fallback).
+double krylov_time_cg1(tf_handle* h, int n_iters)
+{
+    const tf_solver_opts saved = h->opts;
+    h->opts.krylov_rtol = 0.0;
+    h->opts.krylov_atol = 0.0;
+    h->opts.krylov_max_iter = n_iters - 1;  // launch n_iters sees the cap: one replay
+    h->opts.iters_per_sync = n_iters;
+    const bool was = h->t_spmv.enabled;
+    h->t_spmv.enabled = true;
+    double ms = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {  // the first run captures the graph
+        h->t_spmv.total_ms = 0.0;
+        h->t_spmv.launches = 0;
+        if (h->opts.matrix_precision == TF_PREC_FP32) solve_kind<float>(h, h->val32, TF_KRYLOV_CG1);
+        else solve_kind<double>(h, h->val, TF_KRYLOV_CG1);
+        ms = h->t_spmv.launches ? h->t_spmv.total_ms / h->t_spmv.launches : 0.0;
+    }
+    h->t_spmv.enabled = was;
+    h->t_spmv.total_ms = 0.0;
+    h->t_spmv.launches = 0;
+    h->opts = saved;
+    return ms;
+}
+
 // Solve M du = g for every active tissue; du is written to h->du.
 int krylov_solve(tf_handle* h)
 {
@@ -769,6 +881,7 @@ int krylov_solve(tf_handle* h)
     if (st == TF_KRYLOV_STALL && (kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1)) {
         // not positive definite (or too slow): BiCGSTAB from zero
         const int cg_iters = h->last_krylov_iters;
+        h->fallbacks += 1;
         st = run(TF_KRYLOV_BICGSTAB);
         h->last_krylov_iters += cg_iters;
     }
@@ -779,10 +892,10 @@ int krylov_solve(tf_handle* h)
 void launch_spmv(tf_handle* h, const double* x, double* y)
 {
     if (h->opts.matrix_precision == TF_PREC_FP32)
-        hipLaunchKernelGGL(k_spmv<float>, dim3(h->T), dim3(WG), 0, h->stream, h->row_ptr, h->col, h->val32, h->nb,
+        hipLaunchKernelGGL(k_spmv<float>, dim3(h->T), dim3(WG), 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, h->row_ptr, h->col, h->val32, h->nb,
                            h->tile_r0, h->tile_r1, x, y);
     else
-        hipLaunchKernelGGL(k_spmv<double>, dim3(h->T), dim3(WG), 0, h->stream, h->row_ptr, h->col, h->val, h->nb,
+        hipLaunchKernelGGL(k_spmv<double>, dim3(h->T), dim3(WG), 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, h->row_ptr, h->col, h->val, h->nb,
                            h->tile_r0, h->tile_r1, x, y);
     TF_HIP(hipGetLastError());
 }

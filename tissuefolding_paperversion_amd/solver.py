@@ -210,6 +210,23 @@ class NewtonRaphsonSolver:
         self._check(self._lib.tf_get_segment_stats(self.h, L.ptr(it), L.ptr(dn), L.ptr(cv)))
         return it, dn, cv
 
+    # ---- tuning -----------------------------------------------------------------------------------
+    def set_tiling(self, wg, tile_cap):
+        self._check(self._lib.tf_set_tiling(self.h, int(wg), int(tile_cap)))
+
+    def tiling(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self._lib.tf_get_tiling(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, dt=60.0, n_iters=64):
+        """Time the single-kernel CG under every supported tiling and keep the fastest."""
+        n = C.c_int32()
+        wg, cap, ms = np.zeros(16, np.int32), np.zeros(16, np.int32), np.zeros(16)
+        self._check(self._lib.tf_autotune(self.h, dt, int(n_iters), C.byref(n), L.ptr(wg), L.ptr(cap), L.ptr(ms)))
+        k = n.value
+        return [dict(wg=int(wg[i]), tile_cap=int(cap[i]), ms_per_iter=float(ms[i])) for i in range(k)]
+
     # ---- measurement ------------------------------------------------------------------------------
     def kernel_timing(self, name, enable=-1):
         avg, n = C.c_double(), C.c_int64()
