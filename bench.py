@@ -55,7 +55,7 @@ def parse():
 def traffic_from_profiles(workload):
     """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary, scaled by the
     copy-kernel calibration recorded in the same file (None if absent)."""
-    path = os.path.join(ROOT, "profiles", f"spmv_{workload}_pmc.txt")
+    path = os.path.join(ROOT, "profiles", f"k_cg1_{workload}_pmc.txt")
     if not os.path.exists(path):
         return None
     txt = open(path).read()

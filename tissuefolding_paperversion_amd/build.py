@@ -1,6 +1,7 @@
 """Build recipe for the HIP library (gfx950) -- run as ``python -m tissuefolding_paperversion_amd.build``.
 
-Produces ``tissuefolding_paperversion_amd/lib/libtfnr.so`` in-tree with hipcc.  Packed FP32
+Produces ``tissuefolding_paperversion_amd/lib/libtfnr.so`` in-tree with hipcc, the C++ driver
+``lib/tf_driver`` (g++) and the copy-kernel calibration program ``lib/tf_copy_calib`` (hipcc).  Packed FP32
 instructions are disabled for all device code (``-packed-fp32-ops``).
 """
 from __future__ import annotations
@@ -46,6 +47,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("g++ failed building tf_driver")
+    # copy kernel of known size, the calibration for the profiler's HBM byte counters
+    cal = os.path.join(LIBDIR, "tf_copy_calib")
+    cmd = [HIPCC] + [f for f in flags() if f not in ("-shared", "-fPIC")] + [os.path.join(CSRC, "tf_copy_calib.hip"),
+                                                                       "-o", cal]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building tf_copy_calib")
     return LIB
 
 
