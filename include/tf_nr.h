@@ -113,6 +113,21 @@ int tf_set_fg(tf_handle* h, const double* Fg);
 int tf_get_fg(tf_handle* h, double* Fg, double* InvFg);
 int tf_set_material(tf_handle* h, const double* lambda, const double* mu, const double* eta_int);
 
+/* ---- DoF constraints beyond FixedPos ----------------------------------------------------------- */
+/* Slave/master DoF binding: NewtonRaphsonSolver::slaveMasterList with
+ * calculateBoundKWithSlavesMasterDoF and equateSlaveDisplacementsToMasters
+ * (NewtonRaphsonSolver.cpp:154-235, :397-399).  pairs[2k] = slave DoF, pairs[2k+1] = master DoF
+ * (DoF = 3*node + dim).  Every later assembly applies M <- N^T M N + I_slaves and g <- N^T g after
+ * the fixed DoFs, and every solve sets dU[slave] = dU[master].  Replaces the previous list
+ * (n_pairs = 0 removes all binding) and widens the Jacobian pattern (tf_get_sizes reports the new
+ * block count).  A slave is bound at most once, a slave is never a master, and both DoFs belong to
+ * one tissue; otherwise TF_INVALID. */
+int tf_set_bound_dofs(tf_handle* h, int32_t n_pairs, const int32_t* pairs);
+/* Ablated nodes (Simulation::AblatedNodes): a zero Jacobian diagonal entry of theirs is set to 1
+ * before the fixed DoFs are applied (NewtonRaphsonSolver::checkJacobianForAblatedNodes,
+ * NewtonRaphsonSolver.cpp:350-361).  Replaces the previous list; n = 0 clears it. */
+int tf_set_ablated_nodes(tf_handle* h, int32_t n, const int32_t* nodes);
+
 /* ---- per-step set-up ----------------------------------------------------------------------- */
 /* Simulation::updateGrowthRotationMatrices + calculateGrowth + ShapeBase::growShapeByFg
  * (Simulation.cpp:5876-5909, :9651-9799; ShapeBase.cpp:396-421, :1215-1251, :1656-1811). */

@@ -61,6 +61,26 @@ public:
     }
     void updateNodeViscositySurfaces() { check(tf_update_viscosity_surfaces(h_), h_); }
 
+    // NewtonRaphsonSolver::slaveMasterList (each entry {slave DoF, master DoF}); the binding is
+    // applied by every later assembly (calculateBoundKWithSlavesMasterDoF) and solve
+    // (equateSlaveDisplacementsToMasters).
+    void setSlaveMasterList(const std::vector<std::vector<int>>& slaveMasterList)
+    {
+        std::vector<int32_t> p;
+        for (const auto& sm : slaveMasterList) {
+            if (sm.size() != 2) throw Error(TF_INVALID, "slave/master entries hold two DoFs");
+            p.push_back(sm[0]);
+            p.push_back(sm[1]);
+        }
+        check(tf_set_bound_dofs(h_, (int32_t)slaveMasterList.size(), p.empty() ? nullptr : p.data()), h_);
+    }
+    // Simulation::AblatedNodes, used by checkJacobianForAblatedNodes.
+    void setAblatedNodes(const std::vector<int>& nodes)
+    {
+        std::vector<int32_t> n(nodes.begin(), nodes.end());
+        check(tf_set_ablated_nodes(h_, (int32_t)n.size(), n.empty() ? nullptr : n.data()), h_);
+    }
+
     // ---- NewtonRaphsonSolver call sequence ----
     void constructUnMatrix() { check(tf_begin_step(h_), h_); }
     void initialteUkMatrix() {}  // folded into constructUnMatrix (uk <- un)

@@ -179,6 +179,8 @@ class OracleTissue:
         self.last_rotations = 0
         self._csr = None
         self.stats = []
+        self.bound = []      # slaveMasterList: (slave DoF, master DoF) pairs in binding order
+        self.ablated = []    # AblatedNodes
 
     def _model(self):
         mm = _Model()
@@ -231,16 +233,52 @@ class OracleTissue:
         return rp, col, val
 
     def assemble(self, uk, un, dt, dense=None):
-        """(K_ref, g): K in the reference orientation (dense ndarray or scipy CSR)."""
+        """(K_ref, g): K in the reference orientation (dense ndarray or scipy CSR), with ablated
+        nodes, fixed DoFs and slave/master binding applied as in Simulation::updateStepNR."""
         fe, fv, Ke, Kv = self.elements(uk, un, dt)
         g = self.residual(uk, un, dt, fe, fv)
         if dense is None:
             dense = 3 * self.N <= 3000
         if dense:
-            return self.dense_K(dt, Ke, Kv), g
-        import scipy.sparse as sp
-        rp, col, val = self.csr_K(dt, Ke, Kv)
-        return sp.csr_matrix((val, col, rp), shape=(3 * self.N, 3 * self.N)), g
+            K = self.dense_K(dt, Ke, Kv)
+        else:
+            import scipy.sparse as sp
+            rp, col, val = self.csr_K(dt, Ke, Kv)
+            K = sp.csr_matrix((val, col, rp), shape=(3 * self.N, 3 * self.N))
+        if self.ablated or self.bound:
+            if not isinstance(K, np.ndarray):
+                raise NotImplementedError("ablated nodes and bound DoFs use the dense oracle")
+            self.apply_ablated(K)
+            self.apply_binding(K, g)
+        return K, g
+
+    def apply_ablated(self, K):
+        """checkJacobianForAblatedNodes (NewtonRaphsonSolver.cpp:350-361).  The reference runs it
+        before calcutateFixedK; fixed DoFs only rewrite their own rows and columns, so applying
+        it after them gives the same matrix."""
+        for node in self.ablated:
+            for i in range(3):
+                d = 3 * node + i
+                if K[d, d] == 0:
+                    K[d, d] = 1.0
+
+    def apply_binding(self, K, g):
+        """calculateBoundKWithSlavesMasterDoF (NewtonRaphsonSolver.cpp:154-217): pair by pair,
+        g_m += g_s, g_s = 0; row m += row s, row s = 0; col m += col s, col s = 0; K_ss = 1."""
+        for s_, m_ in self.bound:
+            g[m_] += g[s_]
+            g[s_] = 0.0
+            K[m_, :] += K[s_, :]
+            K[s_, :] = 0.0
+            K[:, m_] += K[:, s_]
+            K[:, s_] = 0.0
+            K[s_, s_] = 1.0
+
+    def equate_slaves(self, dU):
+        """equateSlaveDisplacementsToMasters (NewtonRaphsonSolver.cpp:219-235)."""
+        for s_, m_ in self.bound:
+            dU[s_] = dU[m_]
+        return dU
 
     def solve(self, K, g):
         """Solve K^T dU = g (the reference's transposed solve)."""
@@ -284,7 +322,7 @@ class OracleTissue:
         d = np.inf
         while not conv:
             K, g = self.assemble(uk, un, dt, dense)
-            dU = self.solve(K, g)
+            dU = self.equate_slaves(self.solve(K, g))
             d = float(np.linalg.norm(dU))
             conv = d <= tol
             uk += dU

@@ -25,7 +25,8 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_set_material", "tf_grow", "tf_update_viscosity_surfaces", "tf_get_viscosity_surfaces",
            "tf_begin_step", "tf_assemble", "tf_get_residual", "tf_get_jacobian_bsr", "tf_element_dump", "tf_solve",
            "tf_update_uk", "tf_nr_step", "tf_timestep", "tf_get_segment_stats", "tf_kernel_timing",
-           "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune"]
+           "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune", "tf_set_bound_dofs",
+           "tf_set_ablated_nodes"]
 
 
 class TfError(RuntimeError):
@@ -104,6 +105,8 @@ def load(path: str | None = None):
         "tf_set_tiling": ([vp, i32, i32], i32),
         "tf_get_tiling": ([vp, vp, vp, vp], i32),
         "tf_autotune": ([vp, dbl, i32, vp, vp, vp, vp], i32),
+        "tf_set_bound_dofs": ([vp, i32, vp], i32),
+        "tf_set_ablated_nodes": ([vp, i32, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -75,7 +75,9 @@ void free_all(tf_handle* h)
                     h->status, h->row_ptr, h->col, h->blk_row, h->diag_blk, h->val, h->val32, h->dinv, h->blk_ptr,
                     h->blk_src, h->node_ptr, h->node_src, h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr,
                     h->seg_node_ptr, h->seg_counter, h->partial, h->seg, h->seg_norm, h->kr, h->kr0, h->kp, h->kph,
-                    h->kv, h->ks, h->ksh, h->kt, h->kx};
+                    h->kv, h->ks, h->ksh, h->kt, h->kx, h->fold_ptr, h->fold_tgt, h->fold_src, h->fold_const,
+                    h->val_base, h->bind_nodes, h->bind_m_ptr, h->bind_m_dof, h->bind_s_dof, h->bind_pairs,
+                    h->ablated};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
@@ -188,6 +190,231 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
 }
 
 // ---------------------------------------------------------------------------------------------
+// BSR pattern, element-block -> BSR-block scatter lists and, when slave/master DoF pairs are
+// bound, the fold lists that turn the assembled M into N^T M N + I_slaves.
+//
+// The element pattern couples node A to every node of every element containing A.  Binding a
+// slave DoF s of node S to a master DoF m of node Mn adds the blocks the fold fills in: with
+// T(X) = {X} u {nodes of masters of X's DoFs}, every element block (r, c) spreads to
+// T(r) x T(c).  Fold targets are the entries in a row or column of a bound DoF: slave rows and
+// columns become unit rows/columns (SC/NewtonRaphsonSolver.cpp:205-209), every other target
+// entry (a, b) is the sum of M[c][d] over c in {a} u slaves(a), d in {b} u slaves(b) (the rows
+// and columns the reference adds into the master's, SC/NewtonRaphsonSolver.cpp:188-206).
+void build_pattern(tf_handle* h)
+{
+    const int N = h->N, E = h->E;
+    const std::vector<int32_t>& conn = h->h_conn;
+    const std::vector<int32_t>& node_ptr = h->h_node_ptr;
+    const std::vector<int32_t>& node_src = h->h_node_src;
+    const int nbound = (int)h->h_bound.size() / 2;
+    // masters of each node's DoFs (node level) and DoF-level slave/master maps
+    std::vector<int32_t> slave_of((size_t)3 * N, -1);
+    std::vector<std::vector<int32_t>> slaves_of_master;
+    std::vector<int32_t> master_index((size_t)3 * N, -1), master_list;
+    std::vector<std::vector<int32_t>> node_masters(nbound ? N : 0);
+    for (int k = 0; k < nbound; ++k) {
+        const int sd = h->h_bound[2 * k], md = h->h_bound[2 * k + 1];
+        slave_of[sd] = md;
+        if (master_index[md] < 0) {
+            master_index[md] = (int32_t)master_list.size();
+            master_list.push_back(md);
+            slaves_of_master.emplace_back();
+        }
+        slaves_of_master[master_index[md]].push_back(sd);
+        node_masters[sd / 3].push_back(md / 3);
+    }
+    for (auto& v : node_masters) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    // ---- node pattern ----
+    std::vector<std::vector<int32_t>> rows(N);
+    for (int A = 0; A < N; ++A) {
+        std::vector<int32_t>& r = rows[A];
+        for (int k = node_ptr[A]; k < node_ptr[A + 1]; ++k) {
+            const int e = node_src[k] / 6;
+            for (int b = 0; b < 6; ++b) r.push_back(conn[6 * e + b]);
+        }
+        r.push_back(A);  // every node keeps its diagonal block (isolated and ablated nodes too)
+    }
+    if (nbound) {
+        // spread every existing block (r, c) to T(r) x T(c)
+        std::vector<std::vector<int32_t>> extra(N);
+        for (int A = 0; A < N; ++A) {
+            std::vector<int32_t> ta{A};
+            ta.insert(ta.end(), node_masters[A].begin(), node_masters[A].end());
+            for (int c : rows[A]) {
+                std::vector<int32_t> tc{c};
+                tc.insert(tc.end(), node_masters[c].begin(), node_masters[c].end());
+                for (int a : ta)
+                    for (int b : tc)
+                        if (a != A || b != c) extra[a].push_back(b);
+            }
+        }
+        for (int A = 0; A < N; ++A) rows[A].insert(rows[A].end(), extra[A].begin(), extra[A].end());
+    }
+    std::vector<int32_t> row_ptr(N + 1, 0), col;
+    col.reserve((size_t)N * 20);
+    for (int A = 0; A < N; ++A) {
+        std::vector<int32_t>& r = rows[A];
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        col.insert(col.end(), r.begin(), r.end());
+        row_ptr[A + 1] = (int32_t)col.size();
+        std::vector<int32_t>().swap(r);
+    }
+    if (col.size() > (size_t)INT32_MAX / 9) throw std::runtime_error("Jacobian pattern too large for int32 indexing");
+    const int nb = (int)col.size();
+    std::vector<int32_t> blk_row(nb), diag_blk(N, -1);
+    for (int A = 0; A < N; ++A)
+        for (int j = row_ptr[A]; j < row_ptr[A + 1]; ++j) {
+            blk_row[j] = A;
+            if (col[j] == A) diag_blk[A] = j;
+        }
+    auto find_blk = [&](int A, int B) -> int {
+        const int32_t* c0 = col.data() + row_ptr[A];
+        const int32_t* c1 = col.data() + row_ptr[A + 1];
+        const int32_t* it = std::lower_bound(c0, c1, B);
+        return (it != c1 && *it == B) ? (int)(it - col.data()) : -1;
+    };
+    // ---- element block -> BSR block, then contributors per block (ascending e*36+ab) ----
+    std::vector<int32_t> eblk((size_t)36 * E);
+    std::vector<int32_t> blk_ptr(nb + 1, 0);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b) {
+                const int j = find_blk(conn[6 * e + a], conn[6 * e + b]);
+                eblk[(size_t)36 * e + 6 * a + b] = j;
+                blk_ptr[j + 1]++;
+            }
+    for (int j = 0; j < nb; ++j) blk_ptr[j + 1] += blk_ptr[j];
+    std::vector<int32_t> blk_src((size_t)36 * E);
+    {
+        std::vector<int32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
+        for (size_t k = 0; k < (size_t)36 * E; ++k) blk_src[fill[eblk[k]]++] = (int32_t)k;
+    }
+    // ---- fold lists ----
+    std::vector<int32_t> fold_ptr{0}, bind_nodes, m_ptr{0}, m_dof, s_dof, pairs_dev;
+    std::vector<int64_t> fold_tgt, fold_src;
+    std::vector<double> fold_const;
+    if (nbound) {
+        std::vector<uint8_t> is_tgt((size_t)9 * nb, 0);
+        // column -> blocks map, for the column targets
+        std::vector<int32_t> cptr(N + 1, 0), cblk(nb);
+        for (int j = 0; j < nb; ++j) cptr[col[j] + 1]++;
+        for (int n = 0; n < N; ++n) cptr[n + 1] += cptr[n];
+        {
+            std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+            for (int j = 0; j < nb; ++j) cblk[fill[col[j]]++] = j;
+        }
+        std::vector<int32_t> bound_dofs;
+        for (int k = 0; k < 2 * nbound; ++k) bound_dofs.push_back(h->h_bound[k]);
+        std::sort(bound_dofs.begin(), bound_dofs.end());
+        bound_dofs.erase(std::unique(bound_dofs.begin(), bound_dofs.end()), bound_dofs.end());
+        std::vector<int64_t> targets;
+        for (int x : bound_dofs) {
+            const int X = x / 3, xi = x % 3;
+            for (int j = row_ptr[X]; j < row_ptr[X + 1]; ++j)
+                for (int jj = 0; jj < 3; ++jj) {
+                    const int64_t t = (int64_t)(3 * xi + jj) * nb + j;
+                    if (!is_tgt[t]) { is_tgt[t] = 1; targets.push_back(t); }
+                }
+            for (int k = cptr[X]; k < cptr[X + 1]; ++k)
+                for (int ii = 0; ii < 3; ++ii) {
+                    const int64_t t = (int64_t)(3 * ii + xi) * nb + cblk[k];
+                    if (!is_tgt[t]) { is_tgt[t] = 1; targets.push_back(t); }
+                }
+            bind_nodes.push_back(X);
+        }
+        std::sort(targets.begin(), targets.end());
+        std::sort(bind_nodes.begin(), bind_nodes.end());
+        bind_nodes.erase(std::unique(bind_nodes.begin(), bind_nodes.end()), bind_nodes.end());
+        auto group = [&](int dof) {
+            std::vector<int32_t> g{dof};
+            if (master_index[dof] >= 0) {
+                const auto& sl = slaves_of_master[master_index[dof]];
+                g.insert(g.end(), sl.begin(), sl.end());
+            }
+            return g;
+        };
+        for (int64_t t : targets) {
+            const int q = (int)(t / nb), j = (int)(t % nb);
+            const int a = 3 * blk_row[j] + q / 3, b = 3 * col[j] + q % 3;
+            fold_tgt.push_back(t);
+            if (slave_of[a] >= 0 || slave_of[b] >= 0) {
+                fold_const.push_back(a == b ? 1.0 : 0.0);
+            } else {
+                fold_const.push_back(0.0);
+                for (int c : group(a))
+                    for (int d : group(b)) {
+                        const int jb = find_blk(c / 3, d / 3);
+                        if (jb >= 0) fold_src.push_back((int64_t)(3 * (c % 3) + d % 3) * nb + jb);
+                    }
+            }
+            fold_ptr.push_back((int32_t)fold_src.size());
+        }
+        for (size_t k = 0; k < master_list.size(); ++k) {
+            m_dof.push_back(master_list[k]);
+            s_dof.insert(s_dof.end(), slaves_of_master[k].begin(), slaves_of_master[k].end());
+            m_ptr.push_back((int32_t)s_dof.size());
+        }
+        pairs_dev = h->h_bound;
+    }
+    // ---- device arrays ----
+    for (int32_t** p : {&h->row_ptr, &h->col, &h->blk_row, &h->diag_blk, &h->blk_ptr, &h->blk_src, &h->fold_ptr,
+                        &h->bind_nodes, &h->bind_m_ptr, &h->bind_m_dof, &h->bind_s_dof, &h->bind_pairs})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (int64_t** p : {&h->fold_tgt, &h->fold_src})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (double** p : {&h->val, &h->val_base, &h->fold_const})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (h->val32) { (void)hipFree(h->val32); h->val32 = nullptr; }
+    h->nb = nb;
+    h->row_ptr = dev_alloc<int32_t>(N + 1);
+    h->col = dev_alloc<int32_t>(nb);
+    h->blk_row = dev_alloc<int32_t>(nb);
+    h->diag_blk = dev_alloc<int32_t>(N);
+    h->val = dev_alloc<double>((size_t)9 * nb);
+    h->val32 = dev_alloc<float>((size_t)9 * nb);
+    h->blk_ptr = dev_alloc<int32_t>(nb + 1);
+    h->blk_src = dev_alloc<int32_t>((size_t)36 * E);
+    h2d(h, h->row_ptr, row_ptr.data(), N + 1);
+    h2d(h, h->col, col.data(), nb);
+    h2d(h, h->blk_row, blk_row.data(), nb);
+    h2d(h, h->diag_blk, diag_blk.data(), N);
+    h2d(h, h->blk_ptr, blk_ptr.data(), nb + 1);
+    h2d(h, h->blk_src, blk_src.data(), blk_src.size());
+    h->n_bound = nbound;
+    h->n_fold = (int32_t)fold_tgt.size();
+    h->n_bind_nodes = (int32_t)bind_nodes.size();
+    h->n_masters = (int32_t)m_dof.size();
+    if (nbound) {
+        h->val_base = dev_alloc<double>((size_t)9 * nb);
+        h->fold_ptr = dev_alloc<int32_t>(fold_ptr.size());
+        h->fold_tgt = dev_alloc<int64_t>(fold_tgt.size());
+        h->fold_src = dev_alloc<int64_t>(std::max<size_t>(1, fold_src.size()));
+        h->fold_const = dev_alloc<double>(fold_const.size());
+        h->bind_nodes = dev_alloc<int32_t>(bind_nodes.size());
+        h->bind_m_ptr = dev_alloc<int32_t>(m_ptr.size());
+        h->bind_m_dof = dev_alloc<int32_t>(m_dof.size());
+        h->bind_s_dof = dev_alloc<int32_t>(s_dof.size());
+        h->bind_pairs = dev_alloc<int32_t>(pairs_dev.size());
+        h2d(h, h->fold_ptr, fold_ptr.data(), fold_ptr.size());
+        h2d(h, h->fold_tgt, fold_tgt.data(), fold_tgt.size());
+        if (!fold_src.empty()) h2d(h, h->fold_src, fold_src.data(), fold_src.size());
+        h2d(h, h->fold_const, fold_const.data(), fold_const.size());
+        h2d(h, h->bind_nodes, bind_nodes.data(), bind_nodes.size());
+        h2d(h, h->bind_m_ptr, m_ptr.data(), m_ptr.size());
+        h2d(h, h->bind_m_dof, m_dof.data(), m_dof.size());
+        h2d(h, h->bind_s_dof, s_dof.data(), s_dof.size());
+        h2d(h, h->bind_pairs, pairs_dev.data(), pairs_dev.size());
+    }
+    TF_HIP(hipStreamSynchronize(h->stream));
+    h->h_row_ptr = row_ptr;
+    h->h_col = col;
+}
+
+// ---------------------------------------------------------------------------------------------
 extern "C" {
 
 void tf_default_options(tf_solver_opts* o)
@@ -265,54 +492,8 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
             std::vector<int32_t> fill(node_ptr.begin(), node_ptr.end() - 1);
             for (int k = 0; k < 6 * E; ++k) node_src[fill[h->h_conn[k]]++] = k;  // k = e*6 + a, ascending
         }
-        // ---- BSR pattern: row A couples to every node of every element containing A ----
-        std::vector<int32_t> row_ptr(N + 1, 0), col;
-        col.reserve((size_t)N * 20);
-        std::vector<int32_t> tmp;
-        for (int A = 0; A < N; ++A) {
-            tmp.clear();
-            for (int k = node_ptr[A]; k < node_ptr[A + 1]; ++k) {
-                const int e = node_src[k] / 6;
-                for (int b = 0; b < 6; ++b) tmp.push_back(h->h_conn[6 * e + b]);
-            }
-            if (tmp.empty()) tmp.push_back(A);  // isolated node keeps its diagonal
-            std::sort(tmp.begin(), tmp.end());
-            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            col.insert(col.end(), tmp.begin(), tmp.end());
-            row_ptr[A + 1] = (int32_t)col.size();
-        }
-        const int nb = (int)col.size();
-        h->nb = nb;
-        std::vector<int32_t> blk_row(nb), diag_blk(N, -1);
-        for (int A = 0; A < N; ++A)
-            for (int j = row_ptr[A]; j < row_ptr[A + 1]; ++j) {
-                blk_row[j] = A;
-                if (col[j] == A) diag_blk[A] = j;
-            }
-        // ---- element block -> BSR block, then contributors per block (ascending e*36+ab) ----
-        std::vector<int32_t> eblk((size_t)36 * E);
-        std::vector<int32_t> blk_ptr(nb + 1, 0);
-        for (int e = 0; e < E; ++e)
-            for (int a = 0; a < 6; ++a) {
-                const int A = h->h_conn[6 * e + a];
-                const int32_t* c0 = col.data() + row_ptr[A];
-                const int32_t* c1 = col.data() + row_ptr[A + 1];
-                for (int b = 0; b < 6; ++b) {
-                    const int B = h->h_conn[6 * e + b];
-                    const int j = (int)(std::lower_bound(c0, c1, B) - col.data());
-                    eblk[(size_t)36 * e + 6 * a + b] = j;
-                    blk_ptr[j + 1]++;
-                }
-            }
-        for (int j = 0; j < nb; ++j) blk_ptr[j + 1] += blk_ptr[j];
-        std::vector<int32_t> blk_src((size_t)36 * E);
-        {
-            std::vector<int32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
-            for (size_t k = 0; k < (size_t)36 * E; ++k) blk_src[fill[eblk[k]]++] = (int32_t)k;
-        }
-        h->h_row_ptr = row_ptr;
-        h->h_col = col;
-
+        h->h_node_ptr = node_ptr;
+        h->h_node_src = node_src;
         // ---- device allocations ----
         h->conn = dev_alloc<int32_t>((size_t)6 * E);
         h->xref = dev_alloc<double>((size_t)18 * E);
@@ -335,15 +516,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->kelem = dev_alloc<double>((size_t)324 * E);
         h->felem = dev_alloc<double>((size_t)18 * E);
         h->status = dev_alloc<int32_t>(4);
-        h->row_ptr = dev_alloc<int32_t>(N + 1);
-        h->col = dev_alloc<int32_t>(nb);
-        h->blk_row = dev_alloc<int32_t>(nb);
-        h->diag_blk = dev_alloc<int32_t>(N);
-        h->val = dev_alloc<double>((size_t)9 * nb);
-        h->val32 = dev_alloc<float>((size_t)9 * nb);
         h->dinv = dev_alloc<double>((size_t)9 * N);
-        h->blk_ptr = dev_alloc<int32_t>(nb + 1);
-        h->blk_src = dev_alloc<int32_t>((size_t)36 * E);
         h->node_ptr = dev_alloc<int32_t>(N + 1);
         h->node_src = dev_alloc<int32_t>((size_t)6 * E);
         h->seg_node_ptr = dev_alloc<int32_t>(S + 1);
@@ -391,14 +564,9 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h2d(h, h->du, zero3N.data(), zero3N.size());
         std::vector<double> zN(N, 0.0);
         h2d(h, h->sa, zN.data(), N);
-        h2d(h, h->row_ptr, row_ptr.data(), N + 1);
-        h2d(h, h->col, col.data(), nb);
-        h2d(h, h->blk_row, blk_row.data(), nb);
-        h2d(h, h->diag_blk, diag_blk.data(), N);
-        h2d(h, h->blk_ptr, blk_ptr.data(), nb + 1);
-        h2d(h, h->blk_src, blk_src.data(), blk_src.size());
         h2d(h, h->node_ptr, node_ptr.data(), N + 1);
         h2d(h, h->node_src, node_src.data(), node_src.size());
+        build_pattern(h);
         build_tiles(h, 256, 512);
         h2d(h, h->seg_node_ptr, h->h_node_seg_ptr.data(), S + 1);
         std::vector<int32_t> zS(S, 0);
@@ -672,6 +840,7 @@ static int solve_impl(tf_handle* h)
     if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     int st = krylov_solve(h);
     h->opts.krylov = saved.krylov;
+    launch_bind_du(h);  // slaves follow their masters (SC/NewtonRaphsonSolver.cpp:397-399)
     return st;
 }
 
@@ -830,6 +999,53 @@ int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats)
         stats->ms_total = ms_since(t0);
     }
     return st;
+    TF_GUARD_END(h)
+}
+
+int tf_set_bound_dofs(tf_handle* h, int32_t n_pairs, const int32_t* pairs)
+{
+    if (!h || n_pairs < 0 || (n_pairs > 0 && !pairs)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    const int nd = 3 * h->N;
+    std::vector<int32_t> role((size_t)nd, 0);  // 1 = slave, 2 = master
+    auto seg_of = [&](int node) {
+        return (int)(std::upper_bound(h->h_node_seg_ptr.begin(), h->h_node_seg_ptr.end(), node) -
+                     h->h_node_seg_ptr.begin()) - 1;
+    };
+    for (int k = 0; k < n_pairs; ++k) {
+        const int sd = pairs[2 * k], md = pairs[2 * k + 1];
+        const std::string tag = "bound pair " + std::to_string(k) + " (" + std::to_string(sd) + ", " +
+                                std::to_string(md) + "): ";
+        if (sd < 0 || sd >= nd || md < 0 || md >= nd) return fail(h, TF_INVALID, tag + "DoF out of range");
+        if (sd == md) return fail(h, TF_INVALID, tag + "a DoF cannot be bound to itself");
+        if (role[sd] == 1) return fail(h, TF_INVALID, tag + "the slave DoF is already bound");
+        if (role[sd] == 2 || role[md] == 1)
+            return fail(h, TF_INVALID, tag + "chained binding (a slave that is also a master) is not supported");
+        if (seg_of(sd / 3) != seg_of(md / 3)) return fail(h, TF_INVALID, tag + "slave and master in different tissues");
+        role[sd] = 1;
+        role[md] = 2;
+    }
+    h->h_bound.assign(pairs, pairs + 2 * (size_t)n_pairs);
+    build_pattern(h);
+    build_tiles(h, h->wg, h->tile_cap);
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_ablated_nodes(tf_handle* h, int32_t n, const int32_t* nodes)
+{
+    if (!h || n < 0 || (n > 0 && !nodes)) return TF_INVALID;
+    TF_GUARD_BEGIN
+    std::vector<uint8_t> flag(h->N, 0);
+    for (int k = 0; k < n; ++k) {
+        if (nodes[k] < 0 || nodes[k] >= h->N) return fail(h, TF_INVALID, "ablated node id out of range");
+        flag[nodes[k]] = 1;
+    }
+    if (!h->ablated) h->ablated = dev_alloc<uint8_t>(h->N);
+    h2d(h, h->ablated, flag.data(), h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (n == 0) { (void)hipFree(h->ablated); h->ablated = nullptr; }
+    return TF_OK;
     TF_GUARD_END(h)
 }
 

@@ -214,8 +214,9 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_
                                                 const int32_t* __restrict__ blk_row, const int32_t* __restrict__ col,
                                                 const double* __restrict__ kelem, const double* __restrict__ sa,
                                                 const double* __restrict__ eta_ext, const uint8_t* __restrict__ fixed,
-                                                double inv_dt, int nb, double* __restrict__ val,
-                                                float* __restrict__ val32, double* __restrict__ dinv)
+                                                const uint8_t* __restrict__ ablated, double inv_dt, int nb,
+                                                double* __restrict__ val, float* __restrict__ val32,
+                                                double* __restrict__ dinv)
 {
     const int blk = blockIdx.x * blockDim.x + threadIdx.x;
     if (blk >= nb) return;
@@ -231,6 +232,11 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_
         const double s = sa[A] * inv_dt;
 #pragma unroll
         for (int i = 0; i < 3; ++i) acc[4 * i] += s * eta_ext[3 * A + i];
+        // nodes without mass keep a solvable system (SC/NewtonRaphsonSolver.cpp:350-361)
+        if (ablated && ablated[A])
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                if (acc[4 * i] == 0.0) acc[4 * i] = 1.0;
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -249,6 +255,60 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_
 #pragma unroll
         for (int q = 0; q < 9; ++q) dinv[9 * A + q] = inv[q];
     }
+}
+
+// ---- slave/master binding (SC/NewtonRaphsonSolver.cpp:154-235) ----------------------------
+// Fold targets: val[t] = const + sum of val_base[src] over the target's sources.  Slaves are
+// never masters, so every target reads only unfolded values.
+__global__ __launch_bounds__(256) void k_bind_values(const int32_t* __restrict__ fold_ptr,
+                                                     const int64_t* __restrict__ fold_tgt,
+                                                     const int64_t* __restrict__ fold_src,
+                                                     const double* __restrict__ fold_const, int n,
+                                                     const double* __restrict__ val_base, double* __restrict__ val,
+                                                     float* __restrict__ val32)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double acc = fold_const[t];
+    for (int k = fold_ptr[t]; k < fold_ptr[t + 1]; ++k) acc += val_base[fold_src[k]];
+    val[fold_tgt[t]] = acc;
+    if (val32) val32[fold_tgt[t]] = (float)acc;
+}
+
+__global__ __launch_bounds__(64) void k_bind_dinv(const int32_t* __restrict__ nodes, int n,
+                                                  const int32_t* __restrict__ diag_blk, const double* __restrict__ val,
+                                                  int nb, double* __restrict__ dinv)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int A = nodes[k], j = diag_blk[A];
+    double m[9], inv[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = val[(size_t)q * nb + j];
+    tf::inv3(m, inv);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dinv[9 * A + q] = inv[q];
+}
+
+// g_master += g_slave (slaves in binding order), g_slave = 0
+__global__ __launch_bounds__(256) void k_bind_g(const int32_t* __restrict__ m_ptr, const int32_t* __restrict__ m_dof,
+                                                const int32_t* __restrict__ s_dof, int n, double* __restrict__ g)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double acc = g[m_dof[k]];
+    for (int j = m_ptr[k]; j < m_ptr[k + 1]; ++j) {
+        acc += g[s_dof[j]];
+        g[s_dof[j]] = 0.0;
+    }
+    g[m_dof[k]] = acc;
+}
+
+// dU_slave = dU_master after the solve
+__global__ __launch_bounds__(256) void k_bind_du(const int32_t* __restrict__ pairs, int n, double* __restrict__ du)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) du[pairs[2 * k]] = du[pairs[2 * k + 1]];
 }
 
 __global__ __launch_bounds__(256) void k_residual(const int32_t* __restrict__ node_ptr, const int32_t* __restrict__ node_src,
@@ -527,9 +587,10 @@ void launch_assemble_gather(tf_handle* h, double dt)
     hipEvent_t ev[2];
     timer_begin(h, h->t_gather, ev);
     hipLaunchKernelGGL(k_gather, dim3((h->nb + 255) / 256), dim3(256), 0, h->stream, h->blk_ptr, h->blk_src, h->blk_row,
-                       h->col, h->kelem, h->sa, h->eta_ext, h->fixed, 1.0 / dt, h->nb, h->val,
+                       h->col, h->kelem, h->sa, h->eta_ext, h->fixed, h->ablated, 1.0 / dt, h->nb, h->val,
                        h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr, h->dinv);
     TF_HIP(hipGetLastError());
+    if (h->n_bound) launch_bind_matrix(h);
     timer_end(h, h->t_gather, ev);
 }
 
@@ -537,6 +598,33 @@ void launch_residual(tf_handle* h, double dt)
 {
     hipLaunchKernelGGL(k_residual, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src, h->felem,
                        h->uk, h->un, h->sa, h->eta_ext, h->fixed, 1.0 / dt, h->N, h->g);
+    TF_HIP(hipGetLastError());
+    if (h->n_bound) launch_bind_residual(h);
+}
+
+void launch_bind_matrix(tf_handle* h)
+{
+    launch_copy(h, h->val_base, h->val, 9 * (int64_t)h->nb);
+    hipLaunchKernelGGL(k_bind_values, dim3((h->n_fold + 255) / 256), dim3(256), 0, h->stream, h->fold_ptr,
+                       h->fold_tgt, h->fold_src, h->fold_const, h->n_fold, h->val_base, h->val,
+                       h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr);
+    hipLaunchKernelGGL(k_bind_dinv, dim3((h->n_bind_nodes + 63) / 64), dim3(64), 0, h->stream, h->bind_nodes,
+                       h->n_bind_nodes, h->diag_blk, h->val, h->nb, h->dinv);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_bind_residual(tf_handle* h)
+{
+    hipLaunchKernelGGL(k_bind_g, dim3((h->n_masters + 255) / 256), dim3(256), 0, h->stream, h->bind_m_ptr,
+                       h->bind_m_dof, h->bind_s_dof, h->n_masters, h->g);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_bind_du(tf_handle* h)
+{
+    if (!h->n_bound) return;
+    hipLaunchKernelGGL(k_bind_du, dim3((h->n_bound + 255) / 256), dim3(256), 0, h->stream, h->bind_pairs, h->n_bound,
+                       h->du);
     TF_HIP(hipGetLastError());
 }
 

@@ -105,6 +105,20 @@ struct tf_handle {
     double *dinv = nullptr;       // [N][9]
     int32_t *blk_ptr = nullptr, *blk_src = nullptr;
     int32_t *node_ptr = nullptr, *node_src = nullptr;
+    std::vector<int32_t> h_node_ptr, h_node_src;
+
+    // slave/master DoF binding (empty unless tf_set_bound_dofs was called) and ablated nodes
+    std::vector<int32_t> h_bound;     // [2 * n_bound] (slave DoF, master DoF) in call order
+    int32_t n_bound = 0, n_fold = 0, n_bind_nodes = 0, n_masters = 0;
+    int32_t *fold_ptr = nullptr;      // [n_fold + 1] sources of each fold target
+    int64_t *fold_tgt = nullptr;      // [n_fold] q*nb + blk of each target entry
+    int64_t *fold_src = nullptr;      // q*nb + blk of each source entry (in val_base)
+    double *fold_const = nullptr;     // [n_fold] constant term (1 on a slave diagonal)
+    double *val_base = nullptr;       // [9][nb] M before the fold
+    int32_t *bind_nodes = nullptr;    // [n_bind_nodes] nodes whose diagonal block the fold changes
+    int32_t *bind_m_ptr = nullptr, *bind_m_dof = nullptr, *bind_s_dof = nullptr;  // masters -> slaves
+    int32_t *bind_pairs = nullptr;    // [2 * n_bound]
+    uint8_t *ablated = nullptr;       // [N] or null
 
     // tiles and segments
     int32_t *tile_r0 = nullptr, *tile_r1 = nullptr, *tile_seg = nullptr;
@@ -148,6 +162,10 @@ void launch_fill(tf_handle* h, double* dst, double v, int64_t n);
 void launch_seg_norm(tf_handle* h, const double* x);   // per-segment ||x||_2 into h->seg_norm
 void launch_update_uk(tf_handle* h);                    // uk += du on active segments
 void build_tiles(tf_handle* h, int wg, int tile_cap);  // (re)partitions rows into tiles (tf_api.hip)
+void build_pattern(tf_handle* h);  // BSR pattern, scatter lists and bound-DoF fold lists (tf_api.hip)
+void launch_bind_matrix(tf_handle* h);    // M <- N^T M N + I_slaves on the fold targets
+void launch_bind_residual(tf_handle* h);  // g <- N^T g
+void launch_bind_du(tf_handle* h);        // du[slave] <- du[master]
 // ---- Krylov (tf_krylov.hip) -----------------------------------------------------------------
 int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
 double krylov_time_cg1(tf_handle* h, int n_iters);  // ms per single-kernel CG iteration

@@ -125,6 +125,21 @@ class NewtonRaphsonSolver:
         c = None if eta is None else L.f64(eta)
         self._check(self._lib.tf_set_material(self.h, L.ptr(a), L.ptr(b), L.ptr(c)))
 
+    # ---- DoF constraints ------------------------------------------------------------------------
+    def set_bound_dofs(self, pairs):
+        """Bind slave DoFs to master DoFs: ``pairs`` is a sequence of (slave DoF, master DoF)
+        (``NewtonRaphsonSolver::slaveMasterList``); an empty sequence removes all binding."""
+        p = L.i32(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)) if len(pairs) else None
+        n = 0 if p is None else p.shape[0]
+        self._check(self._lib.tf_set_bound_dofs(self.h, n, L.ptr(p)))
+        nb = C.c_int32()
+        self._lib.tf_get_sizes(self.h, None, None, C.byref(nb), None)
+        self.nb = nb.value
+
+    def set_ablated_nodes(self, nodes):
+        a = L.i32(np.asarray(nodes, dtype=np.int32)) if len(nodes) else None
+        self._check(self._lib.tf_set_ablated_nodes(self.h, 0 if a is None else len(a), L.ptr(a)))
+
     # ---- per step -----------------------------------------------------------------------------
     def grow(self, dt) -> int:
         n = C.c_int32()
