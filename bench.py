@@ -11,6 +11,8 @@ vector updates and dot products fused in) is timed with HIP events on the stream
 around each replay of a captured chunk of iterations; its algorithmic bytes per launch are
 nb*(72+4) + (N+1)*4 (matrix) + 12*3N*8 (x, r, w, s, u, p read and written once) + N*72 (block
 inverses).
+A second leg runs the batched 256-disc sweep (config 4: one tissue segment per disc in one
+handle, autotuned tiling) and, for comparison, the same discs one handle at a time.
 The CPU baseline is the FP64 CPU oracle (same element math, OpenMP element loop, CSR assembly
 and an MKL PARDISO direct transposed solve) timed on the host cores over one full timestep.
 """
@@ -45,6 +47,8 @@ def parse():
     p.add_argument("--workload", default="config2", choices=["config1", "config2"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched 256-disc sweep")
+    p.add_argument("--batch-steps", type=int, default=3)
     p.add_argument("--krylov", type=int, default=-1)
     p.add_argument("--matrix-fp32", action="store_true")
     p.add_argument("--no-graph", action="store_true")
@@ -101,6 +105,68 @@ def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing, tune=Tru
     x = s.positions
     sec = time.perf_counter() - t0
     return s, stats, sec, x
+
+
+def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
+    """The 256-disc sweep (one tissue segment per disc in one handle) against the same discs run
+    one handle at a time, plus a bounded CPU-oracle sample scaled by prism count."""
+    from tissuefolding_paperversion_amd import configs
+
+    ts = configs.config4()
+    dt = 60.0
+    n_prisms = sum(t.n_elements for t in ts)
+    s = NewtonRaphsonSolver(ts)
+    for _ in range(warmup):
+        s.runOneStep(dt)
+    s.autotune(dt, n_iters=64)
+    s.positions
+    t0 = time.perf_counter()
+    stats = [s.runOneStep(dt) for _ in range(steps)]
+    s.positions
+    sec = time.perf_counter() - t0
+    seg_it = s.segment_stats()[0]
+    wg, cap, tiles = s.tiling()
+    s.close()
+    # the same discs, one handle each, one timed step after one untimed step
+    seq = 0.0
+    for t in ts:
+        h = NewtonRaphsonSolver(t)
+        h.runOneStep(dt)
+        h.positions
+        t1 = time.perf_counter()
+        h.runOneStep(dt)
+        h.positions
+        seq += time.perf_counter() - t1
+        h.close()
+    out = {"workload": f"config 4: {len(ts)} discs, {n_prisms} prisms, {sum(t.n_nodes for t in ts)} nodes "
+                       "(log-uniform 500-8000 prisms, 2-3 layers, per-disc E, nu, growth), dt = 60 s",
+           "unit": "disc-timesteps/s", "value": len(ts) * steps / sec, "steps": steps,
+           "ms_per_step": 1e3 * sec / steps, "prism_timesteps_per_s": n_prisms * steps / sec,
+           "nr_iterations_per_step_max_over_discs": float(np.mean([st["nr_iterations"] for st in stats])),
+           "nr_iterations_last_step_per_disc": {"min": int(seg_it.min()), "max": int(seg_it.max())},
+           "krylov_iterations_per_nr_iteration": float(sum(st["krylov_iterations"] for st in stats)
+                                                       / max(1, sum(st["nr_iterations"] for st in stats))),
+           "converged_steps": int(sum(st["converged"] for st in stats)),
+           "tiling": {"workgroup": wg, "tile_blocks": cap, "tiles": tiles},
+           "one_handle_per_disc": {"value": len(ts) / seq, "ms_per_step": 1e3 * seq,
+                                   "batched_speedup": (len(ts) * steps / sec) / (len(ts) / seq)}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the CPU oracle, timed as the baseline only
+
+        done, prisms, t2 = 0, 0, time.perf_counter()
+        for t in ts:
+            O.OracleTissue(t, solver="auto").timestep(dt)
+            done += 1
+            prisms += t.n_elements
+            if time.perf_counter() - t2 > 8.0:
+                break
+        csec = time.perf_counter() - t2
+        out["cpu_baseline"] = {"value": len(ts) / (csec * n_prisms / prisms), "unit": "disc-timesteps/s",
+                               "cores": int(O.lib().tfo_num_threads()), "kind": "port",
+                               "sample": f"one timestep of the first {done} discs ({prisms} prisms) in sequence, "
+                                         "scaled to the whole sweep by prism count"}
+    return out
 
 
 def main():
@@ -176,6 +242,8 @@ def main():
                            "speedup_vs_fp64": (a.steps / msec) / (a.steps / sec)}
             sm.close()
         out["mixed"] = mixed
+    if not a.no_batch:
+        out["batched"] = batched_leg(NewtonRaphsonSolver, a.batch_steps, 1, cpu=not a.no_cpu_baseline)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(configs.config2() if a.workload == "config2" else configs.config1(), dt)
     print(json.dumps(out), flush=True)

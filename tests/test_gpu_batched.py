@@ -1,0 +1,72 @@
+"""Batched sweep: many independent discs in one handle (one tissue segment each).
+
+Per-disc bar: the batched trajectory equals the disc's own single-handle run (to 1e-12
+relative; the row tiles never straddle two tissues and every reduction is per tissue, so the
+bits are in fact the same) with the same NR iteration count, and a sample of discs matches the
+CPU oracle within 1e-8 with the oracle's NR iteration counts.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from tissuefolding_paperversion_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(tissue, **opts):
+    from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
+    return NewtonRaphsonSolver(tissue, opts or None)
+
+
+def _small_sweep():
+    return configs.config4(n_discs=12, seed=9, min_prisms=100.0, max_prisms=1500.0)
+
+
+def test_batched_discs_equal_single_runs():
+    ts = _small_sweep()
+    batch = _solver(ts)
+    steps = 2
+    seg_iters = []
+    for _ in range(steps):
+        st = batch.runOneStep(60.0)
+        assert st["converged"] == 1
+        seg_iters.append(batch.segment_stats()[0].copy())
+    xb = batch.positions
+    off = 0
+    for k, t in enumerate(ts):
+        s = _solver(t)
+        for step in range(steps):
+            st = s.runOneStep(60.0)
+            assert st["converged"] == 1
+            assert st["nr_iterations"] == seg_iters[step][k]
+        xs = s.positions
+        assert rel_err(xb[off:off + 3 * t.n_nodes], xs) <= 1e-12
+        off += 3 * t.n_nodes
+
+
+def test_batched_sample_matches_oracle(oracle):
+    ts = _small_sweep()
+    batch = _solver(ts)
+    sample = [0, 5]
+    ors = [oracle.OracleTissue(ts[k]) for k in sample]
+    offs = np.cumsum([0] + [3 * t.n_nodes for t in ts])
+    for _ in range(2):
+        batch.runOneStep(60.0)
+        its = batch.segment_stats()[0]
+        xb = batch.positions
+        for k, o in zip(sample, ors):
+            it_o, _ = o.timestep(60.0)
+            assert its[k] == it_o
+            assert rel_err(xb[offs[k]:offs[k + 1]], o.pos) < 1e-8
+
+
+def test_batched_autotuned_tiling():
+    ts = _small_sweep()
+    a, b = _solver(ts), _solver(ts)
+    table = b.autotune(60.0, n_iters=16)
+    assert len(table) >= 6
+    for _ in range(2):
+        a.runOneStep(60.0)
+        assert b.runOneStep(60.0)["converged"] == 1
+    assert rel_err(b.positions, a.positions) < 1e-10

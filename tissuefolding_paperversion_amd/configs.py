@@ -156,3 +156,39 @@ def config1() -> Tissue:
 def config2(h: float = 3.66) -> Tissue:
     """Wing-disc-like ellipse 200 x 120 um, ~13k triangles x 3 layers (~40k prisms)."""
     return wing_disc_tissue(h=h, name="wing-disc ellipse 200x120um, 3 layers (~40k prisms)")
+
+
+def batch_disc(n_prisms_target: float, n_layers: int, E: float, nu: float, rate_per_hour: float, seed: int,
+               h: float = 3.0, layer_h: float = 4.0, name: str = "disc") -> Tissue:
+    """One disc of a batched sweep: radius chosen so that triangles x layers is close to
+    ``n_prisms_target`` (hex lattice of spacing h: a triangle covers sqrt(3)/4 h^2), boundary
+    conditions of :func:`disc_tissue`, and a radially graded in-plane growth rate
+    r(rho) = rate * (1.5 - (rho/R)^2), whose mean over the disc is ``rate`` and whose gradient
+    stresses the tissue (so that every step needs several Newton iterations)."""
+    n_tri = n_prisms_target / n_layers
+    radius = math.sqrt(n_tri * math.sqrt(3.0) / 4.0 * h * h / math.pi)
+    z = tuple(layer_h * k for k in range(n_layers + 1))
+    t = disc_tissue(radius=radius, h=h, z_levels=z, seed=seed, E=E, nu=nu, growth_per_hour=(0.0, 0.0, 0.0),
+                    name=name)
+    cen = t.mesh.xref.reshape(-1, 6, 3).mean(axis=1)
+    rho2 = (cen[:, 0] ** 2 + cen[:, 1] ** 2) / radius ** 2
+    r = rate_per_hour * 0.01 * (1.5 - rho2)
+    t.rates = per_hour(np.stack([r, r, np.zeros_like(r)], axis=1))
+    return t
+
+
+def config4(n_discs: int = 256, seed: int = 4, min_prisms: float = 500.0, max_prisms: float = 8000.0) -> list[Tissue]:
+    """Batched sweep: ``n_discs`` independent discs with prism counts log-uniform in
+    [min_prisms, max_prisms], 2-3 layers, E ~ U(500, 5000) Pa, nu ~ U(0.25, 0.45) and mean growth
+    rate ~ U(0.5, 3) %/hr (all seeded), dt = 60 s, 60 steps."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_discs):
+        target = math.exp(rng.uniform(math.log(min_prisms), math.log(max_prisms)))
+        layers = int(rng.integers(2, 4))
+        E = rng.uniform(500.0, 5000.0)
+        nu = rng.uniform(0.25, 0.45)
+        rate = rng.uniform(0.5, 3.0)
+        out.append(batch_disc(target, layers, E, nu, rate, seed=1000 * seed + k,
+                              name=f"disc {k}: {layers} layers, E={E:.0f} Pa, nu={nu:.3f}, {rate:.2f} %/hr"))
+    return out
