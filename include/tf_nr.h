@@ -166,6 +166,33 @@ int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats);
 /* Per-tissue results of the last NR step: iterations, last ||dU||, converged flag. */
 int tf_get_segment_stats(tf_handle* h, int32_t* nr_iters, double* du_norm, int32_t* converged);
 
+/* ---- linear algebra on the assembled Jacobian and the discrete adjoint ---------------------------
+ * No reference counterpart: the reference has no sensitivities.  They serve gradient-based
+ * fitting of growth rates and stiffnesses through the implicit timestep. */
+/* Fg and InvFg exactly as given (row-major 3x3 per element), e.g. to restore a checkpoint. */
+int tf_set_growth_state(tf_handle* h, const double* Fg, const double* InvFg);
+/* Growth-rotation angle about z applied to each element's increment by the last tf_grow
+ * (0 where the 1-degree rule did not fire). */
+int tf_get_growth_rotation(tf_handle* h, double* theta_z);
+/* y = M x (transpose = 0) or y = M^T x (transpose = 1) with the current FP64 Jacobian. */
+int tf_apply_jacobian(tf_handle* h, int32_t transpose, const double* x, double* y);
+/* Solves M x = b or M^T x = b with the Krylov solver and the current Jacobian (FP64 values). */
+int tf_solve_system(tf_handle* h, int32_t transpose, const double* b, double* x, int32_t* krylov_iters,
+                    double* rel_residual);
+/* Vector-Jacobian product of the residual g at the current state with y: y^T dg/dInvFg
+ * ([E][9]), y^T dg/dlambda and y^T dg/dmu ([E]). */
+int tf_adjoint_elements(tf_handle* h, const double* y, double* invfg_bar, double* lambda_bar, double* mu_bar);
+/* Zeroes the accumulated parameter adjoints and the growth-state adjoint. */
+int tf_adjoint_reset(tf_handle* h);
+/* One reverse timestep.  State on entry: positions = x_{n+1} (converged), un = x_n, surfaces at
+ * x_n, Fg/InvFg = Fg_{n+1}.  Inputs: xbar_next = dL/dx_{n+1}, fg_prev = Fg_n, theta_z = the
+ * step's growth-rotation angles (held fixed).  Output: xbar_prev = dL/dx_n.  Accumulates dL/drate
+ * ([E][3]), dL/dlambda, dL/dmu and carries dL/dFg to step n.  Needs eta_int = 0 and no bound or
+ * ablated DoFs (TF_INVALID otherwise). */
+int tf_adjoint_step(tf_handle* h, double dt, const double* xbar_next, const double* fg_prev, const double* theta_z,
+                    double* xbar_prev, int32_t* krylov_iters);
+int tf_adjoint_get(tf_handle* h, double* rate_bar, double* lambda_bar, double* mu_bar, double* fg_bar);
+
 /* ---- tuning (no reference counterpart: schedule choices of this implementation) ----------------- */
 /* Row tiling of the Krylov kernels: workgroup size wg in {256, 512, 1024} and tile capacity
  * wg * {1, 2, 4} <= 2048 blocks. */

@@ -26,7 +26,8 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_begin_step", "tf_assemble", "tf_get_residual", "tf_get_jacobian_bsr", "tf_element_dump", "tf_solve",
            "tf_update_uk", "tf_nr_step", "tf_timestep", "tf_get_segment_stats", "tf_kernel_timing",
            "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune", "tf_set_bound_dofs",
-           "tf_set_ablated_nodes"]
+           "tf_set_ablated_nodes", "tf_set_growth_state", "tf_get_growth_rotation", "tf_apply_jacobian",
+           "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get"]
 
 
 class TfError(RuntimeError):
@@ -107,6 +108,14 @@ def load(path: str | None = None):
         "tf_autotune": ([vp, dbl, i32, vp, vp, vp, vp], i32),
         "tf_set_bound_dofs": ([vp, i32, vp], i32),
         "tf_set_ablated_nodes": ([vp, i32, vp], i32),
+        "tf_set_growth_state": ([vp, vp, vp], i32),
+        "tf_get_growth_rotation": ([vp, vp], i32),
+        "tf_apply_jacobian": ([vp, i32, vp, vp], i32),
+        "tf_solve_system": ([vp, i32, vp, vp, vp, vp], i32),
+        "tf_adjoint_elements": ([vp, vp, vp, vp, vp], i32),
+        "tf_adjoint_reset": ([vp], i32),
+        "tf_adjoint_step": ([vp, dbl, vp, vp, vp, vp, vp], i32),
+        "tf_adjoint_get": ([vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -192,3 +192,16 @@ def config4(n_discs: int = 256, seed: int = 4, min_prisms: float = 500.0, max_pr
         out.append(batch_disc(target, layers, E, nu, rate, seed=1000 * seed + k,
                               name=f"disc {k}: {layers} layers, E={E:.0f} Pa, nu={nu:.3f}, {rate:.2f} %/hr"))
     return out
+
+
+def config5(seed_target: int = 5, n_steps: int = 30, checkpoint_every: int = 5, h: float = 3.66):
+    """Inverse-growth workload: the config-2 ellipse over ``n_steps`` steps; the target is a
+    forward run with every element's growth rate scaled by (1 + 0.05 N(0,1)) (seeded); the loss
+    is 1/2 sum over apical nodes of |x - x*|^2; parameters are the per-element rate scales
+    (nominal 1) and the three per-layer Young's moduli.  Returns (tissue, target_scale,
+    n_steps, checkpoint_every)."""
+    t = config2(h=h)
+    t.n_steps = n_steps
+    rng = np.random.default_rng(seed_target)
+    scale = 1.0 + 0.05 * rng.standard_normal(t.n_elements)
+    return t, scale, n_steps, checkpoint_every

@@ -77,7 +77,9 @@ void free_all(tf_handle* h)
                     h->seg_node_ptr, h->seg_counter, h->partial, h->seg, h->seg_norm, h->kr, h->kr0, h->kp, h->kph,
                     h->kv, h->ks, h->ksh, h->kt, h->kx, h->fold_ptr, h->fold_tgt, h->fold_src, h->fold_const,
                     h->val_base, h->bind_nodes, h->bind_m_ptr, h->bind_m_dof, h->bind_s_dof, h->bind_pairs,
-                    h->ablated};
+                    h->ablated, h->blk_tr, h->rot_tz, h->val_t, h->g_save, h->adj_xbar, h->adj_rhs, h->adj_pass,
+                    h->adj_y, h->adj_un_bar, h->adj_invfg_bar, h->adj_fg_bar, h->adj_fg_prev, h->adj_rate_bar,
+                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
@@ -293,6 +295,12 @@ void build_pattern(tf_handle* h)
         std::vector<int32_t> fill(blk_ptr.begin(), blk_ptr.end() - 1);
         for (size_t k = 0; k < (size_t)36 * E; ++k) blk_src[fill[eblk[k]]++] = (int32_t)k;
     }
+    // the pattern is structurally symmetric: block (A, B) has a partner (B, A)
+    std::vector<int32_t> blk_tr(nb);
+    for (int j = 0; j < nb; ++j) {
+        blk_tr[j] = find_blk(col[j], blk_row[j]);
+        if (blk_tr[j] < 0) throw std::runtime_error("Jacobian pattern is not structurally symmetric");
+    }
     // ---- fold lists ----
     std::vector<int32_t> fold_ptr{0}, bind_nodes, m_ptr{0}, m_dof, s_dof, pairs_dev;
     std::vector<int64_t> fold_tgt, fold_src;
@@ -361,12 +369,12 @@ void build_pattern(tf_handle* h)
         pairs_dev = h->h_bound;
     }
     // ---- device arrays ----
-    for (int32_t** p : {&h->row_ptr, &h->col, &h->blk_row, &h->diag_blk, &h->blk_ptr, &h->blk_src, &h->fold_ptr,
+    for (int32_t** p : {&h->row_ptr, &h->col, &h->blk_row, &h->diag_blk, &h->blk_ptr, &h->blk_src, &h->blk_tr, &h->fold_ptr,
                         &h->bind_nodes, &h->bind_m_ptr, &h->bind_m_dof, &h->bind_s_dof, &h->bind_pairs})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (int64_t** p : {&h->fold_tgt, &h->fold_src})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&h->val, &h->val_base, &h->fold_const})
+    for (double** p : {&h->val, &h->val_base, &h->fold_const, &h->val_t})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (h->val32) { (void)hipFree(h->val32); h->val32 = nullptr; }
     h->nb = nb;
@@ -378,6 +386,8 @@ void build_pattern(tf_handle* h)
     h->val32 = dev_alloc<float>((size_t)9 * nb);
     h->blk_ptr = dev_alloc<int32_t>(nb + 1);
     h->blk_src = dev_alloc<int32_t>((size_t)36 * E);
+    h->blk_tr = dev_alloc<int32_t>(nb);
+    h2d(h, h->blk_tr, blk_tr.data(), nb);
     h2d(h, h->row_ptr, row_ptr.data(), N + 1);
     h2d(h, h->col, col.data(), nb);
     h2d(h, h->blk_row, blk_row.data(), nb);
@@ -513,6 +523,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->fixed = dev_alloc<uint8_t>((size_t)3 * N);
         h->sa = dev_alloc<double>(N);
         h->face_area = dev_alloc<double>((size_t)2 * E);
+        h->rot_tz = dev_alloc<double>(E);
         h->kelem = dev_alloc<double>((size_t)324 * E);
         h->felem = dev_alloc<double>((size_t)18 * E);
         h->status = dev_alloc<int32_t>(4);
@@ -564,6 +575,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h2d(h, h->du, zero3N.data(), zero3N.size());
         std::vector<double> zN(N, 0.0);
         h2d(h, h->sa, zN.data(), N);
+        h2d(h, h->rot_tz, zeroE.data(), E);
         h2d(h, h->node_ptr, node_ptr.data(), N + 1);
         h2d(h, h->node_src, node_src.data(), node_src.size());
         build_pattern(h);
@@ -1045,6 +1057,211 @@ int tf_set_ablated_nodes(tf_handle* h, int32_t n, const int32_t* nodes)
     h2d(h, h->ablated, flag.data(), h->N);
     TF_HIP(hipStreamSynchronize(h->stream));
     if (n == 0) { (void)hipFree(h->ablated); h->ablated = nullptr; }
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+// ---------------------------------------------------------------------------------------------
+// discrete adjoint and general linear algebra on the assembled Jacobian
+static void ensure_adjoint(tf_handle* h)
+{
+    const size_t n3 = 3 * (size_t)h->N, E = h->E;
+    for (double** p : {&h->g_save, &h->adj_xbar, &h->adj_rhs, &h->adj_pass, &h->adj_y, &h->adj_un_bar})
+        if (!*p) *p = dev_alloc<double>(n3);
+    for (double** p : {&h->adj_invfg_bar, &h->adj_fg_bar, &h->adj_fg_prev})
+        if (!*p) *p = dev_alloc<double>(9 * E);
+    if (!h->adj_rate_bar) h->adj_rate_bar = dev_alloc<double>(3 * E);
+    for (double** p : {&h->adj_lam_bar, &h->adj_mu_bar, &h->adj_lam_acc, &h->adj_mu_acc, &h->adj_tz})
+        if (!*p) *p = dev_alloc<double>(E);
+}
+
+static void drop_graph(tf_handle* h)
+{
+    if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+}
+
+// M x = b or M^T x = b for device vectors, with the current Jacobian; FP64 matrix values.
+static int solve_system_impl(tf_handle* h, int transpose, const double* d_b, double* d_x)
+{
+    ensure_adjoint(h);
+    const int64_t n3 = 3 * (int64_t)h->N;
+    launch_copy(h, h->g_save, h->g, n3);
+    launch_copy(h, h->g, d_b, n3);
+    hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    const tf_solver_opts saved = h->opts;
+    h->opts.matrix_precision = TF_PREC_FP64;
+    if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+    if (transpose) {
+        if (!h->val_t) h->val_t = dev_alloc<double>((size_t)9 * h->nb);
+        launch_transpose_vals(h, h->val_t);
+        std::swap(h->val, h->val_t);
+        launch_dinv_all(h);
+        drop_graph(h);
+    }
+    int st = TF_OK;
+    try {
+        st = krylov_solve(h);
+    } catch (...) {
+        if (transpose) { std::swap(h->val, h->val_t); drop_graph(h); }
+        h->opts = saved;
+        throw;
+    }
+    if (transpose) {
+        std::swap(h->val, h->val_t);
+        launch_dinv_all(h);
+        drop_graph(h);
+    }
+    h->opts = saved;
+    launch_copy(h, d_x, h->du, n3);
+    launch_copy(h, h->g, h->g_save, n3);
+    return st;
+}
+
+int tf_set_growth_state(tf_handle* h, const double* Fg, const double* InvFg)
+{
+    if (!h || !Fg || !InvFg) return TF_INVALID;
+    TF_GUARD_BEGIN
+    std::vector<double> a = to_soa(Fg, h->E, 9), b = to_soa(InvFg, h->E, 9);
+    h2d(h, h->fg, a.data(), a.size());
+    h2d(h, h->invfg, b.data(), b.size());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_growth_rotation(tf_handle* h, double* theta_z)
+{
+    if (!h || !theta_z) return TF_INVALID;
+    TF_GUARD_BEGIN
+    d2h(h, theta_z, h->rot_tz, h->E);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_apply_jacobian(tf_handle* h, int32_t transpose, const double* x, double* y)
+{
+    if (!h || !x || !y) return TF_INVALID;
+    TF_GUARD_BEGIN
+    ensure_adjoint(h);
+    const size_t n3 = 3 * (size_t)h->N;
+    h2d(h, h->adj_xbar, x, n3);
+    if (transpose) launch_spmv_t(h, h->adj_xbar, h->adj_y);
+    else {
+        const int mp = h->opts.matrix_precision;
+        h->opts.matrix_precision = TF_PREC_FP64;
+        launch_spmv(h, h->adj_xbar, h->adj_y);
+        h->opts.matrix_precision = mp;
+    }
+    d2h(h, y, h->adj_y, n3);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_solve_system(tf_handle* h, int32_t transpose, const double* b, double* x, int32_t* krylov_iters,
+                    double* rel_residual)
+{
+    if (!h || !b || !x) return TF_INVALID;
+    TF_GUARD_BEGIN
+    ensure_adjoint(h);
+    const size_t n3 = 3 * (size_t)h->N;
+    h2d(h, h->adj_rhs, b, n3);
+    const int st = solve_system_impl(h, transpose, h->adj_rhs, h->adj_y);
+    d2h(h, x, h->adj_y, n3);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (krylov_iters) *krylov_iters = h->last_krylov_iters;
+    if (rel_residual) {
+        double worst = 0.0;
+        for (int s = 0; s < h->S; ++s)
+            if (h->h_seg[s].bnorm2 > 0) worst = std::max(worst, std::sqrt(h->h_seg[s].rnorm2 / h->h_seg[s].bnorm2));
+        *rel_residual = worst;
+    }
+    if (st != TF_OK) return fail(h, st, "Krylov solver did not reach its tolerance");
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+static int adjoint_supported(tf_handle* h)
+{
+    if (h->any_viscous) return fail(h, TF_INVALID, "the adjoint supports eta_int = 0 only");
+    if (h->n_bound) return fail(h, TF_INVALID, "the adjoint does not support bound DoFs");
+    if (h->ablated) return fail(h, TF_INVALID, "the adjoint does not support ablated nodes");
+    return TF_OK;
+}
+
+int tf_adjoint_elements(tf_handle* h, const double* y, double* invfg_bar, double* lambda_bar, double* mu_bar)
+{
+    if (!h || !y) return TF_INVALID;
+    TF_GUARD_BEGIN
+    ensure_adjoint(h);
+    h2d(h, h->adj_y, y, 3 * (size_t)h->N);
+    launch_adj_elements(h, h->adj_y, h->adj_invfg_bar, h->adj_lam_bar, h->adj_mu_bar);
+    if (invfg_bar) d2h(h, invfg_bar, h->adj_invfg_bar, 9 * (size_t)h->E);
+    if (lambda_bar) d2h(h, lambda_bar, h->adj_lam_bar, h->E);
+    if (mu_bar) d2h(h, mu_bar, h->adj_mu_bar, h->E);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_adjoint_reset(tf_handle* h)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    ensure_adjoint(h);
+    launch_fill(h, h->adj_fg_bar, 0.0, 9 * (int64_t)h->E);
+    launch_fill(h, h->adj_rate_bar, 0.0, 3 * (int64_t)h->E);
+    launch_fill(h, h->adj_lam_acc, 0.0, h->E);
+    launch_fill(h, h->adj_mu_acc, 0.0, h->E);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_adjoint_step(tf_handle* h, double dt, const double* xbar_next, const double* fg_prev, const double* theta_z,
+                    double* xbar_prev, int32_t* krylov_iters)
+{
+    if (!h || !(dt > 0) || !xbar_next || !fg_prev || !theta_z || !xbar_prev) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (adjoint_supported(h) != TF_OK) return TF_INVALID;
+    ensure_adjoint(h);
+    const size_t n3 = 3 * (size_t)h->N;
+    h2d(h, h->adj_xbar, xbar_next, n3);
+    h2d(h, h->adj_fg_prev, fg_prev, 9 * (size_t)h->E);
+    h2d(h, h->adj_tz, theta_z, h->E);
+    // the Jacobian at the converged state of the step
+    assemble_impl(h, dt);
+    int32_t st4[4];
+    d2h(h, st4, h->status, 4);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (st4[0] > 0) return fail(h, TF_NEG_JACOBIAN, std::to_string(st4[0]) + " Gauss points with det Fe <= 0");
+    launch_adj_split(h, h->adj_xbar, h->adj_rhs, h->adj_pass);
+    const int st = solve_system_impl(h, 1, h->adj_rhs, h->adj_y);
+    launch_adj_elements(h, h->adj_y, h->adj_invfg_bar, h->adj_lam_bar, h->adj_mu_bar);
+    launch_accumulate(h, h->adj_lam_bar, h->adj_lam_acc, h->E);
+    launch_accumulate(h, h->adj_mu_bar, h->adj_mu_acc, h->E);
+    launch_adj_un(h, dt, h->adj_y, h->adj_un_bar);
+    launch_accumulate(h, h->adj_pass, h->adj_un_bar, (int64_t)n3);
+    launch_adj_growth(h, dt, h->adj_tz, h->adj_fg_prev, h->adj_invfg_bar, h->adj_fg_bar, h->adj_rate_bar);
+    d2h(h, xbar_prev, h->adj_un_bar, n3);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    if (krylov_iters) *krylov_iters = h->last_krylov_iters;
+    if (st != TF_OK) return fail(h, st, "adjoint Krylov solve did not reach its tolerance");
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_adjoint_get(tf_handle* h, double* rate_bar, double* lambda_bar, double* mu_bar, double* fg_bar)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    ensure_adjoint(h);
+    if (rate_bar) d2h(h, rate_bar, h->adj_rate_bar, 3 * (size_t)h->E);
+    if (lambda_bar) d2h(h, lambda_bar, h->adj_lam_acc, h->E);
+    if (mu_bar) d2h(h, mu_bar, h->adj_mu_acc, h->E);
+    if (fg_bar) d2h(h, fg_bar, h->adj_fg_bar, 9 * (size_t)h->E);
+    TF_HIP(hipStreamSynchronize(h->stream));
     return TF_OK;
     TF_GUARD_END(h)
 }

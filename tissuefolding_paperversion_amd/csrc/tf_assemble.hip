@@ -420,10 +420,12 @@ __device__ inline void polar_rotation(const double* F, double* R)
 __global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, const double* __restrict__ xref,
                                               const double* __restrict__ pos, const double* __restrict__ rate,
                                               double dt, int allow_rotation, int E, double* __restrict__ fg,
-                                              double* __restrict__ invfg, int32_t* __restrict__ status)
+                                              double* __restrict__ invfg, double* __restrict__ rot_tz,
+                                              int32_t* __restrict__ status)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
+    double applied_tz = 0.0;
     double inc[9] = {exp(rate[e] * dt), 0, 0, 0, exp(rate[E + e] * dt), 0, 0, 0, exp(rate[2 * E + e] * dt)};
     if (allow_rotation) {
         // mean F at the current positions (ShapeBase.cpp:1676-1689)
@@ -467,11 +469,13 @@ __global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, 
                 double T[9];
                 tf::mm3(Rz, inc, T);
                 tf::mm3(T, RzT, inc);
+                applied_tz = tz;
                 atomicAdd(&status[1], 1);
             }
         }
     }
     double F0[9], F1[9], Fi[9];
+    rot_tz[e] = applied_tz;
     for (int q = 0; q < 9; ++q) F0[q] = fg[q * E + e];
     tf::mm3(inc, F0, F1);
     tf::inv3(F1, Fi);
@@ -638,7 +642,7 @@ void launch_element_dump(tf_handle* h, int32_t e, double dt, double* d_out)
 void launch_grow(tf_handle* h, double dt)
 {
     hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->xref, h->uk, h->rate, dt,
-                       h->opts.growth_rotation, h->E, h->fg, h->invfg, h->status);
+                       h->opts.growth_rotation, h->E, h->fg, h->invfg, h->rot_tz, h->status);
     TF_HIP(hipGetLastError());
 }
 

@@ -119,6 +119,16 @@ struct tf_handle {
     int32_t *bind_m_ptr = nullptr, *bind_m_dof = nullptr, *bind_s_dof = nullptr;  // masters -> slaves
     int32_t *bind_pairs = nullptr;    // [2 * n_bound]
     uint8_t *ablated = nullptr;       // [N] or null
+    int32_t *blk_tr = nullptr;        // [nb] index of block (B, A) for block (A, B)
+
+    // growth bookkeeping and discrete-adjoint work arrays (allocated on first use)
+    double *rot_tz = nullptr;         // [E] growth-rotation angle applied by the last growth step (0: none)
+    double *val_t = nullptr;          // [9][nb] transposed Jacobian values (transposed solves)
+    double *g_save = nullptr;         // [3N]
+    double *adj_xbar = nullptr, *adj_rhs = nullptr, *adj_pass = nullptr, *adj_y = nullptr, *adj_un_bar = nullptr;  // [3N]
+    double *adj_invfg_bar = nullptr, *adj_fg_bar = nullptr, *adj_fg_prev = nullptr;  // [E][9]
+    double *adj_rate_bar = nullptr;   // [E][3]
+    double *adj_lam_bar = nullptr, *adj_mu_bar = nullptr, *adj_lam_acc = nullptr, *adj_mu_acc = nullptr, *adj_tz = nullptr;  // [E]
 
     // tiles and segments
     int32_t *tile_r0 = nullptr, *tile_r1 = nullptr, *tile_seg = nullptr;
@@ -166,6 +176,16 @@ void build_pattern(tf_handle* h);  // BSR pattern, scatter lists and bound-DoF f
 void launch_bind_matrix(tf_handle* h);    // M <- N^T M N + I_slaves on the fold targets
 void launch_bind_residual(tf_handle* h);  // g <- N^T g
 void launch_bind_du(tf_handle* h);        // du[slave] <- du[master]
+// ---- discrete adjoint (tf_adjoint.hip) -----------------------------------------------------
+void launch_adj_elements(tf_handle* h, const double* y, double* invfg_bar, double* lam_bar, double* mu_bar);
+void launch_adj_un(tf_handle* h, double dt, const double* y, double* un_bar);
+void launch_adj_growth(tf_handle* h, double dt, const double* theta_z, const double* fg_prev, const double* invfg_bar,
+                       double* fg_bar, double* rate_bar);
+void launch_spmv_t(tf_handle* h, const double* x, double* y);
+void launch_transpose_vals(tf_handle* h, double* val_t);
+void launch_dinv_all(tf_handle* h);
+void launch_adj_split(tf_handle* h, const double* xbar, double* rhs, double* pass);
+void launch_accumulate(tf_handle* h, const double* a, double* b, int64_t n);
 // ---- Krylov (tf_krylov.hip) -----------------------------------------------------------------
 int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
 double krylov_time_cg1(tf_handle* h, int n_iters);  // ms per single-kernel CG iteration

@@ -225,6 +225,54 @@ class NewtonRaphsonSolver:
         self._check(self._lib.tf_get_segment_stats(self.h, L.ptr(it), L.ptr(dn), L.ptr(cv)))
         return it, dn, cv
 
+    # ---- linear algebra and the discrete adjoint -----------------------------------------------------
+    def set_growth_state(self, Fg, InvFg):
+        a, b = L.f64(Fg).ravel(), L.f64(InvFg).ravel()
+        self._check(self._lib.tf_set_growth_state(self.h, L.ptr(a), L.ptr(b)))
+
+    def growth_rotation(self):
+        tz = np.zeros(self.E)
+        self._check(self._lib.tf_get_growth_rotation(self.h, L.ptr(tz)))
+        return tz
+
+    def apply_jacobian(self, x, transpose=False):
+        x = L.f64(x).ravel()
+        y = np.zeros(3 * self.N)
+        self._check(self._lib.tf_apply_jacobian(self.h, int(bool(transpose)), L.ptr(x), L.ptr(y)))
+        return y
+
+    def solve_system(self, b, transpose=False):
+        b = L.f64(b).ravel()
+        x = np.zeros(3 * self.N)
+        it, rr = C.c_int32(), C.c_double()
+        self._check(self._lib.tf_solve_system(self.h, int(bool(transpose)), L.ptr(b), L.ptr(x), C.byref(it), C.byref(rr)))
+        self.last_krylov = dict(iterations=it.value, rel_residual=rr.value, status=L.TF_OK)
+        return x
+
+    def adjoint_elements(self, y):
+        y = L.f64(y).ravel()
+        fi, lb, mb = np.zeros(9 * self.E), np.zeros(self.E), np.zeros(self.E)
+        self._check(self._lib.tf_adjoint_elements(self.h, L.ptr(y), L.ptr(fi), L.ptr(lb), L.ptr(mb)))
+        return fi.reshape(-1, 3, 3), lb, mb
+
+    def adjoint_reset(self):
+        self._check(self._lib.tf_adjoint_reset(self.h))
+
+    def adjoint_step(self, dt, xbar_next, fg_prev, theta_z):
+        xb = L.f64(xbar_next).ravel()
+        fp = L.f64(fg_prev).ravel()
+        tz = L.f64(theta_z).ravel()
+        out = np.zeros(3 * self.N)
+        it = C.c_int32()
+        self._check(self._lib.tf_adjoint_step(self.h, dt, L.ptr(xb), L.ptr(fp), L.ptr(tz), L.ptr(out), C.byref(it)))
+        self.last_krylov = dict(iterations=it.value, status=L.TF_OK)
+        return out
+
+    def adjoint_get(self):
+        rb, lb, mb, fb = np.zeros(3 * self.E), np.zeros(self.E), np.zeros(self.E), np.zeros(9 * self.E)
+        self._check(self._lib.tf_adjoint_get(self.h, L.ptr(rb), L.ptr(lb), L.ptr(mb), L.ptr(fb)))
+        return rb.reshape(-1, 3), lb, mb, fb.reshape(-1, 3, 3)
+
     # ---- tuning -----------------------------------------------------------------------------------
     def set_tiling(self, wg, tile_cap):
         self._check(self._lib.tf_set_tiling(self.h, int(wg), int(tile_cap)))
