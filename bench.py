@@ -13,6 +13,9 @@ nb*(72+4) + (N+1)*4 (matrix) + 12*3N*8 (x, r, w, s, u, p read and written once) 
 inverses).
 A second leg runs the batched 256-disc sweep (config 4: one tissue segment per disc in one
 handle, autotuned tiling) and, for comparison, the same discs one handle at a time.
+A third leg times the discrete adjoint on the inverse-growth workload (config 5: 30 steps of
+the same ellipse, gradient of an apical-position loss with respect to 38,826 per-element rate
+scales and 3 layer moduli) against its forward run, with a central-difference check.
 The CPU baseline is the FP64 CPU oracle (same element math, OpenMP element loop, CSR assembly
 and an MKL PARDISO direct transposed solve) timed on the host cores over one full timestep.
 """
@@ -49,6 +52,8 @@ def parse():
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     p.add_argument("--no-batch", action="store_true", help="skip the batched 256-disc sweep")
     p.add_argument("--batch-steps", type=int, default=3)
+    p.add_argument("--no-adjoint", action="store_true", help="skip the config-5 adjoint leg")
+    p.add_argument("--adjoint-fd", type=int, default=7, help="parameters checked by central differences")
     p.add_argument("--krylov", type=int, default=-1)
     p.add_argument("--matrix-fp32", action="store_true")
     p.add_argument("--no-graph", action="store_true")
@@ -242,6 +247,11 @@ def main():
                            "speedup_vs_fp64": (a.steps / msec) / (a.steps / sec)}
             sm.close()
         out["mixed"] = mixed
+    if not a.no_adjoint:
+        from tissuefolding_paperversion_amd.adjoint import config5_check
+        adj = config5_check(n_fd=a.adjoint_fd)
+        adj.pop("rows")
+        out["adjoint"] = adj
     if not a.no_batch:
         out["batched"] = batched_leg(NewtonRaphsonSolver, a.batch_steps, 1, cpu=not a.no_cpu_baseline)
     if not a.no_cpu_baseline:

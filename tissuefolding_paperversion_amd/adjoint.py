@@ -150,3 +150,75 @@ def apical_loss(x, target, apical_dofs):
     d = np.zeros_like(x)
     d[apical_dofs] = x[apical_dofs] - target[apical_dofs]
     return 0.5 * float(np.dot(d, d)), d
+
+
+def config5_check(n_fd: int = 64, seed: int = 6, h: float = 1e-6, n_steps: int | None = None, opts: dict | None = None,
+                  log=None) -> dict:
+    """The inverse-growth workload (config 5): target run, forward and backward timings, and
+    central differences (h relative) of the loss on ``n_fd`` parameters -- the three layer
+    moduli plus ``n_fd - 3`` seeded random rate scales.  Errors are reported per parameter and
+    relative to the largest gradient entry."""
+    import time
+
+    from . import configs
+    from .mesh import APICAL
+
+    t, s_target, T, K = configs.config5()
+    if n_steps:
+        T = n_steps
+    ro = GrowthRollout(t, T, checkpoint_every=K, opts=opts)
+    E0 = ro.layer_moduli()
+    target = ro.forward(s_target, E0)
+    apical = np.where(np.repeat(t.mesh.node_placement == APICAL, 3))[0]
+    s0 = np.ones(t.n_elements)
+    t0 = time.perf_counter()
+    x = ro.forward(s0, E0)
+    t_fwd = time.perf_counter() - t0
+    L0, dLdx = apical_loss(x, target, apical)
+    t0 = time.perf_counter()
+    s_bar, E_bar = ro.backward(dLdx)
+    t_bwd = time.perf_counter() - t0
+    gmax = max(np.max(np.abs(s_bar)), np.max(np.abs(E_bar * E0)))
+    idx = np.random.default_rng(seed).choice(t.n_elements, max(0, n_fd - 3), replace=False)
+    rows = []
+    for k in range(3 if n_fd >= 3 else n_fd):
+        Ep, Em = E0.copy(), E0.copy()
+        Ep[k] *= 1 + h
+        Em[k] *= 1 - h
+        fd = (apical_loss(ro.forward(s0, Ep), target, apical)[0] - apical_loss(ro.forward(s0, Em), target, apical)[0]) / (2 * h)
+        adj = E_bar[k] * E0[k]   # d L / d ln E_k, on the scale of the rate-scale gradients
+        rows.append(("E_layer", k, adj, fd))
+        if log:
+            log(rows[-1])
+    for e in idx:
+        sp, sm = s0.copy(), s0.copy()
+        sp[e] += h
+        sm[e] -= h
+        fd = (apical_loss(ro.forward(sp, E0), target, apical)[0] - apical_loss(ro.forward(sm, E0), target, apical)[0]) / (2 * h)
+        rows.append(("rate_scale", int(e), s_bar[e], fd))
+        if log:
+            log(rows[-1])
+    err_rel_max = max(abs(a - f) / gmax for _, _, a, f in rows) if rows else 0.0
+    per = [abs(a - f) / max(abs(f), 1e-300) for _, _, a, f in rows]
+    return {"workload": f"config 5: {t.name}, {t.n_elements} prisms, {T} steps, checkpoint every {K}",
+            "loss": L0, "forward_s": t_fwd, "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd,
+            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in ro.stats])),
+            "adjoint_krylov_per_step": float(np.mean(ro.adjoint_krylov)),
+            "rotated_elements": int(sum(st["rotated_elements"] for st in ro.stats)),
+            "fd_parameters": len(rows), "fd_h": h,
+            "fd_err_max_rel_to_max_grad": err_rel_max,
+            "fd_err_per_param_median": float(np.median(per)) if per else 0.0,
+            "fd_err_per_param_max": float(np.max(per)) if per else 0.0,
+            "rows": [(k, int(i), float(a), float(f)) for k, i, a, f in rows]}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    p = argparse.ArgumentParser(description="config-5 gradient check (GPU)")
+    p.add_argument("--n-fd", type=int, default=64)
+    p.add_argument("--seed", type=int, default=6)
+    a = p.parse_args()
+    out = config5_check(a.n_fd, a.seed, log=lambda r: print(json.dumps(r), flush=True))
+    print(json.dumps(out), flush=True)
