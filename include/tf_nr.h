@@ -75,6 +75,9 @@ typedef struct tf_solver_opts {
     int32_t matrix_precision;  /* TF_PREC_FP64 or TF_PREC_FP32 storage of the Jacobian values  */
     int32_t growth_rotation;   /* 1: rotate growth by the element's mean rotation (1 deg rule) */
     int32_t use_graph;         /* 1: replay Krylov iteration chunks from a captured hipGraph   */
+    int32_t krylov_forcing;    /* 1: inside the NR loop each linear solve stops at the accuracy
+                                  the next Newton update needs (inexact Newton, per tissue);
+                                  0: every solve to krylov_rtol                                 */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

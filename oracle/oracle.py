@@ -155,6 +155,14 @@ def scalar_csr_pattern(n_nodes, conn):
     return row_ptr, col
 
 
+def _pardiso_safe():
+    """MKL PARDISO loaded next to PyTorch's bundled MKL fails (error -2/-3: the two MKL builds
+    resolve each other's internal symbols), so the automatic choice avoids it once torch is in
+    the process."""
+    import sys
+    return "torch" not in sys.modules
+
+
 class OracleTissue:
     """NR state of one tissue on the CPU, in the reference's order and orientation."""
 
@@ -292,6 +300,8 @@ class OracleTissue:
                 raise RuntimeError("singular Jacobian in dense oracle solve")
             return x
         kind = self.solver
+        if kind == "auto" and not _pardiso_safe():
+            kind = "superlu"
         if kind in ("auto", "pardiso") and lib().tfo_pardiso_available(None):
             Kc = K.tocsr()
             Kc.sort_indices()
@@ -309,6 +319,8 @@ class OracleTissue:
     def solver_kind(self):
         if 3 * self.N <= 3000:
             return "dense-lu"
+        if self.solver == "auto" and not _pardiso_safe():
+            return "superlu"
         if self.solver in ("auto", "pardiso") and lib().tfo_pardiso_available(None):
             return "mkl-pardiso"
         return "superlu"

@@ -121,28 +121,41 @@ class GrowthRollout:
         return s_bar, E_bar
 
 
-try:
-    import torch
+_FN = None
 
-    class TissueRolloutFn(torch.autograd.Function):
-        """x_T = rollout(s, E_layer), differentiable in both arguments."""
 
-        @staticmethod
-        def forward(ctx, s, E_layer, rollout: GrowthRollout):
-            x = rollout.forward(s.detach().cpu().double().numpy(), E_layer.detach().cpu().double().numpy())
-            ctx.rollout = rollout
-            ctx.meta = (s.dtype, s.device, E_layer.dtype, E_layer.device)
-            return torch.from_numpy(x.copy()).to(dtype=s.dtype, device=s.device)
+def _rollout_fn():
+    """The torch.autograd.Function, built on first use so that importing this module does not
+    load torch."""
+    global _FN
+    if _FN is None:
+        import torch
 
-        @staticmethod
-        def backward(ctx, xbar):
-            s_bar, E_bar = ctx.rollout.backward(xbar.detach().cpu().double().numpy())
-            sd, sdev, ed, edev = ctx.meta
-            return (torch.from_numpy(s_bar).to(dtype=sd, device=sdev),
-                    torch.from_numpy(E_bar).to(dtype=ed, device=edev), None)
+        class TissueRolloutFn(torch.autograd.Function):
+            """x_T = rollout(s, E_layer), differentiable in both arguments."""
 
-except ImportError:  # torch is optional for the numpy interface
-    TissueRolloutFn = None
+            @staticmethod
+            def forward(ctx, s, E_layer, rollout: GrowthRollout):
+                x = rollout.forward(s.detach().cpu().double().numpy(), E_layer.detach().cpu().double().numpy())
+                ctx.rollout = rollout
+                ctx.meta = (s.dtype, s.device, E_layer.dtype, E_layer.device)
+                return torch.from_numpy(x.copy()).to(dtype=s.dtype, device=s.device)
+
+            @staticmethod
+            def backward(ctx, xbar):
+                s_bar, E_bar = ctx.rollout.backward(xbar.detach().cpu().double().numpy())
+                sd, sdev, ed, edev = ctx.meta
+                return (torch.from_numpy(s_bar).to(dtype=sd, device=sdev),
+                        torch.from_numpy(E_bar).to(dtype=ed, device=edev), None)
+
+        _FN = TissueRolloutFn
+    return _FN
+
+
+def __getattr__(name):
+    if name == "TissueRolloutFn":
+        return _rollout_fn()
+    raise AttributeError(name)
 
 
 def apical_loss(x, target, apical_dofs):

@@ -79,7 +79,7 @@ void free_all(tf_handle* h)
                     h->val_base, h->bind_nodes, h->bind_m_ptr, h->bind_m_dof, h->bind_s_dof, h->bind_pairs,
                     h->ablated, h->blk_tr, h->rot_tz, h->val_t, h->g_save, h->adj_xbar, h->adj_rhs, h->adj_pass,
                     h->adj_y, h->adj_un_bar, h->adj_invfg_bar, h->adj_fg_bar, h->adj_fg_prev, h->adj_rate_bar,
-                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz};
+                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz, h->seg_rtol2};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
@@ -442,6 +442,7 @@ void tf_default_options(tf_solver_opts* o)
     o->matrix_precision = TF_PREC_FP64;
     o->growth_rotation = 1;
     o->use_graph = 1;
+    o->krylov_forcing = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -534,6 +535,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg_counter = dev_alloc<int32_t>(S);
         h->seg = dev_alloc<SegScalars>(S);
         h->seg_norm = dev_alloc<double>(S);
+        h->seg_rtol2 = dev_alloc<double>(S);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
         for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx})
@@ -592,6 +594,10 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg_nr_iters.assign(S, 0);
         h->seg_converged.assign(S, 0);
         h->seg_du.assign(S, 0.0);
+        h->fc_C.assign(S, -1.0);
+        h->fc_du0.assign(S, -1.0);
+        h->fc_du_prev.assign(S, -1.0);
+        h->h_seg_rtol2.assign(S, 0.0);
     } catch (const tf_hip_error& ex) {
         free_all(h);
         delete h;
@@ -918,6 +924,47 @@ static double ms_since(std::chrono::steady_clock::time_point t0)
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Inexact-Newton forcing (per tissue).  C is the quadratic-convergence constant of the tissue's
+// Newton iteration (||dU_{k+1}|| ~ C ||dU_k||^2, the smallest ratio seen; the first step, solved
+// to krylov_rtol, supplies it) and p the predicted size of this iteration's update:
+//  - p <= nr_tol (the last iteration): the solve only decides convergence -> relative 1e-2;
+//  - next update C p^2 >= 100 nr_tol (far from convergence): the error left in dU may be 5% of
+//    the next update.  A block-Jacobi CG iterate's error is up to ~100x its relative residual
+//    times ||dU|| (soft, weakly pinned modes keep the slowest error), so rtol = 0.05 C p / 100;
+//  - otherwise (the iteration before the last) krylov_rtol, so that the final ||dU|| and with it
+//    the NR iteration count are those of an exact solve.
+static void forcing_tolerances(tf_handle* h, int iter)
+{
+    const double floor_rtol = h->opts.krylov_rtol, tol = h->opts.nr_tol, amplification = 100.0;
+    for (int s = 0; s < h->S; ++s) {
+        double eta = floor_rtol;
+        const double C = h->fc_C[s];
+        double p = -1.0;
+        if (iter == 0) p = h->fc_du0[s] > 0 ? 1.5 * h->fc_du0[s] : -1.0;
+        else if (h->fc_du_prev[s] > 0) p = C * h->fc_du_prev[s] * h->fc_du_prev[s];
+        if (C > 0 && p > 0) {
+            if (p <= tol) eta = 1e-2;
+            else if (C * p * p >= 100.0 * tol) eta = std::min(1e-2, std::max(floor_rtol, 0.05 * C * p / amplification));
+        }
+        h->h_seg_rtol2[s] = eta * eta;
+    }
+    h2d(h, h->seg_rtol2, h->h_seg_rtol2.data(), h->S);
+}
+
+static void forcing_observe(tf_handle* h, int iter)
+{
+    for (int s = 0; s < h->S; ++s) {
+        if (h->seg_nr_iters[s] != iter + 1) continue;  // tissue not iterated this time
+        const double d = h->seg_du[s], prev = h->fc_du_prev[s];
+        if (iter == 0) h->fc_du0[s] = d;
+        else if (prev > 0 && d > 0) {
+            const double c = d / (prev * prev);
+            h->fc_C[s] = h->fc_C[s] > 0 ? std::min(c, 2.0 * h->fc_C[s]) : c;
+        }
+        h->fc_du_prev[s] = d;
+    }
+}
+
 static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
 {
     auto t0 = std::chrono::steady_clock::now();
@@ -943,9 +990,19 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
             break;
         }
         auto ts = std::chrono::steady_clock::now();
-        int kst = solve_impl(h);
+        h->seg_rtol_on = h->opts.krylov_forcing != 0;
+        if (h->seg_rtol_on) forcing_tolerances(h, it);
+        int kst;
+        try {
+            kst = solve_impl(h);
+        } catch (...) {
+            h->seg_rtol_on = false;
+            throw;
+        }
+        h->seg_rtol_on = false;
         krylov_total += h->last_krylov_iters;
         int ust = update_uk_impl(h, it);
+        if (h->opts.krylov_forcing) forcing_observe(h, it);
         ms_sol += ms_since(ts);
         if (ust == TF_NAN) { status = TF_NAN; h->err = "NaN in the Newton update"; break; }
         if (kst != TF_OK && status == TF_OK) status = TF_OK;  // inexact step: NR keeps iterating

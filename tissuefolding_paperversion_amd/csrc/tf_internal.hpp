@@ -149,6 +149,12 @@ struct tf_handle {
     std::vector<int32_t> seg_nr_iters, seg_converged;
     std::vector<double> seg_du;
     int32_t last_krylov_iters = 0;
+    // inexact-Newton forcing, per tissue: the quadratic-convergence constant C estimated from
+    // ||dU_k|| / ||dU_{k-1}||^2, the first update of the previous step, and this iteration's
+    // Krylov relative tolerance (device copy read by the Krylov set-up while seg_rtol_on)
+    std::vector<double> fc_C, fc_du0, fc_du_prev, h_seg_rtol2;
+    double *seg_rtol2 = nullptr;      // [S]
+    bool seg_rtol_on = false;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
 
     // graphs

@@ -159,7 +159,8 @@ __global__ __launch_bounds__(WG) void k_spmv(const int32_t* __restrict__ row_ptr
 }
 
 // one workgroup per tissue: ||b||^2 -> tolerance, initial state of the tissue's solve
-__global__ __launch_bounds__(WG) void k_seg_init(TileArgs ta, double rtol2, double atol2)
+// (seg_rtol2, when given, holds a relative tolerance per tissue; else rtol2 applies to all)
+__global__ __launch_bounds__(WG) void k_seg_init(TileArgs ta, double rtol2, double atol2, const double* seg_rtol2)
 {
     const int sg = blockIdx.x;
     const int sl[1] = {SL_BB};
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(WG) void k_seg_init(TileArgs ta, double rtol2, doub
         SegScalars& S = ta.seg[sg];
         S.bnorm2 = tot[0];
         S.rnorm2 = tot[0];
-        S.tol2 = fmax(rtol2 * tot[0], atol2);
+        S.tol2 = fmax((seg_rtol2 ? seg_rtol2[sg] : rtol2) * tot[0], atol2);
         S.iters = 0;
         S.stalled = 0;
         for (int q = 0; q < 2; ++q) {
@@ -785,7 +786,8 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
     else
         hipLaunchKernelGGL(k_bicg_init, grid, blk, 0, h->stream, ta, h->g, h->kx, h->kr, h->kr0, h->kp, h->kv);
-    hipLaunchKernelGGL(k_seg_init, dim3(h->S), blk, 0, h->stream, ta, rtol2, atol2);
+    hipLaunchKernelGGL(k_seg_init, dim3(h->S), blk, 0, h->stream, ta, rtol2, atol2,
+                       h->seg_rtol_on ? h->seg_rtol2 : nullptr);
     TF_HIP(hipGetLastError());
     int chunk = std::max(2, h->opts.iters_per_sync);
     chunk += chunk & 1;  // iterations come in parity pairs
