@@ -9,8 +9,8 @@ HBM before the timed region.  Prints one JSON line.
 The dominant kernel (k_cg1: one iteration of the single-kernel CG, i.e. the BSR SpMV with the
 vector updates and dot products fused in) is timed with HIP events on the stream it runs on,
 around each replay of a captured chunk of iterations; its algorithmic bytes per launch are
-nb*(72+4) + (N+1)*4 (matrix) + 12*3N*8 (x, r, w, s, u, p read and written once) + N*72 (block
-inverses).
+nb*(72+4) + (N+1)*4 (matrix) + 16*3N*8 (x, p, r, w, s, u, m, q read and written once) + N*72
+(block inverses).
 A second leg runs the batched 256-disc sweep (config 4: one tissue segment per disc in one
 handle, autotuned tiling) and, for comparison, the same discs one handle at a time.
 A third leg times the discrete adjoint on the inverse-growth workload (config 5: 30 steps of

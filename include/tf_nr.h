@@ -58,8 +58,10 @@ typedef struct tf_mesh_desc {
 } tf_mesh_desc;
 
 /* TF_KRYLOV_CG1: CG with one fused kernel per iteration (single-reduction recurrences of
- * Chronopoulos and Gear); TF_KRYLOV_CG: textbook PCG, two kernels per iteration. */
-enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_CG1 = 3 };
+ * Chronopoulos and Gear); TF_KRYLOV_CG: textbook PCG, two kernels per iteration;
+ * TF_KRYLOV_PIPECG: pipelined CG (Ghysels and Vanroose), one kernel per iteration whose SpMV
+ * operand was completed by the previous kernel. */
+enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_CG1 = 3, TF_KRYLOV_PIPECG = 4 };
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
 
 typedef struct tf_solver_opts {

@@ -84,7 +84,7 @@ def test_global_assembly_matches_oracle(oracle, eta_int):
     assert rel_err(M, K.T) < 1e-10
 
 
-@pytest.mark.parametrize("krylov", [L.KRYLOV_BICGSTAB, L.KRYLOV_CG, L.KRYLOV_CG1])
+@pytest.mark.parametrize("krylov", [L.KRYLOV_BICGSTAB, L.KRYLOV_CG, L.KRYLOV_CG1, L.KRYLOV_PIPECG])
 def test_linear_solve_matches_direct(oracle, krylov):
     t = _small_viscous_disc(eta_int=0.0)
     X, x, Fg = _perturbed_state(t, 3, amp=0.01, growth=0.01)
@@ -104,7 +104,7 @@ def test_linear_solve_matches_direct(oracle, krylov):
     assert rel_err(du, ref) < 1e-8
 
 
-@pytest.mark.parametrize("krylov", [L.KRYLOV_CG1, L.KRYLOV_CG, L.KRYLOV_BICGSTAB])
+@pytest.mark.parametrize("krylov", [L.KRYLOV_CG1, L.KRYLOV_CG, L.KRYLOV_BICGSTAB, L.KRYLOV_PIPECG])
 def test_config1_known_answer(oracle, krylov):
     """Uniform in-plane growth of the pinned disc: x(t) = exp(r t) X in plane, z unchanged."""
     t = configs.config1()
@@ -196,9 +196,15 @@ def test_tilings_agree_and_autotune():
     t = configs.wing_disc_tissue(a=40.0, b=25.0, h=4.0, z_levels=(0.0, 2.0, 6.0, 10.0), seed=7, growth_seed=8)
     t.rates *= 15.0
     ref = None
-    for wg, cap in [(256, 256), (256, 1024), (512, 1024), (1024, 2048)]:
+    tested = 0
+    for wg, cap in [(256, 256), (256, 1024), (512, 1024), (1024, 1024), (1024, 2048)]:
         s = _solver(t)
-        s.set_tiling(wg, cap)
+        try:
+            s.set_tiling(wg, cap)
+        except L.TfError as ex:   # the largest tiling needs more LDS than some devices allow
+            assert ex.code == L.TF_INVALID and (wg, cap) == (1024, 2048)
+            continue
+        tested += 1
         assert s.tiling()[:2] == (wg, cap)
         for _ in range(2):
             assert s.runOneStep(t.dt)["converged"] == 1
@@ -207,9 +213,10 @@ def test_tilings_agree_and_autotune():
             ref = x
         else:
             assert rel_err(x, ref) < 1e-10
+    assert tested >= 4
     s = _solver(t)
     table = s.autotune(t.dt, n_iters=16)
-    assert len(table) >= 6 and all(v["ms_per_iter"] > 0 for v in table)
-    best = min(table, key=lambda v: v["ms_per_iter"])
+    assert len(table) >= 6 and sum(v["ms_per_iter"] > 0 for v in table) >= 6
+    best = min((v for v in table if v["ms_per_iter"] > 0), key=lambda v: v["ms_per_iter"])
     assert s.tiling()[:2] == (best["wg"], best["tile_cap"])
     assert s.runOneStep(t.dt)["converged"] == 1

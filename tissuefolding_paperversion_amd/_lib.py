@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libtfnr.so")
 TF_OK, TF_NOT_CONVERGED, TF_KRYLOV_STALL, TF_NEG_JACOBIAN, TF_NAN, TF_OOM, TF_HIP_ERROR, TF_INVALID = range(8)
 STATUS_NAMES = {0: "TF_OK", 1: "TF_NOT_CONVERGED", 2: "TF_KRYLOV_STALL", 3: "TF_NEG_JACOBIAN", 4: "TF_NAN",
                 5: "TF_OOM", 6: "TF_HIP_ERROR", 7: "TF_INVALID"}
-KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_GMRES, KRYLOV_CG1 = -1, 0, 1, 2, 3
+KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_GMRES, KRYLOV_CG1, KRYLOV_PIPECG = -1, 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 
 # Every symbol include/tf_nr.h declares (checked by the CPU test suite).

@@ -75,7 +75,7 @@ void free_all(tf_handle* h)
                     h->status, h->row_ptr, h->col, h->blk_row, h->diag_blk, h->val, h->val32, h->dinv, h->blk_ptr,
                     h->blk_src, h->node_ptr, h->node_src, h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr,
                     h->seg_node_ptr, h->seg_counter, h->partial, h->seg, h->seg_norm, h->kr, h->kr0, h->kp, h->kph,
-                    h->kv, h->ks, h->ksh, h->kt, h->kx, h->fold_ptr, h->fold_tgt, h->fold_src, h->fold_const,
+                    h->kv, h->ks, h->ksh, h->kt, h->kx, h->km, h->kq, h->fold_ptr, h->fold_tgt, h->fold_src, h->fold_const,
                     h->val_base, h->bind_nodes, h->bind_m_ptr, h->bind_m_dof, h->bind_s_dof, h->bind_pairs,
                     h->ablated, h->blk_tr, h->rot_tz, h->val_t, h->g_save, h->adj_xbar, h->adj_rhs, h->adj_pass,
                     h->adj_y, h->adj_un_bar, h->adj_invfg_bar, h->adj_fg_bar, h->adj_fg_prev, h->adj_rate_bar,
@@ -150,10 +150,19 @@ void timer_collect(KernelTimer& t)
 // ---------------------------------------------------------------------------------------------
 // Row tiles: runs of block rows of one tissue with at most tile_cap blocks and wg/3 rows (so a
 // tile's DoFs never outnumber the lanes of a workgroup).  Rebuilt when the tiling is retuned.
+// LDS of the single-kernel CG for a tiling: products (3 per block), two exchange rows, and the
+// reduction scratch (well under 2 KB)
+bool tiling_fits(const tf_handle* h, int wg, int tile_cap)
+{
+    return sizeof(double) * (3 * (size_t)tile_cap + 2 * (size_t)wg) + 2048 <= (size_t)h->max_lds;
+}
+
 void build_tiles(tf_handle* h, int wg, int tile_cap)
 {
     if (!(wg == 256 || wg == 512 || wg == 1024) || tile_cap % wg || tile_cap > TF_TILE_CAP_MAX || tile_cap / wg > 4)
         throw std::runtime_error("unsupported tiling: wg in {256, 512, 1024}, tile_cap = wg * {1, 2, 4} <= 2048");
+    if (!tiling_fits(h, wg, tile_cap))
+        throw std::runtime_error("tiling needs more LDS than a workgroup may allocate on this device");
     const std::vector<int32_t>& row_ptr = h->h_row_ptr;
     std::vector<int32_t> tr0, tr1, tseg, seg_tile_ptr(h->S + 1, 0);
     const int max_rows = wg / 3;
@@ -462,6 +471,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         tf_default_options(&h->opts);
         h->N = N; h->E = E; h->S = S;
         TF_HIP(hipGetDevice(&h->device));
+        TF_HIP(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
         TF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         h->h_conn.assign(d->conn, d->conn + (size_t)6 * E);
         h->h_node_seg_ptr.resize(S + 1);
@@ -538,7 +548,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg_rtol2 = dev_alloc<double>(S);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
-        for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx})
+        for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx, &h->km, &h->kq})
             *v = dev_alloc<double>((size_t)3 * N);
 
         // ---- uploads ----
@@ -622,7 +632,7 @@ void tf_destroy(tf_handle* h)
 int tf_set_options(tf_handle* h, const tf_solver_opts* o)
 {
     if (!h || !o) return TF_INVALID;
-    if (o->krylov < -1 || o->krylov > TF_KRYLOV_CG1 || o->krylov_max_iter <= 0 || o->nr_max_iter < 0 ||
+    if (o->krylov < -1 || o->krylov > TF_KRYLOV_PIPECG || o->krylov_max_iter <= 0 || o->nr_max_iter < 0 ||
         !(o->krylov_rtol >= 0) || !(o->nr_tol >= 0) ||
         (o->matrix_precision != TF_PREC_FP64 && o->matrix_precision != TF_PREC_FP32) ||
         o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16)
@@ -1357,6 +1367,12 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
     double best = 1e300;
     int bi = 0;
     for (int v = 0; v < nv; ++v) {
+        if (!tiling_fits(h, variants[v][0], variants[v][1])) {
+            if (wg_out) wg_out[v] = variants[v][0];
+            if (cap_out) cap_out[v] = variants[v][1];
+            if (ms_out) ms_out[v] = -1.0;  // does not fit this device's LDS
+            continue;
+        }
         build_tiles(h, variants[v][0], variants[v][1]);
         const double ms = krylov_time_cg1(h, n_iters);
         if (wg_out) wg_out[v] = variants[v][0];
@@ -1412,8 +1428,10 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver
-        if (kind == TF_KRYLOV_CG1)  // SpMV + x,r,w,s,u,p read and written once + block-Jacobi inverses
-            *bytes = mat + n * 8.0 * 12.0 + N * 72.0;
+        if (kind == TF_KRYLOV_CG1)  // SpMV; x,p,r,w,s,u,m,q read and x,p,r,w,s,u,m,q written once; D^-1 rows
+            *bytes = mat + n * 8.0 * 16.0 + N * 72.0;
+        else if (kind == TF_KRYLOV_PIPECG)  // SpMV on m; z,q,s,p,x,r,u,w,m read, the same and m' written; D^-1 rows
+            *bytes = mat + n * 8.0 * 18.0 + N * 72.0;
         else if (kind == TF_KRYLOV_CG)  // SpMV on p = z + beta p_old: z, p_old read; p_new, q written
             *bytes = mat + n * 8.0 * 4.0;
         else  // BiCGSTAB SpMV with one fused dot product: x, w read; y written

@@ -66,6 +66,7 @@ struct tf_handle {
     // sizes
     int32_t N = 0, E = 0, nb = 0, S = 1, T = 0;
     int32_t wg = 256, tile_cap = 512;  // single-kernel CG workgroup size and tile capacity (blocks)
+    int32_t max_lds = 65536;           // bytes of LDS one workgroup may allocate (device attribute)
     // options
     tf_solver_opts opts;
     std::string err;
@@ -143,7 +144,7 @@ struct tf_handle {
 
     // Krylov work vectors [3N]
     double *kr = nullptr, *kr0 = nullptr, *kp = nullptr, *kph = nullptr, *kv = nullptr, *ks = nullptr,
-           *ksh = nullptr, *kt = nullptr, *kx = nullptr;
+           *ksh = nullptr, *kt = nullptr, *kx = nullptr, *km = nullptr, *kq = nullptr;
 
     // per-segment NR bookkeeping on the host
     std::vector<int32_t> seg_nr_iters, seg_converged;
@@ -178,6 +179,7 @@ void launch_fill(tf_handle* h, double* dst, double v, int64_t n);
 void launch_seg_norm(tf_handle* h, const double* x);   // per-segment ||x||_2 into h->seg_norm
 void launch_update_uk(tf_handle* h);                    // uk += du on active segments
 void build_tiles(tf_handle* h, int wg, int tile_cap);  // (re)partitions rows into tiles (tf_api.hip)
+bool tiling_fits(const tf_handle* h, int wg, int tile_cap);  // LDS of the single-kernel CG fits
 void build_pattern(tf_handle* h);  // BSR pattern, scatter lists and bound-DoF fold lists (tf_api.hip)
 void launch_bind_matrix(tf_handle* h);    // M <- N^T M N + I_slaves on the fold targets
 void launch_bind_residual(tf_handle* h);  // g <- N^T g

@@ -443,19 +443,15 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 
 // ---------------------------------------------------------------------------------------------
 // Single-kernel CG (the single-reduction recurrences of Chronopoulos and Gear):
-//   x_{k} = x_{k-1} + a_{k-1} p_{k-1},   r_k = r_{k-1} - a_{k-1} s_{k-1},   u_k = D^-1 r_k,
+//   s_{k-1} = w_{k-1} + b_{k-1} s_{k-2},  r_k = r_{k-1} - a_{k-1} s_{k-1},  u_k = D^-1 r_k,
+//   p_{k-1} = u_{k-1} + b_{k-1} p_{k-2},  x_k = x_{k-1} + a_{k-1} p_{k-1},
 //   w_k = M u_k,  g_k = (r_k, u_k),  d_k = (w_k, u_k),
-//   b_k = g_k / g_{k-1},  a_k = g_k / (d_k - b_k g_k / a_{k-1}),
-//   p_k = u_k + b_k p_{k-1},  s_k = w_k + b_k s_{k-1}  (s_k = M p_k).
-// Kernel k forms a_{k-1}, b_{k-1} from the partials of kernel k-1, updates its own rows, and
-// rebuilds r_k and u_k of neighbouring block columns on the fly from r_{k-1}, w_{k-1}, s_{k-2}
-// (ping-pong buffers, so no tile overwrites what another tile still reads) before the SpMV.
-__device__ inline void cg1_update(double w, double s_old, double r_old, double beta, double alpha, double& s_new,
-                                  double& r_new)
-{
-    s_new = w + beta * s_old;
-    r_new = r_old - alpha * s_new;
-}
+//   b_k = g_k / g_{k-1},  a_k = g_k / (d_k - b_k g_k / a_{k-1}).
+// Launch k forms a_{k-1}, b_{k-1} from the partials of launch k-1 and needs u_k of every
+// neighbouring block column for w_k = M u_k.  D^-1 is linear, so
+//   u_k = u_{k-1} - a_{k-1} (m_{k-1} + b_{k-1} q_{k-2}),  m = D^-1 w,  q = D^-1 s,
+// and each launch writes u, m, q for its own nodes (ping-pong buffers): a neighbour costs three
+// 3-vectors, and w stays an explicit product with u (no recurrence for w, as in pipelined CG).
 __device__ inline double dinv_apply3(const double* __restrict__ D, double r0, double r1, double r2)
 {
     return D[0] * r0 + D[1] * r1 + D[2] * r2;
@@ -465,41 +461,47 @@ template <typename VT>
 __global__ __launch_bounds__(WG) void k_cg1_init(TileArgs ta, const int32_t* __restrict__ row_ptr,
                                                  const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
                                                  const double* __restrict__ dinv, const double* __restrict__ b,
-                                                 double* __restrict__ x, double* __restrict__ r0, double* __restrict__ w0,
-                                                 double* __restrict__ s_m1, double* __restrict__ u, double* __restrict__ p)
+                                                 double* __restrict__ x, double* __restrict__ r, double* __restrict__ w,
+                                                 double* __restrict__ s, double* __restrict__ p, double* __restrict__ u0,
+                                                 double* __restrict__ m0, double* __restrict__ q0, int tile_cap)
 {
-    extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
+    extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks), then the tile's w
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
     const bool active = ta.seg[sg].active;
     const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
     const int d0 = 3 * rr0, d1 = 3 * rr1;
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    double* wloc = prod + 3 * (size_t)tile_cap;
+    tile_products<VT>(col, val, nb, b0, b1,
+                      [&](int c, double& a0, double& a1, double& a2) {
+                          const double q0v = active ? b[3 * c] : 0.0, q1v = active ? b[3 * c + 1] : 0.0,
+                                       q2v = active ? b[3 * c + 2] : 0.0;
+                          const double* D = dinv + 9 * c;
+                          a0 = dinv_apply3(D, q0v, q1v, q2v);
+                          a1 = dinv_apply3(D + 3, q0v, q1v, q2v);
+                          a2 = dinv_apply3(D + 6, q0v, q1v, q2v);
+                      },
+                      prod);
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, rw = d / 3, comp = d % 3;
         const double bv = active ? b[d] : 0.0;
-        x[d] = 0.0; r0[d] = bv; s_m1[d] = 0.0; p[d] = 0.0;
+        const double* D = dinv + 9 * rw + 3 * comp;
+        const double* bb = b + 3 * rw;
+        const double uv = active ? D[0] * bb[0] + D[1] * bb[1] + D[2] * bb[2] : 0.0;
+        const double wv = row_sum(row_ptr, rw, comp, b0, prod);
+        x[d] = 0.0; r[d] = bv; s[d] = 0.0; p[d] = 0.0; w[d] = wv; u0[d] = uv; q0[d] = 0.0;
+        wloc[task] = wv;
+        g += bv * uv;
+        dl += wv * uv;
+        rr += bv * bv;
     }
     __syncthreads();
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) u[d] = dinv_row(dinv, r0, d);
-    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
-    tile_products<VT>(col, val, nb, b0, b1,
-                          [&](int c, double& a0, double& a1, double& a2) {
-                              const double q0 = active ? b[3 * c] : 0.0, q1 = active ? b[3 * c + 1] : 0.0,
-                                           q2 = active ? b[3 * c + 2] : 0.0;
-                              const double* D = dinv + 9 * c;
-                              a0 = dinv_apply3(D, q0, q1, q2);
-                              a1 = dinv_apply3(D + 3, q0, q1, q2);
-                              a2 = dinv_apply3(D + 6, q0, q1, q2);
-                          },
-                          prod);
-    double g = 0.0, dl = 0.0, rr = 0.0;
-    for (int task = threadIdx.x; task < 3 * (rr1 - rr0); task += WG) {
-        const int r = rr0 + task / 3, comp = task % 3;
-        const int d = 3 * r + comp;
-        const double wv = row_sum(row_ptr, r, comp, b0, prod);
-        w0[d] = wv;
-        g += r0[d] * u[d];
-        dl += wv * u[d];
-        rr += r0[d] * r0[d];
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, nd = task - d % 3;
+        const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
+        m0[d] = D[0] * wloc[nd] + D[1] * wloc[nd + 1] + D[2] * wloc[nd + 2];
     }
     const int sl[4] = {SL_RHO0, SL_DL0, SL_RR0, SL_BB};
     double vv[4] = {g, dl, rr, rr};
@@ -507,19 +509,21 @@ __global__ __launch_bounds__(WG) void k_cg1_init(TileArgs ta, const int32_t* __r
 }
 
 // One iteration per launch.  Everything that does not depend on this iteration's scalars (the
-// tile's matrix blocks, the neighbour columns' r, w, s and block inverses, the own rows' vectors)
-// is loaded into registers first, so the loads are in flight while the workgroup re-sums the
-// partials; then the scalars are formed, own rows updated and the products parked in LDS.
+// tile's matrix blocks, the neighbour columns' u, m, q, the own rows' vectors and block-inverse
+// rows) is loaded into registers first, so the loads are in flight while the workgroup re-sums
+// the partials; then the scalars are formed, own rows updated and the products parked in LDS.
 // WGT lanes per workgroup, BPT matrix blocks per lane (tile capacity WGT*BPT), at most one own
-// DoF per lane (tiles hold at most WGT/3 block rows).
+// DoF per lane (tiles hold at most WGT/3 block rows).  Launch parity P reads u, m, q of parity
+// 1-P and writes parity P; x, p, r, w, s are read and written by their own lane only.
 template <typename VT, int P, int WGT, int BPT>
 __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, int nb, const double* __restrict__ dinv,
-                                             double* __restrict__ x, const double* __restrict__ r_old,
-                                             double* __restrict__ r_new, const double* __restrict__ w_old,
-                                             double* __restrict__ w_new, const double* __restrict__ s_old,
-                                             double* __restrict__ s_new, double* __restrict__ u, double* __restrict__ p,
-                                             int maxit, int tile_cap)
+                                             double* __restrict__ x, double* __restrict__ p, double* __restrict__ r,
+                                             double* __restrict__ w, double* __restrict__ s,
+                                             const double* __restrict__ u_old, double* __restrict__ u_new,
+                                             const double* __restrict__ m_old, double* __restrict__ m_new,
+                                             const double* __restrict__ q_old, double* __restrict__ q_new, int maxit,
+                                             int tile_cap)
 {
     extern __shared__ double prod[];
     const int t = blockIdx.x;
@@ -530,22 +534,20 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
     const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
     const int d0 = 3 * rr0, d1 = 3 * rr1;
     // ---- prefetch ----
-    double mv[BPT][9], cw[BPT][3], cs[BPT][3], cr[BPT][3], cd[BPT][9];
+    double mv[BPT][9], cu[BPT][3], cm[BPT][3], cq[BPT][3];
 #pragma unroll
-    for (int q = 0; q < BPT; ++q) {
-        const int j = b0 + threadIdx.x + q * WGT;
+    for (int k = 0; k < BPT; ++k) {
+        const int j = b0 + threadIdx.x + k * WGT;
         if (j < b1) {
             const int c = col[j];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) mv[q][k] = val[(size_t)k * nb + j];
+            for (int e = 0; e < 9; ++e) mv[k][e] = val[(size_t)e * nb + j];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                cw[q][k] = w_old[3 * c + k];
-                cs[q][k] = s_old[3 * c + k];
-                cr[q][k] = r_old[3 * c + k];
+            for (int e = 0; e < 3; ++e) {
+                cu[k][e] = u_old[3 * c + e];
+                cm[k][e] = m_old[3 * c + e];
+                cq[k][e] = q_old[3 * c + e];
             }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) cd[q][k] = dinv[9 * c + k];
         }
     }
     const int d = d0 + threadIdx.x;
@@ -553,12 +555,13 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
     double ow = 0.0, os = 0.0, orr = 0.0, ou = 0.0, op = 0.0, ox = 0.0, od[3] = {0.0, 0.0, 0.0};
     int rb0 = 0, rb1 = 0;
     if (own) {
-        ow = w_old[d]; os = s_old[d]; orr = r_old[d]; ou = u[d]; op = p[d]; ox = x[d];
+        ow = w[d]; os = s[d]; orr = r[d]; ou = u_old[d]; op = p[d]; ox = x[d];
         const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
         od[0] = D[0]; od[1] = D[1]; od[2] = D[2];
         rb0 = row_ptr[d / 3]; rb1 = row_ptr[d / 3 + 1];
     }
     double* rloc = prod + 3 * (size_t)tile_cap;  // the tile's new residual, exchanged in LDS
+    double* sloc = rloc + WGT;                    // the tile's new s
     // ---- scalars of iteration k-1: partials of launch k-1 (slots of parity 1-P) and the
     //      history launch k-1 left in the parity-P slots of the tissue's scalars ----
     const int sl[3] = {SL_RHO0 + (1 - P), SL_DL0 + (1 - P), SL_RR0 + (1 - P)};
@@ -599,47 +602,226 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
     }
     // ---- own rows ----
     if (own) {
-        double sn, rn;
-        cg1_update(ow, os, orr, beta, alpha, sn, rn);
+        const double sn = ow + beta * os;
+        const double rn = orr - alpha * sn;
         const double pn = ou + beta * op;
-        s_new[d] = sn;
-        r_new[d] = rn;
+        s[d] = sn;
+        r[d] = rn;
         rloc[d - d0] = rn;
+        sloc[d - d0] = sn;
         p[d] = pn;
         x[d] = ox + alpha * pn;
     }
-    // ---- products with the neighbours' u = D^-1 r (rebuilt from the prefetched columns) ----
+    // ---- products with the neighbours' u_k = u_{k-1} - a (m_{k-1} + b q_{k-2}) ----
 #pragma unroll
-    for (int q = 0; q < BPT; ++q) {
-        const int j = b0 + threadIdx.x + q * WGT;
+    for (int k = 0; k < BPT; ++k) {
+        const int j = b0 + threadIdx.x + k * WGT;
         if (j < b1) {
-            double sn, q0, q1, q2;
-            cg1_update(cw[q][0], cs[q][0], cr[q][0], beta, alpha, sn, q0);
-            cg1_update(cw[q][1], cs[q][1], cr[q][1], beta, alpha, sn, q1);
-            cg1_update(cw[q][2], cs[q][2], cr[q][2], beta, alpha, sn, q2);
-            const double a0 = dinv_apply3(cd[q], q0, q1, q2);
-            const double a1 = dinv_apply3(cd[q] + 3, q0, q1, q2);
-            const double a2 = dinv_apply3(cd[q] + 6, q0, q1, q2);
+            const double a0 = cu[k][0] - alpha * (cm[k][0] + beta * cq[k][0]);
+            const double a1 = cu[k][1] - alpha * (cm[k][1] + beta * cq[k][1]);
+            const double a2 = cu[k][2] - alpha * (cm[k][2] + beta * cq[k][2]);
             double* pr = prod + 3 * (j - b0);
-            pr[0] = mv[q][0] * a0 + mv[q][1] * a1 + mv[q][2] * a2;
-            pr[1] = mv[q][3] * a0 + mv[q][4] * a1 + mv[q][5] * a2;
-            pr[2] = mv[q][6] * a0 + mv[q][7] * a1 + mv[q][8] * a2;
+            pr[0] = mv[k][0] * a0 + mv[k][1] * a1 + mv[k][2] * a2;
+            pr[1] = mv[k][3] * a0 + mv[k][4] * a1 + mv[k][5] * a2;
+            pr[2] = mv[k][6] * a0 + mv[k][7] * a1 + mv[k][8] * a2;
         }
+    }
+    __syncthreads();
+    double g = 0.0, dl = 0.0, rrn = 0.0, wv = 0.0;
+    const int nd = d - d0 - d % 3;  // first DoF of this lane's node within the tile
+    if (own) {
+        const double rv = rloc[d - d0];
+        const double un = od[0] * rloc[nd] + od[1] * rloc[nd + 1] + od[2] * rloc[nd + 2];
+        u_new[d] = un;
+        q_new[d] = od[0] * sloc[nd] + od[1] * sloc[nd + 1] + od[2] * sloc[nd + 2];
+        const int comp = d % 3;
+        for (int j = rb0; j < rb1; ++j) wv += prod[3 * (j - b0) + comp];
+        w[d] = wv;
+        g = rv * un;
+        dl = wv * un;
+        rrn = rv * rv;
+    }
+    __syncthreads();  // every lane has read rloc/sloc: reuse rloc for the tile's new w
+    if (own) rloc[d - d0] = wv;
+    __syncthreads();
+    if (own) m_new[d] = od[0] * rloc[nd] + od[1] * rloc[nd + 1] + od[2] * rloc[nd + 2];
+    const int so[3] = {SL_RHO0 + P, SL_DL0 + P, SL_RR0 + P};
+    double vv[3] = {g, dl, rrn};
+    tile_write<3, WGT>(ta, t, so, vv);
+    if (lead) {
+        S.alpha_hist[1 - P] = alpha;
+        S.gam_hist[1 - P] = gk;
+        S.iters_h[1 - P] = it + 1;
+        S.iters = it + 1;
+        const bool better = rr < 0.5 * best;
+        S.best_rr_h[1 - P] = better ? rr : best;
+        S.best_it_h[1 - P] = better ? it : best_it;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pipelined CG (Ghysels and Vanroose), one kernel per iteration:
+//   g_i = (r_i, u_i), d_i = (w_i, u_i), n_i = M m_i with m_i = D^-1 w_i,
+//   b_i = g_i / g_{i-1}, a_i = g_i / (d_i - b_i g_i / a_{i-1})   (b_0 = 0, a_0 = g_0 / d_0),
+//   z = n + b z, q = m + b q, s = w + b s, p = u + b p,
+//   x += a p, r -= a s, u -= a q, w -= a z,  and m_{i+1} = D^-1 w_{i+1}.
+// The SpMV operand m_i is complete when launch i starts (launch i-1 wrote it for its own rows), so
+// a block needs only its column's m (3 doubles) and the products start before the scalars are
+// known.  m is ping-ponged (launch P reads m[1-P], writes m[P]); every other vector is touched by
+// its own row's lane only.
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_pipe_init(TileArgs ta, const int32_t* __restrict__ row_ptr,
+                                                  const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
+                                                  const double* __restrict__ dinv, const double* __restrict__ b,
+                                                  double* __restrict__ x, double* __restrict__ r, double* __restrict__ u,
+                                                  double* __restrict__ w, double* __restrict__ z, double* __restrict__ q,
+                                                  double* __restrict__ s, double* __restrict__ p, double* __restrict__ m0,
+                                                  int tile_cap)
+{
+    extern __shared__ double prod[];  // 3 doubles per block of the tile, then the tile's w
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    const bool active = ta.seg[sg].active;
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    double* wloc = prod + 3 * (size_t)tile_cap;
+    tile_products<VT>(col, val, nb, b0, b1,
+                      [&](int c, double& a0, double& a1, double& a2) {
+                          const double q0 = active ? b[3 * c] : 0.0, q1 = active ? b[3 * c + 1] : 0.0,
+                                       q2 = active ? b[3 * c + 2] : 0.0;
+                          const double* D = dinv + 9 * c;
+                          a0 = dinv_apply3(D, q0, q1, q2);
+                          a1 = dinv_apply3(D + 3, q0, q1, q2);
+                          a2 = dinv_apply3(D + 6, q0, q1, q2);
+                      },
+                      prod);
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, rw = d / 3, comp = d % 3;
+        const double bv = active ? b[d] : 0.0;
+        const double* D = dinv + 9 * rw + 3 * comp;
+        const double* bb = b + 3 * rw;
+        const double uv = active ? D[0] * bb[0] + D[1] * bb[1] + D[2] * bb[2] : 0.0;
+        const double wv = row_sum(row_ptr, rw, comp, b0, prod);
+        x[d] = 0.0; r[d] = bv; u[d] = uv; w[d] = wv;
+        z[d] = 0.0; q[d] = 0.0; s[d] = 0.0; p[d] = 0.0;
+        wloc[task] = wv;
+        g += bv * uv;
+        dl += wv * uv;
+        rr += bv * bv;
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, nd = task - d % 3;
+        const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
+        m0[d] = D[0] * wloc[nd] + D[1] * wloc[nd + 1] + D[2] * wloc[nd + 2];
+    }
+    const int sl[4] = {SL_RHO0, SL_DL0, SL_RR0, SL_BB};
+    double vv[4] = {g, dl, rr, rr};
+    tile_write<4>(ta, t, sl, vv);
+}
+
+template <typename VT, int P, int WGT, int BPT>
+__global__ __launch_bounds__(WGT) void k_pipe(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                                              const VT* __restrict__ val, int nb, const double* __restrict__ dinv,
+                                              double* __restrict__ x, double* __restrict__ r, double* __restrict__ u,
+                                              double* __restrict__ w, double* __restrict__ z, double* __restrict__ q,
+                                              double* __restrict__ s, double* __restrict__ p,
+                                              const double* __restrict__ m_old, double* __restrict__ m_new, int maxit,
+                                              int tile_cap)
+{
+    extern __shared__ double prod[];
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    SegScalars& S = ta.seg[sg];
+    if (seg_done(S)) return;
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    // ---- n = M m: independent of this iteration's scalars, so it goes first ----
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        const int j = b0 + threadIdx.x + k * WGT;
+        if (j < b1) {
+            const int c = col[j];
+            const double a0 = m_old[3 * c], a1 = m_old[3 * c + 1], a2 = m_old[3 * c + 2];
+            double mv[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) mv[e] = (double)val[(size_t)e * nb + j];
+            double* pr = prod + 3 * (j - b0);
+            pr[0] = mv[0] * a0 + mv[1] * a1 + mv[2] * a2;
+            pr[1] = mv[3] * a0 + mv[4] * a1 + mv[5] * a2;
+            pr[2] = mv[6] * a0 + mv[7] * a1 + mv[8] * a2;
+        }
+    }
+    const int d = d0 + threadIdx.x;
+    const bool own = d < d1;
+    double oz = 0, oq = 0, os = 0, op = 0, ox = 0, orr = 0, ou = 0, ow = 0, om = 0, od[3] = {0, 0, 0};
+    int rb0 = 0, rb1 = 0;
+    if (own) {
+        oz = z[d]; oq = q[d]; os = s[d]; op = p[d]; ox = x[d]; orr = r[d]; ou = u[d]; ow = w[d]; om = m_old[d];
+        const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
+        od[0] = D[0]; od[1] = D[1]; od[2] = D[2];
+        rb0 = row_ptr[d / 3]; rb1 = row_ptr[d / 3 + 1];
+    }
+    double* wloc = prod + 3 * (size_t)tile_cap;
+    // ---- scalars of iteration i from the partials of launch i-1 ----
+    const int sl[3] = {SL_RHO0 + (1 - P), SL_DL0 + (1 - P), SL_RR0 + (1 - P)};
+    double q3[3];
+    seg_sums<3, WGT>(ta, sg, sl, q3);  // its barriers also publish the products above
+    const double gk = q3[0], dk = q3[1], rr = q3[2];
+    const int it = S.iters_h[P];
+    const double best = S.best_rr_h[P];
+    const int best_it = S.best_it_h[P];
+    const bool lead = first_tile(ta, t, sg) && threadIdx.x == 0;
+    {
+        const bool stagnant = it - best_it > 256;
+        const bool conv = rr <= S.tol2 || (stagnant && best <= 100.0 * S.tol2);
+        const bool cap = !conv && (it >= maxit || !isfinite(rr) || stagnant);
+        if (conv || cap) {
+            if (lead) {
+                S.rnorm2 = rr;
+                S.iters = it;
+                if (conv) S.converged = 1;
+                else S.stalled = 1;
+            }
+            return;
+        }
+    }
+    const bool first = it == 0;
+    const double beta = first ? 0.0 : gk / S.gam_hist[P];
+    const double den = first ? dk : dk - beta * gk / S.alpha_hist[P];
+    const double alpha = gk / den;
+    if (!(den > 0.0) || !isfinite(alpha)) {
+        if (lead) {
+            S.iters = it;
+            S.stalled = 1;
+        }
+        return;
+    }
+    double rn = 0.0, un = 0.0, wn = 0.0;
+    if (own) {
+        const int comp = d % 3;
+        double nv = 0.0;
+        for (int j = rb0; j < rb1; ++j) nv += prod[3 * (j - b0) + comp];
+        const double zn = nv + beta * oz, qn = om + beta * oq, sn = ow + beta * os, pn = ou + beta * op;
+        z[d] = zn; q[d] = qn; s[d] = sn; p[d] = pn;
+        x[d] = ox + alpha * pn;
+        rn = orr - alpha * sn;
+        un = ou - alpha * qn;
+        wn = ow - alpha * zn;
+        r[d] = rn; u[d] = un; w[d] = wn;
+        wloc[d - d0] = wn;
     }
     __syncthreads();
     double g = 0.0, dl = 0.0, rrn = 0.0;
     if (own) {
-        const int nd = d - d0 - d % 3;  // first DoF of this node within the tile
-        const double rv = rloc[d - d0];
-        const double un = od[0] * rloc[nd] + od[1] * rloc[nd + 1] + od[2] * rloc[nd + 2];
-        u[d] = un;
-        const int comp = d % 3;
-        double wv = 0.0;
-        for (int j = rb0; j < rb1; ++j) wv += prod[3 * (j - b0) + comp];
-        w_new[d] = wv;
-        g = rv * un;
-        dl = wv * un;
-        rrn = rv * rv;
+        const int nd = d - d0 - d % 3;
+        m_new[d] = od[0] * wloc[nd] + od[1] * wloc[nd + 1] + od[2] * wloc[nd + 2];
+        g = rn * un;
+        dl = wn * un;
+        rrn = rn * rn;
     }
     const int so[3] = {SL_RHO0 + P, SL_DL0 + P, SL_RR0 + P};
     double vv[3] = {g, dl, rrn};
@@ -718,22 +900,22 @@ void enqueue_cg_iter(tf_handle* h, const VT* val)
     hipLaunchKernelGGL(k_cg_x<P>, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks);
 }
 
-// Single-kernel CG buffers: r (kr, kr0), w (kv, kt), s (ks, ksh) ping-pong; u = kph, p = kp, x = kx.
-// Kernel k has parity P = k & 1 and reads the buffers of parity 1-P (r, w) and P (s).
+// Single-kernel CG buffers: x = kx, p = kp, r = kr, w = kv, s = ks (own rows only) and the
+// ping-pong pairs u = (kph, km), m = (kt, ksh), q = (kr0, kq); launch parity P reads 1-P.
 template <typename VT, int P>
 void enqueue_cg1_iter(tf_handle* h, const VT* val)
 {
     TileArgs ta = tile_args(h);
-    double* R[2] = {h->kr, h->kr0};
-    double* W[2] = {h->kv, h->kt};
-    double* Sv[2] = {h->ks, h->ksh};
+    double* U[2] = {h->kph, h->km};
+    double* Mv[2] = {h->kt, h->ksh};
+    double* Q[2] = {h->kr0, h->kq};
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    const size_t lds = sizeof(double) * (3 * (size_t)h->tile_cap + h->wg);
+    const size_t lds = sizeof(double) * (3 * (size_t)h->tile_cap + 2 * (size_t)h->wg);
 #define TF_CG1(WGT, BPT)                                                                                          \
     hipLaunchKernelGGL((k_cg1<VT, P, WGT, BPT>), dim3(h->T), dim3(WGT), lds, h->stream, ta, h->row_ptr, h->col, val, \
-                       h->nb, h->dinv, h->kx, R[1 - P], R[P], W[1 - P], W[P], Sv[P], Sv[1 - P], h->kph, h->kp,        \
-                       h->opts.krylov_max_iter, h->tile_cap)
+                       h->nb, h->dinv, h->kx, h->kp, h->kr, h->kv, h->ks, U[1 - P], U[P], Mv[1 - P], Mv[P], Q[1 - P],  \
+                       Q[P], h->opts.krylov_max_iter, h->tile_cap)
     const int bpt = h->tile_cap / h->wg;
     if (h->wg == 512) {
         if (bpt == 1) TF_CG1(512, 1); else if (bpt == 2) TF_CG1(512, 2); else TF_CG1(512, 4);
@@ -746,11 +928,40 @@ void enqueue_cg1_iter(tf_handle* h, const VT* val)
     timer_end(h, h->t_spmv, ev);
 }
 
+// Pipelined CG buffers: x = kx, r = kr, u = kr0, w = kv, z = kt, q = ksh, s = ks, p = kp,
+// m ping-pong = (kph, km); launch of parity P reads m[1-P] and writes m[P].
+template <typename VT, int P>
+void enqueue_pipe_iter(tf_handle* h, const VT* val)
+{
+    TileArgs ta = tile_args(h);
+    double* Mb[2] = {h->kph, h->km};
+    hipEvent_t ev[2];
+    timer_begin(h, h->t_spmv, ev);
+    const size_t lds = sizeof(double) * (3 * (size_t)h->tile_cap + h->wg);
+#define TF_PIPE(WGT, BPT)                                                                                           \
+    hipLaunchKernelGGL((k_pipe<VT, P, WGT, BPT>), dim3(h->T), dim3(WGT), lds, h->stream, ta, h->row_ptr, h->col, val, \
+                       h->nb, h->dinv, h->kx, h->kr, h->kr0, h->kv, h->kt, h->ksh, h->ks, h->kp, Mb[1 - P], Mb[P],       \
+                       h->opts.krylov_max_iter, h->tile_cap)
+    const int bpt = h->tile_cap / h->wg;
+    if (h->wg == 512) {
+        if (bpt == 1) TF_PIPE(512, 1); else if (bpt == 2) TF_PIPE(512, 2); else TF_PIPE(512, 4);
+    } else if (h->wg == 1024) {
+        if (bpt == 1) TF_PIPE(1024, 1); else TF_PIPE(1024, 2);
+    } else {
+        if (bpt == 1) TF_PIPE(256, 1); else if (bpt == 2) TF_PIPE(256, 2); else TF_PIPE(256, 4);
+    }
+#undef TF_PIPE
+    timer_end(h, h->t_spmv, ev);
+}
+
 template <typename VT>
 void enqueue_chunk(tf_handle* h, const VT* val, int kind, int iters)
 {
     for (int i = 0; i < iters; i += 2) {
-        if (kind == TF_KRYLOV_CG1) {
+        if (kind == TF_KRYLOV_PIPECG) {
+            enqueue_pipe_iter<VT, 1>(h, val);  // chunks start at an odd kernel (kernel 0 is the init)
+            enqueue_pipe_iter<VT, 0>(h, val);
+        } else if (kind == TF_KRYLOV_CG1) {
             enqueue_cg1_iter<VT, 1>(h, val);  // chunks start at an odd kernel (kernel 0 is the init)
             enqueue_cg1_iter<VT, 0>(h, val);
         } else if (kind == TF_KRYLOV_CG) {
@@ -779,9 +990,14 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const dim3 grid(h->T), blk(WG);
     const double rtol2 = h->opts.krylov_rtol * h->opts.krylov_rtol;
     const double atol2 = h->opts.krylov_atol * h->opts.krylov_atol;
-    if (kind == TF_KRYLOV_CG1)
-        hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv, h->g,
-                           h->kx, h->kr, h->kv, h->ksh, h->kph, h->kp);
+    if (kind == TF_KRYLOV_PIPECG)
+        hipLaunchKernelGGL(k_pipe_init<VT>, grid, blk, sizeof(double) * (3 * (size_t)h->tile_cap + h->wg), h->stream, ta,
+                           h->row_ptr, h->col, val, h->nb, h->dinv, h->g, h->kx, h->kr, h->kr0, h->kv, h->kt, h->ksh,
+                           h->ks, h->kp, h->kph, h->tile_cap);
+    else if (kind == TF_KRYLOV_CG1)
+        hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, sizeof(double) * (3 * (size_t)h->tile_cap + h->wg), h->stream, ta,
+                           h->row_ptr, h->col, val, h->nb, h->dinv, h->g, h->kx, h->kr, h->kv, h->ks, h->kp, h->kph,
+                           h->kt, h->kr0, h->tile_cap);
     else if (kind == TF_KRYLOV_CG)
         hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
     else
@@ -794,7 +1010,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const bool timing = h->t_spmv.enabled;
     // Graph replay.  While timing, only the single-kernel CG keeps the graph: a replayed chunk is
     // then exactly `chunk` launches of the dominant kernel, timed by events around the replay.
-    const bool graph = h->opts.use_graph != 0 && (!timing || kind == TF_KRYLOV_CG1);
+    const bool graph = h->opts.use_graph != 0 && (!timing || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG);
     int done_iters = 0;
     if (graph && timing && !h->t_spmv.sample[0]) {
         TF_HIP(hipEventCreate(&h->t_spmv.sample[0]));
@@ -880,7 +1096,7 @@ int krylov_solve(tf_handle* h)
     const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
     auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     int st = run(kind);
-    if (st == TF_KRYLOV_STALL && (kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1)) {
+    if (st == TF_KRYLOV_STALL && (kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG)) {
         // not positive definite (or too slow): BiCGSTAB from zero
         const int cg_iters = h->last_krylov_iters;
         h->fallbacks += 1;
