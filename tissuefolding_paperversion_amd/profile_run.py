@@ -19,6 +19,8 @@ def main(argv=None):
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-graph", action="store_true")
+    p.add_argument("--tiling", type=int, nargs=2, metavar=("WG", "TILE_BLOCKS"),
+                   help="Krylov tiling (workgroup size, blocks per tile) instead of the default")
     a = p.parse_args(argv)
 
     from . import configs
@@ -26,6 +28,8 @@ def main(argv=None):
 
     t = configs.config2() if a.workload == "config2" else configs.config1()
     s = NewtonRaphsonSolver(t, {"use_graph": 0 if a.no_graph else 1})
+    if a.tiling:
+        s.set_tiling(*a.tiling)
     for _ in range(a.warmup):
         s.runOneStep(t.dt)
     stats = [s.runOneStep(t.dt) for _ in range(a.steps)]
