@@ -84,7 +84,7 @@ void free_all(tf_handle* h)
         if (p) (void)hipFree(p);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
-    if (h->krylov_graph) (void)hipGraphExecDestroy(h->krylov_graph);
+    drop_krylov_graphs(h);
     for (KernelTimer* t : {&h->t_elem, &h->t_gather, &h->t_spmv}) {
         for (hipEvent_t e : t->pending) (void)hipEventDestroy(e);
         for (hipEvent_t e : t->sample)
@@ -150,6 +150,15 @@ void timer_collect(KernelTimer& t)
 // ---------------------------------------------------------------------------------------------
 // Row tiles: runs of block rows of one tissue with at most tile_cap blocks and wg/3 rows (so a
 // tile's DoFs never outnumber the lanes of a workgroup).  Rebuilt when the tiling is retuned.
+void drop_krylov_graphs(tf_handle* h)
+{
+    for (int k = 0; k < 2; ++k)
+        if (h->krylov_graph[k]) {
+            (void)hipGraphExecDestroy(h->krylov_graph[k]);
+            h->krylov_graph[k] = nullptr;
+        }
+}
+
 // LDS of the single-kernel CG for a tiling: products (3 per block), two exchange rows, and the
 // reduction scratch (well under 2 KB)
 bool tiling_fits(const tf_handle* h, int wg, int tile_cap)
@@ -197,7 +206,7 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
     h->h_seg_tile_ptr = seg_tile_ptr;
     h->wg = wg;
     h->tile_cap = tile_cap;
-    if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+    drop_krylov_graphs(h);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -639,7 +648,7 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         return fail(h, TF_INVALID, "invalid solver options");
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
     h->opts = *o;
-    if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+    drop_krylov_graphs(h);
     return TF_OK;
 }
 
@@ -1144,7 +1153,7 @@ static void ensure_adjoint(tf_handle* h)
 
 static void drop_graph(tf_handle* h)
 {
-    if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+    drop_krylov_graphs(h);
 }
 
 // M x = b or M^T x = b for device vectors, with the current Jacobian; FP64 matrix values.

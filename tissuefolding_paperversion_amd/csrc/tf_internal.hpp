@@ -159,9 +159,10 @@ struct tf_handle {
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
 
     // graphs
-    hipGraphExec_t krylov_graph = nullptr;
-    int krylov_graph_iters = 0;
-    int krylov_graph_kind = -1;
+    // one captured Krylov chunk per Jacobian value type (FP64, FP32)
+    hipGraphExec_t krylov_graph[2] = {nullptr, nullptr};
+    int krylov_graph_iters[2] = {0, 0};
+    int krylov_graph_kind[2] = {-1, -1};
 
     // timing
     KernelTimer t_elem, t_gather, t_spmv;
@@ -180,6 +181,7 @@ void launch_seg_norm(tf_handle* h, const double* x);   // per-segment ||x||_2 in
 void launch_update_uk(tf_handle* h);                    // uk += du on active segments
 void build_tiles(tf_handle* h, int wg, int tile_cap);  // (re)partitions rows into tiles (tf_api.hip)
 bool tiling_fits(const tf_handle* h, int wg, int tile_cap);  // LDS of the single-kernel CG fits
+void drop_krylov_graphs(tf_handle* h);  // destroys the captured Krylov chunks
 void build_pattern(tf_handle* h);  // BSR pattern, scatter lists and bound-DoF fold lists (tf_api.hip)
 void launch_bind_matrix(tf_handle* h);    // M <- N^T M N + I_slaves on the fold targets
 void launch_bind_residual(tf_handle* h);  // g <- N^T g

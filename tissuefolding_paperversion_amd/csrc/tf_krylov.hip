@@ -1019,21 +1019,23 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     while (true) {
         if (graph) {
             const int key = kind * 16 + (int)sizeof(VT) * 2;
-            if (!h->krylov_graph || h->krylov_graph_iters != chunk || h->krylov_graph_kind != key) {
-                if (h->krylov_graph) { (void)hipGraphExecDestroy(h->krylov_graph); h->krylov_graph = nullptr; }
+            const int gi = sizeof(VT) == sizeof(double) ? 0 : 1;
+            hipGraphExec_t& graph_exec = h->krylov_graph[gi];
+            if (!graph_exec || h->krylov_graph_iters[gi] != chunk || h->krylov_graph_kind[gi] != key) {
+                if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
                 hipGraph_t gr;
                 TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
                 h->t_spmv.capturing = true;
                 enqueue_chunk<VT>(h, val, kind, chunk);
                 h->t_spmv.capturing = false;
                 TF_HIP(hipStreamEndCapture(h->stream, &gr));
-                TF_HIP(hipGraphInstantiate(&h->krylov_graph, gr, nullptr, nullptr, 0));
+                TF_HIP(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
                 TF_HIP(hipGraphDestroy(gr));
-                h->krylov_graph_iters = chunk;
-                h->krylov_graph_kind = key;
+                h->krylov_graph_iters[gi] = chunk;
+                h->krylov_graph_kind[gi] = key;
             }
             if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[0], h->stream));
-            TF_HIP(hipGraphLaunch(h->krylov_graph, h->stream));
+            TF_HIP(hipGraphLaunch(graph_exec, h->stream));
             if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[1], h->stream));
         } else {
             enqueue_chunk<VT>(h, val, kind, chunk);
