@@ -80,6 +80,7 @@ typedef struct tf_solver_opts {
     int32_t krylov_forcing;    /* 1: inside the NR loop each linear solve stops at the accuracy
                                   the next Newton update needs (inexact Newton, per tissue);
                                   0: every solve to krylov_rtol                                 */
+    int32_t xcd_tile_map;      /* 1: each XCD runs a contiguous run of Krylov row tiles         */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

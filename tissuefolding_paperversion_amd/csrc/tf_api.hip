@@ -461,6 +461,7 @@ void tf_default_options(tf_solver_opts* o)
     o->growth_rotation = 1;
     o->use_graph = 1;
     o->krylov_forcing = 1;
+    o->xcd_tile_map = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -481,6 +482,8 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->N = N; h->E = E; h->S = S;
         TF_HIP(hipGetDevice(&h->device));
         TF_HIP(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+        if (hipDeviceGetAttribute(&h->n_xcd, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || h->n_xcd < 1)
+            h->n_xcd = 1;
         TF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         h->h_conn.assign(d->conn, d->conn + (size_t)6 * E);
         h->h_node_seg_ptr.resize(S + 1);

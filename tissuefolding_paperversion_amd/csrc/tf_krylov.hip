@@ -34,7 +34,18 @@ struct TileArgs {
     double* partial;
     SegScalars* seg;
     int T;
+    int nxcd;  // XCDs the dispatcher deals workgroups to round-robin (1: identity tile map)
 };
+
+// Workgroup b runs on XCD b % nxcd.  Giving XCD x a contiguous run of tiles keeps a tile's
+// neighbouring block columns -- mostly in the adjacent tiles -- in the same XCD's L2 instead of
+// every XCD fetching nearly all of the gathered vectors.  A bijection of [0, T).
+__device__ inline int xcd_tile(int b, int T, int nxcd)
+{
+    if (nxcd <= 1) return b;
+    const int x = b % nxcd, k = b / nxcd, q = T / nxcd, rem = T % nxcd;
+    return x * q + (x < rem ? x : rem) + k;
+}
 
 __device__ inline double wave_sum(double v)
 {
@@ -526,7 +537,7 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
                                              int tile_cap)
 {
     extern __shared__ double prod[];
-    const int t = blockIdx.x;
+    const int t = xcd_tile(blockIdx.x, ta.T, ta.nxcd);
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
@@ -732,7 +743,7 @@ __global__ __launch_bounds__(WGT) void k_pipe(TileArgs ta, const int32_t* __rest
                                               int tile_cap)
 {
     extern __shared__ double prod[];
-    const int t = blockIdx.x;
+    const int t = xcd_tile(blockIdx.x, ta.T, ta.nxcd);
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
@@ -860,7 +871,8 @@ __global__ __launch_bounds__(WG) void k_seg_norm_fin(TileArgs ta, double* __rest
 
 TileArgs tile_args(tf_handle* h)
 {
-    return TileArgs{h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->partial, h->seg, h->T};
+    return TileArgs{h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->partial, h->seg, h->T,
+                    h->opts.xcd_tile_map ? h->n_xcd : 1};
 }
 
 template <typename VT, int P>
