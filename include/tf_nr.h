@@ -201,11 +201,12 @@ int tf_adjoint_get(tf_handle* h, double* rate_bar, double* lambda_bar, double* m
 
 /* ---- tuning (no reference counterpart: schedule choices of this implementation) ----------------- */
 /* Row tiling of the Krylov kernels: workgroup size wg in {256, 512, 1024} and tile capacity
- * wg * {1, 2, 4} <= 2048 blocks. */
+ * 1 <= tile_cap <= min(4 wg, 2048) blocks (a tile never holds more than wg/3 block rows). */
 int tf_set_tiling(tf_handle* h, int32_t wg, int32_t tile_cap);
 int tf_get_tiling(const tf_handle* h, int32_t* wg, int32_t* tile_cap, int32_t* n_tiles);
 /* Times n_iters single-kernel CG iterations on the Jacobian at the current positions for every
- * supported tiling, keeps the fastest; per-variant results (up to 16) are written to the arrays. */
+ * candidate tiling (power-of-two capacities, and capacities that cut the rows into 1-4 tiles per
+ * compute unit), keeps the fastest; per-variant results (up to 32) are written to the arrays. */
 int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, int32_t* wg, int32_t* tile_cap,
                 double* ms_per_iter);
 

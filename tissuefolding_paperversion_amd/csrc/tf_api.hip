@@ -1,5 +1,6 @@
 // tf_api.hip -- C ABI of the MI355X Newton-Raphson step: set-up, state transfer, NR driver.
 #include "tf_internal.hpp"
+#include <array>
 
 #include <algorithm>
 #include <chrono>
@@ -168,8 +169,8 @@ bool tiling_fits(const tf_handle* h, int wg, int tile_cap)
 
 void build_tiles(tf_handle* h, int wg, int tile_cap)
 {
-    if (!(wg == 256 || wg == 512 || wg == 1024) || tile_cap % wg || tile_cap > TF_TILE_CAP_MAX || tile_cap / wg > 4)
-        throw std::runtime_error("unsupported tiling: wg in {256, 512, 1024}, tile_cap = wg * {1, 2, 4} <= 2048");
+    if (!(wg == 256 || wg == 512 || wg == 1024) || tile_cap < 1 || tile_cap > TF_TILE_CAP_MAX || tile_cap > 4 * wg)
+        throw std::runtime_error("unsupported tiling: wg in {256, 512, 1024}, 1 <= tile_cap <= min(4 wg, 2048)");
     if (!tiling_fits(h, wg, tile_cap))
         throw std::runtime_error("tiling needs more LDS than a workgroup may allocate on this device");
     const std::vector<int32_t>& row_ptr = h->h_row_ptr;
@@ -482,6 +483,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->N = N; h->E = E; h->S = S;
         TF_HIP(hipGetDevice(&h->device));
         TF_HIP(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+        TF_HIP(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
         if (hipDeviceGetAttribute(&h->n_xcd, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || h->n_xcd < 1)
             h->n_xcd = 1;
         TF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -1363,14 +1365,25 @@ int tf_get_tiling(const tf_handle* h, int32_t* wg, int32_t* tile_cap, int32_t* n
     return TF_OK;
 }
 
+#define TF_AUTOTUNE_MAX 32
 int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, int32_t* wg_out, int32_t* cap_out,
                 double* ms_out)
 {
     if (!h || !(dt > 0) || n_iters < 2) return TF_INVALID;
     TF_GUARD_BEGIN
-    static const int variants[][2] = {{256, 256}, {256, 512}, {256, 1024}, {512, 512},
-                                      {512, 1024}, {512, 2048}, {1024, 1024}, {1024, 2048}};
-    const int nv = (int)(sizeof(variants) / sizeof(variants[0]));
+    // power-of-two capacities, plus "balanced" capacities that cut the rows into about
+    // k x (compute units) tiles, so that every CU holds the same number of tiles
+    std::vector<std::array<int, 2>> variants = {{256, 256}, {256, 512}, {256, 1024}, {512, 512},
+                                                {512, 1024}, {512, 2048}, {1024, 1024}, {1024, 2048}};
+    int max_row = 1;
+    for (int r = 0; r < h->N; ++r) max_row = std::max(max_row, h->h_row_ptr[r + 1] - h->h_row_ptr[r]);
+    for (int wg : {256, 512})
+        for (int k = 1; k <= 4; ++k) {
+            const int cap = (int)((h->nb + (int64_t)k * h->n_cu - 1) / ((int64_t)k * h->n_cu)) + max_row / 2;
+            if (cap >= max_row && cap <= std::min(4 * wg, TF_TILE_CAP_MAX) && cap % wg)
+                variants.push_back({wg, cap});
+        }
+    const int nv = std::min((int)variants.size(), TF_AUTOTUNE_MAX);
     // a representative system: the Jacobian at the current positions
     launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
     hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);

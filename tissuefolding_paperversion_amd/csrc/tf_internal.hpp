@@ -68,6 +68,7 @@ struct tf_handle {
     int32_t wg = 256, tile_cap = 512;  // single-kernel CG workgroup size and tile capacity (blocks)
     int32_t max_lds = 65536;           // bytes of LDS one workgroup may allocate (device attribute)
     int32_t n_xcd = 1;                 // accelerator complex dies (XCDs) of the device
+    int32_t n_cu = 256;                // compute units of the device
     // options
     tf_solver_opts opts;
     std::string err;

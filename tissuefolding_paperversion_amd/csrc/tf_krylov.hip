@@ -928,7 +928,7 @@ void enqueue_cg1_iter(tf_handle* h, const VT* val)
     hipLaunchKernelGGL((k_cg1<VT, P, WGT, BPT>), dim3(h->T), dim3(WGT), lds, h->stream, ta, h->row_ptr, h->col, val, \
                        h->nb, h->dinv, h->kx, h->kp, h->kr, h->kv, h->ks, U[1 - P], U[P], Mv[1 - P], Mv[P], Q[1 - P],  \
                        Q[P], h->opts.krylov_max_iter, h->tile_cap)
-    const int bpt = h->tile_cap / h->wg;
+    const int bpt = h->tile_cap <= h->wg ? 1 : (h->tile_cap <= 2 * h->wg ? 2 : 4);
     if (h->wg == 512) {
         if (bpt == 1) TF_CG1(512, 1); else if (bpt == 2) TF_CG1(512, 2); else TF_CG1(512, 4);
     } else if (h->wg == 1024) {
@@ -954,7 +954,7 @@ void enqueue_pipe_iter(tf_handle* h, const VT* val)
     hipLaunchKernelGGL((k_pipe<VT, P, WGT, BPT>), dim3(h->T), dim3(WGT), lds, h->stream, ta, h->row_ptr, h->col, val, \
                        h->nb, h->dinv, h->kx, h->kr, h->kr0, h->kv, h->kt, h->ksh, h->ks, h->kp, Mb[1 - P], Mb[P],       \
                        h->opts.krylov_max_iter, h->tile_cap)
-    const int bpt = h->tile_cap / h->wg;
+    const int bpt = h->tile_cap <= h->wg ? 1 : (h->tile_cap <= 2 * h->wg ? 2 : 4);
     if (h->wg == 512) {
         if (bpt == 1) TF_PIPE(512, 1); else if (bpt == 2) TF_PIPE(512, 2); else TF_PIPE(512, 4);
     } else if (h->wg == 1024) {

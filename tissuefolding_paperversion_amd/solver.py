@@ -285,7 +285,7 @@ class NewtonRaphsonSolver:
     def autotune(self, dt=60.0, n_iters=64):
         """Time the single-kernel CG under every supported tiling and keep the fastest."""
         n = C.c_int32()
-        wg, cap, ms = np.zeros(16, np.int32), np.zeros(16, np.int32), np.zeros(16)
+        wg, cap, ms = np.zeros(32, np.int32), np.zeros(32, np.int32), np.zeros(32)
         self._check(self._lib.tf_autotune(self.h, dt, int(n_iters), C.byref(n), L.ptr(wg), L.ptr(cap), L.ptr(ms)))
         k = n.value
         return [dict(wg=int(wg[i]), tile_cap=int(cap[i]), ms_per_iter=float(ms[i])) for i in range(k)]
