@@ -1,4 +1,5 @@
-"""Copies the reference's small input meshes (data files) into tests/golden/.
+"""Copies the reference's small input meshes and its sample model input (data files) into
+tests/golden/ (the sample model input as ``modelinput_sample``).
 
 Run in a container where the reference tree is mounted:
     python tests/golden/make_fixtures.py /root/reference
@@ -17,3 +18,6 @@ if __name__ == "__main__":
     for m in MESHES:
         shutil.copyfile(os.path.join(src, m), os.path.join(here, m))
         print("copied", m)
+    shutil.copyfile(os.path.join(ref, "TissueFolding", "TissueFolding", "modelinput"),
+                    os.path.join(here, "modelinput_sample"))
+    print("copied modelinput -> modelinput_sample")
