@@ -81,6 +81,8 @@ typedef struct tf_solver_opts {
                                   the next Newton update needs (inexact Newton, per tissue);
                                   0: every solve to krylov_rtol                                 */
     int32_t xcd_tile_map;      /* 1: each XCD runs a contiguous run of Krylov row tiles         */
+    int32_t krylov_warm_start; /* 1: the solves of NR iterations 0 and 1 start from the same
+                                  iteration's update of the previous timestep                  */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

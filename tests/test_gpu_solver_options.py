@@ -1,0 +1,63 @@
+"""Inexact-Newton forcing, Krylov warm starts and the XCD tile map change how the linear
+systems are solved, not the Newton iterates they produce: against exact solves from zero the NR
+iteration counts must be equal and the positions within 1e-10 relative, with fewer Krylov
+iterations."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from tissuefolding_paperversion_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(tissue, **opts):
+    from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
+    return NewtonRaphsonSolver(tissue, opts or None)
+
+
+def _ellipse():
+    t = configs.wing_disc_tissue(a=60.0, b=36.0, h=3.66, z_levels=(0.0, 2.0, 7.0, 12.0), seed=2, growth_seed=3,
+                                 name="options ellipse")
+    return t
+
+
+def test_forcing_and_warm_start_keep_the_newton_iterates():
+    t = _ellipse()
+    exact = _solver(t, krylov_forcing=0, krylov_warm_start=0)
+    fast = _solver(t)
+    assert fast.options()["krylov_forcing"] == 1 and fast.options()["krylov_warm_start"] == 1
+    kr_e = kr_f = 0
+    for step in range(6):
+        a, b = exact.runOneStep(t.dt), fast.runOneStep(t.dt)
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert a["nr_iterations"] == b["nr_iterations"]
+        if step >= 2:
+            kr_e += a["krylov_iterations"]
+            kr_f += b["krylov_iterations"]
+        assert rel_err(fast.positions, exact.positions) < 1e-10
+    assert kr_f < 0.8 * kr_e, (kr_f, kr_e)
+
+
+def test_warm_start_alone():
+    t = _ellipse()
+    cold = _solver(t, krylov_forcing=0, krylov_warm_start=0)
+    warm = _solver(t, krylov_forcing=0, krylov_warm_start=1)
+    kc = kw = 0
+    for step in range(5):
+        a, b = cold.runOneStep(t.dt), warm.runOneStep(t.dt)
+        assert a["nr_iterations"] == b["nr_iterations"]
+        if step >= 1:
+            kc += a["krylov_iterations"]
+            kw += b["krylov_iterations"]
+    assert rel_err(warm.positions, cold.positions) < 1e-10
+    assert kw < kc, (kw, kc)
+
+
+def test_xcd_tile_map_is_bitwise_neutral():
+    t = _ellipse()
+    a, b = _solver(t, xcd_tile_map=0), _solver(t, xcd_tile_map=1)
+    for _ in range(2):
+        a.runOneStep(t.dt)
+        b.runOneStep(t.dt)
+    assert np.array_equal(a.positions, b.positions)

@@ -48,7 +48,7 @@ class SolverOpts(C.Structure):
                 ("krylov_atol", C.c_double), ("gmres_restart", C.c_int32), ("iters_per_sync", C.c_int32),
                 ("nr_tol", C.c_double), ("nr_max_iter", C.c_int32), ("tangent_precision", C.c_int32),
                 ("matrix_precision", C.c_int32), ("growth_rotation", C.c_int32), ("use_graph", C.c_int32),
-                ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32)]
+                ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32)]
 
 
 class StepStats(C.Structure):

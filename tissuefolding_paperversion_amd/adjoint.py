@@ -37,9 +37,10 @@ class GrowthRollout:
         self.layer = np.asarray(tissue.mesh.layer, np.int64)
         self.n_layers = int(self.layer.max()) + 1
         self.nu = np.asarray(tissue.nu, float)
-        # exact linear solves: with inexact-Newton forcing the tolerances depend on the NR history,
-        # and a window re-run from its checkpoint would not reproduce the forward states bit for bit
-        self.solver = NewtonRaphsonSolver(tissue, dict({"krylov_forcing": 0}, **(opts or {})))
+        # exact linear solves from zero: with inexact-Newton forcing or warm starts the solves
+        # depend on the NR history, and a window re-run from its checkpoint would not reproduce the
+        # forward states bit for bit
+        self.solver = NewtonRaphsonSolver(tissue, dict({"krylov_forcing": 0, "krylov_warm_start": 0}, **(opts or {})))
         self.x0 = tissue.mesh.pos.ravel().copy()
         E = tissue.n_elements
         self.eye = np.tile(np.eye(3), (E, 1, 1))

@@ -80,7 +80,7 @@ void free_all(tf_handle* h)
                     h->val_base, h->bind_nodes, h->bind_m_ptr, h->bind_m_dof, h->bind_s_dof, h->bind_pairs,
                     h->ablated, h->blk_tr, h->rot_tz, h->val_t, h->g_save, h->adj_xbar, h->adj_rhs, h->adj_pass,
                     h->adj_y, h->adj_un_bar, h->adj_invfg_bar, h->adj_fg_bar, h->adj_fg_prev, h->adj_rate_bar,
-                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz, h->seg_rtol2};
+                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz, h->seg_rtol2, h->warm[0], h->warm[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
@@ -463,6 +463,7 @@ void tf_default_options(tf_solver_opts* o)
     o->use_graph = 1;
     o->krylov_forcing = 1;
     o->xcd_tile_map = 1;
+    o->krylov_warm_start = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -560,6 +561,8 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg = dev_alloc<SegScalars>(S);
         h->seg_norm = dev_alloc<double>(S);
         h->seg_rtol2 = dev_alloc<double>(S);
+        h->warm[0] = dev_alloc<double>((size_t)3 * N);
+        h->warm[1] = dev_alloc<double>((size_t)3 * N);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
         for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx, &h->km, &h->kq})
@@ -1016,14 +1019,22 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         auto ts = std::chrono::steady_clock::now();
         h->seg_rtol_on = h->opts.krylov_forcing != 0;
         if (h->seg_rtol_on) forcing_tolerances(h, it);
+        const bool warm = h->opts.krylov_warm_start != 0 && it < 2;
+        h->warm_guess = (warm && h->warm_valid[it]) ? h->warm[it] : nullptr;
         int kst;
         try {
             kst = solve_impl(h);
         } catch (...) {
             h->seg_rtol_on = false;
+            h->warm_guess = nullptr;
             throw;
         }
         h->seg_rtol_on = false;
+        h->warm_guess = nullptr;
+        if (warm) {  // keep this update as the guess for the next timestep's iteration it
+            launch_copy(h, h->warm[it], h->du, 3 * (int64_t)h->N);
+            h->warm_valid[it] = kst == TF_OK;
+        }
         krylov_total += h->last_krylov_iters;
         int ust = update_uk_impl(h, it);
         if (h->opts.krylov_forcing) forcing_observe(h, it);

@@ -158,6 +158,11 @@ struct tf_handle {
     std::vector<double> fc_C, fc_du0, fc_du_prev, h_seg_rtol2;
     double *seg_rtol2 = nullptr;      // [S]
     bool seg_rtol_on = false;
+    // Krylov warm start: the update of Newton iterations 0 and 1 of the previous timestep, used
+    // as the initial guess of the same iteration's solve (warm_guess points at it during a solve)
+    double *warm[2] = {nullptr, nullptr};
+    bool warm_valid[2] = {false, false};
+    const double *warm_guess = nullptr;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
 
     // graphs

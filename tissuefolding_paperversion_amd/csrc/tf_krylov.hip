@@ -519,6 +519,81 @@ __global__ __launch_bounds__(WG) void k_cg1_init(TileArgs ta, const int32_t* __r
     tile_write<4>(ta, t, sl, vv);
 }
 
+// Warm start of the single-kernel CG from a guess x0 (the same Newton iteration's update of the
+// previous timestep), in two launches: (a) x = x0, r0 = b - M x0, u0 = D^-1 r0; (b) w0 = M u0,
+// m0 = D^-1 w0 and the first partials.  Tissues not being solved start from x = 0, r = 0.
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_cg1_warm_a(TileArgs ta, const int32_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
+                                                   const double* __restrict__ dinv, const double* __restrict__ b,
+                                                   const double* __restrict__ x0, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ s, double* __restrict__ p,
+                                                   double* __restrict__ u0, double* __restrict__ q0, int tile_cap)
+{
+    extern __shared__ double prod[];
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    const bool active = ta.seg[sg].active;
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    double* rloc = prod + 3 * (size_t)tile_cap;
+    tile_products<VT>(col, val, nb, b0, b1, XLOAD(x0), prod);
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, rw = d / 3, comp = d % 3;
+        const double mx = row_sum(row_ptr, rw, comp, b0, prod);
+        const double rv = active ? b[d] - mx : 0.0;
+        x[d] = active ? x0[d] : 0.0;
+        r[d] = rv; s[d] = 0.0; p[d] = 0.0; q0[d] = 0.0;
+        rloc[task] = rv;
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, nd = task - d % 3;
+        const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
+        u0[d] = D[0] * rloc[nd] + D[1] * rloc[nd + 1] + D[2] * rloc[nd + 2];
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_cg1_warm_b(TileArgs ta, const int32_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
+                                                   const double* __restrict__ dinv, const double* __restrict__ b,
+                                                   const double* __restrict__ r, const double* __restrict__ u0,
+                                                   double* __restrict__ w, double* __restrict__ m0, int tile_cap)
+{
+    extern __shared__ double prod[];
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    const bool active = ta.seg[sg].active;
+    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
+    const int d0 = 3 * rr0, d1 = 3 * rr1;
+    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
+    double* wloc = prod + 3 * (size_t)tile_cap;
+    tile_products<VT>(col, val, nb, b0, b1, XLOAD(u0), prod);
+    double g = 0.0, dl = 0.0, rr = 0.0, bb = 0.0;
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, rw = d / 3, comp = d % 3;
+        const double wv = row_sum(row_ptr, rw, comp, b0, prod);
+        w[d] = wv;
+        wloc[task] = wv;
+        const double rv = r[d], uv = u0[d], bv = active ? b[d] : 0.0;
+        g += rv * uv;
+        dl += wv * uv;
+        rr += rv * rv;
+        bb += bv * bv;
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < d1 - d0; task += WG) {
+        const int d = d0 + task, nd = task - d % 3;
+        const double* D = dinv + 9 * (d / 3) + 3 * (d % 3);
+        m0[d] = D[0] * wloc[nd] + D[1] * wloc[nd + 1] + D[2] * wloc[nd + 2];
+    }
+    const int sl[4] = {SL_RHO0, SL_DL0, SL_RR0, SL_BB};
+    double vv[4] = {g, dl, rr, bb};
+    tile_write<4>(ta, t, sl, vv);
+}
+
 // One iteration per launch.  Everything that does not depend on this iteration's scalars (the
 // tile's matrix blocks, the neighbour columns' u, m, q, the own rows' vectors and block-inverse
 // rows) is loaded into registers first, so the loads are in flight while the workgroup re-sums
@@ -1006,7 +1081,13 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         hipLaunchKernelGGL(k_pipe_init<VT>, grid, blk, sizeof(double) * (3 * (size_t)h->tile_cap + h->wg), h->stream, ta,
                            h->row_ptr, h->col, val, h->nb, h->dinv, h->g, h->kx, h->kr, h->kr0, h->kv, h->kt, h->ksh,
                            h->ks, h->kp, h->kph, h->tile_cap);
-    else if (kind == TF_KRYLOV_CG1)
+    else if (kind == TF_KRYLOV_CG1 && h->warm_guess) {
+        const size_t lds = sizeof(double) * (3 * (size_t)h->tile_cap + h->wg);
+        hipLaunchKernelGGL(k_cg1_warm_a<VT>, grid, blk, lds, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv, h->g,
+                           h->warm_guess, h->kx, h->kr, h->ks, h->kp, h->kph, h->kr0, h->tile_cap);
+        hipLaunchKernelGGL(k_cg1_warm_b<VT>, grid, blk, lds, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->dinv, h->g,
+                           h->kr, h->kph, h->kv, h->kt, h->tile_cap);
+    } else if (kind == TF_KRYLOV_CG1)
         hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, sizeof(double) * (3 * (size_t)h->tile_cap + h->wg), h->stream, ta,
                            h->row_ptr, h->col, val, h->nb, h->dinv, h->g, h->kx, h->kr, h->kv, h->ks, h->kp, h->kph,
                            h->kt, h->kr0, h->tile_cap);
