@@ -55,6 +55,8 @@ typedef struct tf_mesh_desc {
     int32_t n_segments;           /* independent tissues in this handle (>= 1)                  */
     const int32_t* node_seg_ptr;  /* [n_segments+1] contiguous node ranges, NULL if 1 segment   */
     const int32_t* elem_seg_ptr;  /* [n_segments+1] contiguous element ranges, NULL if 1 seg.   */
+    const int8_t* node_placement; /* [n_nodes] Node::tissuePlacement: 0 basal, 1 apical, 2 mid,  */
+                                  /*   3 lateral; NULL = corners of exposed caps (implicit packing) */
 } tf_mesh_desc;
 
 /* TF_KRYLOV_CG1: CG with one fused kernel per iteration (single-reduction recurrences of
@@ -83,6 +85,9 @@ typedef struct tf_solver_opts {
     int32_t xcd_tile_map;      /* 1: each XCD runs a contiguous run of Krylov row tiles         */
     int32_t krylov_warm_start; /* 1: the solves of NR iterations 0 and 1 start from the same
                                   iteration's update of the previous timestep                  */
+    int32_t implicit_packing;  /* 1: tf_timestep detects packing pairs after the growth update,
+                                  as Simulation::runOneStep does (Simulation.cpp:37, :5911-5920);
+                                  0: tf_timestep clears the pair list                          */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {
@@ -91,7 +96,7 @@ typedef struct tf_step_stats {
     int32_t converged;           /* 1 when every tissue met nr_tol                             */
     int32_t rotated_elements;    /* growth increments rotated this step                        */
     int32_t krylov_fallbacks;    /* linear solves that fell back from CG to BiCGSTAB           */
-    int32_t pad;
+    int32_t packing_pairs;       /* implicit-packing pairs in force during the step            */
     double last_du_norm;         /* max over tissues of the last ||dU||_2                      */
     double ms_grow, ms_assemble, ms_solve, ms_total;
 } tf_step_stats;
@@ -136,6 +141,32 @@ int tf_set_bound_dofs(tf_handle* h, int32_t n_pairs, const int32_t* pairs);
  * NewtonRaphsonSolver.cpp:350-361).  Replaces the previous list; n = 0 clears it. */
 int tf_set_ablated_nodes(tf_handle* h, int32_t n, const int32_t* nodes);
 
+/* ---- implicit packing ------------------------------------------------------------------------ */
+/* Node masses (Node::mass) from the current Fg: every prism adds det(Fg) V_ref / 6 to its nodes
+ * (Simulation::updateNodeMasses, Simulation.cpp:7552-7560; ShapeBase.cpp:1243-1245, :3938-3942;
+ * V_ref by Prism::calculateReferenceVolume, Prism.cpp:876-908).  Ablated nodes get 0. */
+int tf_get_node_masses(tf_handle* h, double* mass);
+/* Simulation::updateNodeMasses then Simulation::detectPacingNodes (Simulation.cpp:8167-8440, with
+ * getAverageSideLength :3205-3265 and calculateBoundingBox :9148-9189) on the current positions,
+ * for every tissue.  The pairs and their weights hold for every following assembly until the
+ * next detection or tf_set_packing_pairs: each assembly adds the sigmoid packing forces to g and
+ * their Jacobian to M (calculatePackingForcesImplicit3D, calculatePackingJacobian3D,
+ * Simulation.cpp:7149-7295, addPackingForces :8869-8906).  New pairs widen the Jacobian pattern.
+ * n_pairs may be NULL. */
+int tf_detect_packing(tf_handle* h, int32_t* n_pairs);
+/* Replaces the pair list (pacingNodeCouples0/1, initialWeightPointx/y/z): pairs[2k], pairs[2k+1]
+ * are nodes of one tissue, weights[3k..3k+2] the pair's unit weights, threshold[s] the packing
+ * threshold T of tissue s (packingThreshold), > 0 where a tissue has pairs.  Node masses are
+ * updated from the current Fg.  Every unordered pair may appear once.  n_pairs = 0 clears. */
+int tf_set_packing_pairs(tf_handle* h, int32_t n_pairs, const int32_t* pairs, const double* weights,
+                         const double* threshold);
+/* The pair list in force: n_pairs always; pairs [2*n], weights [3*n], threshold [n_segments] and
+ * detect_grid [50*n_segments] (the 10 x 5 detection thresholds of each tissue, row-major) when not
+ * NULL.  Thresholds and grid are those of the last detection (threshold: or of
+ * tf_set_packing_pairs). */
+int tf_get_packing_pairs(tf_handle* h, int32_t* n_pairs, int32_t* pairs, double* weights, double* threshold,
+                         double* detect_grid);
+
 /* ---- per-step set-up ----------------------------------------------------------------------- */
 /* Simulation::updateGrowthRotationMatrices + calculateGrowth + ShapeBase::growShapeByFg
  * (Simulation.cpp:5876-5909, :9651-9799; ShapeBase.cpp:396-421, :1215-1251, :1656-1811). */
@@ -168,8 +199,8 @@ int tf_solve(tf_handle* h, double* dU, int32_t* krylov_iters, double* rel_residu
 int tf_update_uk(tf_handle* h, double* du_norm, int32_t* converged);
 /* The whole NR loop of one step (Simulation::updateStepNR). */
 int tf_nr_step(tf_handle* h, double dt, tf_step_stats* stats);
-/* Growth + viscosity surfaces + NR loop (the NR-adjacent part of Simulation::runOneStep,
- * Simulation.cpp:5820-5974). */
+/* Growth + viscosity surfaces + node masses and packing detection (when implicit_packing) + NR
+ * loop (the NR-adjacent part of Simulation::runOneStep, Simulation.cpp:5820-5974). */
 int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats);
 /* Per-tissue results of the last NR step: iterations, last ||dU||, converged flag. */
 int tf_get_segment_stats(tf_handle* h, int32_t* nr_iters, double* du_norm, int32_t* converged);

@@ -28,6 +28,7 @@ dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 bp = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
 
 
 class _Model(C.Structure):
@@ -72,6 +73,13 @@ def lib():
         L.tfo_inv3.argtypes = [dp, dp]
         L.tfo_det3.argtypes = [dp]
         L.tfo_det3.restype = C.c_double
+        L.tfo_reference_volume.argtypes = [dp]
+        L.tfo_reference_volume.restype = C.c_double
+        L.tfo_node_masses.argtypes = [C.c_int, C.c_int, ip, dp, dp, C.c_void_p, dp]
+        L.tfo_packing_thresholds.argtypes = [C.c_int, C.c_int, ip, bp, dp, C.POINTER(C.c_double), dp, dp]
+        L.tfo_packing_detect.argtypes = [C.c_int, dp, i8p, dp, ip, ip, dp, dp, C.c_int, ip, dp]
+        L.tfo_packing_detect.restype = C.c_int
+        L.tfo_packing_forces.argtypes = [C.c_int, ip, dp, dp, dp, C.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -139,6 +147,72 @@ def node_pattern(n_nodes, conn):
     return np.cumsum(row_ptr), cols
 
 
+# --------------------------------------------------------------------------------------------
+# implicit packing
+# --------------------------------------------------------------------------------------------
+def reference_volume(X):
+    """Prism::calculateReferenceVolume (Prism.cpp:876-908): eight centroid tetrahedra."""
+    return lib().tfo_reference_volume(_c(X).ravel())
+
+
+def node_masses(n_nodes, conn, Xref, Fg, ablated=None):
+    """Simulation::updateNodeMasses (Simulation.cpp:7552-7560): det(Fg) V_ref / 6 per prism node."""
+    m = np.zeros(n_nodes)
+    ab = None
+    if ablated is not None and len(ablated):
+        ab = np.zeros(n_nodes, np.uint8)
+        ab[np.asarray(ablated, int)] = 1
+    conn = _c(conn, np.int32)
+    lib().tfo_node_masses(n_nodes, conn.shape[0], conn.ravel(), _c(Xref).ravel(), _c(Fg).ravel(),
+                          None if ab is None else ab.ctypes.data, m)
+    return m
+
+
+def elem_basal_side(mesh):
+    """1 for elements whose basal side lengths count (basal placement or spanning the tissue)."""
+    from tissuefolding_paperversion_amd.mesh import BASAL
+    pl, spans = mesh.element_placement()
+    return ((pl == BASAL) | spans).astype(np.uint8)
+
+
+def packing_thresholds(conn, basal_side, pos):
+    """(T, grid (10, 5), bb) of Simulation::getAverageSideLength / detectPacingNodes."""
+    conn = _c(conn, np.int32)
+    T = C.c_double(0.0)
+    grid, bb = np.zeros(50), np.zeros(4)
+    pos = _c(pos).ravel()
+    lib().tfo_packing_thresholds(pos.size // 3, conn.shape[0], conn.ravel(), _c(basal_side, np.uint8), pos,
+                                 C.byref(T), grid, bb)
+    return T.value, grid.reshape(10, 5), bb
+
+
+def packing_detect(pos, placement, mass, adj, grid, bb):
+    """Simulation::detectPacingNodes pair scan: (pairs (P, 2) int32, weights (P, 3))."""
+    pos = _c(pos).ravel()
+    n = pos.size // 3
+    rp, cols = adj
+    args = (n, pos, _c(placement, np.int8), _c(mass), _c(rp, np.int32), _c(cols, np.int32), _c(grid).ravel(),
+            _c(bb))
+    cap = 1024
+    while True:
+        pairs, w = np.zeros(2 * cap, np.int32), np.zeros(3 * cap)
+        k = lib().tfo_packing_detect(*args, cap, pairs, w)
+        if k <= cap:
+            return pairs[:2 * k].reshape(-1, 2), w[:3 * k].reshape(-1, 3)
+        cap = k
+
+
+def packing_forces(pairs, weights, mass, pos, T):
+    """(F (3N,), D (P, 3)) of calculatePackingForcesImplicit3D / calculatePackingJacobian3D."""
+    pos = _c(pos).ravel()
+    F = np.zeros(pos.size)
+    P = len(pairs)
+    D = np.zeros(3 * P)
+    if P:
+        lib().tfo_packing_forces(P, _c(pairs, np.int32).ravel(), _c(weights).ravel(), _c(mass), pos, T, F, D)
+    return F, D.reshape(-1, 3)
+
+
 def scalar_csr_pattern(n_nodes, conn):
     """Scalar (3N) CSR pattern expanded from the node pattern, columns sorted."""
     rp, cols = node_pattern(n_nodes, conn)
@@ -166,7 +240,7 @@ def _pardiso_safe():
 class OracleTissue:
     """NR state of one tissue on the CPU, in the reference's order and orientation."""
 
-    def __init__(self, tissue, solver="auto", allow_rotation=True):
+    def __init__(self, tissue, solver="auto", allow_rotation=True, packing=True):
         m = tissue.mesh
         self.t = tissue
         self.N, self.E = m.n_nodes, m.n_elements
@@ -189,6 +263,16 @@ class OracleTissue:
         self.stats = []
         self.bound = []      # slaveMasterList: (slave DoF, master DoF) pairs in binding order
         self.ablated = []    # AblatedNodes
+        # implicit packing (Simulation.cpp:37): pair list of the current step
+        self.packing = packing
+        self.placement = _c(m.node_placement, np.int8)
+        self.basal_side = elem_basal_side(m)
+        self.adj = node_pattern(self.N, self.conn)
+        self.mass = np.zeros(self.N)
+        self.pairs = np.zeros((0, 2), np.int32)
+        self.pw = np.zeros((0, 3))
+        self.pT = 0.0
+        self.pgrid = np.zeros((10, 5))
 
     def _model(self):
         mm = _Model()
@@ -207,6 +291,36 @@ class OracleTissue:
 
     def update_viscosity_surfaces(self):
         lib().tfo_viscosity_surfaces(self.N, self.E, self.conn, self.exposed, self.pos, self.Sa)
+
+    def detect_packing(self):
+        """updateNodeMasses then detectPacingNodes on the current positions; returns the pair count."""
+        self.mass = node_masses(self.N, self.conn, self.Xref, self.Fg, self.ablated)
+        self.pT, self.pgrid, bb = packing_thresholds(self.conn, self.basal_side, self.pos)
+        self.pairs, self.pw = packing_detect(self.pos, self.placement, self.mass, self.adj, self.pgrid, bb)
+        return len(self.pairs)
+
+    def add_packing(self, K, g, uk):
+        """Packing forces into g and their Jacobian into K (Simulation::updateStepNR adds both
+        before the ablated-node check, fixed DoFs and binding; the fixed-DoF rule overwrites
+        fixed rows and columns, so entries there are left out)."""
+        if not len(self.pairs):
+            return K
+        F, D = packing_forces(self.pairs, self.pw, self.mass, uk, self.pT)
+        free = self.fixed == 0
+        g[free] += F[free]
+        rows, cols, vals = [], [], []
+        for p, (a, b) in enumerate(self.pairs):
+            for k in range(3):
+                da, db = 3 * a + k, 3 * b + k
+                for r, c_, v in ((da, da, -D[p, k]), (da, db, D[p, k]), (db, db, -D[p, k]), (db, da, D[p, k])):
+                    if free[r] and free[c_]:
+                        rows.append(r); cols.append(c_); vals.append(v)
+        if isinstance(K, np.ndarray):
+            np.add.at(K, (np.array(rows, int), np.array(cols, int)), np.array(vals))
+            return K
+        import scipy.sparse as sp
+        n = 3 * self.N
+        return (K + sp.csr_matrix((vals, (rows, cols)), shape=(n, n))).tocsr()
 
     # -- NR pieces ---------------------------------------------------------------------------
     def elements(self, uk, un, dt, want_K=True):
@@ -253,6 +367,7 @@ class OracleTissue:
             import scipy.sparse as sp
             rp, col, val = self.csr_K(dt, Ke, Kv)
             K = sp.csr_matrix((val, col, rp), shape=(3 * self.N, 3 * self.N))
+        K = self.add_packing(K, g, uk)
         if self.ablated or self.bound:
             if not isinstance(K, np.ndarray):
                 raise NotImplementedError("ablated nodes and bound DoFs use the dense oracle")
@@ -345,10 +460,14 @@ class OracleTissue:
         return it, d
 
     def timestep(self, dt, tol=1e-8, max_iter=20, dense=None):
-        """Growth (with the rotation rule), viscosity surfaces, then the NR loop."""
+        """Growth (with the rotation rule), viscosity surfaces, node masses and packing detection
+        (when packing is on, as in Simulation::runOneStep, Simulation.cpp:5902-5920), then the NR loop."""
         self.grow(dt)
         self.update_viscosity_surfaces()
         t0 = time.perf_counter()
+        if self.packing:
+            self.detect_packing()
         it, d = self.nr_step(dt, tol, max_iter, dense)
-        self.stats.append(dict(nr_iters=it, dU=d, seconds=time.perf_counter() - t0, rotations=self.last_rotations))
+        self.stats.append(dict(nr_iters=it, dU=d, seconds=time.perf_counter() - t0, rotations=self.last_rotations,
+                               packing_pairs=len(self.pairs)))
         return it, d

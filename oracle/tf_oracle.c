@@ -893,3 +893,227 @@ double tfo_element_energy(const double X[18], const double x[18], const double I
     }
     return Pi;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Implicit packing: node masses, pair detection, forces and Jacobian        */
+/* ------------------------------------------------------------------------- */
+#define PK_MULTIPLIER 5000.0  /* packingMultiplier (Simulation.cpp:8187)          */
+#define PK_SATURATION 5.0     /* sigmoidSaturationForPacking (Simulation.cpp:8188) */
+
+/* Prism::calculateReferenceVolume (Prism.cpp:876-908) with
+ * ShapeBase::calculateVolumeForInputShapeStructure (ShapeBase.cpp:3362-3397): eight triangles
+ * (the two caps, each lateral quad split in two) each close a tetrahedron with the centroid;
+ * volume = sum of |height| * area / 3. */
+double tfo_reference_volume(const double X[18])
+{
+    static const int tri[8][3] = {{0, 1, 2}, {3, 4, 5}, {0, 1, 3}, {1, 3, 4}, {1, 2, 4}, {2, 4, 5}, {0, 2, 3}, {2, 3, 5}};
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < NN; ++a)
+        for (int j = 0; j < 3; ++j) c[j] += X[3 * a + j];
+    for (int j = 0; j < 3; ++j) c[j] /= NN;
+    double V = 0.0;
+    for (int t = 0; t < 8; ++t) {
+        const double* p0 = X + 3 * tri[t][0];
+        const double* p1 = X + 3 * tri[t][1];
+        const double* p2 = X + 3 * tri[t][2];
+        double v1[3], v2[3], vm[3], b[3];
+        for (int j = 0; j < 3; ++j) {
+            v1[j] = p1[j] - p0[j];
+            v2[j] = p2[j] - p0[j];
+            vm[j] = c[j] - p0[j];
+        }
+        b[0] = v1[1] * v2[2] - v1[2] * v2[1];
+        b[1] = v1[2] * v2[0] - v1[0] * v2[2];
+        b[2] = v1[0] * v2[1] - v1[1] * v2[0];
+        double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        double area = nb / 2;
+        double height = (vm[0] * b[0] + vm[1] * b[1] + vm[2] * b[2]) / nb;
+        if (height < 0) height *= -1.0;
+        V += 1.0 / 3.0 * (height * area);
+    }
+    return V;
+}
+
+/* Simulation::updateNodeMasses (Simulation.cpp:7552-7560): every prism adds
+ * VolumePerNode = det(Fg) V_ref / 6 (ShapeBase.cpp:1243-1245, :3938-3942) to its six nodes, in
+ * element order.  Ablated nodes (node_ablated[n] != 0) get mass 0. */
+void tfo_node_masses(int n_nodes, int n_elem, const int32_t* conn, const double* Xref, const double* Fg,
+                     const uint8_t* node_ablated, double* mass)
+{
+    memset(mass, 0, (size_t)n_nodes * sizeof(double));
+    for (int e = 0; e < n_elem; ++e) {
+        double vpn = tfo_det3(Fg + 9 * e) * tfo_reference_volume(Xref + 18 * e) / NN;
+        for (int a = 0; a < NN; ++a) mass[conn[6 * e + a]] += vpn;
+    }
+    if (node_ablated)
+        for (int n = 0; n < n_nodes; ++n)
+            if (node_ablated[n]) mass[n] = 0.0;
+}
+
+/* Prism::getApicalSideLengthAverage / getBasalSideLengthAverage (Prism.cpp:1443-1490): mean
+ * length of the cap's three edges, zero-length edges left out (0 when all are). */
+static double side_length_average(const double* pos, const int32_t* c6, int apical)
+{
+    static const int pr[2][3][2] = {{{0, 1}, {0, 2}, {1, 2}}, {{3, 4}, {3, 5}, {4, 5}}};
+    double dsum = 0.0;
+    int counter = 0;
+    for (int i = 0; i < 3; ++i) {
+        const double* p = pos + 3 * c6[pr[apical][i][0]];
+        const double* q = pos + 3 * c6[pr[apical][i][1]];
+        double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > 0) {
+            dsum += pow(d2, 0.5);
+            counter++;
+        }
+    }
+    if (counter > 0) dsum /= counter;
+    return dsum;
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* Packing thresholds of one tissue, restating Simulation::calculateBoundingBox
+ * (Simulation.cpp:9148-9189), ShapeBase::calculateRelativePosInBoundingBox (ShapeBase.cpp:1842-1848)
+ * and Simulation::getAverageSideLength (Simulation.cpp:3205-3265) with the scaling of
+ * detectPacingNodes (:8175-8186), on the current positions:
+ *   - bounding box: "if (x < min) min = x; else if (x > max) max = x" from (1e5, -1e5), so a
+ *     node that lowers the running minimum is never compared with the maximum;
+ *   - every (non-ablated, columnar) element adds its apical side average, and its basal one when
+ *     the element is basal or spans the tissue, to the columnar mean (zero averages left out);
+ *   - grid bin (floor(rx), floor(ry / 2)) of the element centre's position relative to the box,
+ *     clamped to 10 x 5, sums the last of those values; the bin counters live in a 10 x 4 array
+ *     indexed [i][j] with j < 5, so bins (i, 4) and (i + 1, 0) share one counter (flat index
+ *     4 i + j; the reference's [9][4] lies past its array, here it is a counter of its own);
+ *   - T = 0.4 * columnar mean (packingThreshold), grid thresholds 0.4 * bin mean.
+ * elem_basal_side[e] = 1 when the element is basal or spans the tissue.  Outputs: T, grid[50]
+ * (row-major [10][5]), bb[4] = (min x, min y, size x, size y). */
+void tfo_packing_thresholds(int n_nodes, int n_elem, const int32_t* conn, const uint8_t* elem_basal_side,
+                            const double* pos, double* T, double grid[50], double bb[4])
+{
+    double bmin[3] = {100000.0, 100000.0, 100000.0}, bmax[3] = {-100000.0, -100000.0, -100000.0};
+    for (int n = 0; n < n_nodes; ++n)
+        for (int i = 0; i < 3; ++i) {
+            double p = pos[3 * n + i];
+            if (p < bmin[i]) bmin[i] = p;
+            else if (p > bmax[i]) bmax[i] = p;
+        }
+    double size[2] = {bmax[0] - bmin[0], bmax[1] - bmin[1]};
+    double sum[50];
+    double counter[41];
+    memset(sum, 0, sizeof(sum));
+    memset(counter, 0, sizeof(counter));
+    double dsumCol = 0.0;
+    int colCounter = 0;
+    for (int e = 0; e < n_elem; ++e) {
+        const int32_t* c6 = conn + 6 * e;
+        double cur = side_length_average(pos, c6, 1);
+        if (cur > 0) { dsumCol += cur; colCounter++; }
+        if (elem_basal_side[e]) {
+            cur = side_length_average(pos, c6, 0);
+            if (cur > 0) { dsumCol += cur; colCounter++; }
+        }
+        double cen[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < NN; ++a)
+            for (int j = 0; j < 3; ++j) cen[j] += pos[3 * c6[a] + j];
+        for (int j = 0; j < 3; ++j) cen[j] /= NN;
+        double rx = (cen[0] - bmin[0]) / size[0];
+        double ry = (cen[1] - bmin[1]) / size[1];
+        int relX = clampi((int)floor(rx), 0, 9);
+        int relY = clampi((int)floor(ry / 2.0), 0, 4);
+        sum[5 * relX + relY] += cur;
+        counter[4 * relX + relY] += 1.0;
+    }
+    *T = colCounter > 0 ? 0.4 * (dsumCol / (double)colCounter) : 0.0;
+    for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 5; ++j) {
+            double v = sum[5 * i + j];
+            if (counter[4 * i + j] > 0) v /= counter[4 * i + j];
+            grid[5 * i + j] = 0.4 * v;
+        }
+    bb[0] = bmin[0];
+    bb[1] = bmin[1];
+    bb[2] = size[0];
+    bb[3] = size[1];
+}
+
+/* Simulation::detectPacingNodes (Simulation.cpp:8167-8440) for one tissue, literally O(N^2):
+ *   - a node can pack when its mass is non-zero and it is apical or basal (Node::checkIfNodeHasPacking,
+ *     Node.cpp:133-151; there are no lateral elements here);
+ *   - master i (ascending) and slave j > i (ascending) pack when they have the same placement, do
+ *     not share an element (Node::immediateNeigs, ShapeBase.cpp:3105-3122; adj_ptr/adj_col hold
+ *     each node's sorted element neighbours) and |x_i - x_j|^2 < t^2, t the grid threshold of the
+ *     master's bin (floor((x - min x) / size x), floor((y - min y) / size y)), clamped to the grid;
+ *   - the node range is cut into nSeg = N / max(1, N / 16) chunks (integer division) and the pairs
+ *     found from masters in the last chunk are dropped (the merge loop stops at nSeg - 1);
+ *   - weights (dx, dy, dz) / d with (dx, dy, dz) = x_i - x_j (initialWeightPointx/y/z).
+ * Returns the number of pairs (written up to max_pairs). */
+int tfo_packing_detect(int n_nodes, const double* pos, const int8_t* placement, const double* mass,
+                       const int32_t* adj_ptr, const int32_t* adj_col, const double grid[50], const double bb[4],
+                       int max_pairs, int32_t* pairs, double* weights)
+{
+    int seg_size = n_nodes / 16;
+    if (seg_size < 1) seg_size = 1;
+    const int n_seg = n_nodes / seg_size;
+    const int master_end = (n_seg - 1) * seg_size;
+    int np = 0;
+    for (int i = 0; i < master_end; ++i) {
+        if (mass[i] == 0 || (placement[i] != 0 && placement[i] != 1)) continue;
+        const double* p = pos + 3 * i;
+        int gx = clampi((int)floor((p[0] - bb[0]) / bb[2]), 0, 9);
+        int gy = clampi((int)floor((p[1] - bb[1]) / bb[3]), 0, 4);
+        double t2 = grid[5 * gx + gy] * grid[5 * gx + gy];
+        for (int j = i + 1; j < n_nodes; ++j) {
+            if (mass[j] == 0 || placement[j] != placement[i]) continue;
+            int neig = 0;
+            for (int k = adj_ptr[i]; k < adj_ptr[i + 1]; ++k)
+                if (adj_col[k] == j) { neig = 1; break; }
+            if (neig) continue;
+            const double* q = pos + 3 * j;
+            double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < t2) {
+                if (np < max_pairs) {
+                    double d = pow(d2, 0.5);
+                    pairs[2 * np] = i;
+                    pairs[2 * np + 1] = j;
+                    weights[3 * np] = dx / d;
+                    weights[3 * np + 1] = dy / d;
+                    weights[3 * np + 2] = dz / d;
+                }
+                np++;
+            }
+        }
+    }
+    return np;
+}
+
+/* Simulation::calculatePackingForcesImplicit3D (Simulation.cpp:7149-7203) and
+ * calculatePackingJacobian3D (:7205-7295) at positions pos: per pair and component k,
+ *   d = x0_k - x1_k, negated when w_k > 0;  s = 1 / (1 + exp(-(S / T) d));
+ *   F_k = M * mbar * s * w_k  (mbar the pair's mean mass) added to node 0, subtracted from node 1;
+ *   D_k = dF_k / dx0_k = s (1 - s) M mbar w_k (S / T), negated when w_k > 0.
+ * F (3N) is accumulated in pair order; D (3 per pair) feeds K[a][a] -= D, K[a][b] += D,
+ * K[b][b] -= D, K[b][a] += D on the component's diagonal. */
+void tfo_packing_forces(int n_pairs, const int32_t* pairs, const double* weights, const double* mass,
+                        const double* pos, double T, double* F, double* D)
+{
+    const double c = PK_SATURATION / T;
+    for (int p = 0; p < n_pairs; ++p) {
+        const int id0 = pairs[2 * p], id1 = pairs[2 * p + 1];
+        const double mbar = 0.5 * (mass[id0] + mass[id1]);
+        for (int k = 0; k < 3; ++k) {
+            const double w = weights[3 * p + k];
+            double d = pos[3 * id0 + k] - pos[3 * id1 + k];
+            if (w > 0) d *= -1.0;
+            double f = PK_MULTIPLIER * mbar / (1 + exp(c * (-1.0 * d)));
+            f *= w;
+            F[3 * id0 + k] += f;
+            F[3 * id1 + k] -= f;
+            double s = 1 / (1 + exp(c * (-1.0 * d)));
+            double dd = s * (1 - s) * PK_MULTIPLIER * mbar * w * c;
+            if (w > 0) dd *= -1.0;
+            D[3 * p + k] = dd;
+        }
+    }
+}

@@ -27,7 +27,8 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_update_uk", "tf_nr_step", "tf_timestep", "tf_get_segment_stats", "tf_kernel_timing",
            "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune", "tf_set_bound_dofs",
            "tf_set_ablated_nodes", "tf_set_growth_state", "tf_get_growth_rotation", "tf_apply_jacobian",
-           "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get"]
+           "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get",
+           "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs"]
 
 
 class TfError(RuntimeError):
@@ -40,7 +41,8 @@ class MeshDesc(C.Structure):
     _fields_ = [("n_nodes", C.c_int32), ("n_elements", C.c_int32), ("node_pos", C.c_void_p), ("conn", C.c_void_p),
                 ("ref_pos", C.c_void_p), ("lam", C.c_void_p), ("mu", C.c_void_p), ("eta_int", C.c_void_p),
                 ("eta_ext", C.c_void_p), ("fixed", C.c_void_p), ("exposed", C.c_void_p), ("growth_rate", C.c_void_p),
-                ("n_segments", C.c_int32), ("node_seg_ptr", C.c_void_p), ("elem_seg_ptr", C.c_void_p)]
+                ("n_segments", C.c_int32), ("node_seg_ptr", C.c_void_p), ("elem_seg_ptr", C.c_void_p),
+                ("node_placement", C.c_void_p)]
 
 
 class SolverOpts(C.Structure):
@@ -48,12 +50,13 @@ class SolverOpts(C.Structure):
                 ("krylov_atol", C.c_double), ("gmres_restart", C.c_int32), ("iters_per_sync", C.c_int32),
                 ("nr_tol", C.c_double), ("nr_max_iter", C.c_int32), ("tangent_precision", C.c_int32),
                 ("matrix_precision", C.c_int32), ("growth_rotation", C.c_int32), ("use_graph", C.c_int32),
-                ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32)]
+                ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32),
+                ("implicit_packing", C.c_int32)]
 
 
 class StepStats(C.Structure):
     _fields_ = [("nr_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("converged", C.c_int32),
-                ("rotated_elements", C.c_int32), ("krylov_fallbacks", C.c_int32), ("pad", C.c_int32),
+                ("rotated_elements", C.c_int32), ("krylov_fallbacks", C.c_int32), ("packing_pairs", C.c_int32),
                 ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double)]
 
@@ -117,6 +120,10 @@ def load(path: str | None = None):
         "tf_adjoint_reset": ([vp], i32),
         "tf_adjoint_step": ([vp, dbl, vp, vp, vp, vp, vp], i32),
         "tf_adjoint_get": ([vp, vp, vp, vp, vp], i32),
+        "tf_get_node_masses": ([vp, vp], i32),
+        "tf_detect_packing": ([vp, vp], i32),
+        "tf_set_packing_pairs": ([vp, i32, vp, vp, vp], i32),
+        "tf_get_packing_pairs": ([vp, vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

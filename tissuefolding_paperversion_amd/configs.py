@@ -177,6 +177,35 @@ def batch_disc(n_prisms_target: float, n_layers: int, E: float, nu: float, rate_
     return t
 
 
+def facing_discs_tissue(radius: float = 20.0, h: float = 5.85, z_levels=(0.0, 5.0, 10.0), gap: float = 1.5,
+                        seed: int = 1, E: float = 1000.0, nu: float = 0.3, growth_per_hour=(0.01, 0.01, 0.0),
+                        eta_ext: float = 0.0, dt: float = 60.0, n_steps: int = 5,
+                        name: str = "facing discs") -> Tissue:
+    """Packing workload: two discs in one tissue, the second the mirror image of the first in
+    z, so that their apical surfaces face each other across ``gap`` (below the packing
+    threshold, 0.4 x the mean side length).  Every apical node of the lower disc then packs
+    against the node above it.  The mirror disc lists its prism nodes as (0, 2, 1, 3, 5, 4), so
+    that its basal cap (on top) keeps a positive orientation, and its basal nodes are fixed in z
+    like the lower disc's; each disc has the in-plane pins of :func:`disc_tissue`."""
+    lower = disc_tissue(radius=radius, h=h, z_levels=z_levels, seed=seed, E=E, nu=nu,
+                        growth_per_hour=growth_per_hour, eta_ext=eta_ext, dt=dt, n_steps=n_steps)
+    m = lower.mesh
+    n = m.n_nodes
+    top = float(z_levels[-1])
+    pos_b = m.pos.copy()
+    pos_b[:, 2] = 2.0 * top + gap - pos_b[:, 2]
+    conn_b = m.conn[:, [0, 2, 1, 3, 5, 4]] + n
+    pos = np.concatenate([m.pos, pos_b])
+    conn = np.concatenate([m.conn, conn_b]).astype(np.int32)
+    mesh = TissueMesh(pos, conn, pos[conn].reshape(-1, 18).copy(), np.concatenate([m.node_placement] * 2),
+                      np.concatenate([m.node_type] * 2), np.concatenate([m.at_circumference] * 2),
+                      None if m.layer is None else np.concatenate([m.layer] * 2))
+    _, lam, mu = assign_material(mesh, E, E, E, nu)
+    return Tissue(mesh, lam, mu, np.zeros(mesh.n_elements), np.concatenate([lower.eta_ext] * 2),
+                  np.concatenate([lower.fixed] * 2), np.concatenate([lower.rates] * 2), mesh.exposed_faces(),
+                  dt, n_steps, name, np.full(mesh.n_elements, float(E)), np.full(mesh.n_elements, nu))
+
+
 def config4(n_discs: int = 256, seed: int = 4, min_prisms: float = 500.0, max_prisms: float = 8000.0) -> list[Tissue]:
     """Batched sweep: ``n_discs`` independent discs with prism counts log-uniform in
     [min_prisms, max_prisms], 2-3 layers, E ~ U(500, 5000) Pa, nu ~ U(0.25, 0.45) and mean growth

@@ -83,6 +83,7 @@ void free_all(tf_handle* h)
                     h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz, h->seg_rtol2, h->warm[0], h->warm[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    packing_free(h);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
     drop_krylov_graphs(h);
@@ -257,6 +258,11 @@ void build_pattern(tf_handle* h)
             for (int b = 0; b < 6; ++b) r.push_back(conn[6 * e + b]);
         }
         r.push_back(A);  // every node keeps its diagonal block (isolated and ablated nodes too)
+    }
+    for (int64_t key : h->pk.h_blocks) {  // blocks of implicit-packing pairs
+        const int A = (int)(key / N), B = (int)(key % N);
+        rows[A].push_back(B);
+        rows[B].push_back(A);
     }
     if (nbound) {
         // spread every existing block (r, c) to T(r) x T(c)
@@ -464,6 +470,7 @@ void tf_default_options(tf_solver_opts* o)
     o->krylov_forcing = 1;
     o->xcd_tile_map = 1;
     o->krylov_warm_start = 1;
+    o->implicit_packing = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -609,6 +616,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h2d(h, h->node_src, node_src.data(), node_src.size());
         build_pattern(h);
         build_tiles(h, 256, 512);
+        packing_init(h, d->node_placement, d->exposed, xr.data());
         h2d(h, h->seg_node_ptr, h->h_node_seg_ptr.data(), S + 1);
         std::vector<int32_t> zS(S, 0);
         h2d(h, h->seg_counter, zS.data(), S);
@@ -652,7 +660,8 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
     if (o->krylov < -1 || o->krylov > TF_KRYLOV_PIPECG || o->krylov_max_iter <= 0 || o->nr_max_iter < 0 ||
         !(o->krylov_rtol >= 0) || !(o->nr_tol >= 0) ||
         (o->matrix_precision != TF_PREC_FP64 && o->matrix_precision != TF_PREC_FP32) ||
-        o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16)
+        o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16 ||
+        (o->implicit_packing != 0 && o->implicit_packing != 1))
         return fail(h, TF_INVALID, "invalid solver options");
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
     h->opts = *o;
@@ -1065,6 +1074,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         stats->nr_iterations = maxit;
         stats->krylov_iterations = krylov_total;
         stats->krylov_fallbacks = h->fallbacks;
+        stats->packing_pairs = h->pk.n;
         stats->converged = allc;
         stats->last_du_norm = worst;
         stats->ms_assemble = ms_asm;
@@ -1095,6 +1105,8 @@ int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats)
     int32_t st4[4];
     d2h(h, st4, h->status, 4);
     TF_HIP(hipStreamSynchronize(h->stream));
+    if (h->opts.implicit_packing) packing_detect(h);
+    else packing_clear(h);
     double ms_grow = ms_since(t0);
     int st = nr_step_impl(h, dt, stats);
     if (stats) {
@@ -1132,6 +1144,105 @@ int tf_set_bound_dofs(tf_handle* h, int32_t n_pairs, const int32_t* pairs)
     h->h_bound.assign(pairs, pairs + 2 * (size_t)n_pairs);
     build_pattern(h);
     build_tiles(h, h->wg, h->tile_cap);
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_node_masses(tf_handle* h, double* mass)
+{
+    if (!h || !mass) return TF_INVALID;
+    TF_GUARD_BEGIN
+    launch_packing_mass(h);
+    d2h(h, mass, h->pk.mass, h->N);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_detect_packing(tf_handle* h, int32_t* n_pairs)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    const int n = packing_detect(h);
+    if (n_pairs) *n_pairs = n;
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_packing_pairs(tf_handle* h, int32_t n_pairs, const int32_t* pairs, const double* weights,
+                         const double* threshold)
+{
+    if (!h || n_pairs < 0 || (n_pairs > 0 && (!pairs || !weights || !threshold))) return TF_INVALID;
+    TF_GUARD_BEGIN
+    auto seg_of = [&](int node) {
+        return (int)(std::upper_bound(h->h_node_seg_ptr.begin(), h->h_node_seg_ptr.end(), node) -
+                     h->h_node_seg_ptr.begin()) - 1;
+    };
+    std::vector<int32_t> a(n_pairs), b(n_pairs);
+    std::vector<int64_t> keys(n_pairs);
+    std::vector<uint8_t> seg_used(h->S, 0);
+    for (int k = 0; k < n_pairs; ++k) {
+        a[k] = pairs[2 * k];
+        b[k] = pairs[2 * k + 1];
+        const std::string tag = "packing pair " + std::to_string(k) + ": ";
+        if (a[k] < 0 || a[k] >= h->N || b[k] < 0 || b[k] >= h->N) return fail(h, TF_INVALID, tag + "node out of range");
+        if (a[k] == b[k]) return fail(h, TF_INVALID, tag + "a node cannot pack against itself");
+        const int s = seg_of(a[k]);
+        if (s != seg_of(b[k])) return fail(h, TF_INVALID, tag + "nodes in different tissues");
+        if (!(threshold[s] > 0)) return fail(h, TF_INVALID, tag + "its tissue needs a threshold > 0");
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(weights[3 * k + c])) return fail(h, TF_INVALID, tag + "weights must be finite");
+        seg_used[s] = 1;
+        keys[k] = (int64_t)std::min(a[k], b[k]) * h->N + std::max(a[k], b[k]);
+    }
+    std::vector<int64_t> sorted = keys;
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(h, TF_INVALID, "a node pair appears twice in the packing list");
+    packing_prepare(h);
+    launch_packing_mass(h);
+    if (n_pairs > 0) {
+        packing_reserve(h, n_pairs);
+        h2d(h, h->pk.a, a.data(), n_pairs);
+        h2d(h, h->pk.b, b.data(), n_pairs);
+        h2d(h, h->pk.w, weights, (size_t)3 * n_pairs);
+        std::vector<double> st((size_t)h->S * PK_NSTAT);
+        d2h(h, st.data(), h->pk.seg_stats, st.size());
+        TF_HIP(hipStreamSynchronize(h->stream));
+        for (int s = 0; s < h->S; ++s)
+            if (seg_used[s]) st[(size_t)s * PK_NSTAT + PK_ST_T] = threshold[s];
+        h2d(h, h->pk.seg_stats, st.data(), st.size());
+    }
+    packing_install(h, a, b);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_get_packing_pairs(tf_handle* h, int32_t* n_pairs, int32_t* pairs, double* weights, double* threshold,
+                         double* detect_grid)
+{
+    if (!h || !n_pairs) return TF_INVALID;
+    TF_GUARD_BEGIN
+    const int P = h->pk.n;
+    *n_pairs = P;
+    if (pairs)
+        for (int k = 0; k < P; ++k) {
+            pairs[2 * k] = h->pk.h_a[k];
+            pairs[2 * k + 1] = h->pk.h_b[k];
+        }
+    if (weights && P) d2h(h, weights, h->pk.w, (size_t)3 * P);
+    if (threshold || detect_grid) {
+        std::vector<double> st((size_t)h->S * PK_NSTAT);
+        d2h(h, st.data(), h->pk.seg_stats, st.size());
+        TF_HIP(hipStreamSynchronize(h->stream));
+        for (int s = 0; s < h->S; ++s) {
+            if (threshold) threshold[s] = st[(size_t)s * PK_NSTAT + PK_ST_T];
+            if (detect_grid)
+                for (int q = 0; q < 50; ++q) detect_grid[50 * s + q] = st[(size_t)s * PK_NSTAT + PK_ST_GRID + q];
+        }
+    }
+    TF_HIP(hipStreamSynchronize(h->stream));
     return TF_OK;
     TF_GUARD_END(h)
 }
@@ -1279,6 +1390,7 @@ static int adjoint_supported(tf_handle* h)
     if (h->any_viscous) return fail(h, TF_INVALID, "the adjoint supports eta_int = 0 only");
     if (h->n_bound) return fail(h, TF_INVALID, "the adjoint does not support bound DoFs");
     if (h->ablated) return fail(h, TF_INVALID, "the adjoint does not support ablated nodes");
+    if (h->pk.n) return fail(h, TF_INVALID, "the adjoint does not support implicit-packing pairs");
     return TF_OK;
 }
 

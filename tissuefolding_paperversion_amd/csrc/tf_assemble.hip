@@ -214,7 +214,8 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_
                                                 const int32_t* __restrict__ blk_row, const int32_t* __restrict__ col,
                                                 const double* __restrict__ kelem, const double* __restrict__ sa,
                                                 const double* __restrict__ eta_ext, const uint8_t* __restrict__ fixed,
-                                                const uint8_t* __restrict__ ablated, double inv_dt, int nb,
+                                                const uint8_t* __restrict__ ablated,
+                                                const double* __restrict__ pack_d, double inv_dt, int nb,
                                                 double* __restrict__ val, float* __restrict__ val32,
                                                 double* __restrict__ dinv)
 {
@@ -232,6 +233,11 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_
         const double s = sa[A] * inv_dt;
 #pragma unroll
         for (int i = 0; i < 3; ++i) acc[4 * i] += s * eta_ext[3 * A + i];
+        // implicit packing, added before the ablated-node and fixed-DoF rules
+        // (SC/Simulation.cpp:6489-6505)
+        if (pack_d)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[4 * i] += pack_d[3 * A + i];
         // nodes without mass keep a solvable system (SC/NewtonRaphsonSolver.cpp:350-361)
         if (ablated && ablated[A])
 #pragma unroll
@@ -315,7 +321,8 @@ __global__ __launch_bounds__(256) void k_residual(const int32_t* __restrict__ no
                                                   const double* __restrict__ felem, const double* __restrict__ uk,
                                                   const double* __restrict__ un, const double* __restrict__ sa,
                                                   const double* __restrict__ eta_ext, const uint8_t* __restrict__ fixed,
-                                                  double inv_dt, int N, double* __restrict__ g)
+                                                  const double* __restrict__ pack_f, double inv_dt, int N,
+                                                  double* __restrict__ g)
 {
     const int A = blockIdx.x * blockDim.x + threadIdx.x;
     if (A >= N) return;
@@ -331,7 +338,8 @@ __global__ __launch_bounds__(256) void k_residual(const int32_t* __restrict__ no
         const int d = 3 * A + i;
         const double vel = (uk[d] - un[d]) * inv_dt;
         const double gext = (s * eta_ext[d] * vel) * -1.0;
-        g[d] = fixed[d] ? 0.0 : acc[i] + gext;
+        const double gpack = pack_f ? pack_f[d] : 0.0;  // packing forces join gExt (SC/Simulation.cpp:8869-8906)
+        g[d] = fixed[d] ? 0.0 : acc[i] + gext + gpack;
     }
 }
 
@@ -590,10 +598,12 @@ void launch_assemble_gather(tf_handle* h, double dt)
 {
     hipEvent_t ev[2];
     timer_begin(h, h->t_gather, ev);
+    launch_packing_nodes(h);
     hipLaunchKernelGGL(k_gather, dim3((h->nb + 255) / 256), dim3(256), 0, h->stream, h->blk_ptr, h->blk_src, h->blk_row,
-                       h->col, h->kelem, h->sa, h->eta_ext, h->fixed, h->ablated, 1.0 / dt, h->nb, h->val,
-                       h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr, h->dinv);
+                       h->col, h->kelem, h->sa, h->eta_ext, h->fixed, h->ablated, h->pk.n ? h->pk.d : nullptr, 1.0 / dt,
+                       h->nb, h->val, h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr, h->dinv);
     TF_HIP(hipGetLastError());
+    launch_packing_offdiag(h);
     if (h->n_bound) launch_bind_matrix(h);
     timer_end(h, h->t_gather, ev);
 }
@@ -601,7 +611,7 @@ void launch_assemble_gather(tf_handle* h, double dt)
 void launch_residual(tf_handle* h, double dt)
 {
     hipLaunchKernelGGL(k_residual, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src, h->felem,
-                       h->uk, h->un, h->sa, h->eta_ext, h->fixed, 1.0 / dt, h->N, h->g);
+                       h->uk, h->un, h->sa, h->eta_ext, h->fixed, h->pk.n ? h->pk.f : nullptr, 1.0 / dt, h->N, h->g);
     TF_HIP(hipGetLastError());
     if (h->n_bound) launch_bind_residual(h);
 }

@@ -62,6 +62,46 @@ struct KernelTimer {
     bool capturing = false;           // inside a stream capture: no per-launch events
 };
 
+// Implicit packing: pairs of apical (or basal) nodes of one tissue closer than a threshold are
+// pushed apart by a sigmoid force whose Jacobian enters M (blocks (a, b), (b, a) and the
+// diagonal).  Per-tissue statistics stats[s * PK_NSTAT + k]:
+#define PK_NSTAT 64
+enum {
+    PK_ST_T = 0,      // packingThreshold T (force scale)
+    PK_ST_CELL = 1,   // spatial-hash cell edge (>= every detection threshold; 0: no detection)
+    PK_ST_LO = 2,     // [3] minimum x, y, z of the tissue's nodes (hash origin)
+    PK_ST_BB = 5,     // [4] reference bounding box: min x, min y, size x, size y
+    PK_ST_GRID = 9    // [50] detection threshold grid [10][5], row-major
+};
+struct PackingState {
+    // node and element tables (built at creation, detection tables on first use)
+    std::vector<int8_t> h_node_pl;
+    int8_t* node_pl = nullptr;        // [N] Node::tissuePlacement: 0 basal, 1 apical, 2 mid, 3 lateral
+    uint8_t* elem_basal = nullptr;    // [E] 1: the element's basal side lengths count (basal or spanning)
+    double* vref = nullptr;           // [E] reference volume
+    double* mass = nullptr;           // [N] Node::mass (det(Fg) V_ref / 6 summed over the node's prisms)
+    double* seg_stats = nullptr;      // [S][PK_NSTAT]
+    int32_t *adj_ptr = nullptr, *adj_col = nullptr;  // element neighbours of every node, sorted
+    int32_t* node_seg = nullptr;      // [N] tissue of every node
+    int32_t* master_end = nullptr;    // [S] first node of the tissue's last detection chunk
+    int32_t* seg_elem_ptr = nullptr;  // [S+1]
+    int32_t* ebin = nullptr;          // [E] grid bin of every element (scratch)
+    double* ecur = nullptr;           // [E] the bin's value of every element (scratch)
+    int32_t H = 0;                    // hash slots (a power of two)
+    int32_t *hcnt = nullptr, *hstart = nullptr, *cell_nodes = nullptr, *node_slot = nullptr;
+    int32_t *pcount = nullptr, *pstart = nullptr, *scan_tmp = nullptr;
+    // pair list (pacingNodeCouples0/1 and initialWeightPointx/y/z)
+    int32_t n = 0, cap = 0;
+    int32_t *a = nullptr, *b = nullptr;  // [cap] node ids
+    double* w = nullptr;                 // [3 cap] weights
+    int32_t* slot = nullptr;             // [2 cap] BSR blocks (a, b) and (b, a)
+    std::vector<int32_t> h_a, h_b;
+    int32_t nn = 0;                      // nodes with pairs
+    int32_t *nodes = nullptr, *nptr = nullptr, *nref = nullptr;  // their pairs: p (node a), ~p (node b)
+    double *f = nullptr, *d = nullptr;   // [3N] packing force and Jacobian diagonal of every DoF
+    std::vector<int64_t> h_blocks;       // pair blocks (min * N + max) kept in the BSR pattern
+};
+
 struct tf_handle {
     // sizes
     int32_t N = 0, E = 0, nb = 0, S = 1, T = 0;
@@ -171,6 +211,9 @@ struct tf_handle {
     int krylov_graph_iters[2] = {0, 0};
     int krylov_graph_kind[2] = {-1, -1};
 
+    // implicit packing
+    PackingState pk;
+
     // timing
     KernelTimer t_elem, t_gather, t_spmv;
 };
@@ -193,6 +236,17 @@ void build_pattern(tf_handle* h);  // BSR pattern, scatter lists and bound-DoF f
 void launch_bind_matrix(tf_handle* h);    // M <- N^T M N + I_slaves on the fold targets
 void launch_bind_residual(tf_handle* h);  // g <- N^T g
 void launch_bind_du(tf_handle* h);        // du[slave] <- du[master]
+// ---- implicit packing (tf_packing.hip) -------------------------------------------------------
+void packing_init(tf_handle* h, const int8_t* node_placement, const uint8_t* exposed, const double* ref_pos_aos);
+void packing_free(tf_handle* h);
+void packing_prepare(tf_handle* h);             // detection tables (first use)
+void packing_reserve(tf_handle* h, int n);      // pair arrays for n pairs
+int packing_detect(tf_handle* h);               // masses + detection on uk; returns the pair count
+void packing_install(tf_handle* h, const std::vector<int32_t>& a, const std::vector<int32_t>& b);
+void packing_clear(tf_handle* h);
+void launch_packing_mass(tf_handle* h);
+void launch_packing_nodes(tf_handle* h);        // pk.f, pk.d of the nodes with pairs at uk
+void launch_packing_offdiag(tf_handle* h);      // pair blocks of M, after k_gather
 // ---- discrete adjoint (tf_adjoint.hip) -----------------------------------------------------
 void launch_adj_elements(tf_handle* h, const double* y, double* invfg_bar, double* lam_bar, double* mu_bar);
 void launch_adj_un(tf_handle* h, double dt, const double* y, double* un_bar);

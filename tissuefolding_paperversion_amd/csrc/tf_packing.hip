@@ -1,0 +1,869 @@
+// tf_packing.hip -- implicit packing: node masses, per-tissue pair detection on a spatial hash,
+// and the pairs' sigmoid forces and Jacobian entries (gfx950, FP64).
+//
+// Detection (once per timestep, SC/Simulation.cpp:8167-8440) runs in five steps:
+//   k_pk_mass  : one lane per node sums det(Fg) V_ref / 6 over its prisms (updateNodeMasses).
+//   k_pk_stats : one workgroup per tissue: the reference's bounding box (its running-minimum
+//                rule included), side-length averages, the 10 x 5 threshold grid and the
+//                packing threshold T.
+//   k_pk_cells : eligible nodes (mass > 0, apical or basal) go to a hashed uniform grid whose
+//                cell is at least the largest detection threshold (Teschner et al., "Optimized
+//                spatial hashing for collision detection of deformable objects", VMV 2003), as
+//                a counting sort: count, exclusive scan, fill.
+//   k_pk_pairs : one lane per master node searches its 27 neighbour cells for slaves j > i of
+//                its tissue that the reference's O(N^2) scan would accept; a count pass sizes
+//                the list, a fill pass writes every master's pairs in ascending slave order, so
+//                that the list comes out in the reference's order (master, then slave).
+// Per NR iteration (SC/Simulation.cpp:6489-6499, :7149-7295):
+//   k_pk_nodes   : one lane per node with pairs sums its pairs' forces and Jacobian diagonal in
+//                  pair order (consumed by k_residual and k_gather).
+//   k_pk_offdiag : one lane per pair adds its off-diagonal entries to blocks (a, b) and (b, a),
+//                  which the BSR pattern carries for every pair.
+#include "tf_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace {
+
+constexpr double PK_MULT = 5000.0;  // packingMultiplier (SC/Simulation.cpp:8187)
+constexpr double PK_SAT = 5.0;      // sigmoidSaturationForPacking (SC/Simulation.cpp:8188)
+constexpr int PK_MAX_PARTNERS = 64; // slaves one master may collect in one detection
+constexpr int SWG = 1024;           // workgroup of the per-tissue statistics kernel
+constexpr int SCAN_WG = 1024, SCAN_PER = 4, SCAN_TILE = SCAN_WG * SCAN_PER;
+
+template <typename T>
+T* pk_alloc(size_t n)
+{
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw tf_hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    return (T*)p;
+}
+template <typename T>
+void pk_free(T*& p)
+{
+    if (p) { (void)hipFree(p); p = nullptr; }
+}
+template <typename T>
+void pk_h2d(tf_handle* h, T* dst, const T* src, size_t n)
+{
+    if (!n) return;
+    TF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+}
+
+// Grid bin of a relative coordinate: floor(v) clamped to [0, n-1] (negative and NaN -> 0).
+__device__ __forceinline__ int bin_index(double v, int n)
+{
+    if (!(v >= 0.0)) return 0;
+    if (v >= (double)n) return n - 1;
+    return (int)v;
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int s, int cx, int cy, int cz, uint32_t mask)
+{
+    return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u ^ (uint32_t)s * 2654435761u) &
+           mask;
+}
+
+// Prism::getApicalSideLengthAverage / getBasalSideLengthAverage (SC/Prism.cpp:1443-1490).
+__device__ double side_average(const double* __restrict__ uk, const int c[6], int apical)
+{
+    const int o = apical ? 3 : 0;
+    const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    double dsum = 0.0;
+    int counter = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double* p = uk + 3 * c[o + pr[k][0]];
+        const double* q = uk + 3 * c[o + pr[k][1]];
+        const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > 0) {
+            dsum += sqrt(d2);
+            counter++;
+        }
+    }
+    if (counter > 0) dsum /= counter;
+    return dsum;
+}
+
+template <typename T, typename Op>
+__device__ T block_reduce(T v, T* sh, Op op)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = SWG / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = op(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    const T r = sh[0];
+    __syncthreads();
+    return r;
+}
+struct OpAdd { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
+struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
+struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+
+// ---- node masses (SC/Simulation.cpp:7552-7560, SC/ShapeBase.cpp:1243-1245, :3938-3942) -----
+__global__ __launch_bounds__(256) void k_pk_mass(const int32_t* __restrict__ node_ptr, const int32_t* __restrict__ node_src,
+                                                 const double* __restrict__ fg, const double* __restrict__ vref,
+                                                 const uint8_t* __restrict__ ablated, int E, int N, double* __restrict__ mass)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double m = 0.0;
+    for (int k = node_ptr[n]; k < node_ptr[n + 1]; ++k) {
+        const int e = node_src[k] / 6;
+        double F[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) F[q] = fg[(size_t)q * E + e];
+        const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                           F[2] * (F[3] * F[7] - F[4] * F[6]);
+        m += det * vref[e] / 6.0;
+    }
+    mass[n] = (ablated && ablated[n]) ? 0.0 : m;
+}
+
+// ---- per-tissue bounding box, side lengths and threshold grid ------------------------------
+// Simulation::calculateBoundingBox (SC/Simulation.cpp:9148-9189), relative positions
+// (SC/ShapeBase.cpp:1842-1848), Simulation::getAverageSideLength (SC/Simulation.cpp:3205-3265)
+// and the 0.4 scalings of detectPacingNodes (:8175-8186).  The box keeps the reference's
+// "if (x < min) ... else if (x > max)" from (1e5, -1e5): a node that lowers the running minimum
+// is not a candidate for the maximum, so the maximum is taken over the other nodes; each thread
+// walks a contiguous chunk of nodes starting from the exclusive prefix minimum of the chunks
+// before it.  Grid bins sum the element's last side average; their counters are shared between
+// bins (i, 4) and (i + 1, 0), as the reference's 10 x 4 counter array indexed with j < 5 does.
+__global__ __launch_bounds__(SWG) void k_pk_stats(const int32_t* __restrict__ seg_node_ptr,
+                                                  const int32_t* __restrict__ seg_elem_ptr, const double* __restrict__ uk,
+                                                  const int32_t* __restrict__ conn, int E,
+                                                  const uint8_t* __restrict__ elem_basal, int32_t* __restrict__ ebin,
+                                                  double* __restrict__ ecur, double* __restrict__ stats)
+{
+    __shared__ double sh[SWG];
+    __shared__ int shi[SWG];
+    __shared__ unsigned long long bins_used;
+    const int s = blockIdx.x, t = threadIdx.x;
+    const int n0 = seg_node_ptr[s], n1 = seg_node_ptr[s + 1];
+    const int e0 = seg_elem_ptr[s], e1 = seg_elem_ptr[s + 1];
+    const int nn = n1 - n0, chunk = (nn + SWG - 1) / SWG;
+    const int c0 = n0 + min(nn, t * chunk), c1 = n0 + min(nn, (t + 1) * chunk);
+    if (t == 0) bins_used = 0ull;
+
+    double lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { lo[d] = 1.0e300; hi[d] = -1.0e300; }
+    for (int n = c0; n < c1; ++n)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double p = uk[3 * n + d];
+            lo[d] = fmin(lo[d], p);
+            hi[d] = fmax(hi[d], p);
+        }
+    double bbmin[2], bbmax[2];
+    for (int d = 0; d < 2; ++d) {
+        // inclusive prefix minimum of the chunk minima (Hillis-Steele), then exclusive
+        sh[t] = lo[d];
+        __syncthreads();
+        for (int off = 1; off < SWG; off <<= 1) {
+            const double v = t >= off ? fmin(sh[t], sh[t - off]) : sh[t];
+            __syncthreads();
+            sh[t] = v;
+            __syncthreads();
+        }
+        double run = t > 0 ? fmin(100000.0, sh[t - 1]) : 100000.0;
+        __syncthreads();
+        double qmax = -100000.0;
+        for (int n = c0; n < c1; ++n) {
+            const double p = uk[3 * n + d];
+            if (p < run) run = p;
+            else if (p > qmax) qmax = p;
+        }
+        bbmin[d] = fmin(100000.0, block_reduce(lo[d], sh, OpMin()));
+        bbmax[d] = block_reduce(qmax, sh, OpMax());
+    }
+    double tlo[3], thi[3];
+    for (int d = 0; d < 3; ++d) {
+        tlo[d] = block_reduce(lo[d], sh, OpMin());
+        thi[d] = block_reduce(hi[d], sh, OpMax());
+    }
+    const double szx = bbmax[0] - bbmin[0], szy = bbmax[1] - bbmin[1];
+
+    double dsum = 0.0;
+    int dcnt = 0;
+    unsigned long long used = 0ull;
+    for (int e = e0 + t; e < e1; e += SWG) {
+        int c[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) c[a] = conn[(size_t)a * E + e];
+        double cur = side_average(uk, c, 1);
+        if (cur > 0) { dsum += cur; dcnt++; }
+        if (elem_basal[e]) {
+            cur = side_average(uk, c, 0);
+            if (cur > 0) { dsum += cur; dcnt++; }
+        }
+        double cx = 0.0, cy = 0.0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) { cx += uk[3 * c[a]]; cy += uk[3 * c[a] + 1]; }
+        cx /= 6; cy /= 6;
+        const int bx = bin_index((cx - bbmin[0]) / szx, 10);
+        const int by = bin_index(((cy - bbmin[1]) / szy) / 2.0, 5);
+        const int b = 5 * bx + by;
+        ebin[e] = b;
+        ecur[e] = cur;
+        used |= 1ull << b;
+    }
+    atomicOr(&bins_used, used);
+    dsum = block_reduce(dsum, sh, OpAdd());
+    dcnt = block_reduce(dcnt, shi, OpAdd());  // also orders the bins_used updates before the read
+    const unsigned long long ub = bins_used;
+    double* st = stats + (size_t)s * PK_NSTAT;
+    double hc = 0.0;
+    for (int b = 0; b < 50; ++b) {
+        double g = 0.0;
+        if ((ub >> b) & 1ull) {
+            const int alias = 4 * (b / 5) + b % 5;
+            double sb = 0.0;
+            int cb = 0;
+            for (int e = e0 + t; e < e1; e += SWG) {
+                const int eb = ebin[e];
+                if (eb == b) sb += ecur[e];
+                if (4 * (eb / 5) + eb % 5 == alias) cb++;
+            }
+            sb = block_reduce(sb, sh, OpAdd());
+            cb = block_reduce(cb, shi, OpAdd());
+            g = 0.4 * (cb > 0 ? sb / cb : sb);
+        }
+        if (t == 0) st[PK_ST_GRID + b] = g;
+        hc = fmax(hc, g);
+    }
+    if (t == 0) {
+        const double ext = fmax(thi[0] - tlo[0], fmax(thi[1] - tlo[1], thi[2] - tlo[2]));
+        st[PK_ST_T] = dcnt > 0 ? 0.4 * (dsum / (double)dcnt) : 0.0;
+        st[PK_ST_CELL] = hc > 0 ? fmax(hc, ext / 1048576.0) : 0.0;
+        st[PK_ST_LO + 0] = tlo[0];
+        st[PK_ST_LO + 1] = tlo[1];
+        st[PK_ST_LO + 2] = tlo[2];
+        st[PK_ST_BB + 0] = bbmin[0];
+        st[PK_ST_BB + 1] = bbmin[1];
+        st[PK_ST_BB + 2] = szx;
+        st[PK_ST_BB + 3] = szy;
+    }
+}
+
+// ---- spatial hash ------------------------------------------------------------------------
+__device__ __forceinline__ bool can_pack(const double* __restrict__ mass, const int8_t* __restrict__ pl, int n)
+{
+    // Node::checkIfNodeHasPacking (SC/Node.cpp:133-151); no lateral elements in this model
+    return mass[n] != 0.0 && (pl[n] == 0 || pl[n] == 1);
+}
+
+__global__ __launch_bounds__(256) void k_pk_cells(int N, const int32_t* __restrict__ node_seg,
+                                                  const double* __restrict__ stats, const double* __restrict__ uk,
+                                                  const double* __restrict__ mass, const int8_t* __restrict__ pl,
+                                                  uint32_t hmask, int32_t* __restrict__ node_slot, int32_t* __restrict__ hcnt)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int s = node_seg[n];
+    const double* st = stats + (size_t)s * PK_NSTAT;
+    const double hc = st[PK_ST_CELL];
+    if (!(hc > 0.0) || !can_pack(mass, pl, n)) { node_slot[n] = -1; return; }
+    const int cx = (int)((uk[3 * n] - st[PK_ST_LO]) / hc);
+    const int cy = (int)((uk[3 * n + 1] - st[PK_ST_LO + 1]) / hc);
+    const int cz = (int)((uk[3 * n + 2] - st[PK_ST_LO + 2]) / hc);
+    const uint32_t slot = cell_hash(s, cx, cy, cz, hmask);
+    node_slot[n] = (int32_t)slot;
+    atomicAdd(hcnt + slot, 1);
+}
+
+__global__ __launch_bounds__(256) void k_pk_fill_cells(int N, const int32_t* __restrict__ node_slot,
+                                                       int32_t* __restrict__ cursor, int32_t* __restrict__ cell_nodes)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int slot = node_slot[n];
+    if (slot >= 0) cell_nodes[atomicAdd(cursor + slot, 1)] = n;
+}
+
+// ---- pair search ----------------------------------------------------------------------------
+// The acceptance test of detectPacingNodes (SC/Simulation.cpp:8238-8299) for master i and slave
+// j > i of the same tissue: same placement, not sharing a prism (Node::immediateNeigs,
+// SC/ShapeBase.cpp:3105-3122), |x_i - x_j|^2 < t^2 with t the grid threshold of the master's bin;
+// masters from the tissue's last detection chunk are dropped as the reference's merge loop does.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_pk_pairs(int N, const int32_t* __restrict__ node_seg,
+                                                  const int32_t* __restrict__ seg_node_ptr,
+                                                  const int32_t* __restrict__ master_end,
+                                                  const double* __restrict__ stats, const double* __restrict__ uk,
+                                                  const int8_t* __restrict__ pl, const int32_t* __restrict__ node_slot,
+                                                  const int32_t* __restrict__ hstart, const int32_t* __restrict__ cell_nodes,
+                                                  const int32_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj_col,
+                                                  uint32_t hmask, int32_t* __restrict__ pcount,
+                                                  const int32_t* __restrict__ pstart, int32_t* __restrict__ pa,
+                                                  int32_t* __restrict__ pb, double* __restrict__ pw,
+                                                  int32_t* __restrict__ status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int cnt = 0;
+    int js[FILL ? PK_MAX_PARTNERS : 1];
+    const int s = node_seg[i];
+    const double* st = stats + (size_t)s * PK_NSTAT;
+    const double xi = uk[3 * i], yi = uk[3 * i + 1], zi = uk[3 * i + 2];
+    if (node_slot[i] >= 0 && i < master_end[s]) {
+        const int gx = bin_index((xi - st[PK_ST_BB]) / st[PK_ST_BB + 2], 10);
+        const int gy = bin_index((yi - st[PK_ST_BB + 1]) / st[PK_ST_BB + 3], 5);
+        const double thr = st[PK_ST_GRID + 5 * gx + gy];
+        const double t2 = thr * thr;
+        if (t2 > 0.0) {
+            const double hc = st[PK_ST_CELL];
+            const int cx = (int)((xi - st[PK_ST_LO]) / hc);
+            const int cy = (int)((yi - st[PK_ST_LO + 1]) / hc);
+            const int cz = (int)((zi - st[PK_ST_LO + 2]) / hc);
+            uint32_t slots[27];
+            int ns = 0;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const uint32_t sl = cell_hash(s, cx + dx, cy + dy, cz + dz, hmask);
+                        bool dup = false;
+                        for (int k = 0; k < ns; ++k) dup |= slots[k] == sl;
+                        if (!dup) slots[ns++] = sl;
+                    }
+            const int nend = seg_node_ptr[s + 1];
+            const int8_t pli = pl[i];
+            const int a0 = adj_ptr[i], a1 = adj_ptr[i + 1];
+            for (int k = 0; k < ns; ++k) {
+                const int q1 = hstart[slots[k] + 1];
+                for (int q = hstart[slots[k]]; q < q1; ++q) {
+                    const int j = cell_nodes[q];
+                    if (j <= i || j >= nend || pl[j] != pli) continue;
+                    const double dx = xi - uk[3 * j], dy = yi - uk[3 * j + 1], dz = zi - uk[3 * j + 2];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (!(d2 < t2)) continue;
+                    int lo = a0, hi = a1;  // element neighbours of i, sorted
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (adj_col[mid] < j) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < a1 && adj_col[lo] == j) continue;
+                    if (FILL && cnt < PK_MAX_PARTNERS) js[cnt] = j;
+                    cnt++;
+                }
+            }
+        }
+    }
+    if (!FILL) {
+        pcount[i] = cnt;
+        return;
+    }
+    if (cnt > PK_MAX_PARTNERS) {
+        atomicMax(status + 2, cnt);
+        cnt = PK_MAX_PARTNERS;
+    }
+    const int base = pstart[i];
+    cnt = min(cnt, pstart[i + 1] - base);  // never past the room the count pass reserved
+    for (int k = 1; k < cnt; ++k) {  // ascending slave order
+        const int v = js[k];
+        int m = k - 1;
+        while (m >= 0 && js[m] > v) { js[m + 1] = js[m]; --m; }
+        js[m + 1] = v;
+    }
+    for (int k = 0; k < cnt; ++k) {
+        const int j = js[k];
+        const double dx = xi - uk[3 * j], dy = yi - uk[3 * j + 1], dz = zi - uk[3 * j + 2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        pa[base + k] = i;
+        pb[base + k] = j;
+        pw[3 * (size_t)(base + k)] = dx / d;
+        pw[3 * (size_t)(base + k) + 1] = dy / d;
+        pw[3 * (size_t)(base + k) + 2] = dz / d;
+    }
+}
+
+// ---- exclusive scan of int32 (n <= SCAN_TILE^2): out[0..n], out[n] = total ---------------------
+__global__ __launch_bounds__(SCAN_WG) void k_scan_tiles(const int32_t* __restrict__ in, int n, int32_t* __restrict__ out,
+                                                        int32_t* __restrict__ tile_sum)
+{
+    __shared__ int32_t sh[SCAN_WG];
+    const int t = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)t * SCAN_PER;
+    int32_t v[SCAN_PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        v[k] = base + k < (size_t)n ? in[base + k] : 0;
+        sum += v[k];
+    }
+    sh[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < SCAN_WG; off <<= 1) {
+        const int32_t x = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    int32_t run = sh[t] - sum;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        if (base + k < (size_t)n) out[base + k] = run;
+        run += v[k];
+    }
+    if (t == SCAN_WG - 1) tile_sum[blockIdx.x] = sh[t];
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_top(int32_t* __restrict__ tile_sum, int n_tiles)
+{
+    __shared__ int32_t sh[SCAN_WG];
+    const int t = threadIdx.x;
+    int32_t v[SCAN_PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int i = t * SCAN_PER + k;
+        v[k] = i < n_tiles ? tile_sum[i] : 0;
+        sum += v[k];
+    }
+    sh[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < SCAN_WG; off <<= 1) {
+        const int32_t x = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    int32_t run = sh[t] - sum;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int i = t * SCAN_PER + k;
+        if (i <= n_tiles) tile_sum[i] = run;  // tile_sum[n_tiles] = total
+        run += v[k];
+    }
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_add(int32_t* __restrict__ out, int n, const int32_t* __restrict__ tile_sum,
+                                                      int n_tiles)
+{
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_PER;
+    const int32_t add = tile_sum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k)
+        if (base + k < (size_t)n) out[base + k] += add;
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sum[n_tiles];
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_copy_i32(int32_t* dst, const int32_t* src, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ---- forces and Jacobian of one pair (SC/Simulation.cpp:7149-7203, :7205-7295) -------------
+// Component k: d = x_a,k - x_b,k, negated when w_k > 0; s = 1 / (1 + exp(-(S/T) d));
+// F_k = M mbar s w_k (on a; -F_k on b); D_k = dF_k/dx_a,k = s (1 - s) M mbar w_k (S/T), negated
+// when w_k > 0.  The reference's K gets K[a][a] -= D, K[a][b] += D, K[b][b] -= D, K[b][a] += D.
+__device__ void pair_eval(int p, const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                          const double* __restrict__ pw, const int32_t* __restrict__ node_seg,
+                          const double* __restrict__ stats, const double* __restrict__ mass,
+                          const double* __restrict__ uk, double F[3], double D[3])
+{
+    const int a = pa[p], b = pb[p];
+    const double c = PK_SAT / stats[(size_t)node_seg[a] * PK_NSTAT + PK_ST_T];
+    const double mbar = 0.5 * (mass[a] + mass[b]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double w = pw[3 * (size_t)p + k];
+        double d = uk[3 * a + k] - uk[3 * b + k];
+        if (w > 0) d *= -1.0;
+        const double ex = exp(c * (-1.0 * d));
+        F[k] = PK_MULT * mbar / (1 + ex) * w;
+        const double sg = 1 / (1 + ex);
+        double dd = sg * (1 - sg) * PK_MULT * mbar * w * c;
+        if (w > 0) dd *= -1.0;
+        D[k] = dd;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pk_nodes(int nn, const int32_t* __restrict__ nodes, const int32_t* __restrict__ nptr,
+                                                  const int32_t* __restrict__ nref, const int32_t* __restrict__ pa,
+                                                  const int32_t* __restrict__ pb, const double* __restrict__ pw,
+                                                  const int32_t* __restrict__ node_seg, const double* __restrict__ stats,
+                                                  const double* __restrict__ mass, const double* __restrict__ uk,
+                                                  double* __restrict__ pf, double* __restrict__ pd)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    double f[3] = {0, 0, 0}, dg[3] = {0, 0, 0};
+    for (int r = nptr[k]; r < nptr[k + 1]; ++r) {
+        const int ref = nref[r];
+        const int p = ref >= 0 ? ref : ~ref;
+        double F[3], D[3];
+        pair_eval(p, pa, pb, pw, node_seg, stats, mass, uk, F, D);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            f[c] += ref >= 0 ? F[c] : -F[c];
+            dg[c] -= D[c];
+        }
+    }
+    const int n = nodes[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        pf[3 * n + c] = f[c];
+        pd[3 * n + c] = dg[c];
+    }
+}
+
+// Off-diagonal entries after k_gather; entries in a fixed row or column stay as the fixed-DoF
+// rule left them (SC/NewtonRaphsonSolver.cpp:237-255 runs after the packing Jacobian).
+__global__ __launch_bounds__(256) void k_pk_offdiag(int np, const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                                                    const double* __restrict__ pw, const int32_t* __restrict__ slot,
+                                                    const int32_t* __restrict__ node_seg, const double* __restrict__ stats,
+                                                    const double* __restrict__ mass, const double* __restrict__ uk,
+                                                    const uint8_t* __restrict__ fixed, int nb, double* __restrict__ val,
+                                                    float* __restrict__ val32)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double F[3], D[3];
+    pair_eval(p, pa, pb, pw, node_seg, stats, mass, uk, F, D);
+    const int a = pa[p], b = pb[p];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (fixed[3 * a + k] || fixed[3 * b + k]) continue;
+        for (int e = 0; e < 2; ++e) {
+            const size_t idx = (size_t)(4 * k) * nb + slot[2 * p + e];
+            const double v = val[idx] + D[k];
+            val[idx] = v;
+            if (val32) val32[idx] = (float)v;
+        }
+    }
+}
+
+void scan_i32(tf_handle* h, const int32_t* in, int n, int32_t* out)
+{
+    const int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles >= SCAN_TILE) throw std::runtime_error("packing scan too long");
+    PackingState& pk = h->pk;
+    if (tiles == 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, h->stream, out, 0, 1);
+        TF_HIP(hipGetLastError());
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_WG), 0, h->stream, in, n, out, pk.scan_tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_WG), 0, h->stream, pk.scan_tmp, tiles);
+    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(SCAN_WG), 0, h->stream, out, n, pk.scan_tmp, tiles);
+    TF_HIP(hipGetLastError());
+}
+
+// Prism::calculateReferenceVolume (SC/Prism.cpp:876-908) with
+// ShapeBase::calculateVolumeForInputShapeStructure (SC/ShapeBase.cpp:3362-3397): eight triangles
+// (the caps and two per lateral quad) each close a tetrahedron with the centroid.
+double reference_volume(const double* X)
+{
+    static const int tri[8][3] = {{0, 1, 2}, {3, 4, 5}, {0, 1, 3}, {1, 3, 4}, {1, 2, 4}, {2, 4, 5}, {0, 2, 3}, {2, 3, 5}};
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 6; ++a)
+        for (int j = 0; j < 3; ++j) c[j] += X[3 * a + j];
+    for (int j = 0; j < 3; ++j) c[j] /= 6;
+    double V = 0.0;
+    for (int t = 0; t < 8; ++t) {
+        const double *p0 = X + 3 * tri[t][0], *p1 = X + 3 * tri[t][1], *p2 = X + 3 * tri[t][2];
+        double v1[3], v2[3], vm[3];
+        for (int j = 0; j < 3; ++j) {
+            v1[j] = p1[j] - p0[j];
+            v2[j] = p2[j] - p0[j];
+            vm[j] = c[j] - p0[j];
+        }
+        const double b[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2], v1[0] * v2[1] - v1[1] * v2[0]};
+        const double nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        const double height = std::fabs((vm[0] * b[0] + vm[1] * b[1] + vm[2] * b[2]) / nb);
+        V += 1.0 / 3.0 * (height * (nb / 2));
+    }
+    return V;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+void packing_init(tf_handle* h, const int8_t* node_placement, const uint8_t* exposed, const double* ref_pos_aos)
+{
+    const int N = h->N, E = h->E;
+    std::vector<int8_t> pl(N, 2);
+    if (node_placement) {
+        for (int n = 0; n < N; ++n) {
+            if (node_placement[n] < 0 || node_placement[n] > 3)
+                throw std::runtime_error("node placement must be 0 (basal), 1 (apical), 2 (mid) or 3 (lateral)");
+            pl[n] = node_placement[n];
+        }
+    } else {
+        // from the exposed faces: corners of an exposed basal cap are basal, of an apical cap apical
+        for (int e = 0; e < E; ++e)
+            for (int a = 0; a < 6; ++a) {
+                const int bit = a < 3 ? 1 : 2;
+                if (exposed && (exposed[e] & bit)) pl[h->h_conn[6 * e + a]] = a < 3 ? 0 : 1;
+            }
+    }
+    // element placement (ShapeBase::setTissuePlacement, SC/ShapeBase.cpp:718-779): basal side
+    // lengths count for basal elements and elements spanning the tissue
+    std::vector<uint8_t> basal(E, 0);
+    std::vector<double> vref(E);
+    for (int e = 0; e < E; ++e) {
+        bool ap = false, ba = false, lat = false;
+        for (int a = 0; a < 6; ++a) {
+            const int p = pl[h->h_conn[6 * e + a]];
+            ap |= p == 1;
+            ba |= p == 0;
+            lat |= p == 3;
+        }
+        basal[e] = !lat && ba;
+        vref[e] = reference_volume(ref_pos_aos + (size_t)18 * e);
+    }
+    PackingState& pk = h->pk;
+    pk.h_node_pl = pl;
+    pk.node_pl = pk_alloc<int8_t>(N);
+    pk.elem_basal = pk_alloc<uint8_t>(E);
+    pk.vref = pk_alloc<double>(E);
+    pk.mass = pk_alloc<double>(N);
+    pk.seg_stats = pk_alloc<double>((size_t)h->S * PK_NSTAT);
+    pk.f = pk_alloc<double>((size_t)3 * N);
+    pk.d = pk_alloc<double>((size_t)3 * N);
+    pk_h2d(h, pk.node_pl, pl.data(), N);
+    pk_h2d(h, pk.elem_basal, basal.data(), E);
+    pk_h2d(h, pk.vref, vref.data(), E);
+    std::vector<double> z((size_t)std::max<int64_t>((int64_t)3 * N, (int64_t)h->S * PK_NSTAT), 0.0);
+    pk_h2d(h, pk.mass, z.data(), N);
+    pk_h2d(h, pk.seg_stats, z.data(), (size_t)h->S * PK_NSTAT);
+    pk_h2d(h, pk.f, z.data(), (size_t)3 * N);
+    pk_h2d(h, pk.d, z.data(), (size_t)3 * N);
+}
+
+void packing_free(tf_handle* h)
+{
+    PackingState& pk = h->pk;
+    pk_free(pk.node_pl); pk_free(pk.elem_basal); pk_free(pk.vref); pk_free(pk.mass); pk_free(pk.seg_stats);
+    pk_free(pk.f); pk_free(pk.d); pk_free(pk.adj_ptr); pk_free(pk.adj_col); pk_free(pk.node_seg);
+    pk_free(pk.master_end); pk_free(pk.seg_elem_ptr); pk_free(pk.ebin); pk_free(pk.ecur); pk_free(pk.hcnt);
+    pk_free(pk.hstart); pk_free(pk.cell_nodes); pk_free(pk.node_slot); pk_free(pk.pcount); pk_free(pk.pstart);
+    pk_free(pk.scan_tmp); pk_free(pk.a); pk_free(pk.b); pk_free(pk.w); pk_free(pk.slot); pk_free(pk.nodes);
+    pk_free(pk.nptr); pk_free(pk.nref);
+}
+
+// Detection work arrays and node tables, allocated on first use.
+void packing_prepare(tf_handle* h)
+{
+    PackingState& pk = h->pk;
+    if (pk.adj_ptr) return;
+    const int N = h->N, E = h->E, S = h->S;
+    // element neighbours of every node, sorted (the element pattern without bound or pair blocks)
+    std::vector<int32_t> aptr(N + 1, 0), acol;
+    acol.reserve((size_t)N * 20);
+    std::vector<int32_t> row;
+    for (int A = 0; A < N; ++A) {
+        row.clear();
+        for (int k = h->h_node_ptr[A]; k < h->h_node_ptr[A + 1]; ++k) {
+            const int e = h->h_node_src[k] / 6;
+            for (int b = 0; b < 6; ++b) row.push_back(h->h_conn[6 * e + b]);
+        }
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        acol.insert(acol.end(), row.begin(), row.end());
+        aptr[A + 1] = (int32_t)acol.size();
+    }
+    std::vector<int32_t> nseg(N), mend(S);
+    for (int s = 0; s < S; ++s) {
+        const int n0 = h->h_node_seg_ptr[s], n1 = h->h_node_seg_ptr[s + 1];
+        for (int n = n0; n < n1; ++n) nseg[n] = s;
+        // parallellisation chunks of detectPacingNodes (SC/Simulation.cpp:8204-8218, :8422-8437)
+        const int nn = n1 - n0;
+        const int seg_size = std::max(1, nn / 16);
+        const int n_chunks = nn / seg_size;
+        mend[s] = n0 + (n_chunks - 1) * seg_size;
+    }
+    int H = 1024;
+    while (H < 2 * N) H <<= 1;
+    pk.H = H;
+    pk.adj_ptr = pk_alloc<int32_t>(N + 1);
+    pk.adj_col = pk_alloc<int32_t>(acol.size());
+    pk.node_seg = pk_alloc<int32_t>(N);
+    pk.master_end = pk_alloc<int32_t>(S);
+    pk.seg_elem_ptr = pk_alloc<int32_t>(S + 1);
+    pk.ebin = pk_alloc<int32_t>(E);
+    pk.ecur = pk_alloc<double>(E);
+    pk.hcnt = pk_alloc<int32_t>(H);
+    pk.hstart = pk_alloc<int32_t>(H + 1);
+    pk.cell_nodes = pk_alloc<int32_t>(N);
+    pk.node_slot = pk_alloc<int32_t>(N);
+    pk.pcount = pk_alloc<int32_t>(N);
+    pk.pstart = pk_alloc<int32_t>(N + 1);
+    pk.scan_tmp = pk_alloc<int32_t>((std::max(H, N) + SCAN_TILE - 1) / SCAN_TILE + 1);
+    pk_h2d(h, pk.adj_ptr, aptr.data(), N + 1);
+    pk_h2d(h, pk.adj_col, acol.data(), acol.size());
+    pk_h2d(h, pk.node_seg, nseg.data(), N);
+    pk_h2d(h, pk.master_end, mend.data(), S);
+    pk_h2d(h, pk.seg_elem_ptr, h->h_elem_seg_ptr.data(), S + 1);
+}
+
+void launch_packing_mass(tf_handle* h)
+{
+    hipLaunchKernelGGL(k_pk_mass, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src, h->fg,
+                       h->pk.vref, h->ablated, h->E, h->N, h->pk.mass);
+    TF_HIP(hipGetLastError());
+}
+
+void packing_reserve(tf_handle* h, int n)
+{
+    PackingState& pk = h->pk;
+    if (n <= pk.cap && pk.a) return;
+    const int cap = std::max({n, 2 * pk.cap, 256});
+    pk_free(pk.a); pk_free(pk.b); pk_free(pk.w); pk_free(pk.slot);
+    pk.a = pk_alloc<int32_t>(cap);
+    pk.b = pk_alloc<int32_t>(cap);
+    pk.w = pk_alloc<double>((size_t)3 * cap);
+    pk.slot = pk_alloc<int32_t>((size_t)2 * cap);
+    pk.cap = cap;
+}
+
+// Pattern, slots and node lists for the pairs now in pk.a / pk.b (host copies a, b).
+void packing_install(tf_handle* h, const std::vector<int32_t>& a, const std::vector<int32_t>& b)
+{
+    PackingState& pk = h->pk;
+    const int P = (int)a.size(), N = h->N;
+    auto find_blk = [&](int A, int B) -> int {
+        const int32_t* c0 = h->h_col.data() + h->h_row_ptr[A];
+        const int32_t* c1 = h->h_col.data() + h->h_row_ptr[A + 1];
+        const int32_t* it = std::lower_bound(c0, c1, B);
+        return (it != c1 && *it == B) ? (int)(it - h->h_col.data()) : -1;
+    };
+    bool missing = false;
+    for (int p = 0; p < P && !missing; ++p) missing = find_blk(a[p], b[p]) < 0;
+    if (missing) {
+        // the pattern keeps the blocks of earlier pairs while they are few, so that contacts
+        // that open and close do not rebuild it every step
+        std::vector<int64_t> keys;
+        for (int p = 0; p < P; ++p)
+            keys.push_back((int64_t)std::min(a[p], b[p]) * N + std::max(a[p], b[p]));
+        if (pk.h_blocks.size() <= 2 * (size_t)P + 64) keys.insert(keys.end(), pk.h_blocks.begin(), pk.h_blocks.end());
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        pk.h_blocks = keys;
+        build_pattern(h);
+        build_tiles(h, h->wg, h->tile_cap);
+    }
+    std::vector<int32_t> slot((size_t)2 * P);
+    for (int p = 0; p < P; ++p) {
+        slot[2 * p] = find_blk(a[p], b[p]);
+        slot[2 * p + 1] = find_blk(b[p], a[p]);
+        if (slot[2 * p] < 0 || slot[2 * p + 1] < 0) throw std::runtime_error("packing pair missing from the pattern");
+    }
+    // nodes with pairs and their pair references in pair order (p: node a of pair p, ~p: node b)
+    std::vector<int32_t> cnt(N, 0);
+    for (int p = 0; p < P; ++p) { cnt[a[p]]++; cnt[b[p]]++; }
+    std::vector<int32_t> nodes, nptr{0}, pos(N, -1);
+    for (int n = 0; n < N; ++n)
+        if (cnt[n]) {
+            pos[n] = nptr.back();
+            nodes.push_back(n);
+            nptr.push_back(nptr.back() + cnt[n]);
+        }
+    std::vector<int32_t> nref((size_t)2 * P);
+    for (int p = 0; p < P; ++p) {
+        nref[pos[a[p]]++] = p;
+        nref[pos[b[p]]++] = ~p;
+    }
+    pk_free(pk.nodes); pk_free(pk.nptr); pk_free(pk.nref);
+    pk.nodes = pk_alloc<int32_t>(nodes.size());
+    pk.nptr = pk_alloc<int32_t>(nptr.size());
+    pk.nref = pk_alloc<int32_t>(nref.size());
+    if (P) {
+        pk_h2d(h, pk.slot, slot.data(), slot.size());
+        pk_h2d(h, pk.nodes, nodes.data(), nodes.size());
+        pk_h2d(h, pk.nptr, nptr.data(), nptr.size());
+        pk_h2d(h, pk.nref, nref.data(), nref.size());
+    }
+    launch_fill(h, pk.f, 0.0, 3 * (int64_t)N);
+    launch_fill(h, pk.d, 0.0, 3 * (int64_t)N);
+    pk.n = P;
+    pk.nn = (int32_t)nodes.size();
+    pk.h_a = a;
+    pk.h_b = b;
+}
+
+int packing_detect(tf_handle* h)
+{
+    packing_prepare(h);
+    PackingState& pk = h->pk;
+    const int N = h->N, S = h->S;
+    const uint32_t hmask = (uint32_t)pk.H - 1u;
+    launch_packing_mass(h);
+    hipLaunchKernelGGL(k_pk_stats, dim3(S), dim3(SWG), 0, h->stream, h->seg_node_ptr, pk.seg_elem_ptr, h->uk, h->conn, h->E,
+                       pk.elem_basal, pk.ebin, pk.ecur, pk.seg_stats);
+    hipLaunchKernelGGL(k_fill_i32, dim3((pk.H + 255) / 256), dim3(256), 0, h->stream, pk.hcnt, 0, pk.H);
+    hipLaunchKernelGGL(k_pk_cells, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg, pk.seg_stats, h->uk,
+                       pk.mass, pk.node_pl, hmask, pk.node_slot, pk.hcnt);
+    TF_HIP(hipGetLastError());
+    scan_i32(h, pk.hcnt, pk.H, pk.hstart);
+    hipLaunchKernelGGL(k_copy_i32, dim3((pk.H + 255) / 256), dim3(256), 0, h->stream, pk.hcnt, pk.hstart, pk.H);
+    hipLaunchKernelGGL(k_pk_fill_cells, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_slot, pk.hcnt,
+                       pk.cell_nodes);
+    hipLaunchKernelGGL(k_pk_pairs<false>, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg, h->seg_node_ptr,
+                       pk.master_end, pk.seg_stats, h->uk, pk.node_pl, pk.node_slot, pk.hstart, pk.cell_nodes, pk.adj_ptr,
+                       pk.adj_col, hmask, pk.pcount, nullptr, nullptr, nullptr, nullptr, nullptr);
+    TF_HIP(hipGetLastError());
+    scan_i32(h, pk.pcount, N, pk.pstart);
+    int32_t total = 0;
+    TF_HIP(hipMemcpyAsync(&total, pk.pstart + N, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+    std::vector<int32_t> a(total), b(total);
+    if (total > 0) {
+        packing_reserve(h, total);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, h->stream, h->status + 2, 0, 1);
+        hipLaunchKernelGGL(k_pk_pairs<true>, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg,
+                           h->seg_node_ptr, pk.master_end, pk.seg_stats, h->uk, pk.node_pl, pk.node_slot, pk.hstart,
+                           pk.cell_nodes, pk.adj_ptr, pk.adj_col, hmask, nullptr, pk.pstart, pk.a, pk.b, pk.w, h->status);
+        TF_HIP(hipGetLastError());
+        int32_t over = 0;
+        TF_HIP(hipMemcpyAsync(&over, h->status + 2, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        TF_HIP(hipMemcpyAsync(a.data(), pk.a, sizeof(int32_t) * total, hipMemcpyDeviceToHost, h->stream));
+        TF_HIP(hipMemcpyAsync(b.data(), pk.b, sizeof(int32_t) * total, hipMemcpyDeviceToHost, h->stream));
+        TF_HIP(hipStreamSynchronize(h->stream));
+        if (over > 0)
+            throw std::runtime_error("a node has " + std::to_string(over) + " packing partners (at most " +
+                                     std::to_string(PK_MAX_PARTNERS) + " are supported)");
+    }
+    packing_install(h, a, b);
+    return total;
+}
+
+void packing_clear(tf_handle* h)
+{
+    if (h->pk.n == 0) return;
+    packing_install(h, {}, {});
+}
+
+void launch_packing_nodes(tf_handle* h)
+{
+    PackingState& pk = h->pk;
+    if (!pk.n) return;
+    hipLaunchKernelGGL(k_pk_nodes, dim3((pk.nn + 255) / 256), dim3(256), 0, h->stream, pk.nn, pk.nodes, pk.nptr, pk.nref,
+                       pk.a, pk.b, pk.w, pk.node_seg, pk.seg_stats, pk.mass, h->uk, pk.f, pk.d);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_packing_offdiag(tf_handle* h)
+{
+    PackingState& pk = h->pk;
+    if (!pk.n) return;
+    hipLaunchKernelGGL(k_pk_offdiag, dim3((pk.n + 255) / 256), dim3(256), 0, h->stream, pk.n, pk.a, pk.b, pk.w, pk.slot,
+                       pk.node_seg, pk.seg_stats, pk.mass, h->uk, h->fixed, h->nb, h->val,
+                       h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr);
+    TF_HIP(hipGetLastError());
+}

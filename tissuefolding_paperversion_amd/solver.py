@@ -43,6 +43,7 @@ class NewtonRaphsonSolver:
             lam=L.f64(cat(lambda t: t.lam)), mu=L.f64(cat(lambda t: t.mu)), eta=L.f64(cat(lambda t: t.eta_int)),
             eta_ext=L.f64(cat(lambda t: t.eta_ext)), fixed=L.u8(cat(lambda t: t.fixed)),
             exposed=L.u8(cat(lambda t: t.exposed)), rate=L.f64(cat(lambda t: t.rates.ravel())),
+            placement=np.ascontiguousarray(cat(lambda t: t.mesh.node_placement), dtype=np.int8),
         )
         k = self._keep
         d = L.MeshDesc()
@@ -50,6 +51,7 @@ class NewtonRaphsonSolver:
         d.node_pos, d.conn, d.ref_pos = L.ptr(k["pos"]), L.ptr(k["conn"]), L.ptr(k["xref"])
         d.lam, d.mu, d.eta_int = L.ptr(k["lam"]), L.ptr(k["mu"]), L.ptr(k["eta"])
         d.eta_ext, d.fixed, d.exposed, d.growth_rate = L.ptr(k["eta_ext"]), L.ptr(k["fixed"]), L.ptr(k["exposed"]), L.ptr(k["rate"])
+        d.node_placement = L.ptr(k["placement"])
         d.n_segments = len(self.tissues)
         d.node_seg_ptr = L.ptr(self.node_seg_ptr) if len(self.tissues) > 1 else None
         d.elem_seg_ptr = L.ptr(self.elem_seg_ptr) if len(self.tissues) > 1 else None
@@ -140,6 +142,46 @@ class NewtonRaphsonSolver:
         a = L.i32(np.asarray(nodes, dtype=np.int32)) if len(nodes) else None
         self._check(self._lib.tf_set_ablated_nodes(self.h, 0 if a is None else len(a), L.ptr(a)))
 
+    # ---- implicit packing -----------------------------------------------------------------------
+    def node_masses(self) -> np.ndarray:
+        """Node::mass from the current Fg (Simulation::updateNodeMasses)."""
+        m = np.zeros(self.N)
+        self._check(self._lib.tf_get_node_masses(self.h, L.ptr(m)))
+        return m
+
+    def detectPacingNodes(self) -> int:
+        """Node masses and packing pairs on the current positions (Simulation::detectPacingNodes);
+        returns the pair count."""
+        n = C.c_int32()
+        self._check(self._lib.tf_detect_packing(self.h, C.byref(n)))
+        self._refresh_nb()
+        return n.value
+
+    def set_packing_pairs(self, pairs, weights, threshold):
+        """Replace the pair list: ``pairs`` (P, 2) node ids, ``weights`` (P, 3), ``threshold``
+        one packing threshold per tissue."""
+        p = L.i32(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        w = L.f64(np.asarray(weights, float).reshape(-1, 3))
+        t = L.f64(np.atleast_1d(np.asarray(threshold, float)))
+        self._check(self._lib.tf_set_packing_pairs(self.h, p.shape[0], L.ptr(p) if len(p) else None,
+                                                   L.ptr(w) if len(w) else None, L.ptr(t)))
+        self._refresh_nb()
+
+    def packing_pairs(self):
+        """(pairs (P, 2), weights (P, 3), threshold per tissue (S,), detection grids (S, 10, 5))."""
+        n = C.c_int32()
+        self._check(self._lib.tf_get_packing_pairs(self.h, C.byref(n), None, None, None, None))
+        P, S = n.value, len(self.tissues)
+        p, w = np.zeros(2 * P, np.int32), np.zeros(3 * P)
+        t, g = np.zeros(S), np.zeros(50 * S)
+        self._check(self._lib.tf_get_packing_pairs(self.h, C.byref(n), L.ptr(p), L.ptr(w), L.ptr(t), L.ptr(g)))
+        return p.reshape(-1, 2), w.reshape(-1, 3), t, g.reshape(S, 10, 5)
+
+    def _refresh_nb(self):
+        nb = C.c_int32()
+        self._lib.tf_get_sizes(self.h, None, None, C.byref(nb), None)
+        self.nb = nb.value
+
     # ---- per step -----------------------------------------------------------------------------
     def grow(self, dt) -> int:
         n = C.c_int32()
@@ -212,9 +254,11 @@ class NewtonRaphsonSolver:
         return out
 
     def runOneStep(self, dt):
-        """Growth, viscosity surfaces and the NR loop of one timestep."""
+        """Growth, viscosity surfaces, node masses and packing detection (option implicit_packing)
+        and the NR loop of one timestep."""
         st = L.StepStats()
         rc = self._check(self._lib.tf_timestep(self.h, dt, C.byref(st)), allow=(L.TF_NOT_CONVERGED,))
+        self._refresh_nb()
         out = st.as_dict()
         out["status"] = rc
         return out
