@@ -10,7 +10,7 @@ import pytest
 
 from conftest import rel_err
 from test_oracle_constraints import bound_pairs
-from test_oracle_packing import permuted
+from test_oracle_packing import max_first, permuted
 from tissuefolding_paperversion_amd import configs
 from tissuefolding_paperversion_amd import _lib as L
 
@@ -24,6 +24,8 @@ def _solver(tissue, **opts):
 
 def _facing(perm=None, **kw):
     t = configs.facing_discs_tissue(**kw)
+    if perm == "max_first":
+        return max_first(t)
     return t if perm is None else permuted(t, perm)
 
 
@@ -41,7 +43,7 @@ def test_node_masses_match_oracle(oracle):
     assert m2[3] == 0.0 and m2[4] == 0.0
 
 
-@pytest.mark.parametrize("perm", [None, 5, 9])
+@pytest.mark.parametrize("perm", [None, 5, 9, "max_first"])
 def test_detection_matches_oracle(oracle, perm):
     t = _facing(perm)
     s = _solver(t)

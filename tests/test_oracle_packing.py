@@ -255,3 +255,42 @@ def test_empty_pair_lists_for_the_workloads(make):
     ot = O.OracleTissue(t)
     ot.grow(t.dt)
     assert ot.detect_packing() == 0
+
+
+def max_first(t, shift=0.05):
+    """The tissue renumbered so that node 0 is a basal rim node moved to be the unique largest x:
+    the reference's bounding box then ends at the next largest x, where the apical rim nodes of
+    both discs sit, so those can no longer be masters."""
+    from tissuefolding_paperversion_amd.mesh import TissueMesh
+    m = t.mesh
+    n_a = m.n_nodes // 2
+    basal = np.where(m.node_placement[:n_a] == BASAL)[0]
+    nb = int(basal[np.argmax(m.pos[basal, 0])])
+    perm = np.arange(m.n_nodes)
+    perm[[0, nb]] = perm[[nb, 0]]
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(m.n_nodes)
+    pos = m.pos[perm].copy()
+    pos[0, 0] += shift
+    conn = inv[m.conn].astype(np.int32)
+    mesh = TissueMesh(pos, conn, pos[conn].reshape(-1, 18).copy(), m.node_placement[perm], m.node_type[perm],
+                      m.at_circumference[perm], m.layer)
+    fx = t.fixed.reshape(-1, 3)[perm].ravel()
+    ee = t.eta_ext.reshape(-1, 3)[perm].ravel()
+    return configs.Tissue(mesh, t.lam, t.mu, t.eta_int, ee, fx, t.rates, t.exposed, t.dt, t.n_steps,
+                          t.name + " (node 0 at max x)", t.E_young, t.nu)
+
+
+def test_bounding_box_rule_changes_the_pairs():
+    """On max_first the running-minimum box drops the pair of the largest-x apical rim node; a box
+    ending at the true maximum would keep it."""
+    t = max_first(configs.facing_discs_tissue())
+    ot = O.OracleTissue(t)
+    ot.detect_packing()
+    T, grid, bb = O.packing_thresholds(t.mesh.conn, O.elem_basal_side(t.mesh), t.mesh.pos)
+    assert bb[0] + bb[2] < t.mesh.pos[:, 0].max()
+    bb_true = bb.copy()
+    bb_true[2] = t.mesh.pos[:, 0].max() - bb[0]
+    pairs_true, _ = O.packing_detect(t.mesh.pos, ot.placement, ot.mass, ot.adj, grid, bb_true)
+    assert len(pairs_true) == len(ot.pairs) + 1
+    assert {tuple(p) for p in ot.pairs} < {tuple(p) for p in pairs_true}

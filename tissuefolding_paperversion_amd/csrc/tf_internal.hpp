@@ -86,7 +86,8 @@ struct PackingState {
     int32_t* master_end = nullptr;    // [S] first node of the tissue's last detection chunk
     int32_t* seg_elem_ptr = nullptr;  // [S+1]
     int32_t* ebin = nullptr;          // [E] grid bin of every element (scratch)
-    double* ecur = nullptr;           // [E] the bin's value of every element (scratch)
+    double* ev = nullptr;             // [5][E] side-length values and centre of every element (scratch)
+    double* vpn = nullptr;            // [E] det(Fg) V_ref / 6 of every element (scratch)
     int32_t H = 0;                    // hash slots (a power of two)
     int32_t *hcnt = nullptr, *hstart = nullptr, *cell_nodes = nullptr, *node_slot = nullptr;
     int32_t *pcount = nullptr, *pstart = nullptr, *scan_tmp = nullptr;
@@ -96,7 +97,7 @@ struct PackingState {
     double* w = nullptr;                 // [3 cap] weights
     int32_t* slot = nullptr;             // [2 cap] BSR blocks (a, b) and (b, a)
     std::vector<int32_t> h_a, h_b;
-    int32_t nn = 0;                      // nodes with pairs
+    int32_t nn = 0, node_cap = 0, ref_cap = 0;  // nodes with pairs; capacities of the node lists
     int32_t *nodes = nullptr, *nptr = nullptr, *nref = nullptr;  // their pairs: p (node a), ~p (node b)
     double *f = nullptr, *d = nullptr;   // [3N] packing force and Jacobian diagonal of every DoF
     std::vector<int64_t> h_blocks;       // pair blocks (min * N + max) kept in the BSR pattern

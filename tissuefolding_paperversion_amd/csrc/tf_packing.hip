@@ -2,18 +2,19 @@
 // and the pairs' sigmoid forces and Jacobian entries (gfx950, FP64).
 //
 // Detection (once per timestep, SC/Simulation.cpp:8167-8440) runs in five steps:
-//   k_pk_mass  : one lane per node sums det(Fg) V_ref / 6 over its prisms (updateNodeMasses).
-//   k_pk_stats : one workgroup per tissue: the reference's bounding box (its running-minimum
-//                rule included), side-length averages, the 10 x 5 threshold grid and the
-//                packing threshold T.
+//   k_pk_vpn, k_pk_mass : det(Fg) V_ref / 6 per prism, summed per node (updateNodeMasses).
+//   k_pk_evals, k_pk_stats : side-length averages and centres per prism, then one workgroup per
+//                tissue reduces them with the reference's bounding box (its running-minimum
+//                rule included) into the 10 x 5 threshold grid and the packing threshold T.
 //   k_pk_cells : eligible nodes (mass > 0, apical or basal) go to a hashed uniform grid whose
 //                cell is at least the largest detection threshold (Teschner et al., "Optimized
 //                spatial hashing for collision detection of deformable objects", VMV 2003), as
 //                a counting sort: count, exclusive scan, fill.
-//   k_pk_pairs : one lane per master node searches its 27 neighbour cells for slaves j > i of
-//                its tissue that the reference's O(N^2) scan would accept; a count pass sizes
-//                the list, a fill pass writes every master's pairs in ascending slave order, so
-//                that the list comes out in the reference's order (master, then slave).
+//   k_pk_count, k_pk_fill : every master node's 27 neighbour cells are searched for the slaves
+//                j > i of its tissue that the reference's O(N^2) scan would accept; the count
+//                pass (32 lanes per master) sizes the list, the fill pass (only when pairs exist)
+//                writes every master's pairs in ascending slave order, so that the list comes
+//                out in the reference's order (master, then slave).
 // Per NR iteration (SC/Simulation.cpp:6489-6499, :7149-7295):
 //   k_pk_nodes   : one lane per node with pairs sums its pairs' forces and Jacobian diagonal in
 //                  pair order (consumed by k_residual and k_gather).
@@ -108,41 +109,81 @@ __device__ T block_reduce(T v, T* sh, Op op)
 struct OpAdd { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
 struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
 struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+struct OpMinI { __device__ int operator()(int a, int b) const { return min(a, b); } };
 
 // ---- node masses (SC/Simulation.cpp:7552-7560, SC/ShapeBase.cpp:1243-1245, :3938-3942) -----
+// VolumePerNode = det(Fg) V_ref / 6 per prism, then every node sums its prisms' values in
+// element order.
+__global__ __launch_bounds__(256) void k_pk_vpn(const double* __restrict__ fg, const double* __restrict__ vref, int E,
+                                                double* __restrict__ vpn)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double F[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) F[q] = fg[(size_t)q * E + e];
+    const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                       F[2] * (F[3] * F[7] - F[4] * F[6]);
+    vpn[e] = det * vref[e] / 6.0;
+}
+
 __global__ __launch_bounds__(256) void k_pk_mass(const int32_t* __restrict__ node_ptr, const int32_t* __restrict__ node_src,
-                                                 const double* __restrict__ fg, const double* __restrict__ vref,
-                                                 const uint8_t* __restrict__ ablated, int E, int N, double* __restrict__ mass)
+                                                 const double* __restrict__ vpn, const uint8_t* __restrict__ ablated, int N,
+                                                 double* __restrict__ mass)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double m = 0.0;
-    for (int k = node_ptr[n]; k < node_ptr[n + 1]; ++k) {
-        const int e = node_src[k] / 6;
-        double F[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) F[q] = fg[(size_t)q * E + e];
-        const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-                           F[2] * (F[3] * F[7] - F[4] * F[6]);
-        m += det * vref[e] / 6.0;
-    }
+    for (int k = node_ptr[n]; k < node_ptr[n + 1]; ++k) m += vpn[node_src[k] / 6];
     mass[n] = (ablated && ablated[n]) ? 0.0 : m;
 }
 
-// ---- per-tissue bounding box, side lengths and threshold grid ------------------------------
+// ---- per-element side lengths and centres (one lane per element) ---------------------------
+// ev[0]: the element's contribution to the columnar side-length sum (its positive apical
+// average, plus its positive basal average when basal sides count), ev[1]: how many averages
+// that is, ev[2]: the value its grid bin sums (the basal average when basal sides count, else
+// the apical one), ev[3], ev[4]: centre x, y (mean of the six nodes, ShapeBase::getCentre,
+// SC/ShapeBase.cpp:240-250).  Structure of arrays: ev[k * E + e].
+__global__ __launch_bounds__(256) void k_pk_evals(const double* __restrict__ uk, const int32_t* __restrict__ conn, int E,
+                                                  const uint8_t* __restrict__ elem_basal, double* __restrict__ ev)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int c[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) c[a] = conn[(size_t)a * E + e];
+    double cur = side_average(uk, c, 1);
+    double sum = 0.0, cnt = 0.0;
+    if (cur > 0) { sum += cur; cnt += 1.0; }
+    if (elem_basal[e]) {
+        cur = side_average(uk, c, 0);
+        if (cur > 0) { sum += cur; cnt += 1.0; }
+    }
+    double cx = 0.0, cy = 0.0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) { cx += uk[3 * c[a]]; cy += uk[3 * c[a] + 1]; }
+    ev[e] = sum;
+    ev[(size_t)E + e] = cnt;
+    ev[2 * (size_t)E + e] = cur;
+    ev[3 * (size_t)E + e] = cx / 6;
+    ev[4 * (size_t)E + e] = cy / 6;
+}
+
+// ---- per-tissue bounding box, thresholds and grid (one workgroup per tissue) -----------------
 // Simulation::calculateBoundingBox (SC/Simulation.cpp:9148-9189), relative positions
 // (SC/ShapeBase.cpp:1842-1848), Simulation::getAverageSideLength (SC/Simulation.cpp:3205-3265)
-// and the 0.4 scalings of detectPacingNodes (:8175-8186).  The box keeps the reference's
-// "if (x < min) ... else if (x > max)" from (1e5, -1e5): a node that lowers the running minimum
-// is not a candidate for the maximum, so the maximum is taken over the other nodes; each thread
-// walks a contiguous chunk of nodes starting from the exclusive prefix minimum of the chunks
-// before it.  Grid bins sum the element's last side average; their counters are shared between
-// bins (i, 4) and (i + 1, 0), as the reference's 10 x 4 counter array indexed with j < 5 does.
+// and the 0.4 scalings of detectPacingNodes (:8175-8186).
+// The box: "if (x < min) min = x; else if (x > max) max = x" from (1e5, -1e5) leaves out of
+// the maximum every node that lowers the running minimum.  Such a node lies below all earlier
+// ones, so it cannot be the largest of the nodes after the initial strictly decreasing run
+// x_0 > x_1 > ... > x_{r-1}; every node of that run lowers the minimum.  Hence
+// max = max_{i >= r} x_i with r the first i >= 1 where x_i >= x_{i-1}.
+// The grid: bins sum the element's bin value; their counters are shared between bins (i, 4)
+// and (i + 1, 0), as the reference's 10 x 4 counter array indexed with j < 5 does.
 __global__ __launch_bounds__(SWG) void k_pk_stats(const int32_t* __restrict__ seg_node_ptr,
                                                   const int32_t* __restrict__ seg_elem_ptr, const double* __restrict__ uk,
-                                                  const int32_t* __restrict__ conn, int E,
-                                                  const uint8_t* __restrict__ elem_basal, int32_t* __restrict__ ebin,
-                                                  double* __restrict__ ecur, double* __restrict__ stats)
+                                                  const double* __restrict__ ev, int E, int32_t* __restrict__ ebin,
+                                                  double* __restrict__ stats)
 {
     __shared__ double sh[SWG];
     __shared__ int shi[SWG];
@@ -150,88 +191,76 @@ __global__ __launch_bounds__(SWG) void k_pk_stats(const int32_t* __restrict__ se
     const int s = blockIdx.x, t = threadIdx.x;
     const int n0 = seg_node_ptr[s], n1 = seg_node_ptr[s + 1];
     const int e0 = seg_elem_ptr[s], e1 = seg_elem_ptr[s + 1];
-    const int nn = n1 - n0, chunk = (nn + SWG - 1) / SWG;
-    const int c0 = n0 + min(nn, t * chunk), c1 = n0 + min(nn, (t + 1) * chunk);
     if (t == 0) bins_used = 0ull;
 
     double lo[3], hi[3];
+    int rx = n1, ry = n1;
 #pragma unroll
     for (int d = 0; d < 3; ++d) { lo[d] = 1.0e300; hi[d] = -1.0e300; }
-    for (int n = c0; n < c1; ++n)
+    for (int n = n0 + t; n < n1; n += SWG) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double p = uk[3 * n + d];
             lo[d] = fmin(lo[d], p);
             hi[d] = fmax(hi[d], p);
         }
-    double bbmin[2], bbmax[2];
-    for (int d = 0; d < 2; ++d) {
-        // inclusive prefix minimum of the chunk minima (Hillis-Steele), then exclusive
-        sh[t] = lo[d];
-        __syncthreads();
-        for (int off = 1; off < SWG; off <<= 1) {
-            const double v = t >= off ? fmin(sh[t], sh[t - off]) : sh[t];
-            __syncthreads();
-            sh[t] = v;
-            __syncthreads();
+        if (n > n0) {
+            if (uk[3 * n] >= uk[3 * (n - 1)]) rx = min(rx, n);
+            if (uk[3 * n + 1] >= uk[3 * (n - 1) + 1]) ry = min(ry, n);
         }
-        double run = t > 0 ? fmin(100000.0, sh[t - 1]) : 100000.0;
-        __syncthreads();
-        double qmax = -100000.0;
-        for (int n = c0; n < c1; ++n) {
-            const double p = uk[3 * n + d];
-            if (p < run) run = p;
-            else if (p > qmax) qmax = p;
-        }
-        bbmin[d] = fmin(100000.0, block_reduce(lo[d], sh, OpMin()));
-        bbmax[d] = block_reduce(qmax, sh, OpMax());
     }
-    double tlo[3], thi[3];
+    rx = block_reduce(rx, shi, OpMinI());
+    ry = block_reduce(ry, shi, OpMinI());
+    double qx = -100000.0, qy = -100000.0;
+    for (int n = n0 + t; n < n1; n += SWG) {
+        if (n >= rx) qx = fmax(qx, uk[3 * n]);
+        if (n >= ry) qy = fmax(qy, uk[3 * n + 1]);
+    }
+    double bbmin[2], bbmax[2], tlo[3], thi[3];
+    bbmax[0] = block_reduce(qx, sh, OpMax());
+    bbmax[1] = block_reduce(qy, sh, OpMax());
     for (int d = 0; d < 3; ++d) {
         tlo[d] = block_reduce(lo[d], sh, OpMin());
         thi[d] = block_reduce(hi[d], sh, OpMax());
     }
+    bbmin[0] = fmin(100000.0, tlo[0]);
+    bbmin[1] = fmin(100000.0, tlo[1]);
     const double szx = bbmax[0] - bbmin[0], szy = bbmax[1] - bbmin[1];
 
-    double dsum = 0.0;
-    int dcnt = 0;
+    double dsum = 0.0, sb0 = 0.0;  // sb0, cb0: bin (0, 0), where a current bounding box puts every element
+    double dcnt = 0.0;
+    int cb0 = 0;
     unsigned long long used = 0ull;
     for (int e = e0 + t; e < e1; e += SWG) {
-        int c[6];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) c[a] = conn[(size_t)a * E + e];
-        double cur = side_average(uk, c, 1);
-        if (cur > 0) { dsum += cur; dcnt++; }
-        if (elem_basal[e]) {
-            cur = side_average(uk, c, 0);
-            if (cur > 0) { dsum += cur; dcnt++; }
-        }
-        double cx = 0.0, cy = 0.0;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) { cx += uk[3 * c[a]]; cy += uk[3 * c[a] + 1]; }
-        cx /= 6; cy /= 6;
-        const int bx = bin_index((cx - bbmin[0]) / szx, 10);
-        const int by = bin_index(((cy - bbmin[1]) / szy) / 2.0, 5);
+        dsum += ev[e];
+        dcnt += ev[(size_t)E + e];
+        const double cur = ev[2 * (size_t)E + e];
+        const int bx = bin_index((ev[3 * (size_t)E + e] - bbmin[0]) / szx, 10);
+        const int by = bin_index(((ev[4 * (size_t)E + e] - bbmin[1]) / szy) / 2.0, 5);
         const int b = 5 * bx + by;
         ebin[e] = b;
-        ecur[e] = cur;
         used |= 1ull << b;
+        if (b == 0) { sb0 += cur; cb0++; }
     }
     atomicOr(&bins_used, used);
-    dsum = block_reduce(dsum, sh, OpAdd());
-    dcnt = block_reduce(dcnt, shi, OpAdd());  // also orders the bins_used updates before the read
+    dsum = block_reduce(dsum, sh, OpAdd());  // also orders the bins_used updates before the read
+    dcnt = block_reduce(dcnt, sh, OpAdd());
+    sb0 = block_reduce(sb0, sh, OpAdd());
+    cb0 = block_reduce(cb0, shi, OpAdd());
     const unsigned long long ub = bins_used;
     double* st = stats + (size_t)s * PK_NSTAT;
     double hc = 0.0;
     for (int b = 0; b < 50; ++b) {
         double g = 0.0;
-        if ((ub >> b) & 1ull) {
+        if (b == 0) {  // alias index 0 belongs to bin (0, 0) alone
+            g = 0.4 * (cb0 > 0 ? sb0 / cb0 : sb0);
+        } else if ((ub >> b) & 1ull) {  // other bins: only with a box older than the positions
             const int alias = 4 * (b / 5) + b % 5;
             double sb = 0.0;
             int cb = 0;
             for (int e = e0 + t; e < e1; e += SWG) {
                 const int eb = ebin[e];
-                if (eb == b) sb += ecur[e];
+                if (eb == b) sb += ev[2 * (size_t)E + e];
                 if (4 * (eb / 5) + eb % 5 == alias) cb++;
             }
             sb = block_reduce(sb, sh, OpAdd());
@@ -243,7 +272,7 @@ __global__ __launch_bounds__(SWG) void k_pk_stats(const int32_t* __restrict__ se
     }
     if (t == 0) {
         const double ext = fmax(thi[0] - tlo[0], fmax(thi[1] - tlo[1], thi[2] - tlo[2]));
-        st[PK_ST_T] = dcnt > 0 ? 0.4 * (dsum / (double)dcnt) : 0.0;
+        st[PK_ST_T] = dcnt > 0 ? 0.4 * (dsum / dcnt) : 0.0;
         st[PK_ST_CELL] = hc > 0 ? fmax(hc, ext / 1048576.0) : 0.0;
         st[PK_ST_LO + 0] = tlo[0];
         st[PK_ST_LO + 1] = tlo[1];
@@ -295,89 +324,146 @@ __global__ __launch_bounds__(256) void k_pk_fill_cells(int N, const int32_t* __r
 // j > i of the same tissue: same placement, not sharing a prism (Node::immediateNeigs,
 // SC/ShapeBase.cpp:3105-3122), |x_i - x_j|^2 < t^2 with t the grid threshold of the master's bin;
 // masters from the tissue's last detection chunk are dropped as the reference's merge loop does.
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_pk_pairs(int N, const int32_t* __restrict__ node_seg,
+struct Master {
+    double x, y, z, t2;
+    int cx, cy, cz, s, nend, a0, a1;
+    int8_t pl;
+};
+
+__device__ __forceinline__ bool master_setup(int i, const int32_t* __restrict__ node_seg,
+                                             const int32_t* __restrict__ seg_node_ptr,
+                                             const int32_t* __restrict__ master_end, const double* __restrict__ stats,
+                                             const double* __restrict__ uk, const int8_t* __restrict__ pl,
+                                             const int32_t* __restrict__ node_slot, const int32_t* __restrict__ adj_ptr,
+                                             Master& m)
+{
+    if (node_slot[i] < 0) return false;
+    m.s = node_seg[i];
+    if (i >= master_end[m.s]) return false;
+    const double* st = stats + (size_t)m.s * PK_NSTAT;
+    m.x = uk[3 * i]; m.y = uk[3 * i + 1]; m.z = uk[3 * i + 2];
+    const int gx = bin_index((m.x - st[PK_ST_BB]) / st[PK_ST_BB + 2], 10);
+    const int gy = bin_index((m.y - st[PK_ST_BB + 1]) / st[PK_ST_BB + 3], 5);
+    const double thr = st[PK_ST_GRID + 5 * gx + gy];
+    m.t2 = thr * thr;
+    if (!(m.t2 > 0.0)) return false;
+    const double hc = st[PK_ST_CELL];
+    m.cx = (int)((m.x - st[PK_ST_LO]) / hc);
+    m.cy = (int)((m.y - st[PK_ST_LO + 1]) / hc);
+    m.cz = (int)((m.z - st[PK_ST_LO + 2]) / hc);
+    m.nend = seg_node_ptr[m.s + 1];
+    m.a0 = adj_ptr[i];
+    m.a1 = adj_ptr[i + 1];
+    m.pl = pl[i];
+    return true;
+}
+
+// hash slot of neighbour cell k (0..26) of the master's cell
+__device__ __forceinline__ uint32_t neighbour_slot(const Master& m, int k, uint32_t hmask)
+{
+    return cell_hash(m.s, m.cx + k % 3 - 1, m.cy + (k / 3) % 3 - 1, m.cz + k / 9 - 1, hmask);
+}
+
+__device__ __forceinline__ bool accept(const Master& m, int i, int j, const double* __restrict__ uk,
+                                       const int8_t* __restrict__ pl, const int32_t* __restrict__ adj_col)
+{
+    if (j <= i || j >= m.nend || pl[j] != m.pl) return false;
+    const double dx = m.x - uk[3 * j], dy = m.y - uk[3 * j + 1], dz = m.z - uk[3 * j + 2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 < m.t2)) return false;
+    int lo = m.a0, hi = m.a1;  // element neighbours of i, sorted
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (adj_col[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    return !(lo < m.a1 && adj_col[lo] == j);
+}
+
+// Count pass: 32 lanes per master, lane k searches neighbour cell k (cells whose hash slot an
+// earlier lane already covers are skipped); the lanes' counts are summed by shuffles.
+__global__ __launch_bounds__(256) void k_pk_count(int N, const int32_t* __restrict__ node_seg,
                                                   const int32_t* __restrict__ seg_node_ptr,
                                                   const int32_t* __restrict__ master_end,
                                                   const double* __restrict__ stats, const double* __restrict__ uk,
                                                   const int8_t* __restrict__ pl, const int32_t* __restrict__ node_slot,
                                                   const int32_t* __restrict__ hstart, const int32_t* __restrict__ cell_nodes,
                                                   const int32_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj_col,
-                                                  uint32_t hmask, int32_t* __restrict__ pcount,
-                                                  const int32_t* __restrict__ pstart, int32_t* __restrict__ pa,
-                                                  int32_t* __restrict__ pb, double* __restrict__ pw,
-                                                  int32_t* __restrict__ status)
+                                                  uint32_t hmask, int32_t* __restrict__ pcount)
+{
+    const int i = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+    const int lane = threadIdx.x & 31;
+    if (i >= N) return;  // uniform over the 32 lanes of a master
+    Master m;
+    if (!master_setup(i, node_seg, seg_node_ptr, master_end, stats, uk, pl, node_slot, adj_ptr, m)) {
+        if (lane == 0) pcount[i] = 0;
+        return;
+    }
+    const uint32_t sl = neighbour_slot(m, min(lane, 26), hmask);
+    bool dup = lane >= 27;
+    for (int k = 0; k < 27; ++k) {
+        const uint32_t o = __shfl(sl, k, 32);
+        dup |= (k < lane) && (o == sl);
+    }
+    int cnt = 0;
+    if (!dup) {
+        const int q1 = hstart[sl + 1];
+        for (int q = hstart[sl]; q < q1; ++q) cnt += accept(m, i, cell_nodes[q], uk, pl, adj_col) ? 1 : 0;
+    }
+    for (int off = 16; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 32);
+    if (lane == 0) pcount[i] = cnt;
+}
+
+// Fill pass (only when pairs exist): one lane per master collects its slaves, sorts them and
+// writes its pairs at its offset, so that the list is ordered by master, then slave.
+__global__ __launch_bounds__(256) void k_pk_fill(int N, const int32_t* __restrict__ node_seg,
+                                                 const int32_t* __restrict__ seg_node_ptr,
+                                                 const int32_t* __restrict__ master_end,
+                                                 const double* __restrict__ stats, const double* __restrict__ uk,
+                                                 const int8_t* __restrict__ pl, const int32_t* __restrict__ node_slot,
+                                                 const int32_t* __restrict__ hstart, const int32_t* __restrict__ cell_nodes,
+                                                 const int32_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj_col,
+                                                 uint32_t hmask, const int32_t* __restrict__ pstart, int32_t* __restrict__ pa,
+                                                 int32_t* __restrict__ pb, double* __restrict__ pw,
+                                                 int32_t* __restrict__ status)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    const int base = pstart[i], room = pstart[i + 1] - base;
+    if (room == 0) return;
+    Master m;
+    if (!master_setup(i, node_seg, seg_node_ptr, master_end, stats, uk, pl, node_slot, adj_ptr, m)) return;
+    int js[PK_MAX_PARTNERS];
     int cnt = 0;
-    int js[FILL ? PK_MAX_PARTNERS : 1];
-    const int s = node_seg[i];
-    const double* st = stats + (size_t)s * PK_NSTAT;
-    const double xi = uk[3 * i], yi = uk[3 * i + 1], zi = uk[3 * i + 2];
-    if (node_slot[i] >= 0 && i < master_end[s]) {
-        const int gx = bin_index((xi - st[PK_ST_BB]) / st[PK_ST_BB + 2], 10);
-        const int gy = bin_index((yi - st[PK_ST_BB + 1]) / st[PK_ST_BB + 3], 5);
-        const double thr = st[PK_ST_GRID + 5 * gx + gy];
-        const double t2 = thr * thr;
-        if (t2 > 0.0) {
-            const double hc = st[PK_ST_CELL];
-            const int cx = (int)((xi - st[PK_ST_LO]) / hc);
-            const int cy = (int)((yi - st[PK_ST_LO + 1]) / hc);
-            const int cz = (int)((zi - st[PK_ST_LO + 2]) / hc);
-            uint32_t slots[27];
-            int ns = 0;
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const uint32_t sl = cell_hash(s, cx + dx, cy + dy, cz + dz, hmask);
-                        bool dup = false;
-                        for (int k = 0; k < ns; ++k) dup |= slots[k] == sl;
-                        if (!dup) slots[ns++] = sl;
-                    }
-            const int nend = seg_node_ptr[s + 1];
-            const int8_t pli = pl[i];
-            const int a0 = adj_ptr[i], a1 = adj_ptr[i + 1];
-            for (int k = 0; k < ns; ++k) {
-                const int q1 = hstart[slots[k] + 1];
-                for (int q = hstart[slots[k]]; q < q1; ++q) {
-                    const int j = cell_nodes[q];
-                    if (j <= i || j >= nend || pl[j] != pli) continue;
-                    const double dx = xi - uk[3 * j], dy = yi - uk[3 * j + 1], dz = zi - uk[3 * j + 2];
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    if (!(d2 < t2)) continue;
-                    int lo = a0, hi = a1;  // element neighbours of i, sorted
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (adj_col[mid] < j) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    if (lo < a1 && adj_col[lo] == j) continue;
-                    if (FILL && cnt < PK_MAX_PARTNERS) js[cnt] = j;
-                    cnt++;
-                }
-            }
+    uint32_t slots[27];
+    for (int k = 0; k < 27; ++k) {
+        const uint32_t sl = neighbour_slot(m, k, hmask);
+        bool dup = false;
+        for (int q = 0; q < k; ++q) dup |= slots[q] == sl;
+        slots[k] = sl;
+        if (dup) continue;
+        const int q1 = hstart[sl + 1];
+        for (int q = hstart[sl]; q < q1; ++q) {
+            const int j = cell_nodes[q];
+            if (!accept(m, i, j, uk, pl, adj_col)) continue;
+            if (cnt < PK_MAX_PARTNERS) js[cnt] = j;
+            cnt++;
         }
-    }
-    if (!FILL) {
-        pcount[i] = cnt;
-        return;
     }
     if (cnt > PK_MAX_PARTNERS) {
         atomicMax(status + 2, cnt);
         cnt = PK_MAX_PARTNERS;
     }
-    const int base = pstart[i];
-    cnt = min(cnt, pstart[i + 1] - base);  // never past the room the count pass reserved
+    cnt = min(cnt, room);  // never past the room the count pass reserved
     for (int k = 1; k < cnt; ++k) {  // ascending slave order
         const int v = js[k];
-        int m = k - 1;
-        while (m >= 0 && js[m] > v) { js[m + 1] = js[m]; --m; }
-        js[m + 1] = v;
+        int q = k - 1;
+        while (q >= 0 && js[q] > v) { js[q + 1] = js[q]; --q; }
+        js[q + 1] = v;
     }
     for (int k = 0; k < cnt; ++k) {
         const int j = js[k];
-        const double dx = xi - uk[3 * j], dy = yi - uk[3 * j + 1], dz = zi - uk[3 * j + 2];
+        const double dx = m.x - uk[3 * j], dy = m.y - uk[3 * j + 1], dz = m.z - uk[3 * j + 2];
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         pa[base + k] = i;
         pb[base + k] = j;
@@ -617,10 +703,9 @@ void packing_init(tf_handle* h, const int8_t* node_placement, const uint8_t* exp
     std::vector<uint8_t> basal(E, 0);
     std::vector<double> vref(E);
     for (int e = 0; e < E; ++e) {
-        bool ap = false, ba = false, lat = false;
+        bool ba = false, lat = false;
         for (int a = 0; a < 6; ++a) {
             const int p = pl[h->h_conn[6 * e + a]];
-            ap |= p == 1;
             ba |= p == 0;
             lat |= p == 3;
         }
@@ -633,6 +718,7 @@ void packing_init(tf_handle* h, const int8_t* node_placement, const uint8_t* exp
     pk.elem_basal = pk_alloc<uint8_t>(E);
     pk.vref = pk_alloc<double>(E);
     pk.mass = pk_alloc<double>(N);
+    pk.vpn = pk_alloc<double>(E);
     pk.seg_stats = pk_alloc<double>((size_t)h->S * PK_NSTAT);
     pk.f = pk_alloc<double>((size_t)3 * N);
     pk.d = pk_alloc<double>((size_t)3 * N);
@@ -651,7 +737,7 @@ void packing_free(tf_handle* h)
     PackingState& pk = h->pk;
     pk_free(pk.node_pl); pk_free(pk.elem_basal); pk_free(pk.vref); pk_free(pk.mass); pk_free(pk.seg_stats);
     pk_free(pk.f); pk_free(pk.d); pk_free(pk.adj_ptr); pk_free(pk.adj_col); pk_free(pk.node_seg);
-    pk_free(pk.master_end); pk_free(pk.seg_elem_ptr); pk_free(pk.ebin); pk_free(pk.ecur); pk_free(pk.hcnt);
+    pk_free(pk.master_end); pk_free(pk.seg_elem_ptr); pk_free(pk.ebin); pk_free(pk.ev); pk_free(pk.vpn); pk_free(pk.hcnt);
     pk_free(pk.hstart); pk_free(pk.cell_nodes); pk_free(pk.node_slot); pk_free(pk.pcount); pk_free(pk.pstart);
     pk_free(pk.scan_tmp); pk_free(pk.a); pk_free(pk.b); pk_free(pk.w); pk_free(pk.slot); pk_free(pk.nodes);
     pk_free(pk.nptr); pk_free(pk.nref);
@@ -697,7 +783,7 @@ void packing_prepare(tf_handle* h)
     pk.master_end = pk_alloc<int32_t>(S);
     pk.seg_elem_ptr = pk_alloc<int32_t>(S + 1);
     pk.ebin = pk_alloc<int32_t>(E);
-    pk.ecur = pk_alloc<double>(E);
+    pk.ev = pk_alloc<double>((size_t)5 * E);
     pk.hcnt = pk_alloc<int32_t>(H);
     pk.hstart = pk_alloc<int32_t>(H + 1);
     pk.cell_nodes = pk_alloc<int32_t>(N);
@@ -714,8 +800,9 @@ void packing_prepare(tf_handle* h)
 
 void launch_packing_mass(tf_handle* h)
 {
-    hipLaunchKernelGGL(k_pk_mass, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src, h->fg,
-                       h->pk.vref, h->ablated, h->E, h->N, h->pk.mass);
+    hipLaunchKernelGGL(k_pk_vpn, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->fg, h->pk.vref, h->E, h->pk.vpn);
+    hipLaunchKernelGGL(k_pk_mass, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->node_ptr, h->node_src,
+                       h->pk.vpn, h->ablated, h->N, h->pk.mass);
     TF_HIP(hipGetLastError());
 }
 
@@ -737,6 +824,7 @@ void packing_install(tf_handle* h, const std::vector<int32_t>& a, const std::vec
 {
     PackingState& pk = h->pk;
     const int P = (int)a.size(), N = h->N;
+    if (P == 0 && pk.n == 0) return;  // nothing in force before or after
     auto find_blk = [&](int A, int B) -> int {
         const int32_t* c0 = h->h_col.data() + h->h_row_ptr[A];
         const int32_t* c1 = h->h_col.data() + h->h_row_ptr[A + 1];
@@ -779,18 +867,22 @@ void packing_install(tf_handle* h, const std::vector<int32_t>& a, const std::vec
         nref[pos[a[p]]++] = p;
         nref[pos[b[p]]++] = ~p;
     }
-    pk_free(pk.nodes); pk_free(pk.nptr); pk_free(pk.nref);
-    pk.nodes = pk_alloc<int32_t>(nodes.size());
-    pk.nptr = pk_alloc<int32_t>(nptr.size());
-    pk.nref = pk_alloc<int32_t>(nref.size());
     if (P) {
+        if ((int)nodes.size() > pk.node_cap || 2 * P > pk.ref_cap) {  // node lists grow only
+            pk_free(pk.nodes); pk_free(pk.nptr); pk_free(pk.nref);
+            pk.node_cap = std::max<int>(2 * pk.node_cap, (int)nodes.size());
+            pk.ref_cap = std::max<int>(2 * pk.ref_cap, 2 * P);
+            pk.nodes = pk_alloc<int32_t>(pk.node_cap);
+            pk.nptr = pk_alloc<int32_t>(pk.node_cap + 1);
+            pk.nref = pk_alloc<int32_t>(pk.ref_cap);
+        }
         pk_h2d(h, pk.slot, slot.data(), slot.size());
         pk_h2d(h, pk.nodes, nodes.data(), nodes.size());
         pk_h2d(h, pk.nptr, nptr.data(), nptr.size());
         pk_h2d(h, pk.nref, nref.data(), nref.size());
+        launch_fill(h, pk.f, 0.0, 3 * (int64_t)N);
+        launch_fill(h, pk.d, 0.0, 3 * (int64_t)N);
     }
-    launch_fill(h, pk.f, 0.0, 3 * (int64_t)N);
-    launch_fill(h, pk.d, 0.0, 3 * (int64_t)N);
     pk.n = P;
     pk.nn = (int32_t)nodes.size();
     pk.h_a = a;
@@ -804,8 +896,10 @@ int packing_detect(tf_handle* h)
     const int N = h->N, S = h->S;
     const uint32_t hmask = (uint32_t)pk.H - 1u;
     launch_packing_mass(h);
-    hipLaunchKernelGGL(k_pk_stats, dim3(S), dim3(SWG), 0, h->stream, h->seg_node_ptr, pk.seg_elem_ptr, h->uk, h->conn, h->E,
-                       pk.elem_basal, pk.ebin, pk.ecur, pk.seg_stats);
+    hipLaunchKernelGGL(k_pk_evals, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->uk, h->conn, h->E, pk.elem_basal,
+                       pk.ev);
+    hipLaunchKernelGGL(k_pk_stats, dim3(S), dim3(SWG), 0, h->stream, h->seg_node_ptr, pk.seg_elem_ptr, h->uk, pk.ev, h->E,
+                       pk.ebin, pk.seg_stats);
     hipLaunchKernelGGL(k_fill_i32, dim3((pk.H + 255) / 256), dim3(256), 0, h->stream, pk.hcnt, 0, pk.H);
     hipLaunchKernelGGL(k_pk_cells, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg, pk.seg_stats, h->uk,
                        pk.mass, pk.node_pl, hmask, pk.node_slot, pk.hcnt);
@@ -814,9 +908,9 @@ int packing_detect(tf_handle* h)
     hipLaunchKernelGGL(k_copy_i32, dim3((pk.H + 255) / 256), dim3(256), 0, h->stream, pk.hcnt, pk.hstart, pk.H);
     hipLaunchKernelGGL(k_pk_fill_cells, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_slot, pk.hcnt,
                        pk.cell_nodes);
-    hipLaunchKernelGGL(k_pk_pairs<false>, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg, h->seg_node_ptr,
-                       pk.master_end, pk.seg_stats, h->uk, pk.node_pl, pk.node_slot, pk.hstart, pk.cell_nodes, pk.adj_ptr,
-                       pk.adj_col, hmask, pk.pcount, nullptr, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_pk_count, dim3((unsigned)(((size_t)N * 32 + 255) / 256)), dim3(256), 0, h->stream, N, pk.node_seg,
+                       h->seg_node_ptr, pk.master_end, pk.seg_stats, h->uk, pk.node_pl, pk.node_slot, pk.hstart,
+                       pk.cell_nodes, pk.adj_ptr, pk.adj_col, hmask, pk.pcount);
     TF_HIP(hipGetLastError());
     scan_i32(h, pk.pcount, N, pk.pstart);
     int32_t total = 0;
@@ -826,9 +920,9 @@ int packing_detect(tf_handle* h)
     if (total > 0) {
         packing_reserve(h, total);
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, h->stream, h->status + 2, 0, 1);
-        hipLaunchKernelGGL(k_pk_pairs<true>, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg,
-                           h->seg_node_ptr, pk.master_end, pk.seg_stats, h->uk, pk.node_pl, pk.node_slot, pk.hstart,
-                           pk.cell_nodes, pk.adj_ptr, pk.adj_col, hmask, nullptr, pk.pstart, pk.a, pk.b, pk.w, h->status);
+        hipLaunchKernelGGL(k_pk_fill, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg, h->seg_node_ptr,
+                           pk.master_end, pk.seg_stats, h->uk, pk.node_pl, pk.node_slot, pk.hstart, pk.cell_nodes,
+                           pk.adj_ptr, pk.adj_col, hmask, pk.pstart, pk.a, pk.b, pk.w, h->status);
         TF_HIP(hipGetLastError());
         int32_t over = 0;
         TF_HIP(hipMemcpyAsync(&over, h->status + 2, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
