@@ -81,6 +81,44 @@ public:
         check(tf_set_ablated_nodes(h_, (int32_t)n.size(), n.empty() ? nullptr : n.data()), h_);
     }
 
+    // Simulation::updateNodeMasses + detectPacingNodes on the current positions; the pairs act on
+    // every later assembly (calculatePackingForcesImplicit3D, calculatePackingJacobian3D).
+    int detectPacingNodes()
+    {
+        int32_t n = 0;
+        check(tf_detect_packing(h_, &n), h_);
+        return n;
+    }
+    // pacingNodeCouples0/1 with initialWeightPointx/y/z and packingThreshold (one per tissue),
+    // e.g. from the reference's own detection or a Save_Packing file.
+    void setPackingPairs(const std::vector<int>& couples0, const std::vector<int>& couples1,
+                         const std::vector<double>& wx, const std::vector<double>& wy, const std::vector<double>& wz,
+                         const std::vector<double>& packingThreshold)
+    {
+        const size_t n = couples0.size();
+        if (couples1.size() != n || wx.size() != n || wy.size() != n || wz.size() != n)
+            throw Error(TF_INVALID, "packing couples and weights differ in length");
+        std::vector<int32_t> p(2 * n);
+        std::vector<double> w(3 * n);
+        for (size_t k = 0; k < n; ++k) {
+            p[2 * k] = couples0[k];
+            p[2 * k + 1] = couples1[k];
+            w[3 * k] = wx[k];
+            w[3 * k + 1] = wy[k];
+            w[3 * k + 2] = wz[k];
+        }
+        check(tf_set_packing_pairs(h_, (int32_t)n, n ? p.data() : nullptr, n ? w.data() : nullptr,
+                                   packingThreshold.data()),
+              h_);
+    }
+    // Node::mass of every node from the current Fg
+    std::vector<double> nodeMasses()
+    {
+        std::vector<double> m(nNodes_);
+        check(tf_get_node_masses(h_, m.data()), h_);
+        return m;
+    }
+
     // ---- NewtonRaphsonSolver call sequence ----
     void constructUnMatrix() { check(tf_begin_step(h_), h_); }
     void initialteUkMatrix() {}  // folded into constructUnMatrix (uk <- un)
