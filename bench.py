@@ -73,7 +73,8 @@ def traffic_from_profiles(workload):
 
 
 def cpu_baseline(tissue, dt):
-    """One full timestep of the CPU oracle (growth, surfaces, NR loop with PARDISO)."""
+    """One full timestep of the CPU oracle (growth, surfaces, node masses and the literal O(N^2)
+    packing detection, NR loop with PARDISO)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the CPU oracle, timed as the baseline only
 
@@ -83,8 +84,9 @@ def cpu_baseline(tissue, dt):
     it, _ = o.timestep(dt)
     sec = time.perf_counter() - t0
     return {"value": 1.0 / sec, "unit": "timesteps/s", "cores": int(cores), "kind": "port",
-            "sample": f"one full implicit timestep (growth + surfaces + {it} NR iterations, literal 729-term "
-                      f"element tangent, CSR assembly, {o.solver_kind()} transposed solve) of the same workload",
+            "sample": f"one full implicit timestep (growth + surfaces + packing detection ({len(o.pairs)} pairs) + "
+                      f"{it} NR iterations, literal 729-term element tangent, CSR assembly, {o.solver_kind()} "
+                      f"transposed solve) of the same workload",
             "seconds": sec, "nr_iterations": it}
 
 
@@ -222,6 +224,8 @@ def main():
         "krylov_iterations_per_nr_iteration": float(np.sum(kr) / max(1, np.sum(nr))),
         "converged_steps": int(sum(st["converged"] for st in stats)),
         "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in stats)),
+        "packing": {"pairs_per_step_max": int(max(st["packing_pairs"] for st in stats)),
+                    "ms_growth_surfaces_detection_per_step": float(np.mean([st["ms_grow"] for st in stats]))},
         "roofline": {"kernel": KERNEL_NAMES.get(a.krylov, KERNEL_NAMES[-1]), "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(a.workload), "avg_launch_ms": spmv_ms, "launches": spmv_n,
