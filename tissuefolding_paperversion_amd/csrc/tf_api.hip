@@ -1031,15 +1031,18 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         const bool warm = h->opts.krylov_warm_start != 0 && it < 2;
         h->warm_guess = (warm && h->warm_valid[it]) ? h->warm[it] : nullptr;
         int kst;
+        h->krylov_call = it;
         try {
             kst = solve_impl(h);
         } catch (...) {
             h->seg_rtol_on = false;
             h->warm_guess = nullptr;
+            h->krylov_call = -1;
             throw;
         }
         h->seg_rtol_on = false;
         h->warm_guess = nullptr;
+        h->krylov_call = -1;
         if (warm) {  // keep this update as the guess for the next timestep's iteration it
             launch_copy(h, h->warm[it], h->du, 3 * (int64_t)h->N);
             h->warm_valid[it] = kst == TF_OK;

@@ -205,6 +205,11 @@ struct tf_handle {
     bool warm_valid[2] = {false, false};
     const double *warm_guess = nullptr;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
+    // Krylov iterations the solve of each Newton iteration took in the previous timestep: the
+    // next solve of that iteration replays most of that many iterations before its first
+    // convergence check (krylov_call: the Newton iteration being solved, -1 outside the loop)
+    std::vector<int32_t> krylov_pred;
+    int32_t krylov_call = -1;
 
     // graphs
     // one captured Krylov chunk per Jacobian value type (FP64, FP32)
