@@ -123,6 +123,14 @@ int tf_get_positions(tf_handle* h, double* x);
 /* Growth and material state (ShapeBase::Fg / InvFg, ShapeBase.cpp:1215-1251; lambda, mu). */
 int tf_set_growth_rates(tf_handle* h, const double* rate);
 int tf_set_fg(tf_handle* h, const double* Fg);
+/* Per-element growth increments for the following growth updates (tf_grow, tf_timestep): the
+ * product of the step's growth-function increments, row-major 3x3 per element, as
+ * Simulation::calculateGrowth builds them (Simulation.cpp:9651-9670, ShapeBase::updateGrowthIncrement
+ * ShapeBase.cpp:567-586) -- angled uniform and ring functions (calculateFgFromRates,
+ * ShapeBase.cpp:396-421) and grid functions (calculateFgFromGridCorners, :432-565).  They replace
+ * the diagonal exp(rate dt) increments; the growth-rotation rule still applies.  NULL returns to
+ * the rates. */
+int tf_set_growth_increments(tf_handle* h, const double* dFg);
 int tf_get_fg(tf_handle* h, double* Fg, double* InvFg);
 int tf_set_material(tf_handle* h, const double* lambda, const double* mu, const double* eta_int);
 

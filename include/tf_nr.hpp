@@ -177,6 +177,12 @@ public:
     void setPositions(const std::vector<double>& x) { check(tf_set_positions(h_, x.data()), h_); }
     // ShapeBase::setFg for every element: Fg row-major 3x3 per element (InvFg is derived)
     void setFg(const std::vector<double>& Fg) { check(tf_set_fg(h_, Fg.data()), h_); }
+    // ShapeBase::growthIncrement of every element (row-major 3x3), as Simulation::calculateGrowth
+    // builds it from the growth functions; an empty vector returns to the growth rates
+    void setGrowthIncrements(const std::vector<double>& dFg)
+    {
+        check(tf_set_growth_increments(h_, dFg.empty() ? nullptr : dFg.data()), h_);
+    }
     void elementDump(int e, double dt, double fe[18], double fv[18], double Ke[324], double Kv[324])
     {
         check(tf_element_dump(h_, e, dt, fe, fv, Ke, Kv), h_);

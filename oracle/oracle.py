@@ -67,7 +67,7 @@ def lib():
         L.tfo_pardiso_solve_transposed.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_int]
         L.tfo_num_threads.restype = C.c_int
         L.tfo_polar_rotation.argtypes = [dp, dp]
-        L.tfo_grow_all.argtypes = [C.c_int, ip, dp, dp, dp, C.c_double, C.c_int, dp, dp]
+        L.tfo_grow_all.argtypes = [C.c_int, ip, dp, dp, dp, C.c_double, C.c_int, dp, dp, C.c_void_p]
         L.tfo_grow_all.restype = C.c_int
         L.tfo_mean_F.argtypes = [dp, dp, dp]
         L.tfo_inv3.argtypes = [dp, dp]
@@ -263,6 +263,7 @@ class OracleTissue:
         self.stats = []
         self.bound = []      # slaveMasterList: (slave DoF, master DoF) pairs in binding order
         self.ablated = []    # AblatedNodes
+        self.growth_inc = None  # per-element growth increments (E, 3, 3) replacing the rates, or None
         # implicit packing (Simulation.cpp:37): pair list of the current step
         self.packing = packing
         self.placement = _c(m.node_placement, np.int8)
@@ -285,8 +286,10 @@ class OracleTissue:
 
     # -- per-step updates ------------------------------------------------------------------
     def grow(self, dt):
+        inc = None if self.growth_inc is None else _c(self.growth_inc).ravel()
         self.last_rotations = lib().tfo_grow_all(self.E, self.conn, self.Xref, self.pos, self.rates, dt,
-                                                 1 if self.allow_rotation else 0, self.Fg, self.InvFg)
+                                                 1 if self.allow_rotation else 0, self.Fg, self.InvFg,
+                                                 None if inc is None else inc.ctypes.data)
         return self.last_rotations
 
     def update_viscosity_surfaces(self):

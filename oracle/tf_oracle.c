@@ -793,10 +793,22 @@ void tfo_polar_rotation(const double F[9], double R[9])
  * Reproduces updateGrowthRotationMatrices -> cleanUpGrowthRates -> calculateGrowthUniform ->
  * growShapeByFg for a columnar element with no plastic deformation or shape change.
  * Returns 1 when the growth increment was rotated. */
+int tfo_grow_element_inc(const double inc_in[9], const double F_mean[9], int allow_rotation, double Fg[9],
+                         double InvFg[9]);
 int tfo_grow_element(const double rates[3], double dt, const double F_mean[9], int allow_rotation, double Fg[9],
                      double InvFg[9])
 {
     double inc[9] = {exp(rates[0] * dt), 0, 0, 0, exp(rates[1] * dt), 0, 0, 0, exp(rates[2] * dt)};
+    return tfo_grow_element_inc(inc, F_mean, allow_rotation, Fg, InvFg);
+}
+
+/* The same update from a given growth increment (the product of the step's growth functions'
+ * increments, ShapeBase::updateGrowthIncrement, ShapeBase.cpp:567-586). */
+int tfo_grow_element_inc(const double inc_in[9], const double F_mean[9], int allow_rotation, double Fg[9],
+                         double InvFg[9])
+{
+    double inc[9];
+    memcpy(inc, inc_in, sizeof(inc));
     int rotated = 0;
     if (allow_rotation) {
         double R[9];
@@ -849,9 +861,10 @@ void tfo_mean_F(const double X[18], const double x[18], double Fm[9])
     }
 }
 
-/* Growth update for all elements; returns the number of rotated increments. */
+/* Growth update for all elements from rates (inc == NULL) or from per-element increments
+ * (inc[9e..9e+8], row-major); returns the number of rotated increments. */
 int tfo_grow_all(int n_elem, const int32_t* conn, const double* Xref, const double* pos, const double* rates, double dt,
-                 int allow_rotation, double* Fg, double* InvFg)
+                 int allow_rotation, double* Fg, double* InvFg, const double* inc)
 {
     int nrot = 0;
 #pragma omp parallel for reduction(+ : nrot) schedule(static)
@@ -860,7 +873,8 @@ int tfo_grow_all(int n_elem, const int32_t* conn, const double* Xref, const doub
         for (int a = 0; a < NN; ++a)
             for (int i = 0; i < 3; ++i) x[3 * a + i] = pos[3 * conn[6 * e + a] + i];
         tfo_mean_F(Xref + 18 * e, x, Fm);
-        nrot += tfo_grow_element(rates + 3 * e, dt, Fm, allow_rotation, Fg + 9 * e, InvFg + 9 * e);
+        nrot += inc ? tfo_grow_element_inc(inc + 9 * e, Fm, allow_rotation, Fg + 9 * e, InvFg + 9 * e)
+                    : tfo_grow_element(rates + 3 * e, dt, Fm, allow_rotation, Fg + 9 * e, InvFg + 9 * e);
     }
     return nrot;
 }

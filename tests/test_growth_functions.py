@@ -1,0 +1,252 @@
+"""Pins of the host-side growth functions (uniform at an angle, ring, grid) and of the geometry
+they read (bounding box, relative positions, column copies, initial heights).
+
+Checks: closed-form increments (R diag(exp(r dt)) R^T, products in function order), bilinear
+interpolation reproducing a linear rate field exactly, the step interpolation picking the nearest
+corner, hand-worked corner-angle cases (a 90-degree corner rotated back with its x/y rates swapped,
+eliminated corners renormalised away), the ring's linear profile, the z-range test and the
+grid-file layout.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from tissuefolding_paperversion_amd import configs
+from tissuefolding_paperversion_amd import growth as G
+from tissuefolding_paperversion_amd import modelinput
+from tissuefolding_paperversion_amd.mesh import APICAL, BASAL, ellipse_mesh
+
+
+def _rot(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+
+
+@pytest.fixture(scope="module")
+def mesh3():
+    return ellipse_mesh(40.0, 25.0, 5.0, (0.0, 2.0, 7.0, 12.0), seed=3)
+
+
+def test_uniform_at_angle_and_function_order(mesh3):
+    geo = G.GrowthGeometry(mesh3)
+    dt = 60.0
+    f1 = G.UniformGrowth(0.0, 1e9, (3e-5, 1e-5, 0.0), math.radians(30))
+    f2 = G.UniformGrowth(0.0, 1e9, (0.0, 2e-5, 1e-6), math.radians(-50))
+    inc = G.increments([f1, f2], geo, mesh3.pos, 0.0, dt)
+    D1 = np.diag(np.exp(np.array(f1.rate) * dt))
+    D2 = np.diag(np.exp(np.array(f2.rate) * dt))
+    A = _rot(f1.angle) @ D1 @ _rot(f1.angle).T
+    B = _rot(f2.angle) @ D2 @ _rot(f2.angle).T
+    np.testing.assert_allclose(inc[0], B @ A, rtol=1e-15, atol=1e-15)
+    assert not np.allclose(B @ A, A @ B, rtol=0, atol=1e-12)  # the order is tested
+    # inactive outside its time window
+    f3 = G.UniformGrowth(100.0, 200.0, (1e-3, 1e-3, 1e-3))
+    np.testing.assert_array_equal(G.increments([f3], geo, mesh3.pos, 0.0, dt)[5], np.eye(3))
+
+
+def _grid(nx, ny, fn, angle=0.0):
+    rates = np.zeros((nx, ny, 3))
+    for i in range(nx):
+        for j in range(ny):
+            rates[i, j] = fn(i, j)
+    return G.GridGrowth(0.0, 1e9, rates, np.full((nx, ny), angle))
+
+
+def test_grid_constant_equals_uniform(mesh3):
+    geo = G.GrowthGeometry(mesh3)
+    r = (2e-5, 1e-5, 0.0)
+    for interp in (0, 1):
+        inc = G.increments([_grid(5, 4, lambda i, j: r)], geo, mesh3.pos, 0.0, 60.0, True, interp)
+        np.testing.assert_allclose(inc, np.tile(np.diag(np.exp(np.array(r) * 60.0)), (mesh3.n_elements, 1, 1)),
+                                   rtol=1e-15)
+
+
+def test_grid_bilinear_reproduces_linear_field(mesh3):
+    geo = G.GrowthGeometry(mesh3)
+    a, b, c = 1e-5, 4e-6, -2e-6
+    nx, ny = 6, 5
+    gf = _grid(nx, ny, lambda i, j: (a + b * i + c * j, 1e-5, 0.0))
+    inc = G.increments([gf], geo, mesh3.pos, 0.0, 60.0, True, 1)
+    rel = geo.initial_rel
+    px = np.clip(rel[:, 0] * (nx - 1), 0, nx - 1)
+    py = np.clip(rel[:, 1] * (ny - 1), 0, ny - 1)
+    expect = np.exp((a + b * px + c * py) * 60.0)
+    np.testing.assert_allclose(inc[:, 0, 0], expect, rtol=1e-13)
+
+
+def test_grid_step_picks_nearest_corner(mesh3):
+    geo = G.GrowthGeometry(mesh3)
+    nx, ny = 4, 3
+    gf = _grid(nx, ny, lambda i, j: (1e-6 * (1 + i + 10 * j), 1e-6, 0.0))
+    inc = G.increments([gf], geo, mesh3.pos, 0.0, 60.0, True, 0)
+    ix, iy, fx, fy = G.grid_index(geo.initial_rel, nx, ny)
+    ci = ix + (fx > 0.5)
+    cj = iy + (fy > 0.5)
+    np.testing.assert_allclose(inc[:, 0, 0], np.exp(1e-6 * (1 + ci + 10 * cj) * 60.0), rtol=1e-15)
+
+
+def test_compatible_angles_hand_cases():
+    deg = math.radians
+    rates = np.zeros((2, 2, 3))
+    rates[:, :, 0], rates[:, :, 1] = 3e-5, 1e-5          # aspect ratio 3: angles kept
+    angles = np.array([[deg(10), deg(10)], [deg(100), deg(10)]])   # corners 0=(0,0) 1=(1,0) 2=(0,1) 3=(1,1)
+    g, ang, elim = G.compatible_corners(rates, angles)[(0, 0)]
+    np.testing.assert_allclose(ang, [deg(10)] * 4, atol=1e-15)     # 100 deg rotated back by 90
+    np.testing.assert_allclose(g[1], [1e-5, 3e-5, 0.0])            # ... with x and y swapped
+    np.testing.assert_allclose(g[0], [3e-5, 1e-5, 0.0])
+    assert not elim.any()
+    # corner 3 nearly isotropic: its angle is eliminated, the bilinear mean renormalised
+    rates[1, 1, 1] = 2.9e-5
+    gf = G.GridGrowth(0.0, 1e9, rates, angles)
+    g, ang, elim = gf.corners[(0, 0)]
+    assert elim.tolist() == [False, False, False, True]
+    inc = G.grid_increments(gf, np.array([[0.5, 0.5]]), np.zeros(1), 60.0, 1)
+    growth = (np.array(g) * 0.25).sum(axis=0)
+    R = _rot(deg(10))                                              # (3 x 10 deg x 0.25) / 0.75
+    np.testing.assert_allclose(inc[0], R @ np.diag(np.exp(growth * 60.0)) @ R.T, rtol=1e-14, atol=1e-15)
+    # -80 deg lies within 45 deg of 100 deg once brought into [0, pi]
+    angles2 = np.array([[deg(100), deg(-80)], [deg(100), deg(100)]])
+    _, ang2, _ = G.compatible_corners(rates, angles2)[(0, 0)]
+    np.testing.assert_allclose(ang2[:3], [deg(100)] * 3, atol=1e-13)
+
+
+def test_ring_profile(mesh3):
+    geo = G.GrowthGeometry(mesh3)
+    f = G.RingGrowth(0.0, 1e9, (0.0, 0.0), 10.0, 30.0, (4e-5, 4e-5, 0.0))
+    inc = G.increments([f], geo, mesh3.pos, 0.0, 60.0)
+    cen = mesh3.pos[mesh3.conn].mean(axis=1)
+    d = np.hypot(cen[:, 0], cen[:, 1])
+    inside = (d > 10.0) & (d < 30.0)
+    sf = 1.0 - (d - 10.0) / 20.0
+    np.testing.assert_allclose(inc[inside, 0, 0], np.exp(4e-5 * sf[inside] * 60.0), rtol=1e-8)
+    np.testing.assert_array_equal(inc[~inside], np.tile(np.eye(3), ((~inside).sum(), 1, 1)))
+    assert inside.sum() > 10 and (~inside).sum() > 10
+
+
+def test_geometry_height_columns_and_zrange(mesh3):
+    geo = G.GrowthGeometry(mesh3)
+    assert geo.height == pytest.approx(12.0) and geo.layers == 3
+    pl, _ = mesh3.element_placement()
+    # apical relative height 0, basal 1 (mid-layer between)
+    cz = mesh3.pos[mesh3.conn][:, :, 2].mean(axis=1)
+    np.testing.assert_allclose(geo.initial_z, 1.0 - cz / 12.0, atol=1e-15)
+    # the basal element of a column takes the apical element's relative position, the middle one
+    # keeps its own (the reference copies to the first layers - 2 elements of the column)
+    ap, col = next(iter(geo.columns.items()))
+    assert len(col) == 3 and pl[col[0]] == BASAL and pl[col[2]] == APICAL
+    np.testing.assert_array_equal(geo.initial_rel[col[0]], geo.initial_rel[col[2]])
+    own = (mesh3.pos[mesh3.conn[col[1]]].sum(axis=0) / 6)[:2]
+    lo, hi = G.reference_bounding_box(mesh3.pos)
+    np.testing.assert_allclose(geo.initial_rel[col[1]], (own - lo[:2]) / (hi[:2] - lo[:2]), rtol=1e-15)
+    # zRange [0, 0.3]: only elements with their initial relative height in range grow
+    gf = _grid(3, 3, lambda i, j: (2e-5, 2e-5, 0.0))
+    gf.zmin, gf.zmax = 0.0, 0.3
+    inc = G.increments([gf], geo, mesh3.pos, 0.0, 60.0)
+    grow = inc[:, 0, 0] > 1.0
+    np.testing.assert_array_equal(grow, geo.initial_z <= 0.3)
+    assert grow.any() and not grow.all()
+
+
+def test_grid_file_layout(tmp_path):
+    """Rows are written from the last y index down; rates per hour, angles in degrees."""
+    nx, ny = 3, 2
+    rows = []
+    for j in reversed(range(ny)):
+        rows.append(" ".join(f"{10 * i + j} {100 + 10 * i + j} 0" for i in range(nx)))
+    for j in reversed(range(ny)):
+        rows.append(" ".join(f"{i + 5 * j}" for i in range(nx)))
+    p = tmp_path / "grid"
+    p.write_text(f"{nx} {ny}\n" + "\n".join(rows) + "\n")
+    rates, angles = G.read_grid_file(str(p))
+    for i in range(nx):
+        for j in range(ny):
+            assert rates[i, j, 0] == pytest.approx((10 * i + j) / 3600.0)
+            assert rates[i, j, 1] == pytest.approx((100 + 10 * i + j) / 3600.0)
+            assert angles[i, j] == pytest.approx(math.radians(i + 5 * j))
+
+
+GRID_INPUT = """InputMeshParameters:
+  MeshInputMode(int-seeDocumentation): 4
+  MeshFile(full-path): {mesh}
+  symmetricInX(bool): 0
+  symmetricInY(bool): 0
+TimeParameters:
+  TimeStep(sec): 60
+  SimulationLength(sec): 600
+PysicalProperties:
+  YoungsModulusApical: 1000
+  YoungsModulusBasal: 1000
+  YoungsModulusMid: 1000
+  PoissonsRatio: 0.3
+  ApicalViscosity(Pa-sec): 0
+  BasalViscosity(Pa-sec): 0
+  MidLineViscosity(Pa-sec): 0
+ExternalViscositySetup:
+  ExtendToWholeTissue: 0
+  DiscProperApicalExternalViscosity: 200
+  DiscProperBasalExternalViscosity: 200
+GrowthOptions:
+  NumberofGrowthFunctions(int): 2
+  GridGrowthsPinnedOnInitialMesh(bool): {pinned}
+  PinningUpdateTimes(number-times(sec)):  1 180
+  GridGrowthsInterpolationType(0=step,1=linear): 1
+  GrowthFunctionType(int-seeDocumentation): 3
+  InitialTime(sec): 0
+  FinalTime(sec): 100000
+  ApplyToColumnarLayer(bool): 1
+  ApplyToPeripodialMembrane(bool): 0
+  ApplyToBasalECM(bool): 0
+  ApplyToLateralECM(bool): 0
+  Filename(full-path): grid_rates
+  zRange: 0.0 1.0
+  GrowthFunctionType(int-seeDocumentation): 2
+  InitialTime(sec): 0
+  FinalTime(sec): 100000
+  ApplyToColumnarLayer(bool): 1
+  ApplyToPeripodialMembrane(bool): 0
+  ApplyToBasalECM(bool): 0
+  ApplyToLateralECM(bool): 0
+  Centre: 5.0 0.0
+  InnerRadius: 5.0
+  OuterRadius: 25.0
+  MaxValue(fractionPerHour-DV,AP,AB): 0.05 0.02 0.0
+  Angle(degrees): 20
+"""
+
+
+def write_grid_input(tmp_path, pinned=1, nx=5, ny=4):
+    """A model input with a grid function (anisotropic rates, angles from -60 to 80 degrees) and
+    a ring function, on the 40 x 25 um ellipse written as a .mesh file."""
+    from tissuefolding_paperversion_amd.mesh import write_mesh
+    m = ellipse_mesh(40.0, 25.0, 5.0, (0.0, 2.0, 7.0, 12.0), seed=3)
+    write_mesh(m, str(tmp_path / "tissue.mesh"))
+    rng = np.random.default_rng(12)
+    rows = []
+    for j in reversed(range(ny)):
+        rows.append(" ".join(f"{rng.uniform(0.01, 0.06):.4f} {rng.uniform(0.005, 0.03):.4f} 0.0" for _ in range(nx)))
+    for j in reversed(range(ny)):
+        rows.append(" ".join(f"{rng.uniform(-60, 80):.2f}" for _ in range(nx)))
+    (tmp_path / "grid_rates").write_text(f"{nx} {ny}\n" + "\n".join(rows) + "\n")
+    p = tmp_path / "modelinput"
+    p.write_text(GRID_INPUT.format(mesh=str(tmp_path / "tissue.mesh"), pinned=pinned))
+    return str(p)
+
+
+def test_grid_model_input_reads(tmp_path):
+    mi = modelinput.read(write_grid_input(tmp_path))
+    assert mi.unsupported() == []
+    assert mi.needs_increments() and mi.grid_pinned and mi.pin_times == (180.0,) and mi.grid_interpolation == 1
+    fs = mi.growth_functions()
+    assert [f.kind for f in fs] == [3, 2]
+    assert fs[0].rates.shape == (5, 4, 3)
+    assert (fs[1].inner, fs[1].outer, fs[1].centre) == (5.0, 25.0, (5.0, 0.0))
+    assert fs[1].rate[0] == pytest.approx(0.05 / 3600) and fs[1].angle == pytest.approx(math.radians(20))
+    t = modelinput.build_tissue(mi)
+    tr = modelinput.GrowthTracker(mi, t.mesh)
+    inc = tr.increments(0.0, lambda: t.mesh.pos.ravel())
+    assert inc.shape == (t.n_elements, 3, 3)
+    assert np.all(np.linalg.det(inc) > 1.0)
+    # some elements grow at an angle (off-diagonal xy terms)
+    assert np.abs(inc[:, 0, 1]).max() > 1e-6

@@ -28,7 +28,8 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune", "tf_set_bound_dofs",
            "tf_set_ablated_nodes", "tf_set_growth_state", "tf_get_growth_rotation", "tf_apply_jacobian",
            "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get",
-           "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs"]
+           "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs",
+           "tf_set_growth_increments"]
 
 
 class TfError(RuntimeError):
@@ -124,6 +125,7 @@ def load(path: str | None = None):
         "tf_detect_packing": ([vp, vp], i32),
         "tf_set_packing_pairs": ([vp, i32, vp, vp, vp], i32),
         "tf_get_packing_pairs": ([vp, vp, vp, vp, vp, vp], i32),
+        "tf_set_growth_increments": ([vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -71,7 +71,7 @@ int fail(tf_handle* h, int code, const std::string& msg)
 
 void free_all(tf_handle* h)
 {
-    void* ptrs[] = {h->conn, h->xref, h->fg, h->invfg, h->lam, h->mu, h->eta, h->rate, h->exposed, h->elem_seg,
+    void* ptrs[] = {h->conn, h->xref, h->fg, h->invfg, h->lam, h->mu, h->eta, h->rate, h->ginc, h->exposed, h->elem_seg,
                     h->uk, h->un, h->g, h->du, h->eta_ext, h->fixed, h->sa, h->face_area, h->kelem, h->felem,
                     h->status, h->row_ptr, h->col, h->blk_row, h->diag_blk, h->val, h->val32, h->dinv, h->blk_ptr,
                     h->blk_src, h->node_ptr, h->node_src, h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr,
@@ -713,6 +713,23 @@ int tf_set_growth_rates(tf_handle* h, const double* rate)
     std::vector<double> r = to_soa(rate, h->E, 3);
     h2d(h, h->rate, r.data(), r.size());
     TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_growth_increments(tf_handle* h, const double* dFg)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (!dFg) {
+        if (h->ginc) { (void)hipFree(h->ginc); h->ginc = nullptr; }
+        return TF_OK;
+    }
+    for (size_t k = 0; k < (size_t)9 * h->E; ++k)
+        if (!std::isfinite(dFg[k])) return fail(h, TF_INVALID, "growth increments must be finite");
+    if (!h->ginc) h->ginc = dev_alloc<double>((size_t)9 * h->E);
+    std::vector<double> soa = to_soa(dFg, h->E, 9);
+    h2d(h, h->ginc, soa.data(), soa.size());
     return TF_OK;
     TF_GUARD_END(h)
 }
@@ -1394,6 +1411,7 @@ static int adjoint_supported(tf_handle* h)
     if (h->n_bound) return fail(h, TF_INVALID, "the adjoint does not support bound DoFs");
     if (h->ablated) return fail(h, TF_INVALID, "the adjoint does not support ablated nodes");
     if (h->pk.n) return fail(h, TF_INVALID, "the adjoint does not support implicit-packing pairs");
+    if (h->ginc) return fail(h, TF_INVALID, "the adjoint differentiates rate-based growth, not given increments");
     return TF_OK;
 }
 

@@ -427,7 +427,8 @@ __device__ inline void polar_rotation(const double* F, double* R)
 
 __global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, const double* __restrict__ xref,
                                               const double* __restrict__ pos, const double* __restrict__ rate,
-                                              double dt, int allow_rotation, int E, double* __restrict__ fg,
+                                              const double* __restrict__ ginc, double dt, int allow_rotation, int E,
+                                              double* __restrict__ fg,
                                               double* __restrict__ invfg, double* __restrict__ rot_tz,
                                               int32_t* __restrict__ status)
 {
@@ -435,6 +436,9 @@ __global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, 
     if (e >= E) return;
     double applied_tz = 0.0;
     double inc[9] = {exp(rate[e] * dt), 0, 0, 0, exp(rate[E + e] * dt), 0, 0, 0, exp(rate[2 * E + e] * dt)};
+    if (ginc)  // the product of the step's growth functions' increments, computed by the host
+#pragma unroll
+        for (int q = 0; q < 9; ++q) inc[q] = ginc[(size_t)q * E + e];
     if (allow_rotation) {
         // mean F at the current positions (ShapeBase.cpp:1676-1689)
         double X[18], x[18];
@@ -651,8 +655,8 @@ void launch_element_dump(tf_handle* h, int32_t e, double dt, double* d_out)
 
 void launch_grow(tf_handle* h, double dt)
 {
-    hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->xref, h->uk, h->rate, dt,
-                       h->opts.growth_rotation, h->E, h->fg, h->invfg, h->rot_tz, h->status);
+    hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->xref, h->uk, h->rate,
+                       h->ginc, dt, h->opts.growth_rotation, h->E, h->fg, h->invfg, h->rot_tz, h->status);
     TF_HIP(hipGetLastError());
 }
 

@@ -126,6 +126,7 @@ struct tf_handle {
     double *fg = nullptr, *invfg = nullptr;  // [9][E]
     double *lam = nullptr, *mu = nullptr, *eta = nullptr;  // [E]
     double *rate = nullptr;       // [3][E]
+    double *ginc = nullptr;       // [9][E] growth increments given by the host (replace the rates), or null
     uint8_t *exposed = nullptr;   // [E]
     int32_t *elem_seg = nullptr;  // [E]
     int any_viscous = 0;

@@ -12,13 +12,15 @@ keeps the keys of the sections that drive the implicit step:
   viscosity (``Simulation::checkForNodeFixing``, Simulation.cpp:3695-3800, ``fixX`` :3416-3430);
 - ``PysicalProperties`` (:1346-1434): E by placement, Poisson's ratio, internal viscosity;
 - ``ExternalViscositySetup`` (:750-813): disc-proper apical and basal external viscosity;
-- ``GrowthOptions`` (:268-441): growth functions of type 1 (uniform, ``calculateGrowthUniform``,
-  Simulation.cpp:9757-9797) with a time window, applied to the columnar layer.
+- ``GrowthOptions`` (:268-441, :443-707): growth functions of type 1 (uniform, optionally at an
+  angle), 2 (ring) and 3 (grid from a rate file), applied to the columnar layer, with the grid
+  options (pinned on the initial mesh, pinning update times, step or linear interpolation); their
+  per-step increments are evaluated by :mod:`growth`.
 
 Anything else the file switches on, and the step cannot reproduce, is reported by
 :meth:`ModelInput.unsupported` and refused by :func:`build_tissue`: explicit ECM, symmetry
-planes, peripodial membrane, ring and grid growth, growth at an angle, node binding by ellipses,
-adhesion and collapse, noise on the physical properties.
+planes, peripodial membrane, growth of the peripodial membrane or the ECM, node binding by
+ellipses, adhesion and collapse, noise on the physical properties.
 """
 from __future__ import annotations
 
@@ -29,6 +31,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import configs
+from . import growth as G
 from .mesh import APICAL, BASAL, TissueMesh, read_mesh
 
 
@@ -80,6 +83,10 @@ class GrowthFunction:
     lateral_ecm: bool
     rate: tuple[float, float, float]   # per second (DV, AP, AB)
     angle: float                       # radians
+    centre: tuple[float, float] = (0.0, 0.0)   # ring
+    radii: tuple[float, float] = (0.0, 0.0)    # ring: inner, outer
+    filename: str = ""                          # grid: rate file
+    zrange: tuple[float, float] = (0.0, 1.0)    # grid
 
 
 @dataclass
@@ -101,6 +108,9 @@ class ModelInput:
     ext_extend: bool = False
     ext_visc: tuple[float, float] = (0.0, 0.0)                 # disc proper apical, basal
     growth: list[GrowthFunction] = field(default_factory=list)
+    grid_pinned: bool = True                                   # GridGrowthsPinnedOnInitialMesh
+    pin_times: tuple = ()                                      # PinningUpdateTimes (s)
+    grid_interpolation: int = 1                                # 0 = step, 1 = linear
     flags: dict = field(default_factory=dict)                  # other switches, by name
 
     def unsupported(self) -> list[str]:
@@ -117,13 +127,35 @@ class ModelInput:
             if self.flags.get(name):
                 bad.append(name)
         for k, g in enumerate(self.growth):
-            if g.kind != 1:
-                bad.append(f"growth function {k}: type {g.kind} (only type 1, uniform)")
-            elif g.angle != 0.0:
-                bad.append(f"growth function {k}: angle {math.degrees(g.angle):g} deg")
+            if g.kind not in (1, 2, 3):
+                bad.append(f"growth function {k}: type {g.kind} (types 1, 2 and 3 only)")
             elif g.peripodial or g.basal_ecm or g.lateral_ecm:
                 bad.append(f"growth function {k}: applied to peripodial membrane or ECM")
+            elif g.kind == 3 and g.columnar and not os.path.exists(self._grid_path(g)):
+                bad.append(f"growth function {k}: grid rate file {g.filename} not found")
         return bad
+
+    def _grid_path(self, g: GrowthFunction) -> str:
+        return g.filename if os.path.isabs(g.filename) else os.path.join(os.path.dirname(self.path), g.filename)
+
+    def needs_increments(self) -> bool:
+        """True when some columnar growth function is not a diagonal rate: ring, grid, or an angle."""
+        return any(g.columnar and (g.kind != 1 or g.angle != 0.0) for g in self.growth)
+
+    def growth_functions(self) -> list:
+        """The columnar growth functions as :mod:`growth` objects, in the file's order."""
+        out = []
+        for g in self.growth:
+            if not g.columnar:
+                continue
+            if g.kind == 1:
+                out.append(G.UniformGrowth(g.t0, g.t1, g.rate, g.angle))
+            elif g.kind == 2:
+                out.append(G.RingGrowth(g.t0, g.t1, g.centre, g.radii[0], g.radii[1], g.rate, g.angle))
+            elif g.kind == 3:
+                rates, angles = G.read_grid_file(self._grid_path(g))
+                out.append(G.GridGrowth(g.t0, g.t1, rates, angles, g.zrange[0], g.zrange[1]))
+        return out
 
     def rates_at(self, mesh: TissueMesh, time_sec: float) -> np.ndarray:
         """Per-element growth rates (per second) of the uniform functions active at time_sec:
@@ -133,6 +165,15 @@ class ModelInput:
             if g.kind == 1 and g.columnar and g.t0 <= time_sec < g.t1:
                 r += np.asarray(g.rate)
         return r
+
+
+def _f32(v) -> float:
+    return float(np.float32(float(v)))
+
+
+def _f32_rate(v) -> float:
+    """A rate read into a float and scaled to per second in place (ModelInputObject.cpp:412-420)."""
+    return float(np.float32(float(np.float32(float(v))) * (1.0 / 3600.0)))
 
 
 def _b(v):
@@ -189,6 +230,14 @@ def read(path: str) -> ModelInput:
                 mi.flags[name] = _b(vals[0])
     # growth functions: the keys repeat per function, in the reader's order
     g = S.get("GrowthOptions", [])
+    opts = {k.split("(")[0]: vals for k, vals in g if not k.startswith("GrowthFunctionType")}
+    if "GridGrowthsPinnedOnInitialMesh" in opts:
+        mi.grid_pinned = _b(opts["GridGrowthsPinnedOnInitialMesh"][0])
+    if "PinningUpdateTimes" in opts and opts["PinningUpdateTimes"]:
+        n = int(opts["PinningUpdateTimes"][0])
+        mi.pin_times = tuple(float(x) for x in opts["PinningUpdateTimes"][1:1 + n])
+    if "GridGrowthsInterpolationType" in opts:
+        mi.grid_interpolation = int(opts["GridGrowthsInterpolationType"][0])
     i = 0
     while i < len(g):
         k, vals = g[i]
@@ -199,13 +248,23 @@ def read(path: str) -> ModelInput:
             while j < len(g) and not g[j][0].startswith("GrowthFunctionType"):
                 fields[g[j][0].split("(")[0]] = g[j][1]
                 j += 1
+            common = (_f32(fields["InitialTime"][0]), _f32(fields["FinalTime"][0]),
+                      _b(fields["ApplyToColumnarLayer"][0]), _b(fields["ApplyToPeripodialMembrane"][0]),
+                      _b(fields["ApplyToBasalECM"][0]), _b(fields["ApplyToLateralECM"][0]))
             if kind == 1:
-                r = [float(x) / 3600.0 for x in fields["MaxValue"][:3]]
-                mi.growth.append(GrowthFunction(1, float(fields["InitialTime"][0]), float(fields["FinalTime"][0]),
-                                                _b(fields["ApplyToColumnarLayer"][0]),
-                                                _b(fields["ApplyToPeripodialMembrane"][0]),
-                                                _b(fields["ApplyToBasalECM"][0]), _b(fields["ApplyToLateralECM"][0]),
-                                                (r[0], r[1], r[2]), math.radians(float(fields["Angle"][0]))))
+                r = [_f32_rate(x) for x in fields["MaxValue"][:3]]
+                ang = float(np.float32(float(np.float32(float(fields["Angle"][0]))) * (math.pi / 180.0)))
+                mi.growth.append(GrowthFunction(1, *common, (r[0], r[1], r[2]), ang))
+            elif kind == 2:
+                r = [_f32_rate(x) for x in fields["MaxValue"][:3]]
+                mi.growth.append(GrowthFunction(2, *common, (r[0], r[1], r[2]),
+                                                float(fields["Angle"][0]) * (math.pi / 180.0),
+                                                centre=(_f32(fields["Centre"][0]), _f32(fields["Centre"][1])),
+                                                radii=(_f32(fields["InnerRadius"][0]), _f32(fields["OuterRadius"][0]))))
+            elif kind == 3:
+                z = fields.get("zRange", ["0", "1"])
+                mi.growth.append(GrowthFunction(3, *common, (0.0, 0.0, 0.0), 0.0, filename=fields["Filename"][0],
+                                                zrange=(float(z[0]), float(z[1]))))
             else:
                 mi.growth.append(GrowthFunction(kind, 0.0, 0.0, False, False, False, False, (0.0, 0.0, 0.0), 0.0))
             i = j
@@ -262,18 +321,59 @@ def run(mi: ModelInput, tissue: configs.Tissue, solver, n_steps: int | None = No
     writer = FrameWriter(save_dir, tissue.mesh) if save_dir else None
     every = max(1, int(round(mi.save_interval / mi.dt))) if mi.save_interval > 0 else 0
     stats, last = [], None
+    tracker = GrowthTracker(mi, tissue.mesh) if mi.needs_increments() else None
     try:
         for k in range(steps):
             t = k * mi.dt
-            r = mi.rates_at(tissue.mesh, t)
-            if last is None or not np.array_equal(r, last):
-                solver.set_growth_rates(r)
-                last = r
+            if tracker is None:
+                r = mi.rates_at(tissue.mesh, t)
+                if last is None or not np.array_equal(r, last):
+                    solver.set_growth_rates(r)
+                    last = r
+                factors = np.exp(r * mi.dt)
+            else:
+                inc = tracker.increments(t, lambda: solver.positions)
+                if last is None or not np.array_equal(inc, last):
+                    solver.set_growth_increments(inc)
+                    last = inc
+                factors = np.stack([inc[:, 0, 0], inc[:, 1, 1], inc[:, 2, 2]], axis=1)
             stats.append(solver.runOneStep(mi.dt))
             if writer is not None and every and (k + 1) % every == 0:
                 fg, _ = solver.get_fg()
-                writer.write((k + 1) * mi.dt, solver.positions, fg, np.exp(r * mi.dt))
+                writer.write((k + 1) * mi.dt, solver.positions, fg, factors)
     finally:
         if writer is not None:
             writer.close()
     return stats
+
+
+class GrowthTracker:
+    """Per-step growth increments of a model input's functions, with the reference's bookkeeping
+    of relative positions: the initial ones (set at creation and at every PinningUpdateTime,
+    Simulation::checkForPinningPositionsUpdate, Simulation.cpp:10425-10443) for grids pinned on the
+    initial mesh, else the current ones."""
+
+    def __init__(self, mi: ModelInput, mesh: TissueMesh):
+        self.mi = mi
+        self.functions = mi.growth_functions()
+        self.geo = G.GrowthGeometry(mesh)
+        self.pins_done = [False] * len(mi.pin_times)
+        self.needs_positions = any(f.kind == 2 for f in self.functions) or (
+            not mi.grid_pinned and any(f.kind == 3 for f in self.functions))
+
+    def increments(self, time_sec: float, positions):
+        """(E, 3, 3) increments of the step starting at time_sec; ``positions`` returns the current
+        node positions when they are needed."""
+        pos = None
+        if self.mi.grid_pinned:
+            for k, tp in enumerate(self.mi.pin_times):
+                if time_sec >= tp and not self.pins_done[k]:
+                    pos = np.asarray(positions(), float)
+                    self.geo.initial_rel = self.geo.relative_positions(pos)
+                    self.pins_done[k] = True
+        if self.needs_positions and pos is None:
+            pos = np.asarray(positions(), float)
+        if pos is None:
+            pos = self.geo.mesh.pos
+        return G.increments(self.functions, self.geo, pos, time_sec, self.mi.dt, self.mi.grid_pinned,
+                            self.mi.grid_interpolation)

@@ -117,6 +117,12 @@ class NewtonRaphsonSolver:
         Fg = L.f64(Fg).ravel()
         self._check(self._lib.tf_set_fg(self.h, L.ptr(Fg)))
 
+    def set_growth_increments(self, dFg):
+        """Per-element growth increments (E, 3, 3) for the following growth updates; None returns
+        to the diagonal exp(rate dt) increments."""
+        a = None if dFg is None else L.f64(dFg).ravel()
+        self._check(self._lib.tf_set_growth_increments(self.h, L.ptr(a)))
+
     def set_growth_rates(self, rates):
         r = L.f64(rates).ravel()
         self._check(self._lib.tf_set_growth_rates(self.h, L.ptr(r)))
