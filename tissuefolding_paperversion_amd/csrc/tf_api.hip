@@ -80,10 +80,11 @@ void free_all(tf_handle* h)
                     h->val_base, h->bind_nodes, h->bind_m_ptr, h->bind_m_dof, h->bind_s_dof, h->bind_pairs,
                     h->ablated, h->blk_tr, h->rot_tz, h->val_t, h->g_save, h->adj_xbar, h->adj_rhs, h->adj_pass,
                     h->adj_y, h->adj_un_bar, h->adj_invfg_bar, h->adj_fg_bar, h->adj_fg_prev, h->adj_rate_bar,
-                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz, h->seg_rtol2, h->warm[0], h->warm[1]};
+                    h->adj_lam_bar, h->adj_mu_bar, h->adj_lam_acc, h->adj_mu_acc, h->adj_tz, h->seg_rtol2};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     packing_free(h);
+    warm_start_free(h);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
     drop_krylov_graphs(h);
@@ -568,8 +569,6 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg = dev_alloc<SegScalars>(S);
         h->seg_norm = dev_alloc<double>(S);
         h->seg_rtol2 = dev_alloc<double>(S);
-        h->warm[0] = dev_alloc<double>((size_t)3 * N);
-        h->warm[1] = dev_alloc<double>((size_t)3 * N);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
         for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx, &h->km, &h->kq})
@@ -692,6 +691,7 @@ int tf_set_positions(tf_handle* h, const double* x)
     TF_GUARD_BEGIN
     h2d(h, h->uk, x, (size_t)3 * h->N);
     TF_HIP(hipStreamSynchronize(h->stream));
+    warm_start_clear(h);  // earlier updates belong to another state
     return TF_OK;
     TF_GUARD_END(h)
 }
@@ -1045,8 +1045,8 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         auto ts = std::chrono::steady_clock::now();
         h->seg_rtol_on = h->opts.krylov_forcing != 0;
         if (h->seg_rtol_on) forcing_tolerances(h, it);
-        const bool warm = h->opts.krylov_warm_start != 0 && it < 2;
-        h->warm_guess = (warm && h->warm_valid[it]) ? h->warm[it] : nullptr;
+        const bool warm = h->opts.krylov_warm_start != 0 && it < TF_WARM_ITERS;
+        h->warm_guess = (warm && warm_start_guess(h, it)) ? h->warm_x0 : nullptr;
         int kst;
         h->krylov_call = it;
         try {
@@ -1060,10 +1060,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         h->seg_rtol_on = false;
         h->warm_guess = nullptr;
         h->krylov_call = -1;
-        if (warm) {  // keep this update as the guess for the next timestep's iteration it
-            launch_copy(h, h->warm[it], h->du, 3 * (int64_t)h->N);
-            h->warm_valid[it] = kst == TF_OK;
-        }
+        if (warm && kst == TF_OK) warm_start_push(h, it);  // a guess for the next timestep's iteration it
         krylov_total += h->last_krylov_iters;
         int ust = update_uk_impl(h, it);
         if (h->opts.krylov_forcing) forcing_observe(h, it);

@@ -22,6 +22,9 @@
 
 #define TF_TILE_CAP_MAX 2048   // largest tile capacity (blocks); 48 KB of LDS products
 #define TF_RED_SLOTS 16      // partial-sum slots per tile; partial[slot * T + tile]
+#define TF_WARM_ITERS 4      // Newton iterations whose solves start from earlier timesteps' updates
+#define TF_WARM_HIST 4       // updates kept per Newton iteration
+#define TF_WARM_DOTS (TF_WARM_HIST * (TF_WARM_HIST + 1) / 2 + TF_WARM_HIST)  // Gram entries + right-hand side
 
 // Per-tissue Krylov / NR scalars, one struct per segment, resident on the device.  Within a
 // Krylov iteration the scalars (alpha, beta, omega, ...) are re-derived by every tile from the
@@ -200,11 +203,19 @@ struct tf_handle {
     std::vector<double> fc_C, fc_du0, fc_du_prev, h_seg_rtol2;
     double *seg_rtol2 = nullptr;      // [S]
     bool seg_rtol_on = false;
-    // Krylov warm start: the update of Newton iterations 0 and 1 of the previous timestep, used
-    // as the initial guess of the same iteration's solve (warm_guess points at it during a solve)
-    double *warm[2] = {nullptr, nullptr};
-    bool warm_valid[2] = {false, false};
-    const double *warm_guess = nullptr;
+    // Krylov warm start: for Newton iteration k < TF_WARM_ITERS, the updates that iteration took
+    // in the last TF_WARM_HIST timesteps (warm_hist[k][0] the newest).  A solve starts from their
+    // combination that minimises ||g - M x|| per tissue (warm_x0; warm_guess points at it during
+    // the solve).  Work arrays: M z_j (warm_mz), per-tile and per-tissue Gram entries, coefficients.
+    double* warm_hist[TF_WARM_ITERS][TF_WARM_HIST] = {};
+    int32_t warm_count[TF_WARM_ITERS] = {};
+    double* warm_mz[TF_WARM_HIST] = {};
+    double* warm_x0 = nullptr;
+    double* warm_part = nullptr;      // [TF_WARM_DOTS][T]
+    int32_t warm_part_T = 0;
+    double* warm_seg = nullptr;       // [S][TF_WARM_DOTS]
+    double* warm_y = nullptr;         // [S][TF_WARM_HIST]
+    const double* warm_guess = nullptr;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
     // Krylov iterations the solve of each Newton iteration took in the previous timestep: the
     // next solve of that iteration replays most of that many iterations before its first
@@ -266,6 +277,12 @@ void launch_adj_split(tf_handle* h, const double* xbar, double* rhs, double* pas
 void launch_accumulate(tf_handle* h, const double* a, double* b, int64_t n);
 // ---- Krylov (tf_krylov.hip) -----------------------------------------------------------------
 int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
+// Warm start of the solve of Newton iteration k: x0 in warm_x0 from the kept updates; false when
+// there is none (or the least-squares problem is degenerate for every tissue).
+bool warm_start_guess(tf_handle* h, int k);
+void warm_start_push(tf_handle* h, int k);    // keeps du as the newest update of iteration k
+void warm_start_free(tf_handle* h);
+void warm_start_clear(tf_handle* h);
 double krylov_time_cg1(tf_handle* h, int n_iters);  // ms per single-kernel CG iteration
 void launch_spmv(tf_handle* h, const double* x, double* y);
 

@@ -20,7 +20,9 @@
 //   p for neighbouring rows on the fly and writes it for its own rows (ping-pong buffers).
 #include "tf_internal.hpp"
 #include <algorithm>
+#include <cmath>
 #include <math.h>
+#include <vector>
 
 namespace {
 
@@ -949,6 +951,227 @@ TileArgs tile_args(tf_handle* h)
     return TileArgs{h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->partial, h->seg, h->T,
                     h->opts.xcd_tile_map ? h->n_xcd : 1};
 }
+
+// ---- warm start: least-squares combination of earlier updates ---------------------------------
+struct WarmArgs {
+    const double* mz[TF_WARM_HIST];
+    const double* z[TF_WARM_HIST];
+    const double* g;
+    int m;
+};
+
+// per tile: (Mz_i, Mz_j) for i <= j < m in row-major upper-triangle order, then (Mz_i, g)
+__global__ __launch_bounds__(WG) void k_warm_dots(TileArgs ta, WarmArgs wa, double* __restrict__ part)
+{
+    const int t = blockIdx.x;
+    const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
+    double acc[TF_WARM_DOTS];
+#pragma unroll
+    for (int q = 0; q < TF_WARM_DOTS; ++q) acc[q] = 0.0;
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+        double v[TF_WARM_HIST];
+#pragma unroll
+        for (int i = 0; i < TF_WARM_HIST; ++i) v[i] = i < wa.m ? wa.mz[i][d] : 0.0;
+        const double gd = wa.g[d];
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < TF_WARM_HIST; ++i)
+#pragma unroll
+            for (int j = i; j < TF_WARM_HIST; ++j) acc[q++] += v[i] * v[j];
+#pragma unroll
+        for (int i = 0; i < TF_WARM_HIST; ++i) acc[q++] += v[i] * gd;
+    }
+    constexpr int NW = WG / 64;
+    __shared__ double red[TF_WARM_DOTS][NW];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int q = 0; q < TF_WARM_DOTS; ++q) {
+        const double sv = wave_sum(acc[q]);
+        if (l == 0) red[q][w] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x < TF_WARM_DOTS) {
+        double sv = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) sv += red[threadIdx.x][k];
+        part[(size_t)threadIdx.x * ta.T + t] = sv;
+    }
+}
+
+// per tissue: the tile partials summed in tile order
+__global__ __launch_bounds__(64) void k_warm_seg(TileArgs ta, const double* __restrict__ part, double* __restrict__ out)
+{
+    const int sg = blockIdx.x, q = threadIdx.x;
+    if (q >= TF_WARM_DOTS) return;
+    double sv = 0.0;
+    for (int t = ta.seg_tile_ptr[sg]; t < ta.seg_tile_ptr[sg + 1]; ++t) sv += part[(size_t)q * ta.T + t];
+    out[(size_t)sg * TF_WARM_DOTS + q] = sv;
+}
+
+// x0 = sum_j y[tissue][j] z_j on every row of the tile
+__global__ __launch_bounds__(WG) void k_warm_combine(TileArgs ta, WarmArgs wa, const double* __restrict__ y,
+                                                     double* __restrict__ x0)
+{
+    const int t = blockIdx.x;
+    const int sg = ta.tile_seg[t];
+    const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
+    double c[TF_WARM_HIST];
+#pragma unroll
+    for (int j = 0; j < TF_WARM_HIST; ++j) c[j] = j < wa.m ? y[(size_t)sg * TF_WARM_HIST + j] : 0.0;
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < TF_WARM_HIST; ++j)
+            if (j < wa.m) v += c[j] * wa.z[j][d];
+        x0[d] = v;
+    }
+}
+
+// Least squares min ||b - A y|| from the normal equations G y = r (G = A^T A, m <= 4): the
+// diagonally scaled G is diagonalised by cyclic Jacobi rotations and eigenvalues below 1e-12 of
+// the largest are dropped (nearly parallel updates).  Returns false when nothing is left.
+bool small_lstsq(int m, const double* Gup, const double* r, double* y)
+{
+    double G[TF_WARM_HIST][TF_WARM_HIST], V[TF_WARM_HIST][TF_WARM_HIST], s[TF_WARM_HIST];
+    int q = 0;
+    for (int i = 0; i < m; ++i)
+        for (int j = i; j < TF_WARM_HIST; ++j, ++q)
+            if (j < m) G[i][j] = G[j][i] = Gup[q];
+    for (int i = 0; i < m; ++i) {
+        if (!(G[i][i] > 0.0) || !std::isfinite(G[i][i])) return false;
+        s[i] = 1.0 / std::sqrt(G[i][i]);
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            G[i][j] *= s[i] * s[j];
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = i + 1; j < m; ++j) off += G[i][j] * G[i][j];
+        if (off < 1e-30) break;
+        for (int p = 0; p < m; ++p)
+            for (int k = p + 1; k < m; ++k) {
+                if (G[p][k] == 0.0) continue;
+                const double th = 0.5 * std::atan2(2.0 * G[p][k], G[k][k] - G[p][p]);
+                const double c = std::cos(th), sn = std::sin(th);
+                for (int i = 0; i < m; ++i) {  // G <- J^T G J
+                    const double gp = G[i][p], gk = G[i][k];
+                    G[i][p] = c * gp - sn * gk;
+                    G[i][k] = sn * gp + c * gk;
+                }
+                for (int i = 0; i < m; ++i) {
+                    const double gp = G[p][i], gk = G[k][i];
+                    G[p][i] = c * gp - sn * gk;
+                    G[k][i] = sn * gp + c * gk;
+                }
+                for (int i = 0; i < m; ++i) {
+                    const double vp = V[i][p], vk = V[i][k];
+                    V[i][p] = c * vp - sn * vk;
+                    V[i][k] = sn * vp + c * vk;
+                }
+            }
+    }
+    double lmax = 0.0;
+    for (int i = 0; i < m; ++i) lmax = std::max(lmax, G[i][i]);
+    if (!(lmax > 0.0)) return false;
+    double w[TF_WARM_HIST] = {0.0, 0.0, 0.0, 0.0};
+    bool any = false;
+    for (int e = 0; e < m; ++e) {
+        if (!(G[e][e] > 1e-12 * lmax)) continue;
+        double proj = 0.0;
+        for (int i = 0; i < m; ++i) proj += V[i][e] * s[i] * r[i];
+        for (int i = 0; i < m; ++i) w[i] += V[i][e] * proj / G[e][e];
+        any = true;
+    }
+    for (int i = 0; i < m; ++i) {
+        y[i] = s[i] * w[i];
+        if (!std::isfinite(y[i])) return false;
+    }
+    return any;
+}
+
+}  // namespace
+
+bool warm_start_guess(tf_handle* h, int k)
+{
+    const int m = h->warm_count[k];
+    if (m <= 0) return false;
+    const size_t n3 = 3 * (size_t)h->N;
+    if (!h->warm_x0) {
+        for (int j = 0; j < TF_WARM_HIST; ++j) TF_HIP(hipMalloc((void**)&h->warm_mz[j], n3 * sizeof(double)));
+        TF_HIP(hipMalloc((void**)&h->warm_x0, n3 * sizeof(double)));
+        TF_HIP(hipMalloc((void**)&h->warm_seg, sizeof(double) * TF_WARM_DOTS * h->S));
+        TF_HIP(hipMalloc((void**)&h->warm_y, sizeof(double) * TF_WARM_HIST * h->S));
+    }
+    if (!h->warm_part || h->warm_part_T != h->T) {  // tiles change with the tiling
+        if (h->warm_part) (void)hipFree(h->warm_part);
+        TF_HIP(hipMalloc((void**)&h->warm_part, sizeof(double) * TF_WARM_DOTS * h->T));
+        h->warm_part_T = h->T;
+    }
+    for (int j = 0; j < m; ++j) launch_spmv(h, h->warm_hist[k][j], h->warm_mz[j]);
+    TileArgs ta = tile_args(h);
+    WarmArgs wa{};
+    for (int j = 0; j < m; ++j) {
+        wa.mz[j] = h->warm_mz[j];
+        wa.z[j] = h->warm_hist[k][j];
+    }
+    wa.g = h->g;
+    wa.m = m;
+    hipLaunchKernelGGL(k_warm_dots, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_part);
+    hipLaunchKernelGGL(k_warm_seg, dim3(h->S), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
+    TF_HIP(hipGetLastError());
+    std::vector<double> dots((size_t)TF_WARM_DOTS * h->S), y((size_t)TF_WARM_HIST * h->S, 0.0);
+    TF_HIP(hipMemcpyAsync(dots.data(), h->warm_seg, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+    bool any = false;
+    for (int s = 0; s < h->S; ++s) {
+        const double* D = dots.data() + (size_t)s * TF_WARM_DOTS;
+        const double* rhs = D + TF_WARM_HIST * (TF_WARM_HIST + 1) / 2;
+        double ys[TF_WARM_HIST] = {0.0, 0.0, 0.0, 0.0};
+        if (small_lstsq(m, D, rhs, ys)) {
+            for (int j = 0; j < m; ++j) y[(size_t)s * TF_WARM_HIST + j] = ys[j];
+            any = true;
+        }
+    }
+    if (!any) return false;
+    TF_HIP(hipMemcpyAsync(h->warm_y, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_warm_combine, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_y, h->warm_x0);
+    TF_HIP(hipGetLastError());
+    TF_HIP(hipStreamSynchronize(h->stream));  // y is a stack temporary
+    return true;
+}
+
+void warm_start_push(tf_handle* h, int k)
+{
+    const size_t n3 = 3 * (size_t)h->N;
+    double* last = h->warm_hist[k][TF_WARM_HIST - 1];
+    if (!last) TF_HIP(hipMalloc((void**)&last, n3 * sizeof(double)));
+    for (int j = TF_WARM_HIST - 1; j > 0; --j) h->warm_hist[k][j] = h->warm_hist[k][j - 1];
+    h->warm_hist[k][0] = last;  // the oldest buffer is reused for the newest update
+    launch_copy(h, last, h->du, (int64_t)n3);
+    h->warm_count[k] = std::min(TF_WARM_HIST, h->warm_count[k] + 1);
+}
+
+void warm_start_clear(tf_handle* h)
+{
+    for (int k = 0; k < TF_WARM_ITERS; ++k) h->warm_count[k] = 0;
+}
+
+void warm_start_free(tf_handle* h)
+{
+    for (int k = 0; k < TF_WARM_ITERS; ++k)
+        for (int j = 0; j < TF_WARM_HIST; ++j)
+            if (h->warm_hist[k][j]) { (void)hipFree(h->warm_hist[k][j]); h->warm_hist[k][j] = nullptr; }
+    for (int j = 0; j < TF_WARM_HIST; ++j)
+        if (h->warm_mz[j]) { (void)hipFree(h->warm_mz[j]); h->warm_mz[j] = nullptr; }
+    for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg, &h->warm_y})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    warm_start_clear(h);
+}
+
+namespace {
 
 template <typename VT, int P>
 void enqueue_bicg_iter(tf_handle* h, const VT* val)
