@@ -222,6 +222,8 @@ def main():
                    "tiling": dict(zip(("workgroup", "tile_blocks", "tiles"), s.tiling()))},
         "nr_iterations_per_step": float(np.mean(nr)),
         "krylov_iterations_per_nr_iteration": float(np.sum(kr) / max(1, np.sum(nr))),
+        "krylov_iterations_by_newton_iteration": [float(np.mean([st["krylov_by_newton"][k] for st in stats]))
+                                                  for k in range(int(max(nr)) if len(nr) else 0) if k < 4],
         "converged_steps": int(sum(st["converged"] for st in stats)),
         "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in stats)),
         "packing": {"pairs_per_step_max": int(max(st["packing_pairs"] for st in stats)),

@@ -99,6 +99,7 @@ typedef struct tf_step_stats {
     int32_t packing_pairs;       /* implicit-packing pairs in force during the step            */
     double last_du_norm;         /* max over tissues of the last ||dU||_2                      */
     double ms_grow, ms_assemble, ms_solve, ms_total;
+    int32_t krylov_by_newton[4]; /* Krylov iterations of Newton iterations 0-3 (max over tissues) */
 } tf_step_stats;
 
 /* ---- lifetime --------------------------------------------------------------------------- */

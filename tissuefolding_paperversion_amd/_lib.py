@@ -59,10 +59,13 @@ class StepStats(C.Structure):
     _fields_ = [("nr_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("converged", C.c_int32),
                 ("rotated_elements", C.c_int32), ("krylov_fallbacks", C.c_int32), ("packing_pairs", C.c_int32),
                 ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
-                ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double)]
+                ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double),
+                ("krylov_by_newton", C.c_int32 * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["krylov_by_newton"] = list(d["krylov_by_newton"])
+        return d
 
 
 _lib = None

@@ -979,16 +979,23 @@ static double ms_since(std::chrono::steady_clock::time_point t0)
 
 // Inexact-Newton forcing (per tissue).  C is the quadratic-convergence constant of the tissue's
 // Newton iteration (||dU_{k+1}|| ~ C ||dU_k||^2, the smallest ratio seen; the first step, solved
-// to krylov_rtol, supplies it) and p the predicted size of this iteration's update:
+// to krylov_rtol, supplies it), p the predicted size of this iteration's update and q = C p^2
+// that of the next one:
 //  - p <= nr_tol (the last iteration): the solve only decides convergence -> relative 1e-2;
-//  - next update C p^2 >= 100 nr_tol (far from convergence): the error left in dU may be 5% of
-//    the next update.  A block-Jacobi CG iterate's error is up to ~100x its relative residual
-//    times ||dU|| (soft, weakly pinned modes keep the slowest error), so rtol = 0.05 C p / 100;
-//  - otherwise (the iteration before the last) krylov_rtol, so that the final ||dU|| and with it
-//    the NR iteration count are those of an exact solve.
+//  - q >= 100 nr_tol (far from convergence): the error left in dU may be 5% of the next update.
+//    A block-Jacobi CG iterate's error is up to ~100x its relative residual times ||dU|| (soft,
+//    weakly pinned modes keep the slowest error), so rtol = 0.05 q / (100 p);
+//  - q < nr_tol (the iteration before the last), symmetric solvers: the error may be 5% of the
+//    margin nr_tol - q, so that it cannot move the last ||dU|| across nr_tol:
+//    rtol = 0.05 (nr_tol - q) / (100 p), never below krylov_rtol;
+//  - otherwise krylov_rtol, so that the next ||dU||, and with it the NR iteration count, is that
+//    of an exact solve (BiCGSTAB has no such error bound and keeps exact penultimate solves).
 static void forcing_tolerances(tf_handle* h, int iter)
 {
     const double floor_rtol = h->opts.krylov_rtol, tol = h->opts.nr_tol, amplification = 100.0;
+    const int kind = h->opts.krylov;
+    const bool symmetric = !h->any_viscous && (kind < 0 || kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 ||
+                                               kind == TF_KRYLOV_PIPECG);
     for (int s = 0; s < h->S; ++s) {
         double eta = floor_rtol;
         const double C = h->fc_C[s];
@@ -996,8 +1003,11 @@ static void forcing_tolerances(tf_handle* h, int iter)
         if (iter == 0) p = h->fc_du0[s] > 0 ? 1.5 * h->fc_du0[s] : -1.0;
         else if (h->fc_du_prev[s] > 0) p = C * h->fc_du_prev[s] * h->fc_du_prev[s];
         if (C > 0 && p > 0) {
+            const double q = C * p * p;
             if (p <= tol) eta = 1e-2;
-            else if (C * p * p >= 100.0 * tol) eta = std::min(1e-2, std::max(floor_rtol, 0.05 * C * p / amplification));
+            else if (q >= 100.0 * tol) eta = std::min(1e-2, std::max(floor_rtol, 0.05 * q / (amplification * p)));
+            else if (q < tol && symmetric)
+                eta = std::min(1e-2, std::max(floor_rtol, 0.05 * (tol - q) / (amplification * p)));
         }
         h->h_seg_rtol2[s] = eta * eta;
     }
@@ -1062,6 +1072,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         h->krylov_call = -1;
         if (warm && kst == TF_OK) warm_start_push(h, it);  // a guess for the next timestep's iteration it
         krylov_total += h->last_krylov_iters;
+        if (stats && it < 4) stats->krylov_by_newton[it] = h->last_krylov_iters;
         int ust = update_uk_impl(h, it);
         if (h->opts.krylov_forcing) forcing_observe(h, it);
         ms_sol += ms_since(ts);
