@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--krylov", type=int, default=-1)
     p.add_argument("--matrix-fp32", action="store_true")
     p.add_argument("--no-graph", action="store_true")
+    p.add_argument("--preconditioner", default="block-jacobi", choices=["block-jacobi", "two-level"])
+    p.add_argument("--coarse-aggregates", type=int, default=0)
+    p.add_argument("--coarse-refresh", type=int, default=10)
     p.add_argument("--no-kernel-timing", action="store_true", help="skip the HIP-event timing of the dominant kernel")
     return p.parse_args()
 
@@ -192,6 +195,10 @@ def main():
         opts["matrix_precision"] = 1
     if a.no_graph:
         opts["use_graph"] = 0
+    if a.preconditioner == "two-level":
+        opts["preconditioner"] = 1
+        opts["coarse_aggregates"] = a.coarse_aggregates
+        opts["coarse_refresh"] = a.coarse_refresh
     timing = 0 if a.no_kernel_timing else 1
     s, stats, sec, x_plain = run_timed(NewtonRaphsonSolver, tissue, opts, a.warmup, a.steps, timing)
     spmv_ms, spmv_n = s.kernel_timing("spmv", enable=0)
@@ -219,6 +226,7 @@ def main():
                    "prisms": tissue.n_elements, "nodes": tissue.n_nodes, "bsr_blocks": s.nb,
                    "krylov": "auto (single-kernel CG, BiCGSTAB fallback)" if a.krylov < 0 else str(a.krylov),
                    "matrix_values": "fp32" if a.matrix_fp32 else "fp64",
+                   "preconditioner": a.preconditioner,
                    "tiling": dict(zip(("workgroup", "tile_blocks", "tiles"), s.tiling()))},
         "nr_iterations_per_step": float(np.mean(nr)),
         "krylov_iterations_per_nr_iteration": float(np.sum(kr) / max(1, np.sum(nr))),

@@ -65,6 +65,13 @@ typedef struct tf_mesh_desc {
  * operand was completed by the previous kernel. */
 enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_CG1 = 3, TF_KRYLOV_PIPECG = 4 };
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
+/* TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M (the default).
+ * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi plus an additive coarse
+ * correction P (P^T M P)^-1 P^T r, P the rigid-body modes of node aggregates (square cells of
+ * the tissue's x-y extent, whole layer columns each).  The coarse operator is rebuilt at the
+ * first Newton iteration of every coarse_refresh-th timestep and for every solve outside the
+ * Newton loop; handles with several tissues use block Jacobi. */
+enum { TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
 
 typedef struct tf_solver_opts {
     int32_t krylov;            /* TF_KRYLOV_*                                                  */
@@ -88,6 +95,11 @@ typedef struct tf_solver_opts {
     int32_t implicit_packing;  /* 1: tf_timestep detects packing pairs after the growth update,
                                   as Simulation::runOneStep does (Simulation.cpp:37, :5911-5920);
                                   0: tf_timestep clears the pair list                          */
+    int32_t preconditioner;    /* TF_PRECOND_*: the preconditioner of the CG solvers             */
+    int32_t coarse_aggregates; /* TF_PRECOND_TWO_LEVEL: target number of node aggregates of the
+                                  coarse space (0: one per 190 nodes)                           */
+    int32_t coarse_refresh;    /* TF_PRECOND_TWO_LEVEL: timesteps between rebuilds of the coarse
+                                  operator (0 or 1: every timestep)                             */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

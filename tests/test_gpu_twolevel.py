@@ -1,0 +1,137 @@
+"""Two-level preconditioner (block Jacobi plus a rigid-body coarse space over node aggregates).
+
+The preconditioner changes how each linear system is solved, not its solution.  Bars:
+- single solves within 1e-8 relative of the block-Jacobi solve and of a direct sparse solve of
+  the same system (krylov_rtol 1e-12), in fewer Krylov iterations;
+- fixed DoFs and bound slaves of du exactly as block Jacobi leaves them (zero, or the master's);
+- trajectories with the block-Jacobi NR iteration counts and positions within 1e-10 relative
+  (1e-8 against the oracle);
+- handles with several tissues fall back to block Jacobi (bitwise equal results).
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from conftest import rel_err
+from test_oracle_constraints import bound_pairs
+from tissuefolding_paperversion_amd import configs
+from tissuefolding_paperversion_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+TL = dict(preconditioner=L.PRECOND_TWO_LEVEL)
+
+
+def _solver(tissue, **opts):
+    from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
+    return NewtonRaphsonSolver(tissue, opts or None)
+
+
+def _ellipse():
+    return configs.wing_disc_tissue(a=60.0, b=36.0, h=3.66, z_levels=(0.0, 2.0, 7.0, 12.0), seed=2, growth_seed=3,
+                                    name="two-level ellipse")
+
+
+def _assembled(t, perturb=0.05, bind=False, **opts):
+    s = _solver(t, **opts)
+    if bind:
+        s.set_bound_dofs(bound_pairs(t))
+    rng = np.random.default_rng(11)
+    X = t.mesh.pos.ravel().copy()
+    s.updateNodeViscositySurfaces()
+    s.constructUnMatrix()
+    s.positions = X + perturb * rng.standard_normal(X.shape)
+    s.calculateForcesAndJacobianMatrixNR(t.dt)
+    return s
+
+
+@pytest.mark.parametrize("make,aggregates", [(_ellipse, 0), (_ellipse, 7), (lambda: configs.disc_tissue(radius=30.0), 0)])
+def test_single_solve_matches_block_jacobi(make, aggregates):
+    t = make()
+    bj = _assembled(t)
+    du_bj = bj.solveForDeltaU()
+    it_bj = bj.last_krylov["iterations"]
+    tl = _assembled(t, coarse_aggregates=aggregates, **TL)
+    du_tl = tl.solveForDeltaU()
+    it_tl = tl.last_krylov["iterations"]
+    print(f"{t.name} aggregates={aggregates}: block Jacobi {it_bj} its, two-level {it_tl} its", flush=True)
+    assert tl.last_krylov["rel_residual"] <= 1e-12
+    assert rel_err(du_tl, du_bj) < 1e-8
+    ref = spla.spsolve(tl.jacobian_csr().tocsc(), tl.gSum)  # direct sparse solve of the same system
+    assert rel_err(du_tl, ref) < 1e-8
+    fixed = np.flatnonzero(t.fixed)
+    assert np.all(du_tl[fixed] == 0.0)
+    assert it_tl < it_bj
+
+
+def test_bound_slaves_follow_masters():
+    t = configs.disc_tissue(radius=30.0)
+    pairs = np.asarray(bound_pairs(t))
+    bj = _assembled(t, bind=True)
+    tl = _assembled(t, bind=True, **TL)
+    du_bj, du_tl = bj.solveForDeltaU(), tl.solveForDeltaU()
+    assert rel_err(du_tl, du_bj) < 1e-8
+    assert np.array_equal(du_tl[pairs[:, 0]], du_tl[pairs[:, 1]])
+
+
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_FP32])
+def test_trajectory_keeps_the_newton_iterates(prec):
+    t = _ellipse()
+    bj = _solver(t, matrix_precision=prec)
+    tl = _solver(t, matrix_precision=prec, **TL)
+    kb = kt = 0
+    for step in range(6):
+        a, b = bj.runOneStep(t.dt), tl.runOneStep(t.dt)
+        print(f"step {step}: NR {a['nr_iterations']} / {b['nr_iterations']}, "
+              f"Krylov {a['krylov_iterations']} / {b['krylov_iterations']}", flush=True)
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert a["nr_iterations"] == b["nr_iterations"]
+        kb += a["krylov_iterations"]
+        kt += b["krylov_iterations"]
+        assert rel_err(tl.positions, bj.positions) < 1e-10
+    assert kt < kb, (kt, kb)
+
+
+def test_trajectory_matches_oracle_with_fixed_dofs(oracle):
+    t = configs.disc_tissue(radius=20.0)
+    s = _solver(t, **TL)
+    o = oracle.OracleTissue(t)
+    X0 = t.mesh.pos.ravel().copy()
+    fixed = np.flatnonzero(t.fixed)
+    for step in range(4):
+        st = s.runOneStep(t.dt)
+        it_o, _ = o.timestep(t.dt)
+        assert st["converged"] == 1
+        assert st["nr_iterations"] == it_o
+        assert rel_err(s.positions, o.pos) < 1e-8
+        assert np.array_equal(s.positions[fixed], X0[fixed])
+
+
+def test_packing_pairs_with_two_level(oracle):
+    t = configs.facing_discs_tissue()
+    s = _solver(t, **TL)
+    o = oracle.OracleTissue(t)
+    for step in range(3):
+        st = s.runOneStep(t.dt)
+        it_o, _ = o.timestep(t.dt)
+        assert st["packing_pairs"] == o.stats[-1]["packing_pairs"] > 0
+        assert st["nr_iterations"] == it_o
+        assert rel_err(s.positions, o.pos) < 1e-8
+
+
+def test_several_tissues_use_block_jacobi():
+    ts = [configs.disc_tissue(radius=15.0, n_steps=1), configs.disc_tissue(radius=12.0, n_steps=1, seed=3)]
+    a, b = _solver(ts), _solver(ts, **TL)
+    for _ in range(2):
+        sa, sb = a.runOneStep(ts[0].dt), b.runOneStep(ts[0].dt)
+        assert sa["krylov_iterations"] == sb["krylov_iterations"]
+    assert np.array_equal(a.positions, b.positions)
+
+
+def test_option_validation():
+    t = configs.disc_tissue(radius=10.0, n_steps=1)
+    s = _solver(t)
+    with pytest.raises(L.TfError):
+        s.set_options(preconditioner=2)
+    with pytest.raises(L.TfError):
+        s.set_options(coarse_aggregates=-1)

@@ -18,6 +18,7 @@ STATUS_NAMES = {0: "TF_OK", 1: "TF_NOT_CONVERGED", 2: "TF_KRYLOV_STALL", 3: "TF_
                 5: "TF_OOM", 6: "TF_HIP_ERROR", 7: "TF_INVALID"}
 KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_GMRES, KRYLOV_CG1, KRYLOV_PIPECG = -1, 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+PRECOND_BLOCK_JACOBI, PRECOND_TWO_LEVEL = 0, 1
 
 # Every symbol include/tf_nr.h declares (checked by the CPU test suite).
 EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf_set_options", "tf_get_options",
@@ -52,7 +53,8 @@ class SolverOpts(C.Structure):
                 ("nr_tol", C.c_double), ("nr_max_iter", C.c_int32), ("tangent_precision", C.c_int32),
                 ("matrix_precision", C.c_int32), ("growth_rotation", C.c_int32), ("use_graph", C.c_int32),
                 ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32),
-                ("implicit_packing", C.c_int32)]
+                ("implicit_packing", C.c_int32), ("preconditioner", C.c_int32), ("coarse_aggregates", C.c_int32),
+                ("coarse_refresh", C.c_int32)]
 
 
 class StepStats(C.Structure):

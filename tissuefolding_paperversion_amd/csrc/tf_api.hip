@@ -85,6 +85,7 @@ void free_all(tf_handle* h)
         if (p) (void)hipFree(p);
     packing_free(h);
     warm_start_free(h);
+    twolevel_free(h);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
     drop_krylov_graphs(h);
@@ -210,6 +211,7 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
     h->wg = wg;
     h->tile_cap = tile_cap;
     drop_krylov_graphs(h);
+    twolevel_invalidate(h);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -472,6 +474,8 @@ void tf_default_options(tf_solver_opts* o)
     o->xcd_tile_map = 1;
     o->krylov_warm_start = 1;
     o->implicit_packing = 1;
+    o->preconditioner = TF_PRECOND_BLOCK_JACOBI;
+    o->coarse_refresh = 10;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -660,7 +664,9 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         !(o->krylov_rtol >= 0) || !(o->nr_tol >= 0) ||
         (o->matrix_precision != TF_PREC_FP64 && o->matrix_precision != TF_PREC_FP32) ||
         o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16 ||
-        (o->implicit_packing != 0 && o->implicit_packing != 1))
+        (o->implicit_packing != 0 && o->implicit_packing != 1) ||
+        (o->preconditioner != TF_PRECOND_BLOCK_JACOBI && o->preconditioner != TF_PRECOND_TWO_LEVEL) ||
+        o->coarse_aggregates < 0 || o->coarse_refresh < 0)
         return fail(h, TF_INVALID, "invalid solver options");
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
     h->opts = *o;

@@ -232,6 +232,27 @@ struct tf_handle {
     // implicit packing
     PackingState pk;
 
+    // two-level preconditioner (one tissue): block Jacobi plus a coarse space of the rigid-body
+    // modes of node aggregates (columns through all layers, cut by an xy grid); see tf_krylov.hip
+    struct TwoLevel {
+        bool built = false, fresh = false, failed = false;
+        int32_t n_agg = 0, nc = 0, n_pairs = 0, G = 0, T = -1;
+        int32_t age = 0;                                    // timesteps since the coarse operator was built
+        int32_t *node_agg = nullptr, *node_coff = nullptr;  // [N] aggregate, its first coarse unknown
+        int32_t *agg_nodes = nullptr;                       // [N] nodes of every aggregate, in chunks
+        int32_t *chunk_ptr = nullptr, *agg_chunk = nullptr; // [C+1] chunk bounds, [A+1] chunks of every aggregate
+        int32_t n_chunk = 0;
+        double* pcoef = nullptr;                            // [N][3][6] rows of P
+        int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank
+        int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
+        double *chunk_part = nullptr;                       // [C][8]
+        double *Ac = nullptr, *Ainv = nullptr, *zc = nullptr;
+        int32_t rows = 0;                                   // coarse rows per workgroup of k_tl_coarse
+        int32_t* info = nullptr;
+        void* blas = nullptr;                               // rocblas_handle
+        std::vector<int32_t> h_node_agg;
+    } tl;
+
     // timing
     KernelTimer t_elem, t_gather, t_spmv;
 };
@@ -283,6 +304,8 @@ bool warm_start_guess(tf_handle* h, int k);
 void warm_start_push(tf_handle* h, int k);    // keeps du as the newest update of iteration k
 void warm_start_free(tf_handle* h);
 void warm_start_clear(tf_handle* h);
+void twolevel_free(tf_handle* h);
+void twolevel_invalidate(tf_handle* h);  // pattern or tiling changed
 double krylov_time_cg1(tf_handle* h, int n_iters);  // ms per single-kernel CG iteration
 void launch_spmv(tf_handle* h, const double* x, double* y);
 

@@ -19,6 +19,8 @@
 //   iteration: the direction update p = z + beta p_old is folded into the SpMV, which forms
 //   p for neighbouring rows on the fly and writes it for its own rows (ping-pong buffers).
 #include "tf_internal.hpp"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <cmath>
 #include <math.h>
@@ -452,6 +454,285 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
     double vv[2] = {a, b};
     tile_write<2>(ta, t, so, vv);
     if (first_tile(ta, t, sg) && threadIdx.x == 0) S.iters += 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two-level preconditioner (one tissue): z = D^-1 r + P Ac^-1 P^T r, with P the rigid-body modes
+// of node aggregates (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is four
+// launches:
+//   k_cg_q       (tiles)      p = z + beta p, q = M p, (p, q)
+//   k_tl_xr      (chunks)     x += alpha p, r -= alpha q on a chunk of an aggregate's nodes,
+//                             the chunk's (r, D^-1 r), (r, r) and P_chunk^T r
+//   k_tl_coarse  (row blocks) r_c from the chunk sums, z_c = Ac^-1 r_c; the tissue's (r, z) and
+//                             (r, r) as tile partials
+//   k_tl_z       (tiles)      z = D^-1 r + P z_c
+// (r, z) = (r, D^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
+constexpr int TL_CHUNK = 64;  // nodes per chunk of an aggregate (one wave of k_tl_xr each)
+constexpr int TL_PREF = 16;   // doubles per lane of a coarse row prefetched before r_c is known
+
+struct TLArgs {
+    const int32_t* node_coff;  // [N] first coarse unknown of the node's aggregate
+    const int32_t* agg_nodes;  // [N] nodes of every aggregate (ascending), aggregate after aggregate
+    const int32_t* chunk_ptr;  // [C+1] chunks of at most TL_CHUNK nodes of one aggregate in agg_nodes
+    const int32_t* agg_chunk;  // [A+1] chunks of every aggregate
+    const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
+    const int32_t* coff;       // [A] first coarse unknown of every aggregate
+    const int32_t* krank;      // [A] coarse unknowns of every aggregate
+    double* chunk_part;        // [C][8] (r, D^-1 r), (r, r) and the 6 mode sums of every chunk
+    const double* Ainv;        // [nc][nc]
+    double* zc;                // [nc + 6] (the last 6 stay zero)
+    int nc, n_agg, n_chunk, G, rows;  // rows: coarse rows per workgroup of k_tl_coarse
+};
+
+// One wave per chunk.  INIT: r as given (no update); otherwise alpha = (r, z) / (p, q) from the
+// tile partials.  The chunk's node data is loaded before alpha is reduced.
+template <int P, bool INIT>
+__global__ __launch_bounds__(64) void k_tl_xr(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
+                                              const double* __restrict__ p, const double* __restrict__ q,
+                                              double* __restrict__ x, double* __restrict__ r)
+{
+    SegScalars& S = ta.seg[0];
+    if (seg_done(S)) return;
+    const int c = blockIdx.x;
+    const int k = tl.chunk_ptr[c] + threadIdx.x;
+    const bool own = k < tl.chunk_ptr[c + 1];
+    const int n = own ? tl.agg_nodes[k] : 0;
+    double rv[3], xv[3], pv[3], qv[3], D[9], Pn[18];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        rv[i] = r[3 * n + i];
+        if (!INIT) { xv[i] = x[3 * n + i]; pv[i] = p[3 * n + i]; qv[i] = q[3 * n + i]; }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) D[i] = dinv[9 * (size_t)n + i];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Pn[i] = tl.pcoef[18 * (size_t)n + i];
+    if (!INIT) {
+        const int sl[2] = {SL_RZ0 + P, SL_PQ};
+        double sc[2];
+        seg_sums<2, 64>(ta, 0, sl, sc);
+        if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite
+            if (blockIdx.x == 0 && threadIdx.x == 0) S.stalled = 1;
+            return;
+        }
+        const double alpha = sc[0] / sc[1];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            xv[i] = xv[i] + alpha * pv[i];
+            rv[i] = rv[i] - alpha * qv[i];
+        }
+        if (own)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) { x[3 * n + i] = xv[i]; r[3 * n + i] = rv[i]; }
+    }
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, D^-1 r), (r, r), P^T r
+    if (own) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double zi = D[3 * i] * rv[0] + D[3 * i + 1] * rv[1] + D[3 * i + 2] * rv[2];
+            v[0] += rv[i] * zi;
+            v[1] += rv[i] * rv[i];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) v[2 + j] += Pn[6 * i + j] * rv[i];
+        }
+    }
+    double mine = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double t = wave_sum(v[j]);
+        if (threadIdx.x == j) mine = t;
+    }
+    if (threadIdx.x < 8) tl.chunk_part[8 * (size_t)c + threadIdx.x] = mine;
+}
+
+// r_c from the chunk sums (every workgroup, fixed order), then tl.rows rows of z_c = Ac^-1 r_c
+// (one wave per row; the wave's first row is loaded while r_c is summed).  Workgroup g writes
+// the tile partials g, g + G, ... of slots rz_slot and rr_slot: its (r_c, z_c) at g and zeros
+// elsewhere; workgroup 0 adds the tissue's (r, D^-1 r) and (r, r) at tile 0.
+__global__ __launch_bounds__(WG) void k_tl_coarse(TileArgs ta, TLArgs tl, int rz_slot, int rr_slot, int count_iter)
+{
+    SegScalars& S = ta.seg[0];
+    if (seg_done(S)) return;
+    extern __shared__ double rcs[];
+    constexpr int NW = WG / 64;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int i0 = blockIdx.x * tl.rows, i1 = min(tl.nc, i0 + tl.rows);
+    double pre[TL_PREF];
+    {
+        const int i = i0 + w;
+        const double* A = tl.Ainv + (size_t)min(i, tl.nc - 1) * tl.nc;
+#pragma unroll
+        for (int m = 0; m < TL_PREF; ++m) {
+            const int j = l + 64 * m;
+            pre[m] = (i < i1 && j < tl.nc) ? A[j] : 0.0;
+        }
+    }
+    for (int a = threadIdx.x; a < tl.n_agg; a += WG) {
+        double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int c = tl.agg_chunk[a]; c < tl.agg_chunk[a + 1]; ++c)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) sv[j] += tl.chunk_part[8 * (size_t)c + 2 + j];
+        const int off = tl.coff[a], k = tl.krank[a];
+        for (int j = 0; j < k; ++j) rcs[off + j] = sv[j];
+    }
+    double fine[2] = {0.0, 0.0};
+    if (blockIdx.x == 0)
+        for (int c = threadIdx.x; c < tl.n_chunk; c += WG) {
+            fine[0] += tl.chunk_part[8 * (size_t)c];
+            fine[1] += tl.chunk_part[8 * (size_t)c + 1];
+        }
+    __syncthreads();
+    double cz = 0.0;
+    for (int i = i0 + w; i < i1; i += NW) {
+        const double* A = tl.Ainv + (size_t)i * tl.nc;
+        double sv = 0.0;
+        if (i == i0 + w) {
+#pragma unroll
+            for (int m = 0; m < TL_PREF; ++m) {
+                const int j = l + 64 * m;
+                if (j < tl.nc) sv += pre[m] * rcs[j];
+            }
+            for (int j = l + 64 * TL_PREF; j < tl.nc; j += 64) sv += A[j] * rcs[j];
+        } else {
+            for (int j = l; j < tl.nc; j += 64) sv += A[j] * rcs[j];
+        }
+        sv = wave_sum(sv);
+        if (l == 0) {
+            tl.zc[i] = sv;
+            cz += sv * rcs[i];
+        }
+    }
+    __shared__ double wred[3][NW];
+    fine[0] = wave_sum(fine[0]);
+    fine[1] = wave_sum(fine[1]);
+    if (l == 0) { wred[0][w] = cz; wred[1][w] = fine[0]; wred[2][w] = fine[1]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NW; ++k) { t[0] += wred[0][k]; t[1] += wred[1][k]; t[2] += wred[2][k]; }
+        const int g = blockIdx.x;
+        ta.partial[(size_t)rz_slot * ta.T + g] = g == 0 ? t[1] + t[0] : t[0];
+        ta.partial[(size_t)rr_slot * ta.T + g] = g == 0 ? t[2] : 0.0;
+        if (g == 0 && count_iter) S.iters += 1;
+    }
+    for (int t = blockIdx.x + tl.G + (int)threadIdx.x * tl.G; t < ta.T; t += WG * tl.G) {
+        ta.partial[(size_t)rz_slot * ta.T + t] = 0.0;
+        ta.partial[(size_t)rr_slot * ta.T + t] = 0.0;
+    }
+}
+
+// z = D^-1 r + P z_c on the tile's rows
+__global__ __launch_bounds__(WG) void k_tl_z(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
+                                             const double* __restrict__ r, double* __restrict__ z)
+{
+    const int t = blockIdx.x;
+    if (seg_done(ta.seg[0])) return;
+    const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
+    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
+        const int n = d / 3, i = d % 3;
+        const double* Pn = tl.pcoef + 18 * (size_t)n + 6 * i;
+        const double* zc = tl.zc + tl.node_coff[n];
+        double v = dinv_row(dinv, r, d);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v += Pn[j] * zc[j];
+        z[d] = v;
+    }
+}
+
+// r = b - M x0 (x0 given) or b, x = x0 or 0, p_old = 0, and (b, b)
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_tl_init(TileArgs ta, const int32_t* __restrict__ row_ptr,
+                                                const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
+                                                const double* __restrict__ b, const double* __restrict__ x0,
+                                                double* __restrict__ x, double* __restrict__ r, double* __restrict__ p_old)
+{
+    extern __shared__ double prod[];
+    const int t = blockIdx.x;
+    const bool active = ta.seg[ta.tile_seg[t]].active;
+    const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
+    const int b0 = row_ptr[r0], b1 = row_ptr[r1];
+    if (x0) tile_products<VT>(col, val, nb, b0, b1, XLOAD(x0), prod);
+    double bb = 0.0;
+    for (int task = threadIdx.x; task < 3 * (r1 - r0); task += WG) {
+        const int rr = r0 + task / 3, comp = task % 3;
+        const int d = 3 * rr + comp;
+        const double bv = active ? b[d] : 0.0;
+        double rv = bv, xv = 0.0;
+        if (x0 && active) {
+            xv = x0[d];
+            rv = bv - row_sum(row_ptr, rr, comp, b0, prod);
+        }
+        x[d] = xv;
+        r[d] = rv;
+        p_old[d] = 0.0;
+        bb += bv * bv;
+    }
+    const int sl[1] = {SL_BB};
+    double vv[1] = {bb};
+    tile_write<1>(ta, t, sl, vv);
+}
+
+// Coarse operator Ac = P^T M P: one workgroup per pair of aggregates (a, b); every thread sums
+// the 6x6 products P_A^T M_AB P_B of a strided subset of the pair's blocks, then a fixed tree.
+__global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restrict__ pair_a,
+                                              const int32_t* __restrict__ pair_b, const int32_t* __restrict__ pair_ptr,
+                                              const int32_t* __restrict__ pair_blk, const int32_t* __restrict__ blk_row,
+                                              const int32_t* __restrict__ col, const double* __restrict__ val, int nb,
+                                              double* __restrict__ Ac)
+{
+    const int pr = blockIdx.x;
+    const int a = pair_a[pr], b = pair_b[pr];
+    double acc[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+    for (int k = pair_ptr[pr] + threadIdx.x; k < pair_ptr[pr + 1]; k += WG) {
+        const int blk = pair_blk[k];
+        const double* PA = tl.pcoef + 18 * (size_t)blk_row[blk];
+        const double* PB = tl.pcoef + 18 * (size_t)col[blk];
+        double m[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = val[(size_t)q * nb + blk];
+        double mp[18];  // M_AB P_B, 3 x 6
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                mp[6 * i + j] = m[3 * i] * PB[j] + m[3 * i + 1] * PB[6 + j] + m[3 * i + 2] * PB[12 + j];
+#pragma unroll
+        for (int pi = 0; pi < 6; ++pi)
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                acc[6 * pi + j] += PA[pi] * mp[j] + PA[6 + pi] * mp[6 + j] + PA[12 + pi] * mp[12 + j];
+    }
+    constexpr int NW = WG / 64;
+    __shared__ double red[36][NW];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int q = 0; q < 36; ++q) {
+        const double t = wave_sum(acc[q]);
+        if (l == 0) red[q][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 36) {
+        const int pi = threadIdx.x / 6, ri = threadIdx.x % 6;
+        if (pi < tl.krank[a] && ri < tl.krank[b]) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
+            Ac[(size_t)(tl.coff[a] + pi) * tl.nc + tl.coff[b] + ri] = t;
+        }
+    }
+}
+
+// the full symmetric inverse from the triangle the Cholesky inverse leaves (column-major lower:
+// element (i, j), i >= j, at Ac[j * n + i])
+__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ Ac, double* __restrict__ Ainv, int n)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)n * n) return;
+    const int i = (int)(k / n), j = (int)(k % n);
+    Ainv[k] = i >= j ? Ac[(size_t)j * n + i] : Ac[(size_t)i * n + j];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1171,6 +1452,299 @@ void warm_start_free(tf_handle* h)
     warm_start_clear(h);
 }
 
+// ---- two-level preconditioner: coarse space and coarse operator -------------------------------
+namespace {
+
+template <typename T>
+T* tl_alloc(size_t n)
+{
+    T* p = nullptr;
+    TF_HIP(hipMalloc((void**)&p, std::max<size_t>(1, n) * sizeof(T)));
+    return p;
+}
+
+template <typename T>
+void tl_upload(tf_handle* h, T*& dst, const std::vector<T>& v)
+{
+    if (dst) { (void)hipFree(dst); dst = nullptr; }
+    dst = tl_alloc<T>(v.size());
+    if (!v.empty()) TF_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+}
+
+void tl_release(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.agg_nodes, &tl.chunk_ptr, &tl.agg_chunk, &tl.coff, &tl.krank, &tl.pair_a,
+                        &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (double** p : {&tl.pcoef, &tl.chunk_part, &tl.Ac, &tl.Ainv, &tl.zc})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    tl.built = tl.fresh = tl.failed = false;
+}
+
+// Aggregates: the nodes' (x, y) in square cells of side sqrt(area / target), so that every
+// aggregate holds whole columns of the tissue's layers; cells under 16 nodes join the nearest
+// (by centroid) cell that has 16 or more.  Numbered in the order of their cell keys.
+std::vector<int32_t> tl_aggregates(const std::vector<double>& x, int N, int target, int& n_agg)
+{
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int n = 0; n < N; ++n)
+        for (int c = 0; c < 2; ++c) { lo[c] = std::min(lo[c], x[3 * n + c]); hi[c] = std::max(hi[c], x[3 * n + c]); }
+    const double area = std::max(hi[0] - lo[0], 1e-12) * std::max(hi[1] - lo[1], 1e-12);
+    const double side = std::sqrt(area / std::max(1, target));
+    std::vector<int64_t> key(N);
+    for (int n = 0; n < N; ++n) {
+        const int64_t cx = (int64_t)std::floor((x[3 * n] - lo[0]) / side);
+        const int64_t cy = (int64_t)std::floor((x[3 * n + 1] - lo[1]) / side);
+        key[n] = cx * 1000003 + cy;
+    }
+    std::vector<int64_t> keys(key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int K = (int)keys.size();
+    std::vector<int32_t> lab(N), cnt(K, 0);
+    std::vector<double> cen(2 * K, 0.0);
+    for (int n = 0; n < N; ++n) {
+        lab[n] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[n]) - keys.begin());
+        cnt[lab[n]] += 1;
+        cen[2 * lab[n]] += x[3 * n];
+        cen[2 * lab[n] + 1] += x[3 * n + 1];
+    }
+    for (int k = 0; k < K; ++k) { cen[2 * k] /= cnt[k]; cen[2 * k + 1] /= cnt[k]; }
+    std::vector<int32_t> remap(K);
+    bool any_big = false;
+    for (int k = 0; k < K; ++k) any_big = any_big || cnt[k] >= 16;
+    for (int k = 0; k < K; ++k) {
+        remap[k] = k;
+        if (!any_big) { remap[k] = 0; continue; }
+        if (cnt[k] >= 16) continue;
+        double best = INFINITY;
+        for (int j = 0; j < K; ++j) {
+            if (cnt[j] < 16) continue;
+            const double dx = cen[2 * j] - cen[2 * k], dy = cen[2 * j + 1] - cen[2 * k + 1];
+            const double d = dx * dx + dy * dy;
+            if (d < best) { best = d; remap[k] = j; }
+        }
+    }
+    std::vector<int32_t> id(K, -1);
+    n_agg = 0;
+    for (int k = 0; k < K; ++k)
+        if (remap[k] == k) id[k] = n_agg++;
+    for (int n = 0; n < N; ++n) lab[n] = id[remap[lab[n]]];
+    return lab;
+}
+
+// Orthonormal basis of the rigid-body modes (3 translations, 3 rotations about the centroid)
+// restricted to the aggregate's nodes: modified Gram-Schmidt run twice, a mode whose remainder
+// falls under 1e-10 of its norm is dropped.  DoFs flagged in excl (fixed DoFs and bound slaves,
+// whose rows and columns of M are unit ones) get zero rows, so that the correction never moves
+// them.  Writes rows 3n..3n+2 of the kept columns into pcoef[18 n ..] and returns the rank.
+int tl_modes(const std::vector<double>& x, const std::vector<uint8_t>& excl, const std::vector<int32_t>& nodes,
+             std::vector<double>& pcoef)
+{
+    const size_t m = 3 * nodes.size();
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int n : nodes)
+        for (int d = 0; d < 3; ++d) c[d] += x[3 * n + d];
+    for (int d = 0; d < 3; ++d) c[d] /= (double)nodes.size();
+    std::vector<std::vector<double>> B(6, std::vector<double>(m, 0.0));
+    for (size_t k = 0; k < nodes.size(); ++k) {
+        const int n = nodes[k];
+        const double px = x[3 * n] - c[0], py = x[3 * n + 1] - c[1], pz = x[3 * n + 2] - c[2];
+        for (int d = 0; d < 3; ++d) B[d][3 * k + d] = 1.0;
+        B[3][3 * k] = -py; B[3][3 * k + 1] = px;      // rotation about z
+        B[4][3 * k + 1] = -pz; B[4][3 * k + 2] = py;  // about x
+        B[5][3 * k] = pz; B[5][3 * k + 2] = -px;      // about y
+        for (int d = 0; d < 3; ++d)
+            if (excl[3 * n + d])
+                for (int j = 0; j < 6; ++j) B[j][3 * k + d] = 0.0;
+    }
+    std::vector<std::vector<double>> Q;
+    for (int j = 0; j < 6; ++j) {
+        std::vector<double> v = B[j];
+        double n0 = 0.0;
+        for (double a : v) n0 += a * a;
+        n0 = std::sqrt(n0);
+        if (!(n0 > 0.0)) continue;
+        for (int pass = 0; pass < 2; ++pass)
+            for (const auto& q : Q) {
+                double d = 0.0;
+                for (size_t i = 0; i < m; ++i) d += q[i] * v[i];
+                for (size_t i = 0; i < m; ++i) v[i] -= d * q[i];
+            }
+        double nv = 0.0;
+        for (double a : v) nv += a * a;
+        nv = std::sqrt(nv);
+        if (nv <= 1e-10 * n0) continue;
+        for (double& a : v) a /= nv;
+        Q.push_back(std::move(v));
+    }
+    for (size_t k = 0; k < nodes.size(); ++k)
+        for (int i = 0; i < 3; ++i)
+            for (size_t j = 0; j < Q.size(); ++j) pcoef[18 * (size_t)nodes[k] + 6 * i + j] = Q[j][3 * k + i];
+    return (int)Q.size();
+}
+
+}  // namespace
+
+// Coarse space of the current positions and pattern: aggregates, their modes and node lists, and
+// the aggregate pairs that share Jacobian blocks.
+static void twolevel_build(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    tl_release(h);
+    const int N = h->N, T = h->T;
+    std::vector<double> x(3 * (size_t)N);
+    TF_HIP(hipMemcpyAsync(x.data(), h->uk, x.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    std::vector<uint8_t> excl(3 * (size_t)N);
+    TF_HIP(hipMemcpyAsync(excl.data(), h->fixed, excl.size(), hipMemcpyDeviceToHost, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+    for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
+    const int max_nc = std::min(4096, h->max_lds / (int)sizeof(double));
+    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, N / 190);
+    target = std::min(target, max_nc / 6);
+    int A = 0;
+    std::vector<int32_t> agg = tl_aggregates(x, N, target, A);
+    std::vector<std::vector<int32_t>> members(A);
+    for (int n = 0; n < N; ++n) members[agg[n]].push_back(n);
+    std::vector<double> pcoef(18 * (size_t)N, 0.0);
+    std::vector<int32_t> krank(A), coff(A + 1, 0);
+    for (int a = 0; a < A; ++a) {
+        krank[a] = tl_modes(x, excl, members[a], pcoef);
+        coff[a + 1] = coff[a] + krank[a];
+    }
+    const int nc = coff[A];
+    if (nc > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");
+    std::vector<int32_t> agg_nodes, chunk_ptr(1, 0), agg_chunk(A + 1, 0);
+    agg_nodes.reserve(N);
+    for (int a = 0; a < A; ++a) {
+        const int m = (int)members[a].size();
+        for (int k0 = 0; k0 < m; k0 += TL_CHUNK) {
+            const int k1 = std::min(m, k0 + TL_CHUNK);
+            agg_nodes.insert(agg_nodes.end(), members[a].begin() + k0, members[a].begin() + k1);
+            chunk_ptr.push_back((int32_t)agg_nodes.size());
+        }
+        agg_chunk[a + 1] = (int32_t)chunk_ptr.size() - 1;
+    }
+    const int C = (int)chunk_ptr.size() - 1;
+    // aggregate pairs with their Jacobian blocks in block order
+    const std::vector<int32_t>& rp = h->h_row_ptr;
+    const std::vector<int32_t>& cl = h->h_col;
+    std::vector<int64_t> pkey;
+    pkey.reserve(h->nb);
+    for (int r = 0; r < N; ++r)
+        for (int j = rp[r]; j < rp[r + 1]; ++j) pkey.push_back((int64_t)agg[r] * A + agg[cl[j]]);
+    std::vector<int64_t> pk(pkey);
+    std::sort(pk.begin(), pk.end());
+    pk.erase(std::unique(pk.begin(), pk.end()), pk.end());
+    const int P = (int)pk.size();
+    std::vector<int32_t> pair_a(P), pair_b(P), pair_ptr(P + 1, 0), pair_blk(pkey.size());
+    std::vector<int32_t> bpair(pkey.size());
+    for (size_t j = 0; j < pkey.size(); ++j) {
+        bpair[j] = (int32_t)(std::lower_bound(pk.begin(), pk.end(), pkey[j]) - pk.begin());
+        pair_ptr[bpair[j] + 1] += 1;
+    }
+    for (int q = 0; q < P; ++q) {
+        pair_ptr[q + 1] += pair_ptr[q];
+        pair_a[q] = (int32_t)(pk[q] / A);
+        pair_b[q] = (int32_t)(pk[q] % A);
+    }
+    {
+        std::vector<int32_t> fill(pair_ptr.begin(), pair_ptr.end() - 1);
+        for (size_t j = 0; j < pkey.size(); ++j) pair_blk[fill[bpair[j]]++] = (int32_t)j;
+    }
+    tl_upload(h, tl.node_agg, agg);
+    {
+        std::vector<int32_t> node_coff(N);
+        for (int n = 0; n < N; ++n) node_coff[n] = coff[agg[n]];
+        tl_upload(h, tl.node_coff, node_coff);
+        TF_HIP(hipStreamSynchronize(h->stream));
+    }
+    tl_upload(h, tl.agg_nodes, agg_nodes);
+    tl_upload(h, tl.chunk_ptr, chunk_ptr);
+    tl_upload(h, tl.agg_chunk, agg_chunk);
+    tl_upload(h, tl.pcoef, pcoef);
+    tl_upload(h, tl.coff, coff);
+    tl_upload(h, tl.krank, krank);
+    tl_upload(h, tl.pair_a, pair_a);
+    tl_upload(h, tl.pair_b, pair_b);
+    tl_upload(h, tl.pair_ptr, pair_ptr);
+    tl_upload(h, tl.pair_blk, pair_blk);
+    // one coarse row per wave, at most one workgroup per tile (the partials k_tl_coarse writes)
+    tl.G = std::min(T, (nc + WG / 64 - 1) / (WG / 64));
+    tl.rows = (nc + tl.G - 1) / tl.G;
+    tl.G = (nc + tl.rows - 1) / tl.rows;
+    tl.chunk_part = tl_alloc<double>(8 * (size_t)C);
+    tl.Ac = tl_alloc<double>((size_t)nc * nc);
+    tl.Ainv = tl_alloc<double>((size_t)nc * nc);
+    tl.zc = tl_alloc<double>(nc + 6);
+    launch_fill(h, tl.zc, 0.0, nc + 6);
+    tl.info = tl_alloc<int32_t>(1);
+    if (!tl.blas) {
+        rocblas_handle bh = nullptr;
+        if (rocblas_create_handle(&bh) != rocblas_status_success)
+            throw std::runtime_error("two-level preconditioner: rocblas_create_handle failed");
+        tl.blas = bh;
+    }
+    if (rocblas_set_stream((rocblas_handle)tl.blas, h->stream) != rocblas_status_success)
+        throw std::runtime_error("two-level preconditioner: rocblas_set_stream failed");
+    TF_HIP(hipStreamSynchronize(h->stream));  // the host vectors above are temporaries
+    tl.n_agg = A;
+    tl.nc = nc;
+    tl.n_pairs = P;
+    tl.n_chunk = C;
+    tl.T = T;
+    tl.h_node_agg = agg;
+    tl.built = true;
+    drop_krylov_graphs(h);  // captured chunks hold the coarse arrays
+}
+
+static TLArgs tl_args(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    return TLArgs{tl.node_coff, tl.agg_nodes, tl.chunk_ptr, tl.agg_chunk, tl.pcoef, tl.coff, tl.krank, tl.chunk_part,
+                  tl.Ainv, tl.zc, tl.nc, tl.n_agg, tl.n_chunk, tl.G, tl.rows};
+}
+
+// Ac = P^T M P from the FP64 Jacobian, its Cholesky factor and inverse (rocSOLVER), symmetrised.
+// A matrix that is not positive definite leaves the two-level path off until the next refresh.
+static void twolevel_refresh(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    const int nc = tl.nc;
+    launch_fill(h, tl.Ac, 0.0, (int64_t)nc * nc);
+    hipLaunchKernelGGL(k_tl_ac, dim3(tl.n_pairs), dim3(WG), 0, h->stream, tl_args(h), tl.pair_a, tl.pair_b, tl.pair_ptr,
+                       tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tl.Ac);
+    TF_HIP(hipGetLastError());
+    rocblas_handle bh = (rocblas_handle)tl.blas;
+    int32_t info = 0;
+    if (rocsolver_dpotrf(bh, rocblas_fill_lower, nc, tl.Ac, nc, tl.info) != rocblas_status_success)
+        throw std::runtime_error("two-level preconditioner: rocsolver_dpotrf failed");
+    TF_HIP(hipMemcpyAsync(&info, tl.info, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+    tl.fresh = true;
+    tl.age = 0;
+    tl.failed = info != 0;
+    if (tl.failed) return;
+    if (rocsolver_dpotri(bh, rocblas_fill_lower, nc, tl.Ac, nc, tl.info) != rocblas_status_success)
+        throw std::runtime_error("two-level preconditioner: rocsolver_dpotri failed");
+    const size_t n2 = (size_t)nc * nc;
+    hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, tl.Ainv, nc);
+    TF_HIP(hipGetLastError());
+}
+
+void twolevel_invalidate(tf_handle* h)
+{
+    h->tl.built = false;
+    h->tl.fresh = false;
+}
+
+void twolevel_free(tf_handle* h)
+{
+    tl_release(h);
+    if (h->tl.blas) { (void)rocblas_destroy_handle((rocblas_handle)h->tl.blas); h->tl.blas = nullptr; }
+}
+
 namespace {
 
 template <typename VT, int P>
@@ -1208,6 +1782,29 @@ void enqueue_cg_iter(tf_handle* h, const VT* val)
                        h->kv, h->opts.krylov_max_iter);
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL(k_cg_x<P>, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks);
+}
+
+constexpr int KIND_TL = 64;  // two-level PCG (internal solver kind)
+
+// Two-level PCG: the CG buffers of enqueue_cg_iter; z = ks holds D^-1 r + P Ac^-1 P^T r.
+template <typename VT, int P>
+void enqueue_tl_iter(tf_handle* h, const VT* val)
+{
+    TileArgs ta = tile_args(h);
+    TLArgs tla = tl_args(h);
+    const dim3 grid(h->T), blk(WG);
+    double* p_old = P ? h->kph : h->kp;
+    double* p_new = P ? h->kp : h->kph;
+    hipEvent_t ev[2];
+    timer_begin(h, h->t_spmv, ev);
+    hipLaunchKernelGGL((k_cg_q<VT, P>), grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr,
+                       h->col, val, h->nb, h->ks, p_old, p_new, h->kv, h->opts.krylov_max_iter);
+    timer_end(h, h->t_spmv, ev);
+    hipLaunchKernelGGL((k_tl_xr<P, false>), dim3(tla.n_chunk), dim3(64), 0, h->stream, ta, tla, h->dinv, p_new, h->kv,
+                       h->kx, h->kr);
+    hipLaunchKernelGGL(k_tl_coarse, dim3(tla.G), blk, sizeof(double) * (size_t)tla.nc, h->stream, ta, tla,
+                       SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
+    hipLaunchKernelGGL(k_tl_z, grid, blk, 0, h->stream, ta, tla, h->dinv, h->kr, h->ks);
 }
 
 // Single-kernel CG buffers: x = kx, p = kp, r = kr, w = kv, s = ks (own rows only) and the
@@ -1274,6 +1871,9 @@ void enqueue_chunk(tf_handle* h, const VT* val, int kind, int iters)
         } else if (kind == TF_KRYLOV_CG1) {
             enqueue_cg1_iter<VT, 1>(h, val);  // chunks start at an odd kernel (kernel 0 is the init)
             enqueue_cg1_iter<VT, 0>(h, val);
+        } else if (kind == KIND_TL) {
+            enqueue_tl_iter<VT, 0>(h, val);
+            enqueue_tl_iter<VT, 1>(h, val);
         } else if (kind == TF_KRYLOV_CG) {
             enqueue_cg_iter<VT, 0>(h, val);
             enqueue_cg_iter<VT, 1>(h, val);
@@ -1314,12 +1914,23 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, sizeof(double) * (3 * (size_t)h->tile_cap + h->wg), h->stream, ta,
                            h->row_ptr, h->col, val, h->nb, h->dinv, h->g, h->kx, h->kr, h->kv, h->ks, h->kp, h->kph,
                            h->kt, h->kr0, h->tile_cap);
+    else if (kind == KIND_TL)
+        hipLaunchKernelGGL(k_tl_init<VT>, grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta,
+                           h->row_ptr, h->col, val, h->nb, h->g, h->warm_guess, h->kx, h->kr, h->kp);
     else if (kind == TF_KRYLOV_CG)
         hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
     else
         hipLaunchKernelGGL(k_bicg_init, grid, blk, 0, h->stream, ta, h->g, h->kx, h->kr, h->kr0, h->kp, h->kv);
     hipLaunchKernelGGL(k_seg_init, dim3(h->S), blk, 0, h->stream, ta, rtol2, atol2,
                        h->seg_rtol_on ? h->seg_rtol2 : nullptr);
+    if (kind == KIND_TL) {  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
+        const TLArgs tla = tl_args(h);
+        hipLaunchKernelGGL((k_tl_xr<0, true>), dim3(tla.n_chunk), dim3(64), 0, h->stream, ta, tla, h->dinv, h->kp, h->kv,
+                           h->kx, h->kr);
+        hipLaunchKernelGGL(k_tl_coarse, dim3(tla.G), blk, sizeof(double) * (size_t)tla.nc, h->stream, ta, tla,
+                           (int)SL_RZ0, (int)SL_RR0, 0);
+        hipLaunchKernelGGL(k_tl_z, grid, blk, 0, h->stream, ta, tla, h->dinv, h->kr, h->ks);
+    }
     TF_HIP(hipGetLastError());
     int chunk = std::max(2, h->opts.iters_per_sync);
     chunk += chunk & 1;  // iterations come in parity pairs
@@ -1426,8 +2037,20 @@ int krylov_solve(tf_handle* h)
     int kind = h->opts.krylov;
     const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
     auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
+    const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
+    if (cg_kind && h->S == 1 && h->opts.preconditioner == TF_PRECOND_TWO_LEVEL) {
+        // the coarse space follows the pattern and tiling; the coarse operator is rebuilt at the
+        // first Newton iteration of every coarse_refresh-th timestep (and for every solve outside
+        // the Newton loop).  A coarse operator from an earlier Jacobian is still symmetric positive
+        // definite, so it changes the iteration count, not the solution.
+        if (!h->tl.built || h->tl.T != h->T) twolevel_build(h);
+        if (h->krylov_call == 0) h->tl.age += 1;
+        if (!h->tl.fresh || h->krylov_call < 0 || (h->krylov_call == 0 && h->tl.age >= std::max(1, h->opts.coarse_refresh)))
+            twolevel_refresh(h);
+        if (!h->tl.failed) kind = KIND_TL;
+    }
     int st = run(kind);
-    if (st == TF_KRYLOV_STALL && (kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG)) {
+    if (st == TF_KRYLOV_STALL && (cg_kind || kind == KIND_TL)) {
         // not positive definite (or too slow): BiCGSTAB from zero
         const int cg_iters = h->last_krylov_iters;
         h->fallbacks += 1;
