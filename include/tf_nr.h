@@ -65,13 +65,15 @@ typedef struct tf_mesh_desc {
  * operand was completed by the previous kernel. */
 enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_CG1 = 3, TF_KRYLOV_PIPECG = 4 };
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
-/* TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M (the default).
+/* TF_PRECOND_AUTO (the default): two-level for a handle of one tissue with 4096 nodes or more,
+ * else block Jacobi.  TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M.
  * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi plus an additive coarse
  * correction P (P^T M P)^-1 P^T r, P the rigid-body modes of node aggregates (square cells of
  * the tissue's x-y extent, whole layer columns each).  The coarse operator is rebuilt at the
  * first Newton iteration of every coarse_refresh-th timestep and for every solve outside the
- * Newton loop; handles with several tissues use block Jacobi. */
-enum { TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
+ * Newton loop; handles with several tissues use block Jacobi.  The coarse inverse is held in FP32
+ * (it only preconditions: residuals, matrix and Krylov vectors stay in the solver's precision). */
+enum { TF_PRECOND_AUTO = -1, TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
 
 typedef struct tf_solver_opts {
     int32_t krylov;            /* TF_KRYLOV_*                                                  */
@@ -112,6 +114,7 @@ typedef struct tf_step_stats {
     double last_du_norm;         /* max over tissues of the last ||dU||_2                      */
     double ms_grow, ms_assemble, ms_solve, ms_total;
     int32_t krylov_by_newton[4]; /* Krylov iterations of Newton iterations 0-3 (max over tissues) */
+    int32_t two_level;           /* 1 when the step's CG solves used the two-level preconditioner */
 } tf_step_stats;
 
 /* ---- lifetime --------------------------------------------------------------------------- */
@@ -258,7 +261,8 @@ int tf_adjoint_get(tf_handle* h, double* rate_bar, double* lambda_bar, double* m
  * 1 <= tile_cap <= min(4 wg, 2048) blocks (a tile never holds more than wg/3 block rows). */
 int tf_set_tiling(tf_handle* h, int32_t wg, int32_t tile_cap);
 int tf_get_tiling(const tf_handle* h, int32_t* wg, int32_t* tile_cap, int32_t* n_tiles);
-/* Times n_iters single-kernel CG iterations on the Jacobian at the current positions for every
+/* Times n_iters Krylov iterations (the two-level PCG when it is active, else the single-kernel
+ * CG) on the Jacobian at the current positions for every
  * candidate tiling (power-of-two capacities, and capacities that cut the rows into 1-4 tiles per
  * compute unit), keeps the fastest; per-variant results (up to 32) are written to the arrays. */
 int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, int32_t* wg, int32_t* tile_cap,
@@ -271,6 +275,11 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
 int tf_kernel_timing(tf_handle* h, const char* name, int32_t enable, double* avg_ms, int64_t* launches);
 /* Algorithmic bytes of one launch of the named kernel (unique minimum traffic). */
 int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes);
+/* Average device time (ms) of one launch of the kernel priced under "spmv" (the dominant Krylov
+ * kernel of the active solver: k_cg_q for the two-level and textbook CG, k_cg1, k_pipe),
+ * n_launches replayed back to back in one graph on the current Jacobian and timed with HIP events
+ * around the replay.  Leaves the Krylov work vectors undefined.  0 for BiCGSTAB. */
+int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ STATUS_NAMES = {0: "TF_OK", 1: "TF_NOT_CONVERGED", 2: "TF_KRYLOV_STALL", 3: "TF_
                 5: "TF_OOM", 6: "TF_HIP_ERROR", 7: "TF_INVALID"}
 KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_GMRES, KRYLOV_CG1, KRYLOV_PIPECG = -1, 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
-PRECOND_BLOCK_JACOBI, PRECOND_TWO_LEVEL = 0, 1
+PRECOND_AUTO, PRECOND_BLOCK_JACOBI, PRECOND_TWO_LEVEL = -1, 0, 1
 
 # Every symbol include/tf_nr.h declares (checked by the CPU test suite).
 EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf_set_options", "tf_get_options",
@@ -26,7 +26,7 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_set_material", "tf_grow", "tf_update_viscosity_surfaces", "tf_get_viscosity_surfaces",
            "tf_begin_step", "tf_assemble", "tf_get_residual", "tf_get_jacobian_bsr", "tf_element_dump", "tf_solve",
            "tf_update_uk", "tf_nr_step", "tf_timestep", "tf_get_segment_stats", "tf_kernel_timing",
-           "tf_kernel_bytes", "tf_set_tiling", "tf_get_tiling", "tf_autotune", "tf_set_bound_dofs",
+           "tf_kernel_bytes", "tf_kernel_probe", "tf_set_tiling", "tf_get_tiling", "tf_autotune", "tf_set_bound_dofs",
            "tf_set_ablated_nodes", "tf_set_growth_state", "tf_get_growth_rotation", "tf_apply_jacobian",
            "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get",
            "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs",
@@ -62,7 +62,7 @@ class StepStats(C.Structure):
                 ("rotated_elements", C.c_int32), ("krylov_fallbacks", C.c_int32), ("packing_pairs", C.c_int32),
                 ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double),
-                ("krylov_by_newton", C.c_int32 * 4)]
+                ("krylov_by_newton", C.c_int32 * 4), ("two_level", C.c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -113,6 +113,7 @@ def load(path: str | None = None):
         "tf_get_segment_stats": ([vp, vp, vp, vp], i32),
         "tf_kernel_timing": ([vp, C.c_char_p, i32, vp, vp], i32),
         "tf_kernel_bytes": ([vp, C.c_char_p, vp], i32),
+        "tf_kernel_probe": ([vp, i32, vp], i32),
         "tf_set_tiling": ([vp, i32, i32], i32),
         "tf_get_tiling": ([vp, vp, vp, vp], i32),
         "tf_autotune": ([vp, dbl, i32, vp, vp, vp, vp], i32),

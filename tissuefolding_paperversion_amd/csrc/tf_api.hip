@@ -71,6 +71,7 @@ int fail(tf_handle* h, int code, const std::string& msg)
 
 void free_all(tf_handle* h)
 {
+    drop_krylov_graphs(h);  // before the buffers the captured chunks use
     void* ptrs[] = {h->conn, h->xref, h->fg, h->invfg, h->lam, h->mu, h->eta, h->rate, h->ginc, h->exposed, h->elem_seg,
                     h->uk, h->un, h->g, h->du, h->eta_ext, h->fixed, h->sa, h->face_area, h->kelem, h->felem,
                     h->status, h->row_ptr, h->col, h->blk_row, h->diag_blk, h->val, h->val32, h->dinv, h->blk_ptr,
@@ -474,7 +475,7 @@ void tf_default_options(tf_solver_opts* o)
     o->xcd_tile_map = 1;
     o->krylov_warm_start = 1;
     o->implicit_packing = 1;
-    o->preconditioner = TF_PRECOND_BLOCK_JACOBI;
+    o->preconditioner = TF_PRECOND_AUTO;
     o->coarse_refresh = 10;
 }
 
@@ -665,7 +666,7 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         (o->matrix_precision != TF_PREC_FP64 && o->matrix_precision != TF_PREC_FP32) ||
         o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16 ||
         (o->implicit_packing != 0 && o->implicit_packing != 1) ||
-        (o->preconditioner != TF_PRECOND_BLOCK_JACOBI && o->preconditioner != TF_PRECOND_TWO_LEVEL) ||
+        o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
         o->coarse_aggregates < 0 || o->coarse_refresh < 0)
         return fail(h, TF_INVALID, "invalid solver options");
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
@@ -1044,6 +1045,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     int status = TF_OK;
     int krylov_total = 0;
     h->fallbacks = 0;
+    h->tl_used = false;
     double ms_asm = 0, ms_sol = 0;
     int it = 0;
     while (true) {
@@ -1109,6 +1111,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         stats->krylov_iterations = krylov_total;
         stats->krylov_fallbacks = h->fallbacks;
         stats->packing_pairs = h->pk.n;
+        stats->two_level = h->tl_used ? 1 : 0;
         stats->converged = allc;
         stats->last_du_norm = worst;
         stats->ms_assemble = ms_asm;
@@ -1557,7 +1560,7 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
             continue;
         }
         build_tiles(h, variants[v][0], variants[v][1]);
-        const double ms = krylov_time_cg1(h, n_iters);
+        const double ms = krylov_time_iteration(h, n_iters);
         if (wg_out) wg_out[v] = variants[v][0];
         if (cap_out) cap_out[v] = variants[v][1];
         if (ms_out) ms_out[v] = ms;
@@ -1565,6 +1568,16 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
     }
     build_tiles(h, variants[bi][0], variants[bi][1]);
     if (n_variants) *n_variants = nv;
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms)
+{
+    if (!h || !avg_ms || n_launches < 2) return TF_INVALID;
+    TF_GUARD_BEGIN
+    *avg_ms = krylov_probe(h, n_launches);
+    TF_HIP(hipStreamSynchronize(h->stream));
     return TF_OK;
     TF_GUARD_END(h)
 }
@@ -1611,7 +1624,9 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver
-        if (kind == TF_KRYLOV_CG1)  // SpMV; x,p,r,w,s,u,m,q read and x,p,r,w,s,u,m,q written once; D^-1 rows
+        if (twolevel_active(h))  // k_cg_q: SpMV on p = z + beta p_old: z, p_old read; p_new, q written
+            *bytes = mat + n * 8.0 * 4.0;
+        else if (kind == TF_KRYLOV_CG1)  // SpMV; x,p,r,w,s,u,m,q read and x,p,r,w,s,u,m,q written once; D^-1 rows
             *bytes = mat + n * 8.0 * 16.0 + N * 72.0;
         else if (kind == TF_KRYLOV_PIPECG)  // SpMV on m; z,q,s,p,x,r,u,w,m read, the same and m' written; D^-1 rows
             *bytes = mat + n * 8.0 * 18.0 + N * 72.0;

@@ -62,6 +62,8 @@ struct KernelTimer {
     std::vector<hipEvent_t> pending;  // start/stop pairs not yet read back
     // graph replays: events bracketing a whole replayed chunk
     hipEvent_t sample[2] = {nullptr, nullptr};
+    double replay_ms = 0.0;           // whole replays, per launched iteration (all launches)
+    int64_t replay_launches = 0;
     bool capturing = false;           // inside a stream capture: no per-launch events
 };
 
@@ -222,6 +224,7 @@ struct tf_handle {
     // convergence check (krylov_call: the Newton iteration being solved, -1 outside the loop)
     std::vector<int32_t> krylov_pred;
     int32_t krylov_call = -1;
+    bool tl_used = false;             // a solve of the current step ran the two-level PCG
 
     // graphs
     // one captured Krylov chunk per Jacobian value type (FP64, FP32)
@@ -246,7 +249,8 @@ struct tf_handle {
         int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank
         int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
         double *chunk_part = nullptr;                       // [C][8]
-        double *Ac = nullptr, *Ainv = nullptr, *zc = nullptr;
+        double *Ac = nullptr, *zc = nullptr;
+        float* Ainv = nullptr;                              // [nc][nc] FP32 copy of Ac^-1
         int32_t rows = 0;                                   // coarse rows per workgroup of k_tl_coarse
         int32_t* info = nullptr;
         void* blas = nullptr;                               // rocblas_handle
@@ -306,7 +310,15 @@ void warm_start_free(tf_handle* h);
 void warm_start_clear(tf_handle* h);
 void twolevel_free(tf_handle* h);
 void twolevel_invalidate(tf_handle* h);  // pattern or tiling changed
-double krylov_time_cg1(tf_handle* h, int n_iters);  // ms per single-kernel CG iteration
+// ms per Krylov iteration (whole replays) of the solver krylov_solve would run for a symmetric
+// system: the two-level PCG when twolevel_active, else the single-kernel CG
+double krylov_time_iteration(tf_handle* h, int n_iters);
+bool twolevel_active(const tf_handle* h);  // the CG solves of this handle use the two-level preconditioner
+// Average device time (ms) of one launch of the dominant Krylov kernel (the one tf_kernel_bytes
+// prices under "spmv"), n_launches replayed back to back on the current Jacobian; 0 for BiCGSTAB
+double krylov_probe(tf_handle* h, int n_launches);
+#define TF_TWO_LEVEL_MIN_NODES 4096       // TF_PRECOND_AUTO: two-level from this many nodes
+#define TF_GRAPH_NODES 32                 // kernel launches per captured Krylov graph (at most)
 void launch_spmv(tf_handle* h, const double* x, double* y);
 
 // error helpers

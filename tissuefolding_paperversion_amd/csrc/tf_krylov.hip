@@ -199,6 +199,13 @@ __global__ __launch_bounds__(WG) void k_seg_init(TileArgs ta, double rtol2, doub
     }
 }
 
+// reopens every tissue's solve (timing probes): active, not converged, not stalled
+__global__ void k_seg_reopen(SegScalars* seg, int S)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) { seg[s].converged = 0; seg[s].stalled = 0; seg[s].active = 1; }
+}
+
 // Convergence / cap test shared by the first kernel of every iteration.  Returns true when the
 // tissue stops (all tiles of the tissue take the same decision).
 __device__ inline bool check_stop(const TileArgs& ta, int t, int sg, int slot_rr, int maxit)
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 //   k_tl_z       (tiles)      z = D^-1 r + P z_c
 // (r, z) = (r, D^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
 constexpr int TL_CHUNK = 64;  // nodes per chunk of an aggregate (one wave of k_tl_xr each)
-constexpr int TL_PREF = 16;   // doubles per lane of a coarse row prefetched before r_c is known
+constexpr int TL_PREF = 32;   // entries per lane of a coarse row prefetched before r_c is known
 
 struct TLArgs {
     const int32_t* node_coff;  // [N] first coarse unknown of the node's aggregate
@@ -479,7 +486,7 @@ struct TLArgs {
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
     double* chunk_part;        // [C][8] (r, D^-1 r), (r, r) and the 6 mode sums of every chunk
-    const double* Ainv;        // [nc][nc]
+    const float* Ainv;         // [nc][nc] (a preconditioner: FP32 storage, FP64 sums)
     double* zc;                // [nc + 6] (the last 6 stay zero)
     int nc, n_agg, n_chunk, G, rows;  // rows: coarse rows per workgroup of k_tl_coarse
 };
@@ -557,14 +564,14 @@ __global__ __launch_bounds__(WG) void k_tl_coarse(TileArgs ta, TLArgs tl, int rz
     constexpr int NW = WG / 64;
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int i0 = blockIdx.x * tl.rows, i1 = min(tl.nc, i0 + tl.rows);
-    double pre[TL_PREF];
+    float pre[TL_PREF];
     {
         const int i = i0 + w;
-        const double* A = tl.Ainv + (size_t)min(i, tl.nc - 1) * tl.nc;
+        const float* A = tl.Ainv + (size_t)min(i, tl.nc - 1) * tl.nc;
 #pragma unroll
         for (int m = 0; m < TL_PREF; ++m) {
             const int j = l + 64 * m;
-            pre[m] = (i < i1 && j < tl.nc) ? A[j] : 0.0;
+            pre[m] = (i < i1 && j < tl.nc) ? A[j] : 0.0f;
         }
     }
     for (int a = threadIdx.x; a < tl.n_agg; a += WG) {
@@ -584,17 +591,17 @@ __global__ __launch_bounds__(WG) void k_tl_coarse(TileArgs ta, TLArgs tl, int rz
     __syncthreads();
     double cz = 0.0;
     for (int i = i0 + w; i < i1; i += NW) {
-        const double* A = tl.Ainv + (size_t)i * tl.nc;
+        const float* A = tl.Ainv + (size_t)i * tl.nc;
         double sv = 0.0;
         if (i == i0 + w) {
 #pragma unroll
             for (int m = 0; m < TL_PREF; ++m) {
                 const int j = l + 64 * m;
-                if (j < tl.nc) sv += pre[m] * rcs[j];
+                if (j < tl.nc) sv += (double)pre[m] * rcs[j];
             }
-            for (int j = l + 64 * TL_PREF; j < tl.nc; j += 64) sv += A[j] * rcs[j];
+            for (int j = l + 64 * TL_PREF; j < tl.nc; j += 64) sv += (double)A[j] * rcs[j];
         } else {
-            for (int j = l; j < tl.nc; j += 64) sv += A[j] * rcs[j];
+            for (int j = l; j < tl.nc; j += 64) sv += (double)A[j] * rcs[j];
         }
         sv = wave_sum(sv);
         if (l == 0) {
@@ -725,14 +732,14 @@ __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restri
     }
 }
 
-// the full symmetric inverse from the triangle the Cholesky inverse leaves (column-major lower:
-// element (i, j), i >= j, at Ac[j * n + i])
-__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ Ac, double* __restrict__ Ainv, int n)
+// the full symmetric inverse, rounded to FP32, from the triangle the Cholesky inverse leaves
+// (column-major lower: element (i, j), i >= j, at Ac[j * n + i])
+__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ Ac, float* __restrict__ Ainv, int n)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (size_t)n * n) return;
     const int i = (int)(k / n), j = (int)(k % n);
-    Ainv[k] = i >= j ? Ac[(size_t)j * n + i] : Ac[(size_t)i * n + j];
+    Ainv[k] = (float)(i >= j ? Ac[(size_t)j * n + i] : Ac[(size_t)i * n + j]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1477,8 +1484,9 @@ void tl_release(tf_handle* h)
     for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.agg_nodes, &tl.chunk_ptr, &tl.agg_chunk, &tl.coff, &tl.krank, &tl.pair_a,
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&tl.pcoef, &tl.chunk_part, &tl.Ac, &tl.Ainv, &tl.zc})
+    for (double** p : {&tl.pcoef, &tl.chunk_part, &tl.Ac, &tl.zc})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     tl.built = tl.fresh = tl.failed = false;
 }
 
@@ -1592,6 +1600,7 @@ int tl_modes(const std::vector<double>& x, const std::vector<uint8_t>& excl, con
 static void twolevel_build(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
+    drop_krylov_graphs(h);  // captured chunks hold the coarse arrays freed below
     tl_release(h);
     const int N = h->N, T = h->T;
     std::vector<double> x(3 * (size_t)N);
@@ -1676,7 +1685,7 @@ static void twolevel_build(tf_handle* h)
     tl.G = (nc + tl.rows - 1) / tl.rows;
     tl.chunk_part = tl_alloc<double>(8 * (size_t)C);
     tl.Ac = tl_alloc<double>((size_t)nc * nc);
-    tl.Ainv = tl_alloc<double>((size_t)nc * nc);
+    tl.Ainv = tl_alloc<float>((size_t)nc * nc);
     tl.zc = tl_alloc<double>(nc + 6);
     launch_fill(h, tl.zc, 0.0, nc + 6);
     tl.info = tl_alloc<int32_t>(1);
@@ -1696,7 +1705,6 @@ static void twolevel_build(tf_handle* h)
     tl.T = T;
     tl.h_node_agg = agg;
     tl.built = true;
-    drop_krylov_graphs(h);  // captured chunks hold the coarse arrays
 }
 
 static TLArgs tl_args(tf_handle* h)
@@ -1932,12 +1940,20 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         hipLaunchKernelGGL(k_tl_z, grid, blk, 0, h->stream, ta, tla, h->dinv, h->kr, h->ks);
     }
     TF_HIP(hipGetLastError());
-    int chunk = std::max(2, h->opts.iters_per_sync);
-    chunk += chunk & 1;  // iterations come in parity pairs
+    // A captured graph holds at most TF_GRAPH_NODES kernel launches (an even number of
+    // iterations); a chunk between convergence checks is one replay of it, at most iters_per_sync
+    // iterations.  (The two-level PCG, four launches per iteration, checks every 8 iterations: a
+    // solve's last chunk then overshoots by fewer no-op launches than it costs in checks.)
+    const int kpi = kind == KIND_TL ? 4 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
+    int chunk = std::min(std::max(2, h->opts.iters_per_sync), std::max(2, TF_GRAPH_NODES / kpi));
+    chunk -= chunk & 1;  // iterations come in parity pairs
+    const int g_it = chunk, per_chunk = 1;
     const bool timing = h->t_spmv.enabled;
-    // Graph replay.  While timing, only the single-kernel CG keeps the graph: a replayed chunk is
-    // then exactly `chunk` launches of the dominant kernel, timed by events around the replay.
-    const bool graph = h->opts.use_graph != 0 && (!timing || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG);
+    // Graph replay.  While timing, the kinds with more than one kernel per iteration (other than
+    // the two-level PCG) launch eagerly, each launch of the dominant kernel bracketed by events;
+    // replays are timed whole.
+    const bool graph = h->opts.use_graph != 0 &&
+                       (!timing || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG || kind == KIND_TL);
     int done_iters = 0;
     // first burst: about 80% of the iterations this Newton iteration's solve took last timestep,
     // replayed without a convergence check (a solve that converges inside the burst leaves the
@@ -1946,30 +1962,37 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const int call = h->krylov_call;
     if (graph && call >= 0 && call < (int)h->krylov_pred.size() && h->krylov_pred[call] > 0)
         reps = std::max(1, (int)(0.8 * h->krylov_pred[call]) / chunk);
-    if (graph && timing && !h->t_spmv.sample[0]) {
-        TF_HIP(hipEventCreate(&h->t_spmv.sample[0]));
-        TF_HIP(hipEventCreate(&h->t_spmv.sample[1]));
+    KernelTimer& tm = h->t_spmv;
+    if (graph && timing && !tm.sample[0]) {
+        TF_HIP(hipEventCreate(&tm.sample[0]));
+        TF_HIP(hipEventCreate(&tm.sample[1]));
     }
     while (true) {
         if (graph) {
             const int key = kind * 16 + (int)sizeof(VT) * 2;
             const int gi = sizeof(VT) == sizeof(double) ? 0 : 1;
             hipGraphExec_t& graph_exec = h->krylov_graph[gi];
-            if (!graph_exec || h->krylov_graph_iters[gi] != chunk || h->krylov_graph_kind[gi] != key) {
+            if (!graph_exec || h->krylov_graph_iters[gi] != g_it || h->krylov_graph_kind[gi] != key) {
                 if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
                 hipGraph_t gr;
                 TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                h->t_spmv.capturing = true;
-                enqueue_chunk<VT>(h, val, kind, chunk);
-                h->t_spmv.capturing = false;
+                tm.capturing = true;
+                try {
+                    enqueue_chunk<VT>(h, val, kind, g_it);
+                } catch (...) {
+                    tm.capturing = false;
+                    (void)hipStreamEndCapture(h->stream, &gr);
+                    throw;
+                }
+                tm.capturing = false;
                 TF_HIP(hipStreamEndCapture(h->stream, &gr));
                 TF_HIP(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
                 TF_HIP(hipGraphDestroy(gr));
-                h->krylov_graph_iters[gi] = chunk;
+                h->krylov_graph_iters[gi] = g_it;
                 h->krylov_graph_kind[gi] = key;
             }
             if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[0], h->stream));
-            for (int k = 0; k < reps; ++k) TF_HIP(hipGraphLaunch(graph_exec, h->stream));
+            for (int k = 0; k < reps * per_chunk; ++k) TF_HIP(hipGraphLaunch(graph_exec, h->stream));
             if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[1], h->stream));
         } else {
             enqueue_chunk<VT>(h, val, kind, chunk);
@@ -1981,9 +2004,9 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         const bool fin = all_done(h);
         if (graph && timing) {
             float ms = 0.f;
-            TF_HIP(hipEventElapsedTime(&ms, h->t_spmv.sample[0], h->t_spmv.sample[1]));
-            h->t_spmv.total_ms += ms;
-            h->t_spmv.launches += launched;
+            TF_HIP(hipEventElapsedTime(&ms, tm.sample[0], tm.sample[1]));
+            tm.replay_ms += ms;
+            tm.replay_launches += launched;
         }
         if (fin) break;
         if (done_iters > h->opts.krylov_max_iter + 2 * chunk) break;  // the cap marks the tissue stalled
@@ -2005,29 +2028,153 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
 
 }  // namespace
 
-// Average device time (ms) of one single-kernel CG iteration over n_iters iterations on the
-// current Jacobian, under the current tiling (tolerance 0; no fallback).
-double krylov_time_cg1(tf_handle* h, int n_iters)
+bool twolevel_active(const tf_handle* h)
+{
+    int kind = h->opts.krylov;
+    if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+    const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
+    const int pc = h->opts.preconditioner;
+    const bool want = pc == TF_PRECOND_TWO_LEVEL || (pc == TF_PRECOND_AUTO && h->N >= TF_TWO_LEVEL_MIN_NODES);
+    return cg_kind && want && h->S == 1;
+}
+
+// The coarse space follows the pattern and tiling; the coarse operator is rebuilt at the first
+// Newton iteration of every coarse_refresh-th timestep and for every solve outside the Newton
+// loop.  A coarse operator from an earlier Jacobian is still symmetric positive definite, so it
+// changes the iteration count, not the solution.  False when the coarse operator is not
+// positive definite (block Jacobi then).
+static bool twolevel_prepare(tf_handle* h)
+{
+    if (!h->tl.built || h->tl.T != h->T) twolevel_build(h);
+    if (h->krylov_call == 0) h->tl.age += 1;
+    if (!h->tl.fresh || h->krylov_call < 0 ||
+        (h->krylov_call == 0 && h->tl.age >= std::max(1, h->opts.coarse_refresh)))
+        twolevel_refresh(h);
+    return !h->tl.failed;
+}
+
+// Average device time (ms) of one Krylov iteration over n_iters iterations on the current
+// Jacobian, under the current tiling (tolerance 0; no fallback): whole graph replays divided by
+// the iterations launched.  The two-level PCG when twolevel_active, else the single-kernel CG.
+double krylov_time_iteration(tf_handle* h, int n_iters)
 {
     const tf_solver_opts saved = h->opts;
+    const int kind = twolevel_active(h) && twolevel_prepare(h) ? KIND_TL : TF_KRYLOV_CG1;
     h->opts.krylov_rtol = 0.0;
     h->opts.krylov_atol = 0.0;
-    h->opts.krylov_max_iter = n_iters - 1;  // launch n_iters sees the cap: one replay
-    h->opts.iters_per_sync = n_iters;
-    const bool was = h->t_spmv.enabled;
-    h->t_spmv.enabled = true;
+    h->opts.krylov_max_iter = n_iters - 1;  // launch n_iters sees the cap
+    h->opts.iters_per_sync = std::min(n_iters, 32);  // the chunk the solves replay
+    h->opts.use_graph = 1;
+    KernelTimer& tm = h->t_spmv;
+    hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    TF_HIP(hipGetLastError());
+    const bool was = tm.enabled;
+    const double keep_ms = tm.total_ms;
+    const int64_t keep_n = tm.launches;
+    tm.enabled = true;
     double ms = 0.0;
+    const int call = h->krylov_call;
+    h->krylov_call = -1;
     for (int rep = 0; rep < 2; ++rep) {  // the first run captures the graph
-        h->t_spmv.total_ms = 0.0;
-        h->t_spmv.launches = 0;
-        if (h->opts.matrix_precision == TF_PREC_FP32) solve_kind<float>(h, h->val32, TF_KRYLOV_CG1);
-        else solve_kind<double>(h, h->val, TF_KRYLOV_CG1);
-        ms = h->t_spmv.launches ? h->t_spmv.total_ms / h->t_spmv.launches : 0.0;
+        tm.replay_ms = 0.0;
+        tm.replay_launches = 0;
+        if (h->opts.matrix_precision == TF_PREC_FP32) solve_kind<float>(h, h->val32, kind);
+        else solve_kind<double>(h, h->val, kind);
+        ms = tm.replay_launches ? tm.replay_ms / tm.replay_launches : 0.0;
     }
-    h->t_spmv.enabled = was;
-    h->t_spmv.total_ms = 0.0;
-    h->t_spmv.launches = 0;
+    h->krylov_call = call;
+    tm.enabled = was;
+    tm.total_ms = keep_ms;
+    tm.launches = keep_n;
     h->opts = saved;
+    return ms;
+}
+
+// Back-to-back launches of the dominant Krylov kernel on the current Jacobian, timed whole by
+// events around a graph replay.  A short solve with tolerance 0 first leaves consistent vectors
+// and partials; the tissues' flags are then reopened.  The single-kernel and pipelined CG advance
+// their iterations (the launches are the solver's own); k_cg_q (two-level and textbook CG) is
+// launched with one parity, so every launch repeats the same direction update and SpMV.
+double krylov_probe(tf_handle* h, int n_launches)
+{
+    int kind = h->opts.krylov;
+    if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+    const bool tl = twolevel_active(h);
+    if (kind == TF_KRYLOV_BICGSTAB && !tl) return 0.0;
+    if (kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG) {
+        if (!tl) {
+            const tf_solver_opts saved = h->opts;
+            h->opts.krylov = kind;
+            h->opts.preconditioner = TF_PRECOND_BLOCK_JACOBI;
+            double ms = 0.0;
+            try {
+                ms = krylov_time_iteration(h, n_launches);
+            } catch (...) {
+                h->opts = saved;
+                throw;
+            }
+            h->opts = saved;
+            return ms;
+        }
+    }
+    // k_cg_q: set-up solve of 8 iterations (two-level when active, else textbook CG) on every
+    // tissue (a finished timestep leaves them inactive)
+    hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    TF_HIP(hipGetLastError());
+    const tf_solver_opts saved = h->opts;
+    const int sk = tl && twolevel_prepare(h) ? KIND_TL : TF_KRYLOV_CG;
+    h->opts.krylov_rtol = 0.0;
+    h->opts.krylov_atol = 0.0;
+    h->opts.krylov_max_iter = 8;  // the ninth iteration sees the cap: the solve stops there
+    h->opts.iters_per_sync = 8;
+    const bool was = h->t_spmv.enabled;
+    h->t_spmv.enabled = false;
+    const int call = h->krylov_call;
+    h->krylov_call = -1;
+    double ms = 0.0;
+    try {
+        if (h->opts.matrix_precision == TF_PREC_FP32) solve_kind<float>(h, h->val32, sk);
+        else solve_kind<double>(h, h->val, sk);
+    } catch (...) {
+        h->opts = saved; h->t_spmv.enabled = was; h->krylov_call = call;
+        throw;
+    }
+    h->opts = saved;
+    h->t_spmv.enabled = was;
+    h->krylov_call = call;
+    TileArgs ta = tile_args(h);
+    hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    TF_HIP(hipGetLastError());
+    hipGraph_t gr;
+    hipGraphExec_t ex = nullptr;
+    const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
+    TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < n_launches; ++k) {
+        if (h->opts.matrix_precision == TF_PREC_FP32)
+            hipLaunchKernelGGL((k_cg_q<float, 0>), dim3(h->T), dim3(WG), lds, h->stream, ta, h->row_ptr, h->col, h->val32,
+                               h->nb, h->ks, h->kp, h->kph, h->kv, 1 << 30);
+        else
+            hipLaunchKernelGGL((k_cg_q<double, 0>), dim3(h->T), dim3(WG), lds, h->stream, ta, h->row_ptr, h->col, h->val,
+                               h->nb, h->ks, h->kp, h->kph, h->kv, 1 << 30);
+    }
+    TF_HIP(hipStreamEndCapture(h->stream, &gr));
+    TF_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+    TF_HIP(hipGraphDestroy(gr));
+    hipEvent_t ev[2];
+    TF_HIP(hipEventCreate(&ev[0]));
+    TF_HIP(hipEventCreate(&ev[1]));
+    for (int rep = 0; rep < 2; ++rep) {  // the first replay warms the caches
+        TF_HIP(hipEventRecord(ev[0], h->stream));
+        TF_HIP(hipGraphLaunch(ex, h->stream));
+        TF_HIP(hipEventRecord(ev[1], h->stream));
+        TF_HIP(hipEventSynchronize(ev[1]));
+        float e = 0.f;
+        TF_HIP(hipEventElapsedTime(&e, ev[0], ev[1]));
+        ms = e / n_launches;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    (void)hipGraphExecDestroy(ex);
     return ms;
 }
 
@@ -2038,16 +2185,9 @@ int krylov_solve(tf_handle* h)
     const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
     auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
-    if (cg_kind && h->S == 1 && h->opts.preconditioner == TF_PRECOND_TWO_LEVEL) {
-        // the coarse space follows the pattern and tiling; the coarse operator is rebuilt at the
-        // first Newton iteration of every coarse_refresh-th timestep (and for every solve outside
-        // the Newton loop).  A coarse operator from an earlier Jacobian is still symmetric positive
-        // definite, so it changes the iteration count, not the solution.
-        if (!h->tl.built || h->tl.T != h->T) twolevel_build(h);
-        if (h->krylov_call == 0) h->tl.age += 1;
-        if (!h->tl.fresh || h->krylov_call < 0 || (h->krylov_call == 0 && h->tl.age >= std::max(1, h->opts.coarse_refresh)))
-            twolevel_refresh(h);
-        if (!h->tl.failed) kind = KIND_TL;
+    if (cg_kind && twolevel_active(h) && twolevel_prepare(h)) {
+        kind = KIND_TL;
+        h->tl_used = true;
     }
     int st = run(kind);
     if (st == TF_KRYLOV_STALL && (cg_kind || kind == KIND_TL)) {

@@ -350,3 +350,10 @@ class NewtonRaphsonSolver:
         b = C.c_double()
         self._check(self._lib.tf_kernel_bytes(self.h, name.encode(), C.byref(b)))
         return b.value
+
+    def kernel_probe(self, n_launches=64):
+        """Average device ms of one launch of the dominant Krylov kernel, n_launches back to back
+        on the current Jacobian (HIP events around one graph replay)."""
+        ms = C.c_double()
+        self._check(self._lib.tf_kernel_probe(self.h, n_launches, C.byref(ms)))
+        return ms.value
