@@ -383,48 +383,88 @@ __global__ __launch_bounds__(WG) void k_cg_init(TileArgs ta, const double* __res
     tile_write<3>(ta, t, sl, vv);
 }
 
-// p = z + beta p_old (own rows written to p_new, neighbours formed on the fly), q = M p, (p, q)
-template <typename VT, int P>
-__global__ __launch_bounds__(WG) void k_cg_q(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, int nb, const double* __restrict__ z,
-                                             const double* __restrict__ p_old, double* __restrict__ p_new,
-                                             double* __restrict__ q, int maxit)
+// p = z + beta p_old (own rows written to p_new, neighbours formed on the fly), q = M p, (p, q).
+// Each lane loads its first PF matrix blocks, with z and p_old of their columns, before the tile
+// re-sums the partials for the convergence test and beta (one reduction for the three slots).
+template <typename VT, int P, int WGT>
+__global__ __launch_bounds__(WGT) void k_cg_q(TileArgs ta, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                                              const VT* __restrict__ val, int nb, const double* __restrict__ z,
+                                              const double* __restrict__ p_old, double* __restrict__ p_new,
+                                              double* __restrict__ q, int maxit)
 {
+    constexpr int PF = WGT >= 1024 ? 1 : 2;
     extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
-    const int t = blockIdx.x;
+    const int t = xcd_tile(blockIdx.x, ta.T, ta.nxcd);
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    if (check_stop(ta, t, sg, SL_RR0 + P, maxit)) return;
-    double beta = 0.0;
-    if (S.iters > 0) {
-        const int sl[2] = {SL_RZ0 + P, SL_RZ0 + (1 - P)};
-        double rz[2];
-        seg_sums<2>(ta, sg, sl, rz);
-        beta = rz[0] / rz[1];
-    }
     const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
     const int b0 = row_ptr[r0], b1 = row_ptr[r1];
-    tile_products<VT>(col, val, nb, b0, b1,
-                      [&](int c, double& a0, double& a1, double& a2) {
-                          a0 = z[3 * c] + beta * p_old[3 * c];
-                          a1 = z[3 * c + 1] + beta * p_old[3 * c + 1];
-                          a2 = z[3 * c + 2] + beta * p_old[3 * c + 2];
-                      },
-                      prod);
+    double pv[PF][9], pz[PF][3], pp[PF][3];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const int j = b0 + (int)threadIdx.x + k * WGT;
+        const int c = j < b1 ? col[j] : 0;
+        const int jj = j < b1 ? j : b0;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) pv[k][e] = val[(size_t)e * nb + jj];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) { pz[k][e] = z[3 * c + e]; pp[k][e] = p_old[3 * c + e]; }
+    }
+    const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
+    double sc[3];
+    seg_sums<3, WGT>(ta, sg, sl, sc);
+    {  // the convergence / cap test of check_stop
+        const bool conv = sc[0] <= S.tol2;
+        const bool cap = !conv && (S.iters >= maxit || !isfinite(sc[0]));
+        if (conv || cap) {
+            if (first_tile(ta, t, sg) && threadIdx.x == 0) {
+                S.rnorm2 = sc[0];
+                if (conv) S.converged = 1;
+                else S.stalled = 1;
+            }
+            return;
+        }
+    }
+    const double beta = S.iters > 0 ? sc[1] / sc[2] : 0.0;
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const int j = b0 + (int)threadIdx.x + k * WGT;
+        if (j < b1) {
+            const double x0 = pz[k][0] + beta * pp[k][0], x1 = pz[k][1] + beta * pp[k][1], x2 = pz[k][2] + beta * pp[k][2];
+            double* pr = prod + 3 * (j - b0);
+            pr[0] = pv[k][0] * x0 + pv[k][1] * x1 + pv[k][2] * x2;
+            pr[1] = pv[k][3] * x0 + pv[k][4] * x1 + pv[k][5] * x2;
+            pr[2] = pv[k][6] * x0 + pv[k][7] * x1 + pv[k][8] * x2;
+        }
+    }
+    for (int j = b0 + (int)threadIdx.x + PF * WGT; j < b1; j += WGT) {
+        const int c = col[j];
+        const double x0 = z[3 * c] + beta * p_old[3 * c];
+        const double x1 = z[3 * c + 1] + beta * p_old[3 * c + 1];
+        const double x2 = z[3 * c + 2] + beta * p_old[3 * c + 2];
+        const double v0 = val[j], v1 = val[(size_t)nb + j], v2 = val[2 * (size_t)nb + j];
+        const double v3 = val[3 * (size_t)nb + j], v4 = val[4 * (size_t)nb + j], v5 = val[5 * (size_t)nb + j];
+        const double v6 = val[6 * (size_t)nb + j], v7 = val[7 * (size_t)nb + j], v8 = val[8 * (size_t)nb + j];
+        double* pr = prod + 3 * (j - b0);
+        pr[0] = v0 * x0 + v1 * x1 + v2 * x2;
+        pr[1] = v3 * x0 + v4 * x1 + v5 * x2;
+        pr[2] = v6 * x0 + v7 * x1 + v8 * x2;
+    }
+    __syncthreads();
     double s0 = 0.0;
-    for (int task = threadIdx.x; task < 3 * (r1 - r0); task += WG) {
+    for (int task = threadIdx.x; task < 3 * (r1 - r0); task += WGT) {
         const int r = r0 + task / 3, comp = task % 3;
         const int d = 3 * r + comp;
         const double qv = row_sum(row_ptr, r, comp, b0, prod);
-        const double pv = z[d] + beta * p_old[d];
+        const double pv_ = z[d] + beta * p_old[d];
         q[d] = qv;
-        p_new[d] = pv;
-        s0 += pv * qv;
+        p_new[d] = pv_;
+        s0 += pv_ * qv;
     }
-    const int sl[1] = {SL_PQ};
+    const int so[1] = {SL_PQ};
     double vv[1] = {s0};
-    tile_write<1>(ta, t, sl, vv);
+    tile_write<1, WGT>(ta, t, so, vv);
 }
 
 template <int P>
@@ -1286,14 +1326,15 @@ __global__ __launch_bounds__(WG) void k_warm_dots(TileArgs ta, WarmArgs wa, doub
     }
 }
 
-// per tissue: the tile partials summed in tile order
+// per tissue and dot product (one wave each): the tile partials, lane l summing tiles l, l + 64,
+// ... in order, then a fixed tree over the lanes
 __global__ __launch_bounds__(64) void k_warm_seg(TileArgs ta, const double* __restrict__ part, double* __restrict__ out)
 {
-    const int sg = blockIdx.x, q = threadIdx.x;
-    if (q >= TF_WARM_DOTS) return;
+    const int sg = blockIdx.x, q = blockIdx.y;
     double sv = 0.0;
-    for (int t = ta.seg_tile_ptr[sg]; t < ta.seg_tile_ptr[sg + 1]; ++t) sv += part[(size_t)q * ta.T + t];
-    out[(size_t)sg * TF_WARM_DOTS + q] = sv;
+    for (int t = ta.seg_tile_ptr[sg] + threadIdx.x; t < ta.seg_tile_ptr[sg + 1]; t += 64) sv += part[(size_t)q * ta.T + t];
+    sv = wave_sum(sv);
+    if (threadIdx.x == 0) out[(size_t)sg * TF_WARM_DOTS + q] = sv;
 }
 
 // x0 = sum_j y[tissue][j] z_j on every row of the tile
@@ -1408,7 +1449,7 @@ bool warm_start_guess(tf_handle* h, int k)
     wa.g = h->g;
     wa.m = m;
     hipLaunchKernelGGL(k_warm_dots, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_part);
-    hipLaunchKernelGGL(k_warm_seg, dim3(h->S), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
+    hipLaunchKernelGGL(k_warm_seg, dim3(h->S, TF_WARM_DOTS), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
     TF_HIP(hipGetLastError());
     std::vector<double> dots((size_t)TF_WARM_DOTS * h->S), y((size_t)TF_WARM_HIST * h->S, 0.0);
     TF_HIP(hipMemcpyAsync(dots.data(), h->warm_seg, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, h->stream));
@@ -1610,7 +1651,9 @@ static void twolevel_build(tf_handle* h)
     TF_HIP(hipStreamSynchronize(h->stream));
     for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
     const int max_nc = std::min(4096, h->max_lds / (int)sizeof(double));
-    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, N / 190);
+    // default: one aggregate per 88 nodes, at most 340 (about 2,000 coarse unknowns: the dense FP32
+    // inverse every iteration reads stays near 16 MB)
+    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(340, N / 88));
     target = std::min(target, max_nc / 6);
     int A = 0;
     std::vector<int32_t> agg = tl_aggregates(x, N, target, A);
@@ -1786,13 +1829,33 @@ void enqueue_cg_iter(tf_handle* h, const VT* val)
     double* p_new = P ? h->kp : h->kph;
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_cg_q<VT, P>), grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr, h->col, val, h->nb, h->ks, p_old, p_new,
-                       h->kv, h->opts.krylov_max_iter);
+    hipLaunchKernelGGL((k_cg_q<VT, P, WG>), grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr,
+                       h->col, val, h->nb, h->ks, p_old, p_new, h->kv, h->opts.krylov_max_iter);
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL(k_cg_x<P>, grid, blk, 0, h->stream, ta, h->dinv, p_new, h->kv, h->kx, h->kr, h->ks);
 }
 
 constexpr int KIND_TL = 64;  // two-level PCG (internal solver kind)
+
+// k_cg_q at workgroup size wg (the tiling's), CG buffers: z = ks, p ping-pong kp / kph, q = kv
+template <typename VT, int P>
+void launch_cg_q(tf_handle* h, const VT* val, int wg, int maxit = -1)
+{
+    TileArgs ta = tile_args(h);
+    double* p_old = P ? h->kph : h->kp;
+    double* p_new = P ? h->kp : h->kph;
+    const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
+    if (maxit < 0) maxit = h->opts.krylov_max_iter;
+    if (wg == 1024)
+        hipLaunchKernelGGL((k_cg_q<VT, P, 1024>), dim3(h->T), dim3(1024), lds, h->stream, ta, h->row_ptr, h->col, val, h->nb,
+                           h->ks, p_old, p_new, h->kv, maxit);
+    else if (wg == 512)
+        hipLaunchKernelGGL((k_cg_q<VT, P, 512>), dim3(h->T), dim3(512), lds, h->stream, ta, h->row_ptr, h->col, val, h->nb,
+                           h->ks, p_old, p_new, h->kv, maxit);
+    else
+        hipLaunchKernelGGL((k_cg_q<VT, P, 256>), dim3(h->T), dim3(256), lds, h->stream, ta, h->row_ptr, h->col, val, h->nb,
+                           h->ks, p_old, p_new, h->kv, maxit);
+}
 
 // Two-level PCG: the CG buffers of enqueue_cg_iter; z = ks holds D^-1 r + P Ac^-1 P^T r.
 template <typename VT, int P>
@@ -1805,8 +1868,7 @@ void enqueue_tl_iter(tf_handle* h, const VT* val)
     double* p_new = P ? h->kp : h->kph;
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_cg_q<VT, P>), grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta, h->row_ptr,
-                       h->col, val, h->nb, h->ks, p_old, p_new, h->kv, h->opts.krylov_max_iter);
+    launch_cg_q<VT, P>(h, val, h->wg);
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL((k_tl_xr<P, false>), dim3(tla.n_chunk), dim3(64), 0, h->stream, ta, tla, h->dinv, p_new, h->kv,
                        h->kx, h->kr);
@@ -2142,20 +2204,15 @@ double krylov_probe(tf_handle* h, int n_launches)
     h->opts = saved;
     h->t_spmv.enabled = was;
     h->krylov_call = call;
-    TileArgs ta = tile_args(h);
     hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     TF_HIP(hipGetLastError());
     hipGraph_t gr;
     hipGraphExec_t ex = nullptr;
-    const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
     TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int wg = sk == KIND_TL ? h->wg : WG;  // as the solver launches it
     for (int k = 0; k < n_launches; ++k) {
-        if (h->opts.matrix_precision == TF_PREC_FP32)
-            hipLaunchKernelGGL((k_cg_q<float, 0>), dim3(h->T), dim3(WG), lds, h->stream, ta, h->row_ptr, h->col, h->val32,
-                               h->nb, h->ks, h->kp, h->kph, h->kv, 1 << 30);
-        else
-            hipLaunchKernelGGL((k_cg_q<double, 0>), dim3(h->T), dim3(WG), lds, h->stream, ta, h->row_ptr, h->col, h->val,
-                               h->nb, h->ks, h->kp, h->kph, h->kv, 1 << 30);
+        if (h->opts.matrix_precision == TF_PREC_FP32) launch_cg_q<float, 0>(h, h->val32, wg, 1 << 30);
+        else launch_cg_q<double, 0>(h, h->val, wg, 1 << 30);
     }
     TF_HIP(hipStreamEndCapture(h->stream, &gr));
     TF_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
