@@ -246,6 +246,8 @@ def main():
                                                   for k in range(int(max(nr)) if len(nr) else 0) if k < 4],
         "converged_steps": int(sum(st["converged"] for st in stats)),
         "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in stats)),
+        "ms_per_step_by_phase": {k: float(np.mean([st[k] for st in stats]))
+                                 for k in ("ms_grow", "ms_assemble", "ms_solve", "ms_total")},
         "packing": {"pairs_per_step_max": int(max(st["packing_pairs"] for st in stats)),
                     "ms_growth_surfaces_detection_per_step": float(np.mean([st["ms_grow"] for st in stats]))},
         "roofline": {"kernel": KERNEL_NAMES[1] if two_level else KERNEL_NAMES.get(a.krylov, KERNEL_NAMES[-1]),

@@ -61,3 +61,26 @@ def test_xcd_tile_map_is_bitwise_neutral():
         a.runOneStep(t.dt)
         b.runOneStep(t.dt)
     assert np.array_equal(a.positions, b.positions)
+
+
+@pytest.mark.parametrize("prec", [1, 0])  # two-level and block Jacobi
+def test_inverted_prism_stops_the_step(prec):
+    """A prism turned inside out (det Fe <= 0 at its Gauss points) ends the NR loop with
+    TF_NEG_JACOBIAN before any update, with or without a warm start."""
+    from tissuefolding_paperversion_amd import _lib as L
+    t = _ellipse()
+    s = _solver(t, preconditioner=prec)
+    s.runOneStep(t.dt)
+    s.runOneStep(t.dt)  # the warm-start history is filled
+    x = s.positions.reshape(-1, 3).copy()
+    conn = t.mesh.conn
+    e = 0
+    top, bottom = conn[e, 3:], conn[e, :3]
+    x[top, 2] = x[bottom, 2].min() - 5.0  # apical triangle pushed below the basal one
+    s.positions = x.ravel()
+    before = s.positions
+    with pytest.raises(L.TfError) as err:
+        s.runOneStep(t.dt)
+    assert err.value.code == L.TF_NEG_JACOBIAN
+    assert "det Fe" in str(err.value)
+    assert np.array_equal(s.positions, before)

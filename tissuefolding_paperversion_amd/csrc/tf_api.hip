@@ -89,6 +89,7 @@ void free_all(tf_handle* h)
     twolevel_free(h);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
+    if (h->h_status) (void)hipHostFree(h->h_status);
     drop_krylov_graphs(h);
     for (KernelTimer* t : {&h->t_elem, &h->t_gather, &h->t_spmv}) {
         for (hipEvent_t e : t->pending) (void)hipEventDestroy(e);
@@ -576,6 +577,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg_rtol2 = dev_alloc<double>(S);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
+        TF_HIP(hipHostMalloc((void**)&h->h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
         for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx, &h->km, &h->kq})
             *v = dev_alloc<double>((size_t)3 * N);
 
@@ -1051,20 +1053,22 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     while (true) {
         auto ta = std::chrono::steady_clock::now();
         assemble_impl(h, dt);
-        int32_t st[4];
-        d2h(h, st, h->status, 4);
-        TF_HIP(hipStreamSynchronize(h->stream));
+        // the det Fe status comes back with the warm start's own read-back (one host wait)
+        TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
         ms_asm += ms_since(ta);
-        if (st[0] > 0) {
-            status = TF_NEG_JACOBIAN;
-            h->err = std::to_string(st[0]) + " Gauss points with det Fe <= 0";
-            break;
-        }
         auto ts = std::chrono::steady_clock::now();
         h->seg_rtol_on = h->opts.krylov_forcing != 0;
         if (h->seg_rtol_on) forcing_tolerances(h, it);
         const bool warm = h->opts.krylov_warm_start != 0 && it < TF_WARM_ITERS;
-        h->warm_guess = (warm && warm_start_guess(h, it)) ? h->warm_x0 : nullptr;
+        h->warm_guess = (warm && warm_start_guess(h, it)) ? h->warm_x0 : nullptr;  // waits when it reads back
+        TF_HIP(hipStreamSynchronize(h->stream));  // returns at once after a warm start's read-back
+        if (h->h_status[0] > 0) {
+            h->warm_guess = nullptr;
+            h->seg_rtol_on = false;
+            status = TF_NEG_JACOBIAN;
+            h->err = std::to_string(h->h_status[0]) + " Gauss points with det Fe <= 0";
+            break;
+        }
         int kst;
         h->krylov_call = it;
         try {

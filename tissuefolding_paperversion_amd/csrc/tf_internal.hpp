@@ -190,6 +190,7 @@ struct tf_handle {
     SegScalars *h_seg = nullptr;      // pinned host mirror
     double *seg_norm = nullptr;       // [S] per-segment ||dU||
     double *h_seg_norm = nullptr;     // pinned
+    int32_t *h_status = nullptr;      // pinned copy of status
 
     // Krylov work vectors [3N]
     double *kr = nullptr, *kr0 = nullptr, *kp = nullptr, *kph = nullptr, *kv = nullptr, *ks = nullptr,
@@ -242,8 +243,8 @@ struct tf_handle {
         int32_t n_agg = 0, nc = 0, n_pairs = 0, G = 0, T = -1;
         int32_t age = 0;                                    // timesteps since the coarse operator was built
         int32_t *node_agg = nullptr, *node_coff = nullptr;  // [N] aggregate, its first coarse unknown
-        int32_t *agg_nodes = nullptr;                       // [N] nodes of every aggregate, in chunks
-        int32_t *chunk_ptr = nullptr, *agg_chunk = nullptr; // [C+1] chunk bounds, [A+1] chunks of every aggregate
+        int32_t *chunk_nodes = nullptr;                     // [C][64] nodes of every chunk, -1 padded
+        int32_t *agg_chunk = nullptr;                       // [A+1] chunks of every aggregate
         int32_t n_chunk = 0;
         double* pcoef = nullptr;                            // [N][3][6] rows of P
         int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank

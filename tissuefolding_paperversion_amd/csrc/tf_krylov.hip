@@ -519,9 +519,8 @@ constexpr int TL_PREF = 32;   // entries per lane of a coarse row prefetched bef
 
 struct TLArgs {
     const int32_t* node_coff;  // [N] first coarse unknown of the node's aggregate
-    const int32_t* agg_nodes;  // [N] nodes of every aggregate (ascending), aggregate after aggregate
-    const int32_t* chunk_ptr;  // [C+1] chunks of at most TL_CHUNK nodes of one aggregate in agg_nodes
-    const int32_t* agg_chunk;  // [A+1] chunks of every aggregate
+    const int32_t* chunk_nodes;  // [C][TL_CHUNK] nodes of every chunk (one aggregate's, ascending), -1 past its end
+    const int32_t* agg_chunk;    // [A+1] chunks of every aggregate
     const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
@@ -541,9 +540,9 @@ __global__ __launch_bounds__(64) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
     SegScalars& S = ta.seg[0];
     if (seg_done(S)) return;
     const int c = blockIdx.x;
-    const int k = tl.chunk_ptr[c] + threadIdx.x;
-    const bool own = k < tl.chunk_ptr[c + 1];
-    const int n = own ? tl.agg_nodes[k] : 0;
+    const int nk = tl.chunk_nodes[(size_t)c * TL_CHUNK + threadIdx.x];
+    const bool own = nk >= 0;
+    const int n = own ? nk : 0;
     double rv[3], xv[3], pv[3], qv[3], D[9], Pn[18];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -1522,7 +1521,7 @@ void tl_upload(tf_handle* h, T*& dst, const std::vector<T>& v)
 void tl_release(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.agg_nodes, &tl.chunk_ptr, &tl.agg_chunk, &tl.coff, &tl.krank, &tl.pair_a,
+    for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.chunk_nodes, &tl.agg_chunk, &tl.coff, &tl.krank, &tl.pair_a,
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (double** p : {&tl.pcoef, &tl.chunk_part, &tl.Ac, &tl.zc})
@@ -1667,18 +1666,14 @@ static void twolevel_build(tf_handle* h)
     }
     const int nc = coff[A];
     if (nc > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");
-    std::vector<int32_t> agg_nodes, chunk_ptr(1, 0), agg_chunk(A + 1, 0);
-    agg_nodes.reserve(N);
+    std::vector<int32_t> chunk_nodes, agg_chunk(A + 1, 0);
     for (int a = 0; a < A; ++a) {
         const int m = (int)members[a].size();
-        for (int k0 = 0; k0 < m; k0 += TL_CHUNK) {
-            const int k1 = std::min(m, k0 + TL_CHUNK);
-            agg_nodes.insert(agg_nodes.end(), members[a].begin() + k0, members[a].begin() + k1);
-            chunk_ptr.push_back((int32_t)agg_nodes.size());
-        }
-        agg_chunk[a + 1] = (int32_t)chunk_ptr.size() - 1;
+        for (int k0 = 0; k0 < m; k0 += TL_CHUNK)
+            for (int k = k0; k < k0 + TL_CHUNK; ++k) chunk_nodes.push_back(k < m ? members[a][k] : -1);
+        agg_chunk[a + 1] = (int32_t)(chunk_nodes.size() / TL_CHUNK);
     }
-    const int C = (int)chunk_ptr.size() - 1;
+    const int C = (int)(chunk_nodes.size() / TL_CHUNK);
     // aggregate pairs with their Jacobian blocks in block order
     const std::vector<int32_t>& rp = h->h_row_ptr;
     const std::vector<int32_t>& cl = h->h_col;
@@ -1712,8 +1707,7 @@ static void twolevel_build(tf_handle* h)
         tl_upload(h, tl.node_coff, node_coff);
         TF_HIP(hipStreamSynchronize(h->stream));
     }
-    tl_upload(h, tl.agg_nodes, agg_nodes);
-    tl_upload(h, tl.chunk_ptr, chunk_ptr);
+    tl_upload(h, tl.chunk_nodes, chunk_nodes);
     tl_upload(h, tl.agg_chunk, agg_chunk);
     tl_upload(h, tl.pcoef, pcoef);
     tl_upload(h, tl.coff, coff);
@@ -1753,7 +1747,7 @@ static void twolevel_build(tf_handle* h)
 static TLArgs tl_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    return TLArgs{tl.node_coff, tl.agg_nodes, tl.chunk_ptr, tl.agg_chunk, tl.pcoef, tl.coff, tl.krank, tl.chunk_part,
+    return TLArgs{tl.node_coff, tl.chunk_nodes, tl.agg_chunk, tl.pcoef, tl.coff, tl.krank, tl.chunk_part,
                   tl.Ainv, tl.zc, tl.nc, tl.n_agg, tl.n_chunk, tl.G, tl.rows};
 }
 
