@@ -59,7 +59,7 @@ def parse():
     p.add_argument("--no-graph", action="store_true")
     p.add_argument("--preconditioner", default="auto", choices=["auto", "block-jacobi", "two-level"])
     p.add_argument("--coarse-aggregates", type=int, default=0)
-    p.add_argument("--coarse-refresh", type=int, default=25)
+    p.add_argument("--coarse-refresh", type=int, default=100)
     p.add_argument("--no-kernel-timing", action="store_true", help="skip the HIP-event timing of the dominant kernel")
     return p.parse_args()
 

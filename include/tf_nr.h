@@ -70,8 +70,9 @@ enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
  * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi plus an additive coarse
  * correction P (P^T M P)^-1 P^T r, P the rigid-body modes of node aggregates (square cells of
  * the tissue's x-y extent, whole layer columns each).  The coarse operator is rebuilt at the
- * first Newton iteration of every coarse_refresh-th timestep and for every solve outside the
- * Newton loop; handles with several tissues use block Jacobi.  The coarse inverse is held in FP32
+ * first Newton iteration of a timestep once Newton iteration 1's Krylov count exceeds 1.25x its
+ * count right after the last rebuild, or coarse_refresh timesteps after it, and for every solve
+ * outside the Newton loop; handles with several tissues use block Jacobi.  The coarse inverse is held in FP32
  * (it only preconditions: residuals, matrix and Krylov vectors stay in the solver's precision). */
 enum { TF_PRECOND_AUTO = -1, TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
 
@@ -100,8 +101,8 @@ typedef struct tf_solver_opts {
     int32_t preconditioner;    /* TF_PRECOND_*: the preconditioner of the CG solvers             */
     int32_t coarse_aggregates; /* TF_PRECOND_TWO_LEVEL: target number of node aggregates of the
                                   coarse space (0: one per 88 nodes, at most 340)               */
-    int32_t coarse_refresh;    /* TF_PRECOND_TWO_LEVEL: timesteps between rebuilds of the coarse
-                                  operator (0 or 1: every timestep; default 25)                 */
+    int32_t coarse_refresh;    /* TF_PRECOND_TWO_LEVEL: most timesteps between rebuilds of the
+                                  coarse operator (0 or 1: every timestep; default 100)         */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

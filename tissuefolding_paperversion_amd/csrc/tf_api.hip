@@ -477,7 +477,7 @@ void tf_default_options(tf_solver_opts* o)
     o->krylov_warm_start = 1;
     o->implicit_packing = 1;
     o->preconditioner = TF_PRECOND_AUTO;
-    o->coarse_refresh = 25;
+    o->coarse_refresh = 100;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }

@@ -212,7 +212,6 @@ struct tf_handle {
     // the solve).  Work arrays: M z_j (warm_mz), per-tile and per-tissue Gram entries, coefficients.
     double* warm_hist[TF_WARM_ITERS][TF_WARM_HIST] = {};
     int32_t warm_count[TF_WARM_ITERS] = {};
-    double* warm_mz[TF_WARM_HIST] = {};
     double* warm_x0 = nullptr;
     double* warm_part = nullptr;      // [TF_WARM_DOTS][T]
     int32_t warm_part_T = 0;
@@ -242,6 +241,8 @@ struct tf_handle {
         bool built = false, fresh = false, failed = false;
         int32_t n_agg = 0, nc = 0, n_pairs = 0, G = 0, T = -1;
         int32_t age = 0;                                    // timesteps since the coarse operator was built
+        int32_t base_it1 = -1;                              // Krylov iterations of NR iteration 1 after the build
+        bool stale = false;                                 // a later NR iteration 1 took > 1.25x base_it1
         int32_t *node_agg = nullptr, *node_coff = nullptr;  // [N] aggregate, its first coarse unknown
         int32_t *chunk_nodes = nullptr;                     // [C][64] nodes of every chunk, -1 padded
         int32_t *agg_chunk = nullptr;                       // [A+1] chunks of every aggregate

@@ -1281,32 +1281,56 @@ TileArgs tile_args(tf_handle* h)
 
 // ---- warm start: least-squares combination of earlier updates ---------------------------------
 struct WarmArgs {
-    const double* mz[TF_WARM_HIST];
     const double* z[TF_WARM_HIST];
     const double* g;
     int m;
 };
 
-// per tile: (Mz_i, Mz_j) for i <= j < m in row-major upper-triangle order, then (Mz_i, g)
-__global__ __launch_bounds__(WG) void k_warm_dots(TileArgs ta, WarmArgs wa, double* __restrict__ part)
+// per tile, in one launch: M z_i on the tile's rows for every kept update z_i (one CSR-stream
+// pass per update; the row results stay in registers, at most 4 rows x components per lane),
+// then the partials of (Mz_i, Mz_j) for i <= j in row-major upper-triangle order, then (Mz_i, g)
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_warm_gram(TileArgs ta, WarmArgs wa, const int32_t* __restrict__ row_ptr,
+                                                  const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
+                                                  double* __restrict__ part)
 {
+    constexpr int MAXT = 4;  // tasks per lane: 3 (rows of a tile) <= 1024
+    extern __shared__ double prod[];
     const int t = blockIdx.x;
-    const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
+    const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
+    const int b0 = row_ptr[r0], b1 = row_ptr[r1];
+    const int ntask = 3 * (r1 - r0);
+    double mz[TF_WARM_HIST][MAXT];
+#pragma unroll
+    for (int i = 0; i < TF_WARM_HIST; ++i) {
+#pragma unroll
+        for (int k = 0; k < MAXT; ++k) mz[i][k] = 0.0;
+        if (i < wa.m) {
+            const double* z = wa.z[i];
+            tile_products<VT>(col, val, nb, b0, b1, XLOAD(z), prod);
+#pragma unroll
+            for (int k = 0; k < MAXT; ++k) {
+                const int task = threadIdx.x + k * WG;
+                if (task < ntask) mz[i][k] = row_sum(row_ptr, r0 + task / 3, task % 3, b0, prod);
+            }
+            __syncthreads();  // prod is reused by the next update
+        }
+    }
     double acc[TF_WARM_DOTS];
 #pragma unroll
     for (int q = 0; q < TF_WARM_DOTS; ++q) acc[q] = 0.0;
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
-        double v[TF_WARM_HIST];
 #pragma unroll
-        for (int i = 0; i < TF_WARM_HIST; ++i) v[i] = i < wa.m ? wa.mz[i][d] : 0.0;
-        const double gd = wa.g[d];
+    for (int k = 0; k < MAXT; ++k) {
+        const int task = threadIdx.x + k * WG;
+        if (task >= ntask) continue;
+        const double gd = wa.g[3 * r0 + task];
         int q = 0;
 #pragma unroll
         for (int i = 0; i < TF_WARM_HIST; ++i)
 #pragma unroll
-            for (int j = i; j < TF_WARM_HIST; ++j) acc[q++] += v[i] * v[j];
+            for (int j = i; j < TF_WARM_HIST; ++j) acc[q++] += mz[i][k] * mz[j][k];
 #pragma unroll
-        for (int i = 0; i < TF_WARM_HIST; ++i) acc[q++] += v[i] * gd;
+        for (int i = 0; i < TF_WARM_HIST; ++i) acc[q++] += mz[i][k] * gd;
     }
     constexpr int NW = WG / 64;
     __shared__ double red[TF_WARM_DOTS][NW];
@@ -1428,7 +1452,6 @@ bool warm_start_guess(tf_handle* h, int k)
     if (m <= 0) return false;
     const size_t n3 = 3 * (size_t)h->N;
     if (!h->warm_x0) {
-        for (int j = 0; j < TF_WARM_HIST; ++j) TF_HIP(hipMalloc((void**)&h->warm_mz[j], n3 * sizeof(double)));
         TF_HIP(hipMalloc((void**)&h->warm_x0, n3 * sizeof(double)));
         TF_HIP(hipMalloc((void**)&h->warm_seg, sizeof(double) * TF_WARM_DOTS * h->S));
         TF_HIP(hipMalloc((void**)&h->warm_y, sizeof(double) * TF_WARM_HIST * h->S));
@@ -1438,16 +1461,18 @@ bool warm_start_guess(tf_handle* h, int k)
         TF_HIP(hipMalloc((void**)&h->warm_part, sizeof(double) * TF_WARM_DOTS * h->T));
         h->warm_part_T = h->T;
     }
-    for (int j = 0; j < m; ++j) launch_spmv(h, h->warm_hist[k][j], h->warm_mz[j]);
     TileArgs ta = tile_args(h);
     WarmArgs wa{};
-    for (int j = 0; j < m; ++j) {
-        wa.mz[j] = h->warm_mz[j];
-        wa.z[j] = h->warm_hist[k][j];
-    }
+    for (int j = 0; j < m; ++j) wa.z[j] = h->warm_hist[k][j];
     wa.g = h->g;
     wa.m = m;
-    hipLaunchKernelGGL(k_warm_dots, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_part);
+    const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
+    if (h->opts.matrix_precision == TF_PREC_FP32)
+        hipLaunchKernelGGL(k_warm_gram<float>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val32,
+                           h->nb, h->warm_part);
+    else
+        hipLaunchKernelGGL(k_warm_gram<double>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val,
+                           h->nb, h->warm_part);
     hipLaunchKernelGGL(k_warm_seg, dim3(h->S, TF_WARM_DOTS), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
     TF_HIP(hipGetLastError());
     std::vector<double> dots((size_t)TF_WARM_DOTS * h->S), y((size_t)TF_WARM_HIST * h->S, 0.0);
@@ -1492,8 +1517,6 @@ void warm_start_free(tf_handle* h)
     for (int k = 0; k < TF_WARM_ITERS; ++k)
         for (int j = 0; j < TF_WARM_HIST; ++j)
             if (h->warm_hist[k][j]) { (void)hipFree(h->warm_hist[k][j]); h->warm_hist[k][j] = nullptr; }
-    for (int j = 0; j < TF_WARM_HIST; ++j)
-        if (h->warm_mz[j]) { (void)hipFree(h->warm_mz[j]); h->warm_mz[j] = nullptr; }
     for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg, &h->warm_y})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     warm_start_clear(h);
@@ -1769,6 +1792,8 @@ static void twolevel_refresh(tf_handle* h)
     TF_HIP(hipStreamSynchronize(h->stream));
     tl.fresh = true;
     tl.age = 0;
+    tl.base_it1 = -1;
+    tl.stale = false;
     tl.failed = info != 0;
     if (tl.failed) return;
     if (rocsolver_dpotri(bh, rocblas_fill_lower, nc, tl.Ac, nc, tl.info) != rocblas_status_success)
@@ -2094,19 +2119,30 @@ bool twolevel_active(const tf_handle* h)
     return cg_kind && want && h->S == 1;
 }
 
-// The coarse space follows the pattern and tiling; the coarse operator is rebuilt at the first
-// Newton iteration of every coarse_refresh-th timestep and for every solve outside the Newton
-// loop.  A coarse operator from an earlier Jacobian is still symmetric positive definite, so it
-// changes the iteration count, not the solution.  False when the coarse operator is not
-// positive definite (block Jacobi then).
+// The coarse space follows the pattern and tiling.  The coarse operator is rebuilt at the first
+// Newton iteration of a timestep once the Krylov iterations of Newton iteration 1 (the largest
+// solve) exceed 1.25x their count in the timestep of the last rebuild, or coarse_refresh
+// timesteps after it; and for every solve outside the Newton loop.  A coarse operator from an
+// earlier Jacobian is still symmetric positive definite, so it changes the iteration count, not
+// the solution.  False when the coarse operator is not positive definite (block Jacobi then).
 static bool twolevel_prepare(tf_handle* h)
 {
-    if (!h->tl.built || h->tl.T != h->T) twolevel_build(h);
-    if (h->krylov_call == 0) h->tl.age += 1;
-    if (!h->tl.fresh || h->krylov_call < 0 ||
-        (h->krylov_call == 0 && h->tl.age >= std::max(1, h->opts.coarse_refresh)))
+    tf_handle::TwoLevel& tl = h->tl;
+    if (!tl.built || tl.T != h->T) twolevel_build(h);
+    if (h->krylov_call == 0) tl.age += 1;
+    if (!tl.fresh || h->krylov_call < 0 ||
+        (h->krylov_call == 0 && (tl.stale || tl.age >= std::max(1, h->opts.coarse_refresh))))
         twolevel_refresh(h);
-    return !h->tl.failed;
+    return !tl.failed;
+}
+
+// After a two-level solve of Newton iteration 1: the rebuild's baseline, or the staleness test
+static void twolevel_observe(tf_handle* h, int iters)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    if (h->krylov_call != 1) return;
+    if (tl.base_it1 < 0) tl.base_it1 = iters;
+    else if (iters > 1.25 * tl.base_it1) tl.stale = true;
 }
 
 // Average device time (ms) of one Krylov iteration over n_iters iterations on the current
@@ -2241,6 +2277,7 @@ int krylov_solve(tf_handle* h)
         h->tl_used = true;
     }
     int st = run(kind);
+    if (kind == KIND_TL && st == TF_OK) twolevel_observe(h, h->last_krylov_iters);
     if (st == TF_KRYLOV_STALL && (cg_kind || kind == KIND_TL)) {
         // not positive definite (or too slow): BiCGSTAB from zero
         const int cg_iters = h->last_krylov_iters;
