@@ -508,71 +508,76 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 // of node aggregates (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is four
 // launches:
 //   k_cg_q       (tiles)      p = z + beta p, q = M p, (p, q)
-//   k_tl_xr      (chunks)     x += alpha p, r -= alpha q on a chunk of an aggregate's nodes,
-//                             the chunk's (r, D^-1 r), (r, r) and P_chunk^T r
-//   k_tl_coarse  (row blocks) r_c from the chunk sums, z_c = Ac^-1 r_c; the tissue's (r, z) and
-//                             (r, r) as tile partials
+//   k_tl_xr      (aggregates) x += alpha p, r -= alpha q on the aggregate's nodes, their
+//                             (r, D^-1 r) and (r, r), and r_c = P_a^T r
+//   k_tl_coarse  (row blocks) z_c = Ac^-1 r_c; the tissue's (r, z) and (r, r) as tile partials
 //   k_tl_z       (tiles)      z = D^-1 r + P z_c
 // (r, z) = (r, D^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
-constexpr int TL_CHUNK = 64;  // nodes per chunk of an aggregate (one wave of k_tl_xr each)
 constexpr int TL_PREF = 32;   // entries per lane of a coarse row prefetched before r_c is known
 
 struct TLArgs {
     const int32_t* node_coff;  // [N] first coarse unknown of the node's aggregate
-    const int32_t* chunk_nodes;  // [C][TL_CHUNK] nodes of every chunk (one aggregate's, ascending), -1 past its end
-    const int32_t* agg_chunk;    // [A+1] chunks of every aggregate
+    const int32_t* agg_ptr;    // [A+1] nodes of every aggregate in agg_nodes
+    const int32_t* agg_nodes;  // [N] nodes of every aggregate (ascending), aggregate after aggregate
     const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
-    double* chunk_part;        // [C][8] (r, D^-1 r), (r, r) and the 6 mode sums of every chunk
+    double* agg_part;          // [A][2] (r, D^-1 r) and (r, r) over the aggregate's nodes
+    double* rc;                // [nc] restricted residual P^T r
     const float* Ainv;         // [nc][nc] (a preconditioner: FP32 storage, FP64 sums)
     double* zc;                // [nc + 6] (the last 6 stay zero)
-    int nc, n_agg, n_chunk, G, rows;  // rows: coarse rows per workgroup of k_tl_coarse
+    int nc, n_agg, G, rows;  // rows: coarse rows per workgroup of k_tl_coarse
 };
 
-// One wave per chunk.  INIT: r as given (no update); otherwise alpha = (r, z) / (p, q) from the
-// tile partials.  The chunk's node data is loaded before alpha is reduced.
+// One workgroup per aggregate.  INIT: r as given (no update); otherwise alpha = (r, z) / (p, q)
+// from the tile partials.  Every lane loads its first node's data before alpha is reduced.
 template <int P, bool INIT>
-__global__ __launch_bounds__(64) void k_tl_xr(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
+__global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
                                               const double* __restrict__ p, const double* __restrict__ q,
                                               double* __restrict__ x, double* __restrict__ r)
 {
     SegScalars& S = ta.seg[0];
     if (seg_done(S)) return;
-    const int c = blockIdx.x;
-    const int nk = tl.chunk_nodes[(size_t)c * TL_CHUNK + threadIdx.x];
-    const bool own = nk >= 0;
-    const int n = own ? nk : 0;
-    double rv[3], xv[3], pv[3], qv[3], D[9], Pn[18];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        rv[i] = r[3 * n + i];
-        if (!INIT) { xv[i] = x[3 * n + i]; pv[i] = p[3 * n + i]; qv[i] = q[3 * n + i]; }
-    }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) D[i] = dinv[9 * (size_t)n + i];
-#pragma unroll
-    for (int i = 0; i < 18; ++i) Pn[i] = tl.pcoef[18 * (size_t)n + i];
-    if (!INIT) {
-        const int sl[2] = {SL_RZ0 + P, SL_PQ};
-        double sc[2];
-        seg_sums<2, 64>(ta, 0, sl, sc);
-        if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite
-            if (blockIdx.x == 0 && threadIdx.x == 0) S.stalled = 1;
-            return;
-        }
-        const double alpha = sc[0] / sc[1];
+    const int a = blockIdx.x;
+    const int k0 = tl.agg_ptr[a], k1 = tl.agg_ptr[a + 1];
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, D^-1 r), (r, r), P^T r
+    double alpha = 0.0;
+    bool first = true;
+    for (int kb = k0; kb < k1 || first; kb += WG) {
+        const int k = kb + (int)threadIdx.x;
+        const bool own = k < k1;
+        const int n = own ? tl.agg_nodes[k] : 0;
+        double rv[3], xv[3], pv[3], qv[3], D[9], Pn[18];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            xv[i] = xv[i] + alpha * pv[i];
-            rv[i] = rv[i] - alpha * qv[i];
+            rv[i] = r[3 * n + i];
+            if (!INIT) { xv[i] = x[3 * n + i]; pv[i] = p[3 * n + i]; qv[i] = q[3 * n + i]; }
         }
-        if (own)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) { x[3 * n + i] = xv[i]; r[3 * n + i] = rv[i]; }
-    }
-    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, D^-1 r), (r, r), P^T r
-    if (own) {
+        for (int i = 0; i < 9; ++i) D[i] = dinv[9 * (size_t)n + i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Pn[i] = tl.pcoef[18 * (size_t)n + i];
+        if (!INIT && first) {
+            const int sl[2] = {SL_RZ0 + P, SL_PQ};
+            double sc[2];
+            seg_sums<2>(ta, 0, sl, sc);
+            if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite
+                if (a == 0 && threadIdx.x == 0) S.stalled = 1;
+                return;
+            }
+            alpha = sc[0] / sc[1];
+        }
+        first = false;
+        if (!own) continue;
+        if (!INIT) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                xv[i] = xv[i] + alpha * pv[i];
+                rv[i] = rv[i] - alpha * qv[i];
+                x[3 * n + i] = xv[i];
+                r[3 * n + i] = rv[i];
+            }
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double zi = D[3 * i] * rv[0] + D[3 * i + 1] * rv[1] + D[3 * i + 2] * rv[2];
@@ -582,17 +587,26 @@ __global__ __launch_bounds__(64) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
             for (int j = 0; j < 6; ++j) v[2 + j] += Pn[6 * i + j] * rv[i];
         }
     }
-    double mine = 0.0;
+    constexpr int NW = WG / 64;
+    __shared__ double red[8][NW];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const double t = wave_sum(v[j]);
-        if (threadIdx.x == j) mine = t;
+        if (l == 0) red[j][w] = t;
     }
-    if (threadIdx.x < 8) tl.chunk_part[8 * (size_t)c + threadIdx.x] = mine;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
+        if (threadIdx.x < 2) tl.agg_part[2 * a + threadIdx.x] = t;
+        else if ((int)threadIdx.x - 2 < tl.krank[a]) tl.rc[tl.coff[a] + threadIdx.x - 2] = t;
+    }
 }
 
-// r_c from the chunk sums (every workgroup, fixed order), then tl.rows rows of z_c = Ac^-1 r_c
-// (one wave per row; the wave's first row is loaded while r_c is summed).  Workgroup g writes
+// tl.rows rows of z_c = Ac^-1 r_c per workgroup (one wave per row; the wave's first row is loaded
+// while r_c is staged in LDS).  Workgroup g writes
 // the tile partials g, g + G, ... of slots rz_slot and rr_slot: its (r_c, z_c) at g and zeros
 // elsewhere; workgroup 0 adds the tissue's (r, D^-1 r) and (r, r) at tile 0.
 __global__ __launch_bounds__(WG) void k_tl_coarse(TileArgs ta, TLArgs tl, int rz_slot, int rr_slot, int count_iter)
@@ -613,19 +627,12 @@ __global__ __launch_bounds__(WG) void k_tl_coarse(TileArgs ta, TLArgs tl, int rz
             pre[m] = (i < i1 && j < tl.nc) ? A[j] : 0.0f;
         }
     }
-    for (int a = threadIdx.x; a < tl.n_agg; a += WG) {
-        double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int c = tl.agg_chunk[a]; c < tl.agg_chunk[a + 1]; ++c)
-#pragma unroll
-            for (int j = 0; j < 6; ++j) sv[j] += tl.chunk_part[8 * (size_t)c + 2 + j];
-        const int off = tl.coff[a], k = tl.krank[a];
-        for (int j = 0; j < k; ++j) rcs[off + j] = sv[j];
-    }
+    for (int i = threadIdx.x; i < tl.nc; i += WG) rcs[i] = tl.rc[i];
     double fine[2] = {0.0, 0.0};
     if (blockIdx.x == 0)
-        for (int c = threadIdx.x; c < tl.n_chunk; c += WG) {
-            fine[0] += tl.chunk_part[8 * (size_t)c];
-            fine[1] += tl.chunk_part[8 * (size_t)c + 1];
+        for (int a = threadIdx.x; a < tl.n_agg; a += WG) {
+            fine[0] += tl.agg_part[2 * a];
+            fine[1] += tl.agg_part[2 * a + 1];
         }
     __syncthreads();
     double cz = 0.0;
@@ -1544,10 +1551,10 @@ void tl_upload(tf_handle* h, T*& dst, const std::vector<T>& v)
 void tl_release(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.chunk_nodes, &tl.agg_chunk, &tl.coff, &tl.krank, &tl.pair_a,
+    for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&tl.pcoef, &tl.chunk_part, &tl.Ac, &tl.zc})
+    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac, &tl.zc})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     tl.built = tl.fresh = tl.failed = false;
@@ -1689,14 +1696,12 @@ static void twolevel_build(tf_handle* h)
     }
     const int nc = coff[A];
     if (nc > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");
-    std::vector<int32_t> chunk_nodes, agg_chunk(A + 1, 0);
+    std::vector<int32_t> agg_ptr(A + 1, 0), agg_nodes;
+    agg_nodes.reserve(N);
     for (int a = 0; a < A; ++a) {
-        const int m = (int)members[a].size();
-        for (int k0 = 0; k0 < m; k0 += TL_CHUNK)
-            for (int k = k0; k < k0 + TL_CHUNK; ++k) chunk_nodes.push_back(k < m ? members[a][k] : -1);
-        agg_chunk[a + 1] = (int32_t)(chunk_nodes.size() / TL_CHUNK);
+        agg_nodes.insert(agg_nodes.end(), members[a].begin(), members[a].end());
+        agg_ptr[a + 1] = (int32_t)agg_nodes.size();
     }
-    const int C = (int)(chunk_nodes.size() / TL_CHUNK);
     // aggregate pairs with their Jacobian blocks in block order
     const std::vector<int32_t>& rp = h->h_row_ptr;
     const std::vector<int32_t>& cl = h->h_col;
@@ -1730,8 +1735,8 @@ static void twolevel_build(tf_handle* h)
         tl_upload(h, tl.node_coff, node_coff);
         TF_HIP(hipStreamSynchronize(h->stream));
     }
-    tl_upload(h, tl.chunk_nodes, chunk_nodes);
-    tl_upload(h, tl.agg_chunk, agg_chunk);
+    tl_upload(h, tl.agg_ptr, agg_ptr);
+    tl_upload(h, tl.agg_nodes, agg_nodes);
     tl_upload(h, tl.pcoef, pcoef);
     tl_upload(h, tl.coff, coff);
     tl_upload(h, tl.krank, krank);
@@ -1743,7 +1748,8 @@ static void twolevel_build(tf_handle* h)
     tl.G = std::min(T, (nc + WG / 64 - 1) / (WG / 64));
     tl.rows = (nc + tl.G - 1) / tl.G;
     tl.G = (nc + tl.rows - 1) / tl.rows;
-    tl.chunk_part = tl_alloc<double>(8 * (size_t)C);
+    tl.agg_part = tl_alloc<double>(2 * (size_t)A);
+    tl.rc = tl_alloc<double>(nc);
     tl.Ac = tl_alloc<double>((size_t)nc * nc);
     tl.Ainv = tl_alloc<float>((size_t)nc * nc);
     tl.zc = tl_alloc<double>(nc + 6);
@@ -1761,7 +1767,6 @@ static void twolevel_build(tf_handle* h)
     tl.n_agg = A;
     tl.nc = nc;
     tl.n_pairs = P;
-    tl.n_chunk = C;
     tl.T = T;
     tl.h_node_agg = agg;
     tl.built = true;
@@ -1770,8 +1775,8 @@ static void twolevel_build(tf_handle* h)
 static TLArgs tl_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    return TLArgs{tl.node_coff, tl.chunk_nodes, tl.agg_chunk, tl.pcoef, tl.coff, tl.krank, tl.chunk_part,
-                  tl.Ainv, tl.zc, tl.nc, tl.n_agg, tl.n_chunk, tl.G, tl.rows};
+    return TLArgs{tl.node_coff, tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
+                  tl.Ainv, tl.zc, tl.nc, tl.n_agg, tl.G, tl.rows};
 }
 
 // Ac = P^T M P from the FP64 Jacobian, its Cholesky factor and inverse (rocSOLVER), symmetrised.
@@ -1889,7 +1894,7 @@ void enqueue_tl_iter(tf_handle* h, const VT* val)
     timer_begin(h, h->t_spmv, ev);
     launch_cg_q<VT, P>(h, val, h->wg);
     timer_end(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_tl_xr<P, false>), dim3(tla.n_chunk), dim3(64), 0, h->stream, ta, tla, h->dinv, p_new, h->kv,
+    hipLaunchKernelGGL((k_tl_xr<P, false>), dim3(tla.n_agg), dim3(WG), 0, h->stream, ta, tla, h->dinv, p_new, h->kv,
                        h->kx, h->kr);
     hipLaunchKernelGGL(k_tl_coarse, dim3(tla.G), blk, sizeof(double) * (size_t)tla.nc, h->stream, ta, tla,
                        SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
@@ -2014,7 +2019,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
                        h->seg_rtol_on ? h->seg_rtol2 : nullptr);
     if (kind == KIND_TL) {  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
         const TLArgs tla = tl_args(h);
-        hipLaunchKernelGGL((k_tl_xr<0, true>), dim3(tla.n_chunk), dim3(64), 0, h->stream, ta, tla, h->dinv, h->kp, h->kv,
+        hipLaunchKernelGGL((k_tl_xr<0, true>), dim3(tla.n_agg), dim3(WG), 0, h->stream, ta, tla, h->dinv, h->kp, h->kv,
                            h->kx, h->kr);
         hipLaunchKernelGGL(k_tl_coarse, dim3(tla.G), blk, sizeof(double) * (size_t)tla.nc, h->stream, ta, tla,
                            (int)SL_RZ0, (int)SL_RR0, 0);
@@ -2044,7 +2049,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     if (graph && call >= 0 && call < (int)h->krylov_pred.size() && h->krylov_pred[call] > 0)
         reps = std::max(1, (int)(0.8 * h->krylov_pred[call]) / chunk);
     KernelTimer& tm = h->t_spmv;
-    if (graph && timing && !tm.sample[0]) {
+    if (timing && !tm.sample[0]) {
         TF_HIP(hipEventCreate(&tm.sample[0]));
         TF_HIP(hipEventCreate(&tm.sample[1]));
     }
@@ -2072,18 +2077,20 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
                 h->krylov_graph_iters[gi] = g_it;
                 h->krylov_graph_kind[gi] = key;
             }
-            if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[0], h->stream));
+            if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
             for (int k = 0; k < reps * per_chunk; ++k) TF_HIP(hipGraphLaunch(graph_exec, h->stream));
-            if (timing) TF_HIP(hipEventRecord(h->t_spmv.sample[1], h->stream));
+            if (timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
         } else {
+            if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
             enqueue_chunk<VT>(h, val, kind, chunk);
             TF_HIP(hipGetLastError());
+            if (timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
         }
         const int launched = graph ? chunk * reps : chunk;
         reps = 1;
         done_iters += launched;
         const bool fin = all_done(h);
-        if (graph && timing) {
+        if (timing) {
             float ms = 0.f;
             TF_HIP(hipEventElapsedTime(&ms, tm.sample[0], tm.sample[1]));
             tm.replay_ms += ms;
@@ -2146,8 +2153,8 @@ static void twolevel_observe(tf_handle* h, int iters)
 }
 
 // Average device time (ms) of one Krylov iteration over n_iters iterations on the current
-// Jacobian, under the current tiling (tolerance 0; no fallback): whole graph replays divided by
-// the iterations launched.  The two-level PCG when twolevel_active, else the single-kernel CG.
+// Jacobian, under the current tiling (tolerance 0; no fallback): whole chunks (graph replays
+// under use_graph) divided by the iterations launched.  The two-level PCG when twolevel_active, else the single-kernel CG.
 double krylov_time_iteration(tf_handle* h, int n_iters)
 {
     const tf_solver_opts saved = h->opts;
@@ -2156,7 +2163,6 @@ double krylov_time_iteration(tf_handle* h, int n_iters)
     h->opts.krylov_atol = 0.0;
     h->opts.krylov_max_iter = n_iters - 1;  // launch n_iters sees the cap
     h->opts.iters_per_sync = std::min(n_iters, 32);  // the chunk the solves replay
-    h->opts.use_graph = 1;
     KernelTimer& tm = h->t_spmv;
     hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     TF_HIP(hipGetLastError());
@@ -2183,7 +2189,7 @@ double krylov_time_iteration(tf_handle* h, int n_iters)
 }
 
 // Back-to-back launches of the dominant Krylov kernel on the current Jacobian, timed whole by
-// events around a graph replay.  A short solve with tolerance 0 first leaves consistent vectors
+// events around a graph replay (around the launches themselves without use_graph).  A short solve with tolerance 0 first leaves consistent vectors
 // and partials; the tissues' flags are then reopened.  The single-kernel and pipelined CG advance
 // their iterations (the launches are the solver's own); k_cg_q (two-level and textbook CG) is
 // launched with one parity, so every launch repeats the same direction update and SpMV.
@@ -2236,23 +2242,30 @@ double krylov_probe(tf_handle* h, int n_launches)
     h->krylov_call = call;
     hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     TF_HIP(hipGetLastError());
-    hipGraph_t gr;
-    hipGraphExec_t ex = nullptr;
-    TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     const int wg = sk == KIND_TL ? h->wg : WG;  // as the solver launches it
-    for (int k = 0; k < n_launches; ++k) {
-        if (h->opts.matrix_precision == TF_PREC_FP32) launch_cg_q<float, 0>(h, h->val32, wg, 1 << 30);
-        else launch_cg_q<double, 0>(h, h->val, wg, 1 << 30);
+    auto launches = [&]() {
+        for (int k = 0; k < n_launches; ++k) {
+            if (h->opts.matrix_precision == TF_PREC_FP32) launch_cg_q<float, 0>(h, h->val32, wg, 1 << 30);
+            else launch_cg_q<double, 0>(h, h->val, wg, 1 << 30);
+        }
+    };
+    // one graph replay (use_graph), else the launches enqueued one by one
+    hipGraphExec_t ex = nullptr;
+    if (h->opts.use_graph) {
+        hipGraph_t gr;
+        TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        launches();
+        TF_HIP(hipStreamEndCapture(h->stream, &gr));
+        TF_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        TF_HIP(hipGraphDestroy(gr));
     }
-    TF_HIP(hipStreamEndCapture(h->stream, &gr));
-    TF_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-    TF_HIP(hipGraphDestroy(gr));
     hipEvent_t ev[2];
     TF_HIP(hipEventCreate(&ev[0]));
     TF_HIP(hipEventCreate(&ev[1]));
-    for (int rep = 0; rep < 2; ++rep) {  // the first replay warms the caches
+    for (int rep = 0; rep < 2; ++rep) {  // the first pass warms the caches
         TF_HIP(hipEventRecord(ev[0], h->stream));
-        TF_HIP(hipGraphLaunch(ex, h->stream));
+        if (ex) TF_HIP(hipGraphLaunch(ex, h->stream));
+        else launches();
         TF_HIP(hipEventRecord(ev[1], h->stream));
         TF_HIP(hipEventSynchronize(ev[1]));
         float e = 0.f;
@@ -2261,7 +2274,7 @@ double krylov_probe(tf_handle* h, int n_launches)
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
-    (void)hipGraphExecDestroy(ex);
+    if (ex) (void)hipGraphExecDestroy(ex);
     return ms;
 }
 
