@@ -266,7 +266,7 @@ def main():
         f_plain = elastic_forces(s)
         fscale = np.max(np.abs(f_plain))
         mixed = {}
-        for name, tp in (("fp32_mfma", 1), ("bf16_mfma", 2)):
+        for name, tp in (("fp64_tangent_fp32_matrix", 0), ("fp32_mfma", 1), ("bf16_mfma", 2)):
             mopts = dict(opts, tangent_precision=tp, matrix_precision=1)
             sm, mstats, msec, x_mixed = run_timed(NewtonRaphsonSolver, tissue, mopts, a.warmup, a.steps, 0)
             disp_err = float(np.max(np.abs(x_mixed - x_plain)) / np.max(np.abs(x_plain)))
