@@ -996,12 +996,20 @@ static double ms_since(std::chrono::steady_clock::time_point t0)
 //    weakly pinned modes keep the slowest error), so rtol = 0.05 q / (100 p);
 //  - q < nr_tol (the iteration before the last), symmetric solvers: the error may be 5% of the
 //    margin nr_tol - q, so that it cannot move the last ||dU|| across nr_tol:
-//    rtol = 0.05 (nr_tol - q) / (100 p), never below krylov_rtol;
+//    rtol = 0.05 (nr_tol - q) / (100 p), never below krylov_rtol (20 instead of 100 with the
+//    two-level preconditioner);
+//  - nr_tol <= q < 100 nr_tol, symmetric solvers: the error may be 5% of the margin that keeps the
+//    next update above nr_tol (q - nr_tol) and the one after it, C q^2, on its side of nr_tol
+//    (|sqrt(nr_tol / C) - q|), over the same amplification;
 //  - otherwise krylov_rtol, so that the next ||dU||, and with it the NR iteration count, is that
-//    of an exact solve (BiCGSTAB has no such error bound and keeps exact penultimate solves).
+//    of an exact solve (BiCGSTAB has no such error bound and keeps exact solves there).
 static void forcing_tolerances(tf_handle* h, int iter)
 {
-    const double floor_rtol = h->opts.krylov_rtol, tol = h->opts.nr_tol, amplification = 100.0;
+    // error amplification margin: ||error|| / (||dU|| rtol) of a CG iterate reached ~150 with block
+    // Jacobi (smooth error, weakly pinned meshes) but stayed below 10 with the two-level
+    // preconditioner, whose coarse space removes the smooth components
+    const bool two_level = twolevel_active(h) && !h->tl.failed;
+    const double floor_rtol = h->opts.krylov_rtol, tol = h->opts.nr_tol, amplification = two_level ? 20.0 : 100.0;
     const int kind = h->opts.krylov;
     const bool symmetric = !h->any_viscous && (kind < 0 || kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 ||
                                                kind == TF_KRYLOV_PIPECG);
@@ -1017,6 +1025,13 @@ static void forcing_tolerances(tf_handle* h, int iter)
             else if (q >= 100.0 * tol) eta = std::min(1e-2, std::max(floor_rtol, 0.05 * q / (amplification * p)));
             else if (q < tol && symmetric)
                 eta = std::min(1e-2, std::max(floor_rtol, 0.05 * (tol - q) / (amplification * p)));
+            else if (symmetric) {  // tol <= q < 100 tol
+                // an error e moves the next update to about q +- e: it stays above tol while
+                // e < q - tol, and the update after it (C (q +- e)^2) stays on the side of tol
+                // that C q^2 is on while e < |sqrt(tol / C) - q|
+                const double margin = std::min(q - tol, std::fabs(std::sqrt(tol / C) - q));
+                eta = std::min(1e-2, std::max(floor_rtol, 0.05 * margin / (amplification * p)));
+            }
         }
         h->h_seg_rtol2[s] = eta * eta;
     }
