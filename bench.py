@@ -276,6 +276,11 @@ def main():
             mixed[name] = {"value": a.steps / msec, "ms_per_step": 1e3 * msec / a.steps,
                            "disp_err_rel_after_all_steps": disp_err, "force_err_rel": f_err,
                            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in mstats])),
+                           "krylov_iterations_per_nr_iteration":
+                               float(np.sum([st["krylov_iterations"] for st in mstats]) /
+                                     max(1, np.sum([st["nr_iterations"] for st in mstats]))),
+                           "ms_per_step_by_phase": {k: float(np.mean([st[k] for st in mstats]))
+                                                    for k in ("ms_grow", "ms_assemble", "ms_solve")},
                            "converged_steps": int(sum(st["converged"] for st in mstats)),
                            "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in mstats)),
                            "speedup_vs_fp64": (a.steps / msec) / (a.steps / sec)}

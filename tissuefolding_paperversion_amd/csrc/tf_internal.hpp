@@ -243,15 +243,14 @@ struct tf_handle {
         int32_t age = 0;                                    // timesteps since the coarse operator was built
         int32_t base_it1 = -1;                              // Krylov iterations of NR iteration 1 after the build
         bool stale = false;                                 // a later NR iteration 1 took > 1.25x base_it1
-        int32_t *node_agg = nullptr, *node_coff = nullptr;  // [N] aggregate, its first coarse unknown
+        int32_t* node_agg = nullptr;                        // [N] aggregate of every node
         int32_t *agg_ptr = nullptr, *agg_nodes = nullptr;   // [A+1], [N] nodes of every aggregate
         double* pcoef = nullptr;                            // [N][3][6] rows of P
         int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank
         int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
         double *agg_part = nullptr, *rc = nullptr;          // [A][2], [nc]
-        double *Ac = nullptr, *zc = nullptr;
+        double* Ac = nullptr;
         float* Ainv = nullptr;                              // [nc][nc] FP32 copy of Ac^-1
-        int32_t rows = 0;                                   // coarse rows per workgroup of k_tl_coarse
         int32_t* info = nullptr;
         void* blas = nullptr;                               // rocblas_handle
         std::vector<int32_t> h_node_agg;

@@ -505,18 +505,16 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 
 // ---------------------------------------------------------------------------------------------
 // Two-level preconditioner (one tissue): z = D^-1 r + P Ac^-1 P^T r, with P the rigid-body modes
-// of node aggregates (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is four
+// of node aggregates (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is three
 // launches:
-//   k_cg_q       (tiles)      p = z + beta p, q = M p, (p, q)
-//   k_tl_xr      (aggregates) x += alpha p, r -= alpha q on the aggregate's nodes, their
-//                             (r, D^-1 r) and (r, r), and r_c = P_a^T r
-//   k_tl_coarse  (row blocks) z_c = Ac^-1 r_c; the tissue's (r, z) and (r, r) as tile partials
-//   k_tl_z       (tiles)      z = D^-1 r + P z_c
+//   k_cg_q   (tiles)      p = z + beta p, q = M p, (p, q)
+//   k_tl_xr  (aggregates) x += alpha p, r -= alpha q on the aggregate's nodes, their
+//                         (r, D^-1 r) and (r, r), and r_c = P_a^T r
+//   k_tl_cz  (aggregates) z_c = Ac^-1 r_c on the aggregate's coarse rows, z = D^-1 r + P z_c on
+//                         its nodes; the tissue's (r, z) and (r, r) as tile partials
 // (r, z) = (r, D^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
-constexpr int TL_PREF = 32;   // entries per lane of a coarse row prefetched before r_c is known
 
 struct TLArgs {
-    const int32_t* node_coff;  // [N] first coarse unknown of the node's aggregate
     const int32_t* agg_ptr;    // [A+1] nodes of every aggregate in agg_nodes
     const int32_t* agg_nodes;  // [N] nodes of every aggregate (ascending), aggregate after aggregate
     const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
@@ -525,8 +523,7 @@ struct TLArgs {
     double* agg_part;          // [A][2] (r, D^-1 r) and (r, r) over the aggregate's nodes
     double* rc;                // [nc] restricted residual P^T r
     const float* Ainv;         // [nc][nc] (a preconditioner: FP32 storage, FP64 sums)
-    double* zc;                // [nc + 6] (the last 6 stay zero)
-    int nc, n_agg, G, rows;  // rows: coarse rows per workgroup of k_tl_coarse
+    int nc, n_agg, G;          // G: workgroups of k_tl_cz
 };
 
 // One workgroup per aggregate.  INIT: r as given (no update); otherwise alpha = (r, z) / (p, q)
@@ -605,93 +602,110 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
     }
 }
 
-// tl.rows rows of z_c = Ac^-1 r_c per workgroup (one wave per row; the wave's first row is loaded
-// while r_c is staged in LDS).  Workgroup g writes
-// the tile partials g, g + G, ... of slots rz_slot and rr_slot: its (r_c, z_c) at g and zeros
-// elsewhere; workgroup 0 adds the tissue's (r, D^-1 r) and (r, r) at tile 0.
-__global__ __launch_bounds__(WG) void k_tl_coarse(TileArgs ta, TLArgs tl, int rz_slot, int rr_slot, int count_iter)
+// z_c = Ac^-1 r_c on the coarse rows of the workgroup's aggregates (g, g + G, ...), then
+// z = D^-1 r + P z_c on their nodes.  Each lane loads its columns (t, t + WG, ...) of the first
+// aggregate's rows of Ac^-1 and its first DoF's r, D^-1 row and P row before r_c is staged in LDS.
+// Workgroup g writes the tile partials g of slots rz_slot ((r, D^-1 r) + (r_c, z_c) over its
+// aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
+constexpr int TL_CZ_COLS = 8;  // columns per lane of every coarse row held in registers
+
+__global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
+                                              const double* __restrict__ r, double* __restrict__ z, int rz_slot,
+                                              int rr_slot, int count_iter)
 {
     SegScalars& S = ta.seg[0];
     if (seg_done(S)) return;
-    extern __shared__ double rcs[];
+    extern __shared__ double rcs[];  // r_c, zero-padded to max(nc, WG * TL_CZ_COLS)
     constexpr int NW = WG / 64;
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int i0 = blockIdx.x * tl.rows, i1 = min(tl.nc, i0 + tl.rows);
-    float pre[TL_PREF];
-    {
-        const int i = i0 + w;
-        const float* A = tl.Ainv + (size_t)min(i, tl.nc - 1) * tl.nc;
+    __shared__ double red[6][NW];
+    __shared__ double zcs[6];
+    const int tid = threadIdx.x, w = tid / 64, l = tid % 64;
+    const int nc = tl.nc, G = gridDim.x;
+    double acc_rz = 0.0, acc_rr = 0.0;  // lane 0
+    bool first = true;
+    for (int a = blockIdx.x; a < tl.n_agg; a += G) {
+        const int c0 = tl.coff[a], kr = tl.krank[a];
+        const int k0 = tl.agg_ptr[a], ndof = 3 * (tl.agg_ptr[a + 1] - k0);
+        // Unconditional loads at clamped addresses (a load under a condition is issued in a branch
+        // of its own and waited for before the next): rows past the rank are summed and dropped,
+        // columns past nc meet the zeros that pad r_c in LDS.
+        float pre[6][TL_CZ_COLS];
 #pragma unroll
-        for (int m = 0; m < TL_PREF; ++m) {
-            const int j = l + 64 * m;
-            pre[m] = (i < i1 && j < tl.nc) ? A[j] : 0.0f;
-        }
-    }
-    for (int i = threadIdx.x; i < tl.nc; i += WG) rcs[i] = tl.rc[i];
-    double fine[2] = {0.0, 0.0};
-    if (blockIdx.x == 0)
-        for (int a = threadIdx.x; a < tl.n_agg; a += WG) {
-            fine[0] += tl.agg_part[2 * a];
-            fine[1] += tl.agg_part[2 * a + 1];
-        }
-    __syncthreads();
-    double cz = 0.0;
-    for (int i = i0 + w; i < i1; i += NW) {
-        const float* A = tl.Ainv + (size_t)i * tl.nc;
-        double sv = 0.0;
-        if (i == i0 + w) {
+        for (int k = 0; k < 6; ++k)
 #pragma unroll
-            for (int m = 0; m < TL_PREF; ++m) {
-                const int j = l + 64 * m;
-                if (j < tl.nc) sv += (double)pre[m] * rcs[j];
+            for (int m = 0; m < TL_CZ_COLS; ++m)
+                pre[k][m] = tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + min(tid + WG * m, nc - 1)];
+        // the lane's first DoF of the aggregate (an aggregate has at least one node)
+        const int q0 = min(tid, ndof - 1);
+        int dn = tl.agg_nodes[k0 + q0 / 3], di = q0 % 3;
+        double rv[3], Dv[3], Pv[6];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { rv[i] = r[3 * dn + i]; Dv[i] = dinv[9 * (size_t)dn + 3 * di + i]; }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) Pv[j] = tl.pcoef[18 * (size_t)dn + 6 * di + j];
+        if (first) {
+            for (int i = tid; i < max(nc, WG * TL_CZ_COLS); i += WG) rcs[i] = i < nc ? tl.rc[i] : 0.0;
+            __syncthreads();
+            first = false;
+        }
+        double s[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            s[k] = 0.0;
+#pragma unroll
+            for (int m = 0; m < TL_CZ_COLS; ++m) s[k] += (double)pre[k][m] * rcs[tid + WG * m];
+        }
+        for (int j = tid + WG * TL_CZ_COLS; j < nc; j += WG)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s[k] += (double)tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + j] * rcs[j];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double t = wave_sum(s[k]);
+            if (l == 0) red[k][w] = t;
+        }
+        __syncthreads();
+        if (tid < 6) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) t += red[tid][k];
+            zcs[tid] = tid < kr ? t : 0.0;  // P's columns past the rank are zero
+        }
+        __syncthreads();
+        for (int q = tid; q < ndof; q += WG) {
+            if (q != tid) {
+                dn = tl.agg_nodes[k0 + q / 3];
+                di = q % 3;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { rv[i] = r[3 * dn + i]; Dv[i] = dinv[9 * (size_t)dn + 3 * di + i]; }
+#pragma unroll
+                for (int j = 0; j < 6; ++j) Pv[j] = tl.pcoef[18 * (size_t)dn + 6 * di + j];
             }
-            for (int j = l + 64 * TL_PREF; j < tl.nc; j += 64) sv += (double)A[j] * rcs[j];
-        } else {
-            for (int j = l; j < tl.nc; j += 64) sv += (double)A[j] * rcs[j];
-        }
-        sv = wave_sum(sv);
-        if (l == 0) {
-            tl.zc[i] = sv;
-            cz += sv * rcs[i];
-        }
-    }
-    __shared__ double wred[3][NW];
-    fine[0] = wave_sum(fine[0]);
-    fine[1] = wave_sum(fine[1]);
-    if (l == 0) { wred[0][w] = cz; wred[1][w] = fine[0]; wred[2][w] = fine[1]; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[3] = {0.0, 0.0, 0.0};
+            double v = Dv[0] * rv[0] + Dv[1] * rv[1] + Dv[2] * rv[2];
 #pragma unroll
-        for (int k = 0; k < NW; ++k) { t[0] += wred[0][k]; t[1] += wred[1][k]; t[2] += wred[2][k]; }
-        const int g = blockIdx.x;
-        ta.partial[(size_t)rz_slot * ta.T + g] = g == 0 ? t[1] + t[0] : t[0];
-        ta.partial[(size_t)rr_slot * ta.T + g] = g == 0 ? t[2] : 0.0;
+            for (int j = 0; j < 6; ++j) v += Pv[j] * zcs[j];
+            z[3 * dn + di] = v;
+        }
+        if (tid == 0) {
+            double cz = 0.0;
+            for (int k = 0; k < kr; ++k) cz += rcs[c0 + k] * zcs[k];
+            acc_rz += tl.agg_part[2 * a] + cz;
+            acc_rr += tl.agg_part[2 * a + 1];
+        }
+        __syncthreads();  // red and zcs are reused by the next aggregate
+    }
+    const int g = blockIdx.x;
+    if (tid == 0) {
+        ta.partial[(size_t)rz_slot * ta.T + g] = acc_rz;
+        ta.partial[(size_t)rr_slot * ta.T + g] = acc_rr;
         if (g == 0 && count_iter) S.iters += 1;
     }
-    for (int t = blockIdx.x + tl.G + (int)threadIdx.x * tl.G; t < ta.T; t += WG * tl.G) {
+    for (int t = g + G + tid * G; t < ta.T; t += WG * G) {
         ta.partial[(size_t)rz_slot * ta.T + t] = 0.0;
         ta.partial[(size_t)rr_slot * ta.T + t] = 0.0;
     }
 }
 
-// z = D^-1 r + P z_c on the tile's rows
-__global__ __launch_bounds__(WG) void k_tl_z(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
-                                             const double* __restrict__ r, double* __restrict__ z)
-{
-    const int t = blockIdx.x;
-    if (seg_done(ta.seg[0])) return;
-    const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
-    for (int d = d0 + threadIdx.x; d < d1; d += WG) {
-        const int n = d / 3, i = d % 3;
-        const double* Pn = tl.pcoef + 18 * (size_t)n + 6 * i;
-        const double* zc = tl.zc + tl.node_coff[n];
-        double v = dinv_row(dinv, r, d);
-#pragma unroll
-        for (int j = 0; j < 6; ++j) v += Pn[j] * zc[j];
-        z[d] = v;
-    }
-}
+static size_t tl_cz_lds(const TLArgs& tl) { return sizeof(double) * (size_t)std::max(tl.nc, WG * TL_CZ_COLS); }
 
 // r = b - M x0 (x0 given) or b, x = x0 or 0, p_old = 0, and (b, b)
 template <typename VT>
@@ -1551,10 +1565,10 @@ void tl_upload(tf_handle* h, T*& dst, const std::vector<T>& v)
 void tl_release(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    for (int32_t** p : {&tl.node_agg, &tl.node_coff, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
+    for (int32_t** p : {&tl.node_agg, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac, &tl.zc})
+    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     tl.built = tl.fresh = tl.failed = false;
@@ -1729,12 +1743,6 @@ static void twolevel_build(tf_handle* h)
         for (size_t j = 0; j < pkey.size(); ++j) pair_blk[fill[bpair[j]]++] = (int32_t)j;
     }
     tl_upload(h, tl.node_agg, agg);
-    {
-        std::vector<int32_t> node_coff(N);
-        for (int n = 0; n < N; ++n) node_coff[n] = coff[agg[n]];
-        tl_upload(h, tl.node_coff, node_coff);
-        TF_HIP(hipStreamSynchronize(h->stream));
-    }
     tl_upload(h, tl.agg_ptr, agg_ptr);
     tl_upload(h, tl.agg_nodes, agg_nodes);
     tl_upload(h, tl.pcoef, pcoef);
@@ -1744,16 +1752,13 @@ static void twolevel_build(tf_handle* h)
     tl_upload(h, tl.pair_b, pair_b);
     tl_upload(h, tl.pair_ptr, pair_ptr);
     tl_upload(h, tl.pair_blk, pair_blk);
-    // one coarse row per wave, at most one workgroup per tile (the partials k_tl_coarse writes)
-    tl.G = std::min(T, (nc + WG / 64 - 1) / (WG / 64));
-    tl.rows = (nc + tl.G - 1) / tl.G;
-    tl.G = (nc + tl.rows - 1) / tl.rows;
+    // k_tl_cz: aggregates g, g + G, ... in workgroup g, at most one workgroup per tile (the
+    // partials it writes)
+    tl.G = std::min(T, A);
     tl.agg_part = tl_alloc<double>(2 * (size_t)A);
     tl.rc = tl_alloc<double>(nc);
     tl.Ac = tl_alloc<double>((size_t)nc * nc);
     tl.Ainv = tl_alloc<float>((size_t)nc * nc);
-    tl.zc = tl_alloc<double>(nc + 6);
-    launch_fill(h, tl.zc, 0.0, nc + 6);
     tl.info = tl_alloc<int32_t>(1);
     if (!tl.blas) {
         rocblas_handle bh = nullptr;
@@ -1775,8 +1780,8 @@ static void twolevel_build(tf_handle* h)
 static TLArgs tl_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    return TLArgs{tl.node_coff, tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
-                  tl.Ainv, tl.zc, tl.nc, tl.n_agg, tl.G, tl.rows};
+    return TLArgs{tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
+                  tl.Ainv, tl.nc, tl.n_agg, tl.G};
 }
 
 // Ac = P^T M P from the FP64 Jacobian, its Cholesky factor and inverse (rocSOLVER), symmetrised.
@@ -1896,9 +1901,8 @@ void enqueue_tl_iter(tf_handle* h, const VT* val)
     timer_end(h, h->t_spmv, ev);
     hipLaunchKernelGGL((k_tl_xr<P, false>), dim3(tla.n_agg), dim3(WG), 0, h->stream, ta, tla, h->dinv, p_new, h->kv,
                        h->kx, h->kr);
-    hipLaunchKernelGGL(k_tl_coarse, dim3(tla.G), blk, sizeof(double) * (size_t)tla.nc, h->stream, ta, tla,
-                       SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
-    hipLaunchKernelGGL(k_tl_z, grid, blk, 0, h->stream, ta, tla, h->dinv, h->kr, h->ks);
+    hipLaunchKernelGGL(k_tl_cz, dim3(tla.G), blk, tl_cz_lds(tla), h->stream, ta, tla, h->dinv, h->kr,
+                       h->ks, SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
 }
 
 // Single-kernel CG buffers: x = kx, p = kp, r = kr, w = kv, s = ks (own rows only) and the
@@ -2021,16 +2025,15 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         const TLArgs tla = tl_args(h);
         hipLaunchKernelGGL((k_tl_xr<0, true>), dim3(tla.n_agg), dim3(WG), 0, h->stream, ta, tla, h->dinv, h->kp, h->kv,
                            h->kx, h->kr);
-        hipLaunchKernelGGL(k_tl_coarse, dim3(tla.G), blk, sizeof(double) * (size_t)tla.nc, h->stream, ta, tla,
-                           (int)SL_RZ0, (int)SL_RR0, 0);
-        hipLaunchKernelGGL(k_tl_z, grid, blk, 0, h->stream, ta, tla, h->dinv, h->kr, h->ks);
+        hipLaunchKernelGGL(k_tl_cz, dim3(tla.G), blk, tl_cz_lds(tla), h->stream, ta, tla, h->dinv,
+                           h->kr, h->ks, (int)SL_RZ0, (int)SL_RR0, 0);
     }
     TF_HIP(hipGetLastError());
     // A captured graph holds at most TF_GRAPH_NODES kernel launches (an even number of
     // iterations); a chunk between convergence checks is one replay of it, at most iters_per_sync
-    // iterations.  (The two-level PCG, four launches per iteration, checks every 8 iterations: a
+    // iterations.  (The two-level PCG, three launches per iteration, checks every 10 iterations: a
     // solve's last chunk then overshoots by fewer no-op launches than it costs in checks.)
-    const int kpi = kind == KIND_TL ? 4 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
+    const int kpi = kind == KIND_TL ? 3 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
     int chunk = std::min(std::max(2, h->opts.iters_per_sync), std::max(2, TF_GRAPH_NODES / kpi));
     chunk -= chunk & 1;  // iterations come in parity pairs
     const int g_it = chunk, per_chunk = 1;
