@@ -126,6 +126,23 @@ def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing, tune=Tru
     return s, stats, sec, x
 
 
+def trajectory(NewtonRaphsonSolver, tissue, opts, steps, tiling, ref=None):
+    """Untimed run of the same timesteps on a fresh handle at the given tiling: the positions after
+    every step, or (ref given) the largest relative displacement error against ref over all steps."""
+    s = NewtonRaphsonSolver(tissue, opts)
+    s.set_tiling(tiling[0], tiling[1])
+    xs, err = [], 0.0
+    for k in range(steps):
+        s.runOneStep(tissue.dt)
+        x = s.positions
+        if ref is None:
+            xs.append(x)
+        else:
+            err = max(err, float(np.max(np.abs(x - ref[k])) / np.max(np.abs(ref[k]))))
+    s.close()
+    return xs if ref is None else err
+
+
 def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
     """The 256-disc sweep (one tissue segment per disc in one handle) against the same discs run
     one handle at a time, plus a bounded CPU-oracle sample scaled by prism count."""
@@ -267,13 +284,17 @@ def main():
         # update, Krylov vectors and dots): throughput, and errors against the plain FP64 run
         f_plain = elastic_forces(s)
         fscale = np.max(np.abs(f_plain))
+        til = s.tiling()
+        x_ref = trajectory(NewtonRaphsonSolver, tissue, opts, a.steps, til)  # FP64, every step
         mixed = {}
         for name, tp in (("fp64_tangent_fp32_matrix", 0), ("fp32_mfma", 1), ("bf16_mfma", 2)):
             mopts = dict(opts, tangent_precision=tp, matrix_precision=1)
             sm, mstats, msec, x_mixed = run_timed(NewtonRaphsonSolver, tissue, mopts, a.warmup, a.steps, 0)
             disp_err = float(np.max(np.abs(x_mixed - x_plain)) / np.max(np.abs(x_plain)))
             f_err = float(np.max(np.abs(elastic_forces(sm) - f_plain)) / fscale)
+            step_err = trajectory(NewtonRaphsonSolver, tissue, mopts, a.steps, til, ref=x_ref)
             mixed[name] = {"value": a.steps / msec, "ms_per_step": 1e3 * msec / a.steps,
+                           "disp_err_rel_max_over_steps": step_err,
                            "disp_err_rel_after_all_steps": disp_err, "force_err_rel": f_err,
                            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in mstats])),
                            "krylov_iterations_per_nr_iteration":

@@ -213,7 +213,6 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
     h->wg = wg;
     h->tile_cap = tile_cap;
     drop_krylov_graphs(h);
-    twolevel_invalidate(h);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -229,6 +228,7 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
 // and columns the reference adds into the master's, SC/NewtonRaphsonSolver.cpp:188-206).
 void build_pattern(tf_handle* h)
 {
+    twolevel_invalidate(h);  // the coarse space's block pairs and excluded DoFs follow the pattern
     const int N = h->N, E = h->E;
     const std::vector<int32_t>& conn = h->h_conn;
     const std::vector<int32_t>& node_ptr = h->h_node_ptr;
@@ -639,6 +639,10 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->fc_du0.assign(S, -1.0);
         h->fc_du_prev.assign(S, -1.0);
         h->h_seg_rtol2.assign(S, 0.0);
+        // set-up that depends on the mesh alone, done here rather than inside the first timestep:
+        // the packing detection tables and, when the default solver uses it, the coarse space
+        packing_prepare(h);
+        twolevel_setup(h);
     } catch (const tf_hip_error& ex) {
         free_all(h);
         delete h;
@@ -674,7 +678,10 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
     h->opts = *o;
     drop_krylov_graphs(h);
+    TF_GUARD_BEGIN
+    twolevel_setup(h);
     return TF_OK;
+    TF_GUARD_END(h)
 }
 
 int tf_get_options(const tf_handle* h, tf_solver_opts* o)

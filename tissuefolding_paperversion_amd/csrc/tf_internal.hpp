@@ -239,7 +239,7 @@ struct tf_handle {
     // modes of node aggregates (columns through all layers, cut by an xy grid); see tf_krylov.hip
     struct TwoLevel {
         bool built = false, fresh = false, failed = false;
-        int32_t n_agg = 0, nc = 0, n_pairs = 0, G = 0, T = -1;
+        int32_t n_agg = 0, nc = 0, n_pairs = 0;
         int32_t age = 0;                                    // timesteps since the coarse operator was built
         int32_t base_it1 = -1;                              // Krylov iterations of NR iteration 1 after the build
         bool stale = false;                                 // a later NR iteration 1 took > 1.25x base_it1
@@ -308,7 +308,8 @@ void warm_start_push(tf_handle* h, int k);    // keeps du as the newest update o
 void warm_start_free(tf_handle* h);
 void warm_start_clear(tf_handle* h);
 void twolevel_free(tf_handle* h);
-void twolevel_invalidate(tf_handle* h);  // pattern or tiling changed
+void twolevel_invalidate(tf_handle* h);  // the pattern changed
+void twolevel_setup(tf_handle* h);       // builds the coarse space if twolevel_active and not built
 // ms per Krylov iteration (whole replays) of the solver krylov_solve would run for a symmetric
 // system: the two-level PCG when twolevel_active, else the single-kernel CG
 double krylov_time_iteration(tf_handle* h, int n_iters);

@@ -1686,7 +1686,7 @@ static void twolevel_build(tf_handle* h)
     tf_handle::TwoLevel& tl = h->tl;
     drop_krylov_graphs(h);  // captured chunks hold the coarse arrays freed below
     tl_release(h);
-    const int N = h->N, T = h->T;
+    const int N = h->N;
     std::vector<double> x(3 * (size_t)N);
     TF_HIP(hipMemcpyAsync(x.data(), h->uk, x.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     std::vector<uint8_t> excl(3 * (size_t)N);
@@ -1723,21 +1723,24 @@ static void twolevel_build(tf_handle* h)
     pkey.reserve(h->nb);
     for (int r = 0; r < N; ++r)
         for (int j = rp[r]; j < rp[r + 1]; ++j) pkey.push_back((int64_t)agg[r] * A + agg[cl[j]]);
-    std::vector<int64_t> pk(pkey);
-    std::sort(pk.begin(), pk.end());
-    pk.erase(std::unique(pk.begin(), pk.end()), pk.end());
-    const int P = (int)pk.size();
+    // pairs numbered in increasing key order (a dense table over the A x A keys)
+    std::vector<int32_t> pid((size_t)A * A, -1);
+    for (int64_t k : pkey) pid[k] = 0;
+    int P = 0;
+    for (int32_t& v : pid)
+        if (v >= 0) v = P++;
     std::vector<int32_t> pair_a(P), pair_b(P), pair_ptr(P + 1, 0), pair_blk(pkey.size());
     std::vector<int32_t> bpair(pkey.size());
     for (size_t j = 0; j < pkey.size(); ++j) {
-        bpair[j] = (int32_t)(std::lower_bound(pk.begin(), pk.end(), pkey[j]) - pk.begin());
+        bpair[j] = pid[pkey[j]];
         pair_ptr[bpair[j] + 1] += 1;
     }
-    for (int q = 0; q < P; ++q) {
-        pair_ptr[q + 1] += pair_ptr[q];
-        pair_a[q] = (int32_t)(pk[q] / A);
-        pair_b[q] = (int32_t)(pk[q] % A);
-    }
+    for (int64_t k = 0; k < (int64_t)A * A; ++k)
+        if (pid[k] >= 0) {
+            pair_a[pid[k]] = (int32_t)(k / A);
+            pair_b[pid[k]] = (int32_t)(k % A);
+        }
+    for (int q = 0; q < P; ++q) pair_ptr[q + 1] += pair_ptr[q];
     {
         std::vector<int32_t> fill(pair_ptr.begin(), pair_ptr.end() - 1);
         for (size_t j = 0; j < pkey.size(); ++j) pair_blk[fill[bpair[j]]++] = (int32_t)j;
@@ -1752,9 +1755,6 @@ static void twolevel_build(tf_handle* h)
     tl_upload(h, tl.pair_b, pair_b);
     tl_upload(h, tl.pair_ptr, pair_ptr);
     tl_upload(h, tl.pair_blk, pair_blk);
-    // k_tl_cz: aggregates g, g + G, ... in workgroup g, at most one workgroup per tile (the
-    // partials it writes)
-    tl.G = std::min(T, A);
     tl.agg_part = tl_alloc<double>(2 * (size_t)A);
     tl.rc = tl_alloc<double>(nc);
     tl.Ac = tl_alloc<double>((size_t)nc * nc);
@@ -1772,7 +1772,6 @@ static void twolevel_build(tf_handle* h)
     tl.n_agg = A;
     tl.nc = nc;
     tl.n_pairs = P;
-    tl.T = T;
     tl.h_node_agg = agg;
     tl.built = true;
 }
@@ -1780,8 +1779,10 @@ static void twolevel_build(tf_handle* h)
 static TLArgs tl_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
+    // k_tl_cz: aggregates g, g + G, ... in workgroup g, at most one workgroup per tile (the
+    // partials it writes)
     return TLArgs{tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
-                  tl.Ainv, tl.nc, tl.n_agg, tl.G};
+                  tl.Ainv, tl.nc, tl.n_agg, std::min(h->T, tl.n_agg)};
 }
 
 // Ac = P^T M P from the FP64 Jacobian, its Cholesky factor and inverse (rocSOLVER), symmetrised.
@@ -1811,6 +1812,11 @@ static void twolevel_refresh(tf_handle* h)
     const size_t n2 = (size_t)nc * nc;
     hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, tl.Ainv, nc);
     TF_HIP(hipGetLastError());
+}
+
+void twolevel_setup(tf_handle* h)
+{
+    if (twolevel_active(h) && !h->tl.built) twolevel_build(h);
 }
 
 void twolevel_invalidate(tf_handle* h)
@@ -2138,7 +2144,7 @@ bool twolevel_active(const tf_handle* h)
 static bool twolevel_prepare(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    if (!tl.built || tl.T != h->T) twolevel_build(h);
+    if (!tl.built) twolevel_build(h);
     if (h->krylov_call == 0) tl.age += 1;
     if (!tl.fresh || h->krylov_call < 0 ||
         (h->krylov_call == 0 && (tl.stale || tl.age >= std::max(1, h->opts.coarse_refresh))))
