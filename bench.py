@@ -300,6 +300,9 @@ def main():
                            "krylov_iterations_per_nr_iteration":
                                float(np.sum([st["krylov_iterations"] for st in mstats]) /
                                      max(1, np.sum([st["nr_iterations"] for st in mstats]))),
+                           "krylov_iterations_by_newton_iteration":
+                               [float(np.mean([st["krylov_by_newton"][k] for st in mstats]))
+                                for k in range(min(4, max(st["nr_iterations"] for st in mstats)))],
                            "ms_per_step_by_phase": {k: float(np.mean([st[k] for st in mstats]))
                                                     for k in ("ms_grow", "ms_assemble", "ms_solve")},
                            "converged_steps": int(sum(st["converged"] for st in mstats)),

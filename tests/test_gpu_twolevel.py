@@ -64,6 +64,20 @@ def test_single_solve_matches_block_jacobi(make, aggregates):
     assert it_tl < it_bj
 
 
+def test_aggregate_option_rebuilds_coarse_space():
+    """The coarse space is built when the handle is created; setting coarse_aggregates afterwards
+    (as the constructor's options do) rebuilds it, which shows in the iteration count."""
+    t = _ellipse()
+    its, dus = {}, {}
+    for a in (0, 7):
+        s = _assembled(t, coarse_aggregates=a, **TL)
+        dus[a] = s.solveForDeltaU()
+        its[a] = s.last_krylov["iterations"]
+    print(f"two-level iterations: default aggregates {its[0]}, 7 aggregates {its[7]}", flush=True)
+    assert its[7] > its[0]
+    assert rel_err(dus[7], dus[0]) < 1e-8
+
+
 def test_bound_slaves_follow_masters():
     t = configs.disc_tissue(radius=30.0)
     pairs = np.asarray(bound_pairs(t))

@@ -676,9 +676,11 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         o->coarse_aggregates < 0 || o->coarse_refresh < 0)
         return fail(h, TF_INVALID, "invalid solver options");
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
+    const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates;
     h->opts = *o;
     drop_krylov_graphs(h);
     TF_GUARD_BEGIN
+    if (coarse_changed) twolevel_invalidate(h);  // the aggregates follow the option
     twolevel_setup(h);
     return TF_OK;
     TF_GUARD_END(h)

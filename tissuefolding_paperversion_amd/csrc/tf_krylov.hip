@@ -607,7 +607,7 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
 // aggregate's rows of Ac^-1 and its first DoF's r, D^-1 row and P row before r_c is staged in LDS.
 // Workgroup g writes the tile partials g of slots rz_slot ((r, D^-1 r) + (r_c, z_c) over its
 // aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
-constexpr int TL_CZ_COLS = 8;  // columns per lane of every coarse row held in registers
+constexpr int TL_CZ_COLS = 12;  // columns per lane of every coarse row held in registers (nc <= 3072)
 
 __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
                                               const double* __restrict__ r, double* __restrict__ z, int rz_slot,
@@ -1694,9 +1694,9 @@ static void twolevel_build(tf_handle* h)
     TF_HIP(hipStreamSynchronize(h->stream));
     for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
     const int max_nc = std::min(4096, h->max_lds / (int)sizeof(double));
-    // default: one aggregate per 88 nodes, at most 340 (about 2,000 coarse unknowns: the dense FP32
-    // inverse every iteration reads stays near 16 MB)
-    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(340, N / 88));
+    // default: one aggregate per 66 nodes, at most 512 (at most 3,072 coarse unknowns, the columns
+    // k_tl_cz holds in registers; the dense FP32 inverse every iteration reads stays under 38 MB)
+    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(512, N / 66));
     target = std::min(target, max_nc / 6);
     int A = 0;
     std::vector<int32_t> agg = tl_aggregates(x, N, target, A);
