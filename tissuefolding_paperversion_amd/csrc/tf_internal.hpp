@@ -251,6 +251,9 @@ struct tf_handle {
         double *agg_part = nullptr, *rc = nullptr;          // [A][2], [nc]
         double* Ac = nullptr;
         float* Ainv = nullptr;                              // [nc][nc] FP32 copy of Ac^-1
+        int32_t L = 1, ncol = 0;                            // nodes per column, columns
+        int32_t* col_blk = nullptr;                         // [ncol][L][L] BSR block of every node pair or -1
+        float* dcinv = nullptr;                             // [ncol][3L][3L] FP32 inverse column blocks
         int32_t* info = nullptr;
         void* blas = nullptr;                               // rocblas_handle
         std::vector<int32_t> h_node_agg;

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <math.h>
+#include <numeric>
 #include <vector>
 
 namespace {
@@ -504,57 +505,127 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 }
 
 // ---------------------------------------------------------------------------------------------
-// Two-level preconditioner (one tissue): z = D^-1 r + P Ac^-1 P^T r, with P the rigid-body modes
-// of node aggregates (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is three
-// launches:
+// Two-level preconditioner (one tissue): z = Dc^-1 r + P Ac^-1 P^T r, with Dc the column blocks of
+// M (the 3L x 3L block of each column of L nodes joined by prisms' vertical edges; single nodes
+// when the columns are not uniform), P the rigid-body modes of aggregates of whole columns
+// (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is three launches:
 //   k_cg_q   (tiles)      p = z + beta p, q = M p, (p, q)
 //   k_tl_xr  (aggregates) x += alpha p, r -= alpha q on the aggregate's nodes, their
-//                         (r, D^-1 r) and (r, r), and r_c = P_a^T r
-//   k_tl_cz  (aggregates) z_c = Ac^-1 r_c on the aggregate's coarse rows, z = D^-1 r + P z_c on
+//                         (r, Dc^-1 r) and (r, r), and r_c = P_a^T r
+//   k_tl_cz  (aggregates) z_c = Ac^-1 r_c on the aggregate's coarse rows, z = Dc^-1 r + P z_c on
 //                         its nodes; the tissue's (r, z) and (r, r) as tile partials
-// (r, z) = (r, D^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
+// (r, z) = (r, Dc^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
+// An aggregate lists its nodes column by column, so global column g holds agg_nodes[gL .. gL+L-1].
+constexpr int TL_MAX_LEVELS = 4;  // nodes per column at most (12 x 12 column blocks)
 
 struct TLArgs {
     const int32_t* agg_ptr;    // [A+1] nodes of every aggregate in agg_nodes
-    const int32_t* agg_nodes;  // [N] nodes of every aggregate (ascending), aggregate after aggregate
+    const int32_t* agg_nodes;  // [N] nodes of every aggregate, column by column, aggregate after aggregate
     const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
     double* agg_part;          // [A][2] (r, D^-1 r) and (r, r) over the aggregate's nodes
     double* rc;                // [nc] restricted residual P^T r
     const float* Ainv;         // [nc][nc] (a preconditioner: FP32 storage, FP64 sums)
+    const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
     int nc, n_agg, G;          // G: workgroups of k_tl_cz
 };
 
-// One workgroup per aggregate.  INIT: r as given (no update); otherwise alpha = (r, z) / (p, q)
-// from the tile partials.  Every lane loads its first node's data before alpha is reduced.
-template <int P, bool INIT>
-__global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
-                                              const double* __restrict__ p, const double* __restrict__ q,
-                                              double* __restrict__ x, double* __restrict__ r)
+// Inverse of every column block of M, as a symmetric FP32 copy: 64 / 3L columns per wave, one lane
+// per block row, Gauss-Jordan elimination without pivoting (the blocks of a positive definite M
+// are positive definite; the pivot rows come by lane shuffles).  A column with a pivot that is not
+// positive and finite gets its nodes' 3 x 3 inverses (dinv) instead.
+template <int L>
+__global__ __launch_bounds__(WG) void k_tl_dcol(const int32_t* __restrict__ col_blk, const int32_t* __restrict__ agg_nodes,
+                                               const double* __restrict__ val, int nb, const double* __restrict__ dinv,
+                                               int ncol, float* __restrict__ dcinv)
 {
+    constexpr int D = 3 * L, CPW = 64 / D;
+    const int lane = threadIdx.x % 64, wave = (int)(blockIdx.x * (WG / 64) + threadIdx.x / 64);
+    const int slot = lane / D, li = lane % D, base = slot * D;
+    const int g = wave * CPW + slot;
+    const bool own = slot < CPW && g < ncol;
+    const int gg = own ? g : 0;
+    const int i = li / 3, ci = li % 3;
+    double a[D], v[D];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const int blk = col_blk[((size_t)gg * L + i) * L + j];
+        const int bb = blk >= 0 ? blk : 0;
+#pragma unroll
+        for (int cj = 0; cj < 3; ++cj) {
+            const double m = val[(size_t)(3 * ci + cj) * nb + bb];
+            a[3 * j + cj] = blk >= 0 ? m : 0.0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = j == li ? 1.0 : 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double pa[D], pv[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            pa[j] = __shfl(a[j], base + k, 64);
+            pv[j] = __shfl(v[j], base + k, 64);
+        }
+        const double piv = pa[k];
+        ok = ok && piv > 0.0 && isfinite(piv);
+        const double f = li == k ? 0.0 : a[k] / piv;
+        const double sc = li == k ? 1.0 / piv : 1.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            a[j] = sc * (a[j] - f * pa[j]);
+            v[j] = sc * (v[j] - f * pv[j]);
+        }
+    }
+    if (!own) return;
+    float* out = dcinv + (size_t)g * D * D;
+    if (ok) {
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            if (j >= li) {
+                const float e = (float)v[j];
+                out[li * D + j] = e;
+                out[j * D + li] = e;
+            }
+    } else {
+        const int node = agg_nodes[(size_t)g * L + i];
+#pragma unroll
+        for (int j = 0; j < D; ++j) out[li * D + j] = j / 3 == i ? (float)dinv[9 * (size_t)node + 3 * ci + j % 3] : 0.0f;
+    }
+}
+
+// One workgroup per aggregate, one lane per DoF of its columns (WG / 3L columns per pass).  INIT:
+// r as given (no update); otherwise alpha = (r, z) / (p, q) from the tile partials.  Every lane
+// loads its first DoF's data before alpha is reduced; the column's r is exchanged in LDS.
+template <int P, bool INIT, int L>
+__global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const double* __restrict__ p,
+                                              const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r)
+{
+    constexpr int D = 3 * L, CPP = WG / D;
     SegScalars& S = ta.seg[0];
     if (seg_done(S)) return;
+    __shared__ double rl[WG];
     const int a = blockIdx.x;
-    const int k0 = tl.agg_ptr[a], k1 = tl.agg_ptr[a + 1];
-    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, D^-1 r), (r, r), P^T r
+    const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
+    const int slot = threadIdx.x / D, li = threadIdx.x % D;
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, Dc^-1 r), (r, r), P^T r
     double alpha = 0.0;
-    bool first = true;
-    for (int kb = k0; kb < k1 || first; kb += WG) {
-        const int k = kb + (int)threadIdx.x;
-        const bool own = k < k1;
-        const int n = own ? tl.agg_nodes[k] : 0;
-        double rv[3], xv[3], pv[3], qv[3], D[9], Pn[18];
+    for (int c0 = 0; c0 < ncols; c0 += CPP) {
+        const bool own = slot < CPP && c0 + slot < ncols;
+        const int g = n0 / L + (own ? c0 + slot : 0);
+        const int node = tl.agg_nodes[(size_t)g * L + li / 3];
+        const int d = 3 * node + li % 3;
+        double rv = r[d], xv = 0.0, pv = 0.0, qv = 0.0;
+        if (!INIT) { xv = x[d]; pv = p[d]; qv = q[d]; }
+        float Dr[D];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            rv[i] = r[3 * n + i];
-            if (!INIT) { xv[i] = x[3 * n + i]; pv[i] = p[3 * n + i]; qv[i] = q[3 * n + i]; }
-        }
+        for (int j = 0; j < D; ++j) Dr[j] = tl.dcinv[((size_t)g * D + li) * D + j];
+        double Pr[6];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) D[i] = dinv[9 * (size_t)n + i];
-#pragma unroll
-        for (int i = 0; i < 18; ++i) Pn[i] = tl.pcoef[18 * (size_t)n + i];
-        if (!INIT && first) {
+        for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
+        if (!INIT && c0 == 0) {
             const int sl[2] = {SL_RZ0 + P, SL_PQ};
             double sc[2];
             seg_sums<2>(ta, 0, sl, sc);
@@ -564,25 +635,23 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
             }
             alpha = sc[0] / sc[1];
         }
-        first = false;
-        if (!own) continue;
         if (!INIT) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                xv[i] = xv[i] + alpha * pv[i];
-                rv[i] = rv[i] - alpha * qv[i];
-                x[3 * n + i] = xv[i];
-                r[3 * n + i] = rv[i];
-            }
+            xv = xv + alpha * pv;
+            rv = rv - alpha * qv;
+            if (own) { x[d] = xv; r[d] = rv; }
         }
+        rl[threadIdx.x] = rv;
+        __syncthreads();
+        if (own) {
+            double zi = 0.0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const double zi = D[3 * i] * rv[0] + D[3 * i + 1] * rv[1] + D[3 * i + 2] * rv[2];
-            v[0] += rv[i] * zi;
-            v[1] += rv[i] * rv[i];
+            for (int j = 0; j < D; ++j) zi += (double)Dr[j] * rl[slot * D + j];
+            v[0] += rv * zi;
+            v[1] += rv * rv;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) v[2 + j] += Pn[6 * i + j] * rv[i];
+            for (int j = 0; j < 6; ++j) v[2 + j] += Pr[j] * rv;
         }
+        __syncthreads();  // rl is refilled by the next pass
     }
     constexpr int NW = WG / 64;
     __shared__ double red[8][NW];
@@ -603,29 +672,33 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
 }
 
 // z_c = Ac^-1 r_c on the coarse rows of the workgroup's aggregates (g, g + G, ...), then
-// z = D^-1 r + P z_c on their nodes.  Each lane loads its columns (t, t + WG, ...) of the first
-// aggregate's rows of Ac^-1 and its first DoF's r, D^-1 row and P row before r_c is staged in LDS.
-// Workgroup g writes the tile partials g of slots rz_slot ((r, D^-1 r) + (r_c, z_c) over its
-// aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
+// z = Dc^-1 r + P z_c on their nodes (one lane per DoF of the aggregate's columns, WG / 3L columns
+// per pass, the column's r exchanged in LDS).  Each lane loads its columns (t, t + WG, ...) of the
+// first aggregate's rows of Ac^-1 and its first DoF's r, Dc^-1 row and P row before r_c is staged
+// in LDS.  Workgroup g writes the tile partials g of slots rz_slot ((r, Dc^-1 r) + (r_c, z_c)
+// over its aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
 constexpr int TL_CZ_COLS = 12;  // columns per lane of every coarse row held in registers (nc <= 3072)
 
-__global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ dinv,
-                                              const double* __restrict__ r, double* __restrict__ z, int rz_slot,
-                                              int rr_slot, int count_iter)
+template <int L>
+__global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ r,
+                                              double* __restrict__ z, int rz_slot, int rr_slot, int count_iter)
 {
+    constexpr int D = 3 * L, CPP = WG / D;
     SegScalars& S = ta.seg[0];
     if (seg_done(S)) return;
     extern __shared__ double rcs[];  // r_c, zero-padded to max(nc, WG * TL_CZ_COLS)
     constexpr int NW = WG / 64;
     __shared__ double red[6][NW];
     __shared__ double zcs[6];
+    __shared__ double rl[WG];
     const int tid = threadIdx.x, w = tid / 64, l = tid % 64;
+    const int slot = tid / D, li = tid % D;
     const int nc = tl.nc, G = gridDim.x;
     double acc_rz = 0.0, acc_rr = 0.0;  // lane 0
     bool first = true;
     for (int a = blockIdx.x; a < tl.n_agg; a += G) {
         const int c0 = tl.coff[a], kr = tl.krank[a];
-        const int k0 = tl.agg_ptr[a], ndof = 3 * (tl.agg_ptr[a + 1] - k0);
+        const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
         // Unconditional loads at clamped addresses (a load under a condition is issued in a branch
         // of its own and waited for before the next): rows past the rank are summed and dropped,
         // columns past nc meet the zeros that pad r_c in LDS.
@@ -635,32 +708,36 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
 #pragma unroll
             for (int m = 0; m < TL_CZ_COLS; ++m)
                 pre[k][m] = tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + min(tid + WG * m, nc - 1)];
-        // the lane's first DoF of the aggregate (an aggregate has at least one node)
-        const int q0 = min(tid, ndof - 1);
-        int dn = tl.agg_nodes[k0 + q0 / 3], di = q0 % 3;
-        double rv[3], Dv[3], Pv[6];
+        // the lane's DoF of the first pass
+        bool own = slot < CPP && slot < ncols;
+        int g = n0 / L + (own ? slot : 0);
+        int node = tl.agg_nodes[(size_t)g * L + li / 3];
+        int d = 3 * node + li % 3;
+        double rv = r[d];
+        float Dr[D];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) { rv[i] = r[3 * dn + i]; Dv[i] = dinv[9 * (size_t)dn + 3 * di + i]; }
+        for (int j = 0; j < D; ++j) Dr[j] = tl.dcinv[((size_t)g * D + li) * D + j];
+        double Pr[6];
 #pragma unroll
-        for (int j = 0; j < 6; ++j) Pv[j] = tl.pcoef[18 * (size_t)dn + 6 * di + j];
+        for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
         if (first) {
             for (int i = tid; i < max(nc, WG * TL_CZ_COLS); i += WG) rcs[i] = i < nc ? tl.rc[i] : 0.0;
             __syncthreads();
             first = false;
         }
-        double s[6];
+        double sv[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            s[k] = 0.0;
+            sv[k] = 0.0;
 #pragma unroll
-            for (int m = 0; m < TL_CZ_COLS; ++m) s[k] += (double)pre[k][m] * rcs[tid + WG * m];
+            for (int m = 0; m < TL_CZ_COLS; ++m) sv[k] += (double)pre[k][m] * rcs[tid + WG * m];
         }
         for (int j = tid + WG * TL_CZ_COLS; j < nc; j += WG)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) s[k] += (double)tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + j] * rcs[j];
+            for (int k = 0; k < 6; ++k) sv[k] += (double)tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + j] * rcs[j];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const double t = wave_sum(s[k]);
+            const double t = wave_sum(sv[k]);
             if (l == 0) red[k][w] = t;
         }
         __syncthreads();
@@ -671,19 +748,27 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
             zcs[tid] = tid < kr ? t : 0.0;  // P's columns past the rank are zero
         }
         __syncthreads();
-        for (int q = tid; q < ndof; q += WG) {
-            if (q != tid) {
-                dn = tl.agg_nodes[k0 + q / 3];
-                di = q % 3;
+        for (int cb = 0; cb < ncols; cb += CPP) {
+            if (cb > 0) {
+                own = slot < CPP && cb + slot < ncols;
+                g = n0 / L + (own ? cb + slot : 0);
+                node = tl.agg_nodes[(size_t)g * L + li / 3];
+                d = 3 * node + li % 3;
+                rv = r[d];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) { rv[i] = r[3 * dn + i]; Dv[i] = dinv[9 * (size_t)dn + 3 * di + i]; }
+                for (int j = 0; j < D; ++j) Dr[j] = tl.dcinv[((size_t)g * D + li) * D + j];
 #pragma unroll
-                for (int j = 0; j < 6; ++j) Pv[j] = tl.pcoef[18 * (size_t)dn + 6 * di + j];
+                for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
             }
-            double v = Dv[0] * rv[0] + Dv[1] * rv[1] + Dv[2] * rv[2];
+            rl[tid] = rv;
+            __syncthreads();
+            double v = 0.0;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) v += Pv[j] * zcs[j];
-            z[3 * dn + di] = v;
+            for (int j = 0; j < D; ++j) v += (double)Dr[j] * rl[slot * D + j];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) v += Pr[j] * zcs[j];
+            if (own) z[d] = v;
+            __syncthreads();  // rl is refilled by the next pass
         }
         if (tid == 0) {
             double cz = 0.0;
@@ -693,13 +778,13 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
         }
         __syncthreads();  // red and zcs are reused by the next aggregate
     }
-    const int g = blockIdx.x;
+    const int g0 = blockIdx.x;
     if (tid == 0) {
-        ta.partial[(size_t)rz_slot * ta.T + g] = acc_rz;
-        ta.partial[(size_t)rr_slot * ta.T + g] = acc_rr;
-        if (g == 0 && count_iter) S.iters += 1;
+        ta.partial[(size_t)rz_slot * ta.T + g0] = acc_rz;
+        ta.partial[(size_t)rr_slot * ta.T + g0] = acc_rr;
+        if (g0 == 0 && count_iter) S.iters += 1;
     }
-    for (int t = g + G + tid * G; t < ta.T; t += WG * G) {
+    for (int t = g0 + G + tid * G; t < ta.T; t += WG * G) {
         ta.partial[(size_t)rz_slot * ta.T + t] = 0.0;
         ta.partial[(size_t)rr_slot * ta.T + t] = 0.0;
     }
@@ -1566,18 +1651,20 @@ void tl_release(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
     for (int32_t** p : {&tl.node_agg, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
-                        &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info})
+                        &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
+    if (tl.dcinv) { (void)hipFree(tl.dcinv); tl.dcinv = nullptr; }
     tl.built = tl.fresh = tl.failed = false;
 }
 
-// Aggregates: the nodes' (x, y) in square cells of side sqrt(area / target), so that every
-// aggregate holds whole columns of the tissue's layers; cells under 16 nodes join the nearest
-// (by centroid) cell that has 16 or more.  Numbered in the order of their cell keys.
-std::vector<int32_t> tl_aggregates(const std::vector<double>& x, int N, int target, int& n_agg)
+// Aggregates: the points' (x, y) (one point per node column) in square cells of side
+// sqrt(area / target), so that every aggregate holds whole columns of the tissue's layers; cells
+// under min_count points join the nearest (by centroid) cell that has min_count or more.
+// Numbered in the order of their cell keys.
+std::vector<int32_t> tl_aggregates(const std::vector<double>& x, int N, int target, int min_count, int& n_agg)
 {
     double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
     for (int n = 0; n < N; ++n)
@@ -1605,14 +1692,14 @@ std::vector<int32_t> tl_aggregates(const std::vector<double>& x, int N, int targ
     for (int k = 0; k < K; ++k) { cen[2 * k] /= cnt[k]; cen[2 * k + 1] /= cnt[k]; }
     std::vector<int32_t> remap(K);
     bool any_big = false;
-    for (int k = 0; k < K; ++k) any_big = any_big || cnt[k] >= 16;
+    for (int k = 0; k < K; ++k) any_big = any_big || cnt[k] >= min_count;
     for (int k = 0; k < K; ++k) {
         remap[k] = k;
         if (!any_big) { remap[k] = 0; continue; }
-        if (cnt[k] >= 16) continue;
+        if (cnt[k] >= min_count) continue;
         double best = INFINITY;
         for (int j = 0; j < K; ++j) {
-            if (cnt[j] < 16) continue;
+            if (cnt[j] < min_count) continue;
             const double dx = cen[2 * j] - cen[2 * k], dy = cen[2 * j + 1] - cen[2 * k + 1];
             const double d = dx * dx + dy * dy;
             if (d < best) { best = d; remap[k] = j; }
@@ -1699,9 +1786,50 @@ static void twolevel_build(tf_handle* h)
     int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(512, N / 66));
     target = std::min(target, max_nc / 6);
     int A = 0;
-    std::vector<int32_t> agg = tl_aggregates(x, N, target, A);
-    std::vector<std::vector<int32_t>> members(A);
-    for (int n = 0; n < N; ++n) members[agg[n]].push_back(n);
+    // Node columns: nodes joined by the vertical edges of prisms (corners k and k + 3), each
+    // listed in ascending node order.  Used when every column has the same number of nodes
+    // L <= TL_MAX_LEVELS; otherwise every node is a column of its own (L = 1, 3 x 3 blocks).
+    int L = 1;
+    std::vector<int32_t> root(N);
+    {
+        std::iota(root.begin(), root.end(), 0);
+        auto find = [&](int a) {
+            while (root[a] != a) { root[a] = root[root[a]]; a = root[a]; }
+            return a;
+        };
+        const std::vector<int32_t>& cn = h->h_conn;
+        for (int e = 0; e < h->E; ++e)
+            for (int k = 0; k < 3; ++k) {
+                const int u = find(cn[6 * e + k]), v = find(cn[6 * e + k + 3]);
+                if (u != v) root[std::max(u, v)] = std::min(u, v);  // a set's root is its smallest node
+            }
+        for (int n = 0; n < N; ++n) root[n] = find(n);
+        std::vector<int32_t> cnt(N, 0);
+        for (int n = 0; n < N; ++n) cnt[root[n]] += 1;
+        L = cnt[root[0]];
+        for (int n = 0; n < N && L > 0; ++n)
+            if (cnt[root[n]] != L) L = 0;
+        if (L < 1 || L > TL_MAX_LEVELS) {
+            L = 1;
+            std::iota(root.begin(), root.end(), 0);
+        }
+    }
+    std::vector<int32_t> col_of(N), col_start;  // column of every node; first node of every column
+    for (int n = 0; n < N; ++n)
+        if (root[n] == n) { col_of[n] = (int32_t)col_start.size(); col_start.push_back(n); }
+    for (int n = 0; n < N; ++n) col_of[n] = col_of[root[n]];
+    const int ncol = (int)col_start.size();
+    std::vector<std::vector<int32_t>> col_nodes(ncol);
+    for (int n = 0; n < N; ++n) col_nodes[col_of[n]].push_back(n);
+    // aggregates of whole columns, placed by each column's first node; cells under 16 nodes merge
+    std::vector<double> xc(3 * (size_t)ncol);
+    for (int c = 0; c < ncol; ++c)
+        for (int k = 0; k < 3; ++k) xc[3 * c + k] = x[3 * (size_t)col_start[c] + k];
+    std::vector<int32_t> aggc = tl_aggregates(xc, ncol, target, std::max(1, 16 / L), A);
+    std::vector<int32_t> agg(N);
+    for (int n = 0; n < N; ++n) agg[n] = aggc[col_of[n]];
+    std::vector<std::vector<int32_t>> members(A);  // column by column
+    for (int c = 0; c < ncol; ++c) members[aggc[c]].insert(members[aggc[c]].end(), col_nodes[c].begin(), col_nodes[c].end());
     std::vector<double> pcoef(18 * (size_t)N, 0.0);
     std::vector<int32_t> krank(A), coff(A + 1, 0);
     for (int a = 0; a < A; ++a) {
@@ -1716,6 +1844,17 @@ static void twolevel_build(tf_handle* h)
         agg_nodes.insert(agg_nodes.end(), members[a].begin(), members[a].end());
         agg_ptr[a + 1] = (int32_t)agg_nodes.size();
     }
+    // BSR block of every node pair of every column, in the columns' order in agg_nodes (-1: none)
+    std::vector<int32_t> col_blk((size_t)ncol * L * L, -1);
+    for (int g = 0; g < ncol; ++g)
+        for (int i = 0; i < L; ++i) {
+            const int ni = agg_nodes[(size_t)g * L + i];
+            for (int j = 0; j < L; ++j) {
+                const int nj = agg_nodes[(size_t)g * L + j];
+                for (int b = h->h_row_ptr[ni]; b < h->h_row_ptr[ni + 1]; ++b)
+                    if (h->h_col[b] == nj) { col_blk[((size_t)g * L + i) * L + j] = b; break; }
+            }
+        }
     // aggregate pairs with their Jacobian blocks in block order
     const std::vector<int32_t>& rp = h->h_row_ptr;
     const std::vector<int32_t>& cl = h->h_col;
@@ -1755,6 +1894,10 @@ static void twolevel_build(tf_handle* h)
     tl_upload(h, tl.pair_b, pair_b);
     tl_upload(h, tl.pair_ptr, pair_ptr);
     tl_upload(h, tl.pair_blk, pair_blk);
+    tl_upload(h, tl.col_blk, col_blk);
+    tl.dcinv = tl_alloc<float>((size_t)ncol * 9 * L * L);
+    tl.L = L;
+    tl.ncol = ncol;
     tl.agg_part = tl_alloc<double>(2 * (size_t)A);
     tl.rc = tl_alloc<double>(nc);
     tl.Ac = tl_alloc<double>((size_t)nc * nc);
@@ -1782,7 +1925,48 @@ static TLArgs tl_args(tf_handle* h)
     // k_tl_cz: aggregates g, g + G, ... in workgroup g, at most one workgroup per tile (the
     // partials it writes)
     return TLArgs{tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
-                  tl.Ainv, tl.nc, tl.n_agg, std::min(h->T, tl.n_agg)};
+                  tl.Ainv, tl.dcinv, tl.nc, tl.n_agg, std::min(h->T, tl.n_agg)};
+}
+
+// the two-level kernels at the handle's column length L (1..TL_MAX_LEVELS)
+template <int P, bool INIT>
+static void launch_tl_xr(tf_handle* h, const TileArgs& ta, const TLArgs& tla, const double* p, const double* q,
+                         double* x, double* r)
+{
+    const dim3 grid(tla.n_agg), blk(WG);
+    switch (h->tl.L) {
+    case 1: hipLaunchKernelGGL((k_tl_xr<P, INIT, 1>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
+    case 2: hipLaunchKernelGGL((k_tl_xr<P, INIT, 2>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
+    case 3: hipLaunchKernelGGL((k_tl_xr<P, INIT, 3>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
+    default: hipLaunchKernelGGL((k_tl_xr<P, INIT, 4>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
+    }
+}
+
+static void launch_tl_cz(tf_handle* h, const TileArgs& ta, const TLArgs& tla, int rz_slot, int rr_slot, int count)
+{
+    const dim3 grid(tla.G), blk(WG);
+    const size_t lds = tl_cz_lds(tla);
+    switch (h->tl.L) {
+    case 1: hipLaunchKernelGGL(k_tl_cz<1>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
+    case 2: hipLaunchKernelGGL(k_tl_cz<2>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
+    case 3: hipLaunchKernelGGL(k_tl_cz<3>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
+    default: hipLaunchKernelGGL(k_tl_cz<4>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
+    }
+}
+
+// inverse column blocks of the current Jacobian (every solve)
+static void launch_tl_dcol(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    const int cpw = 64 / (3 * tl.L), waves = (tl.ncol + cpw - 1) / cpw;
+    const dim3 grid((waves + WG / 64 - 1) / (WG / 64)), blk(WG);
+    switch (tl.L) {
+    case 1: hipLaunchKernelGGL(k_tl_dcol<1>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
+    case 2: hipLaunchKernelGGL(k_tl_dcol<2>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
+    case 3: hipLaunchKernelGGL(k_tl_dcol<3>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
+    default: hipLaunchKernelGGL(k_tl_dcol<4>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
+    }
+    TF_HIP(hipGetLastError());
 }
 
 // Ac = P^T M P from the FP64 Jacobian, its Cholesky factor and inverse (rocSOLVER), symmetrised.
@@ -1905,10 +2089,8 @@ void enqueue_tl_iter(tf_handle* h, const VT* val)
     timer_begin(h, h->t_spmv, ev);
     launch_cg_q<VT, P>(h, val, h->wg);
     timer_end(h, h->t_spmv, ev);
-    hipLaunchKernelGGL((k_tl_xr<P, false>), dim3(tla.n_agg), dim3(WG), 0, h->stream, ta, tla, h->dinv, p_new, h->kv,
-                       h->kx, h->kr);
-    hipLaunchKernelGGL(k_tl_cz, dim3(tla.G), blk, tl_cz_lds(tla), h->stream, ta, tla, h->dinv, h->kr,
-                       h->ks, SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
+    launch_tl_xr<P, false>(h, ta, tla, p_new, h->kv, h->kx, h->kr);
+    launch_tl_cz(h, ta, tla, SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
 }
 
 // Single-kernel CG buffers: x = kx, p = kp, r = kr, w = kv, s = ks (own rows only) and the
@@ -2029,10 +2211,8 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
                        h->seg_rtol_on ? h->seg_rtol2 : nullptr);
     if (kind == KIND_TL) {  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
         const TLArgs tla = tl_args(h);
-        hipLaunchKernelGGL((k_tl_xr<0, true>), dim3(tla.n_agg), dim3(WG), 0, h->stream, ta, tla, h->dinv, h->kp, h->kv,
-                           h->kx, h->kr);
-        hipLaunchKernelGGL(k_tl_cz, dim3(tla.G), blk, tl_cz_lds(tla), h->stream, ta, tla, h->dinv,
-                           h->kr, h->ks, (int)SL_RZ0, (int)SL_RR0, 0);
+        launch_tl_xr<0, true>(h, ta, tla, h->kp, h->kv, h->kx, h->kr);
+        launch_tl_cz(h, ta, tla, (int)SL_RZ0, (int)SL_RR0, 0);
     }
     TF_HIP(hipGetLastError());
     // A captured graph holds at most TF_GRAPH_NODES kernel launches (an even number of
@@ -2149,6 +2329,7 @@ static bool twolevel_prepare(tf_handle* h)
     if (!tl.fresh || h->krylov_call < 0 ||
         (h->krylov_call == 0 && (tl.stale || tl.age >= std::max(1, h->opts.coarse_refresh))))
         twolevel_refresh(h);
+    if (!tl.failed) launch_tl_dcol(h);
     return !tl.failed;
 }
 
