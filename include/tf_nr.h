@@ -100,7 +100,7 @@ typedef struct tf_solver_opts {
                                   0: tf_timestep clears the pair list                          */
     int32_t preconditioner;    /* TF_PRECOND_*: the preconditioner of the CG solvers             */
     int32_t coarse_aggregates; /* TF_PRECOND_TWO_LEVEL: target number of node aggregates of the
-                                  coarse space (0: one per 66 nodes, at most 512)               */
+                                  coarse space (0: one per 53 nodes, at most 600)               */
     int32_t coarse_refresh;    /* TF_PRECOND_TWO_LEVEL: most timesteps between rebuilds of the
                                   coarse operator (0 or 1: every timestep; default 100)         */
 } tf_solver_opts;

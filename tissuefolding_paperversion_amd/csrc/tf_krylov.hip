@@ -677,7 +677,7 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
 // first aggregate's rows of Ac^-1 and its first DoF's r, Dc^-1 row and P row before r_c is staged
 // in LDS.  Workgroup g writes the tile partials g of slots rz_slot ((r, Dc^-1 r) + (r_c, z_c)
 // over its aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
-constexpr int TL_CZ_COLS = 12;  // columns per lane of every coarse row held in registers (nc <= 3072)
+constexpr int TL_CZ_COLS = 16;  // columns per lane of every coarse row held in registers (nc <= 4096)
 
 template <int L>
 __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ r,
@@ -1781,9 +1781,9 @@ static void twolevel_build(tf_handle* h)
     TF_HIP(hipStreamSynchronize(h->stream));
     for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
     const int max_nc = std::min(4096, h->max_lds / (int)sizeof(double));
-    // default: one aggregate per 66 nodes, at most 512 (at most 3,072 coarse unknowns, the columns
-    // k_tl_cz holds in registers; the dense FP32 inverse every iteration reads stays under 38 MB)
-    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(512, N / 66));
+    // default: one aggregate per 53 nodes, at most 600 (at most 3,600 coarse unknowns; the dense
+    // FP32 inverse every iteration reads stays under 52 MB, and its rebuild under ~15 ms)
+    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(600, N / 53));
     target = std::min(target, max_nc / 6);
     int A = 0;
     // Node columns: nodes joined by the vertical edges of prisms (corners k and k + 3), each
