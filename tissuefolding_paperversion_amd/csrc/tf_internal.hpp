@@ -250,7 +250,9 @@ struct tf_handle {
         int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
         double *agg_part = nullptr, *rc = nullptr;          // [A][2], [nc]
         double* Ac = nullptr;
-        float* Ainv = nullptr;                              // [nc][nc] FP32 copy of Ac^-1
+        void* Ainv = nullptr;                               // [nc][ldA] FP16 copy of Ac^-1 / scale
+        double* ascale = nullptr;                           // [2] scale and its reciprocal
+        int32_t ldA = 0;                                    // row stride of Ainv (halves)
         int32_t L = 1, ncol = 0;                            // nodes per column, columns
         int32_t* col_blk = nullptr;                         // [ncol][L][L] BSR block of every node pair or -1
         float* dcinv = nullptr;                             // [ncol][3L][3L] FP32 inverse column blocks

@@ -19,6 +19,7 @@
 //   iteration: the direction update p = z + beta p_old is folded into the SpMV, which forms
 //   p for neighbouring rows on the fly and writes it for its own rows (ping-pong buffers).
 #include "tf_internal.hpp"
+#include <hip/hip_fp16.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 #include <algorithm>
@@ -524,9 +525,11 @@ struct TLArgs {
     const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
-    double* agg_part;          // [A][2] (r, D^-1 r) and (r, r) over the aggregate's nodes
+    double* agg_part;          // [A][2] (r, Dc^-1 r) and (r, r) over the aggregate's nodes
     double* rc;                // [nc] restricted residual P^T r
-    const float* Ainv;         // [nc][nc] (a preconditioner: FP32 storage, FP64 sums)
+    const __half2* Ainv;       // [nc][ldA / 2] Ac^-1 / scale in FP16 pairs (a preconditioner; FP64 sums)
+    const double* ascale;      // [2] scale (the largest diagonal entry of Ac^-1) and its reciprocal
+    int ldA;                   // row stride of Ainv in halves (nc rounded up to 8; zero padding)
     const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
     int nc, n_agg, G;          // G: workgroups of k_tl_cz
 };
@@ -673,11 +676,11 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
 
 // z_c = Ac^-1 r_c on the coarse rows of the workgroup's aggregates (g, g + G, ...), then
 // z = Dc^-1 r + P z_c on their nodes (one lane per DoF of the aggregate's columns, WG / 3L columns
-// per pass, the column's r exchanged in LDS).  Each lane loads its columns (t, t + WG, ...) of the
-// first aggregate's rows of Ac^-1 and its first DoF's r, Dc^-1 row and P row before r_c is staged
+// per pass, the column's r exchanged in LDS).  Each lane loads its column pairs (t, t + WG, ...) of
+// the first aggregate's rows of Ac^-1 and its first DoF's r, Dc^-1 row and P row before r_c is staged
 // in LDS.  Workgroup g writes the tile partials g of slots rz_slot ((r, Dc^-1 r) + (r_c, z_c)
 // over its aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
-constexpr int TL_CZ_COLS = 16;  // columns per lane of every coarse row held in registers (nc <= 4096)
+constexpr int TL_CZ_PAIRS = 8;  // column pairs per lane of every coarse row held in registers (nc <= 4096)
 
 template <int L>
 __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ r,
@@ -686,14 +689,14 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
     constexpr int D = 3 * L, CPP = WG / D;
     SegScalars& S = ta.seg[0];
     if (seg_done(S)) return;
-    extern __shared__ double rcs[];  // r_c, zero-padded to max(nc, WG * TL_CZ_COLS)
+    extern __shared__ double rcs[];  // r_c, zero-padded to max(ldA, 2 WG TL_CZ_PAIRS)
     constexpr int NW = WG / 64;
     __shared__ double red[6][NW];
     __shared__ double zcs[6];
     __shared__ double rl[WG];
     const int tid = threadIdx.x, w = tid / 64, l = tid % 64;
     const int slot = tid / D, li = tid % D;
-    const int nc = tl.nc, G = gridDim.x;
+    const int nc = tl.nc, ld2 = tl.ldA / 2, G = gridDim.x;
     double acc_rz = 0.0, acc_rr = 0.0;  // lane 0
     bool first = true;
     for (int a = blockIdx.x; a < tl.n_agg; a += G) {
@@ -702,12 +705,12 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
         // Unconditional loads at clamped addresses (a load under a condition is issued in a branch
         // of its own and waited for before the next): rows past the rank are summed and dropped,
         // columns past nc meet the zeros that pad r_c in LDS.
-        float pre[6][TL_CZ_COLS];
+        __half2 pre[6][TL_CZ_PAIRS];
 #pragma unroll
         for (int k = 0; k < 6; ++k)
 #pragma unroll
-            for (int m = 0; m < TL_CZ_COLS; ++m)
-                pre[k][m] = tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + min(tid + WG * m, nc - 1)];
+            for (int m = 0; m < TL_CZ_PAIRS; ++m)
+                pre[k][m] = tl.Ainv[(size_t)min(c0 + k, nc - 1) * ld2 + min(tid + WG * m, ld2 - 1)];
         // the lane's DoF of the first pass
         bool own = slot < CPP && slot < ncols;
         int g = n0 / L + (own ? slot : 0);
@@ -721,7 +724,7 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
 #pragma unroll
         for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
         if (first) {
-            for (int i = tid; i < max(nc, WG * TL_CZ_COLS); i += WG) rcs[i] = i < nc ? tl.rc[i] : 0.0;
+            for (int i = tid; i < max(2 * ld2, 2 * WG * TL_CZ_PAIRS); i += WG) rcs[i] = i < nc ? tl.rc[i] : 0.0;
             __syncthreads();
             first = false;
         }
@@ -730,11 +733,18 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
         for (int k = 0; k < 6; ++k) {
             sv[k] = 0.0;
 #pragma unroll
-            for (int m = 0; m < TL_CZ_COLS; ++m) sv[k] += (double)pre[k][m] * rcs[tid + WG * m];
+            for (int m = 0; m < TL_CZ_PAIRS; ++m) {
+                const float2 f = __half22float2(pre[k][m]);
+                const int j = 2 * (tid + WG * m);
+                sv[k] += (double)f.x * rcs[j] + (double)f.y * rcs[j + 1];
+            }
         }
-        for (int j = tid + WG * TL_CZ_COLS; j < nc; j += WG)
+        for (int jp = tid + WG * TL_CZ_PAIRS; jp < ld2; jp += WG)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) sv[k] += (double)tl.Ainv[(size_t)min(c0 + k, nc - 1) * nc + j] * rcs[j];
+            for (int k = 0; k < 6; ++k) {
+                const float2 f = __half22float2(tl.Ainv[(size_t)min(c0 + k, nc - 1) * ld2 + jp]);
+                sv[k] += (double)f.x * rcs[2 * jp] + (double)f.y * rcs[2 * jp + 1];
+            }
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const double t = wave_sum(sv[k]);
@@ -745,7 +755,7 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
             double t = 0.0;
 #pragma unroll
             for (int k = 0; k < NW; ++k) t += red[tid][k];
-            zcs[tid] = tid < kr ? t : 0.0;  // P's columns past the rank are zero
+            zcs[tid] = tid < kr ? t * tl.ascale[0] : 0.0;  // P's columns past the rank are zero
         }
         __syncthreads();
         for (int cb = 0; cb < ncols; cb += CPP) {
@@ -790,7 +800,7 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
     }
 }
 
-static size_t tl_cz_lds(const TLArgs& tl) { return sizeof(double) * (size_t)std::max(tl.nc, WG * TL_CZ_COLS); }
+static size_t tl_cz_lds(const TLArgs& tl) { return sizeof(double) * (size_t)std::max(tl.ldA, 2 * WG * TL_CZ_PAIRS); }
 
 // r = b - M x0 (x0 given) or b, x = x0 or 0, p_old = 0, and (b, b)
 template <typename VT>
@@ -879,12 +889,36 @@ __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restri
 
 // the full symmetric inverse, rounded to FP32, from the triangle the Cholesky inverse leaves
 // (column-major lower: element (i, j), i >= j, at Ac[j * n + i])
-__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ Ac, float* __restrict__ Ainv, int n)
+// The scale of the FP16 copy: the largest diagonal entry of Ac^-1, which bounds every entry of a
+// positive definite matrix (|a_ij| <= sqrt(a_ii a_jj)), so the scaled entries lie in [-1, 1].
+__global__ __launch_bounds__(256) void k_tl_scale(const double* __restrict__ Ac, int n, double* __restrict__ sc)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, Ac[(size_t)i * n + i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double s = sh[0] > 0.0 && isfinite(sh[0]) ? sh[0] : 1.0;
+        sc[0] = s;
+        sc[1] = 1.0 / s;
+    }
+}
+
+// Symmetric FP16 copy of Ac^-1 / scale (rocSOLVER leaves the inverse in the lower triangle of the
+// column-major Ac), rows of ld halves with zeros past column n.
+__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ Ac, __half* __restrict__ Ainv, int n, int ld,
+                                                const double* __restrict__ sc)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)n * n) return;
-    const int i = (int)(k / n), j = (int)(k % n);
-    Ainv[k] = (float)(i >= j ? Ac[(size_t)j * n + i] : Ac[(size_t)i * n + j]);
+    if (k >= (size_t)n * ld) return;
+    const int i = (int)(k / ld), j = (int)(k % ld);
+    const double v = j < n ? (i >= j ? Ac[(size_t)j * n + i] : Ac[(size_t)i * n + j]) * sc[1] : 0.0;
+    Ainv[k] = __float2half((float)v);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1656,6 +1690,7 @@ void tl_release(tf_handle* h)
     for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
+    if (tl.ascale) { (void)hipFree(tl.ascale); tl.ascale = nullptr; }
     if (tl.dcinv) { (void)hipFree(tl.dcinv); tl.dcinv = nullptr; }
     tl.built = tl.fresh = tl.failed = false;
 }
@@ -1901,7 +1936,9 @@ static void twolevel_build(tf_handle* h)
     tl.agg_part = tl_alloc<double>(2 * (size_t)A);
     tl.rc = tl_alloc<double>(nc);
     tl.Ac = tl_alloc<double>((size_t)nc * nc);
-    tl.Ainv = tl_alloc<float>((size_t)nc * nc);
+    tl.ldA = (nc + 7) / 8 * 8;
+    tl.Ainv = tl_alloc<__half>((size_t)nc * tl.ldA);
+    tl.ascale = tl_alloc<double>(2);
     tl.info = tl_alloc<int32_t>(1);
     if (!tl.blas) {
         rocblas_handle bh = nullptr;
@@ -1925,7 +1962,8 @@ static TLArgs tl_args(tf_handle* h)
     // k_tl_cz: aggregates g, g + G, ... in workgroup g, at most one workgroup per tile (the
     // partials it writes)
     return TLArgs{tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
-                  tl.Ainv, tl.dcinv, tl.nc, tl.n_agg, std::min(h->T, tl.n_agg)};
+                  reinterpret_cast<const __half2*>(tl.Ainv), tl.ascale, tl.ldA, tl.dcinv, tl.nc, tl.n_agg,
+                  std::min(h->T, tl.n_agg)};
 }
 
 // the two-level kernels at the handle's column length L (1..TL_MAX_LEVELS)
@@ -1993,8 +2031,10 @@ static void twolevel_refresh(tf_handle* h)
     if (tl.failed) return;
     if (rocsolver_dpotri(bh, rocblas_fill_lower, nc, tl.Ac, nc, tl.info) != rocblas_status_success)
         throw std::runtime_error("two-level preconditioner: rocsolver_dpotri failed");
-    const size_t n2 = (size_t)nc * nc;
-    hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, tl.Ainv, nc);
+    hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, tl.Ac, nc, tl.ascale);
+    const size_t n2h = (size_t)nc * tl.ldA;
+    hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, (__half*)tl.Ainv, nc,
+                       tl.ldA, tl.ascale);
     TF_HIP(hipGetLastError());
 }
 
