@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--coarse-aggregates", type=int, default=0)
     p.add_argument("--coarse-refresh", type=int, default=100)
     p.add_argument("--no-kernel-timing", action="store_true", help="skip the HIP-event timing of the dominant kernel")
+    p.add_argument("--tiling", type=int, nargs=2, metavar=("WG", "TILE_BLOCKS"),
+                   help="Krylov tiling of the plain FP64 run instead of the tiling search (profiling runs)")
     return p.parse_args()
 
 
@@ -100,21 +102,26 @@ def elastic_forces(s):
     return s.gSum
 
 
-def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing, tune=True):
+def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing, tune=True, tiling=None):
     """The workload's timesteps from its initial state on a fresh handle.  Outside the timed
     region, a second handle runs the warm-up steps (loading every kernel) and, from that state,
-    the tiling search; the timed handle takes its tiling before its first step."""
+    the tiling search (or takes the given tiling); the timed handle takes its tiling before its
+    first step."""
     dt = tissue.dt
     t = NewtonRaphsonSolver(tissue, opts)
+    if tiling:
+        t.set_tiling(*tiling)
     for _ in range(warmup):
         t.runOneStep(dt)
     wg = cap = None
-    if tune:
+    if tiling:
+        wg, cap = tiling
+    elif tune:
         t.autotune(dt, n_iters=64)
         wg, cap, _ = t.tiling()
     t.close()
     s = NewtonRaphsonSolver(tissue, opts)
-    if tune:
+    if wg is not None:
         s.set_tiling(wg, cap)
     for k in ("assemble_elements", "assemble_gather"):  # eager launches, events around each
         s.kernel_timing(k, enable=timing)
@@ -225,7 +232,7 @@ def main():
     opts["coarse_aggregates"] = a.coarse_aggregates
     opts["coarse_refresh"] = a.coarse_refresh
     timing = 0 if a.no_kernel_timing else 1
-    s, stats, sec, x_plain = run_timed(NewtonRaphsonSolver, tissue, opts, a.warmup, a.steps, timing)
+    s, stats, sec, x_plain = run_timed(NewtonRaphsonSolver, tissue, opts, a.warmup, a.steps, timing, tiling=a.tiling)
     # the dominant Krylov kernel: 64 launches back to back on the final Jacobian, right after the
     # timed steps (HIP events around one graph replay; the timed steps replay their Krylov
     # chunks as graphs with nothing between the launches)
