@@ -103,6 +103,10 @@ typedef struct tf_solver_opts {
                                   coarse space (0: one per 53 nodes, at most 600)               */
     int32_t coarse_refresh;    /* TF_PRECOND_TWO_LEVEL: most timesteps between rebuilds of the
                                   coarse operator (0 or 1: every timestep; default 100)         */
+    int32_t column_blocks;     /* TF_PRECOND_TWO_LEVEL smoother: 1 the blocks of the node columns
+                                  (nodes joined by prisms' vertical edges) when every column has
+                                  the same 1-4 nodes, else 3x3 nodal blocks (default); 0 always
+                                  3x3 nodal blocks                                              */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

@@ -78,6 +78,24 @@ def test_aggregate_option_rebuilds_coarse_space():
     assert rel_err(dus[7], dus[0]) < 1e-8
 
 
+@pytest.mark.parametrize("make", [_ellipse, lambda: configs.disc_tissue(radius=30.0)])
+def test_column_blocks_and_nodal_blocks_agree(make):
+    """The smoother on node columns (4 nodes on the ellipse, 3 on the two-layer disc) and the
+    3x3 nodal one (column_blocks = 0) solve the same system to the same answer; the columns take
+    fewer iterations."""
+    t = make()
+    du, its = {}, {}
+    for cb in (1, 0):
+        s = _assembled(t, column_blocks=cb, **TL)
+        du[cb] = s.solveForDeltaU()
+        its[cb] = s.last_krylov["iterations"]
+        assert s.last_krylov["rel_residual"] <= 1e-12
+    ref = spla.spsolve(s.jacobian_csr().tocsc(), s.gSum)
+    print(f"{t.name}: column blocks {its[1]} its, nodal blocks {its[0]} its", flush=True)
+    assert rel_err(du[1], ref) < 1e-8 and rel_err(du[0], ref) < 1e-8
+    assert its[1] < its[0]
+
+
 def test_bound_slaves_follow_masters():
     t = configs.disc_tissue(radius=30.0)
     pairs = np.asarray(bound_pairs(t))
@@ -149,3 +167,5 @@ def test_option_validation():
         s.set_options(preconditioner=2)
     with pytest.raises(L.TfError):
         s.set_options(coarse_aggregates=-1)
+    with pytest.raises(L.TfError):
+        s.set_options(column_blocks=2)

@@ -478,6 +478,7 @@ void tf_default_options(tf_solver_opts* o)
     o->implicit_packing = 1;
     o->preconditioner = TF_PRECOND_AUTO;
     o->coarse_refresh = 100;
+    o->column_blocks = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -673,10 +674,11 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16 ||
         (o->implicit_packing != 0 && o->implicit_packing != 1) ||
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
-        o->coarse_aggregates < 0 || o->coarse_refresh < 0)
+        o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1))
         return fail(h, TF_INVALID, "invalid solver options");
     if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
-    const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates;
+    const bool coarse_changed =
+        o->coarse_aggregates != h->opts.coarse_aggregates || o->column_blocks != h->opts.column_blocks;
     h->opts = *o;
     drop_krylov_graphs(h);
     TF_GUARD_BEGIN

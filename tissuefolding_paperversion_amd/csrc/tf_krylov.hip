@@ -1822,8 +1822,9 @@ static void twolevel_build(tf_handle* h)
     target = std::min(target, max_nc / 6);
     int A = 0;
     // Node columns: nodes joined by the vertical edges of prisms (corners k and k + 3), each
-    // listed in ascending node order.  Used when every column has the same number of nodes
-    // L <= TL_MAX_LEVELS; otherwise every node is a column of its own (L = 1, 3 x 3 blocks).
+    // listed in ascending node order.  Used when column_blocks is on and every column has the same
+    // number of nodes L <= TL_MAX_LEVELS; otherwise every node is a column of its own (L = 1,
+    // 3 x 3 blocks).
     int L = 1;
     std::vector<int32_t> root(N);
     {
@@ -1841,7 +1842,7 @@ static void twolevel_build(tf_handle* h)
         for (int n = 0; n < N; ++n) root[n] = find(n);
         std::vector<int32_t> cnt(N, 0);
         for (int n = 0; n < N; ++n) cnt[root[n]] += 1;
-        L = cnt[root[0]];
+        L = h->opts.column_blocks ? cnt[root[0]] : 0;
         for (int n = 0; n < N && L > 0; ++n)
             if (cnt[root[n]] != L) L = 0;
         if (L < 1 || L > TL_MAX_LEVELS) {
