@@ -90,6 +90,7 @@ void free_all(tf_handle* h)
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
     if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->hp_rtol2) (void)hipHostFree(h->hp_rtol2);
     drop_krylov_graphs(h);
     for (KernelTimer* t : {&h->t_elem, &h->t_gather, &h->t_spmv}) {
         for (hipEvent_t e : t->pending) (void)hipEventDestroy(e);
@@ -579,6 +580,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
+        TF_HIP(hipHostMalloc((void**)&h->hp_rtol2, sizeof(double) * S, hipHostMallocDefault));
         for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx, &h->km, &h->kq})
             *v = dev_alloc<double>((size_t)3 * N);
 
@@ -1046,7 +1048,9 @@ static void forcing_tolerances(tf_handle* h, int iter)
         }
         h->h_seg_rtol2[s] = eta * eta;
     }
-    h2d(h, h->seg_rtol2, h->h_seg_rtol2.data(), h->S);
+    // pinned staging, no host wait (the previous upload ended at the last solve's waits)
+    std::copy(h->h_seg_rtol2.begin(), h->h_seg_rtol2.end(), h->hp_rtol2);
+    TF_HIP(hipMemcpyAsync(h->seg_rtol2, h->hp_rtol2, sizeof(double) * h->S, hipMemcpyHostToDevice, h->stream));
 }
 
 static void forcing_observe(tf_handle* h, int iter)
@@ -1079,30 +1083,35 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     while (true) {
         auto ta = std::chrono::steady_clock::now();
         assemble_impl(h, dt);
-        // the det Fe status comes back with the warm start's own read-back (one host wait)
+        // the det Fe status comes back ahead of the warm start's read-back (one host wait)
         TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
         ms_asm += ms_since(ta);
         auto ts = std::chrono::steady_clock::now();
         h->seg_rtol_on = h->opts.krylov_forcing != 0;
         if (h->seg_rtol_on) forcing_tolerances(h, it);
         const bool warm = h->opts.krylov_warm_start != 0 && it < TF_WARM_ITERS;
-        h->warm_guess = (warm && warm_start_guess(h, it)) ? h->warm_x0 : nullptr;  // waits when it reads back
-        TF_HIP(hipStreamSynchronize(h->stream));  // returns at once after a warm start's read-back
+        const bool wpend = warm && warm_start_begin(h, it);
+        h->krylov_call = it;
+        krylov_pre_solve(h);  // device work that hides the host's wait for the read-back
+        if (wpend) h->warm_guess = warm_start_finish(h, it) ? h->warm_x0 : nullptr;
+        else TF_HIP(hipStreamSynchronize(h->stream));  // the status
         if (h->h_status[0] > 0) {
             h->warm_guess = nullptr;
             h->seg_rtol_on = false;
+            h->tl_pre_valid = false;
+            h->krylov_call = -1;
             status = TF_NEG_JACOBIAN;
             h->err = std::to_string(h->h_status[0]) + " Gauss points with det Fe <= 0";
             break;
         }
         int kst;
-        h->krylov_call = it;
         try {
             kst = solve_impl(h);
         } catch (...) {
             h->seg_rtol_on = false;
             h->warm_guess = nullptr;
             h->krylov_call = -1;
+            h->tl_pre_valid = false;
             throw;
         }
         h->seg_rtol_on = false;

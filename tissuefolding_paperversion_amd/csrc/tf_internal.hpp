@@ -205,6 +205,9 @@ struct tf_handle {
     // Krylov relative tolerance (device copy read by the Krylov set-up while seg_rtol_on)
     std::vector<double> fc_C, fc_du0, fc_du_prev, h_seg_rtol2;
     double *seg_rtol2 = nullptr;      // [S]
+    double* hp_rtol2 = nullptr;       // [S] pinned staging of h_seg_rtol2 (uploaded without a host wait)
+    // the two-level prepare (column inverses, coarse refresh) already enqueued for the next solve
+    bool tl_pre_valid = false, tl_pre_ok = false;
     bool seg_rtol_on = false;
     // Krylov warm start: for Newton iteration k < TF_WARM_ITERS, the updates that iteration took
     // in the last TF_WARM_HIST timesteps (warm_hist[k][0] the newest).  A solve starts from their
@@ -217,6 +220,9 @@ struct tf_handle {
     int32_t warm_part_T = 0;
     double* warm_seg = nullptr;       // [S][TF_WARM_DOTS]
     double* warm_y = nullptr;         // [S][TF_WARM_HIST]
+    double* hp_warm_dots = nullptr;   // [S][TF_WARM_DOTS] pinned read-back of warm_seg
+    double* hp_warm_y = nullptr;      // [S][TF_WARM_HIST] pinned staging of warm_y
+    hipEvent_t warm_ev = nullptr;     // recorded after the read-back of warm_seg
     const double* warm_guess = nullptr;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
     // Krylov iterations the solve of each Newton iteration took in the previous timestep: the
@@ -308,7 +314,9 @@ void launch_accumulate(tf_handle* h, const double* a, double* b, int64_t n);
 int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
 // Warm start of the solve of Newton iteration k: x0 in warm_x0 from the kept updates; false when
 // there is none (or the least-squares problem is degenerate for every tissue).
-bool warm_start_guess(tf_handle* h, int k);
+bool warm_start_begin(tf_handle* h, int k);   // Gram kernels and their read-back (event), no wait
+bool warm_start_finish(tf_handle* h, int k);  // waits for the read-back only; coefficients, x0 (no wait)
+void krylov_pre_solve(tf_handle* h);          // enqueues the next solve's two-level prepare
 void warm_start_push(tf_handle* h, int k);    // keeps du as the newest update of iteration k
 void warm_start_free(tf_handle* h);
 void warm_start_clear(tf_handle* h);

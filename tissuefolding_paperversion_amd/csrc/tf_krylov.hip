@@ -1586,7 +1586,7 @@ bool small_lstsq(int m, const double* Gup, const double* r, double* y)
 
 }  // namespace
 
-bool warm_start_guess(tf_handle* h, int k)
+bool warm_start_begin(tf_handle* h, int k)
 {
     const int m = h->warm_count[k];
     if (m <= 0) return false;
@@ -1595,6 +1595,9 @@ bool warm_start_guess(tf_handle* h, int k)
         TF_HIP(hipMalloc((void**)&h->warm_x0, n3 * sizeof(double)));
         TF_HIP(hipMalloc((void**)&h->warm_seg, sizeof(double) * TF_WARM_DOTS * h->S));
         TF_HIP(hipMalloc((void**)&h->warm_y, sizeof(double) * TF_WARM_HIST * h->S));
+        TF_HIP(hipHostMalloc((void**)&h->hp_warm_dots, sizeof(double) * TF_WARM_DOTS * h->S, hipHostMallocDefault));
+        TF_HIP(hipHostMalloc((void**)&h->hp_warm_y, sizeof(double) * TF_WARM_HIST * h->S, hipHostMallocDefault));
+        TF_HIP(hipEventCreateWithFlags(&h->warm_ev, hipEventDisableTiming));
     }
     if (!h->warm_part || h->warm_part_T != h->T) {  // tiles change with the tiling
         if (h->warm_part) (void)hipFree(h->warm_part);
@@ -1615,24 +1618,39 @@ bool warm_start_guess(tf_handle* h, int k)
                            h->nb, h->warm_part);
     hipLaunchKernelGGL(k_warm_seg, dim3(h->S, TF_WARM_DOTS), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
     TF_HIP(hipGetLastError());
-    std::vector<double> dots((size_t)TF_WARM_DOTS * h->S), y((size_t)TF_WARM_HIST * h->S, 0.0);
-    TF_HIP(hipMemcpyAsync(dots.data(), h->warm_seg, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, h->stream));
-    TF_HIP(hipStreamSynchronize(h->stream));
+    TF_HIP(hipMemcpyAsync(h->hp_warm_dots, h->warm_seg, sizeof(double) * TF_WARM_DOTS * h->S, hipMemcpyDeviceToHost,
+                          h->stream));
+    TF_HIP(hipEventRecord(h->warm_ev, h->stream));
+    return true;
+}
+
+bool warm_start_finish(tf_handle* h, int k)
+{
+    const int m = h->warm_count[k];
+    TF_HIP(hipEventSynchronize(h->warm_ev));  // the read-back, not the work queued after it
     bool any = false;
     for (int s = 0; s < h->S; ++s) {
-        const double* D = dots.data() + (size_t)s * TF_WARM_DOTS;
+        const double* D = h->hp_warm_dots + (size_t)s * TF_WARM_DOTS;
         const double* rhs = D + TF_WARM_HIST * (TF_WARM_HIST + 1) / 2;
         double ys[TF_WARM_HIST] = {0.0, 0.0, 0.0, 0.0};
+        double* y = h->hp_warm_y + (size_t)s * TF_WARM_HIST;
+        for (int j = 0; j < TF_WARM_HIST; ++j) y[j] = 0.0;
         if (small_lstsq(m, D, rhs, ys)) {
-            for (int j = 0; j < m; ++j) y[(size_t)s * TF_WARM_HIST + j] = ys[j];
+            for (int j = 0; j < m; ++j) y[j] = ys[j];
             any = true;
         }
     }
     if (!any) return false;
-    TF_HIP(hipMemcpyAsync(h->warm_y, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, h->stream));
+    // pinned staging: the upload overlaps the host, and the previous one ended at the last solve's waits
+    TF_HIP(hipMemcpyAsync(h->warm_y, h->hp_warm_y, sizeof(double) * TF_WARM_HIST * h->S, hipMemcpyHostToDevice,
+                          h->stream));
+    TileArgs ta = tile_args(h);
+    WarmArgs wa{};
+    for (int j = 0; j < m; ++j) wa.z[j] = h->warm_hist[k][j];
+    wa.g = h->g;
+    wa.m = m;
     hipLaunchKernelGGL(k_warm_combine, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_y, h->warm_x0);
     TF_HIP(hipGetLastError());
-    TF_HIP(hipStreamSynchronize(h->stream));  // y is a stack temporary
     return true;
 }
 
@@ -1659,6 +1677,9 @@ void warm_start_free(tf_handle* h)
             if (h->warm_hist[k][j]) { (void)hipFree(h->warm_hist[k][j]); h->warm_hist[k][j] = nullptr; }
     for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg, &h->warm_y})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (double** p : {&h->hp_warm_dots, &h->hp_warm_y})
+        if (*p) { (void)hipHostFree(*p); *p = nullptr; }
+    if (h->warm_ev) { (void)hipEventDestroy(h->warm_ev); h->warm_ev = nullptr; }
     warm_start_clear(h);
 }
 
@@ -2509,6 +2530,19 @@ double krylov_probe(tf_handle* h, int n_launches)
     return ms;
 }
 
+// The two-level prepare of the next krylov_solve, enqueued early (behind the warm start's
+// read-back, so that the device has work while the host waits for it).
+void krylov_pre_solve(tf_handle* h)
+{
+    h->tl_pre_valid = false;
+    int kind = h->opts.krylov;
+    if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+    const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
+    if (!cg_kind || !twolevel_active(h)) return;
+    h->tl_pre_ok = twolevel_prepare(h);
+    h->tl_pre_valid = true;
+}
+
 // Solve M du = g for every active tissue; du is written to h->du.
 int krylov_solve(tf_handle* h)
 {
@@ -2516,7 +2550,9 @@ int krylov_solve(tf_handle* h)
     const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
     auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
-    if (cg_kind && twolevel_active(h) && twolevel_prepare(h)) {
+    const bool pre = h->tl_pre_valid;
+    h->tl_pre_valid = false;
+    if (cg_kind && twolevel_active(h) && (pre ? h->tl_pre_ok : twolevel_prepare(h))) {
         kind = KIND_TL;
         h->tl_used = true;
     }
