@@ -223,11 +223,23 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ blk_
     if (blk >= nb) return;
     const int A = blk_row[blk], B = col[blk];
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int k1 = blk_ptr[blk + 1];
-    for (int k = blk_ptr[blk]; k < k1; ++k) {
-        const double* p = kelem + (size_t)blk_src[k] * 9;
+    const int k0 = blk_ptr[blk], k1 = blk_ptr[blk + 1];
+    // four contributors per trip: their indices (clamped, always valid), then their 36 values are
+    // loaded together, and the sums run in the contributors' order
+    for (int k = k0; k < k1; k += 4) {
+        int src[4];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] += p[q];
+        for (int u = 0; u < 4; ++u) src[u] = blk_src[min(k + u, k1 - 1)];
+        double v[4][9];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) v[u][q] = kelem[(size_t)src[u] * 9 + q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < k1)
+#pragma unroll
+                for (int q = 0; q < 9; ++q) acc[q] += v[u][q];
     }
     if (A == B) {
         const double s = sa[A] * inv_dt;
@@ -327,10 +339,20 @@ __global__ __launch_bounds__(256) void k_residual(const int32_t* __restrict__ no
     const int A = blockIdx.x * blockDim.x + threadIdx.x;
     if (A >= N) return;
     double acc[3] = {0, 0, 0};
-    const int k1 = node_ptr[A + 1];
-    for (int k = node_ptr[A]; k < k1; ++k) {
-        const double* p = felem + (size_t)node_src[k] * 3;
-        acc[0] += p[0]; acc[1] += p[1]; acc[2] += p[2];
+    const int k0 = node_ptr[A], k1 = node_ptr[A + 1];
+    // eight contributors per trip (indices clamped, loads issued together), summed in their order
+    for (int k = k0; k < k1; k += 8) {
+        int src[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) src[u] = node_src[min(k + u, k1 - 1)];
+        double v[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[u][i] = felem[(size_t)src[u] * 3 + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k + u < k1) { acc[0] += v[u][0]; acc[1] += v[u][1]; acc[2] += v[u][2]; }
     }
     const double s = sa[A];
 #pragma unroll
