@@ -95,15 +95,16 @@ __device__ double side_average(const double* __restrict__ uk, const int c[6], in
 template <typename T, typename Op>
 __device__ T block_reduce(T v, T* sh, Op op)
 {
+    // a butterfly inside each 64-lane wave, then the SWG / 64 wave results in order: a fixed
+    // combination order (bitwise reproducible) with two barriers instead of one per tree level
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
     const int t = threadIdx.x;
-    sh[t] = v;
+    if ((t & 63) == 0) sh[t / 64] = v;
     __syncthreads();
-    for (int s = SWG / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] = op(sh[t], sh[t + s]);
-        __syncthreads();
-    }
-    const T r = sh[0];
-    __syncthreads();
+    T r = sh[0];
+    for (int k = 1; k < SWG / 64; ++k) r = op(r, sh[k]);
+    __syncthreads();  // sh is reused by the next reduction
     return r;
 }
 struct OpAdd { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
