@@ -67,13 +67,16 @@ enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
 /* TF_PRECOND_AUTO (the default): two-level for a handle of one tissue with 4096 nodes or more,
  * else block Jacobi.  TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M.
- * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi plus an additive coarse
- * correction P (P^T M P)^-1 P^T r, P the rigid-body modes of node aggregates (square cells of
- * the tissue's x-y extent, whole layer columns each).  The coarse operator is rebuilt at the
- * first Newton iteration of a timestep once Newton iteration 1's Krylov count exceeds 1.25x its
- * count right after the last rebuild, or coarse_refresh timesteps after it, and for every solve
- * outside the Newton loop; handles with several tissues use block Jacobi.  The coarse inverse is held in FP32
- * (it only preconditions: residuals, matrix and Krylov vectors stay in the solver's precision). */
+ * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi over the node columns
+ * (the diagonal blocks of M over the nodes that prisms' vertical edges join; 3x3 nodal blocks
+ * when the columns are not uniform or column_blocks = 0) plus an additive coarse correction
+ * P (P^T M P)^-1 P^T r, P the rigid-body modes of aggregates of whole columns (square cells of the
+ * tissue's x-y extent).  The coarse operator is rebuilt at the first Newton iteration of a
+ * timestep once Newton iteration 1's Krylov count exceeds 1.25x its count right after the last
+ * rebuild, or coarse_refresh timesteps after it, and for every solve outside the Newton loop;
+ * handles with several tissues use block Jacobi.  The column-block inverses are held in FP32 and
+ * the coarse inverse in FP16 scaled by its largest diagonal entry (they only precondition:
+ * residuals, matrix and Krylov vectors stay in the solver's precision). */
 enum { TF_PRECOND_AUTO = -1, TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
 
 typedef struct tf_solver_opts {
