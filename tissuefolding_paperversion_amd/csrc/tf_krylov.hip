@@ -612,6 +612,7 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
     __shared__ double rl[WG];
     const int a = blockIdx.x;
     const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
+    const int kr = tl.krank[a], co = tl.coff[a];  // used at the end, loaded with the rest
     const int slot = threadIdx.x / D, li = threadIdx.x % D;
     double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, Dc^-1 r), (r, r), P^T r
     double alpha = 0.0;
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
 #pragma unroll
         for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
         if (threadIdx.x < 2) tl.agg_part[2 * a + threadIdx.x] = t;
-        else if ((int)threadIdx.x - 2 < tl.krank[a]) tl.rc[tl.coff[a] + threadIdx.x - 2] = t;
+        else if ((int)threadIdx.x - 2 < kr) tl.rc[co + threadIdx.x - 2] = t;
     }
 }
 
@@ -702,6 +703,7 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
     for (int a = blockIdx.x; a < tl.n_agg; a += G) {
         const int c0 = tl.coff[a], kr = tl.krank[a];
         const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
+        const double part0 = tl.agg_part[2 * a], part1 = tl.agg_part[2 * a + 1];  // used at the end
         // Unconditional loads at clamped addresses (a load under a condition is issued in a branch
         // of its own and waited for before the next): rows past the rank are summed and dropped,
         // columns past nc meet the zeros that pad r_c in LDS.
@@ -783,8 +785,8 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
         if (tid == 0) {
             double cz = 0.0;
             for (int k = 0; k < kr; ++k) cz += rcs[c0 + k] * zcs[k];
-            acc_rz += tl.agg_part[2 * a] + cz;
-            acc_rr += tl.agg_part[2 * a + 1];
+            acc_rz += part0 + cz;
+            acc_rr += part1;
         }
         __syncthreads();  // red and zcs are reused by the next aggregate
     }
