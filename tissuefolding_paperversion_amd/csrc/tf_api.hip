@@ -89,6 +89,7 @@ void free_all(tf_handle* h)
     twolevel_free(h);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
     if (h->h_seg_norm) (void)hipHostFree(h->h_seg_norm);
+    if (h->tile_info) (void)hipFree(h->tile_info);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->hp_rtol2) (void)hipHostFree(h->hp_rtol2);
     drop_krylov_graphs(h);
@@ -199,6 +200,7 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
     }
     for (int32_t** p : {&h->tile_r0, &h->tile_r1, &h->tile_seg, &h->seg_tile_ptr})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (h->tile_info) { (void)hipFree(h->tile_info); h->tile_info = nullptr; }
     if (h->partial) { (void)hipFree(h->partial); h->partial = nullptr; }
     h->T = (int)tr0.size();
     h->tile_r0 = dev_alloc<int32_t>(h->T);
@@ -210,6 +212,12 @@ void build_tiles(tf_handle* h, int wg, int tile_cap)
     h2d(h, h->tile_r1, tr1.data(), h->T);
     h2d(h, h->tile_seg, tseg.data(), h->T);
     h2d(h, h->seg_tile_ptr, seg_tile_ptr.data(), h->S + 1);
+    std::vector<TileInfo> info(h->T);
+    for (int t = 0; t < h->T; ++t)
+        info[t] = TileInfo{tseg[t], row_ptr[tr0[t]], row_ptr[tr1[t]], tr0[t], tr1[t], seg_tile_ptr[tseg[t]],
+                           seg_tile_ptr[tseg[t] + 1], 0};
+    h->tile_info = dev_alloc<TileInfo>(h->T);
+    h2d(h, h->tile_info, info.data(), h->T);
     h->h_seg_tile_ptr = seg_tile_ptr;
     h->wg = wg;
     h->tile_cap = tile_cap;

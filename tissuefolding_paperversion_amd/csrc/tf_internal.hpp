@@ -108,6 +108,13 @@ struct PackingState {
     std::vector<int64_t> h_blocks;       // pair blocks (min * N + max) kept in the BSR pattern
 };
 
+// Everything a Krylov tile kernel reads about its tile, in one 32-byte record: its tissue, BSR
+// blocks [b0, b1), block rows [r0, r1) and the tissue's tiles [st0, st1).  The first 16 bytes hold
+// what the first loads need (the tissue's state test, then the tile's blocks).
+struct alignas(32) TileInfo {
+    int32_t seg, b0, b1, r0, r1, st0, st1, pad;
+};
+
 struct tf_handle {
     // sizes
     int32_t N = 0, E = 0, nb = 0, S = 1, T = 0;
@@ -182,6 +189,7 @@ struct tf_handle {
 
     // tiles and segments
     int32_t *tile_r0 = nullptr, *tile_r1 = nullptr, *tile_seg = nullptr;
+    TileInfo* tile_info = nullptr;  // [T] the same per tile, with its block range and its tissue's tiles
     int32_t *seg_tile_ptr = nullptr;  // [S+1]
     int32_t *seg_node_ptr = nullptr;  // [S+1]
     int32_t *seg_counter = nullptr;   // [S] tickets for last-tile reductions

@@ -33,6 +33,7 @@ namespace {
 constexpr int WG = 256;  // workgroup of every tile kernel except the single-kernel CG (tunable)
 
 struct TileArgs {
+    const TileInfo* info;  // [T] one record per tile (the arrays below, in one scalar load)
     const int32_t* tile_r0;
     const int32_t* tile_r1;
     const int32_t* tile_seg;
@@ -86,11 +87,10 @@ __device__ inline void tile_write(const TileArgs& ta, int t, const int (&slot)[K
 
 // Tissue totals of K partial slots, identical in every lane and every tile of the tissue.
 template <int K, int WGT = WG>
-__device__ inline void seg_sums(const TileArgs& ta, int sg, const int (&slot)[K], double (&out)[K])
+__device__ inline void seg_sums_range(const TileArgs& ta, int t0, int t1, const int (&slot)[K], double (&out)[K])
 {
     constexpr int NW = WGT / 64;
     __shared__ double red[K][NW];
-    const int t0 = ta.seg_tile_ptr[sg], t1 = ta.seg_tile_ptr[sg + 1];
     double s[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) s[q] = 0.0;
@@ -112,6 +112,12 @@ __device__ inline void seg_sums(const TileArgs& ta, int sg, const int (&slot)[K]
         out[q] = a;
     }
     __syncthreads();
+}
+
+template <int K, int WGT = WG>
+__device__ inline void seg_sums(const TileArgs& ta, int sg, const int (&slot)[K], double (&out)[K])
+{
+    seg_sums_range<K, WGT>(ta, ta.seg_tile_ptr[sg], ta.seg_tile_ptr[sg + 1], slot, out);
 }
 
 __device__ inline bool first_tile(const TileArgs& ta, int t, int sg) { return t == ta.seg_tile_ptr[sg]; }
@@ -397,11 +403,10 @@ __global__ __launch_bounds__(WGT) void k_cg_q(TileArgs ta, const int32_t* __rest
     constexpr int PF = WGT >= 1024 ? 1 : 2;
     extern __shared__ double prod[];  // 3 doubles per block of the tile (tile_cap blocks)
     const int t = xcd_tile(blockIdx.x, ta.T, ta.nxcd);
-    const int sg = ta.tile_seg[t];
+    const TileInfo ti = ta.info[t];  // tissue, rows, blocks and the tissue's tiles: one scalar load
+    const int sg = ti.seg, r0 = ti.r0, r1 = ti.r1, b0 = ti.b0, b1 = ti.b1;
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
-    const int b0 = row_ptr[r0], b1 = row_ptr[r1];
     double pv[PF][9], pz[PF][3], pp[PF][3];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
@@ -415,12 +420,12 @@ __global__ __launch_bounds__(WGT) void k_cg_q(TileArgs ta, const int32_t* __rest
     }
     const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
     double sc[3];
-    seg_sums<3, WGT>(ta, sg, sl, sc);
+    seg_sums_range<3, WGT>(ta, ti.st0, ti.st1, sl, sc);
     {  // the convergence / cap test of check_stop
         const bool conv = sc[0] <= S.tol2;
         const bool cap = !conv && (S.iters >= maxit || !isfinite(sc[0]));
         if (conv || cap) {
-            if (first_tile(ta, t, sg) && threadIdx.x == 0) {
+            if (t == ti.st0 && threadIdx.x == 0) {
                 S.rnorm2 = sc[0];
                 if (conv) S.converged = 1;
                 else S.stalled = 1;
@@ -607,12 +612,12 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
                                               const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r)
 {
     constexpr int D = 3 * L, CPP = WG / D;
-    SegScalars& S = ta.seg[0];
-    if (seg_done(S)) return;
     __shared__ double rl[WG];
     const int a = blockIdx.x;
     const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
     const int kr = tl.krank[a], co = tl.coff[a];  // used at the end, loaded with the rest
+    SegScalars& S = ta.seg[0];
+    if (seg_done(S)) return;
     const int slot = threadIdx.x / D, li = threadIdx.x % D;
     double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, Dc^-1 r), (r, r), P^T r
     double alpha = 0.0;
@@ -1417,7 +1422,7 @@ __global__ __launch_bounds__(WG) void k_seg_norm_fin(TileArgs ta, double* __rest
 
 TileArgs tile_args(tf_handle* h)
 {
-    return TileArgs{h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->partial, h->seg, h->T,
+    return TileArgs{h->tile_info, h->tile_r0, h->tile_r1, h->tile_seg, h->seg_tile_ptr, h->partial, h->seg, h->T,
                     h->opts.xcd_tile_map ? h->n_xcd : 1};
 }
 
