@@ -617,7 +617,9 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
     const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
     const int kr = tl.krank[a], co = tl.coff[a];  // used at the end, loaded with the rest
     SegScalars& S = ta.seg[0];
-    if (seg_done(S)) return;
+    // n0 is never negative: testing it makes the aggregate's offsets load together with the
+    // tissue's state instead of after the branch
+    if (seg_done(S) | (n0 < 0)) return;
     const int slot = threadIdx.x / D, li = threadIdx.x % D;
     double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, Dc^-1 r), (r, r), P^T r
     double alpha = 0.0;
@@ -694,7 +696,9 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
 {
     constexpr int D = 3 * L, CPP = WG / D;
     SegScalars& S = ta.seg[0];
-    if (seg_done(S)) return;
+    // the first aggregate's node offset is never negative: testing it makes the aggregate's
+    // offsets load together with the tissue's state instead of after the branch
+    if (seg_done(S) | (tl.agg_ptr[blockIdx.x] < 0) | (tl.coff[blockIdx.x] < 0)) return;
     extern __shared__ double rcs[];  // r_c, zero-padded to max(ldA, 2 WG TL_CZ_PAIRS)
     constexpr int NW = WG / 64;
     __shared__ double red[6][NW];
