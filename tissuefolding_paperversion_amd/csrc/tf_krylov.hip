@@ -1093,11 +1093,10 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
 {
     extern __shared__ double prod[];
     const int t = xcd_tile(blockIdx.x, ta.T, ta.nxcd);
-    const int sg = ta.tile_seg[t];
+    const TileInfo ti = ta.info[t];  // tissue, blocks, rows and the tissue's tiles: one scalar load
+    const int sg = ti.seg, rr0 = ti.r0, rr1 = ti.r1, b0 = ti.b0, b1 = ti.b1;
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    const int rr0 = ta.tile_r0[t], rr1 = ta.tile_r1[t];
-    const int b0 = row_ptr[rr0], b1 = row_ptr[rr1];
     const int d0 = 3 * rr0, d1 = 3 * rr1;
     // ---- prefetch ----
     double mv[BPT][9], cu[BPT][3], cm[BPT][3], cq[BPT][3];
@@ -1132,12 +1131,12 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
     //      history launch k-1 left in the parity-P slots of the tissue's scalars ----
     const int sl[3] = {SL_RHO0 + (1 - P), SL_DL0 + (1 - P), SL_RR0 + (1 - P)};
     double q3[3];
-    seg_sums<3, WGT>(ta, sg, sl, q3);
+    seg_sums_range<3, WGT>(ta, ti.st0, ti.st1, sl, q3);
     const double gk = q3[0], dk = q3[1], rr = q3[2];
     const int it = S.iters_h[P];
     const double best = S.best_rr_h[P];
     const int best_it = S.best_it_h[P];
-    const bool lead = first_tile(ta, t, sg) && threadIdx.x == 0;
+    const bool lead = t == ti.st0 && threadIdx.x == 0;
     {
         // The single-reduction recurrences can level off just above a very tight tolerance:
         // without a halving of ||r|| for 256 iterations the solve stops, as converged when the
