@@ -21,13 +21,15 @@ def main(argv=None):
     p.add_argument("--no-graph", action="store_true")
     p.add_argument("--tiling", type=int, nargs=2, metavar=("WG", "TILE_BLOCKS"),
                    help="Krylov tiling (workgroup size, blocks per tile) instead of the default")
+    p.add_argument("--preconditioner", default="auto", choices=["auto", "block-jacobi", "two-level"])
     a = p.parse_args(argv)
 
     from . import configs
     from .solver import NewtonRaphsonSolver
 
     t = configs.config2() if a.workload == "config2" else configs.config1()
-    s = NewtonRaphsonSolver(t, {"use_graph": 0 if a.no_graph else 1})
+    pc = {"auto": -1, "block-jacobi": 0, "two-level": 1}[a.preconditioner]
+    s = NewtonRaphsonSolver(t, {"use_graph": 0 if a.no_graph else 1, "preconditioner": pc})
     if a.tiling:
         s.set_tiling(*a.tiling)
     for _ in range(a.warmup):
