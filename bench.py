@@ -215,8 +215,9 @@ def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is described "
-              "in DESIGN.md and is not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the timestep of one tissue (or one "
+              "batch of tissues) is measured on one MI355X; runs over several GPUs are not implemented.",
+              file=sys.stderr)
         sys.exit(2)
     from tissuefolding_paperversion_amd import configs
     from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
