@@ -116,6 +116,12 @@ __global__ void k_seg_deactivate(SegScalars* seg, int S)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S) seg[s].active = 0;
 }
+// tissues whose linear solve stalled leave the NR loop: their update is not applied
+__global__ void k_seg_drop_stalled(SegScalars* seg, int S)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S && seg[s].active && seg[s].stalled) seg[s].active = 0;
+}
 __global__ void k_zero_int(int32_t* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -645,6 +651,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         TF_HIP(hipStreamSynchronize(h->stream));
         h->seg_nr_iters.assign(S, 0);
         h->seg_converged.assign(S, 0);
+        h->seg_failed.assign(S, 0);
         h->seg_du.assign(S, 0.0);
         h->fc_C.assign(S, -1.0);
         h->fc_du0.assign(S, -1.0);
@@ -862,6 +869,7 @@ int tf_begin_step(tf_handle* h)
     TF_HIP(hipStreamSynchronize(h->stream));
     std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
     std::fill(h->seg_converged.begin(), h->seg_converged.end(), 0);
+    std::fill(h->seg_failed.begin(), h->seg_failed.end(), 0);
     return TF_OK;
     TF_GUARD_END(h)
 }
@@ -959,6 +967,7 @@ int tf_solve(tf_handle* h, double* dU, int32_t* krylov_iters, double* rel_residu
             if (h->h_seg[s].bnorm2 > 0) worst = std::max(worst, std::sqrt(h->h_seg[s].rnorm2 / h->h_seg[s].bnorm2));
         *rel_residual = worst;
     }
+    if (st == TF_NAN) return fail(h, st, "NaN or Inf in the right-hand side");
     if (st != TF_OK) return fail(h, st, "Krylov solver did not reach its tolerance");
     return TF_OK;
     TF_GUARD_END(h)
@@ -975,7 +984,7 @@ static int update_uk_impl(tf_handle* h, int iter_index)
     TF_HIP(hipStreamSynchronize(h->stream));
     int st = TF_OK;
     for (int s = 0; s < h->S; ++s) {
-        if (h->seg_converged[s]) continue;
+        if (h->seg_converged[s] || h->seg_failed[s]) continue;
         const double dn = h->h_seg_norm[s];
         h->seg_du[s] = dn;
         h->seg_nr_iters[s] = iter_index + 1;
@@ -1082,11 +1091,20 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
     std::fill(h->seg_converged.begin(), h->seg_converged.end(), 0);
+    std::fill(h->seg_failed.begin(), h->seg_failed.end(), 0);
     int status = TF_OK;
     int krylov_total = 0;
+    bool stalled = false;
     h->fallbacks = 0;
     h->tl_used = false;
     double ms_asm = 0, ms_sol = 0;
+    // the per-solve state the Krylov set-up reads; reset on every exit from a solve
+    auto solve_state_off = [h]() {
+        h->seg_rtol_on = false;
+        h->warm_guess = nullptr;
+        h->krylov_call = -1;
+        h->tl_pre_valid = false;
+    };
     int it = 0;
     while (true) {
         auto ta = std::chrono::steady_clock::now();
@@ -1095,36 +1113,42 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
         ms_asm += ms_since(ta);
         auto ts = std::chrono::steady_clock::now();
-        h->seg_rtol_on = h->opts.krylov_forcing != 0;
-        if (h->seg_rtol_on) forcing_tolerances(h, it);
         const bool warm = h->opts.krylov_warm_start != 0 && it < TF_WARM_ITERS;
-        const bool wpend = warm && warm_start_begin(h, it);
-        h->krylov_call = it;
-        krylov_pre_solve(h);  // device work that hides the host's wait for the read-back
-        if (wpend) h->warm_guess = warm_start_finish(h, it) ? h->warm_x0 : nullptr;
-        else TF_HIP(hipStreamSynchronize(h->stream));  // the status
-        if (h->h_status[0] > 0) {
-            h->warm_guess = nullptr;
-            h->seg_rtol_on = false;
-            h->tl_pre_valid = false;
-            h->krylov_call = -1;
-            status = TF_NEG_JACOBIAN;
-            h->err = std::to_string(h->h_status[0]) + " Gauss points with det Fe <= 0";
-            break;
-        }
         int kst;
         try {
+            h->seg_rtol_on = h->opts.krylov_forcing != 0;
+            if (h->seg_rtol_on) forcing_tolerances(h, it);
+            const bool wpend = warm && warm_start_begin(h, it);
+            h->krylov_call = it;
+            krylov_pre_solve(h);  // device work that hides the host's wait for the read-back
+            if (wpend) h->warm_guess = warm_start_finish(h, it) ? h->warm_x0 : nullptr;
+            else TF_HIP(hipStreamSynchronize(h->stream));  // the status
+            if (h->h_status[0] > 0) {
+                solve_state_off();
+                status = TF_NEG_JACOBIAN;
+                h->err = std::to_string(h->h_status[0]) + " Gauss points with det Fe <= 0";
+                break;
+            }
             kst = solve_impl(h);
         } catch (...) {
-            h->seg_rtol_on = false;
-            h->warm_guess = nullptr;
-            h->krylov_call = -1;
-            h->tl_pre_valid = false;
+            solve_state_off();
             throw;
         }
-        h->seg_rtol_on = false;
-        h->warm_guess = nullptr;
-        h->krylov_call = -1;
+        solve_state_off();
+        if (kst == TF_NAN) {  // a residual that is not finite: nothing to update
+            status = TF_NAN;
+            h->err = "NaN or Inf in the residual";
+            break;
+        }
+        if (kst == TF_KRYLOV_STALL) {
+            // a tissue whose solve stalled (CG and its BiCGSTAB fallback) leaves the loop with the
+            // positions of its last update; the others go on
+            stalled = true;
+            for (int s = 0; s < h->S; ++s)
+                if (h->h_seg[s].active && h->h_seg[s].stalled && !h->seg_converged[s]) h->seg_failed[s] = 1;
+            hipLaunchKernelGGL(k_seg_drop_stalled, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+            TF_HIP(hipGetLastError());
+        }
         if (warm && kst == TF_OK) warm_start_push(h, it);  // a guess for the next timestep's iteration it
         krylov_total += h->last_krylov_iters;
         if (stats && it < 4) stats->krylov_by_newton[it] = h->last_krylov_iters;
@@ -1132,10 +1156,9 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         if (h->opts.krylov_forcing) forcing_observe(h, it);
         ms_sol += ms_since(ts);
         if (ust == TF_NAN) { status = TF_NAN; h->err = "NaN in the Newton update"; break; }
-        if (kst != TF_OK && status == TF_OK) status = TF_OK;  // inexact step: NR keeps iterating
         ++it;
         bool all = true;
-        for (int s = 0; s < h->S; ++s) all &= (bool)h->seg_converged[s];
+        for (int s = 0; s < h->S; ++s) all &= (bool)(h->seg_converged[s] || h->seg_failed[s]);
         if (all) break;
         if (it > h->opts.nr_max_iter) {
             status = TF_NOT_CONVERGED;
@@ -1143,6 +1166,13 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
             hipLaunchKernelGGL(k_seg_deactivate, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
             break;
         }
+    }
+    if (stalled && (status == TF_OK || status == TF_NOT_CONVERGED)) {
+        int n = 0;
+        for (int s = 0; s < h->S; ++s) n += h->seg_failed[s];
+        status = TF_KRYLOV_STALL;
+        h->err = "the linear solve of " + std::to_string(n) + " tissue(s) did not reach its tolerance (CG and "
+                 "BiCGSTAB); their positions are those of their last applied Newton update";
     }
     timer_collect(h->t_elem);
     timer_collect(h->t_gather);

@@ -206,6 +206,7 @@ struct tf_handle {
 
     // per-segment NR bookkeeping on the host
     std::vector<int32_t> seg_nr_iters, seg_converged;
+    std::vector<int32_t> seg_failed;  // the tissue's linear solve stalled this step: it left the NR loop
     std::vector<double> seg_du;
     int32_t last_krylov_iters = 0;
     // inexact-Newton forcing, per tissue: the quadratic-convergence constant C estimated from
@@ -245,6 +246,7 @@ struct tf_handle {
     hipGraphExec_t krylov_graph[2] = {nullptr, nullptr};
     int krylov_graph_iters[2] = {0, 0};
     int krylov_graph_kind[2] = {-1, -1};
+    int krylov_graph_maxit[2] = {-1, -1};  // the iteration cap the captured launches carry
 
     // implicit packing
     PackingState pk;

@@ -421,17 +421,30 @@ __global__ __launch_bounds__(WGT) void k_cg_q(TileArgs ta, const int32_t* __rest
     const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
     double sc[3];
     seg_sums_range<3, WGT>(ta, ti.st0, ti.st1, sl, sc);
-    {  // the convergence / cap test of check_stop
-        const bool conv = sc[0] <= S.tol2;
-        const bool cap = !conv && (S.iters >= maxit || !isfinite(sc[0]));
+    const int it = S.iters;
+    const double best = S.best_rr_h[P];
+    const int best_it = S.best_it_h[P];
+    const bool lead = t == ti.st0 && threadIdx.x == 0;
+    {
+        // the convergence / cap test of check_stop, with the stagnation rule of the single-kernel
+        // CG: 256 iterations without a halving of ||r|| stop the solve (as converged when the best
+        // residual came within 10x of the tolerance, else as stalled: BiCGSTAB takes over)
+        const bool stagnant = it - best_it > 256;
+        const bool conv = sc[0] <= S.tol2 || (stagnant && best <= 100.0 * S.tol2);
+        const bool cap = !conv && (it >= maxit || !isfinite(sc[0]) || stagnant);
         if (conv || cap) {
-            if (t == ti.st0 && threadIdx.x == 0) {
+            if (lead) {
                 S.rnorm2 = sc[0];
                 if (conv) S.converged = 1;
                 else S.stalled = 1;
             }
             return;
         }
+    }
+    if (lead) {
+        const bool better = sc[0] < 0.5 * best;
+        S.best_rr_h[1 - P] = better ? sc[0] : best;
+        S.best_it_h[1 - P] = better ? it : best_it;
     }
     const double beta = S.iters > 0 ? sc[1] / sc[2] : 0.0;
 #pragma unroll
@@ -640,7 +653,9 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
             const int sl[2] = {SL_RZ0 + P, SL_PQ};
             double sc[2];
             seg_sums<2>(ta, 0, sl, sc);
-            if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite
+            // (p, q) > 0 needs M positive definite, (r, z) > 0 the preconditioner (its FP16 coarse
+            // inverse is rounded): either failing stops the solve as stalled
+            if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
                 if (a == 0 && threadIdx.x == 0) S.stalled = 1;
                 return;
             }
@@ -784,8 +799,9 @@ __global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const doub
             rl[tid] = rv;
             __syncthreads();
             double v = 0.0;
+            const int sb = min(slot, CPP - 1) * D;  // lanes past the last whole column read a valid one
 #pragma unroll
-            for (int j = 0; j < D; ++j) v += (double)Dr[j] * rl[slot * D + j];
+            for (int j = 0; j < D; ++j) v += (double)Dr[j] * rl[sb + j];
 #pragma unroll
             for (int j = 0; j < 6; ++j) v += Pr[j] * zcs[j];
             if (own) z[d] = v;
@@ -1892,7 +1908,15 @@ static void twolevel_build(tf_handle* h)
     std::vector<double> xc(3 * (size_t)ncol);
     for (int c = 0; c < ncol; ++c)
         for (int k = 0; k < 3; ++k) xc[3 * c + k] = x[3 * (size_t)col_start[c] + k];
-    std::vector<int32_t> aggc = tl_aggregates(xc, ncol, target, std::max(1, 16 / L), A);
+    // at most six coarse unknowns per aggregate: an elongated tissue can give more aggregates than
+    // the target (cells along the boundary), so the target shrinks until 6 A fits max_nc (one
+    // aggregate at target 1, so this ends)
+    std::vector<int32_t> aggc;
+    while (true) {
+        aggc = tl_aggregates(xc, ncol, target, std::max(1, 16 / L), A);
+        if (6 * A <= max_nc || target <= 1) break;
+        target = std::max(1, std::min(target - 1, (int)(0.9 * target * (max_nc / 6) / A)));
+    }
     std::vector<int32_t> agg(N);
     for (int n = 0; n < N; ++n) agg[n] = aggc[col_of[n]];
     std::vector<std::vector<int32_t>> members(A);  // column by column
@@ -1904,7 +1928,7 @@ static void twolevel_build(tf_handle* h)
         coff[a + 1] = coff[a] + krank[a];
     }
     const int nc = coff[A];
-    if (nc > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");
+    if (nc > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");  // unreachable: 6 A <= max_nc
     std::vector<int32_t> agg_ptr(A + 1, 0), agg_nodes;
     agg_nodes.reserve(N);
     for (int a = 0; a < A; ++a) {
@@ -2319,7 +2343,10 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
             const int key = kind * 16 + (int)sizeof(VT) * 2;
             const int gi = sizeof(VT) == sizeof(double) ? 0 : 1;
             hipGraphExec_t& graph_exec = h->krylov_graph[gi];
-            if (!graph_exec || h->krylov_graph_iters[gi] != g_it || h->krylov_graph_kind[gi] != key) {
+            // the captured launches carry the iteration cap as an argument: a chunk captured
+            // under another cap (a timing probe's) is not reused
+            if (!graph_exec || h->krylov_graph_iters[gi] != g_it || h->krylov_graph_kind[gi] != key ||
+                h->krylov_graph_maxit[gi] != h->opts.krylov_max_iter) {
                 if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
                 hipGraph_t gr;
                 TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -2337,6 +2364,7 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
                 TF_HIP(hipGraphDestroy(gr));
                 h->krylov_graph_iters[gi] = g_it;
                 h->krylov_graph_kind[gi] = key;
+                h->krylov_graph_maxit[gi] = h->opts.krylov_max_iter;
             }
             if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
             for (int k = 0; k < reps * per_chunk; ++k) TF_HIP(hipGraphLaunch(graph_exec, h->stream));
@@ -2363,10 +2391,13 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     timer_collect(h->t_spmv);
     int st = TF_OK;
     int maxit = 0;
+    bool nan = false;
     for (int s = 0; s < h->S; ++s) {
         maxit = std::max(maxit, h->h_seg[s].iters);
         if (h->h_seg[s].active && h->h_seg[s].stalled) st = TF_KRYLOV_STALL;
+        nan = nan || (h->h_seg[s].active && !std::isfinite(h->h_seg[s].bnorm2));
     }
+    if (nan) st = TF_NAN;  // a right-hand side that is not finite: no solver reaches a tolerance
     h->last_krylov_iters = maxit;
     if (call >= 0) {
         if ((int)h->krylov_pred.size() <= call) h->krylov_pred.resize(call + 1, 0);
@@ -2568,6 +2599,7 @@ int krylov_solve(tf_handle* h)
     }
     int st = run(kind);
     if (kind == KIND_TL && st == TF_OK) twolevel_observe(h, h->last_krylov_iters);
+    if (kind == KIND_TL && st != TF_OK) h->tl.fresh = false;  // the next solve rebuilds the coarse operator
     if (st == TF_KRYLOV_STALL && (cg_kind || kind == KIND_TL)) {
         // not positive definite (or too slow): BiCGSTAB from zero
         const int cg_iters = h->last_krylov_iters;
