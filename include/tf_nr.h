@@ -27,7 +27,7 @@ typedef enum tf_status {
     TF_NOT_CONVERGED = 1, /* NR hit its iteration cap (the reference prints and continues) */
     TF_KRYLOV_STALL = 2,  /* Krylov breakdown or iteration cap without reaching tolerance    */
     TF_NEG_JACOBIAN = 3,  /* det Fe <= 0 in some element (inverted prism)                    */
-    TF_NAN = 4,           /* NaN/Inf in the residual or the update                           */
+    TF_NAN = 4,           /* NaN/Inf in det Fe (positions, growth state), residual or update */
     TF_OOM = 5,           /* device allocation failed                                        */
     TF_HIP_ERROR = 6,     /* any other HIP runtime error                                     */
     TF_INVALID = 7        /* invalid argument                                                */
@@ -123,6 +123,10 @@ typedef struct tf_step_stats {
     double ms_grow, ms_assemble, ms_solve, ms_total;
     int32_t krylov_by_newton[4]; /* Krylov iterations of Newton iterations 0-3 (max over tissues) */
     int32_t two_level;           /* 1 when the step's CG solves used the two-level preconditioner */
+    int32_t krylov_stalls;       /* linear solves that stopped short of their tolerance (CG and its
+                                    BiCGSTAB fallback) with a residual at most 1e-2 of the right-hand
+                                    side: taken as inexact Newton steps.  A solve that ends above
+                                    that ends its tissue's step: tf_nr_step returns TF_KRYLOV_STALL */
 } tf_step_stats;
 
 /* ---- lifetime --------------------------------------------------------------------------- */

@@ -60,11 +60,11 @@ def test_nr_cap_returns_not_converged_after_the_reference_count(oracle):
 
 @pytest.mark.parametrize("prec", [0, 1])  # block Jacobi, two-level
 def test_krylov_stall_is_reported_and_the_update_is_not_applied(prec):
-    """A Krylov cap of 3 iterations at tolerance 1e-12: CG and its BiCGSTAB fallback both stop at
-    the cap, the step returns TF_KRYLOV_STALL without applying the update, and the handle goes on
-    with the default cap."""
+    """A Krylov cap of one iteration at tolerance 1e-12: CG stops at the cap with a residual far
+    above 1e-2 of the right-hand side (unusable), its BiCGSTAB fallback too, so the step returns
+    TF_KRYLOV_STALL without applying the update, and the handle goes on with the default cap."""
     t = _ellipse()
-    s = _solver(t, preconditioner=prec, krylov_max_iter=3, **EXACT)
+    s = _solver(t, preconditioner=prec, krylov_max_iter=1, **EXACT)
     before = s.positions
     rc, st = _raw_timestep(s, t.dt)
     assert rc == L.TF_KRYLOV_STALL, L.STATUS_NAMES.get(rc, rc)
@@ -74,6 +74,23 @@ def test_krylov_stall_is_reported_and_the_update_is_not_applied(prec):
     s.set_options(krylov_max_iter=20000)
     out = s.runOneStep(t.dt)
     assert out["status"] == L.TF_OK and out["converged"] == 1 and out["krylov_fallbacks"] == 0
+
+
+def test_capped_solve_with_a_small_residual_is_an_inexact_newton_step():
+    """A Krylov cap of 250 block-Jacobi CG iterations at tolerance 1e-12: the solves stop short of
+    the tolerance (about 650 iterations on this mesh) but well below 1e-2 of the right-hand side
+    (2e-5 for the first one, by a CPU run of the same PCG), so they are kept as inexact Newton
+    steps (no BiCGSTAB restart), counted in krylov_stalls, and Newton still converges to the
+    fixed point of exact solves."""
+    t = _ellipse()
+    exact = _solver(t, preconditioner=0, **EXACT)
+    capped = _solver(t, preconditioner=0, krylov_max_iter=250, **EXACT)
+    for _ in range(2):
+        a, b = exact.runOneStep(t.dt), capped.runOneStep(t.dt)
+        assert a["status"] == L.TF_OK and b["status"] == L.TF_OK
+        assert b["converged"] == 1 and b["krylov_stalls"] >= 1 and b["krylov_fallbacks"] == 0
+        assert a["krylov_stalls"] == 0
+    assert rel_err(capped.positions, exact.positions) < 1e-8
 
 
 def test_singular_jacobian_exits_at_the_iteration_cap():

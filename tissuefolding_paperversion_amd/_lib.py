@@ -62,7 +62,7 @@ class StepStats(C.Structure):
                 ("rotated_elements", C.c_int32), ("krylov_fallbacks", C.c_int32), ("packing_pairs", C.c_int32),
                 ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double),
-                ("krylov_by_newton", C.c_int32 * 4), ("two_level", C.c_int32)]
+                ("krylov_by_newton", C.c_int32 * 4), ("two_level", C.c_int32), ("krylov_stalls", C.c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -116,11 +116,16 @@ __global__ void k_seg_deactivate(SegScalars* seg, int S)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S) seg[s].active = 0;
 }
-// tissues whose linear solve stalled leave the NR loop: their update is not applied
+// a tissue whose stalled solve is unusable (solve_unusable) leaves the NR loop without its update
 __global__ void k_seg_drop_stalled(SegScalars* seg, int S)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S && seg[s].active && seg[s].stalled) seg[s].active = 0;
+    if (s < S && seg[s].active && seg[s].stalled && solve_unusable(seg[s].rnorm2, seg[s].bnorm2)) seg[s].active = 0;
+}
+// the assembly's flags: status[0] inverted Gauss points, status[3] non-finite det Fe
+__global__ void k_zero_assembly_status(int32_t* status)
+{
+    if (threadIdx.x == 0) { status[0] = 0; status[3] = 0; }
 }
 __global__ void k_zero_int(int32_t* p, int n)
 {
@@ -876,7 +881,7 @@ int tf_begin_step(tf_handle* h)
 
 static int assemble_impl(tf_handle* h, double dt)
 {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, h->stream, h->status, 1);
+    hipLaunchKernelGGL(k_zero_assembly_status, dim3(1), dim3(64), 0, h->stream, h->status);
     launch_assemble_elements(h, dt);
     launch_assemble_gather(h, dt);
     launch_residual(h, dt);
@@ -893,6 +898,7 @@ int tf_assemble(tf_handle* h, double dt)
     TF_HIP(hipStreamSynchronize(h->stream));
     timer_collect(h->t_elem);
     timer_collect(h->t_gather);
+    if (st[3] > 0) return fail(h, TF_NAN, std::to_string(st[3]) + " Gauss points with a det Fe that is NaN or Inf");
     if (st[0] > 0) return fail(h, TF_NEG_JACOBIAN, std::to_string(st[0]) + " Gauss points with det Fe <= 0");
     return TF_OK;
     TF_GUARD_END(h)
@@ -1095,6 +1101,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     int status = TF_OK;
     int krylov_total = 0;
     bool stalled = false;
+    int stalls_accepted = 0;
     h->fallbacks = 0;
     h->tl_used = false;
     double ms_asm = 0, ms_sol = 0;
@@ -1123,6 +1130,12 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
             krylov_pre_solve(h);  // device work that hides the host's wait for the read-back
             if (wpend) h->warm_guess = warm_start_finish(h, it) ? h->warm_x0 : nullptr;
             else TF_HIP(hipStreamSynchronize(h->stream));  // the status
+            if (h->h_status[3] > 0) {  // positions or growth state not finite
+                solve_state_off();
+                status = TF_NAN;
+                h->err = std::to_string(h->h_status[3]) + " Gauss points with a det Fe that is NaN or Inf";
+                break;
+            }
             if (h->h_status[0] > 0) {
                 solve_state_off();
                 status = TF_NEG_JACOBIAN;
@@ -1141,11 +1154,18 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
             break;
         }
         if (kst == TF_KRYLOV_STALL) {
-            // a tissue whose solve stalled (CG and its BiCGSTAB fallback) leaves the loop with the
-            // positions of its last update; the others go on
-            stalled = true;
-            for (int s = 0; s < h->S; ++s)
-                if (h->h_seg[s].active && h->h_seg[s].stalled && !h->seg_converged[s]) h->seg_failed[s] = 1;
+            // a tissue whose stalled solve is unusable leaves the loop with the positions of its
+            // last update; a usable one goes on as an inexact Newton step, as do the others
+            for (int s = 0; s < h->S; ++s) {
+                const SegScalars& q = h->h_seg[s];
+                if (!q.active || !q.stalled || h->seg_converged[s]) continue;
+                if (solve_unusable(q.rnorm2, q.bnorm2)) {
+                    h->seg_failed[s] = 1;
+                    stalled = true;
+                } else {
+                    stalls_accepted += 1;
+                }
+            }
             hipLaunchKernelGGL(k_seg_drop_stalled, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
             TF_HIP(hipGetLastError());
         }
@@ -1172,7 +1192,8 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         for (int s = 0; s < h->S; ++s) n += h->seg_failed[s];
         status = TF_KRYLOV_STALL;
         h->err = "the linear solve of " + std::to_string(n) + " tissue(s) did not reach its tolerance (CG and "
-                 "BiCGSTAB); their positions are those of their last applied Newton update";
+                 "BiCGSTAB) and left a residual above 1e-2 of the right-hand side; their positions are those "
+                 "of their last applied Newton update";
     }
     timer_collect(h->t_elem);
     timer_collect(h->t_gather);
@@ -1187,6 +1208,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         stats->nr_iterations = maxit;
         stats->krylov_iterations = krylov_total;
         stats->krylov_fallbacks = h->fallbacks;
+        stats->krylov_stalls = stalls_accepted;
         stats->packing_pairs = h->pk.n;
         stats->two_level = h->tl_used ? 1 : 0;
         stats->converged = allc;

@@ -144,7 +144,10 @@ __global__ __launch_bounds__(WG) void k_elements(const int32_t* __restrict__ con
                 }
                 sV[tid] = vv;
             }
-            if (!(Je > 0.0)) atomicAdd(&status[0], 1);
+            // a det Fe that is not finite (NaN or Inf in the positions or the growth state) is
+            // counted apart from an inverted prism
+            if (!isfinite(Je)) atomicAdd(&status[3], 1);
+            else if (!(Je > 0.0)) atomicAdd(&status[0], 1);
         }
     }
     __syncthreads();

@@ -1,7 +1,8 @@
 // tf_driver.cpp -- minimal C++ host driver over tf_nr.hpp: reads a reference .mesh file, gives
 // every prism the same neo-Hookean material, fixes basal z, grows uniformly in x and y and runs
 // the implicit step the way Simulation::updateStepNR drives NewtonRaphsonSolver.
-//   usage: tf_driver mesh_file [steps] [E] [nu] [growth_per_hour]
+//   usage: tf_driver mesh_file [steps] [E] [nu] [growth_per_hour] [positions_out]
+// positions_out receives the final node positions, one "x y z" line per node (%.17g).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -13,7 +14,7 @@
 int main(int argc, char** argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s mesh_file [steps] [E] [nu] [growth_per_hour]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s mesh_file [steps] [E] [nu] [growth_per_hour] [positions_out]\n", argv[0]);
         return 2;
     }
     const int steps = argc > 2 ? std::atoi(argv[2]) : 3;
@@ -88,6 +89,15 @@ int main(int argc, char** argv)
         }
         std::vector<double> x = s.positions();
         std::printf("node 0 at (%.9f, %.9f, %.9f)\n", x[0], x[1], x[2]);
+        if (argc > 6) {
+            FILE* out = std::fopen(argv[6], "w");
+            if (!out) {
+                std::fprintf(stderr, "cannot write %s\n", argv[6]);
+                return 2;
+            }
+            for (int i = 0; i < n; ++i) std::fprintf(out, "%.17g %.17g %.17g\n", x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+            std::fclose(out);
+        }
     } catch (const tf::Error& ex) {
         std::fprintf(stderr, "tf error %d: %s\n", ex.code, ex.what());
         return 1;

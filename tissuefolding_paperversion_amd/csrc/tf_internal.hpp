@@ -35,7 +35,7 @@
 // launch P reads slot P and writes slot 1-P.
 struct SegScalars {
     double bnorm2;           // ||b||^2
-    double rnorm2;           // ||r||^2 at the last convergence test
+    double rnorm2;           // ||r||^2 of the iterate a stop (convergence, cap, breakdown) left
     double tol2;             // squared absolute stopping threshold
     double pad0;
     double alpha_hist[2];    // single-kernel CG: alpha of the previous iteration
@@ -48,6 +48,13 @@ struct SegScalars {
     int32_t iters;           // Krylov iterations done (reported to the host)
     int32_t active;          // NR still iterating on this segment
 };
+
+// A linear solve that stopped short of its tolerance (iteration cap, stagnation, breakdown) with a
+// residual at most 1e-2 of the right-hand side -- the loosest tolerance the inexact-Newton forcing
+// asks for -- is a usable inexact Newton step.  Every stop records the squared residual of the
+// iterate it leaves in rnorm2 (||b||^2 until the first stop test).  Host and device take the same
+// decision from the same scalars.
+__host__ __device__ inline bool solve_unusable(double rnorm2, double bnorm2) { return !(rnorm2 <= 1e-4 * bnorm2); }
 
 // partial-sum slots
 enum {
@@ -153,7 +160,8 @@ struct tf_handle {
     // assembly output / scratch
     double *kelem = nullptr;      // [E*324]
     double *felem = nullptr;      // [E*18]
-    int32_t *status = nullptr;    // [4] device flags: 0 = non-positive det Fe count, 1 = rotated count
+    int32_t *status = nullptr;    // [4] device flags: 0 = non-positive det Fe count, 1 = rotated count,
+                                  //   2 = packing partners past the list's room, 3 = non-finite det Fe count
 
     // Jacobian
     int32_t *row_ptr = nullptr, *col = nullptr, *blk_row = nullptr, *diag_blk = nullptr;

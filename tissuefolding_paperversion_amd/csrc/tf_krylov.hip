@@ -216,11 +216,12 @@ __global__ void k_seg_reopen(SegScalars* seg, int S)
 
 // Convergence / cap test shared by the first kernel of every iteration.  Returns true when the
 // tissue stops (all tiles of the tissue take the same decision).
-__device__ inline bool check_stop(const TileArgs& ta, int t, int sg, int slot_rr, int maxit)
+__device__ inline bool check_stop(const TileArgs& ta, int t, int sg, int slot_rr, int maxit, double* rr_out = nullptr)
 {
     const int sl[1] = {slot_rr};
     double rr[1];
     seg_sums<1>(ta, sg, sl, rr);
+    if (rr_out) *rr_out = rr[0];
     SegScalars& S = ta.seg[sg];
     const bool conv = rr[0] <= S.tol2;
     const bool cap = !conv && (S.iters >= maxit || !isfinite(rr[0]));
@@ -261,7 +262,8 @@ __global__ __launch_bounds__(WG) void k_bicg_p(TileArgs ta, const double* __rest
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    if (check_stop(ta, t, sg, SL_RR0 + P, maxit)) return;
+    double rr = 0.0;
+    if (check_stop(ta, t, sg, SL_RR0 + P, maxit, &rr)) return;
     double beta = 0.0, omega = 0.0;
     if (S.iters > 0) {
         const int sl[5] = {SL_RHO0 + P, SL_RHO0 + (1 - P), SL_RV, SL_TS, SL_TT};
@@ -270,8 +272,8 @@ __global__ __launch_bounds__(WG) void k_bicg_p(TileArgs ta, const double* __rest
         const double alpha_prev = q[1] / q[2];
         omega = q[3] / q[4];
         beta = (q[0] / q[1]) * (alpha_prev / omega);
-        if (!isfinite(beta) || q[0] == 0.0 || omega == 0.0) {
-            if (first_tile(ta, t, sg) && threadIdx.x == 0) S.stalled = 1;
+        if (!isfinite(beta) || q[0] == 0.0 || omega == 0.0) {  // breakdown: x keeps residual rr
+            if (first_tile(ta, t, sg) && threadIdx.x == 0) { S.rnorm2 = rr; S.stalled = 1; }
             return;
         }
     }
@@ -323,12 +325,12 @@ __global__ __launch_bounds__(WG) void k_bicg_s(TileArgs ta, const double* __rest
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    const int sl[2] = {SL_RHO0 + P, SL_RV};
-    double q[2];
-    seg_sums<2>(ta, sg, sl, q);
+    const int sl[3] = {SL_RHO0 + P, SL_RV, SL_RR0 + P};
+    double q[3];
+    seg_sums<3>(ta, sg, sl, q);
     const double alpha = q[0] / q[1];
-    if (!(q[1] != 0.0) || !isfinite(alpha)) {
-        if (first_tile(ta, t, sg) && threadIdx.x == 0) S.stalled = 1;
+    if (!(q[1] != 0.0) || !isfinite(alpha)) {  // breakdown: x keeps residual q[2]
+        if (first_tile(ta, t, sg) && threadIdx.x == 0) { S.rnorm2 = q[2]; S.stalled = 1; }
         return;
     }
     const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
@@ -496,11 +498,11 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
     const int sg = ta.tile_seg[t];
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;
-    const int sl[2] = {SL_RZ0 + P, SL_PQ};
-    double sc[2];
-    seg_sums<2>(ta, sg, sl, sc);
-    if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite
-        if (first_tile(ta, t, sg) && threadIdx.x == 0) S.stalled = 1;
+    const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
+    double sc[3];
+    seg_sums<3>(ta, sg, sl, sc);
+    if (!(sc[1] > 0.0) || !isfinite(sc[1])) {  // not positive definite: x keeps residual sc[2]
+        if (first_tile(ta, t, sg) && threadIdx.x == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
         return;
     }
     const double alpha = sc[0] / sc[1];
@@ -650,13 +652,13 @@ __global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const doub
 #pragma unroll
         for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
         if (!INIT && c0 == 0) {
-            const int sl[2] = {SL_RZ0 + P, SL_PQ};
-            double sc[2];
-            seg_sums<2>(ta, 0, sl, sc);
+            const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
+            double sc[3];
+            seg_sums<3>(ta, 0, sl, sc);
             // (p, q) > 0 needs M positive definite, (r, z) > 0 the preconditioner (its FP16 coarse
-            // inverse is rounded): either failing stops the solve as stalled
+            // inverse is rounded): either failing stops the solve as stalled, x keeping residual sc[2]
             if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
-                if (a == 0 && threadIdx.x == 0) S.stalled = 1;
+                if (a == 0 && threadIdx.x == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
                 return;
             }
             alpha = sc[0] / sc[1];
@@ -1175,7 +1177,8 @@ __global__ __launch_bounds__(WGT) void k_cg1(TileArgs ta, const int32_t* __restr
     const double den = first ? dk : dk - beta * gk / S.alpha_hist[P];
     const double alpha = gk / den;
     if (!(den > 0.0) || !isfinite(alpha)) {  // M not positive definite along this direction
-        if (lead) {
+        if (lead) {  // x is still the iterate whose residual is rr
+            S.rnorm2 = rr;
             S.iters = it;
             S.stalled = 1;
         }
@@ -1375,7 +1378,8 @@ __global__ __launch_bounds__(WGT) void k_pipe(TileArgs ta, const int32_t* __rest
     const double den = first ? dk : dk - beta * gk / S.alpha_hist[P];
     const double alpha = gk / den;
     if (!(den > 0.0) || !isfinite(alpha)) {
-        if (lead) {
+        if (lead) {  // x is still the iterate whose residual is rr
+            S.rnorm2 = rr;
             S.iters = it;
             S.stalled = 1;
         }
@@ -2600,8 +2604,13 @@ int krylov_solve(tf_handle* h)
     int st = run(kind);
     if (kind == KIND_TL && st == TF_OK) twolevel_observe(h, h->last_krylov_iters);
     if (kind == KIND_TL && st != TF_OK) h->tl.fresh = false;  // the next solve rebuilds the coarse operator
-    if (st == TF_KRYLOV_STALL && (cg_kind || kind == KIND_TL)) {
-        // not positive definite (or too slow): BiCGSTAB from zero
+    bool unusable = false;  // a stalled tissue whose residual is above 1e-2 of its right-hand side
+    for (int s = 0; s < h->S; ++s)
+        unusable = unusable || (h->h_seg[s].active && h->h_seg[s].stalled &&
+                                solve_unusable(h->h_seg[s].rnorm2, h->h_seg[s].bnorm2));
+    if (st == TF_KRYLOV_STALL && (cg_kind || kind == KIND_TL) && unusable) {
+        // not positive definite (or too slow): BiCGSTAB from zero.  A CG solve that stagnated
+        // close to its tolerance is kept as it is (an inexact step).
         const int cg_iters = h->last_krylov_iters;
         h->fallbacks += 1;
         const int call = h->krylov_call;
