@@ -229,6 +229,16 @@ int tf_get_jacobian_bsr(tf_handle* h, int32_t* row_ptr, int32_t* col, double* va
  * ElementalElasticSystemForces, ElementalInternalViscousSystemForces (ShapeBase.cpp:2198-2226),
  * TriPointKe and TriPointKv as stored by the reference (ShapeBase.cpp:2619-2705). */
 int tf_element_dump(tf_handle* h, int32_t e, double dt, double* fe, double* fv, double* Ke, double* Kv);
+/* tf_element_dump for the n prisms e0 .. e0 + n - 1 without allocating: no hipMalloc, hipFree or
+ * allocating library call.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under stream or graph capture, tight inspection loops, harnesses
+ * that forbid allocation).  The kernel writes into a static device array of the library
+ * (room for 512 prisms, 2.8 MB, one copy per GPU, held until the process ends); tf_create resolves
+ * its address.  fe and fv receive [n][18], Ke and Kv [n][324], each prism as tf_element_dump gives
+ * it; any of them may be NULL.  n > 512 returns TF_INVALID.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (several host threads, any handles) race. */
+int tf_element_dump_range_noalloc(tf_handle* h, int32_t e0, int32_t n, double dt, double* fe, double* fv, double* Ke,
+                                  double* Kv);
 /* solveForDeltaU (NewtonRaphsonSolver.cpp:382-404): M dU = g by preconditioned Krylov. */
 int tf_solve(tf_handle* h, double* dU, int32_t* krylov_iters, double* rel_residual);
 /* checkConvergenceViaDeltaU + updateUkInIteration (NewtonRaphsonSolver.cpp:727-761). */

@@ -518,6 +518,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         tf_default_options(&h->opts);
         h->N = N; h->E = E; h->S = S;
         TF_HIP(hipGetDevice(&h->device));
+        scratch_init(h->device);  // the allocation-free element dump's scratch on this device
         TF_HIP(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
         TF_HIP(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
         if (hipDeviceGetAttribute(&h->n_xcd, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || h->n_xcd < 1)
@@ -935,8 +936,8 @@ int tf_element_dump(tf_handle* h, int32_t e, double dt, double* fe, double* fv, 
 {
     if (!h || e < 0 || e >= h->E || !(dt > 0)) return TF_INVALID;
     TF_GUARD_BEGIN
-    double* d_out = dev_alloc<double>(36 + 648);
-    launch_element_dump(h, e, dt, d_out);
+    double* d_out = dev_alloc<double>(TF_DUMP_DOUBLES);
+    launch_element_dump(h, e, 1, dt, d_out);
     std::vector<double> o(36 + 648);
     d2h(h, o.data(), d_out, o.size());
     TF_HIP(hipStreamSynchronize(h->stream));
@@ -945,6 +946,29 @@ int tf_element_dump(tf_handle* h, int32_t e, double dt, double* fe, double* fv, 
     if (fv) std::copy(o.begin() + 18, o.begin() + 36, fv);
     if (Ke) std::copy(o.begin() + 36, o.begin() + 360, Ke);
     if (Kv) std::copy(o.begin() + 360, o.end(), Kv);
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_element_dump_range_noalloc(tf_handle* h, int32_t e0, int32_t n, double dt, double* fe, double* fv, double* Ke,
+                                  double* Kv)
+{
+    if (!h) return TF_INVALID;
+    if (e0 < 0 || n < 1 || e0 > h->E - n || !(dt > 0))
+        return fail(h, TF_INVALID, "element range outside [0, n_elements) or dt <= 0");
+    if (n > TF_DUMP_SCRATCH_PRISMS)
+        return fail(h, TF_INVALID, "at most " + std::to_string(TF_DUMP_SCRATCH_PRISMS) +
+                                       " prisms per call (the static scratch holds that many element dumps)");
+    TF_GUARD_BEGIN
+    double* d_out = scratch_addr(h->device);  // resolved by tf_create on this device
+    if (!d_out) return fail(h, TF_INVALID, "the element-dump scratch of this device is not set up");
+    launch_element_dump(h, e0, n, dt, d_out);
+    const size_t m = (size_t)n;
+    if (fe) d2h(h, fe, d_out, 18 * m);
+    if (fv) d2h(h, fv, d_out + 18 * m, 18 * m);
+    if (Ke) d2h(h, Ke, d_out + 36 * m, 324 * m);
+    if (Kv) d2h(h, Kv, d_out + 360 * m, 324 * m);
+    TF_HIP(hipStreamSynchronize(h->stream));
     return TF_OK;
     TF_GUARD_END(h)
 }

@@ -14,6 +14,8 @@
 #include "tf_internal.hpp"
 #include "tf_element.hpp"
 
+#include <mutex>
+
 namespace {
 
 constexpr int EPB = 32;   // prisms per workgroup
@@ -369,11 +371,18 @@ __global__ __launch_bounds__(256) void k_residual(const int32_t* __restrict__ no
 }
 
 // Element-local arrays of one prism in the reference layout (parity dumps).
+// Prisms e0 .. e0 + n - 1, one workgroup each (prism e0 + k in workgroup k).  out holds n fe[18],
+// then n fv[18], then n Ke[324] (TriPointKe), then n Kv[324] (TriPointKv, stored transposed); every
+// entry is written (nothing accumulates), so out needs no clearing.
 __global__ void k_element_dump(const int32_t* conn, const double* xref, const double* invfg, const double* lam,
                                const double* mu, const double* eta, const double* uk, const double* un,
-                               const uint8_t* fixed, double inv_dt, int E, int e, double* out)
+                               const uint8_t* fixed, double inv_dt, int E, int e0, int n, double* out_base)
 {
-    // out: fe[18], fv[18], Ke[324] (TriPointKe), Kv[324] (TriPointKv, stored transposed)
+    const int k = blockIdx.x, e = e0 + k;
+    double* o_fe = out_base + 18 * (size_t)k;
+    double* o_fv = out_base + 18 * (size_t)n + 18 * (size_t)k;
+    double* o_ke = out_base + 36 * (size_t)n + 324 * (size_t)k;
+    double* o_kv = out_base + 360 * (size_t)n + 324 * (size_t)k;
     const int t = threadIdx.x;  // 36 lanes: one node pair each
     __shared__ tf::GPElastic sE[6];
     __shared__ tf::GPViscous sV[6];
@@ -400,8 +409,8 @@ __global__ void k_element_dump(const int32_t* conn, const double* xref, const do
         }
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) {
-                out[36 + (3 * a + i) * 18 + 3 * b + j] = ke[3 * i + j];
-                out[36 + 324 + (3 * b + j) * 18 + 3 * a + i] = kv[3 * i + j];
+                o_ke[(3 * a + i) * 18 + 3 * b + j] = ke[3 * i + j];
+                o_kv[(3 * b + j) * 18 + 3 * a + i] = kv[3 * i + j];
             }
     }
     if (t < 6) {
@@ -419,8 +428,8 @@ __global__ void k_element_dump(const int32_t* conn, const double* xref, const do
         }
         for (int i = 0; i < 3; ++i) {
             const bool fx = fixed[3 * id + i] != 0;
-            out[3 * a + i] = fx ? 0.0 : -f[i];
-            out[18 + 3 * a + i] = fx ? 0.0 : -fv[i];
+            o_fe[3 * a + i] = fx ? 0.0 : -f[i];
+            o_fv[3 * a + i] = fx ? 0.0 : -fv[i];
         }
     }
 }
@@ -671,11 +680,37 @@ void launch_bind_du(tf_handle* h)
     TF_HIP(hipGetLastError());
 }
 
-void launch_element_dump(tf_handle* h, int32_t e, double dt, double* d_out)
+void launch_element_dump(tf_handle* h, int32_t e0, int32_t n, double dt, double* d_out)
 {
-    hipLaunchKernelGGL(k_element_dump, dim3(1), dim3(64), 0, h->stream, h->conn, h->xref, h->invfg, h->lam, h->mu, h->eta,
-                       h->uk, h->un, h->fixed, 1.0 / dt, h->E, e, d_out);
+    hipLaunchKernelGGL(k_element_dump, dim3(n), dim3(64), 0, h->stream, h->conn, h->xref, h->invfg, h->lam, h->mu, h->eta,
+                       h->uk, h->un, h->fixed, 1.0 / dt, h->E, e0, n, d_out);
     TF_HIP(hipGetLastError());
+}
+
+// ---- the element dump's static scratch (callers that cannot allocate) ---------------------------
+// One copy per GPU, held until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: its device address comes from hipGetSymbolAddress, once per device with that
+// device current (tf_create), and only that address is handed to kernels and copies.
+static __device__ __align__(256) double scratch[(size_t)TF_DUMP_SCRATCH_PRISMS * TF_DUMP_DOUBLES];
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[TF_MAX_DEVICES] = {};
+
+void scratch_init(int device)
+{
+    if (device < 0 || device >= TF_MAX_DEVICES) throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) throw tf_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                  "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_scratch_addr[device] = static_cast<double*>(p);
+}
+
+double* scratch_addr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < TF_MAX_DEVICES ? g_scratch_addr[device] : nullptr;
 }
 
 void launch_grow(tf_handle* h, double dt)

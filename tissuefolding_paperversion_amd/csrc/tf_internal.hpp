@@ -293,7 +293,16 @@ struct tf_handle {
 void launch_assemble_elements(tf_handle* h, double dt);
 void launch_assemble_gather(tf_handle* h, double dt);
 void launch_residual(tf_handle* h, double dt);
-void launch_element_dump(tf_handle* h, int32_t e, double dt, double* d_out);
+void launch_element_dump(tf_handle* h, int32_t e0, int32_t n, double dt, double* d_out);  // n prisms from e0
+// Static scratch of the allocation-free element dump (tf_assemble.hip): room for
+// TF_DUMP_SCRATCH_PRISMS prisms of TF_DUMP_DOUBLES values, one copy per GPU.  scratch_init resolves
+// its device address for the current device (once per device); scratch_addr returns it (null
+// before scratch_init ran on that device).
+#define TF_DUMP_DOUBLES 684          // fe 18 + fv 18 + Ke 324 + Kv 324 per prism
+#define TF_DUMP_SCRATCH_PRISMS 512   // 2.8 MB
+#define TF_MAX_DEVICES 64
+void scratch_init(int device);
+double* scratch_addr(int device);
 void launch_grow(tf_handle* h, double dt);
 void launch_surfaces(tf_handle* h);
 void launch_copy(tf_handle* h, double* dst, const double* src, int64_t n);
