@@ -278,6 +278,12 @@ int tf_adjoint_step(tf_handle* h, double dt, const double* xbar_next, const doub
                     double* xbar_prev, int32_t* krylov_iters);
 int tf_adjoint_get(tf_handle* h, double* rate_bar, double* lambda_bar, double* mu_bar, double* fg_bar);
 
+/* The two-level preconditioner's coarse operator Ac = P^T M P of the current Jacobian and the FP64
+ * inverse of the operator its last rebuild factored ([nc][nc] row-major each; either may be NULL;
+ * call with both NULL to learn nc).  No reference counterpart: diagnostics of this implementation's
+ * preconditioner.  TF_INVALID before a two-level solve has built one. */
+int tf_get_coarse_operator(tf_handle* h, int32_t* nc, double* Ac, double* Ac_inv);
+
 /* ---- tuning (no reference counterpart: schedule choices of this implementation) ----------------- */
 /* Row tiling of the Krylov kernels: workgroup size wg in {256, 512, 1024} and tile capacity
  * 1 <= tile_cap <= min(4 wg, 2048) blocks (a tile never holds more than wg/3 block rows). */

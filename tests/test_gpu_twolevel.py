@@ -171,3 +171,21 @@ def test_option_validation():
         s.set_options(coarse_aggregates=-1)
     with pytest.raises(L.TfError):
         s.set_options(column_blocks=2)
+
+
+@pytest.mark.parametrize("make", [_ellipse, configs.config2], ids=["ellipse", "config2"])
+def test_coarse_inverse_matches_a_dense_inverse(make):
+    """The coarse operator's inverse from the blocked Gauss-Jordan sweep (FP64, 64-wide pivot
+    blocks, no pivoting) against numpy's LU inverse of the same operator: within 1e-10 relative,
+    symmetric to rounding; the operator itself symmetric positive definite."""
+    s = _assembled(make(), **TL)
+    s.solveForDeltaU()  # a solve outside the Newton loop rebuilds the coarse operator
+    Ac, Ai = s.coarse_operator()
+    nc = Ac.shape[0]
+    print(f"coarse operator: {nc} unknowns", flush=True)
+    assert nc > 64  # more than one pivot block
+    assert np.max(np.abs(Ac - Ac.T)) <= 1e-12 * np.max(np.abs(Ac))
+    np.linalg.cholesky(Ac)
+    ref = np.linalg.inv(Ac)
+    assert np.max(np.abs(Ai - ref)) <= 1e-10 * np.max(np.abs(ref))
+    assert np.array_equal(Ai, Ai.T)

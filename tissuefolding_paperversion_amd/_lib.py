@@ -30,7 +30,7 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_set_ablated_nodes", "tf_set_growth_state", "tf_get_growth_rotation", "tf_apply_jacobian",
            "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get",
            "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs",
-           "tf_set_growth_increments", "tf_element_dump_range_noalloc"]
+           "tf_set_growth_increments", "tf_element_dump_range_noalloc", "tf_get_coarse_operator"]
 
 
 class TfError(RuntimeError):
@@ -107,6 +107,7 @@ def load(path: str | None = None):
         "tf_get_jacobian_bsr": ([vp, vp, vp, vp], i32),
         "tf_element_dump": ([vp, i32, dbl, vp, vp, vp, vp], i32),
         "tf_element_dump_range_noalloc": ([vp, i32, i32, dbl, vp, vp, vp, vp], i32),
+        "tf_get_coarse_operator": ([vp, vp, vp, vp], i32),
         "tf_solve": ([vp, vp, vp, vp], i32),
         "tf_update_uk": ([vp, vp, vp], i32),
         "tf_nr_step": ([vp, dbl, C.POINTER(StepStats)], i32),

@@ -29,8 +29,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + flags() + srcs + ["-o", tmp, "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
-                                      "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [HIPCC] + flags() + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

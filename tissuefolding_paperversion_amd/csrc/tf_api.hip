@@ -1695,6 +1695,16 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
     TF_GUARD_END(h)
 }
 
+int tf_get_coarse_operator(tf_handle* h, int32_t* nc, double* Ac, double* Ac_inv)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (twolevel_export(h, nc, Ac, Ac_inv) != 0)
+        return fail(h, TF_INVALID, "no coarse operator: the two-level preconditioner has not been built by a solve");
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
 int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms)
 {
     if (!h || !avg_ms || n_launches < 2) return TF_INVALID;

@@ -273,15 +273,16 @@ struct tf_handle {
         int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank
         int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
         double *agg_part = nullptr, *rc = nullptr;          // [A][2], [nc]
-        double* Ac = nullptr;
+        double* Ac = nullptr;                               // [ldc][ldc] P^T M P, then -Ac^-1 (lower tiles)
+        int32_t ldc = 0;                                    // nc rounded up to the sweep's tile (64)
+        double *swP = nullptr, *swO = nullptr, *swD = nullptr;  // the sweep's panels [ldc][64] and pivot inverses [2][64][64]
         void* Ainv = nullptr;                               // [nc][ldA] FP16 copy of Ac^-1 / scale
         double* ascale = nullptr;                           // [2] scale and its reciprocal
         int32_t ldA = 0;                                    // row stride of Ainv (halves)
         int32_t L = 1, ncol = 0;                            // nodes per column, columns
         int32_t* col_blk = nullptr;                         // [ncol][L][L] BSR block of every node pair or -1
         float* dcinv = nullptr;                             // [ncol][3L][3L] FP32 inverse column blocks
-        int32_t* info = nullptr;
-        void* blas = nullptr;                               // rocblas_handle
+        int32_t* info = nullptr;                            // the sweep's failure flag
         std::vector<int32_t> h_node_agg;
     } tl;
 
@@ -350,6 +351,7 @@ void warm_start_clear(tf_handle* h);
 void twolevel_free(tf_handle* h);
 void twolevel_invalidate(tf_handle* h);  // the pattern changed
 void twolevel_setup(tf_handle* h);       // builds the coarse space if twolevel_active and not built
+int twolevel_export(tf_handle* h, int32_t* nc, double* Ac, double* Ainv);  // -1: no fresh coarse operator
 // ms per Krylov iteration (whole replays) of the solver krylov_solve would run for a symmetric
 // system: the two-level PCG when twolevel_active, else the single-kernel CG
 double krylov_time_iteration(tf_handle* h, int n_iters);

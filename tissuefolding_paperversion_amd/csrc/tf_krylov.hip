@@ -20,8 +20,6 @@
 //   p for neighbouring rows on the fly and writes it for its own rows (ping-pong buffers).
 #include "tf_internal.hpp"
 #include <hip/hip_fp16.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <cmath>
 #include <math.h>
@@ -870,7 +868,7 @@ __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restri
                                               const int32_t* __restrict__ pair_b, const int32_t* __restrict__ pair_ptr,
                                               const int32_t* __restrict__ pair_blk, const int32_t* __restrict__ blk_row,
                                               const int32_t* __restrict__ col, const double* __restrict__ val, int nb,
-                                              double* __restrict__ Ac)
+                                              double* __restrict__ Ac, int ldc)
 {
     const int pr = blockIdx.x;
     const int a = pair_a[pr], b = pair_b[pr];
@@ -911,20 +909,293 @@ __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restri
             double t = 0.0;
 #pragma unroll
             for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
-            Ac[(size_t)(tl.coff[a] + pi) * tl.nc + tl.coff[b] + ri] = t;
+            Ac[(size_t)(tl.coff[a] + pi) * ldc + tl.coff[b] + ri] = t;
         }
     }
 }
 
-// the full symmetric inverse, rounded to FP32, from the triangle the Cholesky inverse leaves
-// (column-major lower: element (i, j), i >= j, at Ac[j * n + i])
-// The scale of the FP16 copy: the largest diagonal entry of Ac^-1, which bounds every entry of a
-// positive definite matrix (|a_ij| <= sqrt(a_ii a_jj)), so the scaled entries lie in [-1, 1].
-__global__ __launch_bounds__(256) void k_tl_scale(const double* __restrict__ Ac, int n, double* __restrict__ sc)
+// ---- the coarse inverse: blocked Gauss-Jordan sweep ----------------------------------------------
+// Ac (symmetric positive definite, n x n) is held row-major with leading dimension ldc = n rounded
+// up to SW_B, the padding rows and columns holding the identity, so the padded matrix stays
+// positive definite and its inverse's leading n x n block is Ac^-1.  Sweeping the pivot blocks
+// K = 0, 1, ... in turn, each with D = A_KK^-1 (no pivoting: every A_KK met is a Schur complement of
+// a positive definite matrix, hence positive definite),
+//     A_IJ <- A_IJ - A_IK D A_KJ   (I, J != K),   A_IK <- A_IK D,   A_KJ <- D A_KJ,   A_KK <- -D,
+// leaves -Ac^-1 in place.  The matrix stays symmetric, so only the tiles I >= J are kept (diagonal
+// tiles whole).  One sweep step is two launches: k_sw_panel forms P_I = A_IK D and saves the old
+// column O_I = A_IK for every I != K; k_sw_update then updates every lower tile from P and O (no tile
+// reads another's), writes the new column K, and -- first workgroup of the grid -- inverts the
+// next pivot tile right after updating it, so the serial 64-pivot inversion overlaps the update.
+// FP64 throughout; every sum runs in a fixed order.
+constexpr int SW_B = 64;        // tile size of the sweep
+constexpr int SW_LD = SW_B + 1;  // LDS row stride of a staged tile (odd: columns hit distinct banks)
+constexpr int SW_H = 33;         // LDS row stride of half a tile (32 inner columns + 1)
+
+typedef double sw_v4d __attribute__((ext_vector_type(4)));
+
+// 64 x 64 tile products on the matrix cores, one 32 x 32 quarter per wave (v_mfma_f64_16x16x4f64,
+// 2 x 2 blocks of 16 x 16).  Operand layout of the instruction: lane l holds A[l % 16][l / 16] of
+// the 16 x 4 left operand and B[l / 16][l % 16] of the 4 x 16 right one; output register r of lane
+// l is D[l / 16 + 4 r][l % 16].  acc[bi][bj] is block (2 wr + bi, 2 wc + bj) of the wave's quarter.
+struct SwQuarter {
+    int l, wr, wc;
+    __device__ SwQuarter() : l(threadIdx.x & 63), wr((threadIdx.x >> 6) >> 1), wc((threadIdx.x >> 6) & 1) {}
+    __device__ int row(int bi, int r) const { return 16 * (2 * wr + bi) + (l >> 4) + 4 * r; }  // output row of reg r
+    __device__ int col(int bj) const { return 16 * (2 * wc + bj) + (l & 15); }
+};
+
+// acc += sign * A B over 32 inner indices, A given by rows (a[row * lda + k]) and B by columns
+// (b[col * ldb + k]), both staged in LDS
+__device__ inline void sw_mfma_rows_cols(const SwQuarter& q, const double* a, int lda, const double* b, int ldb, double sign,
+                                         sw_v4d (&acc)[2][2])
+{
+    const int r = q.l & 15, k0 = q.l >> 4;
+#pragma unroll
+    for (int k = 0; k < 32; k += 4) {
+        const double a0 = sign * a[(16 * (2 * q.wr) + r) * lda + k + k0];
+        const double a1 = sign * a[(16 * (2 * q.wr + 1) + r) * lda + k + k0];
+        const double b0 = b[(16 * (2 * q.wc) + r) * ldb + k + k0];
+        const double b1 = b[(16 * (2 * q.wc + 1) + r) * ldb + k + k0];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+}
+
+// In-place Gauss-Jordan inverse of a 64 x 64 tile held in registers by 256 threads: thread t holds
+// v[c] = T[t / 4][16 (t % 4) + c].  Each pivot p publishes its row and column through LDS (two
+// buffers of 68 + 64 doubles, alternated, so one barrier per pivot suffices: a buffer is written
+// again only after every thread passed the barrier behind its last reads).  A pivot that is not
+// positive and finite sets *bad.
+__device__ void sw_tile_inverse(double (&v)[16], double* buf, int* bad)
+{
+    const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16, pad = j0 >> 4;
+    bool ok = true;
+    for (int p = 0; p < SW_B; ++p) {
+        double* rowb = buf + (p & 1) * 132;  // pivot row, entry j at j + j / 16
+        double* colb = rowb + 68;            // pivot column
+        if (i == p) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) rowb[j0 + c + pad] = v[c];
+        }
+        if (p >= j0 && p < j0 + 16) {
+            double cv = 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) cv = j0 + c == p ? v[c] : cv;
+            colb[i] = cv;
+        }
+        __syncthreads();
+        // every load unconditional and issued together (a load used in one arm of a select is sunk
+        // into a branch of its own and waited for alone); the pivot column is patched afterwards by
+        // register selects
+        const double piv = rowb[p + (p >> 4)];
+        const double f = colb[i];
+        double rp[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) rp[c] = rowb[j0 + c + pad];
+        // the row is needed here: its loads are issued with the pivot's, ahead of the reciprocal,
+        // instead of one by one after it
+        asm volatile("" ::"v"(rp[0]), "v"(rp[1]), "v"(rp[2]), "v"(rp[3]), "v"(rp[4]), "v"(rp[5]), "v"(rp[6]), "v"(rp[7]));
+        asm volatile("" ::"v"(rp[8]), "v"(rp[9]), "v"(rp[10]), "v"(rp[11]), "v"(rp[12]), "v"(rp[13]), "v"(rp[14]),
+                     "v"(rp[15]));
+        ok = ok && piv > 0.0 && isfinite(piv);
+        // 1 / piv: the hardware reciprocal refined by two Newton steps (within an ulp or two; the
+        // inverse only preconditions)
+        double ip = __builtin_amdgcn_rcp(piv);
+        ip = ip * (2.0 - piv * ip);
+        ip = ip * (2.0 - piv * ip);
+        const double fi = i == p ? 0.0 : f * ip;  // the pivot row is scaled, the others reduced
+        const double sc = i == p ? ip : 1.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = sc * (v[c] - fi * rp[c]);
+        const double pc = i == p ? ip : -fi;  // the new pivot-column entry
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = j0 + c == p ? pc : v[c];
+    }
+    if (!ok && t == 0) *bad = 1;
+}
+
+// Ac's buffer before the assembly: zero, with ones on the padding diagonal (rows n .. ldc - 1)
+__global__ __launch_bounds__(256) void k_sw_clear(double* __restrict__ A, int n, int ldc)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)ldc * ldc) return;
+    const int i = (int)(k / ldc), j = (int)(k % ldc);
+    A[k] = (i == j && i >= n) ? 1.0 : 0.0;
+}
+
+// D_0 = A_00^-1, and the failure flag cleared
+__global__ __launch_bounds__(256) void k_sw_first(const double* __restrict__ A, int ldc, double* __restrict__ D, int* bad)
+{
+    __shared__ double buf[2 * 132];
+    const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16;
+    if (t == 0) *bad = 0;
+    double v[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = A[(size_t)i * ldc + j0 + c];
+    sw_tile_inverse(v, buf, bad);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) D[i * SW_B + j0 + c] = v[c];
+}
+
+// Step K, workgroup I != K: O_I = A_IK (from the stored lower tile: A_IK itself for I > K, the
+// transpose of A_KI for I < K) and P_I = O_I D on the matrix cores (D staged by half tiles)
+__global__ __launch_bounds__(256) void k_sw_panel(const double* __restrict__ A, int ldc, int K, const double* __restrict__ D,
+                                                  double* __restrict__ Pb, double* __restrict__ Ob)
+{
+    const int I = blockIdx.x;
+    if (I == K) return;
+    __shared__ double sO[SW_B * SW_LD];  // O_I, 33.3 KB
+    __shared__ double sD[32 * SW_B];     // half of D's rows, 16 KB
+    const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int j = j0 + c;
+        // thread (i, j) loads stored element (row i, col j) of the tile holding A_IK or A_KI
+        const double v = I > K ? A[(size_t)(SW_B * I + i) * ldc + SW_B * K + j] : A[(size_t)(SW_B * K + i) * ldc + SW_B * I + j];
+        if (I > K) sO[i * SW_LD + j] = v;
+        else sO[j * SW_LD + i] = v;  // O_I = (A_KI)^T
+    }
+    __syncthreads();
+    double* O = Ob + (size_t)I * SW_B * SW_B;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) O[i * SW_B + j0 + c] = sO[i * SW_LD + j0 + c];
+    const SwQuarter q;
+    sw_v4d acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = sw_v4d{0.0, 0.0, 0.0, 0.0};
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sD[t + 256 * e] = D[SW_B * 32 * h + t + 256 * e];  // rows 32 h .. 32 h + 31
+        __syncthreads();
+        // B[m][j] = D[m][j]: lane l takes sD[(m0 + l / 16) * 64 + col]
+        const int r = q.l & 15, k0 = q.l >> 4;
+#pragma unroll
+        for (int k = 0; k < 32; k += 4) {
+            const double a0 = sO[(16 * (2 * q.wr) + r) * SW_LD + 32 * h + k + k0];
+            const double a1 = sO[(16 * (2 * q.wr + 1) + r) * SW_LD + 32 * h + k + k0];
+            const double b0 = sD[(k + k0) * SW_B + 16 * (2 * q.wc) + r];
+            const double b1 = sD[(k + k0) * SW_B + 16 * (2 * q.wc + 1) + r];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double* P = Pb + (size_t)I * SW_B * SW_B;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) P[q.row(bi, r) * SW_B + q.col(bj)] = acc[bi][bj][r];
+}
+
+// lower tile number b (row-major over I >= J) -> (I, J)
+__device__ inline void sw_tile_of(int b, int& I, int& J)
+{
+    I = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > b) --I;
+    while ((I + 1) * (I + 2) / 2 <= b) ++I;
+    J = b - I * (I + 1) / 2;
+}
+
+// Step K over every lower tile (grid nt (nt + 1) / 2).  Workgroup 0 takes tile (K+1, K+1) -- and
+// tile (0, 0) the workgroup that tile would have had -- so that the next pivot tile is updated and
+// inverted (into Dn) first.
+__global__ __launch_bounds__(256) void k_sw_update(double* __restrict__ A, int ldc, int K, int nt,
+                                                   const double* __restrict__ Pb, const double* __restrict__ Ob,
+                                                   const double* __restrict__ D, double* __restrict__ Dn, int* bad)
+{
+    int b = blockIdx.x;
+    const int look = K + 1 < nt ? (K + 1) * (K + 2) / 2 + (K + 1) : -1;  // index of tile (K+1, K+1)
+    if (look >= 0) b = b == 0 ? look : (b == look ? 0 : b);
+    int I, J;
+    sw_tile_of(b, I, J);
+    const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16;
+    double* a = A + (size_t)(SW_B * I + i) * ldc + SW_B * J + j0;  // this thread's 16 entries
+    if (I == K && J == K) {  // A_KK <- -D
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = -D[i * SW_B + j0 + c];
+        return;
+    }
+    if (J == K) {  // I > K: A_IK <- P_I
+        const double* P = Pb + (size_t)I * SW_B * SW_B;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = P[i * SW_B + j0 + c];
+        return;
+    }
+    if (I == K) {  // J < K: A_KJ <- D A_KJ = (P_J)^T
+        const double* P = Pb + (size_t)J * SW_B * SW_B;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = P[(j0 + c) * SW_B + i];
+        return;
+    }
+    // A_IJ <- A_IJ - P_I O_J^T on the matrix cores, the inner dimension in two halves staged in LDS;
+    // the next pivot tile is then inverted through the same buffer (33.8 KB)
+    __shared__ double sbuf[2 * SW_B * SW_H];
+    double* sP = sbuf;
+    double* sO = sbuf + SW_B * SW_H;
+    const SwQuarter q;
+    double* c0 = A + (size_t)(SW_B * I) * ldc + SW_B * J;  // the tile
+    sw_v4d acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[bi][bj][r] = c0[(size_t)q.row(bi, r) * ldc + q.col(bj)];
+    const double* P = Pb + (size_t)I * SW_B * SW_B;
+    const double* O = Ob + (size_t)J * SW_B * SW_B;
+    for (int h = 0; h < 2; ++h) {
+        // rows 0..63, inner columns 32 h .. 32 h + 31 of P_I and O_J: 2048 values each, 8 per thread
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int x = t + 256 * e, r = x >> 5, m = x & 31;
+            sP[r * SW_H + m] = P[r * SW_B + 32 * h + m];
+            sO[r * SW_H + m] = O[r * SW_B + 32 * h + m];
+        }
+        __syncthreads();
+        sw_mfma_rows_cols(q, sP, SW_H, sO, SW_H, -1.0, acc);  // B[k][j] = O_J[j][k]
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) c0[(size_t)q.row(bi, r) * ldc + q.col(bj)] = acc[bi][bj][r];
+    if (I == K + 1 && J == K + 1) {  // the next pivot: invert the updated tile now
+        double* T = sbuf;            // SW_B * SW_LD doubles fit; the last barrier above freed the buffer
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) T[q.row(bi, r) * SW_LD + q.col(bj)] = acc[bi][bj][r];
+        __syncthreads();
+        double v[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = T[i * SW_LD + j0 + c];
+        __syncthreads();  // T is reused as the pivot buffers
+        sw_tile_inverse(v, sbuf, bad);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) Dn[i * SW_B + j0 + c] = v[c];
+    }
+}
+
+// The scale of the FP16 copy: the largest diagonal entry of Ac^-1 (the sweep leaves -Ac^-1), which
+// bounds every entry of a positive definite matrix (|a_ij| <= sqrt(a_ii a_jj)), so the scaled
+// entries lie in [-1, 1].
+__global__ __launch_bounds__(256) void k_tl_scale(const double* __restrict__ A, int n, int ldc, double* __restrict__ sc)
 {
     __shared__ double sh[256];
     double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, Ac[(size_t)i * n + i]);
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, -A[(size_t)i * ldc + i]);
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -938,15 +1209,15 @@ __global__ __launch_bounds__(256) void k_tl_scale(const double* __restrict__ Ac,
     }
 }
 
-// Symmetric FP16 copy of Ac^-1 / scale (rocSOLVER leaves the inverse in the lower triangle of the
-// column-major Ac), rows of ld halves with zeros past column n.
-__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ Ac, __half* __restrict__ Ainv, int n, int ld,
-                                                const double* __restrict__ sc)
+// Symmetric FP16 copy of Ac^-1 / scale from the lower triangle the sweep leaves (negated), rows of
+// ld halves with zeros past column n.
+__global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ A, int ldc, __half* __restrict__ Ainv, int n,
+                                                int ld, const double* __restrict__ sc)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (size_t)n * ld) return;
     const int i = (int)(k / ld), j = (int)(k % ld);
-    const double v = j < n ? (i >= j ? Ac[(size_t)j * n + i] : Ac[(size_t)i * n + j]) * sc[1] : 0.0;
+    const double v = j < n ? -(i >= j ? A[(size_t)i * ldc + j] : A[(size_t)j * ldc + i]) * sc[1] : 0.0;
     Ainv[k] = __float2half((float)v);
 }
 
@@ -1738,7 +2009,7 @@ void tl_release(tf_handle* h)
     for (int32_t** p : {&tl.node_agg, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac})
+    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac, &tl.swP, &tl.swO, &tl.swD})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     if (tl.ascale) { (void)hipFree(tl.ascale); tl.ascale = nullptr; }
@@ -1995,19 +2266,15 @@ static void twolevel_build(tf_handle* h)
     tl.ncol = ncol;
     tl.agg_part = tl_alloc<double>(2 * (size_t)A);
     tl.rc = tl_alloc<double>(nc);
-    tl.Ac = tl_alloc<double>((size_t)nc * nc);
+    tl.ldc = (nc + SW_B - 1) / SW_B * SW_B;  // the sweep's padded size
+    tl.Ac = tl_alloc<double>((size_t)tl.ldc * tl.ldc);
+    tl.swP = tl_alloc<double>((size_t)tl.ldc * SW_B);
+    tl.swO = tl_alloc<double>((size_t)tl.ldc * SW_B);
+    tl.swD = tl_alloc<double>(2 * (size_t)SW_B * SW_B);
     tl.ldA = (nc + 7) / 8 * 8;
     tl.Ainv = tl_alloc<__half>((size_t)nc * tl.ldA);
     tl.ascale = tl_alloc<double>(2);
     tl.info = tl_alloc<int32_t>(1);
-    if (!tl.blas) {
-        rocblas_handle bh = nullptr;
-        if (rocblas_create_handle(&bh) != rocblas_status_success)
-            throw std::runtime_error("two-level preconditioner: rocblas_create_handle failed");
-        tl.blas = bh;
-    }
-    if (rocblas_set_stream((rocblas_handle)tl.blas, h->stream) != rocblas_status_success)
-        throw std::runtime_error("two-level preconditioner: rocblas_set_stream failed");
     TF_HIP(hipStreamSynchronize(h->stream));  // the host vectors above are temporaries
     tl.n_agg = A;
     tl.nc = nc;
@@ -2067,35 +2334,73 @@ static void launch_tl_dcol(tf_handle* h)
     TF_HIP(hipGetLastError());
 }
 
-// Ac = P^T M P from the FP64 Jacobian, its Cholesky factor and inverse (rocSOLVER), symmetrised.
-// A matrix that is not positive definite leaves the two-level path off until the next refresh.
+// Ac = P^T M P from the FP64 Jacobian and its inverse by the blocked Gauss-Jordan sweep (FP64, on
+// the handle's stream, 2 launches per 64-wide pivot block), then the scaled FP16 copy.  A matrix
+// that is not positive definite (a pivot block with a pivot that is not positive and finite) leaves
+// the two-level path off until the next refresh.
 static void twolevel_refresh(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    const int nc = tl.nc;
-    launch_fill(h, tl.Ac, 0.0, (int64_t)nc * nc);
+    const int nc = tl.nc, ldc = tl.ldc, nt = ldc / SW_B;
+    const size_t n2 = (size_t)ldc * ldc;
+    hipLaunchKernelGGL(k_sw_clear, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, nc, ldc);
     hipLaunchKernelGGL(k_tl_ac, dim3(tl.n_pairs), dim3(WG), 0, h->stream, tl_args(h), tl.pair_a, tl.pair_b, tl.pair_ptr,
-                       tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tl.Ac);
+                       tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tl.Ac, ldc);
+    hipLaunchKernelGGL(k_sw_first, dim3(1), dim3(256), 0, h->stream, tl.Ac, ldc, tl.swD, tl.info);
+    for (int K = 0; K < nt; ++K) {
+        const double* D = tl.swD + (size_t)(K & 1) * SW_B * SW_B;
+        double* Dn = tl.swD + (size_t)((K + 1) & 1) * SW_B * SW_B;
+        hipLaunchKernelGGL(k_sw_panel, dim3(nt), dim3(256), 0, h->stream, tl.Ac, ldc, K, D, tl.swP, tl.swO);
+        hipLaunchKernelGGL(k_sw_update, dim3(nt * (nt + 1) / 2), dim3(256), 0, h->stream, tl.Ac, ldc, K, nt, tl.swP, tl.swO,
+                           D, Dn, tl.info);
+    }
+    // the FP16 copy is enqueued before the failure flag is read back: the device has work while the
+    // host waits, and the copy is simply not used after a failure
+    hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, tl.Ac, nc, ldc, tl.ascale);
+    const size_t n2h = (size_t)nc * tl.ldA;
+    hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, ldc,
+                       (__half*)tl.Ainv, nc, tl.ldA, tl.ascale);
     TF_HIP(hipGetLastError());
-    rocblas_handle bh = (rocblas_handle)tl.blas;
-    int32_t info = 0;
-    if (rocsolver_dpotrf(bh, rocblas_fill_lower, nc, tl.Ac, nc, tl.info) != rocblas_status_success)
-        throw std::runtime_error("two-level preconditioner: rocsolver_dpotrf failed");
-    TF_HIP(hipMemcpyAsync(&info, tl.info, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    int32_t bad = 0;
+    TF_HIP(hipMemcpyAsync(&bad, tl.info, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipStreamSynchronize(h->stream));
     tl.fresh = true;
     tl.age = 0;
     tl.base_it1 = -1;
     tl.stale = false;
-    tl.failed = info != 0;
-    if (tl.failed) return;
-    if (rocsolver_dpotri(bh, rocblas_fill_lower, nc, tl.Ac, nc, tl.info) != rocblas_status_success)
-        throw std::runtime_error("two-level preconditioner: rocsolver_dpotri failed");
-    hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, tl.Ac, nc, tl.ascale);
-    const size_t n2h = (size_t)nc * tl.ldA;
-    hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, (__half*)tl.Ainv, nc,
-                       tl.ldA, tl.ascale);
-    TF_HIP(hipGetLastError());
+    tl.failed = bad != 0;
+}
+
+// The coarse operator P^T M P of the current Jacobian (assembled again into a temporary) and the FP64
+// inverse the last refresh left (-Ac^-1 in the lower tiles, read back symmetric).  Diagnostics.
+int twolevel_export(tf_handle* h, int32_t* nc_out, double* Ac_out, double* Ainv_out)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    if (!tl.built || !tl.fresh || tl.failed) return -1;
+    const int nc = tl.nc, ldc = tl.ldc;
+    if (nc_out) *nc_out = nc;
+    const size_t n2 = (size_t)ldc * ldc;
+    if (Ainv_out) {
+        std::vector<double> A(n2);
+        TF_HIP(hipMemcpyAsync(A.data(), tl.Ac, n2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        TF_HIP(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < nc; ++i)
+            for (int j = 0; j < nc; ++j)
+                Ainv_out[(size_t)i * nc + j] = -(i >= j ? A[(size_t)i * ldc + j] : A[(size_t)j * ldc + i]);
+    }
+    if (Ac_out) {
+        double* tmp = tl_alloc<double>(n2);
+        hipLaunchKernelGGL(k_sw_clear, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tmp, nc, ldc);
+        hipLaunchKernelGGL(k_tl_ac, dim3(tl.n_pairs), dim3(WG), 0, h->stream, tl_args(h), tl.pair_a, tl.pair_b,
+                           tl.pair_ptr, tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tmp, ldc);
+        std::vector<double> A(n2);
+        TF_HIP(hipMemcpyAsync(A.data(), tmp, n2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        TF_HIP(hipStreamSynchronize(h->stream));
+        (void)hipFree(tmp);
+        for (int i = 0; i < nc; ++i)
+            for (int j = 0; j < nc; ++j) Ac_out[(size_t)i * nc + j] = A[(size_t)i * ldc + j];
+    }
+    return 0;
 }
 
 void twolevel_setup(tf_handle* h)
@@ -2112,7 +2417,6 @@ void twolevel_invalidate(tf_handle* h)
 void twolevel_free(tf_handle* h)
 {
     tl_release(h);
-    if (h->tl.blas) { (void)rocblas_destroy_handle((rocblas_handle)h->tl.blas); h->tl.blas = nullptr; }
 }
 
 namespace {

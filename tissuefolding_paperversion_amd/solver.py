@@ -323,6 +323,15 @@ class NewtonRaphsonSolver:
         self._check(self._lib.tf_adjoint_get(self.h, L.ptr(rb), L.ptr(lb), L.ptr(mb), L.ptr(fb)))
         return rb.reshape(-1, 3), lb, mb, fb.reshape(-1, 3, 3)
 
+    def coarse_operator(self):
+        """(Ac, Ac^-1): the two-level coarse operator of the current Jacobian and the FP64 inverse
+        the last coarse rebuild computed (diagnostics)."""
+        n = C.c_int32()
+        self._check(self._lib.tf_get_coarse_operator(self.h, C.byref(n), None, None))
+        Ac, Ai = np.zeros((n.value, n.value)), np.zeros((n.value, n.value))
+        self._check(self._lib.tf_get_coarse_operator(self.h, C.byref(n), L.ptr(Ac), L.ptr(Ai)))
+        return Ac, Ai
+
     # ---- tuning -----------------------------------------------------------------------------------
     def set_tiling(self, wg, tile_cap):
         self._check(self._lib.tf_set_tiling(self.h, int(wg), int(tile_cap)))
