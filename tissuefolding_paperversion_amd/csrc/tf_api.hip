@@ -1757,8 +1757,8 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver
-        if (twolevel_active(h))  // k_cg_q: SpMV on p = z + beta p_old: z, p_old read; p_new, q written
-            *bytes = mat + n * 8.0 * 4.0;
+        if (twolevel_active(h))  // k_tp_q: SpMV on p = z + beta p_old: z, p_old read; p_new, q written; P's rows
+            *bytes = mat + n * 8.0 * 4.0 + N * 18.0 * 8.0;
         else if (kind == TF_KRYLOV_CG1)  // SpMV; x,p,r,w,s,u,m,q read and x,p,r,w,s,u,m,q written once; D^-1 rows
             *bytes = mat + n * 8.0 * 16.0 + N * 72.0;
         else if (kind == TF_KRYLOV_PIPECG)  // SpMV on m; z,q,s,p,x,r,u,w,m read, the same and m' written; D^-1 rows

@@ -283,6 +283,18 @@ struct tf_handle {
         int32_t* col_blk = nullptr;                         // [ncol][L][L] BSR block of every node pair or -1
         float* dcinv = nullptr;                             // [ncol][3L][3L] FP32 inverse column blocks
         int32_t* info = nullptr;                            // the sweep's failure flag
+        // the solve in aggregate order (position r = node agg_nodes[r])
+        int32_t *prow_ptr = nullptr, *pcol = nullptr, *pblk = nullptr;  // [N+1], [nb], [nb] permuted Jacobian pattern
+        int32_t* pass_ptr = nullptr;                        // [A+1] row passes of every aggregate
+        int2* pass_rows = nullptr;                          // [passes] rows [x, y) of every pass
+        int4* first_pass = nullptr;                         // [A] rows and blocks of every aggregate's first pass
+        int32_t* agg_order = nullptr;                       // [A] aggregates by descending size (workgroup order)
+        double* pcoef_p = nullptr;                          // [N][3][6] rows of P by position
+        double *val_p = nullptr;                            // [9][nb] FP64 values in aggregate order
+        float* val32_p = nullptr;                           // [9][nb] FP32 values in aggregate order
+        double *tpart = nullptr, *ptq = nullptr, *rcx = nullptr;  // [SL_COUNT][A], [nc], [2][nc]
+        int32_t* tp_seg_ptr = nullptr;                      // {0, A}: the aggregates as the tissue's "tiles"
+        int32_t tp_max_blocks = 0;                          // most blocks of a pass (LDS products)
         std::vector<int32_t> h_node_agg;
     } tl;
 

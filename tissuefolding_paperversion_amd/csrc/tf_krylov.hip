@@ -862,6 +862,431 @@ __global__ __launch_bounds__(WG) void k_tl_init(TileArgs ta, const int32_t* __re
     tile_write<1>(ta, t, sl, vv);
 }
 
+// ---- the two-level PCG in aggregate order: two launches per iteration --------------------------
+// The solve runs in the order of the aggregates: position r holds node agg_nodes[r], aggregate a holds
+// positions [agg_ptr[a], agg_ptr[a+1]) (column by column), and the Jacobian is a row-permuted copy
+// (val_p, prow_ptr, pcol: the blocks of position r's row, columns as positions).  One workgroup per
+// aggregate then owns a contiguous run of rows of every vector, and the restriction P^T of its rows
+// is a sum inside the workgroup.  An iteration of parity P is
+//   k_tp_q<P>: beta from (r, z); p = z + beta p_old (own rows written, neighbours' formed on the fly);
+//              q = M p; the partials of (p, q) and of P^T q (the coarse unknowns of the aggregate);
+//   k_tp_z<P>: alpha = (r, z) / (p, q); r_c = rcx[P] - alpha P^T q, with rcx[P] = P^T r of the
+//              previous iteration (exact, summed from r by the aggregates), so the coarse right-hand
+//              side carries one step of rounding, never an accumulated drift; z_c = Ac^-1 r_c on the
+//              aggregate's coarse rows; x += alpha p, r -= alpha q, z = Dc^-1 r + P z_c on its rows;
+//              the partials of (r, z) and (r, r) (from the z applied), and rcx[1-P] = P^T r.
+// Every sum runs in a fixed order.  Partials are indexed by aggregate (TileArgs with T = n_agg).
+constexpr int TP_CAP = 2048;  // blocks per pass of a workgroup (48 KB of LDS products)
+
+struct TPArgs {
+    const int32_t* agg_ptr;    // [A+1] positions of every aggregate
+    const int32_t* agg_nodes;  // [N] node of every position
+    const int32_t* prow_ptr;   // [N+1] permuted blocks of every position's row
+    const int32_t* pcol;       // [nb] column position of every permuted block
+    const int32_t* pass_ptr;   // [A+1] passes of every aggregate
+    const int2* pass_rows;     // [passes] rows [x, y) of every pass (<= WG / 3 rows, <= TP_CAP blocks)
+    const int4* first_pass;    // [A] rows [x, y) and blocks [z, w) of every aggregate's first pass
+    const int32_t* order;      // [A] aggregate of every workgroup: largest first (the dispatcher deals
+                               // workgroups in order, so the small ones fill in behind the large ones)
+    const double* pcoef;       // [N][3][6] rows of P by position
+    const int32_t* coff;       // [A] first coarse unknown of every aggregate
+    const int32_t* krank;      // [A] coarse unknowns of every aggregate
+    const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
+    const __half2* Ainv;       // [nc][ldA / 2] Ac^-1 / scale in FP16 pairs
+    const double* ascale;      // [2] scale and its reciprocal
+    double* ptq;               // [nc] P^T q of the current iteration
+    double* rcx;               // [2][nc] P^T r (ping-pong by parity)
+    int ldA, nc, n_agg;
+};
+
+// q = M p over the aggregate's passes; own rows see p_new = z + beta p_old (beta = 0: p = z), and
+// the callback gets (position, component, q value, p value) for every own DoF (q = 0 without
+// products).
+template <typename VT, typename F>
+__device__ inline void tp_spmv(const TPArgs& tp, const VT* __restrict__ val, int nb, const double* __restrict__ z,
+                               const double* __restrict__ p_old, double beta, int ps0, int ps1, double* prod, F own,
+                               bool products = true)
+{
+    for (int k = ps0; k < ps1; ++k) {
+        const int2 rr = tp.pass_rows[k];
+        const int b0 = tp.prow_ptr[rr.x], b1 = tp.prow_ptr[rr.y];
+        for (int j = b0 + (int)threadIdx.x; products && j < b1; j += WG) {
+            const int c = tp.pcol[j];
+            const double x0 = z[3 * c] + beta * p_old[3 * c];
+            const double x1 = z[3 * c + 1] + beta * p_old[3 * c + 1];
+            const double x2 = z[3 * c + 2] + beta * p_old[3 * c + 2];
+            const double v0 = val[j], v1 = val[(size_t)nb + j], v2 = val[2 * (size_t)nb + j];
+            const double v3 = val[3 * (size_t)nb + j], v4 = val[4 * (size_t)nb + j], v5 = val[5 * (size_t)nb + j];
+            const double v6 = val[6 * (size_t)nb + j], v7 = val[7 * (size_t)nb + j], v8 = val[8 * (size_t)nb + j];
+            double* pr = prod + 3 * (j - b0);
+            pr[0] = v0 * x0 + v1 * x1 + v2 * x2;
+            pr[1] = v3 * x0 + v4 * x1 + v5 * x2;
+            pr[2] = v6 * x0 + v7 * x1 + v8 * x2;
+        }
+        __syncthreads();
+        const int task = threadIdx.x;  // at most one own DoF per lane: a pass holds <= WG / 3 rows
+        if (task < 3 * (rr.y - rr.x)) {
+            const int r = rr.x + task / 3, comp = task % 3;
+            double qv = 0.0;
+            const int j1 = tp.prow_ptr[r + 1];
+            for (int j = tp.prow_ptr[r]; products && j < j1; ++j) qv += prod[3 * (j - b0) + comp];
+            const int d = 3 * r + comp;
+            own(r, comp, qv, products ? z[d] + beta * p_old[d] : 0.0);
+        }
+        __syncthreads();  // prod is refilled by the next pass
+    }
+}
+
+// Workgroup sums of K values into partial slots of aggregate a (fixed wave and lane order).
+template <int K>
+__device__ inline void tp_reduce(double (&v)[K], double* out)
+{
+    constexpr int NW = WG / 64;
+    __shared__ double red[K][NW];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = wave_sum(v[k]);
+        if (l == 0) red[k][w] = t;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
+        out[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// r = b - M x0 (x0 given, in positions) or b, x = x0 or 0, p_old = 0, (b, b), and rcx[1] = P^T r
+template <typename VT>
+__global__ __launch_bounds__(WG) void k_tp_init(TileArgs ta, TPArgs tp, const VT* __restrict__ val, int nb,
+                                                const double* __restrict__ b_nat, const double* __restrict__ x0p,
+                                                double* __restrict__ x, double* __restrict__ r,
+                                                double* __restrict__ p_old)
+{
+    extern __shared__ double prod[];
+    const int a = tp.order[blockIdx.x];
+    const bool active = ta.seg[0].active;
+    double bb = 0.0, rc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    auto own = [&](int pos, int comp, double mx, double) {
+        const int d = 3 * pos + comp;
+        const double bv = active ? b_nat[3 * tp.agg_nodes[pos] + comp] : 0.0;
+        const double rv = (x0p && active) ? bv - mx : bv;
+        x[d] = (x0p && active) ? x0p[d] : 0.0;
+        r[d] = rv;
+        p_old[d] = 0.0;
+        bb += bv * bv;
+        const double* Pr = tp.pcoef + 18 * (size_t)pos + 6 * comp;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) rc[j] += Pr[j] * rv;
+    };
+    // the product is M x0 (beta = 0); without x0 there is none
+    tp_spmv<VT>(tp, val, nb, x0p, x0p, 0.0, tp.pass_ptr[a], tp.pass_ptr[a + 1], prod, own, x0p != nullptr);
+    double v[7] = {bb, rc[0], rc[1], rc[2], rc[3], rc[4], rc[5]};
+    __shared__ double out[7];
+    tp_reduce<7>(v, out);
+    if (threadIdx.x == 0) ta.partial[(size_t)SL_BB * ta.T + a] = out[0];
+    const int kr = tp.krank[a];
+    if ((int)threadIdx.x < kr) tp.rcx[tp.nc + tp.coff[a] + threadIdx.x] = out[1 + threadIdx.x];
+}
+
+// beta, p = z + beta p_old, q = M p, (p, q) and P^T q; the convergence, cap and stagnation test.
+// Each lane loads its first TP_PF blocks of the first pass (columns, values, then z and p_old of
+// those columns) and its own DoF's rows of P before the partials are summed, so those loads are in
+// flight while beta is formed.
+constexpr int TP_PF = 2;
+
+template <typename VT, int P>
+__global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* __restrict__ val, int nb,
+                                             const double* __restrict__ z, const double* __restrict__ p_old,
+                                             double* __restrict__ p_new, double* __restrict__ q, int maxit)
+{
+    extern __shared__ double prod[];
+    const int a = tp.order[blockIdx.x];
+    const int ps0 = tp.pass_ptr[a], ps1 = tp.pass_ptr[a + 1];
+    SegScalars& S = ta.seg[0];
+    // ps0 is never negative: testing it makes the aggregate's passes load with the tissue's state
+    if (seg_done(S) | (ps0 < 0)) return;
+    const int tid = threadIdx.x;
+    const int4 rec = tp.first_pass[a];  // rows and blocks of the first pass, one load
+    int2 rr = int2{rec.x, rec.y};
+    int b0 = rec.z, b1 = rec.w;
+    double pv[TP_PF][9], pz[TP_PF][3], pp[TP_PF][3];
+#pragma unroll
+    for (int k = 0; k < TP_PF; ++k) {
+        const int j = b0 + tid + k * WG;
+        const int jj = j < b1 ? j : b0;  // clamped: always a valid address
+        const int c = tp.pcol[jj];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) pv[k][e] = val[(size_t)e * nb + jj];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) { pz[k][e] = z[3 * c + e]; pp[k][e] = p_old[3 * c + e]; }
+    }
+    const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
+    double sc[3];
+    seg_sums_range<3>(ta, 0, ta.T, sl, sc);
+    const int it = S.iters;
+    const double best = S.best_rr_h[P];
+    const int best_it = S.best_it_h[P];
+    const bool lead = a == 0 && tid == 0;
+    {
+        const bool stagnant = it - best_it > 256;
+        const bool conv = sc[0] <= S.tol2 || (stagnant && best <= 100.0 * S.tol2);
+        const bool cap = !conv && (it >= maxit || !isfinite(sc[0]) || stagnant);
+        if (conv || cap) {
+            if (lead) {
+                S.rnorm2 = sc[0];
+                if (conv) S.converged = 1;
+                else S.stalled = 1;
+            }
+            return;
+        }
+    }
+    if (lead) {
+        const bool better = sc[0] < 0.5 * best;
+        S.best_rr_h[1 - P] = better ? sc[0] : best;
+        S.best_it_h[1 - P] = better ? it : best_it;
+    }
+    const double beta = it > 0 ? sc[1] / sc[2] : 0.0;
+    double pq = 0.0, pt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int ps = ps0; ps < ps1; ++ps) {
+        if (ps > ps0) {
+            rr = tp.pass_rows[ps];
+            b0 = tp.prow_ptr[rr.x];
+            b1 = tp.prow_ptr[rr.y];
+        }
+        const int jf = ps == ps0 ? b0 + TP_PF * WG : b0;  // the first pass's prefetched blocks come first
+        if (ps == ps0) {
+#pragma unroll
+            for (int k = 0; k < TP_PF; ++k) {
+                const int j = b0 + tid + k * WG;
+                if (j < b1) {
+                    const double x0 = pz[k][0] + beta * pp[k][0], x1 = pz[k][1] + beta * pp[k][1];
+                    const double x2 = pz[k][2] + beta * pp[k][2];
+                    double* pr = prod + 3 * (j - b0);
+                    pr[0] = pv[k][0] * x0 + pv[k][1] * x1 + pv[k][2] * x2;
+                    pr[1] = pv[k][3] * x0 + pv[k][4] * x1 + pv[k][5] * x2;
+                    pr[2] = pv[k][6] * x0 + pv[k][7] * x1 + pv[k][8] * x2;
+                }
+            }
+        }
+        for (int j = jf + tid; j < b1; j += WG) {
+            const int c = tp.pcol[j];
+            const double x0 = z[3 * c] + beta * p_old[3 * c];
+            const double x1 = z[3 * c + 1] + beta * p_old[3 * c + 1];
+            const double x2 = z[3 * c + 2] + beta * p_old[3 * c + 2];
+            const double v0 = val[j], v1 = val[(size_t)nb + j], v2 = val[2 * (size_t)nb + j];
+            const double v3 = val[3 * (size_t)nb + j], v4 = val[4 * (size_t)nb + j], v5 = val[5 * (size_t)nb + j];
+            const double v6 = val[6 * (size_t)nb + j], v7 = val[7 * (size_t)nb + j], v8 = val[8 * (size_t)nb + j];
+            double* pr = prod + 3 * (j - b0);
+            pr[0] = v0 * x0 + v1 * x1 + v2 * x2;
+            pr[1] = v3 * x0 + v4 * x1 + v5 * x2;
+            pr[2] = v6 * x0 + v7 * x1 + v8 * x2;
+        }
+        __syncthreads();
+        if (tid < 3 * (rr.y - rr.x)) {  // at most one own DoF per lane: a pass holds <= WG / 3 rows
+            const int r = rr.x + tid / 3, comp = tid % 3;
+            const int d = 3 * r + comp;
+            const double* Pr = tp.pcoef + 18 * (size_t)r + 6 * comp;
+            double P6[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) P6[j] = Pr[j];
+            const double pvn = z[d] + beta * p_old[d];
+            double qv = 0.0;
+            const int j1 = tp.prow_ptr[r + 1];
+            for (int j = tp.prow_ptr[r]; j < j1; ++j) qv += prod[3 * (j - b0) + comp];
+            q[d] = qv;
+            p_new[d] = pvn;
+            pq += pvn * qv;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) pt[j] += P6[j] * qv;
+        }
+        __syncthreads();  // prod is refilled by the next pass
+    }
+    double v[7] = {pq, pt[0], pt[1], pt[2], pt[3], pt[4], pt[5]};
+    __shared__ double out[7];
+    tp_reduce<7>(v, out);
+    if (tid == 0) ta.partial[(size_t)SL_PQ * ta.T + a] = out[0];
+    const int kr = tp.krank[a];
+    if (tid < kr) tp.ptq[tp.coff[a] + tid] = out[1 + tid];
+}
+
+// alpha, the coarse right-hand side, z_c, then x, r, z on the aggregate's rows, (r, z), (r, r) and
+// rcx[1-P] = P^T r.  INIT: r as given (the initial residual), r_c = rcx[1], no update.
+// Every global load that does not need alpha is issued first: the aggregate's rows of Ac^-1, this
+// lane's entries of rcx[P] and P^T q, and the first pass's r, x, p, q, Dc^-1 rows and P rows; alpha
+// is then formed, r_c = rcx[P] - alpha P^T q staged in LDS (FP32: it only meets the FP16 coarse
+// inverse), and z_c = Ac^-1 r_c.
+constexpr int TP_PAIRS = 8;  // column pairs per lane of every coarse row held in registers (nc <= 4096)
+
+template <int P, bool INIT, int L>
+__global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const double* __restrict__ p,
+                                             const double* __restrict__ q, double* __restrict__ x,
+                                             double* __restrict__ r, double* __restrict__ z)
+{
+    constexpr int D = 3 * L, CPP = WG / D, NV = 2 * TP_PAIRS;
+    const int a = tp.order[blockIdx.x];
+    const int n0 = tp.agg_ptr[a], ncols = (tp.agg_ptr[a + 1] - n0) / L;
+    const int c0 = tp.coff[a], kr = tp.krank[a];
+    SegScalars& S = ta.seg[0];
+    if (seg_done(S) | (n0 < 0) | (c0 < 0)) return;
+    extern __shared__ float tp_rcs[];  // r_c, ldA entries (zero past nc)
+    __shared__ double rl[WG];
+    __shared__ double zcs[6];
+    const int tid = threadIdx.x, slot = tid / D, li = tid % D;
+    const int nc = tp.nc, ld2 = tp.ldA / 2;
+    __half2 pre[6][TP_PAIRS];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int m = 0; m < TP_PAIRS; ++m)
+            pre[k][m] = tp.Ainv[(size_t)min(c0 + k, nc - 1) * ld2 + min(tid + WG * m, ld2 - 1)];
+    const double* rcp = tp.rcx + (size_t)P * nc;
+    double vx[NV], vq[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+        const int ic = min(tid + WG * m, nc - 1);  // clamped: always a valid address
+        vx[m] = rcp[ic];
+        vq[m] = INIT ? 0.0 : tp.ptq[ic];
+    }
+    bool own = slot < CPP && slot < ncols;
+    int g = n0 / L + (own ? slot : 0);  // global column; its positions gL .. gL + L - 1
+    int pos = g * L + li / 3;
+    int d = 3 * pos + li % 3;
+    double rv = r[d], xv = 0.0, pv = 0.0, qv = 0.0;
+    if (!INIT) { xv = x[d]; pv = p[d]; qv = q[d]; }
+    float Dr[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) Dr[j] = tp.dcinv[((size_t)g * D + li) * D + j];
+    double Pr[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Pr[j] = tp.pcoef[18 * (size_t)pos + 6 * (li % 3) + j];
+    double alpha = 0.0;
+    if (!INIT) {
+        const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
+        double sc[3];
+        seg_sums_range<3>(ta, 0, ta.T, sl, sc);
+        // (p, q) > 0 needs M positive definite, (r, z) > 0 the preconditioner: either failing stops
+        // the solve as stalled, x keeping residual sc[2]
+        if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
+            if (a == 0 && tid == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
+            return;
+        }
+        alpha = sc[0] / sc[1];
+    }
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+        const int i = tid + WG * m;
+        if (i < tp.ldA) tp_rcs[i] = i < nc ? (float)(vx[m] - alpha * vq[m]) : 0.f;
+    }
+    __syncthreads();
+    {
+        double sv[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            sv[k] = 0.0;
+#pragma unroll
+            for (int m = 0; m < TP_PAIRS; ++m) {
+                const int j = 2 * (tid + WG * m);
+                if (j < tp.ldA) {  // pairs past ldA hold clamped copies: skipped
+                    const float2 f = __half22float2(pre[k][m]);
+                    const float2 c2 = *reinterpret_cast<const float2*>(tp_rcs + j);
+                    sv[k] += (double)f.x * c2.x + (double)f.y * c2.y;
+                }
+            }
+        }
+        __shared__ double zc_red[6][WG / 64];
+        const int w = tid / 64, l = tid % 64;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double t = wave_sum(sv[k]);
+            if (l == 0) zc_red[k][w] = t;
+        }
+        __syncthreads();
+        if (tid < 6) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < WG / 64; ++k) t += zc_red[tid][k];
+            zcs[tid] = tid < kr ? t * tp.ascale[0] : 0.0;  // P's columns past the rank are zero
+        }
+        __syncthreads();
+    }
+    double rz = 0.0, rr = 0.0, rc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int sb = min(slot, CPP - 1) * D;  // lanes past the last whole column read a valid one
+    for (int cb = 0; cb < ncols; cb += CPP) {
+        if (cb > 0) {
+            own = slot < CPP && cb + slot < ncols;
+            g = n0 / L + (own ? cb + slot : 0);
+            pos = g * L + li / 3;
+            d = 3 * pos + li % 3;
+            rv = r[d];
+            if (!INIT) { xv = x[d]; pv = p[d]; qv = q[d]; }
+#pragma unroll
+            for (int j = 0; j < D; ++j) Dr[j] = tp.dcinv[((size_t)g * D + li) * D + j];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) Pr[j] = tp.pcoef[18 * (size_t)pos + 6 * (li % 3) + j];
+        }
+        if (!INIT) {
+            xv = xv + alpha * pv;
+            rv = rv - alpha * qv;
+            if (own) { x[d] = xv; r[d] = rv; }
+        }
+        rl[tid] = rv;
+        __syncthreads();
+        double zi = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) zi += (double)Dr[j] * rl[sb + j];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) zi += Pr[j] * zcs[j];
+        if (own) {
+            z[d] = zi;
+            rz += rv * zi;
+            rr += rv * rv;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) rc[j] += Pr[j] * rv;
+        }
+        __syncthreads();  // rl is refilled by the next pass
+    }
+    double v[8] = {rz, rr, rc[0], rc[1], rc[2], rc[3], rc[4], rc[5]};
+    __shared__ double out[8];
+    tp_reduce<8>(v, out);
+    if (tid == 0) {
+        ta.partial[(size_t)(SL_RZ0 + (1 - P)) * ta.T + a] = out[0];
+        ta.partial[(size_t)(SL_RR0 + (1 - P)) * ta.T + a] = out[1];
+        if (!INIT && a == 0) S.iters += 1;
+    }
+    if (tid < kr) tp.rcx[(size_t)(1 - P) * nc + c0 + tid] = out[2 + tid];
+}
+
+static size_t tp_z_lds(int ldA) { return sizeof(float) * (size_t)ldA; }  // r_c
+
+// the solve's vectors between node order and aggregate order
+__global__ __launch_bounds__(256) void k_tp_gather(const int32_t* __restrict__ agg_nodes, int n, const double* __restrict__ src,
+                                                   double* __restrict__ dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 3 * n) dst[k] = src[3 * agg_nodes[k / 3] + k % 3];
+}
+__global__ __launch_bounds__(256) void k_tp_scatter(const int32_t* __restrict__ agg_nodes, int n,
+                                                    const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 3 * n) dst[3 * agg_nodes[k / 3] + k % 3] = src[k];
+}
+// the Jacobian's values in aggregate order: block k of the permuted matrix is block pblk[k]
+template <typename VT>
+__global__ __launch_bounds__(256) void k_tp_permute(const int32_t* __restrict__ pblk, int nb, const VT* __restrict__ val,
+                                                    VT* __restrict__ val_p)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const int b = pblk[k];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) val_p[(size_t)e * nb + k] = val[(size_t)e * nb + b];
+}
+
 // Coarse operator Ac = P^T M P: one workgroup per pair of aggregates (a, b); every thread sums
 // the 6x6 products P_A^T M_AB P_B of a strided subset of the pair's blocks, then a fixed tree.
 __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restrict__ pair_a,
@@ -2007,10 +2432,15 @@ void tl_release(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
     for (int32_t** p : {&tl.node_agg, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
-                        &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk})
+                        &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk, &tl.prow_ptr, &tl.pcol,
+                        &tl.pblk, &tl.pass_ptr, &tl.tp_seg_ptr, &tl.agg_order})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac, &tl.swP, &tl.swO, &tl.swD})
+    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac, &tl.swP, &tl.swO, &tl.swD, &tl.pcoef_p, &tl.val_p,
+                       &tl.tpart, &tl.ptq, &tl.rcx})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (tl.pass_rows) { (void)hipFree(tl.pass_rows); tl.pass_rows = nullptr; }
+    if (tl.first_pass) { (void)hipFree(tl.first_pass); tl.first_pass = nullptr; }
+    if (tl.val32_p) { (void)hipFree(tl.val32_p); tl.val32_p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     if (tl.ascale) { (void)hipFree(tl.ascale); tl.ascale = nullptr; }
     if (tl.dcinv) { (void)hipFree(tl.dcinv); tl.dcinv = nullptr; }
@@ -2137,7 +2567,8 @@ static void twolevel_build(tf_handle* h)
     TF_HIP(hipMemcpyAsync(excl.data(), h->fixed, excl.size(), hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipStreamSynchronize(h->stream));
     for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
-    const int max_nc = std::min(4096, h->max_lds / (int)sizeof(double));
+    // k_tp_z stages two coarse vectors of ldA (nc rounded up to 8) floats in LDS beside ~4 KB of its own
+    const int max_nc = std::min(4096, (h->max_lds - 4096) / (2 * (int)sizeof(float)) / 8 * 8);
     // default: one aggregate per 53 nodes, at most 600 (at most 3,600 coarse unknowns; the dense
     // FP32 inverse every iteration reads stays under 52 MB, and its rebuild under ~15 ms)
     int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(600, N / 53));
@@ -2261,6 +2692,65 @@ static void twolevel_build(tf_handle* h)
     tl_upload(h, tl.pair_ptr, pair_ptr);
     tl_upload(h, tl.pair_blk, pair_blk);
     tl_upload(h, tl.col_blk, col_blk);
+    // the solve in aggregate order: position r holds node agg_nodes[r]; the Jacobian's rows follow
+    // the positions (a row keeps its blocks in column order), columns become positions
+    {
+        std::vector<int32_t> pos_of(N);
+        for (int r = 0; r < N; ++r) pos_of[agg_nodes[r]] = r;
+        std::vector<int32_t> prow(N + 1, 0), pcol(h->nb), pblk(h->nb);
+        for (int r = 0; r < N; ++r) {
+            const int n = agg_nodes[r];
+            prow[r + 1] = prow[r] + (h->h_row_ptr[n + 1] - h->h_row_ptr[n]);
+            for (int j = h->h_row_ptr[n], k = prow[r]; j < h->h_row_ptr[n + 1]; ++j, ++k) {
+                pblk[k] = j;
+                pcol[k] = pos_of[h->h_col[j]];
+            }
+        }
+        // passes: runs of an aggregate's rows with at most WG / 3 rows and TP_CAP blocks
+        std::vector<int32_t> pass_ptr(A + 1, 0);
+        std::vector<int2> pass_rows;
+        tl.tp_max_blocks = 1;
+        for (int a = 0; a < A; ++a) {
+            int r = agg_ptr[a];
+            while (r < agg_ptr[a + 1]) {
+                int r1 = r;
+                while (r1 < agg_ptr[a + 1] && r1 - r < WG / 3 && prow[r1 + 1] - prow[r] <= TP_CAP) ++r1;
+                if (r1 == r) throw std::runtime_error("two-level preconditioner: a block row exceeds a pass");
+                pass_rows.push_back(int2{r, r1});
+                tl.tp_max_blocks = std::max(tl.tp_max_blocks, prow[r1] - prow[r]);
+                r = r1;
+            }
+            pass_ptr[a + 1] = (int32_t)pass_rows.size();
+        }
+        std::vector<double> pcoef_p(18 * (size_t)N);
+        for (int r = 0; r < N; ++r)
+            std::copy(pcoef.begin() + 18 * (size_t)agg_nodes[r], pcoef.begin() + 18 * (size_t)agg_nodes[r] + 18,
+                      pcoef_p.begin() + 18 * (size_t)r);
+        tl_upload(h, tl.prow_ptr, prow);
+        tl_upload(h, tl.pcol, pcol);
+        tl_upload(h, tl.pblk, pblk);
+        tl_upload(h, tl.pass_ptr, pass_ptr);
+        tl_upload(h, tl.pass_rows, pass_rows);
+        std::vector<int4> first(A);
+        for (int a = 0; a < A; ++a) {
+            const int2 r = pass_rows[pass_ptr[a]];
+            first[a] = int4{r.x, r.y, prow[r.x], prow[r.y]};
+        }
+        tl_upload(h, tl.first_pass, first);
+        std::vector<int32_t> order(A);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return prow[agg_ptr[x + 1]] - prow[agg_ptr[x]] > prow[agg_ptr[y + 1]] - prow[agg_ptr[y]];
+        });
+        tl_upload(h, tl.agg_order, order);
+        tl_upload(h, tl.pcoef_p, pcoef_p);
+        std::vector<int32_t> sp = {0, A};
+        tl_upload(h, tl.tp_seg_ptr, sp);
+        tl.tpart = tl_alloc<double>((size_t)TF_RED_SLOTS * A);
+        tl.ptq = tl_alloc<double>(nc);
+        tl.rcx = tl_alloc<double>(2 * (size_t)nc);
+        TF_HIP(hipStreamSynchronize(h->stream));  // the host vectors above are temporaries
+    }
     tl.dcinv = tl_alloc<float>((size_t)ncol * 9 * L * L);
     tl.L = L;
     tl.ncol = ncol;
@@ -2291,6 +2781,41 @@ static TLArgs tl_args(tf_handle* h)
     return TLArgs{tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
                   reinterpret_cast<const __half2*>(tl.Ainv), tl.ascale, tl.ldA, tl.dcinv, tl.nc, tl.n_agg,
                   std::min(h->T, tl.n_agg)};
+}
+
+// the aggregates as the tissue's partial-sum units (one tissue per two-level handle)
+static TileArgs tp_tile_args(tf_handle* h)
+{
+    TileArgs ta = tile_args(h);
+    ta.info = nullptr;
+    ta.partial = h->tl.tpart;
+    ta.seg_tile_ptr = h->tl.tp_seg_ptr;
+    ta.T = h->tl.n_agg;
+    ta.nxcd = 1;
+    return ta;
+}
+
+static TPArgs tp_args(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    return TPArgs{tl.agg_ptr, tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_ptr, tl.pass_rows, tl.first_pass, tl.agg_order,
+                  tl.pcoef_p,
+                  tl.coff, tl.krank,
+                  tl.dcinv, reinterpret_cast<const __half2*>(tl.Ainv), tl.ascale, tl.ptq, tl.rcx, tl.ldA, tl.nc,
+                  tl.n_agg};
+}
+
+template <int P, bool INIT>
+static void launch_tp_z(tf_handle* h, const TileArgs& ta, const TPArgs& tp, const double* p, const double* q)
+{
+    const dim3 grid(tp.n_agg), blk(WG);
+    const size_t lds = tp_z_lds(tp.ldA);
+    switch (h->tl.L) {
+    case 1: hipLaunchKernelGGL((k_tp_z<P, INIT, 1>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
+    case 2: hipLaunchKernelGGL((k_tp_z<P, INIT, 2>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
+    case 3: hipLaunchKernelGGL((k_tp_z<P, INIT, 3>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
+    default: hipLaunchKernelGGL((k_tp_z<P, INIT, 4>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
+    }
 }
 
 // the two-level kernels at the handle's column length L (1..TL_MAX_LEVELS)
@@ -2480,21 +3005,37 @@ void launch_cg_q(tf_handle* h, const VT* val, int wg, int maxit = -1)
                            h->ks, p_old, p_new, h->kv, maxit);
 }
 
-// Two-level PCG: the CG buffers of enqueue_cg_iter; z = ks holds D^-1 r + P Ac^-1 P^T r.
-template <typename VT, int P>
-void enqueue_tl_iter(tf_handle* h, const VT* val)
+// the Jacobian's values in aggregate order, in the solver's value type
+template <typename VT>
+static const VT* tp_values(tf_handle* h)
 {
-    TileArgs ta = tile_args(h);
-    TLArgs tla = tl_args(h);
-    const dim3 grid(h->T), blk(WG);
+    if (sizeof(VT) == sizeof(double)) return reinterpret_cast<const VT*>(h->tl.val_p);
+    return reinterpret_cast<const VT*>(h->tl.val32_p);
+}
+
+static size_t tp_q_lds(tf_handle* h) { return 3 * sizeof(double) * (size_t)h->tl.tp_max_blocks; }
+
+template <typename VT, int P>
+static void launch_tp_q(tf_handle* h, const TileArgs& ta, const TPArgs& tp, int maxit)
+{
     double* p_old = P ? h->kph : h->kp;
     double* p_new = P ? h->kp : h->kph;
+    hipLaunchKernelGGL((k_tp_q<VT, P>), dim3(tp.n_agg), dim3(WG), tp_q_lds(h), h->stream, ta, tp, tp_values<VT>(h), h->nb,
+                       h->ks, p_old, p_new, h->kv, maxit);
+}
+
+// Two-level PCG in aggregate order (k_tp_q, k_tp_z): x = kx, r = kr, z = ks, q = kv, p ping-pong kp / kph,
+// all in positions; the solve gathers its start and scatters du at the end.
+template <typename VT, int P>
+void enqueue_tl_iter(tf_handle* h, const VT*)
+{
+    const TileArgs ta = tp_tile_args(h);
+    const TPArgs tp = tp_args(h);
     hipEvent_t ev[2];
     timer_begin(h, h->t_spmv, ev);
-    launch_cg_q<VT, P>(h, val, h->wg);
+    launch_tp_q<VT, P>(h, ta, tp, h->opts.krylov_max_iter);
     timer_end(h, h->t_spmv, ev);
-    launch_tl_xr<P, false>(h, ta, tla, p_new, h->kv, h->kx, h->kr);
-    launch_tl_cz(h, ta, tla, SL_RZ0 + (1 - P), SL_RR0 + (1 - P), 1);
+    launch_tp_z<P, false>(h, ta, tp, P ? h->kp : h->kph, h->kv);
 }
 
 // Single-kernel CG buffers: x = kx, p = kp, r = kr, w = kv, s = ks (own rows only) and the
@@ -2604,26 +3145,30 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
         hipLaunchKernelGGL(k_cg1_init<VT>, grid, blk, sizeof(double) * (3 * (size_t)h->tile_cap + h->wg), h->stream, ta,
                            h->row_ptr, h->col, val, h->nb, h->dinv, h->g, h->kx, h->kr, h->kv, h->ks, h->kp, h->kph,
                            h->kt, h->kr0, h->tile_cap);
-    else if (kind == KIND_TL)
-        hipLaunchKernelGGL(k_tl_init<VT>, grid, blk, 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, ta,
-                           h->row_ptr, h->col, val, h->nb, h->g, h->warm_guess, h->kx, h->kr, h->kp);
+    else if (kind == KIND_TL) {  // in aggregate order: the start gathered, r_0, P^T r_0
+        const double* x0p = nullptr;
+        if (h->warm_guess) {
+            hipLaunchKernelGGL(k_tp_gather, dim3((3 * h->N + 255) / 256), dim3(256), 0, h->stream, h->tl.agg_nodes, h->N,
+                               h->warm_guess, h->km);
+            x0p = h->km;
+        }
+        hipLaunchKernelGGL(k_tp_init<VT>, dim3(h->tl.n_agg), blk, tp_q_lds(h), h->stream, tp_tile_args(h), tp_args(h),
+                           tp_values<VT>(h), h->nb, h->g, x0p, h->kx, h->kr, h->kp);
+    }
     else if (kind == TF_KRYLOV_CG)
         hipLaunchKernelGGL(k_cg_init, grid, blk, 0, h->stream, ta, h->dinv, h->g, h->kx, h->kr, h->ks, h->kp);
     else
         hipLaunchKernelGGL(k_bicg_init, grid, blk, 0, h->stream, ta, h->g, h->kx, h->kr, h->kr0, h->kp, h->kv);
-    hipLaunchKernelGGL(k_seg_init, dim3(h->S), blk, 0, h->stream, ta, rtol2, atol2,
+    hipLaunchKernelGGL(k_seg_init, dim3(h->S), blk, 0, h->stream, kind == KIND_TL ? tp_tile_args(h) : ta, rtol2, atol2,
                        h->seg_rtol_on ? h->seg_rtol2 : nullptr);
-    if (kind == KIND_TL) {  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
-        const TLArgs tla = tl_args(h);
-        launch_tl_xr<0, true>(h, ta, tla, h->kp, h->kv, h->kx, h->kr);
-        launch_tl_cz(h, ta, tla, (int)SL_RZ0, (int)SL_RR0, 0);
-    }
+    if (kind == KIND_TL)  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
+        launch_tp_z<1, true>(h, tp_tile_args(h), tp_args(h), h->kp, h->kv);
     TF_HIP(hipGetLastError());
     // A captured graph holds at most TF_GRAPH_NODES kernel launches (an even number of
     // iterations); a chunk between convergence checks is one replay of it, at most iters_per_sync
     // iterations.  (The two-level PCG, three launches per iteration, checks every 10 iterations: a
     // solve's last chunk then overshoots by fewer no-op launches than it costs in checks.)
-    const int kpi = kind == KIND_TL ? 3 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
+    const int kpi = kind == KIND_TL ? 2 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
     int chunk = std::min(std::max(2, h->opts.iters_per_sync), std::max(2, TF_GRAPH_NODES / kpi));
     chunk -= chunk & 1;  // iterations come in parity pairs
     const int g_it = chunk, per_chunk = 1;
@@ -2740,7 +3285,20 @@ static bool twolevel_prepare(tf_handle* h)
     if (!tl.fresh || h->krylov_call < 0 ||
         (h->krylov_call == 0 && (tl.stale || tl.age >= std::max(1, h->opts.coarse_refresh))))
         twolevel_refresh(h);
-    if (!tl.failed) launch_tl_dcol(h);
+    if (!tl.failed) {
+        launch_tl_dcol(h);
+        // the values in aggregate order for the solve
+        const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
+        const dim3 grid((h->nb + 255) / 256), blk(256);
+        if (fp32) {
+            if (!tl.val32_p) tl.val32_p = tl_alloc<float>(9 * (size_t)h->nb);
+            hipLaunchKernelGGL(k_tp_permute<float>, grid, blk, 0, h->stream, tl.pblk, h->nb, h->val32, tl.val32_p);
+        } else {
+            if (!tl.val_p) tl.val_p = tl_alloc<double>(9 * (size_t)h->nb);
+            hipLaunchKernelGGL(k_tp_permute<double>, grid, blk, 0, h->stream, tl.pblk, h->nb, h->val, tl.val_p);
+        }
+        TF_HIP(hipGetLastError());
+    }
     return !tl.failed;
 }
 
@@ -2843,11 +3401,14 @@ double krylov_probe(tf_handle* h, int n_launches)
     h->krylov_call = call;
     hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     TF_HIP(hipGetLastError());
-    const int wg = sk == KIND_TL ? h->wg : WG;  // as the solver launches it
     auto launches = [&]() {
+        const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
         for (int k = 0; k < n_launches; ++k) {
-            if (h->opts.matrix_precision == TF_PREC_FP32) launch_cg_q<float, 0>(h, h->val32, wg, 1 << 30);
-            else launch_cg_q<double, 0>(h, h->val, wg, 1 << 30);
+            if (sk == KIND_TL) {  // k_tp_q, as the two-level solve launches it
+                if (fp32) launch_tp_q<float, 0>(h, tp_tile_args(h), tp_args(h), 1 << 30);
+                else launch_tp_q<double, 0>(h, tp_tile_args(h), tp_args(h), 1 << 30);
+            } else if (fp32) launch_cg_q<float, 0>(h, h->val32, WG, 1 << 30);
+            else launch_cg_q<double, 0>(h, h->val, WG, 1 << 30);
         }
     };
     // one graph replay (use_graph), else the launches enqueued one by one
@@ -2906,6 +3467,7 @@ int krylov_solve(tf_handle* h)
         h->tl_used = true;
     }
     int st = run(kind);
+    bool kx_in_positions = kind == KIND_TL;
     if (kind == KIND_TL && st == TF_OK) twolevel_observe(h, h->last_krylov_iters);
     if (kind == KIND_TL && st != TF_OK) h->tl.fresh = false;  // the next solve rebuilds the coarse operator
     bool unusable = false;  // a stalled tissue whose residual is above 1e-2 of its right-hand side
@@ -2921,6 +3483,7 @@ int krylov_solve(tf_handle* h)
         h->krylov_call = -1;  // the prediction belongs to the first method
         try {
             st = run(TF_KRYLOV_BICGSTAB);
+            kx_in_positions = false;
         } catch (...) {
             h->krylov_call = call;
             throw;
@@ -2928,7 +3491,12 @@ int krylov_solve(tf_handle* h)
         h->krylov_call = call;
         h->last_krylov_iters += cg_iters;
     }
-    launch_copy(h, h->du, h->kx, 3 * (int64_t)h->N);
+    if (kx_in_positions)  // the two-level solve ran in aggregate order
+        hipLaunchKernelGGL(k_tp_scatter, dim3((3 * h->N + 255) / 256), dim3(256), 0, h->stream, h->tl.agg_nodes, h->N,
+                           h->kx, h->du);
+    else
+        launch_copy(h, h->du, h->kx, 3 * (int64_t)h->N);
+    TF_HIP(hipGetLastError());
     return st;
 }
 
