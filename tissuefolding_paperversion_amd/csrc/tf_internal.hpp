@@ -272,7 +272,6 @@ struct tf_handle {
         double* pcoef = nullptr;                            // [N][3][6] rows of P
         int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank
         int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
-        double *agg_part = nullptr, *rc = nullptr;          // [A][2], [nc]
         double* Ac = nullptr;                               // [ldc][ldc] P^T M P, then -Ac^-1 (lower tiles)
         int32_t ldc = 0;                                    // nc rounded up to the sweep's tile (64)
         double *swP = nullptr, *swO = nullptr, *swD = nullptr;  // the sweep's panels [ldc][64] and pivot inverses [2][64][64]

@@ -527,29 +527,16 @@ __global__ __launch_bounds__(WG) void k_cg_x(TileArgs ta, const double* __restri
 // Two-level preconditioner (one tissue): z = Dc^-1 r + P Ac^-1 P^T r, with Dc the column blocks of
 // M (the 3L x 3L block of each column of L nodes joined by prisms' vertical edges; single nodes
 // when the columns are not uniform), P the rigid-body modes of aggregates of whole columns
-// (orthonormalised per aggregate) and Ac = P^T M P.  A PCG iteration is three launches:
-//   k_cg_q   (tiles)      p = z + beta p, q = M p, (p, q)
-//   k_tl_xr  (aggregates) x += alpha p, r -= alpha q on the aggregate's nodes, their
-//                         (r, Dc^-1 r) and (r, r), and r_c = P_a^T r
-//   k_tl_cz  (aggregates) z_c = Ac^-1 r_c on the aggregate's coarse rows, z = Dc^-1 r + P z_c on
-//                         its nodes; the tissue's (r, z) and (r, r) as tile partials
-// (r, z) = (r, Dc^-1 r) + (r_c, z_c) since P^T r = r_c.  Every sum runs in a fixed order.
+// (orthonormalised per aggregate) and Ac = P^T M P.  The PCG that applies it runs in aggregate
+// order, two launches per iteration (k_tp_q, k_tp_z below).
 // An aggregate lists its nodes column by column, so global column g holds agg_nodes[gL .. gL+L-1].
 constexpr int TL_MAX_LEVELS = 4;  // nodes per column at most (12 x 12 column blocks)
 
+// the coarse space as the Galerkin product k_tl_ac reads it
 struct TLArgs {
-    const int32_t* agg_ptr;    // [A+1] nodes of every aggregate in agg_nodes
-    const int32_t* agg_nodes;  // [N] nodes of every aggregate, column by column, aggregate after aggregate
-    const double* pcoef;       // [N][3][6] rows of P (columns past the aggregate's rank are zero)
+    const double* pcoef;       // [N][3][6] rows of P by node (columns past the aggregate's rank are zero)
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
-    double* agg_part;          // [A][2] (r, Dc^-1 r) and (r, r) over the aggregate's nodes
-    double* rc;                // [nc] restricted residual P^T r
-    const __half2* Ainv;       // [nc][ldA / 2] Ac^-1 / scale in FP16 pairs (a preconditioner; FP64 sums)
-    const double* ascale;      // [2] scale (the largest diagonal entry of Ac^-1) and its reciprocal
-    int ldA;                   // row stride of Ainv in halves (nc rounded up to 8; zero padding)
-    const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
-    int nc, n_agg, G;          // G: workgroups of k_tl_cz
 };
 
 // Inverse of every column block of M, as a symmetric FP32 copy: 64 / 3L columns per wave, one lane
@@ -615,251 +602,6 @@ __global__ __launch_bounds__(WG) void k_tl_dcol(const int32_t* __restrict__ col_
 #pragma unroll
         for (int j = 0; j < D; ++j) out[li * D + j] = j / 3 == i ? (float)dinv[9 * (size_t)node + 3 * ci + j % 3] : 0.0f;
     }
-}
-
-// One workgroup per aggregate, one lane per DoF of its columns (WG / 3L columns per pass).  INIT:
-// r as given (no update); otherwise alpha = (r, z) / (p, q) from the tile partials.  Every lane
-// loads its first DoF's data before alpha is reduced; the column's r is exchanged in LDS.
-template <int P, bool INIT, int L>
-__global__ __launch_bounds__(WG) void k_tl_xr(TileArgs ta, TLArgs tl, const double* __restrict__ p,
-                                              const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r)
-{
-    constexpr int D = 3 * L, CPP = WG / D;
-    __shared__ double rl[WG];
-    const int a = blockIdx.x;
-    const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
-    const int kr = tl.krank[a], co = tl.coff[a];  // used at the end, loaded with the rest
-    SegScalars& S = ta.seg[0];
-    // n0 is never negative: testing it makes the aggregate's offsets load together with the
-    // tissue's state instead of after the branch
-    if (seg_done(S) | (n0 < 0)) return;
-    const int slot = threadIdx.x / D, li = threadIdx.x % D;
-    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // (r, Dc^-1 r), (r, r), P^T r
-    double alpha = 0.0;
-    for (int c0 = 0; c0 < ncols; c0 += CPP) {
-        const bool own = slot < CPP && c0 + slot < ncols;
-        const int g = n0 / L + (own ? c0 + slot : 0);
-        const int node = tl.agg_nodes[(size_t)g * L + li / 3];
-        const int d = 3 * node + li % 3;
-        double rv = r[d], xv = 0.0, pv = 0.0, qv = 0.0;
-        if (!INIT) { xv = x[d]; pv = p[d]; qv = q[d]; }
-        float Dr[D];
-#pragma unroll
-        for (int j = 0; j < D; ++j) Dr[j] = tl.dcinv[((size_t)g * D + li) * D + j];
-        double Pr[6];
-#pragma unroll
-        for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
-        if (!INIT && c0 == 0) {
-            const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
-            double sc[3];
-            seg_sums<3>(ta, 0, sl, sc);
-            // (p, q) > 0 needs M positive definite, (r, z) > 0 the preconditioner (its FP16 coarse
-            // inverse is rounded): either failing stops the solve as stalled, x keeping residual sc[2]
-            if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
-                if (a == 0 && threadIdx.x == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
-                return;
-            }
-            alpha = sc[0] / sc[1];
-        }
-        if (!INIT) {
-            xv = xv + alpha * pv;
-            rv = rv - alpha * qv;
-            if (own) { x[d] = xv; r[d] = rv; }
-        }
-        rl[threadIdx.x] = rv;
-        __syncthreads();
-        if (own) {
-            double zi = 0.0;
-#pragma unroll
-            for (int j = 0; j < D; ++j) zi += (double)Dr[j] * rl[slot * D + j];
-            v[0] += rv * zi;
-            v[1] += rv * rv;
-#pragma unroll
-            for (int j = 0; j < 6; ++j) v[2 + j] += Pr[j] * rv;
-        }
-        __syncthreads();  // rl is refilled by the next pass
-    }
-    constexpr int NW = WG / 64;
-    __shared__ double red[8][NW];
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const double t = wave_sum(v[j]);
-        if (l == 0) red[j][w] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) {
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
-        if (threadIdx.x < 2) tl.agg_part[2 * a + threadIdx.x] = t;
-        else if ((int)threadIdx.x - 2 < kr) tl.rc[co + threadIdx.x - 2] = t;
-    }
-}
-
-// z_c = Ac^-1 r_c on the coarse rows of the workgroup's aggregates (g, g + G, ...), then
-// z = Dc^-1 r + P z_c on their nodes (one lane per DoF of the aggregate's columns, WG / 3L columns
-// per pass, the column's r exchanged in LDS).  Each lane loads its column pairs (t, t + WG, ...) of
-// the first aggregate's rows of Ac^-1 and its first DoF's r, Dc^-1 row and P row before r_c is staged
-// in LDS.  Workgroup g writes the tile partials g of slots rz_slot ((r, Dc^-1 r) + (r_c, z_c)
-// over its aggregates) and rr_slot ((r, r)); the partials G .. T-1 are zero.
-constexpr int TL_CZ_PAIRS = 8;  // column pairs per lane of every coarse row held in registers (nc <= 4096)
-
-template <int L>
-__global__ __launch_bounds__(WG) void k_tl_cz(TileArgs ta, TLArgs tl, const double* __restrict__ r,
-                                              double* __restrict__ z, int rz_slot, int rr_slot, int count_iter)
-{
-    constexpr int D = 3 * L, CPP = WG / D;
-    SegScalars& S = ta.seg[0];
-    // the first aggregate's node offset is never negative: testing it makes the aggregate's
-    // offsets load together with the tissue's state instead of after the branch
-    if (seg_done(S) | (tl.agg_ptr[blockIdx.x] < 0) | (tl.coff[blockIdx.x] < 0)) return;
-    extern __shared__ double rcs[];  // r_c, zero-padded to max(ldA, 2 WG TL_CZ_PAIRS)
-    constexpr int NW = WG / 64;
-    __shared__ double red[6][NW];
-    __shared__ double zcs[6];
-    __shared__ double rl[WG];
-    const int tid = threadIdx.x, w = tid / 64, l = tid % 64;
-    const int slot = tid / D, li = tid % D;
-    const int nc = tl.nc, ld2 = tl.ldA / 2, G = gridDim.x;
-    double acc_rz = 0.0, acc_rr = 0.0;  // lane 0
-    bool first = true;
-    for (int a = blockIdx.x; a < tl.n_agg; a += G) {
-        const int c0 = tl.coff[a], kr = tl.krank[a];
-        const int n0 = tl.agg_ptr[a], ncols = (tl.agg_ptr[a + 1] - n0) / L;
-        const double part0 = tl.agg_part[2 * a], part1 = tl.agg_part[2 * a + 1];  // used at the end
-        // Unconditional loads at clamped addresses (a load under a condition is issued in a branch
-        // of its own and waited for before the next): rows past the rank are summed and dropped,
-        // columns past nc meet the zeros that pad r_c in LDS.
-        __half2 pre[6][TL_CZ_PAIRS];
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-#pragma unroll
-            for (int m = 0; m < TL_CZ_PAIRS; ++m)
-                pre[k][m] = tl.Ainv[(size_t)min(c0 + k, nc - 1) * ld2 + min(tid + WG * m, ld2 - 1)];
-        // the lane's DoF of the first pass
-        bool own = slot < CPP && slot < ncols;
-        int g = n0 / L + (own ? slot : 0);
-        int node = tl.agg_nodes[(size_t)g * L + li / 3];
-        int d = 3 * node + li % 3;
-        double rv = r[d];
-        float Dr[D];
-#pragma unroll
-        for (int j = 0; j < D; ++j) Dr[j] = tl.dcinv[((size_t)g * D + li) * D + j];
-        double Pr[6];
-#pragma unroll
-        for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
-        if (first) {
-            for (int i = tid; i < max(2 * ld2, 2 * WG * TL_CZ_PAIRS); i += WG) rcs[i] = i < nc ? tl.rc[i] : 0.0;
-            __syncthreads();
-            first = false;
-        }
-        double sv[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            sv[k] = 0.0;
-#pragma unroll
-            for (int m = 0; m < TL_CZ_PAIRS; ++m) {
-                const float2 f = __half22float2(pre[k][m]);
-                const int j = 2 * (tid + WG * m);
-                sv[k] += (double)f.x * rcs[j] + (double)f.y * rcs[j + 1];
-            }
-        }
-        for (int jp = tid + WG * TL_CZ_PAIRS; jp < ld2; jp += WG)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const float2 f = __half22float2(tl.Ainv[(size_t)min(c0 + k, nc - 1) * ld2 + jp]);
-                sv[k] += (double)f.x * rcs[2 * jp] + (double)f.y * rcs[2 * jp + 1];
-            }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const double t = wave_sum(sv[k]);
-            if (l == 0) red[k][w] = t;
-        }
-        __syncthreads();
-        if (tid < 6) {
-            double t = 0.0;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) t += red[tid][k];
-            zcs[tid] = tid < kr ? t * tl.ascale[0] : 0.0;  // P's columns past the rank are zero
-        }
-        __syncthreads();
-        for (int cb = 0; cb < ncols; cb += CPP) {
-            if (cb > 0) {
-                own = slot < CPP && cb + slot < ncols;
-                g = n0 / L + (own ? cb + slot : 0);
-                node = tl.agg_nodes[(size_t)g * L + li / 3];
-                d = 3 * node + li % 3;
-                rv = r[d];
-#pragma unroll
-                for (int j = 0; j < D; ++j) Dr[j] = tl.dcinv[((size_t)g * D + li) * D + j];
-#pragma unroll
-                for (int j = 0; j < 6; ++j) Pr[j] = tl.pcoef[18 * (size_t)node + 6 * (li % 3) + j];
-            }
-            rl[tid] = rv;
-            __syncthreads();
-            double v = 0.0;
-            const int sb = min(slot, CPP - 1) * D;  // lanes past the last whole column read a valid one
-#pragma unroll
-            for (int j = 0; j < D; ++j) v += (double)Dr[j] * rl[sb + j];
-#pragma unroll
-            for (int j = 0; j < 6; ++j) v += Pr[j] * zcs[j];
-            if (own) z[d] = v;
-            __syncthreads();  // rl is refilled by the next pass
-        }
-        if (tid == 0) {
-            double cz = 0.0;
-            for (int k = 0; k < kr; ++k) cz += rcs[c0 + k] * zcs[k];
-            acc_rz += part0 + cz;
-            acc_rr += part1;
-        }
-        __syncthreads();  // red and zcs are reused by the next aggregate
-    }
-    const int g0 = blockIdx.x;
-    if (tid == 0) {
-        ta.partial[(size_t)rz_slot * ta.T + g0] = acc_rz;
-        ta.partial[(size_t)rr_slot * ta.T + g0] = acc_rr;
-        if (g0 == 0 && count_iter) S.iters += 1;
-    }
-    for (int t = g0 + G + tid * G; t < ta.T; t += WG * G) {
-        ta.partial[(size_t)rz_slot * ta.T + t] = 0.0;
-        ta.partial[(size_t)rr_slot * ta.T + t] = 0.0;
-    }
-}
-
-static size_t tl_cz_lds(const TLArgs& tl) { return sizeof(double) * (size_t)std::max(tl.ldA, 2 * WG * TL_CZ_PAIRS); }
-
-// r = b - M x0 (x0 given) or b, x = x0 or 0, p_old = 0, and (b, b)
-template <typename VT>
-__global__ __launch_bounds__(WG) void k_tl_init(TileArgs ta, const int32_t* __restrict__ row_ptr,
-                                                const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
-                                                const double* __restrict__ b, const double* __restrict__ x0,
-                                                double* __restrict__ x, double* __restrict__ r, double* __restrict__ p_old)
-{
-    extern __shared__ double prod[];
-    const int t = blockIdx.x;
-    const bool active = ta.seg[ta.tile_seg[t]].active;
-    const int r0 = ta.tile_r0[t], r1 = ta.tile_r1[t];
-    const int b0 = row_ptr[r0], b1 = row_ptr[r1];
-    if (x0) tile_products<VT>(col, val, nb, b0, b1, XLOAD(x0), prod);
-    double bb = 0.0;
-    for (int task = threadIdx.x; task < 3 * (r1 - r0); task += WG) {
-        const int rr = r0 + task / 3, comp = task % 3;
-        const int d = 3 * rr + comp;
-        const double bv = active ? b[d] : 0.0;
-        double rv = bv, xv = 0.0;
-        if (x0 && active) {
-            xv = x0[d];
-            rv = bv - row_sum(row_ptr, rr, comp, b0, prod);
-        }
-        x[d] = xv;
-        r[d] = rv;
-        p_old[d] = 0.0;
-        bb += bv * bv;
-    }
-    const int sl[1] = {SL_BB};
-    double vv[1] = {bb};
-    tile_write<1>(ta, t, sl, vv);
 }
 
 // ---- the two-level PCG in aggregate order: two launches per iteration --------------------------
@@ -2435,7 +2177,7 @@ void tl_release(tf_handle* h)
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk, &tl.prow_ptr, &tl.pcol,
                         &tl.pblk, &tl.pass_ptr, &tl.tp_seg_ptr, &tl.agg_order})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&tl.pcoef, &tl.agg_part, &tl.rc, &tl.Ac, &tl.swP, &tl.swO, &tl.swD, &tl.pcoef_p, &tl.val_p,
+    for (double** p : {&tl.pcoef, &tl.Ac, &tl.swP, &tl.swO, &tl.swD, &tl.pcoef_p, &tl.val_p,
                        &tl.tpart, &tl.ptq, &tl.rcx})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.pass_rows) { (void)hipFree(tl.pass_rows); tl.pass_rows = nullptr; }
@@ -2754,8 +2496,6 @@ static void twolevel_build(tf_handle* h)
     tl.dcinv = tl_alloc<float>((size_t)ncol * 9 * L * L);
     tl.L = L;
     tl.ncol = ncol;
-    tl.agg_part = tl_alloc<double>(2 * (size_t)A);
-    tl.rc = tl_alloc<double>(nc);
     tl.ldc = (nc + SW_B - 1) / SW_B * SW_B;  // the sweep's padded size
     tl.Ac = tl_alloc<double>((size_t)tl.ldc * tl.ldc);
     tl.swP = tl_alloc<double>((size_t)tl.ldc * SW_B);
@@ -2776,11 +2516,7 @@ static void twolevel_build(tf_handle* h)
 static TLArgs tl_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    // k_tl_cz: aggregates g, g + G, ... in workgroup g, at most one workgroup per tile (the
-    // partials it writes)
-    return TLArgs{tl.agg_ptr, tl.agg_nodes, tl.pcoef, tl.coff, tl.krank, tl.agg_part, tl.rc,
-                  reinterpret_cast<const __half2*>(tl.Ainv), tl.ascale, tl.ldA, tl.dcinv, tl.nc, tl.n_agg,
-                  std::min(h->T, tl.n_agg)};
+    return TLArgs{tl.pcoef, tl.coff, tl.krank};
 }
 
 // the aggregates as the tissue's partial-sum units (one tissue per two-level handle)
@@ -2815,32 +2551,6 @@ static void launch_tp_z(tf_handle* h, const TileArgs& ta, const TPArgs& tp, cons
     case 2: hipLaunchKernelGGL((k_tp_z<P, INIT, 2>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
     case 3: hipLaunchKernelGGL((k_tp_z<P, INIT, 3>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
     default: hipLaunchKernelGGL((k_tp_z<P, INIT, 4>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
-    }
-}
-
-// the two-level kernels at the handle's column length L (1..TL_MAX_LEVELS)
-template <int P, bool INIT>
-static void launch_tl_xr(tf_handle* h, const TileArgs& ta, const TLArgs& tla, const double* p, const double* q,
-                         double* x, double* r)
-{
-    const dim3 grid(tla.n_agg), blk(WG);
-    switch (h->tl.L) {
-    case 1: hipLaunchKernelGGL((k_tl_xr<P, INIT, 1>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
-    case 2: hipLaunchKernelGGL((k_tl_xr<P, INIT, 2>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
-    case 3: hipLaunchKernelGGL((k_tl_xr<P, INIT, 3>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
-    default: hipLaunchKernelGGL((k_tl_xr<P, INIT, 4>), grid, blk, 0, h->stream, ta, tla, p, q, x, r); break;
-    }
-}
-
-static void launch_tl_cz(tf_handle* h, const TileArgs& ta, const TLArgs& tla, int rz_slot, int rr_slot, int count)
-{
-    const dim3 grid(tla.G), blk(WG);
-    const size_t lds = tl_cz_lds(tla);
-    switch (h->tl.L) {
-    case 1: hipLaunchKernelGGL(k_tl_cz<1>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
-    case 2: hipLaunchKernelGGL(k_tl_cz<2>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
-    case 3: hipLaunchKernelGGL(k_tl_cz<3>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
-    default: hipLaunchKernelGGL(k_tl_cz<4>, grid, blk, lds, h->stream, ta, tla, h->kr, h->ks, rz_slot, rr_slot, count); break;
     }
 }
 
