@@ -2,22 +2,33 @@
 """Benchmark: implicit NR timesteps/s of the wing-disc ellipse (~40k prisms) on one MI355X.
 
 One step = one implicit timestep: growth update (with the growth-rotation rule), nodal viscosity
-surfaces, and the full Newton-Raphson loop until ||dU||_2 <= 1e-8 (assembly of forces and the
-Jacobian, fixed DoFs, block-Jacobi preconditioned Krylov solve, update).  Inputs are resident in
-HBM before the timed region.  Prints one JSON line.
+surfaces, packing detection, and the full Newton-Raphson loop until ||dU||_2 <= 1e-8 (assembly
+of forces and the Jacobian, fixed DoFs, the preconditioned Krylov solve, update).  Inputs are
+resident in HBM before the timed region.  Prints one JSON line.
 
-The dominant kernel (k_cg1: one iteration of the single-kernel CG, i.e. the BSR SpMV with the
-vector updates and dot products fused in) is timed with HIP events on the stream it runs on,
-around each replay of a captured chunk of iterations; its algorithmic bytes per launch are
-nb*(72+4) + (N+1)*4 (matrix) + 16*3N*8 (x, p, r, w, s, u, m, q read and written once) + N*72
-(block inverses).
+Roofline: the two-level PCG iteration is two launches, k_tp_q (p = z + beta p, the BSR3 SpMV
+q = M p, (p, q), P^T q) and k_tp_z (alpha, the coarse correction through the FP16 inverse, x, r, z
+updates, (r, z), (r, r), P^T r).  Each is timed by HIP events around 64 back-to-back launches in
+one graph replay on the final Jacobian (tf_kernel_probe_named); the slower is the dominant kernel
+of the roofline line.  Inside the timed region, HIP events around every replay of the captured
+Krylov chunks give the device time per PCG iteration (both launches) as the solves ran it.
+Algorithmic bytes per launch (tf_kernel_bytes):
+  k_tp_q: nb (72 + 4) + (N + 1) 4 (matrix) + 4 x 3N x 8 (z, p_old read; p, q written) + N 18 x 8 (P)
+  k_tp_z: nc ldA 2 (FP16 Ac^-1) + ncol (3L)^2 4 (column-block inverses) + N 18 x 8 (P)
+          + 7 x 3N x 8 (x, r read and written; p, q read; z written) + 3 nc 8 (P^T r, P^T q)
+The achievable HBM rate is measured on the box by a 1 GiB device-to-device copy
+(tf_copy_bandwidth); whole-step algorithmic bytes (assembly kernels per NR iteration, both PCG
+launches per Krylov iteration) over ms_per_step give the step's own fraction of peak.
 A second leg runs the batched 256-disc sweep (config 4: one tissue segment per disc in one
 handle, autotuned tiling) and, for comparison, the same discs one handle at a time.
 A third leg times the discrete adjoint on the inverse-growth workload (config 5: 30 steps of
 the same ellipse, gradient of an apical-position loss with respect to 38,826 per-element rate
 scales and 3 layer moduli) against its forward run, with a central-difference check.
-The CPU baseline is the FP64 CPU oracle (same element math, OpenMP element loop, CSR assembly
-and an MKL PARDISO direct transposed solve) timed on the host cores over one full timestep.
+The CPU baseline is the FP64 CPU oracle (same element math, OpenMP element loop, CSR assembly,
+MKL PARDISO direct transposed solve) on the host cores, without the O(N^2) packing scan (no
+pairs form on this workload), over a bounded sample of timesteps (--cpu-steps, --cpu-runs;
+median over runs).  Beside it, the reference's literal pipeline (dense 3N x 3N K, O(n^2) CSR
+scan, fresh PARDISO factorisation per NR iteration) on the small 1,040-prism disc.
 """
 from __future__ import annotations
 
@@ -25,6 +36,7 @@ import argparse
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -37,8 +49,9 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {
     -1: "k_cg1: one CG iteration (BSR3 FP64 SpMV, CSR-stream tiles, fused vector updates and dot products)",
     3: "k_cg1: one CG iteration (BSR3 FP64 SpMV, CSR-stream tiles, fused vector updates and dot products)",
-    1: "k_cg_q: BSR3 FP64 SpMV with the CG direction update and one dot product",
     0: "k_bicg_spmv: BSR3 FP64 SpMV with fused dot products",
+    "k_tp_q": "k_tp_q: two-level PCG, p = z + beta p_old, BSR3 FP64 SpMV q = M p, (p, q), P^T q",
+    "k_tp_z": "k_tp_z: two-level PCG, alpha, coarse correction (FP16 Ac^-1), x/r/z updates, (r, z), (r, r), P^T r",
 }
 
 
@@ -49,6 +62,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="config2", choices=["config1", "config2"])
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=2, help="CPU-oracle timesteps per run (bounded sample)")
+    p.add_argument("--cpu-runs", type=int, default=1, help="CPU-oracle runs (the median is reported)")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     p.add_argument("--no-batch", action="store_true", help="skip the batched 256-disc sweep")
     p.add_argument("--batch-steps", type=int, default=3)
@@ -77,22 +92,61 @@ def traffic_from_profiles(kernel, workload):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(tissue, dt):
-    """One full timestep of the CPU oracle (growth, surfaces, node masses and the literal O(N^2)
-    packing detection, NR loop with PARDISO)."""
+def host_cpus():
+    """Host CPU topology from lscpu (sockets, cores per socket, threads per core, model) and the
+    CPUs this process may run on."""
+    info = {"usable_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "Model name", "CPU(s)"):
+                info[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(tissue, dt, steps, runs):
+    """The FP64 CPU oracle on the same workload from its initial state: growth, surfaces, node
+    masses and the NR loop with PARDISO, without the literal O(N^2) packing scan (the unit of work
+    leaves it out; the workload forms no pairs).  steps timesteps per run, median over runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the CPU oracle, timed as the baseline only
 
-    o = O.OracleTissue(tissue, solver="auto")
-    cores = O.lib().tfo_num_threads()
+    secs, its = [], []
+    for _ in range(runs):
+        o = O.OracleTissue(tissue, solver="auto", packing=False)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            its.append(o.timestep(dt)[0])
+        secs.append(time.perf_counter() - t0)
+    sec = float(np.median(secs))
+    threads = int(O.lib().tfo_num_threads())
+    return {"value": steps / sec, "unit": "timesteps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} implicit timesteps from t = 0 (growth, surfaces, node masses, {int(np.mean(its))} NR "
+                      f"iterations each: literal 729-term element tangent, CSR assembly, {o.solver_kind()} "
+                      f"transposed solve) of the same workload, median of {runs} run(s); packing scan left out",
+            "seconds_per_run": secs, "threads": threads, "host": host_cpus()}
+
+
+def cpu_literal(dt=60.0, steps=2):
+    """The reference's literal pipeline on the 1,040-prism disc: dense 3N x 3N K, the O(n^2)
+    scan of K into CSR, a fresh PARDISO factorisation every NR iteration, packing detection."""
+    from tissuefolding_paperversion_amd import configs
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the CPU oracle, timed as the baseline only
+
+    t = configs.config1()
+    o = O.OracleTissue(t, solver="literal", packing=True)
     t0 = time.perf_counter()
-    it, _ = o.timestep(dt)
+    for _ in range(steps):
+        o.timestep(dt, dense=True)
     sec = time.perf_counter() - t0
-    return {"value": 1.0 / sec, "unit": "timesteps/s", "cores": int(cores), "kind": "port",
-            "sample": f"one full implicit timestep (growth + surfaces + packing detection ({len(o.pairs)} pairs) + "
-                      f"{it} NR iterations, literal 729-term element tangent, CSR assembly, {o.solver_kind()} "
-                      f"transposed solve) of the same workload",
-            "seconds": sec, "nr_iterations": it}
+    return {"value": steps / sec, "unit": "timesteps/s", "workload": f"{t.name}: {t.n_elements} prisms, "
+            f"{3 * t.n_nodes} DoF (dense K {9 * t.n_nodes ** 2 * 8 / 1e6:.0f} MB)", "steps": steps,
+            "solver": o.solver_kind(), "threads": int(O.lib().tfo_num_threads())}
 
 
 def elastic_forces(s):
@@ -125,6 +179,7 @@ def run_timed(NewtonRaphsonSolver, tissue, opts, warmup, steps, timing, tune=Tru
         s.set_tiling(wg, cap)
     for k in ("assemble_elements", "assemble_gather"):  # eager launches, events around each
         s.kernel_timing(k, enable=timing)
+    s.krylov_timing(enable=timing)  # events around every replay of a captured Krylov chunk
     s.positions  # synchronises the stream
     t0 = time.perf_counter()
     stats = [s.runOneStep(dt) for _ in range(steps)]
@@ -234,18 +289,34 @@ def main():
     opts["coarse_refresh"] = a.coarse_refresh
     timing = 0 if a.no_kernel_timing else 1
     s, stats, sec, x_plain = run_timed(NewtonRaphsonSolver, tissue, opts, a.warmup, a.steps, timing, tiling=a.tiling)
-    # the dominant Krylov kernel: 64 launches back to back on the final Jacobian, right after the
-    # timed steps (HIP events around one graph replay; the timed steps replay their Krylov
-    # chunks as graphs with nothing between the launches)
-    spmv_n = 64 if timing else 0
-    spmv_ms = s.kernel_probe(spmv_n) if timing else 0.0
+    replay_ms, launched, krylov_its = s.krylov_timing(enable=0)
     el_ms, el_n = s.kernel_timing("assemble_elements", enable=0)
     ga_ms, ga_n = s.kernel_timing("assemble_gather", enable=0)
-    spmv_bytes = s.kernel_bytes("spmv")
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
-    nr = [st["nr_iterations"] for st in stats]
     two_level = bool(stats[-1]["two_level"])
+    # the Krylov launches: 64 back to back on the final Jacobian, right after the timed steps (HIP
+    # events around one graph replay; the timed steps replay their Krylov chunks as graphs with
+    # nothing between the launches)
+    probes = {}
+    if timing:
+        names = (("k_tp_q", "spmv"), ("k_tp_z", "coarse")) if two_level else (("spmv", "spmv"),)
+        for kname, key in names:
+            ms = s.kernel_probe(64, key)
+            nbytes = s.kernel_bytes(key)
+            probes[kname] = {"avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes,
+                             "achieved_GBps": nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0}
+    dom = max(probes, key=lambda k: probes[k]["avg_launch_ms"]) if probes else None
+    spmv_ms = probes[dom]["avg_launch_ms"] if dom else 0.0
+    spmv_bytes = probes[dom]["algorithmic_bytes_per_launch"] if dom else s.kernel_bytes("spmv")
+    achieved = probes[dom]["achieved_GBps"] if dom else 0.0
+    stream = s.copy_bandwidth(1 << 30, 5) if timing else 0.0
+    nr = [st["nr_iterations"] for st in stats]
     kr = [st["krylov_iterations"] for st in stats]
+    # whole-step algorithmic bytes: the assembly kernels every NR iteration, the Krylov launches
+    # every Krylov iteration (the other kernels of the step are left out: a lower bound)
+    it_bytes = sum(p["algorithmic_bytes_per_launch"] for p in probes.values())
+    step_bytes = (np.sum(nr) * (s.kernel_bytes("assemble_elements") + s.kernel_bytes("assemble_gather")) +
+                  np.sum(kr) * it_bytes) / a.steps
+    step_gbps = step_bytes / (sec / a.steps) / 1e9
     out = {
         "metric": "NR timesteps/s at 40k prisms, plain FP64 vs mixed (disp err <=1e-8 rel)",
         "value": a.steps / sec,
@@ -277,15 +348,28 @@ def main():
                                  for k in ("ms_grow", "ms_assemble", "ms_solve", "ms_total")},
         "packing": {"pairs_per_step_max": int(max(st["packing_pairs"] for st in stats)),
                     "ms_growth_surfaces_detection_per_step": float(np.mean([st["ms_grow"] for st in stats]))},
-        "roofline": {"kernel": KERNEL_NAMES[1] if two_level else KERNEL_NAMES.get(a.krylov, KERNEL_NAMES[-1]),
+        "roofline": {"kernel": KERNEL_NAMES.get(dom, KERNEL_NAMES.get(a.krylov, KERNEL_NAMES[-1])),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles("k_cg_q" if two_level else "k_cg1", a.workload),
-                     "avg_launch_ms": spmv_ms, "launches": spmv_n,
+                     "traffic": traffic_from_profiles(dom if dom in ("k_tp_q", "k_tp_z") else "k_cg1", a.workload),
+                     "avg_launch_ms": spmv_ms, "launches": 64 if timing else 0,
                      "measured": "64 launches back to back on the final Jacobian after the timed steps, "
                                  "HIP events around one graph replay",
-                     "algorithmic_bytes_per_launch": spmv_bytes},
-        "kernels": {"assemble_elements_ms": el_ms, "assemble_gather_ms": ga_ms, "spmv_ms": spmv_ms},
+                     "algorithmic_bytes_per_launch": spmv_bytes,
+                     "kernels": probes,
+                     "in_solve": {"pcg_iteration_ms": replay_ms / krylov_its if krylov_its else 0.0,
+                                  "pcg_iteration_ms_per_launched": replay_ms / launched if launched else 0.0,
+                                  "probe_iteration_ms": sum(p["avg_launch_ms"] for p in probes.values()),
+                                  "krylov_iterations": int(krylov_its), "iterations_launched": int(launched),
+                                  "replay_ms": replay_ms,
+                                  "measured": "HIP events around every Krylov chunk replay of the timed steps "
+                                              "(both launches of an iteration, no-op launches after a solve's "
+                                              "end counted in the time, not in the iterations)"},
+                     "stream_copy_GBps": stream,
+                     "frac_of_stream_copy": achieved / stream if stream > 0 else None,
+                     "step": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBps": step_gbps,
+                              "frac": step_gbps / HBM_PEAK_GBS}},
+        "kernels": {"assemble_elements_ms": el_ms, "assemble_gather_ms": ga_ms, "krylov_dominant_ms": spmv_ms},
     }
     if not a.no_mixed:
         # the same workload with MFMA element tangents and FP32 Jacobian values (FP64 residual,
@@ -341,7 +425,9 @@ def main():
     if not a.no_batch:
         out["batched"] = batched_leg(NewtonRaphsonSolver, a.batch_steps, 1, cpu=not a.no_cpu_baseline)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(configs.config2() if a.workload == "config2" else configs.config1(), dt)
+        out["cpu_baseline"] = cpu_baseline(configs.config2() if a.workload == "config2" else configs.config1(), dt,
+                                           a.cpu_steps, a.cpu_runs)
+        out["cpu_baseline"]["literal_pipeline"] = cpu_literal(dt)
     print(json.dumps(out), flush=True)
 
 

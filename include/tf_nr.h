@@ -301,13 +301,26 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
  * recorded with HIP events on the stream the kernel runs on.  name: "assemble_elements",
  * "assemble_gather", "spmv". */
 int tf_kernel_timing(tf_handle* h, const char* name, int32_t enable, double* avg_ms, int64_t* launches);
-/* Algorithmic bytes of one launch of the named kernel (unique minimum traffic). */
+/* Algorithmic bytes of one launch of the named kernel (unique minimum traffic): "spmv",
+ * "coarse", "assemble_elements", "assemble_gather". */
 int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes);
 /* Average device time (ms) of one launch of the kernel priced under "spmv" (the dominant Krylov
  * kernel of the active solver: k_cg_q for the two-level and textbook CG, k_cg1, k_pipe),
  * n_launches replayed back to back in one graph on the current Jacobian and timed with HIP events
  * around the replay.  Leaves the Krylov work vectors undefined.  0 for BiCGSTAB. */
 int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms);
+/* The same probe for a named kernel: "spmv" (as tf_kernel_probe) or "coarse" (k_tp_z, the
+ * two-level PCG's coarse-correction launch: 0 when the two-level preconditioner is not active). */
+int tf_kernel_probe_named(tf_handle* h, const char* name, int32_t n_launches, double* avg_ms);
+/* Krylov chunk replays timed with HIP events on the handle's stream since the last reset:
+ * total device time of the replays (ms), the iterations they launched (no-op launches after a
+ * solve's end included) and the iterations the solves took.  enable >= 0 resets the sums and
+ * switches the timing on (1) or off (0); -1 reads only. */
+int tf_krylov_timing(tf_handle* h, int32_t enable, double* replay_ms, int64_t* launched, int64_t* iterations);
+/* Device-to-device copy of `bytes` (1 MiB .. 16 GiB) into a second buffer, `reps` times on the
+ * handle's stream: the best read + write bandwidth (GB/s), the achievable HBM rate that the
+ * kernels' roofline fractions are read against. */
+int tf_copy_bandwidth(tf_handle* h, int64_t bytes, int32_t reps, double* best_GBps);
 
 #ifdef __cplusplus
 }

@@ -410,6 +410,8 @@ class OracleTissue:
         """Solve K^T dU = g (the reference's transposed solve)."""
         import scipy.sparse as sp
         n = len(g)
+        if isinstance(K, np.ndarray) and self.solver == "literal":
+            return self._solve_literal(K, g)
         if isinstance(K, np.ndarray):
             x = np.zeros(n)
             Kc = K.copy()
@@ -434,7 +436,28 @@ class OracleTissue:
         import scipy.sparse.linalg as sla
         return sla.spsolve(sp.csc_matrix(K.T), g)
 
+    def _solve_literal(self, K, g):
+        """The reference's own pipeline on a dense K (NewtonRaphsonSolver.cpp:33, the dense 3N x 3N
+        matrix): the O(n^2) scan of K into CSR (:686-707, entries with |v| > 1e-13), then PARDISO
+        with a fresh analysis and factorisation on every call (:418-683), transposed solve."""
+        n = len(g)
+        Kc = _c(K)
+        nnz = lib().tfo_dense_to_csr(n, Kc, None, None, None)
+        ia = np.zeros(n + 1, np.int64)
+        ja = np.zeros(nnz, np.int32)
+        a = np.zeros(nnz)
+        lib().tfo_dense_to_csr(n, Kc, ia.ctypes.data, ja.ctypes.data, a.ctypes.data)
+        if not lib().tfo_pardiso_available(None):
+            raise RuntimeError("MKL PARDISO not available for the literal pipeline")
+        x = np.zeros(n)
+        rc = lib().tfo_pardiso_solve_transposed(n, _c(ia, np.int32), ja, a, _c(g), x, lib().tfo_num_threads())
+        if rc:
+            raise RuntimeError(f"PARDISO error {rc}")
+        return x
+
     def solver_kind(self):
+        if self.solver == "literal":
+            return "literal: dense K, O(n^2) CSR scan, mkl-pardiso"
         if 3 * self.N <= 3000:
             return "dense-lu"
         if self.solver == "auto" and not _pardiso_safe():

@@ -30,7 +30,8 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_set_ablated_nodes", "tf_set_growth_state", "tf_get_growth_rotation", "tf_apply_jacobian",
            "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get",
            "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs",
-           "tf_set_growth_increments", "tf_element_dump_range_noalloc", "tf_get_coarse_operator"]
+           "tf_set_growth_increments", "tf_element_dump_range_noalloc", "tf_get_coarse_operator",
+           "tf_kernel_probe_named", "tf_krylov_timing", "tf_copy_bandwidth"]
 
 
 class TfError(RuntimeError):
@@ -116,6 +117,9 @@ def load(path: str | None = None):
         "tf_kernel_timing": ([vp, C.c_char_p, i32, vp, vp], i32),
         "tf_kernel_bytes": ([vp, C.c_char_p, vp], i32),
         "tf_kernel_probe": ([vp, i32, vp], i32),
+        "tf_kernel_probe_named": ([vp, C.c_char_p, i32, vp], i32),
+        "tf_krylov_timing": ([vp, i32, vp, vp, vp], i32),
+        "tf_copy_bandwidth": ([vp, C.c_int64, i32, vp], i32),
         "tf_set_tiling": ([vp, i32, i32], i32),
         "tf_get_tiling": ([vp, vp, vp, vp], i32),
         "tf_autotune": ([vp, dbl, i32, vp, vp, vp, vp], i32),

@@ -171,7 +171,7 @@ void timer_collect(KernelTimer& t)
 // tile's DoFs never outnumber the lanes of a workgroup).  Rebuilt when the tiling is retuned.
 void drop_krylov_graphs(tf_handle* h)
 {
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 4; ++k)
         if (h->krylov_graph[k]) {
             (void)hipGraphExecDestroy(h->krylov_graph[k]);
             h->krylov_graph[k] = nullptr;
@@ -1715,6 +1715,92 @@ int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms)
     TF_GUARD_END(h)
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_bw_fill(uint4* p, size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        p[i] = make_uint4((unsigned)i, 1u, 2u, 3u);
+}
+__global__ __launch_bounds__(256) void k_bw_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+int tf_copy_bandwidth(tf_handle* h, int64_t bytes, int32_t reps, double* best_GBps)
+{
+    if (!h || !best_GBps || bytes < (1 << 20) || bytes > ((int64_t)16 << 30) || reps < 1) return TF_INVALID;
+    TF_GUARD_BEGIN
+    const size_t n16 = (size_t)bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double best = 0.0;
+    try {
+        TF_HIP(hipMalloc(&a, n16 * 16));
+        TF_HIP(hipMalloc(&b, n16 * 16));
+        TF_HIP(hipEventCreate(&ev[0]));
+        TF_HIP(hipEventCreate(&ev[1]));
+        const int grid = 256 * 8 * 4;  // 8 workgroups per CU, grid-stride over the buffers
+        hipLaunchKernelGGL(k_bw_fill, dim3(grid), dim3(256), 0, h->stream, a, n16);
+        hipLaunchKernelGGL(k_bw_fill, dim3(grid), dim3(256), 0, h->stream, b, n16);
+        TF_HIP(hipGetLastError());
+        for (int r = 0; r < reps; ++r) {
+            TF_HIP(hipEventRecord(ev[0], h->stream));
+            hipLaunchKernelGGL(k_bw_copy, dim3(grid), dim3(256), 0, h->stream, a, b, n16);
+            TF_HIP(hipGetLastError());
+            TF_HIP(hipEventRecord(ev[1], h->stream));
+            TF_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            TF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (ms > 0.f) best = std::max(best, 2.0 * (double)(n16 * 16) / (ms * 1e-3) / 1e9);
+        }
+    } catch (...) {
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    *best_GBps = best;
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_kernel_probe_named(tf_handle* h, const char* name, int32_t n_launches, double* avg_ms)
+{
+    if (!h || !name || !avg_ms || n_launches < 2) return TF_INVALID;
+    int which = -1;
+    if (!std::strcmp(name, "spmv")) which = 0;
+    else if (!std::strcmp(name, "coarse")) which = 1;
+    if (which < 0) return fail(h, TF_INVALID, std::string("unknown kernel ") + name);
+    TF_GUARD_BEGIN
+    *avg_ms = krylov_probe(h, n_launches, which);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_krylov_timing(tf_handle* h, int32_t enable, double* replay_ms, int64_t* launched, int64_t* iterations)
+{
+    if (!h) return TF_INVALID;
+    KernelTimer& t = h->t_spmv;
+    if (replay_ms) *replay_ms = t.replay_ms;
+    if (launched) *launched = t.replay_launches;
+    if (iterations) *iterations = t.replay_iters;
+    if (enable >= 0) {
+        t.enabled = enable != 0;
+        t.replay_ms = 0.0;
+        t.replay_launches = 0;
+        t.replay_iters = 0;
+    }
+    return TF_OK;
+}
+
 int tf_get_segment_stats(tf_handle* h, int32_t* nr_iters, double* du_norm, int32_t* converged)
 {
     if (!h) return TF_INVALID;
@@ -1755,7 +1841,16 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
     const double mat = nb * (vb + 4.0) + (N + 1) * 4.0;  // values + column indices + row pointers
     int kind = h->opts.krylov;
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
-    if (!std::strcmp(name, "spmv")) {
+    if (!std::strcmp(name, "coarse")) {
+        // k_tp_z: the FP16 coarse inverse (nc x ldA halves), the FP32 column-block inverses and
+        // P's rows once; x, r read and written, p, q read, z written; P^T r of the last iteration
+        // and P^T q read, P^T r written
+        if (!twolevel_active(h) || !h->tl.built) { *bytes = 0.0; return TF_OK; }
+        const tf_handle::TwoLevel& tl = h->tl;
+        const double D = 3.0 * tl.L;
+        *bytes = (double)tl.nc * tl.ldA * 2.0 + (double)tl.ncol * D * D * 4.0 + N * 18.0 * 8.0 + n * 8.0 * 7.0 +
+                 tl.nc * 8.0 * 3.0;
+    } else if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver
         if (twolevel_active(h))  // k_tp_q: SpMV on p = z + beta p_old: z, p_old read; p_new, q written; P's rows
             *bytes = mat + n * 8.0 * 4.0 + N * 18.0 * 8.0;

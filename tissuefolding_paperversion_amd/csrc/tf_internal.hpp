@@ -69,8 +69,9 @@ struct KernelTimer {
     std::vector<hipEvent_t> pending;  // start/stop pairs not yet read back
     // graph replays: events bracketing a whole replayed chunk
     hipEvent_t sample[2] = {nullptr, nullptr};
-    double replay_ms = 0.0;           // whole replays, per launched iteration (all launches)
-    int64_t replay_launches = 0;
+    double replay_ms = 0.0;           // whole replays
+    int64_t replay_launches = 0;      // iterations launched by those replays (no-op launches included)
+    int64_t replay_iters = 0;         // iterations the solves took
     bool capturing = false;           // inside a stream capture: no per-launch events
 };
 
@@ -83,7 +84,8 @@ enum {
     PK_ST_CELL = 1,   // spatial-hash cell edge (>= every detection threshold; 0: no detection)
     PK_ST_LO = 2,     // [3] minimum x, y, z of the tissue's nodes (hash origin)
     PK_ST_BB = 5,     // [4] reference bounding box: min x, min y, size x, size y
-    PK_ST_GRID = 9    // [50] detection threshold grid [10][5], row-major
+    PK_ST_GRID = 9,   // [50] detection threshold grid [10][5], row-major
+    PK_ST_EXT = 59    // largest extent of the tissue's nodes over x, y, z
 };
 struct PackingState {
     // node and element tables (built at creation, detection tables on first use)
@@ -96,8 +98,14 @@ struct PackingState {
     int32_t *adj_ptr = nullptr, *adj_col = nullptr;  // element neighbours of every node, sorted
     int32_t* node_seg = nullptr;      // [N] tissue of every node
     int32_t* master_end = nullptr;    // [S] first node of the tissue's last detection chunk
-    int32_t* seg_elem_ptr = nullptr;  // [S+1]
-    int32_t* ebin = nullptr;          // [E] grid bin of every element (scratch)
+    int4* chunks = nullptr;           // [n_chunks] runs of <= 256 elements: tissue, first element, end
+    int32_t* seg_chunk_ptr = nullptr; // [S+1] runs of every tissue
+    double* chunk_part = nullptr;     // [n_chunks][102] grid-bin partial sums of every run
+    int32_t n_chunks = 0;
+    int4* runs = nullptr;             // [n_runs] runs of <= 1024 nodes: tissue, first node, end
+    int32_t* seg_run_ptr = nullptr;   // [S+1] runs of every tissue
+    double* run_part = nullptr;       // [n_runs][10] bounding-box partials of every run
+    int32_t n_runs = 0;
     double* ev = nullptr;             // [5][E] side-length values and centre of every element (scratch)
     double* vpn = nullptr;            // [E] det(Fg) V_ref / 6 of every element (scratch)
     int32_t H = 0;                    // hash slots (a power of two)
@@ -251,10 +259,11 @@ struct tf_handle {
 
     // graphs
     // one captured Krylov chunk per Jacobian value type (FP64, FP32)
-    hipGraphExec_t krylov_graph[2] = {nullptr, nullptr};
-    int krylov_graph_iters[2] = {0, 0};
-    int krylov_graph_kind[2] = {-1, -1};
-    int krylov_graph_maxit[2] = {-1, -1};  // the iteration cap the captured launches carry
+    // captured Krylov chunks: [FP64 values, FP32 values] x [chunk, small chunk]
+    hipGraphExec_t krylov_graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    int krylov_graph_iters[4] = {0, 0, 0, 0};
+    int krylov_graph_kind[4] = {-1, -1, -1, -1};
+    int krylov_graph_maxit[4] = {-1, -1, -1, -1};  // the iteration cap the captured launches carry
 
     // implicit packing
     PackingState pk;
@@ -369,7 +378,7 @@ double krylov_time_iteration(tf_handle* h, int n_iters);
 bool twolevel_active(const tf_handle* h);  // the CG solves of this handle use the two-level preconditioner
 // Average device time (ms) of one launch of the dominant Krylov kernel (the one tf_kernel_bytes
 // prices under "spmv"), n_launches replayed back to back on the current Jacobian; 0 for BiCGSTAB
-double krylov_probe(tf_handle* h, int n_launches);
+double krylov_probe(tf_handle* h, int n_launches, int which = 0);  // which: 0 the SpMV kernel, 1 k_tp_z
 #define TF_TWO_LEVEL_MIN_NODES 4096       // TF_PRECOND_AUTO: two-level from this many nodes
 #define TF_GRAPH_NODES 32                 // kernel launches per captured Krylov graph (at most)
 void launch_spmv(tf_handle* h, const double* x, double* y);

@@ -747,10 +747,9 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
 {
     extern __shared__ double prod[];
     const int a = tp.order[blockIdx.x];
-    const int ps0 = tp.pass_ptr[a], ps1 = tp.pass_ptr[a + 1];
     SegScalars& S = ta.seg[0];
-    // ps0 is never negative: testing it makes the aggregate's passes load with the tissue's state
-    if (seg_done(S) | (ps0 < 0)) return;
+    if (seg_done(S)) return;  // after the solve's end: one load before the exit
+    const int ps0 = tp.pass_ptr[a], ps1 = tp.pass_ptr[a + 1];
     const int tid = threadIdx.x;
     const int4 rec = tp.first_pass[a];  // rows and blocks of the first pass, one load
     int2 rr = int2{rec.x, rec.y};
@@ -870,10 +869,10 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 {
     constexpr int D = 3 * L, CPP = WG / D, NV = 2 * TP_PAIRS;
     const int a = tp.order[blockIdx.x];
+    SegScalars& S = ta.seg[0];
+    if (seg_done(S)) return;  // after the solve's end: one load before the exit
     const int n0 = tp.agg_ptr[a], ncols = (tp.agg_ptr[a + 1] - n0) / L;
     const int c0 = tp.coff[a], kr = tp.krank[a];
-    SegScalars& S = ta.seg[0];
-    if (seg_done(S) | (n0 < 0) | (c0 < 0)) return;
     extern __shared__ float tp_rcs[];  // r_c, ldA entries (zero past nc)
     __shared__ double rl[WG];
     __shared__ double zcs[6];
@@ -2881,7 +2880,8 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const int kpi = kind == KIND_TL ? 2 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
     int chunk = std::min(std::max(2, h->opts.iters_per_sync), std::max(2, TF_GRAPH_NODES / kpi));
     chunk -= chunk & 1;  // iterations come in parity pairs
-    const int g_it = chunk, per_chunk = 1;
+    // a small chunk: a quarter of the chunk (an even count), for the end of a predicted solve
+    const int small = std::max(2, (chunk / 4) & ~1);
     const bool timing = h->t_spmv.enabled;
     // Graph replay.  While timing, the kinds with more than one kernel per iteration (other than
     // the two-level PCG) launch eagerly, each launch of the dominant kernel bracketed by events;
@@ -2889,57 +2889,76 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     const bool graph = h->opts.use_graph != 0 &&
                        (!timing || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG || kind == KIND_TL);
     int done_iters = 0;
-    // first burst: about 80% of the iterations this Newton iteration's solve took last timestep,
-    // replayed without a convergence check (a solve that converges inside the burst leaves the
-    // rest of its launches as no-ops, as in any chunk)
-    int reps = 1;
+    // First burst, replayed without a convergence check: the iterations this Newton iteration's
+    // solve took last timestep plus the launch that sees convergence, rounded up to small chunks
+    // (chunks first).  The solve then continues in small chunks, twice, then in chunks; without a
+    // prediction, in chunks from the start.  A solve that converges inside a replay leaves the
+    // rest of its launches as no-ops.
+    int n_big = 1, n_small = 0;
     const int call = h->krylov_call;
-    if (graph && call >= 0 && call < (int)h->krylov_pred.size() && h->krylov_pred[call] > 0)
-        reps = std::max(1, (int)(0.8 * h->krylov_pred[call]) / chunk);
+    const bool predicted = graph && call >= 0 && call < (int)h->krylov_pred.size() && h->krylov_pred[call] >= 0;
+    if (predicted) {
+        const int target = (h->krylov_pred[call] + 1 + small - 1) / small * small;
+        n_big = target / chunk;
+        n_small = (target - n_big * chunk + small - 1) / small;
+    }
+    int round = 0;
     KernelTimer& tm = h->t_spmv;
     if (timing && !tm.sample[0]) {
         TF_HIP(hipEventCreate(&tm.sample[0]));
         TF_HIP(hipEventCreate(&tm.sample[1]));
     }
-    while (true) {
-        if (graph) {
-            const int key = kind * 16 + (int)sizeof(VT) * 2;
-            const int gi = sizeof(VT) == sizeof(double) ? 0 : 1;
-            hipGraphExec_t& graph_exec = h->krylov_graph[gi];
-            // the captured launches carry the iteration cap as an argument: a chunk captured
-            // under another cap (a timing probe's) is not reused
-            if (!graph_exec || h->krylov_graph_iters[gi] != g_it || h->krylov_graph_kind[gi] != key ||
-                h->krylov_graph_maxit[gi] != h->opts.krylov_max_iter) {
-                if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-                hipGraph_t gr;
-                TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                tm.capturing = true;
-                try {
-                    enqueue_chunk<VT>(h, val, kind, g_it);
-                } catch (...) {
-                    tm.capturing = false;
-                    (void)hipStreamEndCapture(h->stream, &gr);
-                    throw;
-                }
+    auto chunk_graph = [&](int sz, int iters) -> hipGraphExec_t {
+        const int key = kind * 16 + (int)sizeof(VT) * 2;
+        const int gi = (sizeof(VT) == sizeof(double) ? 0 : 2) + sz;
+        hipGraphExec_t& graph_exec = h->krylov_graph[gi];
+        // the captured launches carry the iteration cap as an argument: a chunk captured under
+        // another cap (a timing probe's) is not reused
+        if (!graph_exec || h->krylov_graph_iters[gi] != iters || h->krylov_graph_kind[gi] != key ||
+            h->krylov_graph_maxit[gi] != h->opts.krylov_max_iter) {
+            if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+            hipGraph_t gr;
+            TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            tm.capturing = true;
+            try {
+                enqueue_chunk<VT>(h, val, kind, iters);
+            } catch (...) {
                 tm.capturing = false;
-                TF_HIP(hipStreamEndCapture(h->stream, &gr));
-                TF_HIP(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
-                TF_HIP(hipGraphDestroy(gr));
-                h->krylov_graph_iters[gi] = g_it;
-                h->krylov_graph_kind[gi] = key;
-                h->krylov_graph_maxit[gi] = h->opts.krylov_max_iter;
+                (void)hipStreamEndCapture(h->stream, &gr);
+                throw;
             }
+            tm.capturing = false;
+            TF_HIP(hipStreamEndCapture(h->stream, &gr));
+            TF_HIP(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
+            TF_HIP(hipGraphDestroy(gr));
+            h->krylov_graph_iters[gi] = iters;
+            h->krylov_graph_kind[gi] = key;
+            h->krylov_graph_maxit[gi] = h->opts.krylov_max_iter;
+        }
+        return graph_exec;
+    };
+    while (true) {
+        int launched = chunk;
+        if (graph) {
+            if (round > 0) {
+                const bool fine = predicted && round <= 2;
+                n_big = fine ? 0 : 1;
+                n_small = fine ? 1 : 0;
+            }
+            hipGraphExec_t big = n_big ? chunk_graph(0, chunk) : nullptr;
+            hipGraphExec_t sm = n_small ? chunk_graph(1, small) : nullptr;
             if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
-            for (int k = 0; k < reps * per_chunk; ++k) TF_HIP(hipGraphLaunch(graph_exec, h->stream));
+            for (int k = 0; k < n_big; ++k) TF_HIP(hipGraphLaunch(big, h->stream));
+            for (int k = 0; k < n_small; ++k) TF_HIP(hipGraphLaunch(sm, h->stream));
             if (timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
+            launched = n_big * chunk + n_small * small;
         } else {
             if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
             enqueue_chunk<VT>(h, val, kind, chunk);
             TF_HIP(hipGetLastError());
             if (timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
         }
-        const int launched = graph ? chunk * reps : chunk;
-        reps = 1;
+        ++round;
         done_iters += launched;
         const bool fin = all_done(h);
         if (timing) {
@@ -2962,9 +2981,10 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     }
     if (nan) st = TF_NAN;  // a right-hand side that is not finite: no solver reaches a tolerance
     h->last_krylov_iters = maxit;
+    if (timing) tm.replay_iters += maxit;
     if (call >= 0) {
-        if ((int)h->krylov_pred.size() <= call) h->krylov_pred.resize(call + 1, 0);
-        h->krylov_pred[call] = st == TF_OK ? maxit : 0;
+        if ((int)h->krylov_pred.size() <= call) h->krylov_pred.resize(call + 1, -1);
+        h->krylov_pred[call] = st == TF_OK ? maxit : -1;
     }
     return st;
 }
@@ -3062,14 +3082,16 @@ double krylov_time_iteration(tf_handle* h, int n_iters)
 // and partials; the tissues' flags are then reopened.  The single-kernel and pipelined CG advance
 // their iterations (the launches are the solver's own); k_cg_q (two-level and textbook CG) is
 // launched with one parity, so every launch repeats the same direction update and SpMV.
-double krylov_probe(tf_handle* h, int n_launches)
+double krylov_probe(tf_handle* h, int n_launches, int which)
 {
+    if (which == 1 && !twolevel_active(h)) return 0.0;  // k_tp_z: the two-level PCG only
     int kind = h->opts.krylov;
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     const bool tl = twolevel_active(h);
     if (kind == TF_KRYLOV_BICGSTAB && !tl) return 0.0;
     if (kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG) {
         if (!tl) {
+            if (which != 0) return 0.0;
             const tf_solver_opts saved = h->opts;
             h->opts.krylov = kind;
             h->opts.preconditioner = TF_PRECOND_BLOCK_JACOBI;
@@ -3114,7 +3136,9 @@ double krylov_probe(tf_handle* h, int n_launches)
     auto launches = [&]() {
         const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
         for (int k = 0; k < n_launches; ++k) {
-            if (sk == KIND_TL) {  // k_tp_q, as the two-level solve launches it
+            if (sk == KIND_TL && which == 1)  // k_tp_z: alpha from the same partials every launch
+                launch_tp_z<0, false>(h, tp_tile_args(h), tp_args(h), h->kph, h->kv);
+            else if (sk == KIND_TL) {  // k_tp_q, as the two-level solve launches it
                 if (fp32) launch_tp_q<float, 0>(h, tp_tile_args(h), tp_args(h), 1 << 30);
                 else launch_tp_q<double, 0>(h, tp_tile_args(h), tp_args(h), 1 << 30);
             } else if (fp32) launch_cg_q<float, 0>(h, h->val32, WG, 1 << 30);

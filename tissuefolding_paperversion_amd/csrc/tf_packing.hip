@@ -3,9 +3,11 @@
 //
 // Detection (once per timestep, SC/Simulation.cpp:8167-8440) runs in five steps:
 //   k_pk_vpn, k_pk_mass : det(Fg) V_ref / 6 per prism, summed per node (updateNodeMasses).
-//   k_pk_evals, k_pk_stats : side-length averages and centres per prism, then one workgroup per
-//                tissue reduces them with the reference's bounding box (its running-minimum
-//                rule included) into the 10 x 5 threshold grid and the packing threshold T.
+//   k_pk_evals, k_pk_box(_finish), k_pk_bins, k_pk_finish : side-length averages and centres per
+//                prism; the reference's bounding box per tissue from partials over runs of 1024
+//                nodes (its running-minimum rule included);
+//                grid-bin partial sums per run of 256 prisms; per tissue, the runs' partials in
+//                order into the 10 x 5 threshold grid and the packing threshold T.
 //   k_pk_cells : eligible nodes (mass > 0, apical or basal) go to a hashed uniform grid whose
 //                cell is at least the largest detection threshold (Teschner et al., "Optimized
 //                spatial hashing for collision detection of deformable objects", VMV 2003), as
@@ -31,7 +33,9 @@ namespace {
 constexpr double PK_MULT = 5000.0;  // packingMultiplier (SC/Simulation.cpp:8187)
 constexpr double PK_SAT = 5.0;      // sigmoidSaturationForPacking (SC/Simulation.cpp:8188)
 constexpr int PK_MAX_PARTNERS = 64; // slaves one master may collect in one detection
-constexpr int SWG = 1024;           // workgroup of the per-tissue statistics kernel
+constexpr int PK_CHUNK = 256;       // elements per workgroup of the grid-bin kernel
+constexpr int PK_NODE_RUN = 1024;   // nodes per workgroup of the bounding-box kernel
+constexpr int PK_NPART = 102;       // partials per element run: 50 bin sums, 50 bin counts, 2 side-length sums
 constexpr int SCAN_WG = 1024, SCAN_PER = 4, SCAN_TILE = SCAN_WG * SCAN_PER;
 
 template <typename T>
@@ -92,25 +96,6 @@ __device__ double side_average(const double* __restrict__ uk, const int c[6], in
     return dsum;
 }
 
-template <typename T, typename Op>
-__device__ T block_reduce(T v, T* sh, Op op)
-{
-    // a butterfly inside each 64-lane wave, then the SWG / 64 wave results in order: a fixed
-    // combination order (bitwise reproducible) with two barriers instead of one per tree level
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
-    const int t = threadIdx.x;
-    if ((t & 63) == 0) sh[t / 64] = v;
-    __syncthreads();
-    T r = sh[0];
-    for (int k = 1; k < SWG / 64; ++k) r = op(r, sh[k]);
-    __syncthreads();  // sh is reused by the next reduction
-    return r;
-}
-struct OpAdd { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
-struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
-struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
-struct OpMinI { __device__ int operator()(int a, int b) const { return min(a, b); } };
 
 // ---- node masses (SC/Simulation.cpp:7552-7560, SC/ShapeBase.cpp:1243-1245, :3938-3942) -----
 // VolumePerNode = det(Fg) V_ref / 6 per prism, then every node sums its prisms' values in
@@ -181,107 +166,239 @@ __global__ __launch_bounds__(256) void k_pk_evals(const double* __restrict__ uk,
 // max = max_{i >= r} x_i with r the first i >= 1 where x_i >= x_{i-1}.
 // The grid: bins sum the element's bin value; their counters are shared between bins (i, 4)
 // and (i + 1, 0), as the reference's 10 x 4 counter array indexed with j < 5 does.
-__global__ __launch_bounds__(SWG) void k_pk_stats(const int32_t* __restrict__ seg_node_ptr,
-                                                  const int32_t* __restrict__ seg_elem_ptr, const double* __restrict__ uk,
-                                                  const double* __restrict__ ev, int E, int32_t* __restrict__ ebin,
-                                                  double* __restrict__ stats)
+// One workgroup per run of PK_NODE_RUN nodes of one tissue (4 per lane): the run's minima and
+// maxima of x, y, z, its maxima of x and y, and its first node n > n0 with x_n >= x_{n-1} (y
+// likewise; the tissue's end n1 if none).  part[c][k]: 0-2 minima, 3-5 maxima, 6-7 x and y maxima,
+// 8-9 the first nodes.  Minima and maxima are exact in any order.
+__global__ __launch_bounds__(256) void k_pk_box(const int4* __restrict__ runs, const int32_t* __restrict__ seg_node_ptr,
+                                                const double* __restrict__ uk, double* __restrict__ part)
 {
-    __shared__ double sh[SWG];
-    __shared__ int shi[SWG];
-    __shared__ unsigned long long bins_used;
-    const int s = blockIdx.x, t = threadIdx.x;
-    const int n0 = seg_node_ptr[s], n1 = seg_node_ptr[s + 1];
-    const int e0 = seg_elem_ptr[s], e1 = seg_elem_ptr[s + 1];
-    if (t == 0) bins_used = 0ull;
-
-    double lo[3], hi[3];
-    int rx = n1, ry = n1;
+    __shared__ double sh[4][10];
+    const int4 rn = runs[blockIdx.x];  // tissue, first node, end
+    const int n0 = seg_node_ptr[rn.x], n1 = seg_node_ptr[rn.x + 1];
+    const int t = threadIdx.x, lane = t % 64, w = t / 64;
+    double v[8] = {1.0e300, 1.0e300, 1.0e300, -1.0e300, -1.0e300, -1.0e300, 0.0, 0.0};
+    int rx = n1, ry = n1;  // none in this run: the tissue's end
 #pragma unroll
-    for (int d = 0; d < 3; ++d) { lo[d] = 1.0e300; hi[d] = -1.0e300; }
-    for (int n = n0 + t; n < n1; n += SWG) {
+    for (int k = 0; k < PK_NODE_RUN / 256; ++k) {
+        const int n = rn.y + t + 256 * k;
+        if (n < rn.z) {
+            const double x = uk[3 * n], y = uk[3 * n + 1], z = uk[3 * n + 2];
+            v[0] = fmin(v[0], x); v[1] = fmin(v[1], y); v[2] = fmin(v[2], z);
+            v[3] = fmax(v[3], x); v[4] = fmax(v[4], y); v[5] = fmax(v[5], z);
+            if (n > n0) {
+                if (x >= uk[3 * (n - 1)]) rx = min(rx, n);
+                if (y >= uk[3 * (n - 1) + 1]) ry = min(ry, n);
+            }
+        }
+    }
+    double fx = (double)rx, fy = (double)ry;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double p = uk[3 * n + d];
-            lo[d] = fmin(lo[d], p);
-            hi[d] = fmax(hi[d], p);
+            v[d] = fmin(v[d], __shfl_xor(v[d], off, 64));
+            v[3 + d] = fmax(v[3 + d], __shfl_xor(v[3 + d], off, 64));
         }
-        if (n > n0) {
-            if (uk[3 * n] >= uk[3 * (n - 1)]) rx = min(rx, n);
-            if (uk[3 * n + 1] >= uk[3 * (n - 1) + 1]) ry = min(ry, n);
-        }
+        fx = fmin(fx, __shfl_xor(fx, off, 64));
+        fy = fmin(fy, __shfl_xor(fy, off, 64));
     }
-    rx = block_reduce(rx, shi, OpMinI());
-    ry = block_reduce(ry, shi, OpMinI());
-    double qx = -100000.0, qy = -100000.0;
-    for (int n = n0 + t; n < n1; n += SWG) {
-        if (n >= rx) qx = fmax(qx, uk[3 * n]);
-        if (n >= ry) qy = fmax(qy, uk[3 * n + 1]);
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) sh[w][d] = v[d];
+        sh[w][8] = fx;
+        sh[w][9] = fy;
     }
-    double bbmin[2], bbmax[2], tlo[3], thi[3];
-    bbmax[0] = block_reduce(qx, sh, OpMax());
-    bbmax[1] = block_reduce(qy, sh, OpMax());
-    for (int d = 0; d < 3; ++d) {
-        tlo[d] = block_reduce(lo[d], sh, OpMin());
-        thi[d] = block_reduce(hi[d], sh, OpMax());
+    __syncthreads();
+    if (t < 10 && t != 6 && t != 7) {
+        double r = sh[0][t];
+        for (int k = 1; k < 4; ++k) r = (t < 3 || t >= 8) ? fmin(r, sh[k][t]) : fmax(r, sh[k][t]);
+        part[(size_t)blockIdx.x * 10 + t] = r;
+        if (t == 3 || t == 4) part[(size_t)blockIdx.x * 10 + t + 3] = r;  // the x and y maxima
     }
-    bbmin[0] = fmin(100000.0, tlo[0]);
-    bbmin[1] = fmin(100000.0, tlo[1]);
-    const double szx = bbmax[0] - bbmin[0], szy = bbmax[1] - bbmin[1];
+}
 
-    double dsum = 0.0, sb0 = 0.0;  // sb0, cb0: bin (0, 0), where a current bounding box puts every element
-    double dcnt = 0.0;
-    int cb0 = 0;
-    unsigned long long used = 0ull;
-    for (int e = e0 + t; e < e1; e += SWG) {
-        dsum += ev[e];
-        dcnt += ev[(size_t)E + e];
-        const double cur = ev[2 * (size_t)E + e];
-        const int bx = bin_index((ev[3 * (size_t)E + e] - bbmin[0]) / szx, 10);
-        const int by = bin_index(((ev[4 * (size_t)E + e] - bbmin[1]) / szy) / 2.0, 5);
-        const int b = 5 * bx + by;
-        ebin[e] = b;
-        used |= 1ull << b;
-        if (b == 0) { sb0 += cur; cb0++; }
+// One workgroup per tissue: the runs' partials into the reference's bounding box.  The box's
+// maximum x is the largest x over the nodes from the first n > n0 with x_n >= x_{n-1} on (the
+// nodes before it form the strictly decreasing start that only lowers the running minimum):
+// whole runs past that node give their maxima, the run holding it is read again from it on.
+// Minima and maxima are exact in any order, so the lanes reduce them together: a butterfly in
+// every wave, then the waves' results.
+__global__ __launch_bounds__(256) void k_pk_box_finish(const int32_t* __restrict__ seg_run_ptr,
+                                                       const int32_t* __restrict__ seg_node_ptr,
+                                                       const int4* __restrict__ runs, const double* __restrict__ part,
+                                                       const double* __restrict__ uk, double* __restrict__ stats)
+{
+    __shared__ double sh[4][8];
+    const int s = blockIdx.x, t = threadIdx.x, lane = t % 64, w = t / 64;
+    const int c0 = seg_run_ptr[s], c1 = seg_run_ptr[s + 1];
+    const int n0 = seg_node_ptr[s], n1 = seg_node_ptr[s + 1];
+    // v: minima x, y, z, maxima x, y, z, first nodes x, y (exact as doubles)
+    double v[8] = {1.0e300, 1.0e300, 1.0e300, -1.0e300, -1.0e300, -1.0e300, (double)n1, (double)n1};
+    for (int c = c0 + t; c < c1; c += 256) {
+        const double* pc = part + (size_t)c * 10;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { v[d] = fmin(v[d], pc[d]); v[3 + d] = fmax(v[3 + d], pc[3 + d]); }
+        v[6] = fmin(v[6], pc[8]);
+        v[7] = fmin(v[7], pc[9]);
     }
-    atomicOr(&bins_used, used);
-    dsum = block_reduce(dsum, sh, OpAdd());  // also orders the bins_used updates before the read
-    dcnt = block_reduce(dcnt, sh, OpAdd());
-    sb0 = block_reduce(sb0, sh, OpAdd());
-    cb0 = block_reduce(cb0, shi, OpAdd());
-    const unsigned long long ub = bins_used;
-    double* st = stats + (size_t)s * PK_NSTAT;
-    double hc = 0.0;
-    for (int b = 0; b < 50; ++b) {
-        double g = 0.0;
-        if (b == 0) {  // alias index 0 belongs to bin (0, 0) alone
-            g = 0.4 * (cb0 > 0 ? sb0 / cb0 : sb0);
-        } else if ((ub >> b) & 1ull) {  // other bins: only with a box older than the positions
-            const int alias = 4 * (b / 5) + b % 5;
-            double sb = 0.0;
-            int cb = 0;
-            for (int e = e0 + t; e < e1; e += SWG) {
-                const int eb = ebin[e];
-                if (eb == b) sb += ev[2 * (size_t)E + e];
-                if (4 * (eb / 5) + eb % 5 == alias) cb++;
-            }
-            sb = block_reduce(sb, sh, OpAdd());
-            cb = block_reduce(cb, shi, OpAdd());
-            g = 0.4 * (cb > 0 ? sb / cb : sb);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double o = __shfl_xor(v[k], off, 64);
+            v[k] = (k < 3 || k >= 6) ? fmin(v[k], o) : fmax(v[k], o);
         }
-        if (t == 0) st[PK_ST_GRID + b] = g;
-        hc = fmax(hc, g);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sh[w][k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double r = sh[0][k];
+        for (int j = 1; j < 4; ++j) r = (k < 3 || k >= 6) ? fmin(r, sh[j][k]) : fmax(r, sh[j][k]);
+        v[k] = r;
     }
+    const int rx = (int)v[6], ry = (int)v[7];
+    double qx = -100000.0, qy = -100000.0;
+    for (int c = c0 + t; c < c1; c += 256) {
+        const int4 rn = runs[c];
+        if (rn.y >= rx) qx = fmax(qx, part[(size_t)c * 10 + 6]);
+        if (rn.y >= ry) qy = fmax(qy, part[(size_t)c * 10 + 7]);
+    }
+    // the runs holding rx and ry (runs start at n0 + k PK_NODE_RUN): from that node to the run's end
+    if (rx < n1) {
+        const int end = min(n0 + ((rx - n0) / PK_NODE_RUN + 1) * PK_NODE_RUN, n1);
+        for (int n = rx + t; n < end; n += 256) qx = fmax(qx, uk[3 * n]);
+    }
+    if (ry < n1) {
+        const int end = min(n0 + ((ry - n0) / PK_NODE_RUN + 1) * PK_NODE_RUN, n1);
+        for (int n = ry + t; n < end; n += 256) qy = fmax(qy, uk[3 * n + 1]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        qx = fmax(qx, __shfl_xor(qx, off, 64));
+        qy = fmax(qy, __shfl_xor(qy, off, 64));
+    }
+    __syncthreads();  // sh is reused
+    if (lane == 0) { sh[w][0] = qx; sh[w][1] = qy; }
+    __syncthreads();
     if (t == 0) {
-        const double ext = fmax(thi[0] - tlo[0], fmax(thi[1] - tlo[1], thi[2] - tlo[2]));
-        st[PK_ST_T] = dcnt > 0 ? 0.4 * (dsum / dcnt) : 0.0;
-        st[PK_ST_CELL] = hc > 0 ? fmax(hc, ext / 1048576.0) : 0.0;
-        st[PK_ST_LO + 0] = tlo[0];
-        st[PK_ST_LO + 1] = tlo[1];
-        st[PK_ST_LO + 2] = tlo[2];
-        st[PK_ST_BB + 0] = bbmin[0];
-        st[PK_ST_BB + 1] = bbmin[1];
-        st[PK_ST_BB + 2] = szx;
-        st[PK_ST_BB + 3] = szy;
+        for (int j = 1; j < 4; ++j) { qx = fmax(qx, sh[j][0]); qy = fmax(qy, sh[j][1]); }
+        const double bx = fmin(100000.0, v[0]), by = fmin(100000.0, v[1]);
+        double* st = stats + (size_t)s * PK_NSTAT;
+        st[PK_ST_LO + 0] = v[0];
+        st[PK_ST_LO + 1] = v[1];
+        st[PK_ST_LO + 2] = v[2];
+        st[PK_ST_BB + 0] = bx;
+        st[PK_ST_BB + 1] = by;
+        st[PK_ST_BB + 2] = qx - bx;
+        st[PK_ST_BB + 3] = qy - by;
+        st[PK_ST_EXT] = fmax(v[3] - v[0], fmax(v[4] - v[1], v[5] - v[2]));
+    }
+}
+
+// One workgroup per run of PK_CHUNK elements of one tissue: every element's grid bin, and the
+// run's per-bin sums of the bin value and element counts, plus the side-length sum and count.
+// Inside a wave the lanes of one bin are summed by a butterfly, bin after bin (a wave of
+// neighbouring elements holds few bins); the waves' results are added in wave order, so every
+// partial has a fixed order.  part[c][k]: k < 50 bin sums, 50 <= k < 100 bin counts, 100 the
+// side-length sum, 101 its count.
+__global__ __launch_bounds__(PK_CHUNK) void k_pk_bins(const int4* __restrict__ chunks, const double* __restrict__ ev, int E,
+                                                      const double* __restrict__ stats, double* __restrict__ part)
+{
+    constexpr int W = PK_CHUNK / 64;
+    __shared__ double wsum[W][50], wcnt[W][50], wd[W][2];
+    const int4 ck = chunks[blockIdx.x];  // tissue, first element, end
+    const int t = threadIdx.x, lane = t % 64, w = t / 64;
+    for (int k = t; k < W * 50; k += PK_CHUNK) { (&wsum[0][0])[k] = 0.0; (&wcnt[0][0])[k] = 0.0; }
+    const double* st = stats + (size_t)ck.x * PK_NSTAT;
+    const int e = ck.y + t;
+    const bool own = e < ck.z;
+    int b = -1;
+    double v = 0.0, d0 = 0.0, d1 = 0.0;
+    if (own) {
+        d0 = ev[e];
+        d1 = ev[(size_t)E + e];
+        v = ev[2 * (size_t)E + e];
+        const int bx = bin_index((ev[3 * (size_t)E + e] - st[PK_ST_BB]) / st[PK_ST_BB + 2], 10);
+        const int by = bin_index(((ev[4 * (size_t)E + e] - st[PK_ST_BB + 1]) / st[PK_ST_BB + 3]) / 2.0, 5);
+        b = 5 * bx + by;
+    }
+    __syncthreads();
+    unsigned long long pending = __ballot(own);
+    while (pending) {  // wave-uniform
+        const int bl = __shfl(b, __ffsll(pending) - 1, 64);
+        const unsigned long long m = __ballot(b == bl);
+        double x = b == bl ? v : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) { wsum[w][bl] = x; wcnt[w][bl] = (double)__popcll(m); }
+        pending &= ~m;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        d0 += __shfl_xor(d0, off, 64);
+        d1 += __shfl_xor(d1, off, 64);
+    }
+    if (lane == 0) { wd[w][0] = d0; wd[w][1] = d1; }
+    __syncthreads();
+    if (t < PK_NPART) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) acc += t < 50 ? wsum[k][t] : t < 100 ? wcnt[k][t - 50] : wd[k][t - 100];
+        part[(size_t)blockIdx.x * PK_NPART + t] = acc;
+    }
+}
+
+// One workgroup per tissue: the runs' partials (lane group j takes runs j, j + 10, ... in order,
+// then the 10 group sums are added in group order), then the 10 x 5 grid, the threshold T and
+// the hash cell.
+constexpr int PK_FIN_G = 10;  // lane groups of PK_NPART lanes (1020 of 1024 lanes)
+__global__ __launch_bounds__(1024) void k_pk_finish(const int32_t* __restrict__ seg_chunk_ptr,
+                                                    const double* __restrict__ part, double* __restrict__ stats)
+{
+    __shared__ double grp[PK_FIN_G][PK_NPART];
+    __shared__ double tot[PK_NPART];
+    const int s = blockIdx.x, t = threadIdx.x;
+    const int c0 = seg_chunk_ptr[s], c1 = seg_chunk_ptr[s + 1];
+    const int j = t / PK_NPART, k = t % PK_NPART;
+    if (j < PK_FIN_G) {
+        double acc = 0.0;
+#pragma unroll 4
+        for (int c = c0 + j; c < c1; c += PK_FIN_G) acc += part[(size_t)c * PK_NPART + k];
+        grp[j][k] = acc;
+    }
+    __syncthreads();
+    if (t < PK_NPART) {
+        double acc = grp[0][t];
+        for (int g = 1; g < PK_FIN_G; ++g) acc += grp[g][t];
+        tot[t] = acc;
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    double* st = stats + (size_t)s * PK_NSTAT;
+    double g = 0.0;
+    if (t < 50) {
+        const double sb = tot[t], cb = tot[50 + t];
+        if (t == 0) {  // alias index 0 belongs to bin (0, 0) alone
+            g = 0.4 * (cb > 0 ? sb / cb : sb);
+        } else if (cb > 0) {  // other bins: only with a box older than the positions
+            const int alias = 4 * (t / 5) + t % 5;
+            double ca = 0.0;
+            for (int b = 0; b < 50; ++b)
+                if (4 * (b / 5) + b % 5 == alias) ca += tot[50 + b];
+            g = 0.4 * (ca > 0 ? sb / ca : sb);
+        }
+        st[PK_ST_GRID + t] = g;
+    }
+    double hc = g;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) hc = fmax(hc, __shfl_xor(hc, off, 64));
+    if (t == 0) {
+        const double ds = tot[100], dc = tot[101];
+        st[PK_ST_T] = dc > 0 ? 0.4 * (ds / dc) : 0.0;
+        st[PK_ST_CELL] = hc > 0 ? fmax(hc, st[PK_ST_EXT] / 1048576.0) : 0.0;
     }
 }
 
@@ -738,7 +855,8 @@ void packing_free(tf_handle* h)
     PackingState& pk = h->pk;
     pk_free(pk.node_pl); pk_free(pk.elem_basal); pk_free(pk.vref); pk_free(pk.mass); pk_free(pk.seg_stats);
     pk_free(pk.f); pk_free(pk.d); pk_free(pk.adj_ptr); pk_free(pk.adj_col); pk_free(pk.node_seg);
-    pk_free(pk.master_end); pk_free(pk.seg_elem_ptr); pk_free(pk.ebin); pk_free(pk.ev); pk_free(pk.vpn); pk_free(pk.hcnt);
+    pk_free(pk.master_end); pk_free(pk.chunks); pk_free(pk.seg_chunk_ptr); pk_free(pk.chunk_part);
+    pk_free(pk.runs); pk_free(pk.seg_run_ptr); pk_free(pk.run_part); pk_free(pk.ev); pk_free(pk.vpn); pk_free(pk.hcnt);
     pk_free(pk.hstart); pk_free(pk.cell_nodes); pk_free(pk.node_slot); pk_free(pk.pcount); pk_free(pk.pstart);
     pk_free(pk.scan_tmp); pk_free(pk.a); pk_free(pk.b); pk_free(pk.w); pk_free(pk.slot); pk_free(pk.nodes);
     pk_free(pk.nptr); pk_free(pk.nref);
@@ -782,8 +900,32 @@ void packing_prepare(tf_handle* h)
     pk.adj_col = pk_alloc<int32_t>(acol.size());
     pk.node_seg = pk_alloc<int32_t>(N);
     pk.master_end = pk_alloc<int32_t>(S);
-    pk.seg_elem_ptr = pk_alloc<int32_t>(S + 1);
-    pk.ebin = pk_alloc<int32_t>(E);
+    // runs of PK_CHUNK elements inside every tissue (the grid-bin partial sums)
+    std::vector<int4> chunks;
+    std::vector<int32_t> scp(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        for (int e = h->h_elem_seg_ptr[s]; e < h->h_elem_seg_ptr[s + 1]; e += PK_CHUNK)
+            chunks.push_back(int4{s, e, std::min(e + PK_CHUNK, h->h_elem_seg_ptr[s + 1]), 0});
+        scp[s + 1] = (int32_t)chunks.size();
+    }
+    pk.n_chunks = (int32_t)chunks.size();
+    // runs of PK_NODE_RUN nodes inside every tissue (the bounding-box partials); an empty tissue
+    // gets one empty run
+    std::vector<int4> runs;
+    std::vector<int32_t> srp(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        const int n0 = h->h_node_seg_ptr[s], n1 = h->h_node_seg_ptr[s + 1];
+        if (n1 == n0) runs.push_back(int4{s, n0, n0, 0});
+        for (int n = n0; n < n1; n += PK_NODE_RUN) runs.push_back(int4{s, n, std::min(n + PK_NODE_RUN, n1), 0});
+        srp[s + 1] = (int32_t)runs.size();
+    }
+    pk.n_runs = (int32_t)runs.size();
+    pk.runs = pk_alloc<int4>(runs.size());
+    pk.seg_run_ptr = pk_alloc<int32_t>(S + 1);
+    pk.run_part = pk_alloc<double>(runs.size() * 10);
+    pk.chunks = pk_alloc<int4>(chunks.size());
+    pk.seg_chunk_ptr = pk_alloc<int32_t>(S + 1);
+    pk.chunk_part = pk_alloc<double>(chunks.size() * PK_NPART);
     pk.ev = pk_alloc<double>((size_t)5 * E);
     pk.hcnt = pk_alloc<int32_t>(H);
     pk.hstart = pk_alloc<int32_t>(H + 1);
@@ -796,7 +938,10 @@ void packing_prepare(tf_handle* h)
     pk_h2d(h, pk.adj_col, acol.data(), acol.size());
     pk_h2d(h, pk.node_seg, nseg.data(), N);
     pk_h2d(h, pk.master_end, mend.data(), S);
-    pk_h2d(h, pk.seg_elem_ptr, h->h_elem_seg_ptr.data(), S + 1);
+    pk_h2d(h, pk.chunks, chunks.data(), chunks.size());
+    pk_h2d(h, pk.seg_chunk_ptr, scp.data(), S + 1);
+    pk_h2d(h, pk.runs, runs.data(), runs.size());
+    pk_h2d(h, pk.seg_run_ptr, srp.data(), S + 1);
 }
 
 void launch_packing_mass(tf_handle* h)
@@ -899,8 +1044,13 @@ int packing_detect(tf_handle* h)
     launch_packing_mass(h);
     hipLaunchKernelGGL(k_pk_evals, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->uk, h->conn, h->E, pk.elem_basal,
                        pk.ev);
-    hipLaunchKernelGGL(k_pk_stats, dim3(S), dim3(SWG), 0, h->stream, h->seg_node_ptr, pk.seg_elem_ptr, h->uk, pk.ev, h->E,
-                       pk.ebin, pk.seg_stats);
+    hipLaunchKernelGGL(k_pk_box, dim3(pk.n_runs), dim3(256), 0, h->stream, pk.runs, h->seg_node_ptr, h->uk, pk.run_part);
+    hipLaunchKernelGGL(k_pk_box_finish, dim3(S), dim3(256), 0, h->stream, pk.seg_run_ptr, h->seg_node_ptr, pk.runs,
+                       pk.run_part, h->uk, pk.seg_stats);
+    if (pk.n_chunks > 0)
+        hipLaunchKernelGGL(k_pk_bins, dim3(pk.n_chunks), dim3(PK_CHUNK), 0, h->stream, pk.chunks, pk.ev, h->E,
+                           pk.seg_stats, pk.chunk_part);
+    hipLaunchKernelGGL(k_pk_finish, dim3(S), dim3(1024), 0, h->stream, pk.seg_chunk_ptr, pk.chunk_part, pk.seg_stats);
     hipLaunchKernelGGL(k_fill_i32, dim3((pk.H + 255) / 256), dim3(256), 0, h->stream, pk.hcnt, 0, pk.H);
     hipLaunchKernelGGL(k_pk_cells, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, pk.node_seg, pk.seg_stats, h->uk,
                        pk.mass, pk.node_pl, hmask, pk.node_slot, pk.hcnt);

@@ -360,9 +360,23 @@ class NewtonRaphsonSolver:
         self._check(self._lib.tf_kernel_bytes(self.h, name.encode(), C.byref(b)))
         return b.value
 
-    def kernel_probe(self, n_launches=64):
-        """Average device ms of one launch of the dominant Krylov kernel, n_launches back to back
+    def kernel_probe(self, n_launches=64, name="spmv"):
+        """Average device ms of one launch of a Krylov kernel ("spmv": the SpMV launch of the
+        active solver, "coarse": the two-level coarse-correction launch), n_launches back to back
         on the current Jacobian (HIP events around one graph replay)."""
         ms = C.c_double()
-        self._check(self._lib.tf_kernel_probe(self.h, n_launches, C.byref(ms)))
+        self._check(self._lib.tf_kernel_probe_named(self.h, name.encode(), n_launches, C.byref(ms)))
         return ms.value
+
+    def krylov_timing(self, enable=-1):
+        """(replay ms, iterations launched, iterations taken) of the Krylov chunk replays since the
+        last reset (HIP events on the handle's stream); enable 1/0 resets and switches on/off."""
+        ms, launched, its = C.c_double(), C.c_int64(), C.c_int64()
+        self._check(self._lib.tf_krylov_timing(self.h, enable, C.byref(ms), C.byref(launched), C.byref(its)))
+        return ms.value, launched.value, its.value
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=5):
+        """Best read + write GB/s of a device-to-device copy of nbytes (the achievable HBM rate)."""
+        bw = C.c_double()
+        self._check(self._lib.tf_copy_bandwidth(self.h, int(nbytes), reps, C.byref(bw)))
+        return bw.value
