@@ -337,6 +337,7 @@ def main():
                    "matrix_values": "fp32" if a.matrix_fp32 else "fp64",
                    "preconditioner": ("two-level" if stats[-1]["two_level"] else "block-jacobi") +
                                      (" (auto)" if a.preconditioner == "auto" else ""),
+                   "coarse_unknowns": s.coarse_size() if two_level else 0,
                    "tiling": dict(zip(("workgroup", "tile_blocks", "tiles"), s.tiling()))},
         "nr_iterations_per_step": float(np.mean(nr)),
         "krylov_iterations_per_nr_iteration": float(np.sum(kr) / max(1, np.sum(nr))),
@@ -379,13 +380,16 @@ def main():
         til = s.tiling()
         x_ref = trajectory(NewtonRaphsonSolver, tissue, opts, a.steps, til)  # FP64, every step
         mixed = {}
-        for name, tp in (("fp64_tangent_fp32_matrix", 0), ("fp32_mfma", 1), ("bf16_mfma", 2)):
-            mopts = dict(opts, tangent_precision=tp, matrix_precision=1)
+        # Newton iteration 0 in the reduced precision, the rest in FP64 (mixed_fp64_from 1, the
+        # default), the first two (2), and one leg in the reduced precision throughout (-1)
+        for name, tp, late in (("fp64_tangent_fp32_matrix", 0, 1), ("fp32_mfma", 1, 1), ("bf16_mfma", 2, 1),
+                               ("fp64_tangent_fp32_matrix_from2", 0, 2), ("fp32_mfma_throughout", 1, -1)):
+            mopts = dict(opts, tangent_precision=tp, matrix_precision=1, mixed_fp64_from=late)
             sm, mstats, msec, x_mixed = run_timed(NewtonRaphsonSolver, tissue, mopts, a.warmup, a.steps, 0)
             disp_err = float(np.max(np.abs(x_mixed - x_plain)) / np.max(np.abs(x_plain)))
             f_err = float(np.max(np.abs(elastic_forces(sm) - f_plain)) / fscale)
             step_err = trajectory(NewtonRaphsonSolver, tissue, mopts, a.steps, til, ref=x_ref)
-            mixed[name] = {"value": a.steps / msec, "ms_per_step": 1e3 * msec / a.steps,
+            mixed[name] = {"value": a.steps / msec, "ms_per_step": 1e3 * msec / a.steps, "mixed_fp64_from": late,
                            "disp_err_rel_max_over_steps": step_err,
                            "disp_err_rel_after_all_steps": disp_err, "force_err_rel": f_err,
                            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in mstats])),

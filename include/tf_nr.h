@@ -103,13 +103,19 @@ typedef struct tf_solver_opts {
                                   0: tf_timestep clears the pair list                          */
     int32_t preconditioner;    /* TF_PRECOND_*: the preconditioner of the CG solvers             */
     int32_t coarse_aggregates; /* TF_PRECOND_TWO_LEVEL: target number of node aggregates of the
-                                  coarse space (0: one per 53 nodes, at most 600)               */
+                                  coarse space (0: one per 44 nodes, at most 600, and at most two
+                                  aggregates per compute unit)                                  */
     int32_t coarse_refresh;    /* TF_PRECOND_TWO_LEVEL: most timesteps between rebuilds of the
                                   coarse operator (0 or 1: every timestep; default 100)         */
     int32_t column_blocks;     /* TF_PRECOND_TWO_LEVEL smoother: 1 the blocks of the node columns
                                   (nodes joined by prisms' vertical edges) when every column has
                                   the same 1-4 nodes, else 3x3 nodal blocks (default); 0 always
                                   3x3 nodal blocks                                              */
+    int32_t mixed_fp64_from;   /* with tangent_precision or matrix_precision below FP64: Newton
+                                  iterations from this index on assemble their tangents and solve
+                                  with FP64 Jacobian values (the early, large updates run in the
+                                  reduced precision, the last ones converge as in FP64); default 1,
+                                  < 0: the reduced precision in every iteration                  */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

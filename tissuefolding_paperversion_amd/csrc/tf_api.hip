@@ -499,6 +499,7 @@ void tf_default_options(tf_solver_opts* o)
     o->preconditioner = TF_PRECOND_AUTO;
     o->coarse_refresh = 100;
     o->column_blocks = 1;
+    o->mixed_fp64_from = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -1137,8 +1138,13 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         h->tl_pre_valid = false;
     };
     int it = 0;
+    struct NrIterGuard {  // the Newton iteration index the precision switch reads; -1 outside the loop
+        tf_handle* h;
+        ~NrIterGuard() { h->nr_it = -1; }
+    } nr_guard{h};
     while (true) {
         auto ta = std::chrono::steady_clock::now();
+        h->nr_it = it;
         assemble_impl(h, dt);
         // the det Fe status comes back ahead of the warm start's read-back (one host wait)
         TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
@@ -1837,7 +1843,7 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
 {
     if (!h || !name || !bytes) return TF_INVALID;
     const double E = h->E, N = h->N, nb = h->nb, n = 3.0 * h->N;
-    const double vb = h->opts.matrix_precision == TF_PREC_FP32 ? 36.0 : 72.0;
+    const double vb = fp32_values(h) ? 36.0 : 72.0;
     const double mat = nb * (vb + 4.0) + (N + 1) * 4.0;  // values + column indices + row pointers
     int kind = h->opts.krylov;
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;

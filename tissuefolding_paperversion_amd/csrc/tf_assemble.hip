@@ -618,7 +618,7 @@ void launch_assemble_elements(tf_handle* h, double dt)
     const int grid = (h->E + EPB - 1) / EPB;
     hipEvent_t ev[2];
     timer_begin(h, h->t_elem, ev);
-    const int tp = h->opts.tangent_precision;
+    const int tp = tangent_precision_now(h);
     if (h->any_viscous) {
         if (tp == TF_PREC_FP32) launch_elements_t<true, TF_PREC_FP32>(h, dt, grid);
         else if (tp == TF_PREC_BF16) launch_elements_t<true, TF_PREC_BF16>(h, dt, grid);
@@ -639,7 +639,7 @@ void launch_assemble_gather(tf_handle* h, double dt)
     launch_packing_nodes(h);
     hipLaunchKernelGGL(k_gather, dim3((h->nb + 255) / 256), dim3(256), 0, h->stream, h->blk_ptr, h->blk_src, h->blk_row,
                        h->col, h->kelem, h->sa, h->eta_ext, h->fixed, h->ablated, h->pk.n ? h->pk.d : nullptr, 1.0 / dt,
-                       h->nb, h->val, h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr, h->dinv);
+                       h->nb, h->val, fp32_values(h) ? h->val32 : nullptr, h->dinv);
     TF_HIP(hipGetLastError());
     launch_packing_offdiag(h);
     if (h->n_bound) launch_bind_matrix(h);
@@ -659,7 +659,7 @@ void launch_bind_matrix(tf_handle* h)
     launch_copy(h, h->val_base, h->val, 9 * (int64_t)h->nb);
     hipLaunchKernelGGL(k_bind_values, dim3((h->n_fold + 255) / 256), dim3(256), 0, h->stream, h->fold_ptr,
                        h->fold_tgt, h->fold_src, h->fold_const, h->n_fold, h->val_base, h->val,
-                       h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr);
+                       fp32_values(h) ? h->val32 : nullptr);
     hipLaunchKernelGGL(k_bind_dinv, dim3((h->n_bind_nodes + 63) / 64), dim3(64), 0, h->stream, h->bind_nodes,
                        h->n_bind_nodes, h->diag_blk, h->val, h->nb, h->dinv);
     TF_HIP(hipGetLastError());

@@ -137,6 +137,7 @@ struct tf_handle {
     int32_t max_lds = 65536;           // bytes of LDS one workgroup may allocate (device attribute)
     int32_t n_xcd = 1;                 // accelerator complex dies (XCDs) of the device
     int32_t n_cu = 256;                // compute units of the device
+    int32_t nr_it = -1;                // Newton iteration of the running NR step (-1 outside one)
     // options
     tf_solver_opts opts;
     std::string err;
@@ -333,6 +334,15 @@ void launch_update_uk(tf_handle* h);                    // uk += du on active se
 void build_tiles(tf_handle* h, int wg, int tile_cap);  // (re)partitions rows into tiles (tf_api.hip)
 bool tiling_fits(const tf_handle* h, int wg, int tile_cap);  // LDS of the single-kernel CG fits
 void drop_krylov_graphs(tf_handle* h);  // destroys the captured Krylov chunks
+
+// The precision of the current Newton iteration: FP64 tangents and Jacobian values from
+// opts.mixed_fp64_from on, the requested reduced precision before it and outside the NR loop.
+inline bool nr_late(const tf_handle* h)
+{
+    return h->opts.mixed_fp64_from >= 0 && h->nr_it >= h->opts.mixed_fp64_from;
+}
+inline int tangent_precision_now(const tf_handle* h) { return nr_late(h) ? TF_PREC_FP64 : h->opts.tangent_precision; }
+inline bool fp32_values(const tf_handle* h) { return h->opts.matrix_precision == TF_PREC_FP32 && !nr_late(h); }
 void build_pattern(tf_handle* h);  // BSR pattern, scatter lists and bound-DoF fold lists (tf_api.hip)
 void launch_bind_matrix(tf_handle* h);    // M <- N^T M N + I_slaves on the fold targets
 void launch_bind_residual(tf_handle* h);  // g <- N^T g

@@ -2077,7 +2077,7 @@ bool warm_start_begin(tf_handle* h, int k)
     wa.g = h->g;
     wa.m = m;
     const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
-    if (h->opts.matrix_precision == TF_PREC_FP32)
+    if (fp32_values(h))
         hipLaunchKernelGGL(k_warm_gram<float>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val32,
                            h->nb, h->warm_part);
     else
@@ -2310,9 +2310,14 @@ static void twolevel_build(tf_handle* h)
     for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
     // k_tp_z stages two coarse vectors of ldA (nc rounded up to 8) floats in LDS beside ~4 KB of its own
     const int max_nc = std::min(4096, (h->max_lds - 4096) / (2 * (int)sizeof(float)) / 8 * 8);
-    // default: one aggregate per 53 nodes, at most 600 (at most 3,600 coarse unknowns; the dense
-    // FP32 inverse every iteration reads stays under 52 MB, and its rebuild under ~15 ms)
-    int target = h->opts.coarse_aggregates > 0 ? h->opts.coarse_aggregates : std::max(1, std::min(600, N / 53));
+    // default: one aggregate per 44 nodes, at most 600, and no more aggregates than two per
+    // compute unit: k_tp_z (one workgroup per aggregate) holds two workgroups per compute unit
+    // (its registers), so more aggregates would take a second round of workgroups
+    const bool by_default = h->opts.coarse_aggregates <= 0;
+    const int max_agg = by_default ? std::min(max_nc / 6, 2 * h->n_cu) : max_nc / 6;
+    // (the aggregates cover the tissue's part of a grid over its bounding box, so a target above
+    // max_agg can still give at most max_agg aggregates; the loop below shrinks it when not)
+    int target = by_default ? std::max(1, std::min(600, N / 44)) : h->opts.coarse_aggregates;
     target = std::min(target, max_nc / 6);
     int A = 0;
     // Node columns: nodes joined by the vertical edges of prisms (corners k and k + 3), each
@@ -2356,13 +2361,13 @@ static void twolevel_build(tf_handle* h)
     for (int c = 0; c < ncol; ++c)
         for (int k = 0; k < 3; ++k) xc[3 * c + k] = x[3 * (size_t)col_start[c] + k];
     // at most six coarse unknowns per aggregate: an elongated tissue can give more aggregates than
-    // the target (cells along the boundary), so the target shrinks until 6 A fits max_nc (one
+    // the target (cells along the boundary), so the target shrinks until A fits max_agg (one
     // aggregate at target 1, so this ends)
     std::vector<int32_t> aggc;
     while (true) {
         aggc = tl_aggregates(xc, ncol, target, std::max(1, 16 / L), A);
-        if (6 * A <= max_nc || target <= 1) break;
-        target = std::max(1, std::min(target - 1, (int)(0.9 * target * (max_nc / 6) / A)));
+        if (A <= max_agg || target <= 1) break;
+        target = std::max(1, std::min(target - 1, (int)(0.9 * target * max_agg / A)));
     }
     std::vector<int32_t> agg(N);
     for (int n = 0; n < N; ++n) agg[n] = aggc[col_of[n]];
@@ -3018,7 +3023,7 @@ static bool twolevel_prepare(tf_handle* h)
     if (!tl.failed) {
         launch_tl_dcol(h);
         // the values in aggregate order for the solve
-        const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
+        const bool fp32 = fp32_values(h);
         const dim3 grid((h->nb + 255) / 256), blk(256);
         if (fp32) {
             if (!tl.val32_p) tl.val32_p = tl_alloc<float>(9 * (size_t)h->nb);
@@ -3065,7 +3070,7 @@ double krylov_time_iteration(tf_handle* h, int n_iters)
     for (int rep = 0; rep < 2; ++rep) {  // the first run captures the graph
         tm.replay_ms = 0.0;
         tm.replay_launches = 0;
-        if (h->opts.matrix_precision == TF_PREC_FP32) solve_kind<float>(h, h->val32, kind);
+        if (fp32_values(h)) solve_kind<float>(h, h->val32, kind);
         else solve_kind<double>(h, h->val, kind);
         ms = tm.replay_launches ? tm.replay_ms / tm.replay_launches : 0.0;
     }
@@ -3122,7 +3127,7 @@ double krylov_probe(tf_handle* h, int n_launches, int which)
     h->krylov_call = -1;
     double ms = 0.0;
     try {
-        if (h->opts.matrix_precision == TF_PREC_FP32) solve_kind<float>(h, h->val32, sk);
+        if (fp32_values(h)) solve_kind<float>(h, h->val32, sk);
         else solve_kind<double>(h, h->val, sk);
     } catch (...) {
         h->opts = saved; h->t_spmv.enabled = was; h->krylov_call = call;
@@ -3134,7 +3139,7 @@ double krylov_probe(tf_handle* h, int n_launches, int which)
     hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     TF_HIP(hipGetLastError());
     auto launches = [&]() {
-        const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
+        const bool fp32 = fp32_values(h);
         for (int k = 0; k < n_launches; ++k) {
             if (sk == KIND_TL && which == 1)  // k_tp_z: alpha from the same partials every launch
                 launch_tp_z<0, false>(h, tp_tile_args(h), tp_args(h), h->kph, h->kv);
@@ -3191,7 +3196,7 @@ void krylov_pre_solve(tf_handle* h)
 int krylov_solve(tf_handle* h)
 {
     int kind = h->opts.krylov;
-    const bool fp32 = h->opts.matrix_precision == TF_PREC_FP32;
+    const bool fp32 = fp32_values(h);
     auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
     const bool pre = h->tl_pre_valid;
@@ -3236,7 +3241,7 @@ int krylov_solve(tf_handle* h)
 
 void launch_spmv(tf_handle* h, const double* x, double* y)
 {
-    if (h->opts.matrix_precision == TF_PREC_FP32)
+    if (fp32_values(h))
         hipLaunchKernelGGL(k_spmv<float>, dim3(h->T), dim3(WG), 3 * sizeof(double) * (size_t)h->tile_cap, h->stream, h->row_ptr, h->col, h->val32, h->nb,
                            h->tile_r0, h->tile_r1, x, y);
     else

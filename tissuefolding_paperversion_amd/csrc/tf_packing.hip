@@ -1109,6 +1109,6 @@ void launch_packing_offdiag(tf_handle* h)
     if (!pk.n) return;
     hipLaunchKernelGGL(k_pk_offdiag, dim3((pk.n + 255) / 256), dim3(256), 0, h->stream, pk.n, pk.a, pk.b, pk.w, pk.slot,
                        pk.node_seg, pk.seg_stats, pk.mass, h->uk, h->fixed, h->nb, h->val,
-                       h->opts.matrix_precision == TF_PREC_FP32 ? h->val32 : nullptr);
+                       fp32_values(h) ? h->val32 : nullptr);
     TF_HIP(hipGetLastError());
 }

@@ -323,6 +323,12 @@ class NewtonRaphsonSolver:
         self._check(self._lib.tf_adjoint_get(self.h, L.ptr(rb), L.ptr(lb), L.ptr(mb), L.ptr(fb)))
         return rb.reshape(-1, 3), lb, mb, fb.reshape(-1, 3, 3)
 
+    def coarse_size(self):
+        """Coarse unknowns of the two-level preconditioner (0 before its first build)."""
+        n = C.c_int32(0)
+        rc = self._lib.tf_get_coarse_operator(self.h, C.byref(n), None, None)
+        return n.value if rc == L.TF_OK else 0
+
     def coarse_operator(self):
         """(Ac, Ac^-1): the two-level coarse operator of the current Jacobian and the FP64 inverse
         the last coarse rebuild computed (diagnostics)."""
