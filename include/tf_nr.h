@@ -165,6 +165,45 @@ int tf_set_fg(tf_handle* h, const double* Fg);
  * the diagonal exp(rate dt) increments; the growth-rotation rule still applies.  NULL returns to
  * the rates. */
 int tf_set_growth_increments(tf_handle* h, const double* dFg);
+/* Growth functions evaluated on the device at every growth update (tf_grow, tf_timestep), in
+ * place of host-given increments: Simulation::calculateGrowth (Simulation.cpp:9651-9670) over the
+ * GrowthFunctions list -- uniform (type 1, calculateGrowthUniform :9757-9799), ring (type 2,
+ * calculateGrowthRing :9801-9898) and grid (type 3, calculateGrowthGridBased :10065-10117, with
+ * ShapeBase::calculateFgFromGridCorners, ShapeBase.cpp:432-565) -- columnar layer, tissue
+ * weight 1.  A function is active while t0 <= time < t1 at the step's start time (tf_set_time;
+ * tf_timestep advances the time by dt); active functions multiply from the left in list order. */
+enum { TF_GROWTH_UNIFORM = 1, TF_GROWTH_RING = 2, TF_GROWTH_GRID = 3 };
+typedef struct tf_growth_function {
+    int32_t kind;               /* TF_GROWTH_*                                                  */
+    int32_t nx, ny;             /* grid points along x and y (grid, >= 2 each)                  */
+    double t0, t1;              /* active while t0 <= time < t1 (seconds)                       */
+    double rate[3];             /* 1/s along the rotated x, y and z (uniform; ring: the rate at
+                                   the inner radius, falling linearly to 0 at the outer one)    */
+    double angle;               /* xy angle of the rates, radians (uniform, ring)               */
+    double centre[2];           /* ring centre (read as float, as the reference stores it)      */
+    double inner, outer;        /* ring radii (float)                                           */
+    double zmin, zmax;          /* grid: range of the element's initial relative height         */
+    const double* grid_rates;   /* grid: [nx][ny][3] rates, 1/s                                 */
+    const double* grid_angles;  /* grid: [nx][ny] angles, radians                               */
+} tf_growth_function;
+/* n functions (n = 0 removes them: back to the rates).  initial_z[E]: every element's initial
+ * relative height (ShapeBase::setInitialZPosition); rel_src[E]: the element whose relative
+ * position an element takes (its column's apical element for the first layers - 2 elements of
+ * the column, else itself).  pinned = 1: grids read the relative positions pinned on the mesh at
+ * this call (GridGrowthsPinnedOnInitialMesh) and re-pinned by tf_pin_growth_positions; 0: the
+ * current positions every step.  interpolation: 0 nearest corner, 1 bilinear
+ * (GridGrowthsInterpolationType).  Replaces host-given increments. */
+int tf_set_growth_functions(tf_handle* h, int32_t n, const tf_growth_function* f, const double* initial_z,
+                            const int32_t* rel_src, int32_t pinned, int32_t interpolation);
+/* Pins the grids' relative positions on the current positions (the reference's pinning update at
+ * each PinningUpdateTime, Simulation::checkForPinningPositionsUpdate, Simulation.cpp:10425-10443). */
+int tf_pin_growth_positions(tf_handle* h);
+/* Simulation time (seconds) at the start of the next timestep. */
+int tf_set_time(tf_handle* h, double t);
+int tf_get_time(const tf_handle* h, double* t);
+/* The growth increments of the last growth update, row-major 3x3 per element (device growth
+ * functions or host-given increments; TF_INVALID under rate-based growth). */
+int tf_get_growth_increments(tf_handle* h, double* dFg);
 int tf_get_fg(tf_handle* h, double* Fg, double* InvFg);
 int tf_set_material(tf_handle* h, const double* lambda, const double* mu, const double* eta_int);
 

@@ -31,7 +31,8 @@ EXPORTS = ["tf_create", "tf_destroy", "tf_last_error", "tf_default_options", "tf
            "tf_solve_system", "tf_adjoint_elements", "tf_adjoint_reset", "tf_adjoint_step", "tf_adjoint_get",
            "tf_get_node_masses", "tf_detect_packing", "tf_set_packing_pairs", "tf_get_packing_pairs",
            "tf_set_growth_increments", "tf_element_dump_range_noalloc", "tf_get_coarse_operator",
-           "tf_kernel_probe_named", "tf_krylov_timing", "tf_copy_bandwidth"]
+           "tf_kernel_probe_named", "tf_krylov_timing", "tf_copy_bandwidth", "tf_set_growth_functions",
+           "tf_pin_growth_positions", "tf_set_time", "tf_get_time", "tf_get_growth_increments"]
 
 
 class TfError(RuntimeError):
@@ -56,6 +57,13 @@ class SolverOpts(C.Structure):
                 ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32),
                 ("implicit_packing", C.c_int32), ("preconditioner", C.c_int32), ("coarse_aggregates", C.c_int32),
                 ("coarse_refresh", C.c_int32), ("column_blocks", C.c_int32), ("mixed_fp64_from", C.c_int32)]
+
+
+class GrowthFunction(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("t0", C.c_double), ("t1", C.c_double),
+                ("rate", C.c_double * 3), ("angle", C.c_double), ("centre", C.c_double * 2), ("inner", C.c_double),
+                ("outer", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
+                ("grid_rates", C.POINTER(C.c_double)), ("grid_angles", C.POINTER(C.c_double))]
 
 
 class StepStats(C.Structure):
@@ -120,6 +128,11 @@ def load(path: str | None = None):
         "tf_kernel_probe_named": ([vp, C.c_char_p, i32, vp], i32),
         "tf_krylov_timing": ([vp, i32, vp, vp, vp], i32),
         "tf_copy_bandwidth": ([vp, C.c_int64, i32, vp], i32),
+        "tf_set_growth_functions": ([vp, i32, vp, vp, vp, i32, i32], i32),
+        "tf_pin_growth_positions": ([vp], i32),
+        "tf_set_time": ([vp, dbl], i32),
+        "tf_get_time": ([vp, vp], i32),
+        "tf_get_growth_increments": ([vp, vp], i32),
         "tf_set_tiling": ([vp, i32, i32], i32),
         "tf_get_tiling": ([vp, vp, vp, vp], i32),
         "tf_autotune": ([vp, dbl, i32, vp, vp, vp, vp], i32),

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libtfnr.so")
-SOURCES = ["tf_api.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_adjoint.hip", "tf_packing.hip"]
+SOURCES = ["tf_api.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_adjoint.hip", "tf_packing.hip", "tf_growth.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TF_OFFLOAD_ARCH", "gfx950")
 

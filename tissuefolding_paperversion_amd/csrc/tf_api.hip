@@ -85,6 +85,7 @@ void free_all(tf_handle* h)
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     packing_free(h);
+    growth_functions_free(h);
     warm_start_free(h);
     twolevel_free(h);
     if (h->h_seg) (void)hipHostFree(h->h_seg);
@@ -765,6 +766,7 @@ int tf_set_growth_increments(tf_handle* h, const double* dFg)
 {
     if (!h) return TF_INVALID;
     TF_GUARD_BEGIN
+    growth_functions_free(h);  // host-given increments replace device growth functions
     if (!dFg) {
         if (h->ginc) { (void)hipFree(h->ginc); h->ginc = nullptr; }
         return TF_OK;
@@ -774,6 +776,87 @@ int tf_set_growth_increments(tf_handle* h, const double* dFg)
     if (!h->ginc) h->ginc = dev_alloc<double>((size_t)9 * h->E);
     std::vector<double> soa = to_soa(dFg, h->E, 9);
     h2d(h, h->ginc, soa.data(), soa.size());
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_growth_functions(tf_handle* h, int32_t n, const tf_growth_function* f, const double* initial_z,
+                            const int32_t* rel_src, int32_t pinned, int32_t interpolation)
+{
+    if (!h || n < 0 || (n > 0 && (!f || !initial_z || !rel_src))) return TF_INVALID;
+    TF_GUARD_BEGIN
+    if (n == 0) {
+        growth_functions_free(h);
+        if (h->ginc) { (void)hipFree(h->ginc); h->ginc = nullptr; }
+        return TF_OK;
+    }
+    if (n > 64) return fail(h, TF_INVALID, "at most 64 growth functions");
+    if ((pinned != 0 && pinned != 1) || (interpolation != 0 && interpolation != 1))
+        return fail(h, TF_INVALID, "pinned and interpolation must be 0 or 1");
+    for (int k = 0; k < n; ++k) {
+        const tf_growth_function& g = f[k];
+        const std::string tag = "growth function " + std::to_string(k) + ": ";
+        if (g.kind < TF_GROWTH_UNIFORM || g.kind > TF_GROWTH_GRID) return fail(h, TF_INVALID, tag + "unknown kind");
+        if (!(g.t0 <= g.t1)) return fail(h, TF_INVALID, tag + "t0 > t1");
+        if (!std::isfinite(g.angle)) return fail(h, TF_INVALID, tag + "angle not finite");
+        for (int i = 0; i < 3; ++i)
+            if (!std::isfinite(g.rate[i])) return fail(h, TF_INVALID, tag + "rate not finite");
+        if (g.kind == TF_GROWTH_RING && !(std::isfinite(g.centre[0]) && std::isfinite(g.centre[1]) &&
+                                          std::isfinite(g.inner) && std::isfinite(g.outer) && g.inner < g.outer))
+            return fail(h, TF_INVALID, tag + "ring centre and radii must be finite with inner < outer");
+        if (g.kind == TF_GROWTH_GRID) {
+            if (g.nx < 2 || g.ny < 2 || !g.grid_rates || !g.grid_angles)
+                return fail(h, TF_INVALID, tag + "a grid needs nx, ny >= 2 and its rates and angles");
+            for (size_t q = 0; q < (size_t)g.nx * g.ny; ++q)
+                if (!std::isfinite(g.grid_angles[q]) || !std::isfinite(g.grid_rates[3 * q]) ||
+                    !std::isfinite(g.grid_rates[3 * q + 1]) || !std::isfinite(g.grid_rates[3 * q + 2]))
+                    return fail(h, TF_INVALID, tag + "grid values must be finite");
+        }
+    }
+    for (int e = 0; e < h->E; ++e) {
+        if (rel_src[e] < 0 || rel_src[e] >= h->E) return fail(h, TF_INVALID, "rel_src out of range");
+        if (!std::isfinite(initial_z[e])) return fail(h, TF_INVALID, "initial_z must be finite");
+    }
+    growth_functions_set(h, n, f, initial_z, rel_src, pinned, interpolation);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_pin_growth_positions(tf_handle* h)
+{
+    if (!h) return TF_INVALID;
+    TF_GUARD_BEGIN
+    growth_functions_pin(h);
+    TF_HIP(hipStreamSynchronize(h->stream));
+    return TF_OK;
+    TF_GUARD_END(h)
+}
+
+int tf_set_time(tf_handle* h, double t)
+{
+    if (!h || !std::isfinite(t)) return TF_INVALID;
+    h->time = t;
+    return TF_OK;
+}
+
+int tf_get_time(const tf_handle* h, double* t)
+{
+    if (!h || !t) return TF_INVALID;
+    *t = h->time;
+    return TF_OK;
+}
+
+int tf_get_growth_increments(tf_handle* h, double* dFg)
+{
+    if (!h || !dFg) return TF_INVALID;
+    if (!h->ginc) return fail(h, TF_INVALID, "rate-based growth: no increments");
+    TF_GUARD_BEGIN
+    std::vector<double> soa((size_t)9 * h->E);
+    d2h(h, soa.data(), h->ginc, soa.size());
+    TF_HIP(hipStreamSynchronize(h->stream));
+    for (int e = 0; e < h->E; ++e)
+        for (int q = 0; q < 9; ++q) dFg[9 * (size_t)e + q] = soa[(size_t)q * h->E + e];
     return TF_OK;
     TF_GUARD_END(h)
 }
@@ -1267,6 +1350,7 @@ int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats)
     auto t0 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, h->stream, h->status + 1, 1);
     launch_grow(h, dt);
+    h->time += dt;  // the growth update of this timestep is applied
     launch_surfaces(h);
     int32_t st4[4];
     d2h(h, st4, h->status, 4);

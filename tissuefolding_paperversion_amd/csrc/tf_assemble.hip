@@ -715,6 +715,7 @@ double* scratch_addr(int device)
 
 void launch_grow(tf_handle* h, double dt)
 {
+    if (h->gf.n_fn > 0) growth_functions_eval(h, dt);  // the step's increments into ginc
     hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->xref, h->uk, h->rate,
                        h->ginc, dt, h->opts.growth_rotation, h->E, h->fg, h->invfg, h->rot_tz, h->status);
     TF_HIP(hipGetLastError());

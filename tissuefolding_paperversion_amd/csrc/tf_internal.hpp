@@ -271,6 +271,22 @@ struct tf_handle {
 
     // two-level preconditioner (one tissue): block Jacobi plus a coarse space of the rigid-body
     // modes of node aggregates (columns through all layers, cut by an xy grid); see tf_krylov.hip
+    // growth functions evaluated on the device (tf_growth.hip)
+    struct GrowthFns {
+        int n_fn = 0;
+        std::vector<double> t0, t1;      // activity window of every function
+        std::vector<int> kind;
+        void* fns = nullptr;             // [n_fn] function records
+        double* cells = nullptr;         // grid cells: corner growths, angles, eliminated flags
+        double* rel = nullptr;           // [2][E] pinned relative positions
+        double* z_init = nullptr;        // [E] initial relative heights
+        int32_t* rel_src = nullptr;      // [E] element whose relative position an element takes
+        double* box = nullptr;           // [S][PK_NSTAT] reference bounding boxes (PK_ST_BB)
+        int pinned = 1, interpolation = 1;
+        bool needs_box = false;          // a grid reads the current positions
+    } gf;
+    double time = 0.0;                   // simulation time at the start of the next timestep
+
     struct TwoLevel {
         bool built = false, fresh = false, failed = false;
         int32_t n_agg = 0, nc = 0, n_pairs = 0;
@@ -350,6 +366,14 @@ void launch_bind_du(tf_handle* h);        // du[slave] <- du[master]
 // ---- implicit packing (tf_packing.hip) -------------------------------------------------------
 void packing_init(tf_handle* h, const int8_t* node_placement, const uint8_t* exposed, const double* ref_pos_aos);
 void packing_free(tf_handle* h);
+void launch_reference_box(tf_handle* h, double* stats);  // PK_ST_LO/BB/EXT of every tissue at uk
+// growth functions (tf_growth.hip)
+#define TF_GROWTH_MAX_ACTIVE 16
+void growth_functions_set(tf_handle* h, int n, const tf_growth_function* f, const double* z_init,
+                          const int32_t* rel_src, int pinned, int interpolation);
+void growth_functions_eval(tf_handle* h, double dt);  // increments of the active functions into ginc
+void growth_functions_pin(tf_handle* h);
+void growth_functions_free(tf_handle* h);
 void packing_prepare(tf_handle* h);             // detection tables (first use)
 void packing_reserve(tf_handle* h, int n);      // pair arrays for n pairs
 int packing_detect(tf_handle* h);               // masses + detection on uk; returns the pair count

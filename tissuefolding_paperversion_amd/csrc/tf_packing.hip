@@ -1035,6 +1035,17 @@ void packing_install(tf_handle* h, const std::vector<int32_t>& a, const std::vec
     pk.h_b = b;
 }
 
+// the reference's bounding box of every tissue at the current positions into stats (S x PK_NSTAT)
+void launch_reference_box(tf_handle* h, double* stats)
+{
+    packing_prepare(h);
+    PackingState& pk = h->pk;
+    hipLaunchKernelGGL(k_pk_box, dim3(pk.n_runs), dim3(256), 0, h->stream, pk.runs, h->seg_node_ptr, h->uk, pk.run_part);
+    hipLaunchKernelGGL(k_pk_box_finish, dim3(h->S), dim3(256), 0, h->stream, pk.seg_run_ptr, h->seg_node_ptr, pk.runs,
+                       pk.run_part, h->uk, stats);
+    TF_HIP(hipGetLastError());
+}
+
 int packing_detect(tf_handle* h)
 {
     packing_prepare(h);
@@ -1044,9 +1055,7 @@ int packing_detect(tf_handle* h)
     launch_packing_mass(h);
     hipLaunchKernelGGL(k_pk_evals, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->uk, h->conn, h->E, pk.elem_basal,
                        pk.ev);
-    hipLaunchKernelGGL(k_pk_box, dim3(pk.n_runs), dim3(256), 0, h->stream, pk.runs, h->seg_node_ptr, h->uk, pk.run_part);
-    hipLaunchKernelGGL(k_pk_box_finish, dim3(S), dim3(256), 0, h->stream, pk.seg_run_ptr, h->seg_node_ptr, pk.runs,
-                       pk.run_part, h->uk, pk.seg_stats);
+    launch_reference_box(h, pk.seg_stats);
     if (pk.n_chunks > 0)
         hipLaunchKernelGGL(k_pk_bins, dim3(pk.n_chunks), dim3(PK_CHUNK), 0, h->stream, pk.chunks, pk.ev, h->E,
                            pk.seg_stats, pk.chunk_part);

@@ -321,7 +321,7 @@ def run(mi: ModelInput, tissue: configs.Tissue, solver, n_steps: int | None = No
     writer = FrameWriter(save_dir, tissue.mesh) if save_dir else None
     every = max(1, int(round(mi.save_interval / mi.dt))) if mi.save_interval > 0 else 0
     stats, last = [], None
-    tracker = GrowthTracker(mi, tissue.mesh) if mi.needs_increments() else None
+    tracker = GrowthTracker(mi, tissue.mesh, solver) if mi.needs_increments() else None
     try:
         for k in range(steps):
             t = k * mi.dt
@@ -332,14 +332,14 @@ def run(mi: ModelInput, tissue: configs.Tissue, solver, n_steps: int | None = No
                     last = r
                 factors = np.exp(r * mi.dt)
             else:
-                inc = tracker.increments(t, lambda: solver.positions)
-                if last is None or not np.array_equal(inc, last):
-                    solver.set_growth_increments(inc)
-                    last = inc
-                factors = np.stack([inc[:, 0, 0], inc[:, 1, 1], inc[:, 2, 2]], axis=1)
+                tracker.before_step(t)
+                factors = None
             stats.append(solver.runOneStep(mi.dt))
             if writer is not None and every and (k + 1) % every == 0:
                 fg, _ = solver.get_fg()
+                if factors is None:  # the diagonal of the step's increments, evaluated on the device
+                    inc = solver.growth_increments()
+                    factors = np.stack([inc[:, 0, 0], inc[:, 1, 1], inc[:, 2, 2]], axis=1)
                 writer.write((k + 1) * mi.dt, solver.positions, fg, factors)
     finally:
         if writer is not None:
@@ -348,32 +348,24 @@ def run(mi: ModelInput, tissue: configs.Tissue, solver, n_steps: int | None = No
 
 
 class GrowthTracker:
-    """Per-step growth increments of a model input's functions, with the reference's bookkeeping
-    of relative positions: the initial ones (set at creation and at every PinningUpdateTime,
-    Simulation::checkForPinningPositionsUpdate, Simulation.cpp:10425-10443) for grids pinned on the
-    initial mesh, else the current ones."""
+    """A model input's growth functions on the device (``tf_set_growth_functions``): set once from
+    the initial mesh, the handle's time set before every step, and the grids' relative positions
+    re-pinned at every PinningUpdateTime for grids pinned on the mesh
+    (Simulation::checkForPinningPositionsUpdate, Simulation.cpp:10425-10443)."""
 
-    def __init__(self, mi: ModelInput, mesh: TissueMesh):
+    def __init__(self, mi: ModelInput, mesh: TissueMesh, solver):
         self.mi = mi
+        self.solver = solver
         self.functions = mi.growth_functions()
-        self.geo = G.GrowthGeometry(mesh)
+        geo = G.GrowthGeometry(mesh)
+        solver.set_growth_functions(self.functions, geo.initial_z, geo.rel_src, mi.grid_pinned,
+                                    mi.grid_interpolation)
         self.pins_done = [False] * len(mi.pin_times)
-        self.needs_positions = any(f.kind == 2 for f in self.functions) or (
-            not mi.grid_pinned and any(f.kind == 3 for f in self.functions))
 
-    def increments(self, time_sec: float, positions):
-        """(E, 3, 3) increments of the step starting at time_sec; ``positions`` returns the current
-        node positions when they are needed."""
-        pos = None
+    def before_step(self, time_sec: float):
+        self.solver.set_time(time_sec)
         if self.mi.grid_pinned:
             for k, tp in enumerate(self.mi.pin_times):
                 if time_sec >= tp and not self.pins_done[k]:
-                    pos = np.asarray(positions(), float)
-                    self.geo.initial_rel = self.geo.relative_positions(pos)
+                    self.solver.pin_growth_positions()
                     self.pins_done[k] = True
-        if self.needs_positions and pos is None:
-            pos = np.asarray(positions(), float)
-        if pos is None:
-            pos = self.geo.mesh.pos
-        return G.increments(self.functions, self.geo, pos, time_sec, self.mi.dt, self.mi.grid_pinned,
-                            self.mi.grid_interpolation)

@@ -123,6 +123,50 @@ class NewtonRaphsonSolver:
         a = None if dFg is None else L.f64(dFg).ravel()
         self._check(self._lib.tf_set_growth_increments(self.h, L.ptr(a)))
 
+    def set_growth_functions(self, functions, initial_z, rel_src, pinned=True, interpolation=1):
+        """Growth functions (growth.UniformGrowth, RingGrowth, GridGrowth) evaluated on the device
+        at every growth update from the handle's time; an empty list returns to the rates."""
+        recs = (L.GrowthFunction * max(1, len(functions)))()
+        keep = []
+        for k, f in enumerate(functions):
+            r = recs[k]
+            r.kind, r.t0, r.t1 = f.kind, float(f.t0), float(f.t1)
+            if f.kind in (1, 2):
+                r.rate[:] = [float(v) for v in f.rate]
+                r.angle = float(f.angle)
+            if f.kind == 2:
+                r.centre[:] = [float(f.centre[0]), float(f.centre[1])]
+                r.inner, r.outer = float(f.inner), float(f.outer)
+            if f.kind == 3:
+                rates = L.f64(f.rates)
+                angles = L.f64(f.angles)
+                keep += [rates, angles]
+                r.nx, r.ny = angles.shape
+                r.zmin, r.zmax = float(f.zmin), float(f.zmax)
+                r.grid_rates = rates.ctypes.data_as(C.POINTER(C.c_double))
+                r.grid_angles = angles.ctypes.data_as(C.POINTER(C.c_double))
+        z = L.f64(initial_z).ravel()
+        src = L.i32(rel_src).ravel()
+        self._check(self._lib.tf_set_growth_functions(self.h, len(functions), recs, L.ptr(z), L.ptr(src),
+                                                      int(bool(pinned)), int(interpolation)))
+
+    def pin_growth_positions(self):
+        self._check(self._lib.tf_pin_growth_positions(self.h))
+
+    def set_time(self, t):
+        self._check(self._lib.tf_set_time(self.h, float(t)))
+
+    def get_time(self):
+        t = C.c_double()
+        self._check(self._lib.tf_get_time(self.h, C.byref(t)))
+        return t.value
+
+    def growth_increments(self):
+        """(E, 3, 3) growth increments of the last growth update (device functions or host-given)."""
+        out = np.zeros((self.E, 3, 3))
+        self._check(self._lib.tf_get_growth_increments(self.h, L.ptr(out)))
+        return out
+
     def set_growth_rates(self, rates):
         r = L.f64(rates).ravel()
         self._check(self._lib.tf_set_growth_rates(self.h, L.ptr(r)))
