@@ -66,7 +66,7 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=1, help="CPU-oracle runs (the median is reported)")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     p.add_argument("--no-batch", action="store_true", help="skip the batched 256-disc sweep")
-    p.add_argument("--batch-steps", type=int, default=3)
+    p.add_argument("--batch-steps", type=int, default=60, help="timed timesteps of the 256-disc sweep (config 4: 60)")
     p.add_argument("--no-adjoint", action="store_true", help="skip the config-5 adjoint leg")
     p.add_argument("--adjoint-fd", type=int, default=7, help="parameters checked by central differences")
     p.add_argument("--krylov", type=int, default=-1)
@@ -206,17 +206,23 @@ def trajectory(NewtonRaphsonSolver, tissue, opts, steps, tiling, ref=None):
 
 
 def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
-    """The 256-disc sweep (one tissue segment per disc in one handle) against the same discs run
-    one handle at a time, plus a bounded CPU-oracle sample scaled by prism count."""
+    """The 256-disc sweep (one tissue segment per disc in one handle; two-level preconditioner with
+    one coarse block per disc) from t = 0 on a fresh handle, against the same discs run one handle
+    at a time, plus a bounded CPU-oracle sample scaled by prism count.  A second handle runs the
+    warm-up steps and the tiling search first."""
     from tissuefolding_paperversion_amd import configs
 
     ts = configs.config4()
     dt = 60.0
     n_prisms = sum(t.n_elements for t in ts)
-    s = NewtonRaphsonSolver(ts)
+    w = NewtonRaphsonSolver(ts)
     for _ in range(warmup):
-        s.runOneStep(dt)
-    s.autotune(dt, n_iters=64)
+        w.runOneStep(dt)
+    w.autotune(dt, n_iters=64)
+    wg, cap, _ = w.tiling()
+    w.close()
+    s = NewtonRaphsonSolver(ts)
+    s.set_tiling(wg, cap)
     s.positions
     t0 = time.perf_counter()
     stats = [s.runOneStep(dt) for _ in range(steps)]
@@ -245,6 +251,7 @@ def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
            "krylov_iterations_per_nr_iteration": float(sum(st["krylov_iterations"] for st in stats)
                                                        / max(1, sum(st["nr_iterations"] for st in stats))),
            "converged_steps": int(sum(st["converged"] for st in stats)),
+           "two_level": bool(stats[-1]["two_level"]),
            "tiling": {"workgroup": wg, "tile_blocks": cap, "tiles": tiles},
            "one_handle_per_disc": {"value": len(ts) / seq, "ms_per_step": 1e3 * seq,
                                    "batched_speedup": (len(ts) * steps / sec) / (len(ts) / seq)}}

@@ -9,9 +9,9 @@ with runOneStep's growth and surface updates around it.
   (same NR counts at every step).
 - config 3, the same mesh with FP32 and BF16 MFMA element tangents (FP32 Jacobian values): every one
   of 20 timesteps within 1e-8 of the FP64 run.
-- config 4, 256 discs of 500-8000 prisms in one handle: all 60 timesteps converge for every disc;
-  sampled discs equal their own single-handle runs bit for bit (NR counts too) and two small ones
-  the oracle within 1e-8 over three steps.
+- config 4, 256 discs of 500-8000 prisms in one handle (two-level, one coarse block per disc): all
+  60 timesteps converge for every disc; sampled discs equal their own single-handle runs bit for
+  bit (NR counts too) and two small ones the oracle within 1e-8 over three steps.
 - config 5, the adjoint of 30 timesteps of the ellipse: 64 parameters (3 layer moduli, 61 rate
   scales) against central differences, h = 1e-6, forward runs at nr_tol 1e-11 with exact solves.
 """
@@ -99,12 +99,14 @@ def test_config4_full_sweep(oracle):
     offs = np.cumsum([0] + [3 * t.n_nodes for t in ts])
     small = [int(k) for k in np.argsort([t.n_elements for t in ts])[:2]]
     ors = {k: oracle.OracleTissue(ts[k]) for k in small}
-    iters = []
+    iters, kry, nrs = [], 0, 0
     for step in range(60):
         st = batch.runOneStep(60.0)
         it, dn, cv = batch.segment_stats()
         assert st["converged"] == 1 and cv.all() and np.all(dn <= 1e-8), step
-        assert st["krylov_fallbacks"] == 0, step
+        assert st["krylov_fallbacks"] == 0 and st["two_level"] == 1, step
+        kry += st["krylov_iterations"]
+        nrs += st["nr_iterations"]
         iters.append(it.copy())
         if step < 3:
             xb = batch.positions
@@ -114,11 +116,14 @@ def test_config4_full_sweep(oracle):
                 assert rel_err(xb[offs[k]:offs[k + 1]], o.pos) < 1e-8
     xb = batch.positions
     print(f"config 4: {sum(t.n_elements for t in ts)} prisms, NR iterations per disc-step "
-          f"{np.mean(iters):.2f} (max {np.max(iters)})", flush=True)
+          f"{np.mean(iters):.2f} (max {np.max(iters)}), Krylov iterations per NR iteration {kry / nrs:.1f} "
+          "(two-level, one coarse block per disc)", flush=True)
+    assert kry / nrs <= 40
     sizes = np.array([t.n_elements for t in ts])
     sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 17, 128, 255})
     for k in sample:
-        s = _solver(ts[k], preconditioner=0)  # block Jacobi, as in a handle of several tissues
+        s = _solver(ts[k], preconditioner=1, column_blocks=0)  # two-level with nodal smoother blocks, as in the
+        # handle of discs of 2 and 3 layers
         for step in range(60):
             st = s.runOneStep(60.0)
             assert st["nr_iterations"] == iters[step][k], (k, step)

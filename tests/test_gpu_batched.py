@@ -1,9 +1,10 @@
 """Batched sweep: many independent discs in one handle (one tissue segment each).
 
 Per-disc bar: the batched trajectory equals the disc's own single-handle run (to 1e-12
-relative; the row tiles never straddle two tissues and every reduction is per tissue, so the
-bits are in fact the same) with the same NR iteration count, and a sample of discs matches the
-CPU oracle within 1e-8 with the oracle's NR iteration counts.
+relative; the row tiles and the coarse blocks never straddle two tissues and every reduction is
+per tissue, so the bits are in fact the same) with the same NR iteration count, under block Jacobi
+and under the two-level preconditioner, and a sample of discs matches the CPU oracle within 1e-8
+with the oracle's NR iteration counts.
 """
 import numpy as np
 import pytest
@@ -23,9 +24,13 @@ def _small_sweep():
     return configs.config4(n_discs=12, seed=9, min_prisms=100.0, max_prisms=1500.0)
 
 
-def test_batched_discs_equal_single_runs():
+@pytest.mark.parametrize("prec", [0, 1], ids=["block_jacobi", "two_level"])
+def test_batched_discs_equal_single_runs(prec):
+    """Block Jacobi, and the two-level preconditioner with one coarse block per disc (nodal 3 x 3
+    smoother blocks: the discs have 2 or 3 layers)."""
     ts = _small_sweep()
-    batch = _solver(ts)
+    opts = dict(preconditioner=prec, column_blocks=0)
+    batch = _solver(ts, **opts)
     steps = 2
     seg_iters = []
     for _ in range(steps):
@@ -35,9 +40,10 @@ def test_batched_discs_equal_single_runs():
     xb = batch.positions
     off = 0
     for k, t in enumerate(ts):
-        s = _solver(t)
+        s = _solver(t, **opts)
         for step in range(steps):
             st = s.runOneStep(60.0)
+            assert st["two_level"] == prec
             assert st["converged"] == 1
             assert st["nr_iterations"] == seg_iters[step][k]
         xs = s.positions

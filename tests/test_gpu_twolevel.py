@@ -153,13 +153,30 @@ def test_packing_pairs_with_two_level(oracle):
         assert rel_err(s.positions, o.pos) < 1e-8
 
 
-def test_several_tissues_use_block_jacobi():
-    ts = [configs.disc_tissue(radius=15.0, n_steps=1), configs.disc_tissue(radius=12.0, n_steps=1, seed=3)]
-    a, b = _solver(ts), _solver(ts, **TL)
-    for _ in range(2):
-        sa, sb = a.runOneStep(ts[0].dt), b.runOneStep(ts[0].dt)
-        assert sa["krylov_iterations"] == sb["krylov_iterations"]
-    assert np.array_equal(a.positions, b.positions)
+def test_several_tissues_match_their_own_handles(oracle):
+    """Two-level on a handle of several tissues: every tissue has its own aggregates and its own
+    block of the coarse operator, its partial sums and rebuild decisions are its own, so each
+    tissue's Newton and Krylov iterations and positions equal a handle of its own bit for bit; the
+    first tissue also matches the oracle."""
+    ts = [_ellipse(), configs.wing_disc_tissue(a=50.0, b=40.0, h=3.66, seed=5, growth_seed=6, n_steps=3, name="e2"),
+          configs.wing_disc_tissue(a=70.0, b=30.0, h=3.66, seed=7, growth_seed=8, n_steps=3, name="e3")]
+    batch = _solver(ts, **TL)
+    singles = [_solver(t, **TL) for t in ts]
+    o = oracle.OracleTissue(ts[0])
+    offs = np.cumsum([0] + [3 * t.n_nodes for t in ts])
+    for step in range(3):
+        st = batch.runOneStep(60.0)
+        assert st["converged"] == 1 and st["two_level"] == 1 and st["krylov_fallbacks"] == 0
+        it = batch.segment_stats()[0]
+        for k, s in enumerate(singles):
+            sk = s.runOneStep(60.0)
+            assert sk["two_level"] == 1 and it[k] == sk["nr_iterations"], (step, k)
+        it_o, _ = o.timestep(60.0)
+        assert it[0] == it_o
+    xb = batch.positions
+    for k, s in enumerate(singles):
+        assert np.array_equal(xb[offs[k]:offs[k + 1]], s.positions), k
+    assert rel_err(xb[:offs[1]], o.pos) < 1e-8
 
 
 def test_option_validation():

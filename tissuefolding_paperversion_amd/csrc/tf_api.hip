@@ -1938,7 +1938,7 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
         if (!twolevel_active(h) || !h->tl.built) { *bytes = 0.0; return TF_OK; }
         const tf_handle::TwoLevel& tl = h->tl;
         const double D = 3.0 * tl.L;
-        *bytes = (double)tl.nc * tl.ldA * 2.0 + (double)tl.ncol * D * D * 4.0 + N * 18.0 * 8.0 + n * 8.0 * 7.0 +
+        *bytes = (double)tl.ainv_halves * 2.0 + (double)tl.ncol * D * D * 4.0 + N * 18.0 * 8.0 + n * 8.0 * 7.0 +
                  tl.nc * 8.0 * 3.0;
     } else if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver

@@ -287,38 +287,51 @@ struct tf_handle {
     } gf;
     double time = 0.0;                   // simulation time at the start of the next timestep
 
+    // Two-level preconditioner.  Every tissue has its own aggregates and coarse unknowns (tissue
+    // after tissue); the coarse operator is block diagonal, one dense block per tissue.
     struct TwoLevel {
         bool built = false, fresh = false, failed = false;
         int32_t n_agg = 0, nc = 0, n_pairs = 0;
         int32_t age = 0;                                    // timesteps since the coarse operator was built
-        int32_t base_it1 = -1;                              // Krylov iterations of NR iteration 1 after the build
-        bool stale = false;                                 // a later NR iteration 1 took > 1.25x base_it1
+        std::vector<int32_t> base_it1;                      // [S] Krylov iterations of NR iteration 1 after the build
+        std::vector<uint8_t> stale;                         // [S] a later NR iteration 1 took > 1.25x base_it1
+        int32_t* agg_seg = nullptr;                         // [A] tissue of every aggregate
+        int32_t* seg_c0 = nullptr;                          // [S+1] first coarse unknown of every tissue
+        int32_t* seg_ldc = nullptr;                         // [S] row stride of the tissue's block of Ac
+        int64_t* seg_acoff = nullptr;                       // [S] offset of that block in Ac
+        int4* seg_coarse = nullptr;                         // [S] first coarse unknown, count, row stride of
+                                                            //     Ainv (halves), offset of its block (half pairs)
+        uint8_t* refresh = nullptr;                         // [S] 1: the tissue's block is rebuilt
+        std::vector<int32_t> h_seg_c0, h_seg_ldc, h_seg_lda;
+        std::vector<int64_t> h_seg_acoff, h_seg_aoff;       // (Ainv offsets in halves)
+        int32_t max_ldA = 0;                                // largest Ainv row stride (k_tp_z's LDS)
+        int64_t ainv_halves = 0;                            // Ainv entries over all tissues
         int32_t* node_agg = nullptr;                        // [N] aggregate of every node
         int32_t *agg_ptr = nullptr, *agg_nodes = nullptr;   // [A+1], [N] nodes of every aggregate
         double* pcoef = nullptr;                            // [N][3][6] rows of P
         int32_t *coff = nullptr, *krank = nullptr;          // [A] coarse offset and rank
         int32_t *pair_a = nullptr, *pair_b = nullptr, *pair_ptr = nullptr, *pair_blk = nullptr;
-        double* Ac = nullptr;                               // [ldc][ldc] P^T M P, then -Ac^-1 (lower tiles)
-        int32_t ldc = 0;                                    // nc rounded up to the sweep's tile (64)
+        double* Ac = nullptr;                               // per tissue [ldc][ldc] P^T M P, then -Ac^-1 (lower tiles)
+        int32_t ldc = 0;                                    // largest block stride: nc_s rounded up to the sweep's tile (64)
         double *swP = nullptr, *swO = nullptr, *swD = nullptr;  // the sweep's panels [ldc][64] and pivot inverses [2][64][64]
-        void* Ainv = nullptr;                               // [nc][ldA] FP16 copy of Ac^-1 / scale
-        double* ascale = nullptr;                           // [2] scale and its reciprocal
-        int32_t ldA = 0;                                    // row stride of Ainv (halves)
+        void* Ainv = nullptr;                               // per tissue [nc_s][ldA_s] FP16 copy of Ac^-1 / scale
+        double* ascale = nullptr;                           // [S][2] scale and its reciprocal
+        int32_t ldA = 0;                                    // row stride of tissue 0's Ainv block (halves)
         int32_t L = 1, ncol = 0;                            // nodes per column, columns
         int32_t* col_blk = nullptr;                         // [ncol][L][L] BSR block of every node pair or -1
         float* dcinv = nullptr;                             // [ncol][3L][3L] FP32 inverse column blocks
-        int32_t* info = nullptr;                            // the sweep's failure flag
+        int32_t* info = nullptr;                            // [S] the sweep's failure flag of every tissue
         // the solve in aggregate order (position r = node agg_nodes[r])
         int32_t *prow_ptr = nullptr, *pcol = nullptr, *pblk = nullptr;  // [N+1], [nb], [nb] permuted Jacobian pattern
         int32_t* pass_ptr = nullptr;                        // [A+1] row passes of every aggregate
         int2* pass_rows = nullptr;                          // [passes] rows [x, y) of every pass
         int4* first_pass = nullptr;                         // [A] rows and blocks of every aggregate's first pass
-        int32_t* agg_order = nullptr;                       // [A] aggregates by descending size (workgroup order)
+        int2* agg_order = nullptr;                          // [A] aggregate and tissue, by descending size (workgroup order)
         double* pcoef_p = nullptr;                          // [N][3][6] rows of P by position
         double *val_p = nullptr;                            // [9][nb] FP64 values in aggregate order
         float* val32_p = nullptr;                           // [9][nb] FP32 values in aggregate order
         double *tpart = nullptr, *ptq = nullptr, *rcx = nullptr;  // [SL_COUNT][A], [nc], [2][nc]
-        int32_t* tp_seg_ptr = nullptr;                      // {0, A}: the aggregates as the tissue's "tiles"
+        int32_t* tp_seg_ptr = nullptr;                      // [S+1] the aggregates as every tissue's "tiles"
         int32_t tp_max_blocks = 0;                          // most blocks of a pass (LDS products)
         std::vector<int32_t> h_node_agg;
     } tl;

@@ -537,6 +537,11 @@ struct TLArgs {
     const double* pcoef;       // [N][3][6] rows of P by node (columns past the aggregate's rank are zero)
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
+    const int32_t* agg_seg;    // [A] tissue of every aggregate
+    const int32_t* seg_c0;     // [S+1] first coarse unknown of every tissue
+    const int32_t* seg_ldc;    // [S] row stride of the tissue's block of Ac
+    const int64_t* seg_acoff;  // [S] offset of that block
+    const uint8_t* refresh;    // [S] 1: the tissue's block is rebuilt (others are left alone)
 };
 
 // Inverse of every column block of M, as a symmetric FP32 copy: 64 / 3L columns per wave, one lane
@@ -628,17 +633,19 @@ struct TPArgs {
     const int32_t* pass_ptr;   // [A+1] passes of every aggregate
     const int2* pass_rows;     // [passes] rows [x, y) of every pass (<= WG / 3 rows, <= TP_CAP blocks)
     const int4* first_pass;    // [A] rows [x, y) and blocks [z, w) of every aggregate's first pass
-    const int32_t* order;      // [A] aggregate of every workgroup: largest first (the dispatcher deals
-                               // workgroups in order, so the small ones fill in behind the large ones)
+    const int2* order;         // [A] aggregate and tissue of every workgroup: largest first (the
+                               // dispatcher deals workgroups in order, so the small ones fill in
+                               // behind the large ones)
     const double* pcoef;       // [N][3][6] rows of P by position
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
     const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
-    const __half2* Ainv;       // [nc][ldA / 2] Ac^-1 / scale in FP16 pairs
-    const double* ascale;      // [2] scale and its reciprocal
+    const __half2* Ainv;       // per tissue [nc_s][ldA_s / 2] Ac_s^-1 / scale_s in FP16 pairs
+    const int4* seg_coarse;    // [S] first coarse unknown, count nc_s, ldA_s, offset of the Ainv block (pairs)
+    const double* ascale;      // [S][2] scale and its reciprocal
     double* ptq;               // [nc] P^T q of the current iteration
     double* rcx;               // [2][nc] P^T r (ping-pong by parity)
-    int ldA, nc, n_agg;
+    int nc, n_agg;
 };
 
 // q = M p over the aggregate's passes; own rows see p_new = z + beta p_old (beta = 0: p = z), and
@@ -709,8 +716,9 @@ __global__ __launch_bounds__(WG) void k_tp_init(TileArgs ta, TPArgs tp, const VT
                                                 double* __restrict__ p_old)
 {
     extern __shared__ double prod[];
-    const int a = tp.order[blockIdx.x];
-    const bool active = ta.seg[0].active;
+    const int2 ag = tp.order[blockIdx.x];
+    const int a = ag.x;
+    const bool active = ta.seg[ag.y].active;
     double bb = 0.0, rc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     auto own = [&](int pos, int comp, double mx, double) {
         const int d = 3 * pos + comp;
@@ -746,10 +754,12 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
                                              double* __restrict__ p_new, double* __restrict__ q, int maxit)
 {
     extern __shared__ double prod[];
-    const int a = tp.order[blockIdx.x];
-    SegScalars& S = ta.seg[0];
+    const int2 ag = tp.order[blockIdx.x];
+    const int a = ag.x, sg = ag.y;
+    SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;  // after the solve's end: one load before the exit
     const int ps0 = tp.pass_ptr[a], ps1 = tp.pass_ptr[a + 1];
+    const int st0 = ta.seg_tile_ptr[sg], st1 = ta.seg_tile_ptr[sg + 1];  // the tissue's aggregates
     const int tid = threadIdx.x;
     const int4 rec = tp.first_pass[a];  // rows and blocks of the first pass, one load
     int2 rr = int2{rec.x, rec.y};
@@ -767,11 +777,11 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
     }
     const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
     double sc[3];
-    seg_sums_range<3>(ta, 0, ta.T, sl, sc);
+    seg_sums_range<3>(ta, st0, st1, sl, sc);
     const int it = S.iters;
     const double best = S.best_rr_h[P];
     const int best_it = S.best_it_h[P];
-    const bool lead = a == 0 && tid == 0;
+    const bool lead = a == st0 && tid == 0;
     {
         const bool stagnant = it - best_it > 256;
         const bool conv = sc[0] <= S.tol2 || (stagnant && best <= 100.0 * S.tol2);
@@ -868,29 +878,35 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
                                              double* __restrict__ r, double* __restrict__ z)
 {
     constexpr int D = 3 * L, CPP = WG / D, NV = 2 * TP_PAIRS;
-    const int a = tp.order[blockIdx.x];
-    SegScalars& S = ta.seg[0];
+    const int2 ag = tp.order[blockIdx.x];
+    const int a = ag.x, sg = ag.y;
+    SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;  // after the solve's end: one load before the exit
     const int n0 = tp.agg_ptr[a], ncols = (tp.agg_ptr[a + 1] - n0) / L;
     const int c0 = tp.coff[a], kr = tp.krank[a];
+    const int4 sc4 = tp.seg_coarse[sg];  // the tissue's coarse unknowns [C0, C0 + ncs) and Ainv block
+    const int st0 = ta.seg_tile_ptr[sg], st1 = ta.seg_tile_ptr[sg + 1];
     extern __shared__ float tp_rcs[];  // r_c, ldA entries (zero past nc)
     __shared__ double rl[WG];
     __shared__ double zcs[6];
     const int tid = threadIdx.x, slot = tid / D, li = tid % D;
-    const int nc = tp.nc, ld2 = tp.ldA / 2;
+    const int nc = tp.nc, C0 = sc4.x, ncs = sc4.y, ldA = sc4.z, ld2 = ldA / 2, lc0 = c0 - C0;
+    const int last = max(ncs, 1) - 1;  // a tissue without coarse unknowns has one row of zeros
+    const __half2* Ab = tp.Ainv + sc4.w;
     __half2 pre[6][TP_PAIRS];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int m = 0; m < TP_PAIRS; ++m)
-            pre[k][m] = tp.Ainv[(size_t)min(c0 + k, nc - 1) * ld2 + min(tid + WG * m, ld2 - 1)];
-    const double* rcp = tp.rcx + (size_t)P * nc;
+            pre[k][m] = Ab[(size_t)min(lc0 + k, last) * ld2 + min(tid + WG * m, ld2 - 1)];
+    const double* rcp = tp.rcx + (size_t)P * nc + C0;
+    const double* ptq = tp.ptq + C0;
     double vx[NV], vq[NV];
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
-        const int ic = min(tid + WG * m, nc - 1);  // clamped: always a valid address
+        const int ic = min(tid + WG * m, last);  // clamped: always a valid address
         vx[m] = rcp[ic];
-        vq[m] = INIT ? 0.0 : tp.ptq[ic];
+        vq[m] = INIT ? 0.0 : ptq[ic];
     }
     bool own = slot < CPP && slot < ncols;
     int g = n0 / L + (own ? slot : 0);  // global column; its positions gL .. gL + L - 1
@@ -908,11 +924,11 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     if (!INIT) {
         const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
         double sc[3];
-        seg_sums_range<3>(ta, 0, ta.T, sl, sc);
+        seg_sums_range<3>(ta, st0, st1, sl, sc);
         // (p, q) > 0 needs M positive definite, (r, z) > 0 the preconditioner: either failing stops
         // the solve as stalled, x keeping residual sc[2]
         if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
-            if (a == 0 && tid == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
+            if (a == st0 && tid == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
             return;
         }
         alpha = sc[0] / sc[1];
@@ -920,7 +936,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
         const int i = tid + WG * m;
-        if (i < tp.ldA) tp_rcs[i] = i < nc ? (float)(vx[m] - alpha * vq[m]) : 0.f;
+        if (i < ldA) tp_rcs[i] = i < ncs ? (float)(vx[m] - alpha * vq[m]) : 0.f;
     }
     __syncthreads();
     {
@@ -931,7 +947,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 #pragma unroll
             for (int m = 0; m < TP_PAIRS; ++m) {
                 const int j = 2 * (tid + WG * m);
-                if (j < tp.ldA) {  // pairs past ldA hold clamped copies: skipped
+                if (j < ldA) {  // pairs past ldA hold clamped copies: skipped
                     const float2 f = __half22float2(pre[k][m]);
                     const float2 c2 = *reinterpret_cast<const float2*>(tp_rcs + j);
                     sv[k] += (double)f.x * c2.x + (double)f.y * c2.y;
@@ -950,7 +966,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
             double t = 0.0;
 #pragma unroll
             for (int k = 0; k < WG / 64; ++k) t += zc_red[tid][k];
-            zcs[tid] = tid < kr ? t * tp.ascale[0] : 0.0;  // P's columns past the rank are zero
+            zcs[tid] = tid < kr ? t * tp.ascale[2 * sg] : 0.0;  // P's columns past the rank are zero
         }
         __syncthreads();
     }
@@ -996,7 +1012,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     if (tid == 0) {
         ta.partial[(size_t)(SL_RZ0 + (1 - P)) * ta.T + a] = out[0];
         ta.partial[(size_t)(SL_RR0 + (1 - P)) * ta.T + a] = out[1];
-        if (!INIT && a == 0) S.iters += 1;
+        if (!INIT && a == st0) S.iters += 1;
     }
     if (tid < kr) tp.rcx[(size_t)(1 - P) * nc + c0 + tid] = out[2 + tid];
 }
@@ -1034,10 +1050,12 @@ __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restri
                                               const int32_t* __restrict__ pair_b, const int32_t* __restrict__ pair_ptr,
                                               const int32_t* __restrict__ pair_blk, const int32_t* __restrict__ blk_row,
                                               const int32_t* __restrict__ col, const double* __restrict__ val, int nb,
-                                              double* __restrict__ Ac, int ldc)
+                                              double* __restrict__ Ac)
 {
     const int pr = blockIdx.x;
     const int a = pair_a[pr], b = pair_b[pr];
+    const int sg = tl.agg_seg[a];  // a and b are aggregates of one tissue
+    if (!tl.refresh[sg]) return;
     double acc[36];
 #pragma unroll
     for (int q = 0; q < 36; ++q) acc[q] = 0.0;
@@ -1075,7 +1093,8 @@ __global__ __launch_bounds__(WG) void k_tl_ac(TLArgs tl, const int32_t* __restri
             double t = 0.0;
 #pragma unroll
             for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
-            Ac[(size_t)(tl.coff[a] + pi) * ldc + tl.coff[b] + ri] = t;
+            const int c0 = tl.seg_c0[sg];
+            Ac[tl.seg_acoff[sg] + (size_t)(tl.coff[a] - c0 + pi) * tl.seg_ldc[sg] + tl.coff[b] - c0 + ri] = t;
         }
     }
 }
@@ -2174,8 +2193,12 @@ void tl_release(tf_handle* h)
     tf_handle::TwoLevel& tl = h->tl;
     for (int32_t** p : {&tl.node_agg, &tl.agg_ptr, &tl.agg_nodes, &tl.coff, &tl.krank, &tl.pair_a,
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk, &tl.prow_ptr, &tl.pcol,
-                        &tl.pblk, &tl.pass_ptr, &tl.tp_seg_ptr, &tl.agg_order})
+                        &tl.pblk, &tl.pass_ptr, &tl.tp_seg_ptr, &tl.agg_seg, &tl.seg_c0, &tl.seg_ldc})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (tl.agg_order) { (void)hipFree(tl.agg_order); tl.agg_order = nullptr; }
+    if (tl.seg_acoff) { (void)hipFree(tl.seg_acoff); tl.seg_acoff = nullptr; }
+    if (tl.seg_coarse) { (void)hipFree(tl.seg_coarse); tl.seg_coarse = nullptr; }
+    if (tl.refresh) { (void)hipFree(tl.refresh); tl.refresh = nullptr; }
     for (double** p : {&tl.pcoef, &tl.Ac, &tl.swP, &tl.swO, &tl.swD, &tl.pcoef_p, &tl.val_p,
                        &tl.tpart, &tl.ptq, &tl.rcx})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
@@ -2308,18 +2331,15 @@ static void twolevel_build(tf_handle* h)
     TF_HIP(hipMemcpyAsync(excl.data(), h->fixed, excl.size(), hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipStreamSynchronize(h->stream));
     for (size_t k = 0; k < h->h_bound.size(); k += 2) excl[h->h_bound[k]] = 1;
-    // k_tp_z stages two coarse vectors of ldA (nc rounded up to 8) floats in LDS beside ~4 KB of its own
+    // k_tp_z stages a tissue's coarse vector (ldA floats, nc_s rounded up to 8) in LDS beside ~4 KB of
+    // its own
     const int max_nc = std::min(4096, (h->max_lds - 4096) / (2 * (int)sizeof(float)) / 8 * 8);
-    // default: one aggregate per 44 nodes, at most 600, and no more aggregates than two per
-    // compute unit: k_tp_z (one workgroup per aggregate) holds two workgroups per compute unit
-    // (its registers), so more aggregates would take a second round of workgroups
+    // default: one aggregate per 44 nodes of the tissue, at most 600; a handle of one tissue also
+    // keeps at most two aggregates per compute unit: k_tp_z (one workgroup per aggregate) holds two
+    // workgroups per compute unit (its registers), so more would take a second round of workgroups
     const bool by_default = h->opts.coarse_aggregates <= 0;
-    const int max_agg = by_default ? std::min(max_nc / 6, 2 * h->n_cu) : max_nc / 6;
-    // (the aggregates cover the tissue's part of a grid over its bounding box, so a target above
-    // max_agg can still give at most max_agg aggregates; the loop below shrinks it when not)
-    int target = by_default ? std::max(1, std::min(600, N / 44)) : h->opts.coarse_aggregates;
-    target = std::min(target, max_nc / 6);
-    int A = 0;
+    const int S = h->S;
+    const int max_agg = by_default && S == 1 ? std::min(max_nc / 6, 2 * h->n_cu) : max_nc / 6;
     // Node columns: nodes joined by the vertical edges of prisms (corners k and k + 3), each
     // listed in ascending node order.  Used when column_blocks is on and every column has the same
     // number of nodes L <= TL_MAX_LEVELS; otherwise every node is a column of its own (L = 1,
@@ -2356,19 +2376,42 @@ static void twolevel_build(tf_handle* h)
     const int ncol = (int)col_start.size();
     std::vector<std::vector<int32_t>> col_nodes(ncol);
     for (int n = 0; n < N; ++n) col_nodes[col_of[n]].push_back(n);
-    // aggregates of whole columns, placed by each column's first node; cells under 16 nodes merge
-    std::vector<double> xc(3 * (size_t)ncol);
-    for (int c = 0; c < ncol; ++c)
-        for (int k = 0; k < 3; ++k) xc[3 * c + k] = x[3 * (size_t)col_start[c] + k];
-    // at most six coarse unknowns per aggregate: an elongated tissue can give more aggregates than
-    // the target (cells along the boundary), so the target shrinks until A fits max_agg (one
-    // aggregate at target 1, so this ends)
-    std::vector<int32_t> aggc;
-    while (true) {
-        aggc = tl_aggregates(xc, ncol, target, std::max(1, 16 / L), A);
-        if (A <= max_agg || target <= 1) break;
-        target = std::max(1, std::min(target - 1, (int)(0.9 * target * max_agg / A)));
+    // aggregates of whole columns, tissue by tissue, placed by each column's first node; cells under
+    // 16 nodes merge.  A column lies in one tissue (its nodes share prisms), and the columns of a
+    // tissue are consecutive (numbered by first node).
+    std::vector<int32_t> aggc(ncol), agg_seg, seg_agg_ptr(S + 1, 0);
+    {
+        int c = 0;
+        for (int sg = 0; sg < S; ++sg) {
+            const int n1 = h->h_node_seg_ptr[sg + 1], Ns = n1 - h->h_node_seg_ptr[sg];
+            const int cc0 = c;
+            while (c < ncol && col_start[c] < n1) ++c;
+            const int ncs = c - cc0;
+            std::vector<double> xc(3 * (size_t)ncs);
+            for (int k = 0; k < ncs; ++k)
+                for (int d = 0; d < 3; ++d) xc[3 * k + d] = x[3 * (size_t)col_start[cc0 + k] + d];
+            // (the aggregates cover the tissue's part of a grid over its bounding box, so a target
+            // above max_agg can still give at most max_agg aggregates; the loop shrinks it when
+            // not, down to one aggregate at target 1)
+            int target = by_default ? std::max(1, std::min(600, Ns / 44)) : h->opts.coarse_aggregates;
+            target = std::min(target, max_nc / 6);
+            int As = 0;
+            std::vector<int32_t> lab;
+            if (ncs > 0) {
+                while (true) {
+                    lab = tl_aggregates(xc, ncs, target, std::max(1, 16 / L), As);
+                    if (As <= max_agg || target <= 1) break;
+                    target = std::max(1, std::min(target - 1, (int)(0.9 * target * max_agg / As)));
+                }
+            }
+            const int A0 = seg_agg_ptr[sg];
+            for (int k = 0; k < ncs; ++k) aggc[cc0 + k] = A0 + lab[k];
+            seg_agg_ptr[sg + 1] = A0 + As;
+            agg_seg.insert(agg_seg.end(), As, sg);
+        }
+        if (c != ncol) throw std::runtime_error("two-level preconditioner: node columns out of tissue order");
     }
+    const int A = seg_agg_ptr[S];
     std::vector<int32_t> agg(N);
     for (int n = 0; n < N; ++n) agg[n] = aggc[col_of[n]];
     std::vector<std::vector<int32_t>> members(A);  // column by column
@@ -2380,7 +2423,31 @@ static void twolevel_build(tf_handle* h)
         coff[a + 1] = coff[a] + krank[a];
     }
     const int nc = coff[A];
-    if (nc > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");  // unreachable: 6 A <= max_nc
+    // every tissue's block: coarse unknowns [C0, C0 + nc_s), Ac block of stride ldc_s (a multiple of
+    // the sweep's tile), FP16 inverse of stride ldA_s (a multiple of 8; one row of zeros when the
+    // tissue has no coarse unknown)
+    tl.h_seg_c0.assign(S + 1, 0);
+    tl.h_seg_ldc.assign(S, 0);
+    tl.h_seg_lda.assign(S, 0);
+    tl.h_seg_acoff.assign(S + 1, 0);
+    tl.h_seg_aoff.assign(S + 1, 0);
+    std::vector<int4> seg_coarse(S);
+    tl.max_ldA = 8;
+    int max_ldc = SW_B;
+    for (int sg = 0; sg < S; ++sg) {
+        const int C0 = coff[seg_agg_ptr[sg]], C1 = coff[seg_agg_ptr[sg + 1]], ncs = C1 - C0;
+        if (ncs > max_nc) throw std::runtime_error("two-level preconditioner: coarse space exceeds the LDS of a workgroup");
+        tl.h_seg_c0[sg + 1] = C1;
+        tl.h_seg_ldc[sg] = (ncs + SW_B - 1) / SW_B * SW_B;
+        tl.h_seg_lda[sg] = std::max(8, (ncs + 7) / 8 * 8);
+        tl.h_seg_acoff[sg + 1] = tl.h_seg_acoff[sg] + (int64_t)tl.h_seg_ldc[sg] * tl.h_seg_ldc[sg];
+        tl.h_seg_aoff[sg + 1] = tl.h_seg_aoff[sg] + (int64_t)std::max(ncs, 1) * tl.h_seg_lda[sg];
+        if (tl.h_seg_aoff[sg] / 2 > INT32_MAX) throw std::runtime_error("two-level preconditioner: coarse blocks too large");
+        seg_coarse[sg] = int4{C0, ncs, tl.h_seg_lda[sg], (int32_t)(tl.h_seg_aoff[sg] / 2)};
+        tl.max_ldA = std::max(tl.max_ldA, tl.h_seg_lda[sg]);
+        max_ldc = std::max(max_ldc, tl.h_seg_ldc[sg]);
+    }
+    tl.ainv_halves = tl.h_seg_aoff[S];
     std::vector<int32_t> agg_ptr(A + 1, 0), agg_nodes;
     agg_nodes.reserve(N);
     for (int a = 0; a < A; ++a) {
@@ -2398,15 +2465,24 @@ static void twolevel_build(tf_handle* h)
                     if (h->h_col[b] == nj) { col_blk[((size_t)g * L + i) * L + j] = b; break; }
             }
         }
-    // aggregate pairs with their Jacobian blocks in block order
+    // aggregate pairs with their Jacobian blocks in block order (the pairs of a tissue's aggregates:
+    // M couples no two tissues), numbered in increasing (a, b) order through a dense table per tissue
     const std::vector<int32_t>& rp = h->h_row_ptr;
     const std::vector<int32_t>& cl = h->h_col;
+    std::vector<int64_t> tab_off(S + 1, 0);
+    for (int sg = 0; sg < S; ++sg) {
+        const int64_t As = seg_agg_ptr[sg + 1] - seg_agg_ptr[sg];
+        tab_off[sg + 1] = tab_off[sg] + As * As;
+    }
+    auto key_of = [&](int a, int b) {
+        const int sg = agg_seg[a], A0 = seg_agg_ptr[sg], As = seg_agg_ptr[sg + 1] - A0;
+        return tab_off[sg] + (int64_t)(a - A0) * As + (b - A0);
+    };
     std::vector<int64_t> pkey;
     pkey.reserve(h->nb);
     for (int r = 0; r < N; ++r)
-        for (int j = rp[r]; j < rp[r + 1]; ++j) pkey.push_back((int64_t)agg[r] * A + agg[cl[j]]);
-    // pairs numbered in increasing key order (a dense table over the A x A keys)
-    std::vector<int32_t> pid((size_t)A * A, -1);
+        for (int j = rp[r]; j < rp[r + 1]; ++j) pkey.push_back(key_of(agg[r], agg[cl[j]]));
+    std::vector<int32_t> pid(tab_off[S], -1);
     for (int64_t k : pkey) pid[k] = 0;
     int P = 0;
     for (int32_t& v : pid)
@@ -2417,11 +2493,16 @@ static void twolevel_build(tf_handle* h)
         bpair[j] = pid[pkey[j]];
         pair_ptr[bpair[j] + 1] += 1;
     }
-    for (int64_t k = 0; k < (int64_t)A * A; ++k)
-        if (pid[k] >= 0) {
-            pair_a[pid[k]] = (int32_t)(k / A);
-            pair_b[pid[k]] = (int32_t)(k % A);
+    for (int sg = 0; sg < S; ++sg) {
+        const int A0 = seg_agg_ptr[sg], As = seg_agg_ptr[sg + 1] - A0;
+        for (int64_t k = 0; k < (int64_t)As * As; ++k) {
+            const int32_t q = pid[tab_off[sg] + k];
+            if (q >= 0) {
+                pair_a[q] = A0 + (int32_t)(k / As);
+                pair_b[q] = A0 + (int32_t)(k % As);
+            }
         }
+    }
     for (int q = 0; q < P; ++q) pair_ptr[q + 1] += pair_ptr[q];
     {
         std::vector<int32_t> fill(pair_ptr.begin(), pair_ptr.end() - 1);
@@ -2488,27 +2569,41 @@ static void twolevel_build(tf_handle* h)
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
             return prow[agg_ptr[x + 1]] - prow[agg_ptr[x]] > prow[agg_ptr[y + 1]] - prow[agg_ptr[y]];
         });
-        tl_upload(h, tl.agg_order, order);
+        std::vector<int2> order2(A);
+        for (int k = 0; k < A; ++k) order2[k] = int2{order[k], agg_seg[order[k]]};
+        tl_upload(h, tl.agg_order, order2);
         tl_upload(h, tl.pcoef_p, pcoef_p);
-        std::vector<int32_t> sp = {0, A};
-        tl_upload(h, tl.tp_seg_ptr, sp);
+        tl_upload(h, tl.tp_seg_ptr, seg_agg_ptr);
         tl.tpart = tl_alloc<double>((size_t)TF_RED_SLOTS * A);
-        tl.ptq = tl_alloc<double>(nc);
-        tl.rcx = tl_alloc<double>(2 * (size_t)nc);
+        // (a tissue without coarse unknowns reads one clamped entry at its C0, which may be nc)
+        tl.ptq = tl_alloc<double>(nc + 8);
+        tl.rcx = tl_alloc<double>(2 * (size_t)nc + 8);
+        TF_HIP(hipMemsetAsync(tl.ptq, 0, (nc + 8) * sizeof(double), h->stream));
+        TF_HIP(hipMemsetAsync(tl.rcx, 0, (2 * (size_t)nc + 8) * sizeof(double), h->stream));
         TF_HIP(hipStreamSynchronize(h->stream));  // the host vectors above are temporaries
     }
     tl.dcinv = tl_alloc<float>((size_t)ncol * 9 * L * L);
     tl.L = L;
     tl.ncol = ncol;
-    tl.ldc = (nc + SW_B - 1) / SW_B * SW_B;  // the sweep's padded size
-    tl.Ac = tl_alloc<double>((size_t)tl.ldc * tl.ldc);
+    tl.ldc = max_ldc;  // the sweep's padded size (the largest block)
+    tl.Ac = tl_alloc<double>((size_t)tl.h_seg_acoff[S]);
     tl.swP = tl_alloc<double>((size_t)tl.ldc * SW_B);
     tl.swO = tl_alloc<double>((size_t)tl.ldc * SW_B);
     tl.swD = tl_alloc<double>(2 * (size_t)SW_B * SW_B);
-    tl.ldA = (nc + 7) / 8 * 8;
-    tl.Ainv = tl_alloc<__half>((size_t)nc * tl.ldA);
-    tl.ascale = tl_alloc<double>(2);
-    tl.info = tl_alloc<int32_t>(1);
+    tl.ldA = tl.h_seg_lda[0];
+    tl.Ainv = tl_alloc<__half>((size_t)tl.ainv_halves);
+    TF_HIP(hipMemsetAsync(tl.Ainv, 0, (size_t)tl.ainv_halves * sizeof(__half), h->stream));
+    tl.ascale = tl_alloc<double>(2 * (size_t)S);
+    TF_HIP(hipMemsetAsync(tl.ascale, 0, 2 * (size_t)S * sizeof(double), h->stream));
+    tl.info = tl_alloc<int32_t>(S);
+    tl_upload(h, tl.agg_seg, agg_seg);
+    tl_upload(h, tl.seg_c0, tl.h_seg_c0);
+    tl_upload(h, tl.seg_ldc, tl.h_seg_ldc);
+    tl_upload(h, tl.seg_acoff, tl.h_seg_acoff);
+    tl_upload(h, tl.seg_coarse, seg_coarse);
+    tl.refresh = tl_alloc<uint8_t>(S);
+    tl.base_it1.assign(S, -1);
+    tl.stale.assign(S, 0);
     TF_HIP(hipStreamSynchronize(h->stream));  // the host vectors above are temporaries
     tl.n_agg = A;
     tl.nc = nc;
@@ -2520,10 +2615,10 @@ static void twolevel_build(tf_handle* h)
 static TLArgs tl_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    return TLArgs{tl.pcoef, tl.coff, tl.krank};
+    return TLArgs{tl.pcoef, tl.coff, tl.krank, tl.agg_seg, tl.seg_c0, tl.seg_ldc, tl.seg_acoff, tl.refresh};
 }
 
-// the aggregates as the tissue's partial-sum units (one tissue per two-level handle)
+// the aggregates as the tissues' partial-sum units
 static TileArgs tp_tile_args(tf_handle* h)
 {
     TileArgs ta = tile_args(h);
@@ -2539,17 +2634,15 @@ static TPArgs tp_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
     return TPArgs{tl.agg_ptr, tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_ptr, tl.pass_rows, tl.first_pass, tl.agg_order,
-                  tl.pcoef_p,
-                  tl.coff, tl.krank,
-                  tl.dcinv, reinterpret_cast<const __half2*>(tl.Ainv), tl.ascale, tl.ptq, tl.rcx, tl.ldA, tl.nc,
-                  tl.n_agg};
+                  tl.pcoef_p, tl.coff, tl.krank, tl.dcinv, reinterpret_cast<const __half2*>(tl.Ainv), tl.seg_coarse,
+                  tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg};
 }
 
 template <int P, bool INIT>
 static void launch_tp_z(tf_handle* h, const TileArgs& ta, const TPArgs& tp, const double* p, const double* q)
 {
     const dim3 grid(tp.n_agg), blk(WG);
-    const size_t lds = tp_z_lds(tp.ldA);
+    const size_t lds = tp_z_lds(h->tl.max_ldA);
     switch (h->tl.L) {
     case 1: hipLaunchKernelGGL((k_tp_z<P, INIT, 1>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
     case 2: hipLaunchKernelGGL((k_tp_z<P, INIT, 2>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
@@ -2574,40 +2667,58 @@ static void launch_tl_dcol(tf_handle* h)
 }
 
 // Ac = P^T M P from the FP64 Jacobian and its inverse by the blocked Gauss-Jordan sweep (FP64, on
-// the handle's stream, 2 launches per 64-wide pivot block), then the scaled FP16 copy.  A matrix
-// that is not positive definite (a pivot block with a pivot that is not positive and finite) leaves
-// the two-level path off until the next refresh.
-static void twolevel_refresh(tf_handle* h)
+// the handle's stream, 2 launches per 64-wide pivot block), then the scaled FP16 copy; for every
+// tissue with which[s] set, block by block (the other tissues keep theirs).  A block that is not
+// positive definite (a pivot block with a pivot that is not positive and finite) leaves the
+// two-level path off until the next refresh.
+static void twolevel_refresh(tf_handle* h, const std::vector<uint8_t>& which)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    const int nc = tl.nc, ldc = tl.ldc, nt = ldc / SW_B;
-    const size_t n2 = (size_t)ldc * ldc;
-    hipLaunchKernelGGL(k_sw_clear, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, nc, ldc);
-    hipLaunchKernelGGL(k_tl_ac, dim3(tl.n_pairs), dim3(WG), 0, h->stream, tl_args(h), tl.pair_a, tl.pair_b, tl.pair_ptr,
-                       tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tl.Ac, ldc);
-    hipLaunchKernelGGL(k_sw_first, dim3(1), dim3(256), 0, h->stream, tl.Ac, ldc, tl.swD, tl.info);
-    for (int K = 0; K < nt; ++K) {
-        const double* D = tl.swD + (size_t)(K & 1) * SW_B * SW_B;
-        double* Dn = tl.swD + (size_t)((K + 1) & 1) * SW_B * SW_B;
-        hipLaunchKernelGGL(k_sw_panel, dim3(nt), dim3(256), 0, h->stream, tl.Ac, ldc, K, D, tl.swP, tl.swO);
-        hipLaunchKernelGGL(k_sw_update, dim3(nt * (nt + 1) / 2), dim3(256), 0, h->stream, tl.Ac, ldc, K, nt, tl.swP, tl.swO,
-                           D, Dn, tl.info);
+    const int S = h->S;
+    TF_HIP(hipMemcpyAsync(tl.refresh, which.data(), S, hipMemcpyHostToDevice, h->stream));
+    for (int sg = 0; sg < S; ++sg) {
+        const int ldc = tl.h_seg_ldc[sg], ncs = tl.h_seg_c0[sg + 1] - tl.h_seg_c0[sg];
+        if (!which[sg] || ncs == 0) continue;
+        const size_t n2 = (size_t)ldc * ldc;
+        hipLaunchKernelGGL(k_sw_clear, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream,
+                           tl.Ac + tl.h_seg_acoff[sg], ncs, ldc);
     }
-    // the FP16 copy is enqueued before the failure flag is read back: the device has work while the
-    // host waits, and the copy is simply not used after a failure
-    hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, tl.Ac, nc, ldc, tl.ascale);
-    const size_t n2h = (size_t)nc * tl.ldA;
-    hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, tl.Ac, ldc,
-                       (__half*)tl.Ainv, nc, tl.ldA, tl.ascale);
+    hipLaunchKernelGGL(k_tl_ac, dim3(tl.n_pairs), dim3(WG), 0, h->stream, tl_args(h), tl.pair_a, tl.pair_b, tl.pair_ptr,
+                       tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tl.Ac);
+    std::vector<int32_t> bad(S, 0);
+    for (int sg = 0; sg < S; ++sg) {
+        const int ldc = tl.h_seg_ldc[sg], ncs = tl.h_seg_c0[sg + 1] - tl.h_seg_c0[sg], nt = ldc / SW_B;
+        if (!which[sg] || ncs == 0) continue;
+        double* As = tl.Ac + tl.h_seg_acoff[sg];
+        hipLaunchKernelGGL(k_sw_first, dim3(1), dim3(256), 0, h->stream, As, ldc, tl.swD, tl.info + sg);
+        for (int K = 0; K < nt; ++K) {
+            const double* D = tl.swD + (size_t)(K & 1) * SW_B * SW_B;
+            double* Dn = tl.swD + (size_t)((K + 1) & 1) * SW_B * SW_B;
+            hipLaunchKernelGGL(k_sw_panel, dim3(nt), dim3(256), 0, h->stream, As, ldc, K, D, tl.swP, tl.swO);
+            hipLaunchKernelGGL(k_sw_update, dim3(nt * (nt + 1) / 2), dim3(256), 0, h->stream, As, ldc, K, nt, tl.swP,
+                               tl.swO, D, Dn, tl.info + sg);
+        }
+        // the FP16 copy is enqueued before the failure flags are read back: the device has work
+        // while the host waits, and the copy is simply not used after a failure
+        hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, As, ncs, ldc, tl.ascale + 2 * sg);
+        const size_t n2h = (size_t)ncs * tl.h_seg_lda[sg];
+        hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, As, ldc,
+                           (__half*)tl.Ainv + tl.h_seg_aoff[sg], ncs, tl.h_seg_lda[sg], tl.ascale + 2 * sg);
+        TF_HIP(hipGetLastError());
+        TF_HIP(hipMemcpyAsync(bad.data() + sg, tl.info + sg, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    }
     TF_HIP(hipGetLastError());
-    int32_t bad = 0;
-    TF_HIP(hipMemcpyAsync(&bad, tl.info, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipStreamSynchronize(h->stream));
+    bool failed = false;
+    for (int sg = 0; sg < S; ++sg) {
+        if (!which[sg]) continue;
+        failed = failed || bad[sg] != 0;
+        tl.base_it1[sg] = -1;
+        tl.stale[sg] = 0;
+    }
     tl.fresh = true;
     tl.age = 0;
-    tl.base_it1 = -1;
-    tl.stale = false;
-    tl.failed = bad != 0;
+    tl.failed = failed;
 }
 
 // The coarse operator P^T M P of the current Jacobian (assembled again into a temporary) and the FP64
@@ -2615,8 +2726,8 @@ static void twolevel_refresh(tf_handle* h)
 int twolevel_export(tf_handle* h, int32_t* nc_out, double* Ac_out, double* Ainv_out)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    if (!tl.built || !tl.fresh || tl.failed) return -1;
-    const int nc = tl.nc, ldc = tl.ldc;
+    if (!tl.built || !tl.fresh || tl.failed || h->S != 1) return -1;
+    const int nc = tl.nc, ldc = tl.h_seg_ldc[0];
     if (nc_out) *nc_out = nc;
     const size_t n2 = (size_t)ldc * ldc;
     if (Ainv_out) {
@@ -2630,8 +2741,10 @@ int twolevel_export(tf_handle* h, int32_t* nc_out, double* Ac_out, double* Ainv_
     if (Ac_out) {
         double* tmp = tl_alloc<double>(n2);
         hipLaunchKernelGGL(k_sw_clear, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, h->stream, tmp, nc, ldc);
+        const uint8_t one = 1;
+        TF_HIP(hipMemcpyAsync(tl.refresh, &one, 1, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_tl_ac, dim3(tl.n_pairs), dim3(WG), 0, h->stream, tl_args(h), tl.pair_a, tl.pair_b,
-                           tl.pair_ptr, tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tmp, ldc);
+                           tl.pair_ptr, tl.pair_blk, h->blk_row, h->col, h->val, h->nb, tmp);
         std::vector<double> A(n2);
         TF_HIP(hipMemcpyAsync(A.data(), tmp, n2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         TF_HIP(hipStreamSynchronize(h->stream));
@@ -3003,7 +3116,7 @@ bool twolevel_active(const tf_handle* h)
     const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
     const int pc = h->opts.preconditioner;
     const bool want = pc == TF_PRECOND_TWO_LEVEL || (pc == TF_PRECOND_AUTO && h->N >= TF_TWO_LEVEL_MIN_NODES);
-    return cg_kind && want && h->S == 1;
+    return cg_kind && want;
 }
 
 // The coarse space follows the pattern and tiling.  The coarse operator is rebuilt at the first
@@ -3017,9 +3130,13 @@ static bool twolevel_prepare(tf_handle* h)
     tf_handle::TwoLevel& tl = h->tl;
     if (!tl.built) twolevel_build(h);
     if (h->krylov_call == 0) tl.age += 1;
-    if (!tl.fresh || h->krylov_call < 0 ||
-        (h->krylov_call == 0 && (tl.stale || tl.age >= std::max(1, h->opts.coarse_refresh))))
-        twolevel_refresh(h);
+    const bool all = !tl.fresh || h->krylov_call < 0 || (h->krylov_call == 0 && tl.age >= std::max(1, h->opts.coarse_refresh));
+    std::vector<uint8_t> which(h->S, all ? 1 : 0);
+    bool any = all;
+    if (!all && h->krylov_call == 0)
+        for (int sg = 0; sg < h->S; ++sg)
+            if (tl.stale[sg]) { which[sg] = 1; any = true; }
+    if (any) twolevel_refresh(h, which);
     if (!tl.failed) {
         launch_tl_dcol(h);
         // the values in aggregate order for the solve
@@ -3037,13 +3154,17 @@ static bool twolevel_prepare(tf_handle* h)
     return !tl.failed;
 }
 
-// After a two-level solve of Newton iteration 1: the rebuild's baseline, or the staleness test
-static void twolevel_observe(tf_handle* h, int iters)
+// After a two-level solve of Newton iteration 1, for every tissue: the rebuild's baseline, or the
+// staleness test (a stale tissue's block is rebuilt at the next timestep's first solve)
+static void twolevel_observe(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
     if (h->krylov_call != 1) return;
-    if (tl.base_it1 < 0) tl.base_it1 = iters;
-    else if (iters > 1.25 * tl.base_it1) tl.stale = true;
+    for (int sg = 0; sg < h->S; ++sg) {
+        const int iters = h->h_seg[sg].iters;
+        if (tl.base_it1[sg] < 0) tl.base_it1[sg] = iters;
+        else if (iters > 1.25 * tl.base_it1[sg]) tl.stale[sg] = 1;
+    }
 }
 
 // Average device time (ms) of one Krylov iteration over n_iters iterations on the current
@@ -3207,7 +3328,7 @@ int krylov_solve(tf_handle* h)
     }
     int st = run(kind);
     bool kx_in_positions = kind == KIND_TL;
-    if (kind == KIND_TL && st == TF_OK) twolevel_observe(h, h->last_krylov_iters);
+    if (kind == KIND_TL && st == TF_OK) twolevel_observe(h);
     if (kind == KIND_TL && st != TF_OK) h->tl.fresh = false;  // the next solve rebuilds the coarse operator
     bool unusable = false;  // a stalled tissue whose residual is above 1e-2 of its right-hand side
     for (int s = 0; s < h->S; ++s)
