@@ -62,11 +62,12 @@ typedef struct tf_mesh_desc {
 /* TF_KRYLOV_CG1: CG with one fused kernel per iteration (single-reduction recurrences of
  * Chronopoulos and Gear); TF_KRYLOV_CG: textbook PCG, two kernels per iteration;
  * TF_KRYLOV_PIPECG: pipelined CG (Ghysels and Vanroose), one kernel per iteration whose SpMV
- * operand was completed by the previous kernel. */
-enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2, TF_KRYLOV_CG1 = 3, TF_KRYLOV_PIPECG = 4 };
+ * operand was completed by the previous kernel; TF_KRYLOV_BICGSTAB for the non-symmetric
+ * Jacobian of internal viscosity (value 2 is not used). */
+enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_CG1 = 3, TF_KRYLOV_PIPECG = 4 };
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
-/* TF_PRECOND_AUTO (the default): two-level for a handle of one tissue with 4096 nodes or more,
- * else block Jacobi.  TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M.
+/* TF_PRECOND_AUTO (the default): two-level for a handle with 4096 nodes or more, else block
+ * Jacobi.  TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M.
  * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi over the node columns
  * (the diagonal blocks of M over the nodes that prisms' vertical edges join; 3x3 nodal blocks
  * when the columns are not uniform or column_blocks = 0) plus an additive coarse correction
@@ -84,7 +85,6 @@ typedef struct tf_solver_opts {
     int32_t krylov_max_iter;   /* per linear solve                                             */
     double krylov_rtol;        /* ||r|| <= max(rtol*||g||, atol)                               */
     double krylov_atol;
-    int32_t gmres_restart;     /* restart length for TF_KRYLOV_GMRES                           */
     int32_t iters_per_sync;    /* Krylov iterations between host convergence checks            */
     double nr_tol;             /* ||dU||_2 <= nr_tol (absolute, per tissue) -- reference 1e-8  */
     int32_t nr_max_iter;       /* reference: 20 (stops when the count exceeds it)              */

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libtfnr.so")
 TF_OK, TF_NOT_CONVERGED, TF_KRYLOV_STALL, TF_NEG_JACOBIAN, TF_NAN, TF_OOM, TF_HIP_ERROR, TF_INVALID = range(8)
 STATUS_NAMES = {0: "TF_OK", 1: "TF_NOT_CONVERGED", 2: "TF_KRYLOV_STALL", 3: "TF_NEG_JACOBIAN", 4: "TF_NAN",
                 5: "TF_OOM", 6: "TF_HIP_ERROR", 7: "TF_INVALID"}
-KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_GMRES, KRYLOV_CG1, KRYLOV_PIPECG = -1, 0, 1, 2, 3, 4
+KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_CG, KRYLOV_CG1, KRYLOV_PIPECG = -1, 0, 1, 3, 4
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 PRECOND_AUTO, PRECOND_BLOCK_JACOBI, PRECOND_TWO_LEVEL = -1, 0, 1
 
@@ -51,7 +51,7 @@ class MeshDesc(C.Structure):
 
 class SolverOpts(C.Structure):
     _fields_ = [("krylov", C.c_int32), ("krylov_max_iter", C.c_int32), ("krylov_rtol", C.c_double),
-                ("krylov_atol", C.c_double), ("gmres_restart", C.c_int32), ("iters_per_sync", C.c_int32),
+                ("krylov_atol", C.c_double), ("iters_per_sync", C.c_int32),
                 ("nr_tol", C.c_double), ("nr_max_iter", C.c_int32), ("tangent_precision", C.c_int32),
                 ("matrix_precision", C.c_int32), ("growth_rotation", C.c_int32), ("use_graph", C.c_int32),
                 ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32),

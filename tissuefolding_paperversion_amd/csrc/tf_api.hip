@@ -485,7 +485,6 @@ void tf_default_options(tf_solver_opts* o)
     o->krylov_max_iter = 20000;
     o->krylov_rtol = 1e-12;
     o->krylov_atol = 0.0;
-    o->gmres_restart = 50;
     o->iters_per_sync = 32;
     o->nr_tol = 1e-8;
     o->nr_max_iter = 20;
@@ -693,7 +692,7 @@ void tf_destroy(tf_handle* h)
 int tf_set_options(tf_handle* h, const tf_solver_opts* o)
 {
     if (!h || !o) return TF_INVALID;
-    if (o->krylov < -1 || o->krylov > TF_KRYLOV_PIPECG || o->krylov_max_iter <= 0 || o->nr_max_iter < 0 ||
+    if (o->krylov < -1 || o->krylov == 2 || o->krylov > TF_KRYLOV_PIPECG || o->krylov_max_iter <= 0 || o->nr_max_iter < 0 ||
         !(o->krylov_rtol >= 0) || !(o->nr_tol >= 0) ||
         (o->matrix_precision != TF_PREC_FP64 && o->matrix_precision != TF_PREC_FP32) ||
         o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16 ||
@@ -701,7 +700,6 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
         o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1))
         return fail(h, TF_INVALID, "invalid solver options");
-    if (o->krylov == TF_KRYLOV_GMRES) return fail(h, TF_INVALID, "GMRES is not available in this build");
     const bool coarse_changed =
         o->coarse_aggregates != h->opts.coarse_aggregates || o->column_blocks != h->opts.column_blocks;
     h->opts = *o;

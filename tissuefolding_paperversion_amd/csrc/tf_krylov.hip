@@ -3119,18 +3119,21 @@ bool twolevel_active(const tf_handle* h)
     return cg_kind && want;
 }
 
-// The coarse space follows the pattern and tiling.  The coarse operator is rebuilt at the first
-// Newton iteration of a timestep once the Krylov iterations of Newton iteration 1 (the largest
-// solve) exceed 1.25x their count in the timestep of the last rebuild, or coarse_refresh
-// timesteps after it; and for every solve outside the Newton loop.  A coarse operator from an
+// The coarse space follows the pattern and tiling.  A tissue's block of the coarse operator is
+// rebuilt at the first Newton iteration of a timestep once the tissue's Krylov iterations of
+// Newton iteration 1 (the largest solve) exceed 1.25x their count in the timestep of its last
+// rebuild; every block coarse_refresh timesteps after the last full rebuild, and for every solve
+// outside the Newton loop (timing runs keep the operator in place).  A coarse operator from an
 // earlier Jacobian is still symmetric positive definite, so it changes the iteration count, not
 // the solution.  False when the coarse operator is not positive definite (block Jacobi then).
-static bool twolevel_prepare(tf_handle* h)
+static bool twolevel_prepare(tf_handle* h, bool reuse = false)
 {
     tf_handle::TwoLevel& tl = h->tl;
     if (!tl.built) twolevel_build(h);
     if (h->krylov_call == 0) tl.age += 1;
-    const bool all = !tl.fresh || h->krylov_call < 0 || (h->krylov_call == 0 && tl.age >= std::max(1, h->opts.coarse_refresh));
+    // reuse (timing runs): the coarse operator in place, when there is one
+    const bool all = !tl.fresh || (h->krylov_call < 0 && !reuse) ||
+                     (h->krylov_call == 0 && tl.age >= std::max(1, h->opts.coarse_refresh));
     std::vector<uint8_t> which(h->S, all ? 1 : 0);
     bool any = all;
     if (!all && h->krylov_call == 0)
@@ -3173,7 +3176,7 @@ static void twolevel_observe(tf_handle* h)
 double krylov_time_iteration(tf_handle* h, int n_iters)
 {
     const tf_solver_opts saved = h->opts;
-    const int kind = twolevel_active(h) && twolevel_prepare(h) ? KIND_TL : TF_KRYLOV_CG1;
+    const int kind = twolevel_active(h) && twolevel_prepare(h, true) ? KIND_TL : TF_KRYLOV_CG1;
     h->opts.krylov_rtol = 0.0;
     h->opts.krylov_atol = 0.0;
     h->opts.krylov_max_iter = n_iters - 1;  // launch n_iters sees the cap
@@ -3237,7 +3240,7 @@ double krylov_probe(tf_handle* h, int n_launches, int which)
     hipLaunchKernelGGL(k_seg_reopen, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     TF_HIP(hipGetLastError());
     const tf_solver_opts saved = h->opts;
-    const int sk = tl && twolevel_prepare(h) ? KIND_TL : TF_KRYLOV_CG;
+    const int sk = tl && twolevel_prepare(h, true) ? KIND_TL : TF_KRYLOV_CG;
     h->opts.krylov_rtol = 0.0;
     h->opts.krylov_atol = 0.0;
     h->opts.krylov_max_iter = 8;  // the ninth iteration sees the cap: the solve stops there
