@@ -428,6 +428,18 @@ def main():
                                  "nr_iterations_per_step": [float(np.mean([st["nr_iterations"] for st in bstats])),
                                                             float(np.mean([st["nr_iterations"] for st in mstats]))]}
         out["mixed"] = mixed
+    if not a.no_mixed and two_level:
+        # the plain FP64 run with the preconditioner's column-block and coarse inverses in FP64 too
+        # (every array of the solve FP64), beside the default FP32/FP16 preconditioner storage
+        sf, fstats, fsec, x_f = run_timed(NewtonRaphsonSolver, tissue, dict(opts, precond_fp64=1), a.warmup, a.steps, 0)
+        out["fp64_preconditioner_storage"] = {
+            "value": a.steps / fsec, "ms_per_step": 1e3 * fsec / a.steps,
+            "relative_to_default_storage": (a.steps / fsec) / (a.steps / sec),
+            "disp_err_rel_vs_default_storage": float(np.max(np.abs(x_f - x_plain)) / np.max(np.abs(x_plain))),
+            "krylov_iterations_by_newton_iteration": [float(np.mean([st["krylov_by_newton"][k] for st in fstats]))
+                                                      for k in range(3)],
+            "coarse_kernel_algorithmic_bytes": sf.kernel_bytes("coarse")}
+        sf.close()
     if not a.no_adjoint:
         from tissuefolding_paperversion_amd.adjoint import config5_check
         adj = config5_check(n_fd=a.adjoint_fd)

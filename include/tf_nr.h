@@ -77,7 +77,8 @@ enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
  * rebuild, or coarse_refresh timesteps after it, and for every solve outside the Newton loop;
  * handles with several tissues use block Jacobi.  The column-block inverses are held in FP32 and
  * the coarse inverse in FP16 scaled by its largest diagonal entry (they only precondition:
- * residuals, matrix and Krylov vectors stay in the solver's precision). */
+ * residuals, matrix and Krylov vectors stay in the solver's precision; precond_fp64 holds both
+ * in FP64). */
 enum { TF_PRECOND_AUTO = -1, TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
 
 typedef struct tf_solver_opts {
@@ -111,6 +112,10 @@ typedef struct tf_solver_opts {
                                   (nodes joined by prisms' vertical edges) when every column has
                                   the same 1-4 nodes, else 3x3 nodal blocks (default); 0 always
                                   3x3 nodal blocks                                              */
+    int32_t precond_fp64;      /* TF_PRECOND_TWO_LEVEL: 1 holds the column-block inverses and the
+                                  coarse inverse in FP64 (every array of the solve in FP64);
+                                  0 (default): FP32 column blocks, FP16 coarse inverse scaled by
+                                  its largest diagonal entry                                    */
     int32_t mixed_fp64_from;   /* with tangent_precision or matrix_precision below FP64: Newton
                                   iterations from this index on assemble their tangents and solve
                                   with FP64 Jacobian values (the early, large updates run in the

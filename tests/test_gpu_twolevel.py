@@ -179,6 +179,27 @@ def test_several_tissues_match_their_own_handles(oracle):
     assert rel_err(xb[:offs[1]], o.pos) < 1e-8
 
 
+def test_fp64_preconditioner_storage(oracle):
+    """precond_fp64 holds the column-block inverses and the coarse inverse in FP64, so every array
+    of the solve is FP64: the same Newton iterates as the default FP32/FP16 storage (both within
+    1e-8 of the oracle, same Newton counts) at a similar Krylov count."""
+    t = _ellipse()
+    a, b = _solver(t, **TL), _solver(t, precond_fp64=1, **TL)
+    o = oracle.OracleTissue(t)
+    ka = kb = 0
+    for _ in range(3):
+        sa, sb = a.runOneStep(t.dt), b.runOneStep(t.dt)
+        it_o, _ = o.timestep(t.dt)
+        assert sa["two_level"] == 1 and sb["two_level"] == 1
+        assert sa["nr_iterations"] == sb["nr_iterations"] == it_o
+        ka += sa["krylov_iterations"]
+        kb += sb["krylov_iterations"]
+    print(f"Krylov iterations over 3 steps: FP32/FP16 storage {ka}, FP64 storage {kb}", flush=True)
+    assert kb <= 1.2 * ka
+    assert rel_err(a.positions, o.pos) < 1e-8 and rel_err(b.positions, o.pos) < 1e-8
+    assert b.kernel_bytes("coarse") > 1.5 * a.kernel_bytes("coarse")  # 8-byte inverses instead of 2 and 4
+
+
 def test_option_validation():
     t = configs.disc_tissue(radius=10.0, n_steps=1)
     s = _solver(t)
@@ -188,6 +209,10 @@ def test_option_validation():
         s.set_options(coarse_aggregates=-1)
     with pytest.raises(L.TfError):
         s.set_options(column_blocks=2)
+    with pytest.raises(L.TfError):
+        s.set_options(precond_fp64=2)
+    with pytest.raises(L.TfError):
+        s.set_options(krylov=2)  # the unused solver kind
 
 
 @pytest.mark.parametrize("make", [_ellipse, configs.config2], ids=["ellipse", "config2"])

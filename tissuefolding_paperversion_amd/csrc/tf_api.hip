@@ -698,10 +698,11 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         o->tangent_precision < TF_PREC_FP64 || o->tangent_precision > TF_PREC_BF16 ||
         (o->implicit_packing != 0 && o->implicit_packing != 1) ||
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
-        o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1))
+        o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1) ||
+        (o->precond_fp64 != 0 && o->precond_fp64 != 1))
         return fail(h, TF_INVALID, "invalid solver options");
-    const bool coarse_changed =
-        o->coarse_aggregates != h->opts.coarse_aggregates || o->column_blocks != h->opts.column_blocks;
+    const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates ||
+                                o->column_blocks != h->opts.column_blocks || o->precond_fp64 != h->opts.precond_fp64;
     h->opts = *o;
     drop_krylov_graphs(h);
     TF_GUARD_BEGIN
@@ -1935,8 +1936,8 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
         // and P^T q read, P^T r written
         if (!twolevel_active(h) || !h->tl.built) { *bytes = 0.0; return TF_OK; }
         const tf_handle::TwoLevel& tl = h->tl;
-        const double D = 3.0 * tl.L;
-        *bytes = (double)tl.ainv_halves * 2.0 + (double)tl.ncol * D * D * 4.0 + N * 18.0 * 8.0 + n * 8.0 * 7.0 +
+        const double D = 3.0 * tl.L, eb = h->opts.precond_fp64 ? 8.0 : 2.0, db = h->opts.precond_fp64 ? 8.0 : 4.0;
+        *bytes = (double)tl.ainv_halves * eb + (double)tl.ncol * D * D * db + N * 18.0 * 8.0 + n * 8.0 * 7.0 +
                  tl.nc * 8.0 * 3.0;
     } else if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver

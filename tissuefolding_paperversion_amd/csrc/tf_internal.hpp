@@ -320,6 +320,9 @@ struct tf_handle {
         int32_t L = 1, ncol = 0;                            // nodes per column, columns
         int32_t* col_blk = nullptr;                         // [ncol][L][L] BSR block of every node pair or -1
         float* dcinv = nullptr;                             // [ncol][3L][3L] FP32 inverse column blocks
+        double* dcinv64 = nullptr;                          //   the same in FP64 (precond_fp64)
+        double* Ainv64 = nullptr;                           // per tissue [nc_s][ldA_s] FP64 Ac^-1 (precond_fp64)
+        int64_t* seg_aoff64 = nullptr;                      // [S] offset of the tissue's FP64 block
         int32_t* info = nullptr;                            // [S] the sweep's failure flag of every tissue
         // the solve in aggregate order (position r = node agg_nodes[r])
         int32_t *prow_ptr = nullptr, *pcol = nullptr, *pblk = nullptr;  // [N+1], [nb], [nb] permuted Jacobian pattern

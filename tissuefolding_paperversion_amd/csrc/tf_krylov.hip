@@ -24,6 +24,7 @@
 #include <cmath>
 #include <math.h>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -548,10 +549,10 @@ struct TLArgs {
 // per block row, Gauss-Jordan elimination without pivoting (the blocks of a positive definite M
 // are positive definite; the pivot rows come by lane shuffles).  A column with a pivot that is not
 // positive and finite gets its nodes' 3 x 3 inverses (dinv) instead.
-template <int L>
+template <int L, typename OT>
 __global__ __launch_bounds__(WG) void k_tl_dcol(const int32_t* __restrict__ col_blk, const int32_t* __restrict__ agg_nodes,
                                                const double* __restrict__ val, int nb, const double* __restrict__ dinv,
-                                               int ncol, float* __restrict__ dcinv)
+                                               int ncol, OT* __restrict__ dcinv)
 {
     constexpr int D = 3 * L, CPW = 64 / D;
     const int lane = threadIdx.x % 64, wave = (int)(blockIdx.x * (WG / 64) + threadIdx.x / 64);
@@ -593,19 +594,19 @@ __global__ __launch_bounds__(WG) void k_tl_dcol(const int32_t* __restrict__ col_
         }
     }
     if (!own) return;
-    float* out = dcinv + (size_t)g * D * D;
+    OT* out = dcinv + (size_t)g * D * D;
     if (ok) {
 #pragma unroll
         for (int j = 0; j < D; ++j)
             if (j >= li) {
-                const float e = (float)v[j];
+                const OT e = (OT)v[j];
                 out[li * D + j] = e;
                 out[j * D + li] = e;
             }
     } else {
         const int node = agg_nodes[(size_t)g * L + i];
 #pragma unroll
-        for (int j = 0; j < D; ++j) out[li * D + j] = j / 3 == i ? (float)dinv[9 * (size_t)node + 3 * ci + j % 3] : 0.0f;
+        for (int j = 0; j < D; ++j) out[li * D + j] = j / 3 == i ? (OT)dinv[9 * (size_t)node + 3 * ci + j % 3] : (OT)0;
     }
 }
 
@@ -640,13 +641,22 @@ struct TPArgs {
     const int32_t* coff;       // [A] first coarse unknown of every aggregate
     const int32_t* krank;      // [A] coarse unknowns of every aggregate
     const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
+    const double* dcinv64;     //   the same in FP64 (FP64 preconditioner storage)
     const __half2* Ainv;       // per tissue [nc_s][ldA_s / 2] Ac_s^-1 / scale_s in FP16 pairs
+    const double* Ainv64;      //   per tissue [nc_s][ldA_s] Ac_s^-1 in FP64 (FP64 preconditioner storage)
+    const int64_t* seg_aoff64; //   [S] offset of the tissue's FP64 block
     const int4* seg_coarse;    // [S] first coarse unknown, count nc_s, ldA_s, offset of the Ainv block (pairs)
     const double* ascale;      // [S][2] scale and its reciprocal
     double* ptq;               // [nc] P^T q of the current iteration
     double* rcx;               // [2][nc] P^T r (ping-pong by parity)
     int nc, n_agg;
 };
+template <bool F64>
+__device__ inline std::conditional_t<F64, double, float> dc_row(const TPArgs& tp, size_t i)
+{
+    if constexpr (F64) return tp.dcinv64[i];
+    else return tp.dcinv[i];
+}
 
 // q = M p over the aggregate's passes; own rows see p_new = z + beta p_old (beta = 0: p = z), and
 // the callback gets (position, component, q value, p value) for every own DoF (q = 0 without
@@ -872,7 +882,7 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
 // inverse), and z_c = Ac^-1 r_c.
 constexpr int TP_PAIRS = 8;  // column pairs per lane of every coarse row held in registers (nc <= 4096)
 
-template <int P, bool INIT, int L>
+template <int P, bool INIT, int L, bool F64 = false>
 __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const double* __restrict__ p,
                                              const double* __restrict__ q, double* __restrict__ x,
                                              double* __restrict__ r, double* __restrict__ z)
@@ -886,19 +896,22 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     const int c0 = tp.coff[a], kr = tp.krank[a];
     const int4 sc4 = tp.seg_coarse[sg];  // the tissue's coarse unknowns [C0, C0 + ncs) and Ainv block
     const int st0 = ta.seg_tile_ptr[sg], st1 = ta.seg_tile_ptr[sg + 1];
-    extern __shared__ float tp_rcs[];  // r_c, ldA entries (zero past nc)
+    extern __shared__ float tp_rcs[];  // r_c, ldA entries (zero past nc); FP64 storage: doubles
+    double* const rcd = reinterpret_cast<double*>(tp_rcs);
     __shared__ double rl[WG];
     __shared__ double zcs[6];
     const int tid = threadIdx.x, slot = tid / D, li = tid % D;
     const int nc = tp.nc, C0 = sc4.x, ncs = sc4.y, ldA = sc4.z, ld2 = ldA / 2, lc0 = c0 - C0;
     const int last = max(ncs, 1) - 1;  // a tissue without coarse unknowns has one row of zeros
     const __half2* Ab = tp.Ainv + sc4.w;
-    __half2 pre[6][TP_PAIRS];
+    __half2 pre[6][F64 ? 1 : TP_PAIRS];
+    if constexpr (!F64) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
+        for (int k = 0; k < 6; ++k)
 #pragma unroll
-        for (int m = 0; m < TP_PAIRS; ++m)
-            pre[k][m] = Ab[(size_t)min(lc0 + k, last) * ld2 + min(tid + WG * m, ld2 - 1)];
+            for (int m = 0; m < TP_PAIRS; ++m)
+                pre[k][m] = Ab[(size_t)min(lc0 + k, last) * ld2 + min(tid + WG * m, ld2 - 1)];
+    }
     const double* rcp = tp.rcx + (size_t)P * nc + C0;
     const double* ptq = tp.ptq + C0;
     double vx[NV], vq[NV];
@@ -914,9 +927,10 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     int d = 3 * pos + li % 3;
     double rv = r[d], xv = 0.0, pv = 0.0, qv = 0.0;
     if (!INIT) { xv = x[d]; pv = p[d]; qv = q[d]; }
-    float Dr[D];
+    using DT = std::conditional_t<F64, double, float>;
+    DT Dr[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) Dr[j] = tp.dcinv[((size_t)g * D + li) * D + j];
+    for (int j = 0; j < D; ++j) Dr[j] = dc_row<F64>(tp, ((size_t)g * D + li) * D + j);
     double Pr[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) Pr[j] = tp.pcoef[18 * (size_t)pos + 6 * (li % 3) + j];
@@ -936,7 +950,10 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
         const int i = tid + WG * m;
-        if (i < ldA) tp_rcs[i] = i < ncs ? (float)(vx[m] - alpha * vq[m]) : 0.f;
+        if (i < ldA) {
+            if constexpr (F64) rcd[i] = i < ncs ? vx[m] - alpha * vq[m] : 0.0;
+            else tp_rcs[i] = i < ncs ? (float)(vx[m] - alpha * vq[m]) : 0.f;
+        }
     }
     __syncthreads();
     {
@@ -944,13 +961,22 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             sv[k] = 0.0;
+            if constexpr (F64) {  // the FP64 rows streamed here, 16 bytes per lane and load
+                const double2* row = reinterpret_cast<const double2*>(tp.Ainv64 + tp.seg_aoff64[sg] +
+                                                                     (size_t)min(lc0 + k, last) * ldA);
+                for (int j = 2 * tid; j < ldA; j += 2 * WG) {
+                    const double2 f = row[j / 2];
+                    sv[k] += f.x * rcd[j] + f.y * rcd[j + 1];
+                }
+            } else {
 #pragma unroll
-            for (int m = 0; m < TP_PAIRS; ++m) {
-                const int j = 2 * (tid + WG * m);
-                if (j < ldA) {  // pairs past ldA hold clamped copies: skipped
-                    const float2 f = __half22float2(pre[k][m]);
-                    const float2 c2 = *reinterpret_cast<const float2*>(tp_rcs + j);
-                    sv[k] += (double)f.x * c2.x + (double)f.y * c2.y;
+                for (int m = 0; m < TP_PAIRS; ++m) {
+                    const int j = 2 * (tid + WG * m);
+                    if (j < ldA) {  // pairs past ldA hold clamped copies: skipped
+                        const float2 f = __half22float2(pre[k][m]);
+                        const float2 c2 = *reinterpret_cast<const float2*>(tp_rcs + j);
+                        sv[k] += (double)f.x * c2.x + (double)f.y * c2.y;
+                    }
                 }
             }
         }
@@ -966,7 +992,8 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
             double t = 0.0;
 #pragma unroll
             for (int k = 0; k < WG / 64; ++k) t += zc_red[tid][k];
-            zcs[tid] = tid < kr ? t * tp.ascale[2 * sg] : 0.0;  // P's columns past the rank are zero
+            // P's columns past the rank are zero; the FP16 copy is scaled
+            zcs[tid] = tid < kr ? (F64 ? t : t * tp.ascale[2 * sg]) : 0.0;
         }
         __syncthreads();
     }
@@ -1017,7 +1044,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     if (tid < kr) tp.rcx[(size_t)(1 - P) * nc + c0 + tid] = out[2 + tid];
 }
 
-static size_t tp_z_lds(int ldA) { return sizeof(float) * (size_t)ldA; }  // r_c
+static size_t tp_z_lds(int ldA, bool f64) { return (f64 ? sizeof(double) : sizeof(float)) * (size_t)ldA; }  // r_c
 
 // the solve's vectors between node order and aggregate order
 __global__ __launch_bounds__(256) void k_tp_gather(const int32_t* __restrict__ agg_nodes, int n, const double* __restrict__ src,
@@ -1404,6 +1431,16 @@ __global__ __launch_bounds__(256) void k_tl_symm(const double* __restrict__ A, i
     const int i = (int)(k / ld), j = (int)(k % ld);
     const double v = j < n ? -(i >= j ? A[(size_t)i * ldc + j] : A[(size_t)j * ldc + i]) * sc[1] : 0.0;
     Ainv[k] = __float2half((float)v);
+}
+
+// the same, unscaled in FP64 (FP64 preconditioner storage)
+__global__ __launch_bounds__(256) void k_tl_symm64(const double* __restrict__ A, int ldc, double* __restrict__ Ainv, int n,
+                                                  int ld)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)n * ld) return;
+    const int i = (int)(k / ld), j = (int)(k % ld);
+    Ainv[k] = j < n ? -(i >= j ? A[(size_t)i * ldc + j] : A[(size_t)j * ldc + i]) : 0.0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2208,6 +2245,9 @@ void tl_release(tf_handle* h)
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     if (tl.ascale) { (void)hipFree(tl.ascale); tl.ascale = nullptr; }
     if (tl.dcinv) { (void)hipFree(tl.dcinv); tl.dcinv = nullptr; }
+    if (tl.dcinv64) { (void)hipFree(tl.dcinv64); tl.dcinv64 = nullptr; }
+    if (tl.Ainv64) { (void)hipFree(tl.Ainv64); tl.Ainv64 = nullptr; }
+    if (tl.seg_aoff64) { (void)hipFree(tl.seg_aoff64); tl.seg_aoff64 = nullptr; }
     tl.built = tl.fresh = tl.failed = false;
 }
 
@@ -2582,7 +2622,9 @@ static void twolevel_build(tf_handle* h)
         TF_HIP(hipMemsetAsync(tl.rcx, 0, (2 * (size_t)nc + 8) * sizeof(double), h->stream));
         TF_HIP(hipStreamSynchronize(h->stream));  // the host vectors above are temporaries
     }
-    tl.dcinv = tl_alloc<float>((size_t)ncol * 9 * L * L);
+    const bool f64 = h->opts.precond_fp64 != 0;  // FP64 preconditioner storage
+    if (f64) tl.dcinv64 = tl_alloc<double>((size_t)ncol * 9 * L * L);
+    else tl.dcinv = tl_alloc<float>((size_t)ncol * 9 * L * L);
     tl.L = L;
     tl.ncol = ncol;
     tl.ldc = max_ldc;  // the sweep's padded size (the largest block)
@@ -2591,8 +2633,14 @@ static void twolevel_build(tf_handle* h)
     tl.swO = tl_alloc<double>((size_t)tl.ldc * SW_B);
     tl.swD = tl_alloc<double>(2 * (size_t)SW_B * SW_B);
     tl.ldA = tl.h_seg_lda[0];
-    tl.Ainv = tl_alloc<__half>((size_t)tl.ainv_halves);
-    TF_HIP(hipMemsetAsync(tl.Ainv, 0, (size_t)tl.ainv_halves * sizeof(__half), h->stream));
+    if (f64) {
+        tl.Ainv64 = tl_alloc<double>((size_t)tl.ainv_halves);
+        TF_HIP(hipMemsetAsync(tl.Ainv64, 0, (size_t)tl.ainv_halves * sizeof(double), h->stream));
+        tl_upload(h, tl.seg_aoff64, tl.h_seg_aoff);
+    } else {
+        tl.Ainv = tl_alloc<__half>((size_t)tl.ainv_halves);
+        TF_HIP(hipMemsetAsync(tl.Ainv, 0, (size_t)tl.ainv_halves * sizeof(__half), h->stream));
+    }
     tl.ascale = tl_alloc<double>(2 * (size_t)S);
     TF_HIP(hipMemsetAsync(tl.ascale, 0, 2 * (size_t)S * sizeof(double), h->stream));
     tl.info = tl_alloc<int32_t>(S);
@@ -2634,21 +2682,26 @@ static TPArgs tp_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
     return TPArgs{tl.agg_ptr, tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_ptr, tl.pass_rows, tl.first_pass, tl.agg_order,
-                  tl.pcoef_p, tl.coff, tl.krank, tl.dcinv, reinterpret_cast<const __half2*>(tl.Ainv), tl.seg_coarse,
-                  tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg};
+                  tl.pcoef_p, tl.coff, tl.krank, tl.dcinv, tl.dcinv64, reinterpret_cast<const __half2*>(tl.Ainv),
+                  tl.Ainv64, tl.seg_aoff64, tl.seg_coarse, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg};
 }
 
 template <int P, bool INIT>
 static void launch_tp_z(tf_handle* h, const TileArgs& ta, const TPArgs& tp, const double* p, const double* q)
 {
     const dim3 grid(tp.n_agg), blk(WG);
-    const size_t lds = tp_z_lds(h->tl.max_ldA);
+    const bool f64 = h->opts.precond_fp64 != 0;
+    const size_t lds = tp_z_lds(h->tl.max_ldA, f64);
+#define TF_TPZ(LL)                                                                                                        \
+    if (f64) hipLaunchKernelGGL((k_tp_z<P, INIT, LL, true>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); \
+    else hipLaunchKernelGGL((k_tp_z<P, INIT, LL, false>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks)
     switch (h->tl.L) {
-    case 1: hipLaunchKernelGGL((k_tp_z<P, INIT, 1>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
-    case 2: hipLaunchKernelGGL((k_tp_z<P, INIT, 2>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
-    case 3: hipLaunchKernelGGL((k_tp_z<P, INIT, 3>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
-    default: hipLaunchKernelGGL((k_tp_z<P, INIT, 4>), grid, blk, lds, h->stream, ta, tp, p, q, h->kx, h->kr, h->ks); break;
+    case 1: TF_TPZ(1); break;
+    case 2: TF_TPZ(2); break;
+    case 3: TF_TPZ(3); break;
+    default: TF_TPZ(4); break;
     }
+#undef TF_TPZ
 }
 
 // inverse column blocks of the current Jacobian (every solve)
@@ -2657,12 +2710,18 @@ static void launch_tl_dcol(tf_handle* h)
     tf_handle::TwoLevel& tl = h->tl;
     const int cpw = 64 / (3 * tl.L), waves = (tl.ncol + cpw - 1) / cpw;
     const dim3 grid((waves + WG / 64 - 1) / (WG / 64)), blk(WG);
+#define TF_DCOL(LL)                                                                                                        \
+    if (tl.dcinv64) hipLaunchKernelGGL((k_tl_dcol<LL, double>), grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, \
+                                       h->nb, h->dinv, tl.ncol, tl.dcinv64);                                           \
+    else hipLaunchKernelGGL((k_tl_dcol<LL, float>), grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb,   \
+                            h->dinv, tl.ncol, tl.dcinv)
     switch (tl.L) {
-    case 1: hipLaunchKernelGGL(k_tl_dcol<1>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
-    case 2: hipLaunchKernelGGL(k_tl_dcol<2>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
-    case 3: hipLaunchKernelGGL(k_tl_dcol<3>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
-    default: hipLaunchKernelGGL(k_tl_dcol<4>, grid, blk, 0, h->stream, tl.col_blk, tl.agg_nodes, h->val, h->nb, h->dinv, tl.ncol, tl.dcinv); break;
+    case 1: TF_DCOL(1); break;
+    case 2: TF_DCOL(2); break;
+    case 3: TF_DCOL(3); break;
+    default: TF_DCOL(4); break;
     }
+#undef TF_DCOL
     TF_HIP(hipGetLastError());
 }
 
@@ -2700,10 +2759,15 @@ static void twolevel_refresh(tf_handle* h, const std::vector<uint8_t>& which)
         }
         // the FP16 copy is enqueued before the failure flags are read back: the device has work
         // while the host waits, and the copy is simply not used after a failure
-        hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, As, ncs, ldc, tl.ascale + 2 * sg);
         const size_t n2h = (size_t)ncs * tl.h_seg_lda[sg];
-        hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, As, ldc,
-                           (__half*)tl.Ainv + tl.h_seg_aoff[sg], ncs, tl.h_seg_lda[sg], tl.ascale + 2 * sg);
+        if (tl.Ainv64) {
+            hipLaunchKernelGGL(k_tl_symm64, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, As, ldc,
+                               tl.Ainv64 + tl.h_seg_aoff[sg], ncs, tl.h_seg_lda[sg]);
+        } else {
+            hipLaunchKernelGGL(k_tl_scale, dim3(1), dim3(256), 0, h->stream, As, ncs, ldc, tl.ascale + 2 * sg);
+            hipLaunchKernelGGL(k_tl_symm, dim3((unsigned)((n2h + 255) / 256)), dim3(256), 0, h->stream, As, ldc,
+                               (__half*)tl.Ainv + tl.h_seg_aoff[sg], ncs, tl.h_seg_lda[sg], tl.ascale + 2 * sg);
+        }
         TF_HIP(hipGetLastError());
         TF_HIP(hipMemcpyAsync(bad.data() + sg, tl.info + sg, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     }
