@@ -60,7 +60,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=120, help="timed timesteps (config 2: 120 of dt = 60 s)")
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="config2", choices=["config1", "config2"])
+    p.add_argument("--workload", default="config2", choices=["config1", "config2", "large"],
+                   help="config2: the 38,826-prism ellipse; config1: the 1,040-prism disc; large: the same ellipse "
+                        "scaled to 1,013,310 prisms (disabled: a run of it faulted the GPU, cause not found)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=2, help="CPU-oracle timesteps per run (bounded sample)")
     p.add_argument("--cpu-runs", type=int, default=1, help="CPU-oracle runs (the median is reported)")
@@ -90,6 +92,20 @@ def traffic_from_profiles(kernel, workload):
     txt = open(path).read()
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt, re.M)
     return float(m.group(1)) if m else None
+
+
+def workload_tissue(name):
+    from tissuefolding_paperversion_amd import configs
+
+    if name == "config1":
+        return configs.config1()
+    if name == "large":
+        # The config-2 ellipse scaled by 5.1 in x and y (1,013,310 prisms, 678,392 nodes).  A run of
+        # this workload ended with an illegal memory access at the first convergence check after the
+        # Krylov kernels; its cause has not been found, so the workload stays off.
+        raise SystemExit("the 1,013,310-prism workload is disabled: a run of it faulted the GPU and the cause "
+                         "has not been found")
+    return configs.config2()
 
 
 def host_cpus():
@@ -284,7 +300,7 @@ def main():
     from tissuefolding_paperversion_amd import configs
     from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
 
-    tissue = configs.config2() if a.workload == "config2" else configs.config1()
+    tissue = workload_tissue(a.workload)
     dt = tissue.dt
     opts = {"krylov": a.krylov}
     if a.matrix_fp32:
@@ -325,7 +341,8 @@ def main():
                   np.sum(kr) * it_bytes) / a.steps
     step_gbps = step_bytes / (sec / a.steps) / 1e9
     out = {
-        "metric": "NR timesteps/s at 40k prisms, plain FP64 vs mixed (disp err <=1e-8 rel)",
+        "metric": ("NR timesteps/s at 40k prisms, plain FP64 vs mixed (disp err <=1e-8 rel)" if a.workload == "config2"
+                   else f"NR timesteps/s at {tissue.n_elements} prisms, plain FP64 vs mixed (disp err <=1e-8 rel)"),
         "value": a.steps / sec,
         "unit": "timesteps/s",
         "n_gpus": 1,
@@ -448,7 +465,7 @@ def main():
     if not a.no_batch:
         out["batched"] = batched_leg(NewtonRaphsonSolver, a.batch_steps, 1, cpu=not a.no_cpu_baseline)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(configs.config2() if a.workload == "config2" else configs.config1(), dt,
+        out["cpu_baseline"] = cpu_baseline(workload_tissue(a.workload), dt,
                                            a.cpu_steps, a.cpu_runs)
         out["cpu_baseline"]["literal_pipeline"] = cpu_literal(dt)
     print(json.dumps(out), flush=True)
