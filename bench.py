@@ -70,7 +70,7 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched 256-disc sweep")
     p.add_argument("--batch-steps", type=int, default=60, help="timed timesteps of the 256-disc sweep (config 4: 60)")
     p.add_argument("--no-adjoint", action="store_true", help="skip the config-5 adjoint leg")
-    p.add_argument("--adjoint-fd", type=int, default=7, help="parameters checked by central differences")
+    p.add_argument("--adjoint-fd", type=int, default=16, help="parameters checked by central differences")
     p.add_argument("--krylov", type=int, default=-1)
     p.add_argument("--matrix-fp32", action="store_true")
     p.add_argument("--no-graph", action="store_true")

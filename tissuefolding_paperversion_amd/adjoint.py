@@ -168,12 +168,18 @@ def apical_loss(x, target, apical_dofs):
     return 0.5 * float(np.dot(d, d)), d
 
 
+FD_OPTS = dict(nr_tol=1e-11, krylov_rtol=1e-13)  # forward runs of the difference check: no solver noise
+
+
 def config5_check(n_fd: int = 64, seed: int = 6, h: float = 1e-6, n_steps: int | None = None, opts: dict | None = None,
-                  log=None) -> dict:
-    """The inverse-growth workload (config 5): target run, forward and backward timings, and
-    central differences (h relative) of the loss on ``n_fd`` parameters -- the three layer
-    moduli plus ``n_fd - 3`` seeded random rate scales.  Errors are reported per parameter and
-    relative to the largest gradient entry."""
+                  log=None, fd_opts: dict | None = None) -> dict:
+    """The inverse-growth workload (config 5): target run, forward and backward timings (with
+    ``opts``), and central differences (h relative) of the loss on ``n_fd`` parameters -- the
+    three layer moduli plus ``n_fd - 3`` seeded random rate scales -- against the adjoint
+    gradient of a second rollout at ``fd_opts`` (default: Newton to 1e-11 with exact Krylov
+    solves, so that the differences carry no solver noise).  Errors are reported per parameter,
+    relative to the largest gradient entry, and against the bar |adjoint - FD| <= 1e-5 max(|FD|,
+    1e-2 max|gradient|)."""
     import time
 
     from . import configs
@@ -182,18 +188,24 @@ def config5_check(n_fd: int = 64, seed: int = 6, h: float = 1e-6, n_steps: int |
     t, s_target, T, K = configs.config5()
     if n_steps:
         T = n_steps
-    ro = GrowthRollout(t, T, checkpoint_every=K, opts=opts)
-    E0 = ro.layer_moduli()
-    target = ro.forward(s_target, E0)
+    rt = GrowthRollout(t, T, checkpoint_every=K, opts=opts)  # the timed rollout
+    E0 = rt.layer_moduli()
+    target = rt.forward(s_target, E0)
     apical = np.where(np.repeat(t.mesh.node_placement == APICAL, 3))[0]
     s0 = np.ones(t.n_elements)
     t0 = time.perf_counter()
-    x = ro.forward(s0, E0)
+    x = rt.forward(s0, E0)
     t_fwd = time.perf_counter() - t0
-    L0, dLdx = apical_loss(x, target, apical)
+    _, dLdx = apical_loss(x, target, apical)
     t0 = time.perf_counter()
-    s_bar, E_bar = ro.backward(dLdx)
+    rt.backward(dLdx)
     t_bwd = time.perf_counter() - t0
+    # the difference check on a rollout at tight tolerances (its own target run)
+    fo = dict(opts or {}, **(FD_OPTS if fd_opts is None else fd_opts))
+    ro = GrowthRollout(t, T, checkpoint_every=K, opts=fo)
+    target = ro.forward(s_target, E0)
+    L0, dLdx = apical_loss(ro.forward(s0, E0), target, apical)
+    s_bar, E_bar = ro.backward(dLdx)
     gmax = max(np.max(np.abs(s_bar)), np.max(np.abs(E_bar * E0)))
     idx = np.random.default_rng(seed).choice(t.n_elements, max(0, n_fd - 3), replace=False)
     rows = []
@@ -216,12 +228,13 @@ def config5_check(n_fd: int = 64, seed: int = 6, h: float = 1e-6, n_steps: int |
             log(rows[-1])
     err_rel_max = max(abs(a - f) / gmax for _, _, a, f in rows) if rows else 0.0
     per = [abs(a - f) / max(abs(f), 1e-300) for _, _, a, f in rows]
+    bar = all(abs(a - f) <= 1e-5 * max(abs(f), 1e-2 * gmax) for _, _, a, f in rows)
     return {"workload": f"config 5: {t.name}, {t.n_elements} prisms, {T} steps, checkpoint every {K}",
             "loss": L0, "forward_s": t_fwd, "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd,
-            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in ro.stats])),
-            "adjoint_krylov_per_step": float(np.mean(ro.adjoint_krylov)),
-            "rotated_elements": int(sum(st["rotated_elements"] for st in ro.stats)),
-            "fd_parameters": len(rows), "fd_h": h,
+            "nr_iterations_per_step": float(np.mean([st["nr_iterations"] for st in rt.stats])),
+            "adjoint_krylov_per_step": float(np.mean(rt.adjoint_krylov)),
+            "rotated_elements": int(sum(st["rotated_elements"] for st in rt.stats)),
+            "fd_parameters": len(rows), "fd_h": h, "fd_options": fo, "fd_within_bar": bool(bar),
             "fd_err_max_rel_to_max_grad": err_rel_max,
             "fd_err_per_param_median": float(np.median(per)) if per else 0.0,
             "fd_err_per_param_max": float(np.max(per)) if per else 0.0,
