@@ -68,15 +68,16 @@ enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_CG1 = 3, TF_KRYLOV_PI
 enum { TF_PREC_FP64 = 0, TF_PREC_FP32 = 1, TF_PREC_BF16 = 2 };
 /* TF_PRECOND_AUTO (the default): two-level for a handle with 4096 nodes or more, else block
  * Jacobi.  TF_PRECOND_BLOCK_JACOBI: the inverse 3x3 diagonal blocks of M.
- * TF_PRECOND_TWO_LEVEL (one tissue per handle, CG solvers): block Jacobi over the node columns
- * (the diagonal blocks of M over the nodes that prisms' vertical edges join; 3x3 nodal blocks
- * when the columns are not uniform or column_blocks = 0) plus an additive coarse correction
- * P (P^T M P)^-1 P^T r, P the rigid-body modes of aggregates of whole columns (square cells of the
- * tissue's x-y extent).  The coarse operator is rebuilt at the first Newton iteration of a
- * timestep once Newton iteration 1's Krylov count exceeds 1.25x its count right after the last
- * rebuild, or coarse_refresh timesteps after it, and for every solve outside the Newton loop;
- * handles with several tissues use block Jacobi.  The column-block inverses are held in FP32 and
- * the coarse inverse in FP16 scaled by its largest diagonal entry (they only precondition:
+ * TF_PRECOND_TWO_LEVEL (CG solvers): block Jacobi over the node columns (the diagonal blocks of
+ * M over the nodes that prisms' vertical edges join; 3x3 nodal blocks when the columns are not
+ * uniform or column_blocks = 0) plus an additive coarse correction P (P^T M P)^-1 P^T r, P the
+ * rigid-body modes of aggregates of whole columns (square cells of the tissue's x-y extent).
+ * Every tissue of a handle has its own aggregates and its own block of the coarse operator (at
+ * most 4096 coarse unknowns per tissue).  A tissue's block is rebuilt at the first Newton
+ * iteration of a timestep once its Newton iteration 1 Krylov count exceeds 1.25x its count right
+ * after its last rebuild; every block coarse_refresh timesteps after the last full rebuild, and
+ * for every solve outside the Newton loop.  The column-block inverses are held in FP32 and the
+ * coarse inverse in FP16 scaled by its largest diagonal entry (they only precondition:
  * residuals, matrix and Krylov vectors stay in the solver's precision; precond_fp64 holds both
  * in FP64). */
 enum { TF_PRECOND_AUTO = -1, TF_PRECOND_BLOCK_JACOBI = 0, TF_PRECOND_TWO_LEVEL = 1 };
@@ -354,8 +355,8 @@ int tf_kernel_timing(tf_handle* h, const char* name, int32_t enable, double* avg
 /* Algorithmic bytes of one launch of the named kernel (unique minimum traffic): "spmv",
  * "coarse", "assemble_elements", "assemble_gather". */
 int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes);
-/* Average device time (ms) of one launch of the kernel priced under "spmv" (the dominant Krylov
- * kernel of the active solver: k_cg_q for the two-level and textbook CG, k_cg1, k_pipe),
+/* Average device time (ms) of one launch of the kernel priced under "spmv" (the SpMV launch of
+ * the active solver: k_tp_q for the two-level PCG, k_cg_q for the textbook CG, k_cg1, k_pipe),
  * n_launches replayed back to back in one graph on the current Jacobian and timed with HIP events
  * around the replay.  Leaves the Krylov work vectors undefined.  0 for BiCGSTAB. */
 int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms);

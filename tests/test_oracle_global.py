@@ -82,6 +82,42 @@ def test_dense_csr_and_solvers_agree(oracle):
     assert np.linalg.norm(Kd.T @ xd - gd) < 1e-10 * np.linalg.norm(gd)
 
 
+def test_literal_pipeline_scan_and_solve(oracle):
+    """The reference's literal pipeline (NewtonRaphsonSolver.cpp:686-707: row-major scan of the dense
+    K keeping |v| > 1e-13, then a transposed PARDISO solve): the scan equals numpy's row-major
+    nonzeros of the thresholded K (a transposed index or a dropped threshold fails), and the
+    literal solve equals the dense LU solve when PARDISO can run in this process."""
+    t = configs.disc_tissue(radius=18.0, h=4.5, eta_ext=30.0)
+    o = oracle.OracleTissue(t)
+    o.grow(60.0)
+    o.update_viscosity_surfaces()
+    X = o.pos.copy()
+    K, g = o.assemble(X + 1e-3 * np.sin(np.arange(X.size)), X, 60.0, dense=True)
+    K[0, 1] = 5e-14                        # below the threshold: dropped
+    K[1, 0] = -2e-13                       # above it: kept
+    n = len(g)
+    L = oracle.lib()
+    Kc = np.ascontiguousarray(K)
+    nnz = L.tfo_dense_to_csr(n, oracle._c(Kc), None, None, None)
+    ia = np.zeros(n + 1, np.int64)
+    ja = np.zeros(nnz, np.int32)
+    a = np.zeros(nnz)
+    L.tfo_dense_to_csr(n, oracle._c(Kc), ia.ctypes.data, ja.ctypes.data, a.ctypes.data)
+    keep = np.abs(K) > 1e-13
+    rows, cols = np.nonzero(keep)          # row-major order
+    assert nnz == len(rows)
+    assert np.array_equal(ia, np.concatenate([[0], np.cumsum(keep.sum(axis=1))]))
+    assert np.array_equal(ja, cols)
+    assert np.array_equal(a, K[rows, cols])
+    if not (oracle._pardiso_safe() and L.tfo_pardiso_available(None)):
+        pytest.skip("MKL PARDISO cannot run in this process (absent, or PyTorch's MKL is loaded)")
+    ref = oracle.OracleTissue(t)
+    xd = ref.solve(K.copy(), g)            # dense LU of K^T
+    o.solver = "literal"
+    xl = o.solve(K.copy(), g)
+    assert rel_err(xl, xd) < 1e-10
+
+
 def test_config1_exact_answer(oracle):
     t = configs.config1()
     o = oracle.OracleTissue(t)
