@@ -299,8 +299,6 @@ struct tf_handle {
         int32_t* seg_c0 = nullptr;                          // [S+1] first coarse unknown of every tissue
         int32_t* seg_ldc = nullptr;                         // [S] row stride of the tissue's block of Ac
         int64_t* seg_acoff = nullptr;                       // [S] offset of that block in Ac
-        int4* seg_coarse = nullptr;                         // [S] first coarse unknown, count, row stride of
-                                                            //     Ainv (halves), offset of its block (half pairs)
         uint8_t* refresh = nullptr;                         // [S] 1: the tissue's block is rebuilt
         std::vector<int32_t> h_seg_c0, h_seg_ldc, h_seg_lda;
         std::vector<int64_t> h_seg_acoff, h_seg_aoff;       // (Ainv offsets in halves)
@@ -328,8 +326,8 @@ struct tf_handle {
         int32_t *prow_ptr = nullptr, *pcol = nullptr, *pblk = nullptr;  // [N+1], [nb], [nb] permuted Jacobian pattern
         int32_t* pass_ptr = nullptr;                        // [A+1] row passes of every aggregate
         int2* pass_rows = nullptr;                          // [passes] rows [x, y) of every pass
-        int4* first_pass = nullptr;                         // [A] rows and blocks of every aggregate's first pass
-        int2* agg_order = nullptr;                          // [A] aggregate and tissue, by descending size (workgroup order)
+        int4* wg_rec = nullptr;                             // [A][5] record of every workgroup of the aggregate-order
+                                                            //     PCG (aggregates by descending size)
         double* pcoef_p = nullptr;                          // [N][3][6] rows of P by position
         double *val_p = nullptr;                            // [9][nb] FP64 values in aggregate order
         float* val32_p = nullptr;                           // [9][nb] FP32 values in aggregate order

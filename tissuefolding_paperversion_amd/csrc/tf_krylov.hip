@@ -22,6 +22,7 @@
 #include <hip/hip_fp16.h>
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <math.h>
 #include <numeric>
 #include <type_traits>
@@ -53,11 +54,32 @@ __device__ inline int xcd_tile(int b, int T, int nxcd)
     return x * q + (x < rem ? x : rem) + k;
 }
 
+// v of the lane CTRL names (a DPP row shift or broadcast), 0 where there is none or where the row is
+// masked off (ROW_MASK: one bit per row of 16 lanes)
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_take(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave, returned in every lane; every lane must be active.  An inclusive
+// scan by DPP row shifts (1, 2, 4, 8) inside each row of 16 lanes, then lane 15 of each row added
+// into the next row (rows 1 and 3) and lane 31 into the upper half, so lane 63 holds the total.
+// Register moves between neighbouring lanes instead of LDS permutes: the same fixed order in every
+// wave.
 __device__ inline double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_take<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_take<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_take<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_take<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_take<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_take<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 // Tile partials: reduce v[q] over the workgroup and store partial[slot[q]*T + t].
@@ -626,26 +648,31 @@ __global__ __launch_bounds__(WG) void k_tl_dcol(const int32_t* __restrict__ col_
 // Every sum runs in a fixed order.  Partials are indexed by aggregate (TileArgs with T = n_agg).
 constexpr int TP_CAP = 2048;  // blocks per pass of a workgroup (48 KB of LDS products)
 
+// Everything a workgroup needs before its first data load, in one 80-byte record read by scalar
+// loads that issue together (the aggregate's tissue, offsets and coarse block would otherwise be
+// loads that wait for one another).  Workgroups take the aggregates largest first: the dispatcher
+// deals workgroups in order, so the small ones fill in behind the large ones.
+struct alignas(16) TPRec {
+    int32_t a, sg, st0, st1;  // aggregate, tissue, the tissue's aggregates [st0, st1) (partial slots)
+    int32_t r0, r1, b0, b1;   // the aggregate's first pass: rows [r0, r1), permuted blocks [b0, b1)
+    int32_t ps0, ps1, n0, n1; // its passes [ps0, ps1) and positions [n0, n1)
+    int32_t c0, kr, pad0, pad1;  // its first coarse unknown and rank
+    int4 sc;                  // the tissue's first coarse unknown, count, ldA, Ainv block offset (pairs)
+};
+static_assert(sizeof(TPRec) == 80, "TPRec layout");
+
 struct TPArgs {
-    const int32_t* agg_ptr;    // [A+1] positions of every aggregate
     const int32_t* agg_nodes;  // [N] node of every position
     const int32_t* prow_ptr;   // [N+1] permuted blocks of every position's row
     const int32_t* pcol;       // [nb] column position of every permuted block
-    const int32_t* pass_ptr;   // [A+1] passes of every aggregate
     const int2* pass_rows;     // [passes] rows [x, y) of every pass (<= WG / 3 rows, <= TP_CAP blocks)
-    const int4* first_pass;    // [A] rows [x, y) and blocks [z, w) of every aggregate's first pass
-    const int2* order;         // [A] aggregate and tissue of every workgroup: largest first (the
-                               // dispatcher deals workgroups in order, so the small ones fill in
-                               // behind the large ones)
+    const TPRec* rec;          // [A] the record of every workgroup
     const double* pcoef;       // [N][3][6] rows of P by position
-    const int32_t* coff;       // [A] first coarse unknown of every aggregate
-    const int32_t* krank;      // [A] coarse unknowns of every aggregate
     const float* dcinv;        // [ncol][3L][3L] inverse column blocks (FP32, symmetric)
     const double* dcinv64;     //   the same in FP64 (FP64 preconditioner storage)
     const __half2* Ainv;       // per tissue [nc_s][ldA_s / 2] Ac_s^-1 / scale_s in FP16 pairs
     const double* Ainv64;      //   per tissue [nc_s][ldA_s] Ac_s^-1 in FP64 (FP64 preconditioner storage)
     const int64_t* seg_aoff64; //   [S] offset of the tissue's FP64 block
-    const int4* seg_coarse;    // [S] first coarse unknown, count nc_s, ldA_s, offset of the Ainv block (pairs)
     const double* ascale;      // [S][2] scale and its reciprocal
     double* ptq;               // [nc] P^T q of the current iteration
     double* rcx;               // [2][nc] P^T r (ping-pong by parity)
@@ -696,6 +723,61 @@ __device__ inline void tp_spmv(const TPArgs& tp, const VT* __restrict__ val, int
     }
 }
 
+// The tissue's partials of K slots, loaded before a kernel's data so that the wait for them does
+// not wait for the data (loads return in order): at most TP_SU partials per lane and slot, the
+// tissue's aggregates [st0, st1) at most TP_SU * WG (a tissue has at most max_nc / 6 aggregates),
+// clamped addresses, masked in tp_partials_sum.
+constexpr int TP_SU = 3;
+
+template <int K>
+__device__ inline void tp_partials_load(const TileArgs& ta, int st0, int st1, const int (&sl)[K],
+                                        double (&v)[K][TP_SU])
+{
+#pragma unroll
+    for (int u = 0; u < TP_SU; ++u) {
+        const int k = st0 + (int)threadIdx.x + u * WG;
+        const int kk = k < st1 ? k : st0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q][u] = ta.partial[(size_t)sl[q] * ta.T + kk];
+    }
+}
+
+// Tissue totals from the loaded partials in the order of seg_sums_range (lane sums in index order,
+// then a fixed wave and tree order), identical in every lane and every workgroup of the tissue.
+template <int K>
+__device__ inline void tp_partials_sum(const TileArgs& ta, int st0, int st1, const int (&sl)[K],
+                                       const double (&v)[K][TP_SU], double (&out)[K])
+{
+    constexpr int NW = WG / 64;
+    __shared__ double red[K][NW];
+    double s[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) s[q] = 0.0;
+#pragma unroll
+    for (int u = 0; u < TP_SU; ++u)
+        if (st0 + (int)threadIdx.x + u * WG < st1)
+#pragma unroll
+            for (int q = 0; q < K; ++q) s[q] += v[q][u];
+    for (int k = st0 + (int)threadIdx.x + TP_SU * WG; k < st1; k += WG)  // beyond TP_SU * WG: not built
+#pragma unroll
+        for (int q = 0; q < K; ++q) s[q] += ta.partial[(size_t)sl[q] * ta.T + k];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        s[q] = wave_sum(s[q]);
+        if (l == 0) red[q][w] = s[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) a += red[q][k];
+        out[q] = a;
+    }
+    __syncthreads();
+}
+
 // Workgroup sums of K values into partial slots of aggregate a (fixed wave and lane order).
 template <int K>
 __device__ inline void tp_reduce(double (&v)[K], double* out)
@@ -726,9 +808,9 @@ __global__ __launch_bounds__(WG) void k_tp_init(TileArgs ta, TPArgs tp, const VT
                                                 double* __restrict__ p_old)
 {
     extern __shared__ double prod[];
-    const int2 ag = tp.order[blockIdx.x];
-    const int a = ag.x;
-    const bool active = ta.seg[ag.y].active;
+    const TPRec R = tp.rec[blockIdx.x];
+    const int a = R.a;
+    const bool active = ta.seg[R.sg].active;
     double bb = 0.0, rc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     auto own = [&](int pos, int comp, double mx, double) {
         const int d = 3 * pos + comp;
@@ -743,20 +825,20 @@ __global__ __launch_bounds__(WG) void k_tp_init(TileArgs ta, TPArgs tp, const VT
         for (int j = 0; j < 6; ++j) rc[j] += Pr[j] * rv;
     };
     // the product is M x0 (beta = 0); without x0 there is none
-    tp_spmv<VT>(tp, val, nb, x0p, x0p, 0.0, tp.pass_ptr[a], tp.pass_ptr[a + 1], prod, own, x0p != nullptr);
+    tp_spmv<VT>(tp, val, nb, x0p, x0p, 0.0, R.ps0, R.ps1, prod, own, x0p != nullptr);
     double v[7] = {bb, rc[0], rc[1], rc[2], rc[3], rc[4], rc[5]};
     __shared__ double out[7];
     tp_reduce<7>(v, out);
     if (threadIdx.x == 0) ta.partial[(size_t)SL_BB * ta.T + a] = out[0];
-    const int kr = tp.krank[a];
-    if ((int)threadIdx.x < kr) tp.rcx[tp.nc + tp.coff[a] + threadIdx.x] = out[1 + threadIdx.x];
+    if ((int)threadIdx.x < R.kr) tp.rcx[tp.nc + R.c0 + threadIdx.x] = out[1 + threadIdx.x];
 }
 
 // beta, p = z + beta p_old, q = M p, (p, q) and P^T q; the convergence, cap and stagnation test.
-// Each lane loads its first TP_PF blocks of the first pass (columns, values, then z and p_old of
-// those columns) and its own DoF's rows of P before the partials are summed, so those loads are in
-// flight while beta is formed.
-constexpr int TP_PF = 2;
+// The tissue's partials are loaded first, then each lane's first TP_PF blocks of the first pass
+// (columns, values, then z and p_old of those columns), so all of them are in flight while beta is
+// formed.  TP_PF * WG blocks cover an aggregate of up to about 75 nodes of a 3-layer mesh; blocks
+// past them are loaded after beta.
+constexpr int TP_PF = 5;
 
 template <typename VT, int P>
 __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* __restrict__ val, int nb,
@@ -764,16 +846,15 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
                                              double* __restrict__ p_new, double* __restrict__ q, int maxit)
 {
     extern __shared__ double prod[];
-    const int2 ag = tp.order[blockIdx.x];
-    const int a = ag.x, sg = ag.y;
-    SegScalars& S = ta.seg[sg];
+    const TPRec R = tp.rec[blockIdx.x];
+    SegScalars& S = ta.seg[R.sg];
     if (seg_done(S)) return;  // after the solve's end: one load before the exit
-    const int ps0 = tp.pass_ptr[a], ps1 = tp.pass_ptr[a + 1];
-    const int st0 = ta.seg_tile_ptr[sg], st1 = ta.seg_tile_ptr[sg + 1];  // the tissue's aggregates
     const int tid = threadIdx.x;
-    const int4 rec = tp.first_pass[a];  // rows and blocks of the first pass, one load
-    int2 rr = int2{rec.x, rec.y};
-    int b0 = rec.z, b1 = rec.w;
+    const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
+    double pl[3][TP_SU];
+    tp_partials_load<3>(ta, R.st0, R.st1, sl, pl);
+    int2 rr = int2{R.r0, R.r1};
+    int b0 = R.b0, b1 = R.b1;
     double pv[TP_PF][9], pz[TP_PF][3], pp[TP_PF][3];
 #pragma unroll
     for (int k = 0; k < TP_PF; ++k) {
@@ -785,13 +866,12 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
 #pragma unroll
         for (int e = 0; e < 3; ++e) { pz[k][e] = z[3 * c + e]; pp[k][e] = p_old[3 * c + e]; }
     }
-    const int sl[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
     double sc[3];
-    seg_sums_range<3>(ta, st0, st1, sl, sc);
+    tp_partials_sum<3>(ta, R.st0, R.st1, sl, pl, sc);
     const int it = S.iters;
     const double best = S.best_rr_h[P];
     const int best_it = S.best_it_h[P];
-    const bool lead = a == st0 && tid == 0;
+    const bool lead = R.a == R.st0 && tid == 0;
     {
         const bool stagnant = it - best_it > 256;
         const bool conv = sc[0] <= S.tol2 || (stagnant && best <= 100.0 * S.tol2);
@@ -812,14 +892,14 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
     }
     const double beta = it > 0 ? sc[1] / sc[2] : 0.0;
     double pq = 0.0, pt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int ps = ps0; ps < ps1; ++ps) {
-        if (ps > ps0) {
+    for (int ps = R.ps0; ps < R.ps1; ++ps) {
+        if (ps > R.ps0) {
             rr = tp.pass_rows[ps];
             b0 = tp.prow_ptr[rr.x];
             b1 = tp.prow_ptr[rr.y];
         }
-        const int jf = ps == ps0 ? b0 + TP_PF * WG : b0;  // the first pass's prefetched blocks come first
-        if (ps == ps0) {
+        const int jf = ps == R.ps0 ? b0 + TP_PF * WG : b0;  // the first pass's prefetched blocks come first
+        if (ps == R.ps0) {
 #pragma unroll
             for (int k = 0; k < TP_PF; ++k) {
                 const int j = b0 + tid + k * WG;
@@ -869,54 +949,57 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
     double v[7] = {pq, pt[0], pt[1], pt[2], pt[3], pt[4], pt[5]};
     __shared__ double out[7];
     tp_reduce<7>(v, out);
-    if (tid == 0) ta.partial[(size_t)SL_PQ * ta.T + a] = out[0];
-    const int kr = tp.krank[a];
-    if (tid < kr) tp.ptq[tp.coff[a] + tid] = out[1 + tid];
+    if (tid == 0) ta.partial[(size_t)SL_PQ * ta.T + R.a] = out[0];
+    if (tid < R.kr) tp.ptq[R.c0 + tid] = out[1 + tid];
 }
 
 // alpha, the coarse right-hand side, z_c, then x, r, z on the aggregate's rows, (r, z), (r, r) and
 // rcx[1-P] = P^T r.  INIT: r as given (the initial residual), r_c = rcx[1], no update.
-// Every global load that does not need alpha is issued first: the aggregate's rows of Ac^-1, this
-// lane's entries of rcx[P] and P^T q, and the first pass's r, x, p, q, Dc^-1 rows and P rows; alpha
-// is then formed, r_c = rcx[P] - alpha P^T q staged in LDS (FP32: it only meets the FP16 coarse
-// inverse), and z_c = Ac^-1 r_c.
-constexpr int TP_PAIRS = 8;  // column pairs per lane of every coarse row held in registers (nc <= 4096)
+// Every global load that does not need alpha is issued first, the tissue's partials leading: the
+// aggregate's rows of Ac^-1 (16 bytes per lane and load), this lane's entries of rcx[P] and P^T q,
+// and the first pass's r, x, p, q, Dc^-1 rows and P rows; alpha is then formed, r_c = rcx[P] -
+// alpha P^T q staged in LDS (FP32: it only meets the FP16 coarse inverse), and z_c = Ac^-1 r_c.
+constexpr int TP_Q4 = 2;     // 16-byte loads (8 halves) per lane of every coarse row held in registers
+constexpr int TP_NV = 16;    // entries of rcx and P^T q per lane (nc_s <= 4096)
 
 template <int P, bool INIT, int L, bool F64 = false>
 __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const double* __restrict__ p,
                                              const double* __restrict__ q, double* __restrict__ x,
                                              double* __restrict__ r, double* __restrict__ z)
 {
-    constexpr int D = 3 * L, CPP = WG / D, NV = 2 * TP_PAIRS;
-    const int2 ag = tp.order[blockIdx.x];
-    const int a = ag.x, sg = ag.y;
+    constexpr int D = 3 * L, CPP = WG / D;
+    const TPRec R = tp.rec[blockIdx.x];
+    const int a = R.a, sg = R.sg;
     SegScalars& S = ta.seg[sg];
     if (seg_done(S)) return;  // after the solve's end: one load before the exit
-    const int n0 = tp.agg_ptr[a], ncols = (tp.agg_ptr[a + 1] - n0) / L;
-    const int c0 = tp.coff[a], kr = tp.krank[a];
-    const int4 sc4 = tp.seg_coarse[sg];  // the tissue's coarse unknowns [C0, C0 + ncs) and Ainv block
-    const int st0 = ta.seg_tile_ptr[sg], st1 = ta.seg_tile_ptr[sg + 1];
+    const int tid = threadIdx.x;
+    const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
+    double pl[3][TP_SU];
+    if (!INIT) tp_partials_load<3>(ta, R.st0, R.st1, sl, pl);
+    const int n0 = R.n0, ncols = (R.n1 - n0) / L;
+    const int c0 = R.c0, kr = R.kr;
     extern __shared__ float tp_rcs[];  // r_c, ldA entries (zero past nc); FP64 storage: doubles
     double* const rcd = reinterpret_cast<double*>(tp_rcs);
     __shared__ double rl[WG];
     __shared__ double zcs[6];
-    const int tid = threadIdx.x, slot = tid / D, li = tid % D;
-    const int nc = tp.nc, C0 = sc4.x, ncs = sc4.y, ldA = sc4.z, ld2 = ldA / 2, lc0 = c0 - C0;
+    const int slot = tid / D, li = tid % D;
+    const int nc = tp.nc, C0 = R.sc.x, ncs = R.sc.y, ldA = R.sc.z, ld2 = ldA / 2, lc0 = c0 - C0;
     const int last = max(ncs, 1) - 1;  // a tissue without coarse unknowns has one row of zeros
-    const __half2* Ab = tp.Ainv + sc4.w;
-    __half2 pre[6][F64 ? 1 : TP_PAIRS];
+    uint4 pre[6][F64 ? 1 : TP_Q4];     // 8 halves each: pairs 4 (tid + WG m) .. + 3 of the row
     if constexpr (!F64) {
+        const uint4* Ab = reinterpret_cast<const uint4*>(tp.Ainv + R.sc.w);  // ldA is a multiple of 8 halves
+        const int lq = ld2 / 4;
 #pragma unroll
         for (int k = 0; k < 6; ++k)
 #pragma unroll
-            for (int m = 0; m < TP_PAIRS; ++m)
-                pre[k][m] = Ab[(size_t)min(lc0 + k, last) * ld2 + min(tid + WG * m, ld2 - 1)];
+            for (int m = 0; m < TP_Q4; ++m)
+                pre[k][m] = Ab[(size_t)min(lc0 + k, last) * lq + min(tid + WG * m, lq - 1)];
     }
     const double* rcp = tp.rcx + (size_t)P * nc + C0;
     const double* ptq = tp.ptq + C0;
-    double vx[NV], vq[NV];
+    double vx[TP_NV], vq[TP_NV];
 #pragma unroll
-    for (int m = 0; m < NV; ++m) {
+    for (int m = 0; m < TP_NV; ++m) {
         const int ic = min(tid + WG * m, last);  // clamped: always a valid address
         vx[m] = rcp[ic];
         vq[m] = INIT ? 0.0 : ptq[ic];
@@ -936,19 +1019,18 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     for (int j = 0; j < 6; ++j) Pr[j] = tp.pcoef[18 * (size_t)pos + 6 * (li % 3) + j];
     double alpha = 0.0;
     if (!INIT) {
-        const int sl[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
         double sc[3];
-        seg_sums_range<3>(ta, st0, st1, sl, sc);
+        tp_partials_sum<3>(ta, R.st0, R.st1, sl, pl, sc);
         // (p, q) > 0 needs M positive definite, (r, z) > 0 the preconditioner: either failing stops
         // the solve as stalled, x keeping residual sc[2]
         if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
-            if (a == st0 && tid == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
+            if (a == R.st0 && tid == 0) { S.rnorm2 = sc[2]; S.stalled = 1; }
             return;
         }
         alpha = sc[0] / sc[1];
     }
 #pragma unroll
-    for (int m = 0; m < NV; ++m) {
+    for (int m = 0; m < TP_NV; ++m) {
         const int i = tid + WG * m;
         if (i < ldA) {
             if constexpr (F64) rcd[i] = i < ncs ? vx[m] - alpha * vq[m] : 0.0;
@@ -970,12 +1052,17 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < TP_PAIRS; ++m) {
-                    const int j = 2 * (tid + WG * m);
-                    if (j < ldA) {  // pairs past ldA hold clamped copies: skipped
-                        const float2 f = __half22float2(pre[k][m]);
-                        const float2 c2 = *reinterpret_cast<const float2*>(tp_rcs + j);
-                        sv[k] += (double)f.x * c2.x + (double)f.y * c2.y;
+                for (int m = 0; m < TP_Q4; ++m) {
+                    const int j = 8 * (tid + WG * m);
+                    if (j < ldA) {  // chunks past ldA hold clamped copies: skipped
+                        const __half2* hp = reinterpret_cast<const __half2*>(&pre[k][m]);
+                        const float4 ca = *reinterpret_cast<const float4*>(tp_rcs + j);
+                        const float4 cb = *reinterpret_cast<const float4*>(tp_rcs + j + 4);
+                        const float2 f0 = __half22float2(hp[0]), f1 = __half22float2(hp[1]);
+                        const float2 f2 = __half22float2(hp[2]), f3 = __half22float2(hp[3]);
+                        sv[k] += (double)f0.x * ca.x + (double)f0.y * ca.y + (double)f1.x * ca.z +
+                                 (double)f1.y * ca.w + (double)f2.x * cb.x + (double)f2.y * cb.y +
+                                 (double)f3.x * cb.z + (double)f3.y * cb.w;
                     }
                 }
             }
@@ -1008,7 +1095,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
             rv = r[d];
             if (!INIT) { xv = x[d]; pv = p[d]; qv = q[d]; }
 #pragma unroll
-            for (int j = 0; j < D; ++j) Dr[j] = tp.dcinv[((size_t)g * D + li) * D + j];
+            for (int j = 0; j < D; ++j) Dr[j] = dc_row<F64>(tp, ((size_t)g * D + li) * D + j);
 #pragma unroll
             for (int j = 0; j < 6; ++j) Pr[j] = tp.pcoef[18 * (size_t)pos + 6 * (li % 3) + j];
         }
@@ -1039,7 +1126,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     if (tid == 0) {
         ta.partial[(size_t)(SL_RZ0 + (1 - P)) * ta.T + a] = out[0];
         ta.partial[(size_t)(SL_RR0 + (1 - P)) * ta.T + a] = out[1];
-        if (!INIT && a == st0) S.iters += 1;
+        if (!INIT && a == R.st0) S.iters += 1;
     }
     if (tid < kr) tp.rcx[(size_t)(1 - P) * nc + c0 + tid] = out[2 + tid];
 }
@@ -2232,15 +2319,13 @@ void tl_release(tf_handle* h)
                         &tl.pair_b, &tl.pair_ptr, &tl.pair_blk, &tl.info, &tl.col_blk, &tl.prow_ptr, &tl.pcol,
                         &tl.pblk, &tl.pass_ptr, &tl.tp_seg_ptr, &tl.agg_seg, &tl.seg_c0, &tl.seg_ldc})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    if (tl.agg_order) { (void)hipFree(tl.agg_order); tl.agg_order = nullptr; }
+    if (tl.wg_rec) { (void)hipFree(tl.wg_rec); tl.wg_rec = nullptr; }
     if (tl.seg_acoff) { (void)hipFree(tl.seg_acoff); tl.seg_acoff = nullptr; }
-    if (tl.seg_coarse) { (void)hipFree(tl.seg_coarse); tl.seg_coarse = nullptr; }
     if (tl.refresh) { (void)hipFree(tl.refresh); tl.refresh = nullptr; }
     for (double** p : {&tl.pcoef, &tl.Ac, &tl.swP, &tl.swO, &tl.swD, &tl.pcoef_p, &tl.val_p,
                        &tl.tpart, &tl.ptq, &tl.rcx})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (tl.pass_rows) { (void)hipFree(tl.pass_rows); tl.pass_rows = nullptr; }
-    if (tl.first_pass) { (void)hipFree(tl.first_pass); tl.first_pass = nullptr; }
     if (tl.val32_p) { (void)hipFree(tl.val32_p); tl.val32_p = nullptr; }
     if (tl.Ainv) { (void)hipFree(tl.Ainv); tl.Ainv = nullptr; }
     if (tl.ascale) { (void)hipFree(tl.ascale); tl.ascale = nullptr; }
@@ -2598,20 +2683,25 @@ static void twolevel_build(tf_handle* h)
         tl_upload(h, tl.pblk, pblk);
         tl_upload(h, tl.pass_ptr, pass_ptr);
         tl_upload(h, tl.pass_rows, pass_rows);
-        std::vector<int4> first(A);
-        for (int a = 0; a < A; ++a) {
-            const int2 r = pass_rows[pass_ptr[a]];
-            first[a] = int4{r.x, r.y, prow[r.x], prow[r.y]};
-        }
-        tl_upload(h, tl.first_pass, first);
+        // one record per workgroup, the aggregates by descending block count
         std::vector<int32_t> order(A);
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
             return prow[agg_ptr[x + 1]] - prow[agg_ptr[x]] > prow[agg_ptr[y + 1]] - prow[agg_ptr[y]];
         });
-        std::vector<int2> order2(A);
-        for (int k = 0; k < A; ++k) order2[k] = int2{order[k], agg_seg[order[k]]};
-        tl_upload(h, tl.agg_order, order2);
+        std::vector<TPRec> rec(A);
+        for (int k = 0; k < A; ++k) {
+            const int a = order[k], sg = agg_seg[a];
+            const int2 r = pass_rows[pass_ptr[a]];
+            if (seg_agg_ptr[sg + 1] - seg_agg_ptr[sg] > TP_SU * WG)
+                throw std::runtime_error("two-level preconditioner: more aggregates in a tissue than a workgroup sums");
+            rec[k] = TPRec{a, sg, seg_agg_ptr[sg], seg_agg_ptr[sg + 1], r.x, r.y, prow[r.x], prow[r.y],
+                           pass_ptr[a], pass_ptr[a + 1], agg_ptr[a], agg_ptr[a + 1], coff[a], coff[a + 1] - coff[a],
+                           0, 0, seg_coarse[sg]};
+        }
+        std::vector<int4> rec4(5 * (size_t)A);
+        std::memcpy(rec4.data(), rec.data(), sizeof(TPRec) * (size_t)A);
+        tl_upload(h, tl.wg_rec, rec4);
         tl_upload(h, tl.pcoef_p, pcoef_p);
         tl_upload(h, tl.tp_seg_ptr, seg_agg_ptr);
         tl.tpart = tl_alloc<double>((size_t)TF_RED_SLOTS * A);
@@ -2648,7 +2738,6 @@ static void twolevel_build(tf_handle* h)
     tl_upload(h, tl.seg_c0, tl.h_seg_c0);
     tl_upload(h, tl.seg_ldc, tl.h_seg_ldc);
     tl_upload(h, tl.seg_acoff, tl.h_seg_acoff);
-    tl_upload(h, tl.seg_coarse, seg_coarse);
     tl.refresh = tl_alloc<uint8_t>(S);
     tl.base_it1.assign(S, -1);
     tl.stale.assign(S, 0);
@@ -2681,9 +2770,9 @@ static TileArgs tp_tile_args(tf_handle* h)
 static TPArgs tp_args(tf_handle* h)
 {
     tf_handle::TwoLevel& tl = h->tl;
-    return TPArgs{tl.agg_ptr, tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_ptr, tl.pass_rows, tl.first_pass, tl.agg_order,
-                  tl.pcoef_p, tl.coff, tl.krank, tl.dcinv, tl.dcinv64, reinterpret_cast<const __half2*>(tl.Ainv),
-                  tl.Ainv64, tl.seg_aoff64, tl.seg_coarse, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg};
+    return TPArgs{tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_rows, reinterpret_cast<const TPRec*>(tl.wg_rec),
+                  tl.pcoef_p, tl.dcinv, tl.dcinv64, reinterpret_cast<const __half2*>(tl.Ainv),
+                  tl.Ainv64, tl.seg_aoff64, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg};
 }
 
 template <int P, bool INIT>
