@@ -268,6 +268,8 @@ def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
            "krylov_iterations_per_nr_iteration": float(sum(st["krylov_iterations"] for st in stats)
                                                        / max(1, sum(st["nr_iterations"] for st in stats))),
            "converged_steps": int(sum(st["converged"] for st in stats)),
+           "host_waits_per_nr_iteration": float(sum(st["host_waits"] for st in stats)
+                                                / max(1, sum(st["nr_iterations"] for st in stats))),
            "two_level": bool(stats[-1]["two_level"]),
            "tiling": {"workgroup": wg, "tile_blocks": cap, "tiles": tiles},
            "one_handle_per_disc": {"value": len(ts) / seq, "ms_per_step": 1e3 * seq,
@@ -375,6 +377,7 @@ def main():
                                                   for k in range(int(max(nr)) if len(nr) else 0) if k < 4],
         "converged_steps": int(sum(st["converged"] for st in stats)),
         "krylov_fallbacks": int(sum(st["krylov_fallbacks"] for st in stats)),
+        "host_waits_per_nr_iteration": float(sum(st["host_waits"] for st in stats) / max(1, np.sum(nr))),
         "ms_per_step_by_phase": {k: float(np.mean([st[k] for st in stats]))
                                  for k in ("ms_grow", "ms_assemble", "ms_solve", "ms_total")},
         "packing": {"pairs_per_step_max": int(max(st["packing_pairs"] for st in stats)),

@@ -122,6 +122,13 @@ typedef struct tf_solver_opts {
                                   with FP64 Jacobian values (the early, large updates run in the
                                   reduced precision, the last ones converge as in FP64); default 1,
                                   < 0: the reduced precision in every iteration                  */
+    int32_t nr_graph;          /* 1 (default, with use_graph): a Newton iteration whose solve has a
+                                  prediction from the previous timestep runs assembly, warm start,
+                                  solve, update and convergence test on the device and the host
+                                  reads one report per iteration (a solve that is not finished by
+                                  then continues with host checks); 2: the same, the iteration's
+                                  launches replayed from one captured hipGraph; 0: the host checks
+                                  the assembly status and every Krylov round                     */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {
@@ -139,6 +146,7 @@ typedef struct tf_step_stats {
                                     BiCGSTAB fallback) with a residual at most 1e-2 of the right-hand
                                     side: taken as inexact Newton steps.  A solve that ends above
                                     that ends its tissue's step: tf_nr_step returns TF_KRYLOV_STALL */
+    int32_t host_waits;          /* times the host waited for the device during the NR loop       */
 } tf_step_stats;
 
 /* ---- lifetime --------------------------------------------------------------------------- */

@@ -84,3 +84,25 @@ def test_inverted_prism_stops_the_step(prec):
     assert err.value.code == L.TF_NEG_JACOBIAN
     assert "det Fe" in str(err.value)
     assert np.array_equal(s.positions, before)
+
+
+@pytest.mark.parametrize("prec", [1, 0], ids=["two_level", "block_jacobi"])
+def test_one_read_per_newton_iteration_is_bitwise_neutral(prec):
+    """With a prediction from the previous timestep, a Newton iteration runs its assembly, warm
+    start, solve, update and convergence test on the device behind one gate, and the host reads one
+    report per iteration (nr_graph = 1).  The iterates are those of the host-checked loop
+    (nr_graph = 0), bit for bit, and from the third timestep on the host waits once per Newton
+    iteration, or a little more where a prediction falls short."""
+    t = _ellipse()
+    host, dev = _solver(t, nr_graph=0, preconditioner=prec), _solver(t, nr_graph=1, preconditioner=prec)
+    waits = nr = 0
+    for step in range(6):
+        a, b = host.runOneStep(t.dt), dev.runOneStep(t.dt)
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert (a["nr_iterations"], a["krylov_iterations"]) == (b["nr_iterations"], b["krylov_iterations"])
+        assert np.array_equal(host.positions, dev.positions), step
+        assert a["host_waits"] > a["nr_iterations"]
+        if step >= 2:
+            waits += b["host_waits"]
+            nr += b["nr_iterations"]
+    assert waits <= 1.5 * nr, (waits, nr)

@@ -93,6 +93,8 @@ void free_all(tf_handle* h)
     if (h->tile_info) (void)hipFree(h->tile_info);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->hp_rtol2) (void)hipHostFree(h->hp_rtol2);
+    if (h->nr_rep) (void)hipFree(h->nr_rep);
+    if (h->hp_nr_rep) (void)hipHostFree(h->hp_nr_rep);
     drop_krylov_graphs(h);
     for (KernelTimer* t : {&h->t_elem, &h->t_gather, &h->t_spmv}) {
         for (hipEvent_t e : t->pending) (void)hipEventDestroy(e);
@@ -132,6 +134,50 @@ __global__ void k_zero_int(int32_t* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// The Newton iteration's gate (one workgroup), after the first round of its linear solve: a
+// snapshot of every tissue's Krylov scalars and of the assembly's status into the report, and
+// ok = the assembly found no inverted or non-finite Gauss point, and every active tissue's solve
+// converged, or stalled with a usable residual, on a finite right-hand side.  k_update_uk and
+// k_nr_finish apply the Newton update only when ok; otherwise the host finishes the solve.
+__global__ __launch_bounds__(256) void k_nr_gate(const SegScalars* __restrict__ seg, int S,
+                                                 const int32_t* __restrict__ status, NrReport* __restrict__ rep)
+{
+    __shared__ int fl[3];
+    if (threadIdx.x < 3) fl[threadIdx.x] = 0;
+    __syncthreads();
+    SegScalars* rs = nr_report_seg(rep);
+    int unf = 0, unu = 0, nan = 0;
+    for (int s = threadIdx.x; s < S; s += blockDim.x) {
+        const SegScalars q = seg[s];
+        rs[s] = q;
+        if (!q.active) continue;
+        unf |= !(q.converged || q.stalled);
+        unu |= q.stalled && solve_unusable(q.rnorm2, q.bnorm2);
+        nan |= !isfinite(q.bnorm2);
+    }
+    if (unf) atomicOr(&fl[0], 1);
+    if (unu) atomicOr(&fl[1], 1);
+    if (nan) atomicOr(&fl[2], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; ++k) rep->status[k] = status[k];
+        rep->unfinished = fl[0];
+        rep->unusable = fl[1];
+        rep->nan = fl[2];
+        rep->ok = status[0] == 0 && status[3] == 0 && !fl[0] && !fl[1] && !fl[2];
+    }
+}
+
+// After the gated update: every tissue's ||dU|| into the report and, when the gate was open, the
+// Newton convergence test (k_seg_nr_check)
+__global__ void k_nr_finish(SegScalars* seg, const double* norm, double tol, int S, NrReport* rep)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    nr_report_norm(rep, S)[s] = norm[s];
+    if (rep->ok && seg[s].active && norm[s] <= tol) seg[s].active = 0;
 }
 
 }  // namespace
@@ -500,6 +546,7 @@ void tf_default_options(tf_solver_opts* o)
     o->coarse_refresh = 100;
     o->column_blocks = 1;
     o->mixed_fp64_from = 1;
+    o->nr_graph = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -699,7 +746,7 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         (o->implicit_packing != 0 && o->implicit_packing != 1) ||
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
         o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1) ||
-        (o->precond_fp64 != 0 && o->precond_fp64 != 1))
+        (o->precond_fp64 != 0 && o->precond_fp64 != 1) || o->nr_graph < 0 || o->nr_graph > 2)
         return fail(h, TF_INVALID, "invalid solver options");
     const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates ||
                                 o->column_blocks != h->opts.column_blocks || o->precond_fp64 != h->opts.precond_fp64;
@@ -1086,6 +1133,22 @@ int tf_solve(tf_handle* h, double* dU, int32_t* krylov_iters, double* rel_residu
     TF_GUARD_END(h)
 }
 
+// The host's side of a Newton update: every iterating tissue's ||dU|| (norm), its iteration count
+// and its convergence (||dU|| <= nr_tol)
+static int update_uk_host(tf_handle* h, int iter_index, const double* norm)
+{
+    int st = TF_OK;
+    for (int s = 0; s < h->S; ++s) {
+        if (h->seg_converged[s] || h->seg_failed[s]) continue;
+        const double dn = norm[s];
+        h->seg_du[s] = dn;
+        h->seg_nr_iters[s] = iter_index + 1;
+        if (!std::isfinite(dn)) st = TF_NAN;
+        else if (dn <= h->opts.nr_tol) h->seg_converged[s] = 1;
+    }
+    return st;
+}
+
 static int update_uk_impl(tf_handle* h, int iter_index)
 {
     launch_seg_norm(h, h->du);
@@ -1095,16 +1158,8 @@ static int update_uk_impl(tf_handle* h, int iter_index)
     TF_HIP(hipGetLastError());
     d2h(h, h->h_seg_norm, h->seg_norm, h->S);
     TF_HIP(hipStreamSynchronize(h->stream));
-    int st = TF_OK;
-    for (int s = 0; s < h->S; ++s) {
-        if (h->seg_converged[s] || h->seg_failed[s]) continue;
-        const double dn = h->h_seg_norm[s];
-        h->seg_du[s] = dn;
-        h->seg_nr_iters[s] = iter_index + 1;
-        if (!std::isfinite(dn)) st = TF_NAN;
-        else if (dn <= h->opts.nr_tol) h->seg_converged[s] = 1;
-    }
-    return st;
+    h->host_waits += 1;
+    return update_uk_host(h, iter_index, h->h_seg_norm);
 }
 
 int tf_update_uk(tf_handle* h, double* du_norm, int32_t* converged)
@@ -1197,9 +1252,34 @@ static void forcing_observe(tf_handle* h, int iter)
     }
 }
 
+// The device side of a Newton iteration's end, enqueued right behind the first round of its solve:
+// the solution into du, slaves to their masters, ||dU|| per tissue, the gate, the gated update and
+// convergence test, and the report into pinned memory; then the one host wait of the iteration.
+static void nr_epilogue(tf_handle* h)
+{
+    if (!h->nr_rep) {
+        TF_HIP(hipMalloc((void**)&h->nr_rep, nr_report_bytes(h->S)));
+        TF_HIP(hipHostMalloc((void**)&h->hp_nr_rep, nr_report_bytes(h->S), hipHostMallocDefault));
+    }
+    launch_solution_to_du(h);
+    launch_bind_du(h);  // slaves follow their masters (SC/NewtonRaphsonSolver.cpp:397-399)
+    launch_seg_norm(h, h->du);
+    hipLaunchKernelGGL(k_nr_gate, dim3(1), dim3(256), 0, h->stream, h->seg, h->S, h->status, h->nr_rep);
+    launch_update_uk(h, h->nr_rep);
+    hipLaunchKernelGGL(k_nr_finish, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->seg_norm,
+                       h->opts.nr_tol, h->S, h->nr_rep);
+    TF_HIP(hipGetLastError());
+    TF_HIP(hipMemcpyAsync(h->hp_nr_rep, h->nr_rep, nr_report_bytes(h->S), hipMemcpyDeviceToHost, h->stream));
+    TF_HIP(hipStreamSynchronize(h->stream));
+    h->host_waits += 1;
+    std::memcpy(h->h_status, h->hp_nr_rep->status, 4 * sizeof(int32_t));
+    std::memcpy(h->h_seg, nr_report_seg(h->hp_nr_rep), sizeof(SegScalars) * h->S);
+}
+
 static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
 {
     auto t0 = std::chrono::steady_clock::now();
+    h->host_waits = 0;
     launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
     hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
@@ -1212,12 +1292,15 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     h->fallbacks = 0;
     h->tl_used = false;
     double ms_asm = 0, ms_sol = 0;
+    const int saved_kind = h->opts.krylov;
     // the per-solve state the Krylov set-up reads; reset on every exit from a solve
-    auto solve_state_off = [h]() {
+    auto solve_state_off = [h, saved_kind]() {
         h->seg_rtol_on = false;
         h->warm_guess = nullptr;
         h->krylov_call = -1;
         h->tl_pre_valid = false;
+        h->krun.pending = false;
+        h->opts.krylov = saved_kind;
     };
     int it = 0;
     struct NrIterGuard {  // the Newton iteration index the precision switch reads; -1 outside the loop
@@ -1228,20 +1311,39 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         auto ta = std::chrono::steady_clock::now();
         h->nr_it = it;
         assemble_impl(h, dt);
-        // the det Fe status comes back ahead of the warm start's read-back (one host wait)
-        TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        // One read per Newton iteration: a solve with a prediction runs its first round, the update
+        // and the convergence test behind one gate on the device (graph-replayed rounds, no
+        // per-launch timing).  Otherwise the det Fe status comes back ahead of the solve.
+        const bool one_read = h->opts.nr_graph != 0 && h->opts.use_graph != 0 && !h->t_spmv.enabled;
+        if (!one_read)
+            TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
         ms_asm += ms_since(ta);
         auto ts = std::chrono::steady_clock::now();
         const bool warm = h->opts.krylov_warm_start != 0 && it < TF_WARM_ITERS;
         int kst;
+        bool applied = false;  // the device applied this iteration's update (the gate was open)
         try {
             h->seg_rtol_on = h->opts.krylov_forcing != 0;
             if (h->seg_rtol_on) forcing_tolerances(h, it);
-            const bool wpend = warm && warm_start_begin(h, it);
+            h->warm_guess = warm && warm_start_begin(h, it) ? h->warm_x0 : nullptr;
             h->krylov_call = it;
-            krylov_pre_solve(h);  // device work that hides the host's wait for the read-back
-            if (wpend) h->warm_guess = warm_start_finish(h, it) ? h->warm_x0 : nullptr;
-            else TF_HIP(hipStreamSynchronize(h->stream));  // the status
+            krylov_pre_solve(h);
+            if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+            bool burst = false;
+            if (one_read) {
+                burst = krylov_solve_begin(h);
+                if (burst) {
+                    nr_epilogue(h);
+                } else {
+                    TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                          h->stream));
+                    TF_HIP(hipStreamSynchronize(h->stream));
+                    h->host_waits += 1;
+                }
+            } else {
+                TF_HIP(hipStreamSynchronize(h->stream));  // the status
+                h->host_waits += 1;
+            }
             if (h->h_status[3] > 0) {  // positions or growth state not finite
                 solve_state_off();
                 status = TF_NAN;
@@ -1254,7 +1356,13 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
                 h->err = std::to_string(h->h_status[0]) + " Gauss points with det Fe <= 0";
                 break;
             }
-            kst = solve_impl(h);
+            if (one_read) {
+                applied = burst && h->hp_nr_rep->ok;
+                kst = krylov_solve_end(h, burst, applied);
+            } else {
+                kst = krylov_solve(h);
+            }
+            if (!applied) launch_bind_du(h);  // slaves follow their masters (SC/NewtonRaphsonSolver.cpp:397-399)
         } catch (...) {
             solve_state_off();
             throw;
@@ -1267,7 +1375,8 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         }
         if (kst == TF_KRYLOV_STALL) {
             // a tissue whose stalled solve is unusable leaves the loop with the positions of its
-            // last update; a usable one goes on as an inexact Newton step, as do the others
+            // last update; a usable one goes on as an inexact Newton step, as do the others (an
+            // open gate means that every stall was usable)
             for (int s = 0; s < h->S; ++s) {
                 const SegScalars& q = h->h_seg[s];
                 if (!q.active || !q.stalled || h->seg_converged[s]) continue;
@@ -1278,13 +1387,15 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
                     stalls_accepted += 1;
                 }
             }
-            hipLaunchKernelGGL(k_seg_drop_stalled, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
-            TF_HIP(hipGetLastError());
+            if (!applied) {
+                hipLaunchKernelGGL(k_seg_drop_stalled, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+                TF_HIP(hipGetLastError());
+            }
         }
         if (warm && kst == TF_OK) warm_start_push(h, it);  // a guess for the next timestep's iteration it
         krylov_total += h->last_krylov_iters;
         if (stats && it < 4) stats->krylov_by_newton[it] = h->last_krylov_iters;
-        int ust = update_uk_impl(h, it);
+        int ust = applied ? update_uk_host(h, it, nr_report_norm(h->hp_nr_rep, h->S)) : update_uk_impl(h, it);
         if (h->opts.krylov_forcing) forcing_observe(h, it);
         ms_sol += ms_since(ts);
         if (ust == TF_NAN) { status = TF_NAN; h->err = "NaN in the Newton update"; break; }
@@ -1328,6 +1439,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         stats->ms_assemble = ms_asm;
         stats->ms_solve = ms_sol;
         stats->ms_total = ms_since(t0);
+        stats->host_waits = h->host_waits;
     }
     return status;
 }

@@ -595,9 +595,11 @@ __global__ void k_fill(double* dst, double v, int64_t n)
 
 __global__ __launch_bounds__(256) void k_update_uk(double* __restrict__ uk, const double* __restrict__ du,
                                                    const int32_t* __restrict__ tile_r0, const int32_t* __restrict__ tile_r1,
-                                                   const int32_t* __restrict__ tile_seg, const SegScalars* __restrict__ seg)
+                                                   const int32_t* __restrict__ tile_seg, const SegScalars* __restrict__ seg,
+                                                   const NrReport* __restrict__ gate)
 {
     const int t = blockIdx.x;
+    if (gate && !gate->ok) return;  // the host finishes the solve first
     if (!seg[tile_seg[t]].active) return;
     const int d0 = 3 * tile_r0[t], d1 = 3 * tile_r1[t];
     for (int d = d0 + threadIdx.x; d < d1; d += blockDim.x) uk[d] = uk[d] + du[d];
@@ -744,9 +746,9 @@ void launch_fill(tf_handle* h, double* dst, double v, int64_t n)
     TF_HIP(hipGetLastError());
 }
 
-void launch_update_uk(tf_handle* h)
+void launch_update_uk(tf_handle* h, const NrReport* gate)
 {
     hipLaunchKernelGGL(k_update_uk, dim3(h->T), dim3(256), 0, h->stream, h->uk, h->du, h->tile_r0, h->tile_r1,
-                       h->tile_seg, h->seg);
+                       h->tile_seg, h->seg, gate);
     TF_HIP(hipGetLastError());
 }

@@ -49,6 +49,21 @@ struct SegScalars {
     int32_t active;          // NR still iterating on this segment
 };
 
+// What the host reads once per Newton iteration (one copy): the assembly's status flags, the
+// gate of the device-side update (ok) and why it stayed shut, then S SegScalars (the tissues'
+// Krylov scalars after the solve's first round) and S doubles (||dU|| of every tissue).
+struct NrReport {
+    int32_t status[4];
+    int32_t ok, unfinished, unusable, nan;
+};
+static_assert(sizeof(NrReport) == 32, "NrReport layout");
+__host__ __device__ inline SegScalars* nr_report_seg(NrReport* r) { return reinterpret_cast<SegScalars*>(r + 1); }
+__host__ __device__ inline double* nr_report_norm(NrReport* r, int S)
+{
+    return reinterpret_cast<double*>(nr_report_seg(r) + S);
+}
+inline size_t nr_report_bytes(int S) { return sizeof(NrReport) + (size_t)S * (sizeof(SegScalars) + sizeof(double)); }
+
 // A linear solve that stopped short of its tolerance (iteration cap, stagnation, breakdown) with a
 // residual at most 1e-2 of the right-hand side -- the loosest tolerance the inexact-Newton forcing
 // asks for -- is a usable inexact Newton step.  Every stop records the squared residual of the
@@ -216,6 +231,9 @@ struct tf_handle {
     double *seg_norm = nullptr;       // [S] per-segment ||dU||
     double *h_seg_norm = nullptr;     // pinned
     int32_t *h_status = nullptr;      // pinned copy of status
+    NrReport* nr_rep = nullptr;       // device report of a Newton iteration (nr_report_bytes(S))
+    NrReport* hp_nr_rep = nullptr;    // pinned host copy
+    int32_t host_waits = 0;           // host waits for the device since the step began
 
     // Krylov work vectors [3N]
     double *kr = nullptr, *kr0 = nullptr, *kp = nullptr, *kph = nullptr, *kv = nullptr, *ks = nullptr,
@@ -238,17 +256,14 @@ struct tf_handle {
     // Krylov warm start: for Newton iteration k < TF_WARM_ITERS, the updates that iteration took
     // in the last TF_WARM_HIST timesteps (warm_hist[k][0] the newest).  A solve starts from their
     // combination that minimises ||g - M x|| per tissue (warm_x0; warm_guess points at it during
-    // the solve).  Work arrays: M z_j (warm_mz), per-tile and per-tissue Gram entries, coefficients.
+    // the solve).  Work arrays: per-tile and per-tissue Gram entries; the coefficients are solved
+    // on the device by every tile of the tissue.
     double* warm_hist[TF_WARM_ITERS][TF_WARM_HIST] = {};
     int32_t warm_count[TF_WARM_ITERS] = {};
     double* warm_x0 = nullptr;
     double* warm_part = nullptr;      // [TF_WARM_DOTS][T]
     int32_t warm_part_T = 0;
     double* warm_seg = nullptr;       // [S][TF_WARM_DOTS]
-    double* warm_y = nullptr;         // [S][TF_WARM_HIST]
-    double* hp_warm_dots = nullptr;   // [S][TF_WARM_DOTS] pinned read-back of warm_seg
-    double* hp_warm_y = nullptr;      // [S][TF_WARM_HIST] pinned staging of warm_y
-    hipEvent_t warm_ev = nullptr;     // recorded after the read-back of warm_seg
     const double* warm_guess = nullptr;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
     // Krylov iterations the solve of each Newton iteration took in the previous timestep: the
@@ -256,6 +271,15 @@ struct tf_handle {
     // convergence check (krylov_call: the Newton iteration being solved, -1 outside the loop)
     std::vector<int32_t> krylov_pred;
     int32_t krylov_call = -1;
+    int32_t burst_slack = 0;          // small chunks a predicted first burst adds to the prediction
+    // A solve between its start (set-up and first round enqueued) and its end (tf_krylov.hip):
+    // kind, chunk sizes, the rounds launched and the iterations they held.
+    struct KrylovRun {
+        bool pending = false, graph = false, predicted = false, timing = false, fp32 = false;
+        bool cg = false;              // opts.krylov is a CG kind (a stall may fall back to BiCGSTAB)
+        int32_t kind = 0, chunk = 0, small = 0, n_big = 1, n_small = 0, round = 0, done_iters = 0, call = -1;
+        int32_t last_launched = 0;    // iterations of the last round (its timing events are pending)
+    } krun;
     bool tl_used = false;             // a solve of the current step ran the two-level PCG
 
     // graphs
@@ -360,7 +384,7 @@ void launch_surfaces(tf_handle* h);
 void launch_copy(tf_handle* h, double* dst, const double* src, int64_t n);
 void launch_fill(tf_handle* h, double* dst, double v, int64_t n);
 void launch_seg_norm(tf_handle* h, const double* x);   // per-segment ||x||_2 into h->seg_norm
-void launch_update_uk(tf_handle* h);                    // uk += du on active segments
+void launch_update_uk(tf_handle* h, const NrReport* gate = nullptr);  // uk += du on active segments (if gate->ok)
 void build_tiles(tf_handle* h, int wg, int tile_cap);  // (re)partitions rows into tiles (tf_api.hip)
 bool tiling_fits(const tf_handle* h, int wg, int tile_cap);  // LDS of the single-kernel CG fits
 void drop_krylov_graphs(tf_handle* h);  // destroys the captured Krylov chunks
@@ -408,10 +432,17 @@ void launch_adj_split(tf_handle* h, const double* xbar, double* rhs, double* pas
 void launch_accumulate(tf_handle* h, const double* a, double* b, int64_t n);
 // ---- Krylov (tf_krylov.hip) -----------------------------------------------------------------
 int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
-// Warm start of the solve of Newton iteration k: x0 in warm_x0 from the kept updates; false when
-// there is none (or the least-squares problem is degenerate for every tissue).
-bool warm_start_begin(tf_handle* h, int k);   // Gram kernels and their read-back (event), no wait
-bool warm_start_finish(tf_handle* h, int k);  // waits for the read-back only; coefficients, x0 (no wait)
+// The same solve in two halves: krylov_solve_begin enqueues the set-up and the first round
+// without a host wait and returns true when that round is a predicted burst (a solve that may be
+// complete without a check); krylov_solve_end finishes it (seg_known: h_seg already holds the
+// tissues' scalars after that round; scattered: du already holds the solution) with the BiCGSTAB
+// fallback and the copy of the solution into du.
+bool krylov_solve_begin(tf_handle* h);
+int krylov_solve_end(tf_handle* h, bool seg_known, bool scattered);
+void launch_solution_to_du(tf_handle* h);  // du <- the running solve's x (node order)
+// Warm start of the solve of Newton iteration k: x0 in warm_x0 from the kept updates (zero on a
+// tissue whose least-squares problem is degenerate); false when no update is kept.
+bool warm_start_begin(tf_handle* h, int k);   // Gram kernels, coefficients and x0 on the device, no wait
 void krylov_pre_solve(tf_handle* h);          // enqueues the next solve's two-level prepare
 void warm_start_push(tf_handle* h, int k);    // keeps du as the newest update of iteration k
 void warm_start_free(tf_handle* h);

@@ -2110,16 +2110,83 @@ __global__ __launch_bounds__(64) void k_warm_seg(TileArgs ta, const double* __re
     if (threadIdx.x == 0) out[(size_t)sg * TF_WARM_DOTS + q] = sv;
 }
 
-// x0 = sum_j y[tissue][j] z_j on every row of the tile
-__global__ __launch_bounds__(WG) void k_warm_combine(TileArgs ta, WarmArgs wa, const double* __restrict__ y,
+// Coefficients y of the least-squares combination min ||g - sum_j y_j M z_j|| from the tissue's
+// Gram entries D (upper triangle of G = (Mz_i, Mz_j) row-major over TF_WARM_HIST updates, then
+// r = (Mz_i, g)): G is scaled to unit diagonal and factored by Cholesky in the order of the updates
+// (the newest first); a pivot that falls below 1e-12 drops its update (coefficient 0), so updates
+// nearly in the span of the ones already taken add nothing.  Every loop has a fixed trip count
+// (registers only); all zeros when G has a diagonal that is not positive and finite, or when a
+// coefficient is not finite.
+__device__ inline void warm_coeffs(int m, const double* __restrict__ D, double (&y)[TF_WARM_HIST])
+{
+    constexpr int H = TF_WARM_HIST;
+    double A[H][H], s[H], b[H], L[H][H], w[H], v[H];
+    bool take[H];
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int j = i; j < H; ++j, ++q) A[i][j] = A[j][i] = D[q];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const bool in = i < m;
+        ok = ok && (!in || (A[i][i] > 0.0 && isfinite(A[i][i])));
+        s[i] = in && A[i][i] > 0.0 ? 1.0 / sqrt(A[i][i]) : 0.0;
+        b[i] = s[i] * D[q + i];
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int j = 0; j < H; ++j) { A[i][j] *= s[i] * s[j]; L[i][j] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        double d = A[k][k];
+#pragma unroll
+        for (int j = 0; j < k; ++j) d -= L[k][j] * L[k][j];
+        take[k] = ok && k < m && d > 1e-12 && isfinite(d);
+        const double lkk = take[k] ? sqrt(d) : 1.0;
+        L[k][k] = lkk;
+#pragma unroll
+        for (int i = k + 1; i < H; ++i) {
+            double a = A[i][k];
+#pragma unroll
+            for (int j = 0; j < k; ++j) a -= L[i][j] * L[k][j];
+            L[i][k] = take[k] ? a / lkk : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {  // L w = b over the updates taken
+        double a = b[k];
+#pragma unroll
+        for (int j = 0; j < k; ++j) a -= L[k][j] * w[j];
+        w[k] = take[k] ? a / L[k][k] : 0.0;
+    }
+    bool fin = true;
+#pragma unroll
+    for (int k = H - 1; k >= 0; --k) {  // L^T v = w
+        double a = w[k];
+#pragma unroll
+        for (int i = k + 1; i < H; ++i) a -= L[i][k] * v[i];
+        v[k] = take[k] ? a / L[k][k] : 0.0;
+        y[k] = s[k] * v[k];
+        fin = fin && isfinite(y[k]);
+    }
+    if (!fin)
+#pragma unroll
+        for (int k = 0; k < H; ++k) y[k] = 0.0;
+}
+
+// x0 = sum_j y_j z_j on every row of the tile, y from the tissue's Gram entries (warm_coeffs, the
+// same in every tile of the tissue)
+__global__ __launch_bounds__(WG) void k_warm_combine(TileArgs ta, WarmArgs wa, const double* __restrict__ dots,
                                                      double* __restrict__ x0)
 {
     const int t = blockIdx.x;
     const int sg = ta.tile_seg[t];
     const int d0 = 3 * ta.tile_r0[t], d1 = 3 * ta.tile_r1[t];
     double c[TF_WARM_HIST];
-#pragma unroll
-    for (int j = 0; j < TF_WARM_HIST; ++j) c[j] = j < wa.m ? y[(size_t)sg * TF_WARM_HIST + j] : 0.0;
+    warm_coeffs(wa.m, dots + (size_t)sg * TF_WARM_DOTS, c);
     for (int d = d0 + threadIdx.x; d < d1; d += WG) {
         double v = 0.0;
 #pragma unroll
@@ -2127,71 +2194,6 @@ __global__ __launch_bounds__(WG) void k_warm_combine(TileArgs ta, WarmArgs wa, c
             if (j < wa.m) v += c[j] * wa.z[j][d];
         x0[d] = v;
     }
-}
-
-// Least squares min ||b - A y|| from the normal equations G y = r (G = A^T A, m <= 4): the
-// diagonally scaled G is diagonalised by cyclic Jacobi rotations and eigenvalues below 1e-12 of
-// the largest are dropped (nearly parallel updates).  Returns false when nothing is left.
-bool small_lstsq(int m, const double* Gup, const double* r, double* y)
-{
-    double G[TF_WARM_HIST][TF_WARM_HIST], V[TF_WARM_HIST][TF_WARM_HIST], s[TF_WARM_HIST];
-    int q = 0;
-    for (int i = 0; i < m; ++i)
-        for (int j = i; j < TF_WARM_HIST; ++j, ++q)
-            if (j < m) G[i][j] = G[j][i] = Gup[q];
-    for (int i = 0; i < m; ++i) {
-        if (!(G[i][i] > 0.0) || !std::isfinite(G[i][i])) return false;
-        s[i] = 1.0 / std::sqrt(G[i][i]);
-    }
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j) {
-            G[i][j] *= s[i] * s[j];
-            V[i][j] = i == j ? 1.0 : 0.0;
-        }
-    for (int sweep = 0; sweep < 30; ++sweep) {
-        double off = 0.0;
-        for (int i = 0; i < m; ++i)
-            for (int j = i + 1; j < m; ++j) off += G[i][j] * G[i][j];
-        if (off < 1e-30) break;
-        for (int p = 0; p < m; ++p)
-            for (int k = p + 1; k < m; ++k) {
-                if (G[p][k] == 0.0) continue;
-                const double th = 0.5 * std::atan2(2.0 * G[p][k], G[k][k] - G[p][p]);
-                const double c = std::cos(th), sn = std::sin(th);
-                for (int i = 0; i < m; ++i) {  // G <- J^T G J
-                    const double gp = G[i][p], gk = G[i][k];
-                    G[i][p] = c * gp - sn * gk;
-                    G[i][k] = sn * gp + c * gk;
-                }
-                for (int i = 0; i < m; ++i) {
-                    const double gp = G[p][i], gk = G[k][i];
-                    G[p][i] = c * gp - sn * gk;
-                    G[k][i] = sn * gp + c * gk;
-                }
-                for (int i = 0; i < m; ++i) {
-                    const double vp = V[i][p], vk = V[i][k];
-                    V[i][p] = c * vp - sn * vk;
-                    V[i][k] = sn * vp + c * vk;
-                }
-            }
-    }
-    double lmax = 0.0;
-    for (int i = 0; i < m; ++i) lmax = std::max(lmax, G[i][i]);
-    if (!(lmax > 0.0)) return false;
-    double w[TF_WARM_HIST] = {0.0, 0.0, 0.0, 0.0};
-    bool any = false;
-    for (int e = 0; e < m; ++e) {
-        if (!(G[e][e] > 1e-12 * lmax)) continue;
-        double proj = 0.0;
-        for (int i = 0; i < m; ++i) proj += V[i][e] * s[i] * r[i];
-        for (int i = 0; i < m; ++i) w[i] += V[i][e] * proj / G[e][e];
-        any = true;
-    }
-    for (int i = 0; i < m; ++i) {
-        y[i] = s[i] * w[i];
-        if (!std::isfinite(y[i])) return false;
-    }
-    return any;
 }
 
 }  // namespace
@@ -2204,10 +2206,6 @@ bool warm_start_begin(tf_handle* h, int k)
     if (!h->warm_x0) {
         TF_HIP(hipMalloc((void**)&h->warm_x0, n3 * sizeof(double)));
         TF_HIP(hipMalloc((void**)&h->warm_seg, sizeof(double) * TF_WARM_DOTS * h->S));
-        TF_HIP(hipMalloc((void**)&h->warm_y, sizeof(double) * TF_WARM_HIST * h->S));
-        TF_HIP(hipHostMalloc((void**)&h->hp_warm_dots, sizeof(double) * TF_WARM_DOTS * h->S, hipHostMallocDefault));
-        TF_HIP(hipHostMalloc((void**)&h->hp_warm_y, sizeof(double) * TF_WARM_HIST * h->S, hipHostMallocDefault));
-        TF_HIP(hipEventCreateWithFlags(&h->warm_ev, hipEventDisableTiming));
     }
     if (!h->warm_part || h->warm_part_T != h->T) {  // tiles change with the tiling
         if (h->warm_part) (void)hipFree(h->warm_part);
@@ -2227,39 +2225,7 @@ bool warm_start_begin(tf_handle* h, int k)
         hipLaunchKernelGGL(k_warm_gram<double>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val,
                            h->nb, h->warm_part);
     hipLaunchKernelGGL(k_warm_seg, dim3(h->S, TF_WARM_DOTS), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
-    TF_HIP(hipGetLastError());
-    TF_HIP(hipMemcpyAsync(h->hp_warm_dots, h->warm_seg, sizeof(double) * TF_WARM_DOTS * h->S, hipMemcpyDeviceToHost,
-                          h->stream));
-    TF_HIP(hipEventRecord(h->warm_ev, h->stream));
-    return true;
-}
-
-bool warm_start_finish(tf_handle* h, int k)
-{
-    const int m = h->warm_count[k];
-    TF_HIP(hipEventSynchronize(h->warm_ev));  // the read-back, not the work queued after it
-    bool any = false;
-    for (int s = 0; s < h->S; ++s) {
-        const double* D = h->hp_warm_dots + (size_t)s * TF_WARM_DOTS;
-        const double* rhs = D + TF_WARM_HIST * (TF_WARM_HIST + 1) / 2;
-        double ys[TF_WARM_HIST] = {0.0, 0.0, 0.0, 0.0};
-        double* y = h->hp_warm_y + (size_t)s * TF_WARM_HIST;
-        for (int j = 0; j < TF_WARM_HIST; ++j) y[j] = 0.0;
-        if (small_lstsq(m, D, rhs, ys)) {
-            for (int j = 0; j < m; ++j) y[j] = ys[j];
-            any = true;
-        }
-    }
-    if (!any) return false;
-    // pinned staging: the upload overlaps the host, and the previous one ended at the last solve's waits
-    TF_HIP(hipMemcpyAsync(h->warm_y, h->hp_warm_y, sizeof(double) * TF_WARM_HIST * h->S, hipMemcpyHostToDevice,
-                          h->stream));
-    TileArgs ta = tile_args(h);
-    WarmArgs wa{};
-    for (int j = 0; j < m; ++j) wa.z[j] = h->warm_hist[k][j];
-    wa.g = h->g;
-    wa.m = m;
-    hipLaunchKernelGGL(k_warm_combine, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_y, h->warm_x0);
+    hipLaunchKernelGGL(k_warm_combine, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_seg, h->warm_x0);
     TF_HIP(hipGetLastError());
     return true;
 }
@@ -2285,11 +2251,8 @@ void warm_start_free(tf_handle* h)
     for (int k = 0; k < TF_WARM_ITERS; ++k)
         for (int j = 0; j < TF_WARM_HIST; ++j)
             if (h->warm_hist[k][j]) { (void)hipFree(h->warm_hist[k][j]); h->warm_hist[k][j] = nullptr; }
-    for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg, &h->warm_y})
+    for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    for (double** p : {&h->hp_warm_dots, &h->hp_warm_y})
-        if (*p) { (void)hipHostFree(*p); *p = nullptr; }
-    if (h->warm_ev) { (void)hipEventDestroy(h->warm_ev); h->warm_ev = nullptr; }
     warm_start_clear(h);
 }
 
@@ -2862,6 +2825,7 @@ static void twolevel_refresh(tf_handle* h, const std::vector<uint8_t>& which)
     }
     TF_HIP(hipGetLastError());
     TF_HIP(hipStreamSynchronize(h->stream));
+    h->host_waits += 1;
     bool failed = false;
     for (int sg = 0; sg < S; ++sg) {
         if (!which[sg]) continue;
@@ -3099,13 +3063,82 @@ bool all_done(tf_handle* h)
 {
     TF_HIP(hipMemcpyAsync(h->h_seg, h->seg, sizeof(SegScalars) * h->S, hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipStreamSynchronize(h->stream));
+    h->host_waits += 1;
     for (int s = 0; s < h->S; ++s)
         if (!(h->h_seg[s].converged || h->h_seg[s].stalled)) return false;
     return true;
 }
 
+// The graph of iters iterations of the given kind (sz 0: a chunk, 1: a small chunk), captured on
+// first use and kept per value type and size.  The captured launches carry the iteration cap as an
+// argument: a chunk captured under another cap (a timing probe's) is not reused.
 template <typename VT>
-int solve_kind(tf_handle* h, const VT* val, int kind)
+hipGraphExec_t chunk_graph(tf_handle* h, const VT* val, int kind, int sz, int iters)
+{
+    KernelTimer& tm = h->t_spmv;
+    const int key = kind * 16 + (int)sizeof(VT) * 2;
+    const int gi = (sizeof(VT) == sizeof(double) ? 0 : 2) + sz;
+    hipGraphExec_t& graph_exec = h->krylov_graph[gi];
+    if (!graph_exec || h->krylov_graph_iters[gi] != iters || h->krylov_graph_kind[gi] != key ||
+        h->krylov_graph_maxit[gi] != h->opts.krylov_max_iter) {
+        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+        hipGraph_t gr;
+        TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        tm.capturing = true;
+        try {
+            enqueue_chunk<VT>(h, val, kind, iters);
+        } catch (...) {
+            tm.capturing = false;
+            (void)hipStreamEndCapture(h->stream, &gr);
+            throw;
+        }
+        tm.capturing = false;
+        TF_HIP(hipStreamEndCapture(h->stream, &gr));
+        TF_HIP(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
+        TF_HIP(hipGraphDestroy(gr));
+        h->krylov_graph_iters[gi] = iters;
+        h->krylov_graph_kind[gi] = key;
+        h->krylov_graph_maxit[gi] = h->opts.krylov_max_iter;
+    }
+    return graph_exec;
+}
+
+// One round of a running solve: the first (round 0) is the predicted burst, or one chunk without
+// a prediction; then small chunks, twice, after a predicted burst, then chunks.  No host wait.
+template <typename VT>
+void solve_round(tf_handle* h, const VT* val)
+{
+    tf_handle::KrylovRun& R = h->krun;
+    KernelTimer& tm = h->t_spmv;
+    int launched = R.chunk;
+    if (R.graph) {
+        if (R.round > 0) {
+            const bool fine = R.predicted && R.round <= 2;
+            R.n_big = fine ? 0 : 1;
+            R.n_small = fine ? 1 : 0;
+        }
+        hipGraphExec_t big = R.n_big ? chunk_graph<VT>(h, val, R.kind, 0, R.chunk) : nullptr;
+        hipGraphExec_t sm = R.n_small ? chunk_graph<VT>(h, val, R.kind, 1, R.small) : nullptr;
+        if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
+        for (int k = 0; k < R.n_big; ++k) TF_HIP(hipGraphLaunch(big, h->stream));
+        for (int k = 0; k < R.n_small; ++k) TF_HIP(hipGraphLaunch(sm, h->stream));
+        if (R.timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
+        launched = R.n_big * R.chunk + R.n_small * R.small;
+    } else {
+        if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
+        enqueue_chunk<VT>(h, val, R.kind, R.chunk);
+        TF_HIP(hipGetLastError());
+        if (R.timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
+    }
+    R.round += 1;
+    R.done_iters += launched;
+    R.last_launched = launched;
+}
+
+// The set-up of a solve of the given kind and its first round, enqueued without a host wait
+// (h->krun holds the solve until solve_end).
+template <typename VT>
+void solve_start(tf_handle* h, const VT* val, int kind)
 {
     TileArgs ta = tile_args(h);
     const dim3 grid(h->T), blk(WG);
@@ -3144,103 +3177,73 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     if (kind == KIND_TL)  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
         launch_tp_z<1, true>(h, tp_tile_args(h), tp_args(h), h->kp, h->kv);
     TF_HIP(hipGetLastError());
+    tf_handle::KrylovRun& R = h->krun;
+    R = tf_handle::KrylovRun{};
+    R.kind = kind;
+    R.fp32 = sizeof(VT) == sizeof(float);
     // A captured graph holds at most TF_GRAPH_NODES kernel launches (an even number of
     // iterations); a chunk between convergence checks is one replay of it, at most iters_per_sync
-    // iterations.  (The two-level PCG, three launches per iteration, checks every 10 iterations: a
-    // solve's last chunk then overshoots by fewer no-op launches than it costs in checks.)
+    // iterations.
     const int kpi = kind == KIND_TL ? 2 : kind == TF_KRYLOV_CG ? 2 : kind == TF_KRYLOV_BICGSTAB ? 5 : 1;
     int chunk = std::min(std::max(2, h->opts.iters_per_sync), std::max(2, TF_GRAPH_NODES / kpi));
     chunk -= chunk & 1;  // iterations come in parity pairs
+    R.chunk = chunk;
     // a small chunk: a quarter of the chunk (an even count), for the end of a predicted solve
-    const int small = std::max(2, (chunk / 4) & ~1);
-    const bool timing = h->t_spmv.enabled;
+    R.small = std::max(2, (chunk / 4) & ~1);
+    R.timing = h->t_spmv.enabled;
     // Graph replay.  While timing, the kinds with more than one kernel per iteration (other than
     // the two-level PCG) launch eagerly, each launch of the dominant kernel bracketed by events;
     // replays are timed whole.
-    const bool graph = h->opts.use_graph != 0 &&
-                       (!timing || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG || kind == KIND_TL);
-    int done_iters = 0;
+    R.graph = h->opts.use_graph != 0 &&
+              (!R.timing || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG || kind == KIND_TL);
     // First burst, replayed without a convergence check: the iterations this Newton iteration's
     // solve took last timestep plus the launch that sees convergence, rounded up to small chunks
-    // (chunks first).  The solve then continues in small chunks, twice, then in chunks; without a
-    // prediction, in chunks from the start.  A solve that converges inside a replay leaves the
-    // rest of its launches as no-ops.
-    int n_big = 1, n_small = 0;
-    const int call = h->krylov_call;
-    const bool predicted = graph && call >= 0 && call < (int)h->krylov_pred.size() && h->krylov_pred[call] >= 0;
-    if (predicted) {
-        const int target = (h->krylov_pred[call] + 1 + small - 1) / small * small;
-        n_big = target / chunk;
-        n_small = (target - n_big * chunk + small - 1) / small;
+    // (chunks first), plus burst_slack small chunks.  A solve that converges inside a replay
+    // leaves the rest of its launches as no-ops.
+    R.call = h->krylov_call;
+    R.predicted = R.graph && R.call >= 0 && R.call < (int)h->krylov_pred.size() && h->krylov_pred[R.call] >= 0;
+    if (R.predicted) {
+        const int target = ((h->krylov_pred[R.call] + 1 + R.small - 1) / R.small + h->burst_slack) * R.small;
+        R.n_big = target / R.chunk;
+        R.n_small = (target - R.n_big * R.chunk + R.small - 1) / R.small;
     }
-    int round = 0;
     KernelTimer& tm = h->t_spmv;
-    if (timing && !tm.sample[0]) {
+    if (R.timing && !tm.sample[0]) {
         TF_HIP(hipEventCreate(&tm.sample[0]));
         TF_HIP(hipEventCreate(&tm.sample[1]));
     }
-    auto chunk_graph = [&](int sz, int iters) -> hipGraphExec_t {
-        const int key = kind * 16 + (int)sizeof(VT) * 2;
-        const int gi = (sizeof(VT) == sizeof(double) ? 0 : 2) + sz;
-        hipGraphExec_t& graph_exec = h->krylov_graph[gi];
-        // the captured launches carry the iteration cap as an argument: a chunk captured under
-        // another cap (a timing probe's) is not reused
-        if (!graph_exec || h->krylov_graph_iters[gi] != iters || h->krylov_graph_kind[gi] != key ||
-            h->krylov_graph_maxit[gi] != h->opts.krylov_max_iter) {
-            if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-            hipGraph_t gr;
-            TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            tm.capturing = true;
-            try {
-                enqueue_chunk<VT>(h, val, kind, iters);
-            } catch (...) {
-                tm.capturing = false;
-                (void)hipStreamEndCapture(h->stream, &gr);
-                throw;
-            }
-            tm.capturing = false;
-            TF_HIP(hipStreamEndCapture(h->stream, &gr));
-            TF_HIP(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
-            TF_HIP(hipGraphDestroy(gr));
-            h->krylov_graph_iters[gi] = iters;
-            h->krylov_graph_kind[gi] = key;
-            h->krylov_graph_maxit[gi] = h->opts.krylov_max_iter;
-        }
-        return graph_exec;
-    };
+    R.pending = true;
+    solve_round<VT>(h, val);
+}
+
+// The end of the solve solve_start began: convergence checks (one host wait each; seg_known: the
+// host mirror of the tissues' scalars is already current for the rounds launched so far) and
+// further rounds until every tissue converged or stalled, then the solve's bookkeeping.
+template <typename VT>
+int solve_end(tf_handle* h, const VT* val, bool seg_known)
+{
+    tf_handle::KrylovRun& R = h->krun;
+    KernelTimer& tm = h->t_spmv;
     while (true) {
-        int launched = chunk;
-        if (graph) {
-            if (round > 0) {
-                const bool fine = predicted && round <= 2;
-                n_big = fine ? 0 : 1;
-                n_small = fine ? 1 : 0;
-            }
-            hipGraphExec_t big = n_big ? chunk_graph(0, chunk) : nullptr;
-            hipGraphExec_t sm = n_small ? chunk_graph(1, small) : nullptr;
-            if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
-            for (int k = 0; k < n_big; ++k) TF_HIP(hipGraphLaunch(big, h->stream));
-            for (int k = 0; k < n_small; ++k) TF_HIP(hipGraphLaunch(sm, h->stream));
-            if (timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
-            launched = n_big * chunk + n_small * small;
+        bool fin = true;
+        if (seg_known) {
+            for (int s = 0; s < h->S; ++s) fin = fin && (h->h_seg[s].converged || h->h_seg[s].stalled);
         } else {
-            if (timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
-            enqueue_chunk<VT>(h, val, kind, chunk);
-            TF_HIP(hipGetLastError());
-            if (timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
+            fin = all_done(h);
         }
-        ++round;
-        done_iters += launched;
-        const bool fin = all_done(h);
-        if (timing) {
+        seg_known = false;
+        if (R.timing && R.last_launched > 0) {
             float ms = 0.f;
             TF_HIP(hipEventElapsedTime(&ms, tm.sample[0], tm.sample[1]));
             tm.replay_ms += ms;
-            tm.replay_launches += launched;
+            tm.replay_launches += R.last_launched;
+            R.last_launched = 0;
         }
         if (fin) break;
-        if (done_iters > h->opts.krylov_max_iter + 2 * chunk) break;  // the cap marks the tissue stalled
+        if (R.done_iters > h->opts.krylov_max_iter + 2 * R.chunk) break;  // the cap marks the tissue stalled
+        solve_round<VT>(h, val);
     }
+    R.pending = false;
     timer_collect(h->t_spmv);
     int st = TF_OK;
     int maxit = 0;
@@ -3252,12 +3255,19 @@ int solve_kind(tf_handle* h, const VT* val, int kind)
     }
     if (nan) st = TF_NAN;  // a right-hand side that is not finite: no solver reaches a tolerance
     h->last_krylov_iters = maxit;
-    if (timing) tm.replay_iters += maxit;
-    if (call >= 0) {
-        if ((int)h->krylov_pred.size() <= call) h->krylov_pred.resize(call + 1, -1);
-        h->krylov_pred[call] = st == TF_OK ? maxit : -1;
+    if (R.timing) tm.replay_iters += maxit;
+    if (R.call >= 0) {
+        if ((int)h->krylov_pred.size() <= R.call) h->krylov_pred.resize(R.call + 1, -1);
+        h->krylov_pred[R.call] = st == TF_OK ? maxit : -1;
     }
     return st;
+}
+
+template <typename VT>
+int solve_kind(tf_handle* h, const VT* val, int kind)
+{
+    solve_start<VT>(h, val, kind);
+    return solve_end<VT>(h, val, false);
 }
 
 }  // namespace
@@ -3472,9 +3482,14 @@ void krylov_pre_solve(tf_handle* h)
 // Solve M du = g for every active tissue; du is written to h->du.
 int krylov_solve(tf_handle* h)
 {
+    krylov_solve_begin(h);
+    return krylov_solve_end(h, false, false);
+}
+
+bool krylov_solve_begin(tf_handle* h)
+{
     int kind = h->opts.krylov;
     const bool fp32 = fp32_values(h);
-    auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
     const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
     const bool pre = h->tl_pre_valid;
     h->tl_pre_valid = false;
@@ -3482,8 +3497,28 @@ int krylov_solve(tf_handle* h)
         kind = KIND_TL;
         h->tl_used = true;
     }
-    int st = run(kind);
-    bool kx_in_positions = kind == KIND_TL;
+    if (fp32) solve_start<float>(h, h->val32, kind);
+    else solve_start<double>(h, h->val, kind);
+    h->krun.cg = cg_kind;
+    return h->krun.predicted;
+}
+
+void launch_solution_to_du(tf_handle* h)
+{
+    if (h->krun.kind == KIND_TL)  // the two-level solve runs in aggregate order
+        hipLaunchKernelGGL(k_tp_scatter, dim3((3 * h->N + 255) / 256), dim3(256), 0, h->stream, h->tl.agg_nodes, h->N,
+                           h->kx, h->du);
+    else
+        launch_copy(h, h->du, h->kx, 3 * (int64_t)h->N);
+    TF_HIP(hipGetLastError());
+}
+
+int krylov_solve_end(tf_handle* h, bool seg_known, bool scattered)
+{
+    const int kind = h->krun.kind;
+    const bool fp32 = h->krun.fp32, cg_kind = h->krun.cg;
+    auto run = [&](int k) { return fp32 ? solve_kind<float>(h, h->val32, k) : solve_kind<double>(h, h->val, k); };
+    int st = fp32 ? solve_end<float>(h, h->val32, seg_known) : solve_end<double>(h, h->val, seg_known);
     if (kind == KIND_TL && st == TF_OK) twolevel_observe(h);
     if (kind == KIND_TL && st != TF_OK) h->tl.fresh = false;  // the next solve rebuilds the coarse operator
     bool unusable = false;  // a stalled tissue whose residual is above 1e-2 of its right-hand side
@@ -3499,20 +3534,15 @@ int krylov_solve(tf_handle* h)
         h->krylov_call = -1;  // the prediction belongs to the first method
         try {
             st = run(TF_KRYLOV_BICGSTAB);
-            kx_in_positions = false;
         } catch (...) {
             h->krylov_call = call;
             throw;
         }
         h->krylov_call = call;
         h->last_krylov_iters += cg_iters;
+        scattered = false;
     }
-    if (kx_in_positions)  // the two-level solve ran in aggregate order
-        hipLaunchKernelGGL(k_tp_scatter, dim3((3 * h->N + 255) / 256), dim3(256), 0, h->stream, h->tl.agg_nodes, h->N,
-                           h->kx, h->du);
-    else
-        launch_copy(h, h->du, h->kx, 3 * (int64_t)h->N);
-    TF_HIP(hipGetLastError());
+    if (!scattered) launch_solution_to_du(h);
     return st;
 }
 
