@@ -75,6 +75,9 @@ def parse():
     p.add_argument("--krylov", type=int, default=-1)
     p.add_argument("--matrix-fp32", action="store_true")
     p.add_argument("--no-graph", action="store_true")
+    p.add_argument("--nr-graph", type=int, default=None, choices=[0, 1, 2],
+                   help="Newton iterations: 0 host-checked, 1 one read each (default), 2 one read each, replayed "
+                        "from captured graphs")
     p.add_argument("--preconditioner", default="auto", choices=["auto", "block-jacobi", "two-level"])
     p.add_argument("--coarse-aggregates", type=int, default=0)
     p.add_argument("--coarse-refresh", type=int, default=100)
@@ -222,7 +225,7 @@ def trajectory(NewtonRaphsonSolver, tissue, opts, steps, tiling, ref=None):
     return xs if ref is None else err
 
 
-def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
+def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True, opts=None):
     """The 256-disc sweep (one tissue segment per disc in one handle; two-level preconditioner with
     one coarse block per disc) from t = 0 on a fresh handle, against the same discs run one handle
     at a time, plus a bounded CPU-oracle sample scaled by prism count.  A second handle runs the
@@ -232,13 +235,13 @@ def batched_leg(NewtonRaphsonSolver, steps, warmup, cpu=True):
     ts = configs.config4()
     dt = 60.0
     n_prisms = sum(t.n_elements for t in ts)
-    w = NewtonRaphsonSolver(ts)
+    w = NewtonRaphsonSolver(ts, opts)
     for _ in range(warmup):
         w.runOneStep(dt)
     w.autotune(dt, n_iters=64)
     wg, cap, _ = w.tiling()
     w.close()
-    s = NewtonRaphsonSolver(ts)
+    s = NewtonRaphsonSolver(ts, opts)
     s.set_tiling(wg, cap)
     s.positions
     t0 = time.perf_counter()
@@ -310,6 +313,8 @@ def main():
         opts["matrix_precision"] = 1
     if a.no_graph:
         opts["use_graph"] = 0
+    if a.nr_graph is not None:
+        opts["nr_graph"] = a.nr_graph
     opts["preconditioner"] = {"auto": -1, "block-jacobi": 0, "two-level": 1}[a.preconditioner]
     opts["coarse_aggregates"] = a.coarse_aggregates
     opts["coarse_refresh"] = a.coarse_refresh
@@ -474,7 +479,8 @@ def main():
         adj.pop("rows")
         out["adjoint"] = adj
     if not a.no_batch:
-        out["batched"] = batched_leg(NewtonRaphsonSolver, a.batch_steps, 1, cpu=not a.no_cpu_baseline)
+        out["batched"] = batched_leg(NewtonRaphsonSolver, a.batch_steps, 1, cpu=not a.no_cpu_baseline,
+                                     opts=None if a.nr_graph is None else {"nr_graph": a.nr_graph})
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(workload_tissue(a.workload), dt,
                                            a.cpu_steps, a.cpu_runs)

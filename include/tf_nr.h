@@ -147,6 +147,7 @@ typedef struct tf_step_stats {
                                     side: taken as inexact Newton steps.  A solve that ends above
                                     that ends its tissue's step: tf_nr_step returns TF_KRYLOV_STALL */
     int32_t host_waits;          /* times the host waited for the device during the NR loop       */
+    int32_t nr_graph_builds;     /* Newton iterations captured into a new graph (nr_graph = 2)    */
 } tf_step_stats;
 
 /* ---- lifetime --------------------------------------------------------------------------- */

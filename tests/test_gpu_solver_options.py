@@ -106,3 +106,44 @@ def test_one_read_per_newton_iteration_is_bitwise_neutral(prec):
             waits += b["host_waits"]
             nr += b["nr_iterations"]
     assert waits <= 1.5 * nr, (waits, nr)
+
+
+@pytest.mark.parametrize("case", ["two_level", "block_jacobi", "fp32_first_iteration"])
+def test_newton_iteration_graphs_replay_the_one_read_iterates(case):
+    """nr_graph = 2 replays a Newton iteration (assembly, warm start, two-level set-up, the solve's
+    first burst, update, convergence test) from one captured hipGraph per shape of its launches.
+    The iterates equal those of the eagerly enqueued one-read path bit for bit (graphs captured on
+    first use, replayed afterwards), and the host reads what the eager path reads."""
+    t = _ellipse()
+    opts = {"preconditioner": 0} if case == "block_jacobi" else {"preconditioner": 1}
+    if case == "fp32_first_iteration":
+        opts.update(matrix_precision=1)
+    eager, graph = _solver(t, nr_graph=1, **opts), _solver(t, nr_graph=2, **opts)
+    builds = []
+    waits = nr = 0
+    for step in range(8):
+        a, b = eager.runOneStep(t.dt), graph.runOneStep(t.dt)
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert (a["nr_iterations"], a["krylov_iterations"]) == (b["nr_iterations"], b["krylov_iterations"])
+        assert np.array_equal(eager.positions, graph.positions), step
+        assert b["host_waits"] == a["host_waits"], step  # the same reads, from a graph or eager launches
+        builds.append(b["nr_graph_builds"])
+        if step >= 2:
+            waits += b["host_waits"]
+            nr += b["nr_iterations"]
+    assert builds[0] == 0 and sum(builds) > 0  # nothing to predict from in the first timestep
+    assert waits <= 1.75 * nr, (waits, nr)  # one read per iteration, more where a prediction falls short
+
+
+def test_newton_iteration_graphs_of_several_tissues():
+    """A handle of several tissues under nr_graph = 2: each tissue's positions equal those of the
+    eager one-read path bit for bit."""
+    ts = [configs.batch_disc(n, layers, E, 0.3, rate, seed)
+          for n, layers, E, rate, seed in [(600, 2, 800.0, 1.0, 1), (1500, 3, 2000.0, 2.5, 2), (900, 2, 4000.0, 0.7, 3)]]
+    from tissuefolding_paperversion_amd.solver import NewtonRaphsonSolver
+    eager = NewtonRaphsonSolver(ts, {"nr_graph": 1})
+    graph = NewtonRaphsonSolver(ts, {"nr_graph": 2})
+    for step in range(6):
+        a, b = eager.runOneStep(ts[0].dt), graph.runOneStep(ts[0].dt)
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert np.array_equal(eager.positions, graph.positions), step

@@ -73,7 +73,7 @@ class StepStats(C.Structure):
                 ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double),
                 ("krylov_by_newton", C.c_int32 * 4), ("two_level", C.c_int32), ("krylov_stalls", C.c_int32),
-                ("host_waits", C.c_int32)]
+                ("host_waits", C.c_int32), ("nr_graph_builds", C.c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -223,6 +223,8 @@ void drop_krylov_graphs(tf_handle* h)
             (void)hipGraphExecDestroy(h->krylov_graph[k]);
             h->krylov_graph[k] = nullptr;
         }
+    for (tf_handle::NrGraph& g : h->nr_graphs) (void)hipGraphExecDestroy(g.exec);
+    h->nr_graphs.clear();
 }
 
 // LDS of the single-kernel CG for a tiling: products (3 per block), two exchange rows, and the
@@ -1254,13 +1256,9 @@ static void forcing_observe(tf_handle* h, int iter)
 
 // The device side of a Newton iteration's end, enqueued right behind the first round of its solve:
 // the solution into du, slaves to their masters, ||dU|| per tissue, the gate, the gated update and
-// convergence test, and the report into pinned memory; then the one host wait of the iteration.
-static void nr_epilogue(tf_handle* h)
+// convergence test, and the report (nr_epilogue_read copies it to the host).
+static void nr_epilogue_launch(tf_handle* h)
 {
-    if (!h->nr_rep) {
-        TF_HIP(hipMalloc((void**)&h->nr_rep, nr_report_bytes(h->S)));
-        TF_HIP(hipHostMalloc((void**)&h->hp_nr_rep, nr_report_bytes(h->S), hipHostMallocDefault));
-    }
     launch_solution_to_du(h);
     launch_bind_du(h);  // slaves follow their masters (SC/NewtonRaphsonSolver.cpp:397-399)
     launch_seg_norm(h, h->du);
@@ -1269,6 +1267,12 @@ static void nr_epilogue(tf_handle* h)
     hipLaunchKernelGGL(k_nr_finish, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->seg_norm,
                        h->opts.nr_tol, h->S, h->nr_rep);
     TF_HIP(hipGetLastError());
+}
+
+// The one host wait of a Newton iteration: the report, then the host mirrors of the status and of
+// the tissues' Krylov scalars
+static void nr_epilogue_read(tf_handle* h)
+{
     TF_HIP(hipMemcpyAsync(h->hp_nr_rep, h->nr_rep, nr_report_bytes(h->S), hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipStreamSynchronize(h->stream));
     h->host_waits += 1;
@@ -1276,10 +1280,75 @@ static void nr_epilogue(tf_handle* h)
     std::memcpy(h->h_seg, nr_report_seg(h->hp_nr_rep), sizeof(SegScalars) * h->S);
 }
 
+static void nr_report_alloc(tf_handle* h)
+{
+    if (h->nr_rep) return;
+    TF_HIP(hipMalloc((void**)&h->nr_rep, nr_report_bytes(h->S)));
+    TF_HIP(hipHostMalloc((void**)&h->hp_nr_rep, nr_report_bytes(h->S), hipHostMallocDefault));
+}
+
+// Whether Newton iteration it (krylov_call set) can run from a graph: no per-launch timing, no
+// packing pairs (their lists change size between timesteps), a first burst predicted from the
+// previous timestep, and no coarse build or rebuild due (those read back a flag).
+static bool nr_graph_ready(tf_handle* h, int it)
+{
+    if (h->t_elem.enabled || h->t_gather.enabled || h->t_spmv.enabled || h->pk.n > 0) return false;
+    if (it >= (int)h->krylov_pred.size() || h->krylov_pred[it] < 0) return false;
+    return !krylov_pre_solve_rebuilds(h);
+}
+
+// The device work of Newton iteration it as the one-read path enqueues it, the solve's first round
+// as kernels (for the capture); krylov_solve_plan has sized that round.
+static void nr_iteration_launch(tf_handle* h, double dt, int it, bool warm, bool tl_prep)
+{
+    assemble_impl(h, dt);
+    if (warm) warm_start_launch(h, it);
+    if (tl_prep) krylov_pre_launch(h);
+    krylov_solve_launch(h, true);
+    nr_epilogue_launch(h);
+}
+
+// Replays the graph of Newton iteration it for the shape of its launches, capturing and
+// instantiating it on first use (at most 32 kept, the least recently used dropped)
+static void nr_graph_run(tf_handle* h, double dt, int it, bool warm, bool tl_prep)
+{
+    const tf_handle::KrylovRun& R = h->krun;
+    const tf_handle::NrGraphKey key{it, warm ? h->warm_count[it] : 0, R.kind, R.fp32 ? 1 : 0, tangent_precision_now(h),
+                                    R.n_big, R.n_small, h->seg_rtol_on ? 1 : 0, dt, h->ablated};
+    h->nr_graph_clock += 1;
+    hipGraphExec_t exec = nullptr;
+    for (tf_handle::NrGraph& g : h->nr_graphs)
+        if (g.key == key) { exec = g.exec; g.used = h->nr_graph_clock; break; }
+    if (!exec) {
+        if (h->nr_graphs.size() >= 32) {
+            auto lru = std::min_element(h->nr_graphs.begin(), h->nr_graphs.end(),
+                                        [](const tf_handle::NrGraph& a, const tf_handle::NrGraph& b) { return a.used < b.used; });
+            (void)hipGraphExecDestroy(lru->exec);
+            h->nr_graphs.erase(lru);
+        }
+        hipGraph_t gr;
+        TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            nr_iteration_launch(h, dt, it, warm, tl_prep);
+        } catch (...) {
+            (void)hipStreamEndCapture(h->stream, &gr);
+            throw;
+        }
+        TF_HIP(hipStreamEndCapture(h->stream, &gr));
+        const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        TF_HIP(e);
+        h->nr_graphs.push_back(tf_handle::NrGraph{key, exec, h->nr_graph_clock});
+        h->nr_graph_builds += 1;
+    }
+    TF_HIP(hipGraphLaunch(exec, h->stream));
+}
+
 static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
 {
     auto t0 = std::chrono::steady_clock::now();
     h->host_waits = 0;
+    h->nr_graph_builds = 0;
     launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
     hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
@@ -1308,41 +1377,61 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         ~NrIterGuard() { h->nr_it = -1; }
     } nr_guard{h};
     while (true) {
-        auto ta = std::chrono::steady_clock::now();
         h->nr_it = it;
-        assemble_impl(h, dt);
         // One read per Newton iteration: a solve with a prediction runs its first round, the update
         // and the convergence test behind one gate on the device (graph-replayed rounds, no
-        // per-launch timing).  Otherwise the det Fe status comes back ahead of the solve.
+        // per-launch timing), from one captured graph under nr_graph = 2.  Otherwise the det Fe
+        // status comes back ahead of the solve.
         const bool one_read = h->opts.nr_graph != 0 && h->opts.use_graph != 0 && !h->t_spmv.enabled;
-        if (!one_read)
-            TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
-        ms_asm += ms_since(ta);
+        // a first burst that falls short costs a host wait and the epilogue enqueued behind it; a
+        // small chunk of slack costs a few no-op launches once the solve has converged
+        h->burst_slack = one_read ? TF_ONE_READ_SLACK : 0;
         auto ts = std::chrono::steady_clock::now();
+        double it_asm = 0.0;  // host time enqueueing the assembly (eager launches)
         const bool warm = h->opts.krylov_warm_start != 0 && it < TF_WARM_ITERS;
         int kst;
         bool applied = false;  // the device applied this iteration's update (the gate was open)
         try {
             h->seg_rtol_on = h->opts.krylov_forcing != 0;
             if (h->seg_rtol_on) forcing_tolerances(h, it);
-            h->warm_guess = warm && warm_start_begin(h, it) ? h->warm_x0 : nullptr;
             h->krylov_call = it;
-            krylov_pre_solve(h);
-            if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
             bool burst = false;
-            if (one_read) {
-                burst = krylov_solve_begin(h);
-                if (burst) {
-                    nr_epilogue(h);
+            if (one_read) nr_report_alloc(h);
+            if (one_read && h->opts.nr_graph == 2 && nr_graph_ready(h, it)) {
+                // the host's plan of the iteration (warm start, two-level prepare, the solve's first
+                // burst; no launches), then its launches from one graph
+                const bool w = warm && warm_start_begin(h, it, false);
+                h->warm_guess = w ? h->warm_x0 : nullptr;
+                krylov_pre_solve(h, false);
+                const bool tl_prep = h->tl_pre_valid && h->tl_pre_ok;
+                if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+                burst = krylov_solve_plan(h);
+                if (!burst) throw std::logic_error("a Newton-iteration graph needs a predicted first burst");
+                nr_graph_run(h, dt, it, w, tl_prep);
+                nr_epilogue_read(h);
+            } else {
+                auto ta = std::chrono::steady_clock::now();
+                assemble_impl(h, dt);
+                it_asm = ms_since(ta);
+                h->warm_guess = warm && warm_start_begin(h, it) ? h->warm_x0 : nullptr;
+                krylov_pre_solve(h);
+                if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+                if (one_read) {
+                    burst = krylov_solve_begin(h);
+                    if (burst) {
+                        nr_epilogue_launch(h);
+                        nr_epilogue_read(h);
+                    } else {
+                        TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                              h->stream));
+                        TF_HIP(hipStreamSynchronize(h->stream));
+                        h->host_waits += 1;
+                    }
                 } else {
-                    TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                          h->stream));
-                    TF_HIP(hipStreamSynchronize(h->stream));
+                    TF_HIP(hipMemcpyAsync(h->h_status, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+                    TF_HIP(hipStreamSynchronize(h->stream));  // the status
                     h->host_waits += 1;
                 }
-            } else {
-                TF_HIP(hipStreamSynchronize(h->stream));  // the status
-                h->host_waits += 1;
             }
             if (h->h_status[3] > 0) {  // positions or growth state not finite
                 solve_state_off();
@@ -1368,6 +1457,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
             throw;
         }
         solve_state_off();
+        h->burst_slack = 0;
         if (kst == TF_NAN) {  // a residual that is not finite: nothing to update
             status = TF_NAN;
             h->err = "NaN or Inf in the residual";
@@ -1397,7 +1487,8 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         if (stats && it < 4) stats->krylov_by_newton[it] = h->last_krylov_iters;
         int ust = applied ? update_uk_host(h, it, nr_report_norm(h->hp_nr_rep, h->S)) : update_uk_impl(h, it);
         if (h->opts.krylov_forcing) forcing_observe(h, it);
-        ms_sol += ms_since(ts);
+        ms_asm += it_asm;
+        ms_sol += ms_since(ts) - it_asm;
         if (ust == TF_NAN) { status = TF_NAN; h->err = "NaN in the Newton update"; break; }
         ++it;
         bool all = true;
@@ -1440,6 +1531,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         stats->ms_solve = ms_sol;
         stats->ms_total = ms_since(t0);
         stats->host_waits = h->host_waits;
+        stats->nr_graph_builds = h->nr_graph_builds;
     }
     return status;
 }

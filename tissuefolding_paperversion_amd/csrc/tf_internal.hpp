@@ -234,6 +234,28 @@ struct tf_handle {
     NrReport* nr_rep = nullptr;       // device report of a Newton iteration (nr_report_bytes(S))
     NrReport* hp_nr_rep = nullptr;    // pinned host copy
     int32_t host_waits = 0;           // host waits for the device since the step began
+    // Whole Newton iterations captured as graphs (nr_graph = 2), one per shape of their launches:
+    // Newton iteration, kept warm-start updates, solver kind, value and tangent precision, first
+    // burst, dt, forcing, ablated-node table.  Dropped with the Krylov chunks (pattern, tiling,
+    // options or coarse space changed).
+    struct NrGraphKey {
+        int32_t it, m, kind, fp32, tangent, n_big, n_small, rtol_on;
+        double dt;
+        const void* ablated;
+        bool operator==(const NrGraphKey& o) const
+        {
+            return it == o.it && m == o.m && kind == o.kind && fp32 == o.fp32 && tangent == o.tangent &&
+                   n_big == o.n_big && n_small == o.n_small && rtol_on == o.rtol_on && dt == o.dt && ablated == o.ablated;
+        }
+    };
+    struct NrGraph {
+        NrGraphKey key;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<NrGraph> nr_graphs;
+    uint64_t nr_graph_clock = 0;
+    int32_t nr_graph_builds = 0;      // graphs captured since the step began
 
     // Krylov work vectors [3N]
     double *kr = nullptr, *kr0 = nullptr, *kp = nullptr, *kph = nullptr, *kv = nullptr, *ks = nullptr,
@@ -264,6 +286,8 @@ struct tf_handle {
     double* warm_part = nullptr;      // [TF_WARM_DOTS][T]
     int32_t warm_part_T = 0;
     double* warm_seg = nullptr;       // [S][TF_WARM_DOTS]
+    double** warm_tab = nullptr;      // [TF_WARM_ITERS][TF_WARM_HIST] device copy of warm_hist (kernel input)
+    double** hp_warm_tab = nullptr;   // pinned staging of it
     const double* warm_guess = nullptr;
     int32_t fallbacks = 0;            // CG -> BiCGSTAB fallbacks since the last reset
     // Krylov iterations the solve of each Newton iteration took in the previous timestep: the
@@ -439,11 +463,19 @@ int krylov_solve(tf_handle* h);  // solves M du = g; returns tf_status
 // fallback and the copy of the solution into du.
 bool krylov_solve_begin(tf_handle* h);
 int krylov_solve_end(tf_handle* h, bool seg_known, bool scattered);
+// krylov_solve_begin in two halves: the host's plan of the solve (kind, first round; no launch) and
+// its launches (direct: the first round's iterations enqueued as kernels, for a capture)
+bool krylov_solve_plan(tf_handle* h);
+void krylov_solve_launch(tf_handle* h, bool direct);
 void launch_solution_to_du(tf_handle* h);  // du <- the running solve's x (node order)
 // Warm start of the solve of Newton iteration k: x0 in warm_x0 from the kept updates (zero on a
 // tissue whose least-squares problem is degenerate); false when no update is kept.
-bool warm_start_begin(tf_handle* h, int k);   // Gram kernels, coefficients and x0 on the device, no wait
-void krylov_pre_solve(tf_handle* h);          // enqueues the next solve's two-level prepare
+bool warm_start_begin(tf_handle* h, int k, bool launch = true);  // Gram, coefficients, x0 on the device, no wait
+void warm_start_launch(tf_handle* h, int k);  // the kernels of warm_start_begin (buffers allocated by it)
+void krylov_pre_solve(tf_handle* h, bool launch = true);  // the next solve's two-level prepare
+void krylov_pre_launch(tf_handle* h);         // its per-solve kernels (two-level solves), when krylov_pre_solve
+                                              // did not launch them
+bool krylov_pre_solve_rebuilds(const tf_handle* h);  // krylov_pre_solve would build or rebuild the coarse space
 void warm_start_push(tf_handle* h, int k);    // keeps du as the newest update of iteration k
 void warm_start_free(tf_handle* h);
 void warm_start_clear(tf_handle* h);
@@ -459,6 +491,9 @@ bool twolevel_active(const tf_handle* h);  // the CG solves of this handle use t
 // prices under "spmv"), n_launches replayed back to back on the current Jacobian; 0 for BiCGSTAB
 double krylov_probe(tf_handle* h, int n_launches, int which = 0);  // which: 0 the SpMV kernel, 1 k_tp_z
 #define TF_TWO_LEVEL_MIN_NODES 4096       // TF_PRECOND_AUTO: two-level from this many nodes
+#ifndef TF_ONE_READ_SLACK
+#define TF_ONE_READ_SLACK 1               // small chunks a one-read iteration's first burst adds to its prediction
+#endif
 #define TF_GRAPH_NODES 32                 // kernel launches per captured Krylov graph (at most)
 void launch_spmv(tf_handle* h, const double* x, double* y);
 

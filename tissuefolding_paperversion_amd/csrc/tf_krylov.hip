@@ -2031,7 +2031,7 @@ TileArgs tile_args(tf_handle* h)
 
 // ---- warm start: least-squares combination of earlier updates ---------------------------------
 struct WarmArgs {
-    const double* z[TF_WARM_HIST];
+    const double* const* z;  // [TF_WARM_HIST] the kept updates (a device table, newest first)
     const double* g;
     int m;
 };
@@ -2198,7 +2198,7 @@ __global__ __launch_bounds__(WG) void k_warm_combine(TileArgs ta, WarmArgs wa, c
 
 }  // namespace
 
-bool warm_start_begin(tf_handle* h, int k)
+bool warm_start_begin(tf_handle* h, int k, bool launch)
 {
     const int m = h->warm_count[k];
     if (m <= 0) return false;
@@ -2212,11 +2212,17 @@ bool warm_start_begin(tf_handle* h, int k)
         TF_HIP(hipMalloc((void**)&h->warm_part, sizeof(double) * TF_WARM_DOTS * h->T));
         h->warm_part_T = h->T;
     }
+    if (launch) warm_start_launch(h, k);
+    return true;
+}
+
+void warm_start_launch(tf_handle* h, int k)
+{
     TileArgs ta = tile_args(h);
     WarmArgs wa{};
-    for (int j = 0; j < m; ++j) wa.z[j] = h->warm_hist[k][j];
+    wa.z = h->warm_tab + (size_t)k * TF_WARM_HIST;
     wa.g = h->g;
-    wa.m = m;
+    wa.m = h->warm_count[k];
     const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
     if (fp32_values(h))
         hipLaunchKernelGGL(k_warm_gram<float>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val32,
@@ -2227,18 +2233,28 @@ bool warm_start_begin(tf_handle* h, int k)
     hipLaunchKernelGGL(k_warm_seg, dim3(h->S, TF_WARM_DOTS), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
     hipLaunchKernelGGL(k_warm_combine, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_seg, h->warm_x0);
     TF_HIP(hipGetLastError());
-    return true;
 }
 
 void warm_start_push(tf_handle* h, int k)
 {
     const size_t n3 = 3 * (size_t)h->N;
+    if (!h->warm_tab) {
+        TF_HIP(hipMalloc((void**)&h->warm_tab, sizeof(double*) * TF_WARM_ITERS * TF_WARM_HIST));
+        TF_HIP(hipHostMalloc((void**)&h->hp_warm_tab, sizeof(double*) * TF_WARM_ITERS * TF_WARM_HIST,
+                             hipHostMallocDefault));
+    }
     double* last = h->warm_hist[k][TF_WARM_HIST - 1];
     if (!last) TF_HIP(hipMalloc((void**)&last, n3 * sizeof(double)));
     for (int j = TF_WARM_HIST - 1; j > 0; --j) h->warm_hist[k][j] = h->warm_hist[k][j - 1];
     h->warm_hist[k][0] = last;  // the oldest buffer is reused for the newest update
     launch_copy(h, last, h->du, (int64_t)n3);
     h->warm_count[k] = std::min(TF_WARM_HIST, h->warm_count[k] + 1);
+    // the device table the warm-start kernels read (pinned staging: row k is rewritten only at the
+    // next push of iteration k, a timestep later, after several host waits)
+    double** row = h->hp_warm_tab + (size_t)k * TF_WARM_HIST;
+    for (int j = 0; j < TF_WARM_HIST; ++j) row[j] = h->warm_hist[k][j];
+    TF_HIP(hipMemcpyAsync(h->warm_tab + (size_t)k * TF_WARM_HIST, row, sizeof(double*) * TF_WARM_HIST,
+                          hipMemcpyHostToDevice, h->stream));
 }
 
 void warm_start_clear(tf_handle* h)
@@ -2253,6 +2269,8 @@ void warm_start_free(tf_handle* h)
             if (h->warm_hist[k][j]) { (void)hipFree(h->warm_hist[k][j]); h->warm_hist[k][j] = nullptr; }
     for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (h->warm_tab) { (void)hipFree(h->warm_tab); h->warm_tab = nullptr; }
+    if (h->hp_warm_tab) { (void)hipHostFree(h->hp_warm_tab); h->hp_warm_tab = nullptr; }
     warm_start_clear(h);
 }
 
@@ -3103,42 +3121,56 @@ hipGraphExec_t chunk_graph(tf_handle* h, const VT* val, int kind, int sz, int it
     return graph_exec;
 }
 
-// One round of a running solve: the first (round 0) is the predicted burst, or one chunk without
-// a prediction; then small chunks, twice, after a predicted burst, then chunks.  No host wait.
-template <typename VT>
-void solve_round(tf_handle* h, const VT* val)
+// The next round of a running solve (host side): the first (round 0) is the predicted burst, or
+// one chunk without a prediction; then small chunks, twice, after a predicted burst, then chunks.
+void round_plan(tf_handle* h)
 {
     tf_handle::KrylovRun& R = h->krun;
-    KernelTimer& tm = h->t_spmv;
-    int launched = R.chunk;
-    if (R.graph) {
-        if (R.round > 0) {
-            const bool fine = R.predicted && R.round <= 2;
-            R.n_big = fine ? 0 : 1;
-            R.n_small = fine ? 1 : 0;
-        }
-        hipGraphExec_t big = R.n_big ? chunk_graph<VT>(h, val, R.kind, 0, R.chunk) : nullptr;
-        hipGraphExec_t sm = R.n_small ? chunk_graph<VT>(h, val, R.kind, 1, R.small) : nullptr;
-        if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
-        for (int k = 0; k < R.n_big; ++k) TF_HIP(hipGraphLaunch(big, h->stream));
-        for (int k = 0; k < R.n_small; ++k) TF_HIP(hipGraphLaunch(sm, h->stream));
-        if (R.timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
-        launched = R.n_big * R.chunk + R.n_small * R.small;
-    } else {
-        if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
-        enqueue_chunk<VT>(h, val, R.kind, R.chunk);
-        TF_HIP(hipGetLastError());
-        if (R.timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
+    if (R.graph && R.round > 0) {
+        const bool fine = R.predicted && R.round <= 2;
+        R.n_big = fine ? 0 : 1;
+        R.n_small = fine ? 1 : 0;
     }
+    const int launched = R.graph ? R.n_big * R.chunk + R.n_small * R.small : R.chunk;
     R.round += 1;
     R.done_iters += launched;
     R.last_launched = launched;
 }
 
-// The set-up of a solve of the given kind and its first round, enqueued without a host wait
-// (h->krun holds the solve until solve_end).
+// The launches of the round round_plan sized: graph replays of the captured chunks, or (direct:
+// inside the capture of a larger graph, or without use_graph) the iterations' kernels themselves.
 template <typename VT>
-void solve_start(tf_handle* h, const VT* val, int kind)
+void round_launch(tf_handle* h, const VT* val, bool direct)
+{
+    tf_handle::KrylovRun& R = h->krun;
+    KernelTimer& tm = h->t_spmv;
+    if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
+    if (R.graph && !direct) {
+        hipGraphExec_t big = R.n_big ? chunk_graph<VT>(h, val, R.kind, 0, R.chunk) : nullptr;
+        hipGraphExec_t sm = R.n_small ? chunk_graph<VT>(h, val, R.kind, 1, R.small) : nullptr;
+        for (int k = 0; k < R.n_big; ++k) TF_HIP(hipGraphLaunch(big, h->stream));
+        for (int k = 0; k < R.n_small; ++k) TF_HIP(hipGraphLaunch(sm, h->stream));
+    } else if (R.graph) {
+        for (int k = 0; k < R.n_big; ++k) enqueue_chunk<VT>(h, val, R.kind, R.chunk);
+        for (int k = 0; k < R.n_small; ++k) enqueue_chunk<VT>(h, val, R.kind, R.small);
+    } else {
+        enqueue_chunk<VT>(h, val, R.kind, R.chunk);
+    }
+    TF_HIP(hipGetLastError());
+    if (R.timing) TF_HIP(hipEventRecord(tm.sample[1], h->stream));
+}
+
+template <typename VT>
+void solve_round(tf_handle* h, const VT* val)
+{
+    round_plan(h);
+    round_launch<VT>(h, val, false);
+}
+
+// The set-up kernels of a solve of the given kind: x_0, r_0 and the first direction's ingredients,
+// the tissues' tolerances (k_seg_init).
+template <typename VT>
+void solve_launch_init(tf_handle* h, const VT* val, int kind)
 {
     TileArgs ta = tile_args(h);
     const dim3 grid(h->T), blk(WG);
@@ -3177,10 +3209,16 @@ void solve_start(tf_handle* h, const VT* val, int kind)
     if (kind == KIND_TL)  // z_0 = D^-1 r_0 + P Ac^-1 P^T r_0
         launch_tp_z<1, true>(h, tp_tile_args(h), tp_args(h), h->kp, h->kv);
     TF_HIP(hipGetLastError());
+}
+
+// The host side of a solve's start: kind, chunk sizes and the first burst (h->krun holds the solve
+// until solve_end); no launches.
+void solve_plan(tf_handle* h, int kind, bool fp32)
+{
     tf_handle::KrylovRun& R = h->krun;
     R = tf_handle::KrylovRun{};
     R.kind = kind;
-    R.fp32 = sizeof(VT) == sizeof(float);
+    R.fp32 = fp32;
     // A captured graph holds at most TF_GRAPH_NODES kernel launches (an even number of
     // iterations); a chunk between convergence checks is one replay of it, at most iters_per_sync
     // iterations.
@@ -3213,6 +3251,14 @@ void solve_start(tf_handle* h, const VT* val, int kind)
         TF_HIP(hipEventCreate(&tm.sample[1]));
     }
     R.pending = true;
+}
+
+// The set-up of a solve of the given kind and its first round, enqueued without a host wait.
+template <typename VT>
+void solve_start(tf_handle* h, const VT* val, int kind)
+{
+    solve_launch_init<VT>(h, val, kind);
+    solve_plan(h, kind, sizeof(VT) == sizeof(float));
     solve_round<VT>(h, val);
 }
 
@@ -3282,6 +3328,20 @@ bool twolevel_active(const tf_handle* h)
     return cg_kind && want;
 }
 
+// The per-solve part of the two-level preconditioner: the inverse column blocks of the current
+// Jacobian and its values in aggregate order (the arrays are allocated by twolevel_prepare).
+static void twolevel_launch_prep(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    launch_tl_dcol(h);
+    const dim3 grid((h->nb + 255) / 256), blk(256);
+    if (fp32_values(h))
+        hipLaunchKernelGGL(k_tp_permute<float>, grid, blk, 0, h->stream, tl.pblk, h->nb, h->val32, tl.val32_p);
+    else
+        hipLaunchKernelGGL(k_tp_permute<double>, grid, blk, 0, h->stream, tl.pblk, h->nb, h->val, tl.val_p);
+    TF_HIP(hipGetLastError());
+}
+
 // The coarse space follows the pattern and tiling.  A tissue's block of the coarse operator is
 // rebuilt at the first Newton iteration of a timestep once the tissue's Krylov iterations of
 // Newton iteration 1 (the largest solve) exceed 1.25x their count in the timestep of its last
@@ -3289,7 +3349,7 @@ bool twolevel_active(const tf_handle* h)
 // outside the Newton loop (timing runs keep the operator in place).  A coarse operator from an
 // earlier Jacobian is still symmetric positive definite, so it changes the iteration count, not
 // the solution.  False when the coarse operator is not positive definite (block Jacobi then).
-static bool twolevel_prepare(tf_handle* h, bool reuse = false)
+static bool twolevel_prepare(tf_handle* h, bool reuse = false, bool launch = true)
 {
     tf_handle::TwoLevel& tl = h->tl;
     if (!tl.built) twolevel_build(h);
@@ -3304,18 +3364,12 @@ static bool twolevel_prepare(tf_handle* h, bool reuse = false)
             if (tl.stale[sg]) { which[sg] = 1; any = true; }
     if (any) twolevel_refresh(h, which);
     if (!tl.failed) {
-        launch_tl_dcol(h);
-        // the values in aggregate order for the solve
-        const bool fp32 = fp32_values(h);
-        const dim3 grid((h->nb + 255) / 256), blk(256);
-        if (fp32) {
+        if (fp32_values(h)) {
             if (!tl.val32_p) tl.val32_p = tl_alloc<float>(9 * (size_t)h->nb);
-            hipLaunchKernelGGL(k_tp_permute<float>, grid, blk, 0, h->stream, tl.pblk, h->nb, h->val32, tl.val32_p);
-        } else {
-            if (!tl.val_p) tl.val_p = tl_alloc<double>(9 * (size_t)h->nb);
-            hipLaunchKernelGGL(k_tp_permute<double>, grid, blk, 0, h->stream, tl.pblk, h->nb, h->val, tl.val_p);
+        } else if (!tl.val_p) {
+            tl.val_p = tl_alloc<double>(9 * (size_t)h->nb);
         }
-        TF_HIP(hipGetLastError());
+        if (launch) twolevel_launch_prep(h);
     }
     return !tl.failed;
 }
@@ -3466,17 +3520,36 @@ double krylov_probe(tf_handle* h, int n_launches, int which)
     return ms;
 }
 
-// The two-level prepare of the next krylov_solve, enqueued early (behind the warm start's
-// read-back, so that the device has work while the host waits for it).
-void krylov_pre_solve(tf_handle* h)
+// The two-level prepare of the next krylov_solve (coarse rebuild when due; launch: also the per-solve
+// kernels, else krylov_pre_launch enqueues them)
+void krylov_pre_solve(tf_handle* h, bool launch)
 {
     h->tl_pre_valid = false;
     int kind = h->opts.krylov;
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
     if (!cg_kind || !twolevel_active(h)) return;
-    h->tl_pre_ok = twolevel_prepare(h);
+    h->tl_pre_ok = twolevel_prepare(h, false, launch);
     h->tl_pre_valid = true;
+}
+
+void krylov_pre_launch(tf_handle* h) { twolevel_launch_prep(h); }
+
+// Whether the next krylov_pre_solve of a Newton iteration would build the coarse space or rebuild a
+// coarse block (work with host waits of its own); no side effects.
+bool krylov_pre_solve_rebuilds(const tf_handle* h)
+{
+    int kind = h->opts.krylov;
+    if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
+    const bool cg_kind = kind == TF_KRYLOV_CG || kind == TF_KRYLOV_CG1 || kind == TF_KRYLOV_PIPECG;
+    if (!cg_kind || !twolevel_active(h)) return false;
+    const tf_handle::TwoLevel& tl = h->tl;
+    if (!tl.built || !tl.fresh || h->krylov_call < 0) return true;
+    if (h->krylov_call != 0) return false;
+    if (tl.age + 1 >= std::max(1, h->opts.coarse_refresh)) return true;
+    for (int sg = 0; sg < h->S; ++sg)
+        if (tl.stale[sg]) return true;
+    return false;
 }
 
 // Solve M du = g for every active tissue; du is written to h->du.
@@ -3486,7 +3559,7 @@ int krylov_solve(tf_handle* h)
     return krylov_solve_end(h, false, false);
 }
 
-bool krylov_solve_begin(tf_handle* h)
+bool krylov_solve_plan(tf_handle* h)
 {
     int kind = h->opts.krylov;
     const bool fp32 = fp32_values(h);
@@ -3497,10 +3570,29 @@ bool krylov_solve_begin(tf_handle* h)
         kind = KIND_TL;
         h->tl_used = true;
     }
-    if (fp32) solve_start<float>(h, h->val32, kind);
-    else solve_start<double>(h, h->val, kind);
+    solve_plan(h, kind, fp32);
     h->krun.cg = cg_kind;
+    round_plan(h);
     return h->krun.predicted;
+}
+
+void krylov_solve_launch(tf_handle* h, bool direct)
+{
+    const tf_handle::KrylovRun& R = h->krun;
+    if (R.fp32) {
+        solve_launch_init<float>(h, h->val32, R.kind);
+        round_launch<float>(h, h->val32, direct);
+    } else {
+        solve_launch_init<double>(h, h->val, R.kind);
+        round_launch<double>(h, h->val, direct);
+    }
+}
+
+bool krylov_solve_begin(tf_handle* h)
+{
+    const bool predicted = krylov_solve_plan(h);
+    krylov_solve_launch(h, false);
+    return predicted;
 }
 
 void launch_solution_to_du(tf_handle* h)
