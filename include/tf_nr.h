@@ -129,6 +129,12 @@ typedef struct tf_solver_opts {
                                   then continues with host checks); 2: the same, the iteration's
                                   launches replayed from one captured hipGraph; 0: the host checks
                                   the assembly status and every Krylov round                     */
+    int32_t tl_persistent;     /* TF_PRECOND_TWO_LEVEL: 1 (default) runs a solve of one tissue whose
+                                  aggregates fit one workgroup each as one cooperative launch that
+                                  keeps the aggregate's blocks and coarse rows in registers, with a
+                                  grid barrier after each half of an iteration (same iterates as
+                                  the two launches per iteration it replaces); 0: two launches per
+                                  iteration, replayed from graphs                                */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {
@@ -148,6 +154,7 @@ typedef struct tf_step_stats {
                                     that ends its tissue's step: tf_nr_step returns TF_KRYLOV_STALL */
     int32_t host_waits;          /* times the host waited for the device during the NR loop       */
     int32_t nr_graph_builds;     /* Newton iterations captured into a new graph (nr_graph = 2)    */
+    int32_t krylov_persistent;   /* linear solves run as one persistent launch (tl_persistent)     */
 } tf_step_stats;
 
 /* ---- lifetime --------------------------------------------------------------------------- */

@@ -231,3 +231,25 @@ def test_coarse_inverse_matches_a_dense_inverse(make):
     ref = np.linalg.inv(Ac)
     assert np.max(np.abs(Ai - ref)) <= 1e-10 * np.max(np.abs(ref))
     assert np.array_equal(Ai, Ai.T)
+
+
+@pytest.mark.parametrize("case", ["ellipse", "ellipse_fp64_storage", "config2"])
+def test_persistent_solve_equals_two_launches(case):
+    """tl_persistent runs a two-level solve as one cooperative launch (grid barriers between the
+    halves of an iteration, the coarse rows held in LDS) instead of two launches per iteration.  The
+    arithmetic is the same in the same order: positions, Newton and Krylov counts are equal bit for
+    bit over several timesteps."""
+    t = configs.config2() if case == "config2" else _ellipse()
+    opts = dict(TL)
+    if case == "ellipse_fp64_storage":
+        opts["precond_fp64"] = 1
+    steps = 3 if case == "config2" else 5
+    two, one = _solver(t, tl_persistent=0, **opts), _solver(t, tl_persistent=1, **opts)
+    for step in range(steps):
+        a, b = two.runOneStep(t.dt), one.runOneStep(t.dt)
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert a["two_level"] == 1 and b["two_level"] == 1
+        assert (a["nr_iterations"], a["krylov_iterations"], a["krylov_by_newton"]) == \
+               (b["nr_iterations"], b["krylov_iterations"], b["krylov_by_newton"]), step
+        assert np.array_equal(two.positions, one.positions), step
+        assert a["krylov_persistent"] == 0 and b["krylov_persistent"] == b["nr_iterations"]

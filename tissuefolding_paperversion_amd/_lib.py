@@ -57,7 +57,7 @@ class SolverOpts(C.Structure):
                 ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32),
                 ("implicit_packing", C.c_int32), ("preconditioner", C.c_int32), ("coarse_aggregates", C.c_int32),
                 ("coarse_refresh", C.c_int32), ("column_blocks", C.c_int32), ("precond_fp64", C.c_int32),
-                ("mixed_fp64_from", C.c_int32), ("nr_graph", C.c_int32)]
+                ("mixed_fp64_from", C.c_int32), ("nr_graph", C.c_int32), ("tl_persistent", C.c_int32)]
 
 
 class GrowthFunction(C.Structure):
@@ -73,7 +73,7 @@ class StepStats(C.Structure):
                 ("last_du_norm", C.c_double), ("ms_grow", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double),
                 ("krylov_by_newton", C.c_int32 * 4), ("two_level", C.c_int32), ("krylov_stalls", C.c_int32),
-                ("host_waits", C.c_int32), ("nr_graph_builds", C.c_int32)]
+                ("host_waits", C.c_int32), ("nr_graph_builds", C.c_int32), ("krylov_persistent", C.c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

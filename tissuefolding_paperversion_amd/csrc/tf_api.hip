@@ -549,6 +549,7 @@ void tf_default_options(tf_solver_opts* o)
     o->column_blocks = 1;
     o->mixed_fp64_from = 1;
     o->nr_graph = 1;
+    o->tl_persistent = 1;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -748,7 +749,8 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         (o->implicit_packing != 0 && o->implicit_packing != 1) ||
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
         o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1) ||
-        (o->precond_fp64 != 0 && o->precond_fp64 != 1) || o->nr_graph < 0 || o->nr_graph > 2)
+        (o->precond_fp64 != 0 && o->precond_fp64 != 1) || o->nr_graph < 0 || o->nr_graph > 2 ||
+        (o->tl_persistent != 0 && o->tl_persistent != 1))
         return fail(h, TF_INVALID, "invalid solver options");
     const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates ||
                                 o->column_blocks != h->opts.column_blocks || o->precond_fp64 != h->opts.precond_fp64;
@@ -1349,6 +1351,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
     auto t0 = std::chrono::steady_clock::now();
     h->host_waits = 0;
     h->nr_graph_builds = 0;
+    h->persist_solves = 0;
     launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
     hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
     std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
@@ -1405,7 +1408,14 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
                 krylov_pre_solve(h, false);
                 const bool tl_prep = h->tl_pre_valid && h->tl_pre_ok;
                 if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
-                burst = krylov_solve_plan(h);
+                h->no_persist = true;  // a cooperative launch cannot be captured
+                try {
+                    burst = krylov_solve_plan(h);
+                } catch (...) {
+                    h->no_persist = false;
+                    throw;
+                }
+                h->no_persist = false;
                 if (!burst) throw std::logic_error("a Newton-iteration graph needs a predicted first burst");
                 nr_graph_run(h, dt, it, w, tl_prep);
                 nr_epilogue_read(h);
@@ -1532,6 +1542,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
         stats->ms_total = ms_since(t0);
         stats->host_waits = h->host_waits;
         stats->nr_graph_builds = h->nr_graph_builds;
+        stats->krylov_persistent = h->persist_solves;
     }
     return status;
 }

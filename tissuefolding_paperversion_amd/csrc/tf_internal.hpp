@@ -256,6 +256,7 @@ struct tf_handle {
     std::vector<NrGraph> nr_graphs;
     uint64_t nr_graph_clock = 0;
     int32_t nr_graph_builds = 0;      // graphs captured since the step began
+    int32_t persist_solves = 0;       // solves run as one persistent launch since the step began
 
     // Krylov work vectors [3N]
     double *kr = nullptr, *kr0 = nullptr, *kp = nullptr, *kph = nullptr, *kv = nullptr, *ks = nullptr,
@@ -296,10 +297,13 @@ struct tf_handle {
     std::vector<int32_t> krylov_pred;
     int32_t krylov_call = -1;
     int32_t burst_slack = 0;          // small chunks a predicted first burst adds to the prediction
+    bool no_persist = false;          // plan solves without the persistent launch (inside a graph capture)
     // A solve between its start (set-up and first round enqueued) and its end (tf_krylov.hip):
     // kind, chunk sizes, the rounds launched and the iterations they held.
     struct KrylovRun {
         bool pending = false, graph = false, predicted = false, timing = false, fp32 = false;
+        bool persist = false;         // the two-level solve runs as one cooperative launch (k_tp_run)
+        bool persist_next = false;    // the next round_launch is that launch
         bool cg = false;              // opts.krylov is a CG kind (a stall may fall back to BiCGSTAB)
         int32_t kind = 0, chunk = 0, small = 0, n_big = 1, n_small = 0, round = 0, done_iters = 0, call = -1;
         int32_t last_launched = 0;    // iterations of the last round (its timing events are pending)
@@ -382,6 +386,13 @@ struct tf_handle {
         double *tpart = nullptr, *ptq = nullptr, *rcx = nullptr;  // [SL_COUNT][A], [nc], [2][nc]
         int32_t* tp_seg_ptr = nullptr;                      // [S+1] the aggregates as every tissue's "tiles"
         int32_t tp_max_blocks = 0;                          // most blocks of a pass (LDS products)
+        // the persistent solve (one cooperative launch per solve): eligibility, the column slot of
+        // every permuted block, every aggregate's neighbouring nodes, the grid-barrier words, and
+        // how many workgroups of each kernel instance the device holds at once (-1: not queried)
+        bool run_ok = false;
+        int32_t *pslot = nullptr, *halo_ptr = nullptr, *halo_pos = nullptr;
+        unsigned* bar = nullptr;
+        int32_t run_cap[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
         std::vector<int32_t> h_node_agg;
     } tl;
 

@@ -1133,6 +1133,413 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 
 static size_t tp_z_lds(int ldA, bool f64) { return (f64 ? sizeof(double) : sizeof(float)) * (size_t)ldA; }  // r_c
 
+// ---- the two-level PCG as one persistent launch --------------------------------------------------
+// One workgroup per aggregate, every workgroup resident at once (cooperative launch), runs the whole
+// solve: the halves of the iteration that k_tp_q and k_tp_z launch, each followed by a grid barrier.
+// What a solve does not change and a workgroup can hold -- the aggregate's rows of the coarse inverse
+// (LDS), its column-block inverses and rows of P (registers) -- is loaded once; the own rows of x, r,
+// z, p and q stay in registers; the Jacobian blocks are read again every iteration (registers and LDS
+// cannot hold them beside the rest at two workgroups per compute unit).  What other workgroups read (z and p of the nodes their blocks couple,
+// the partial sums, P^T q and P^T r) goes through memory with agent-scope loads and stores: the L2 of
+// an XCD is not coherent with the others', and those accesses bypass it.  The arithmetic is that of
+// k_tp_q and k_tp_z, in the same order, so the iterates are the same bit for bit.
+// Eligible: one tissue; every aggregate one pass (<= WG / 3 rows, <= TP_PF WG blocks, <= WG / 3L
+// columns) with at most TPR_MAXU own and neighbouring nodes; at least TPR_NG aggregates; a grid of
+// n_agg workgroups that the device holds at once.
+constexpr int TPR_MAXU = 192;           // own + neighbouring nodes of an aggregate (LDS staging)
+constexpr int TPR_NG = 8;               // barrier arrival groups: workgroup b arrives on word b % TPR_NG
+constexpr int TPR_LINE = 32;            // the barrier words 128 bytes apart
+constexpr unsigned TPR_SPIN = 1u << 22; // polls of one barrier wait before it gives up (abort)
+constexpr int TPR_WORDS = (TPR_NG + 2) * TPR_LINE;  // group words, the global word, the abort word
+
+// doubles of k_tp_run's LDS that the products of q = M p and then r_c share
+__host__ __device__ inline int tp_run_share(int ldA, bool f64)
+{
+    const int rc = f64 ? ldA : (ldA + 1) / 2;
+    return 3 * TP_PF * WG > rc ? 3 * TP_PF * WG : rc;
+}
+
+struct TPRunArgs {
+    const int32_t* pslot;     // [nb] node slot of every permuted block's column: own position - n0,
+                              //      or the aggregate's own node count + its index among the neighbours
+    const int32_t* halo_ptr;  // [A+1] neighbouring nodes of every aggregate ...
+    const int32_t* halo_pos;  //      ... as positions
+    unsigned* bar;            // [TPR_WORDS] barrier words, zero at the launch
+    int maxit;
+};
+
+__device__ inline double ld_ag(const double* p)
+{
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_ag(double* p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Grid barrier number k (1-based), in two halves so that loads that do not depend on the barrier can
+// be issued between them.  tpr_arrive: every store of the workgroup completes (s_waitcnt), the
+// workgroup arrives on its group's word, and the group's last arrival adds one to the global word.
+// tpr_wait: the workgroup waits for the global word to reach TPR_NG k.  A wait longer than TPR_SPIN
+// polls sets the abort word; false when abort is set (the caller leaves its loop; every later wait
+// sees it).
+__device__ inline void tpr_arrive(unsigned* bar, unsigned k)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_s_waitcnt(0);
+        unsigned* glob = bar + TPR_NG * TPR_LINE;
+        const int G = (int)gridDim.x, g = (int)blockIdx.x % TPR_NG;
+        const unsigned Gg = (unsigned)((G - g + TPR_NG - 1) / TPR_NG);  // workgroups of group g (G >= TPR_NG)
+        const unsigned old = __hip_atomic_fetch_add(bar + g * TPR_LINE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == Gg * k) __hip_atomic_fetch_add(glob, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ inline bool tpr_wait(unsigned* bar, unsigned k)
+{
+    __shared__ int pass;
+    if (threadIdx.x == 0) {
+        unsigned* glob = bar + TPR_NG * TPR_LINE;
+        unsigned* abort = bar + (TPR_NG + 1) * TPR_LINE;
+        int ok = 1;
+        unsigned n = 0;
+        while (__hip_atomic_load(glob, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)TPR_NG * k) {
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
+            if (++n > TPR_SPIN) {
+                __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+        pass = ok;
+    }
+    __syncthreads();
+    return pass != 0;
+}
+
+// A uniform value the compiler must take as new in every iteration of k_tp_run's loop: addresses
+// derived from it are formed where they are used instead of being hoisted out of the loop and held
+// in vector registers for the whole launch (the empty asm only pins the value to scalar registers).
+template <typename T>
+__device__ inline T opq(T v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+// tp_partials_load through agent-scope loads (the partials change during the launch)
+template <int K>
+__device__ inline void tpr_partials_load(const double* part, int T, int st0, int st1, const int (&sl)[K],
+                                         double (&v)[K][TP_SU])
+{
+#pragma unroll
+    for (int u = 0; u < TP_SU; ++u) {
+        const int k = st0 + (int)threadIdx.x + u * WG;
+        const int kk = k < st1 ? k : st0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q][u] = ld_ag(part + (size_t)sl[q] * T + kk);
+    }
+}
+
+template <typename VT, int L, bool F64>
+__global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const VT* __restrict__ val, int nb,
+                                                  TPRunArgs ra, double* __restrict__ x, double* __restrict__ r,
+                                                  double* __restrict__ z, double* __restrict__ pb0,
+                                                  double* __restrict__ pb1)
+{
+    constexpr int D = 3 * L, CPP = WG / D;
+    // LDS: the products of q = M p (first half) and r_c (second half) share their room; after it the
+    // aggregate's rows of the FP16 coarse inverse (6 rows of ldA halves), loaded once
+    extern __shared__ double tpr_lds[];
+    double* const prod = tpr_lds;
+    float* const tp_rcs = reinterpret_cast<float*>(tpr_lds);
+    double* const rcd = tpr_lds;
+    __shared__ double zst[3 * TPR_MAXU], pst[3 * TPR_MAXU];  // z and p_old of the own and neighbouring nodes
+    __shared__ double rl[WG];
+    __shared__ double zcs[6];
+    __shared__ double zc_red[6][WG / 64];
+    __shared__ double outq[7], outz[8];
+    const int tid = threadIdx.x;
+    const TPRec R = tp.rec[blockIdx.x];
+    const int a = R.a;
+    SegScalars& S = ta.seg[R.sg];
+    if (seg_done(S)) return;  // nothing to solve: every workgroup leaves (the flags are the same for all)
+    const int n0 = R.n0, nown = R.n1 - R.n0, b0 = R.b0, b1 = R.b1;
+    const int h0 = ra.halo_ptr[a], nh = ra.halo_ptr[a + 1] - h0;
+    const bool mine = tid < 3 * nown;  // the lane's DoF 3 n0 + tid (row tid / 3 of the pass, or
+                                       // column slot tid / D, entry tid % D of k_tp_z)
+    const int d = 3 * n0 + (mine ? tid : 0);
+    const int comp = tid % 3;
+    // the blocks' column slots, the own row's block range, P's row, the column-block row (the
+    // blocks' values are read every iteration: registers do not hold them beside the rest)
+    int psl[TP_PF];
+#pragma unroll
+    for (int k = 0; k < TP_PF; ++k) {
+        const int j = b0 + tid + k * WG;
+        psl[k] = ra.pslot[j < b1 ? j : b0];
+    }
+    const int rrow = n0 + (mine ? tid / 3 : 0);
+    const int jlo = tp.prow_ptr[rrow], jhi = tp.prow_ptr[rrow + 1];
+    const size_t prw = 18 * (size_t)rrow + 6 * comp;  // the lane's row of P
+    const int slot = tid / D, li = tid % D;
+    const int g = n0 / L + (mine ? slot : 0);
+    const size_t drw = ((size_t)g * D + li) * D;  // the lane's row of its column-block inverse
+    const int nc = tp.nc, C0 = R.sc.x, ncs = R.sc.y, ldA = R.sc.z, ld2 = ldA / 2, lc0 = R.c0 - C0, kr = R.kr;
+    const int last = max(ncs, 1) - 1;
+    const int lq = ld2 / 4;  // 16-byte chunks (8 halves) of a coarse row
+    uint4* const arow = reinterpret_cast<uint4*>(tpr_lds + tp_run_share(ldA, F64));  // [6][lq]
+    if constexpr (!F64) {
+        const uint4* Ab = reinterpret_cast<const uint4*>(tp.Ainv + R.sc.w);
+        for (int i = tid; i < 6 * lq; i += WG) arow[i] = Ab[(size_t)min(lc0 + i / lq, last) * lq + i % lq];
+    }
+    // the neighbouring nodes' DoFs this lane stages (at most two)
+    int hd[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int t = tid + u * WG;
+        hd[u] = t < 3 * nh ? 3 * ra.halo_pos[h0 + t / 3] + t % 3 : -1;
+    }
+    // the own rows' state
+    double xv = mine ? x[d] : 0.0, rv = mine ? r[d] : 0.0, zv = mine ? z[d] : 0.0, pold = mine ? pb0[d] : 0.0;
+    double pnew = 0.0, qv = 0.0;
+    int it = S.iters, best_it = S.best_it_h[0];
+    double best = S.best_rr_h[0];
+    const double tol2 = S.tol2;
+    const bool lead = a == R.st0 && tid == 0;
+    unsigned nbar = 0;
+    bool aborted = false;
+    // loads that do not depend on other workgroups: the blocks and P's row of q = M p with the
+    // partials at the start of that half; the column-block row and P's row of z between the halves
+    // of the barrier before it
+    VT pv[TP_PF][9];
+    double P6[6];
+    using DT = std::conditional_t<F64, double, float>;
+    DT Dr[D];
+    auto load_q = [&]() {
+        const VT* const vb = opq(val);
+        const int nbv = opq(nb);
+#pragma unroll
+        for (int k = 0; k < TP_PF; ++k) {
+            const int j = b0 + tid + k * WG;
+            const int jj = j < b1 ? j : b0;  // clamped: always a valid address
+#pragma unroll
+            for (int e = 0; e < 9; ++e) pv[k][e] = vb[(size_t)e * nbv + jj];
+        }
+        const double* pc = opq(tp.pcoef) + prw;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) P6[j] = pc[j];
+    };
+    auto load_z = [&]() {
+        TPArgs tq = tp;
+        if constexpr (F64) tq.dcinv64 = opq(tp.dcinv64);
+        else tq.dcinv = opq(tp.dcinv);
+#pragma unroll
+        for (int j = 0; j < D; ++j) Dr[j] = dc_row<F64>(tq, drw + j);
+        const double* pc = opq(tp.pcoef) + prw;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) P6[j] = pc[j];
+    };
+    while (true) {
+        const int P = it & 1;
+        double* const part = opq(ta.partial);
+        double* const zb = opq(z);
+        const double* pob = opq(P ? pb1 : pb0);  // p of the last iteration
+        double* pnb = opq(P ? pb0 : pb1);        // p of this one
+        // ---- q = M p (k_tp_q) ----
+        const int slq[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
+        double pl[3][TP_SU];
+        tpr_partials_load<3>(part, ta.T, R.st0, R.st1, slq, pl);
+        load_q();
+        double hz[2], hp[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int q = hd[u] >= 0 ? hd[u] : 0;
+            hz[u] = ld_ag(zb + q);
+            hp[u] = ld_ag(pob + q);
+        }
+        double sc[3];
+        tp_partials_sum<3>(ta, R.st0, R.st1, slq, pl, sc);
+        if (mine) { zst[tid] = zv; pst[tid] = pold; }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (hd[u] >= 0) { zst[3 * nown + tid + u * WG] = hz[u]; pst[3 * nown + tid + u * WG] = hp[u]; }
+        {
+            const bool stagnant = it - best_it > 256;
+            const bool conv = sc[0] <= tol2 || (stagnant && best <= 100.0 * tol2);
+            const bool cap = !conv && (it >= ra.maxit || !isfinite(sc[0]) || stagnant);
+            if (conv || cap) {
+                if (lead) {
+                    S.rnorm2 = sc[0];
+                    if (conv) S.converged = 1;
+                    else S.stalled = 1;
+                }
+                break;
+            }
+        }
+        if (sc[0] < 0.5 * best) { best = sc[0]; best_it = it; }
+        const double beta = it > 0 ? sc[1] / sc[2] : 0.0;
+        __syncthreads();  // zst, pst staged
+#pragma unroll
+        for (int k = 0; k < TP_PF; ++k) {
+            const int j = b0 + tid + k * WG;
+            if (j < b1) {
+                const int c = 3 * psl[k];
+                const double x0 = zst[c] + beta * pst[c], x1 = zst[c + 1] + beta * pst[c + 1];
+                const double x2 = zst[c + 2] + beta * pst[c + 2];
+                double* pr = prod + 3 * (j - b0);
+                pr[0] = pv[k][0] * x0 + pv[k][1] * x1 + pv[k][2] * x2;
+                pr[1] = pv[k][3] * x0 + pv[k][4] * x1 + pv[k][5] * x2;
+                pr[2] = pv[k][6] * x0 + pv[k][7] * x1 + pv[k][8] * x2;
+            }
+        }
+        __syncthreads();
+        double pq = 0.0, pt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (mine) {
+            pnew = zv + beta * pold;
+            double q = 0.0;
+            for (int j = jlo; j < jhi; ++j) q += prod[3 * (j - b0) + comp];
+            qv = q;
+            pq += pnew * q;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) pt[j] += P6[j] * q;
+            st_ag(pnb + d, pnew);
+        }
+        {
+            double v[7] = {pq, pt[0], pt[1], pt[2], pt[3], pt[4], pt[5]};
+            tp_reduce<7>(v, outq);
+            if (tid == 0) st_ag(part + (size_t)SL_PQ * ta.T + a, outq[0]);
+            if (tid < kr) st_ag(opq(tp.ptq) + R.c0 + tid, outq[1 + tid]);
+        }
+        tpr_arrive(ra.bar, ++nbar);
+        load_z();
+        // P^T r of the last iteration (written before the last barrier) is loaded while this one
+        // completes; P^T q of this iteration after it
+        const double* rcp = opq(tp.rcx) + (size_t)P * nc + C0;
+        double vx[TP_NV];
+#pragma unroll
+        for (int m = 0; m < TP_NV; ++m) vx[m] = ld_ag(rcp + min(tid + WG * m, last));
+        if (!tpr_wait(ra.bar, nbar)) { aborted = true; break; }
+        // ---- alpha, the coarse correction, x, r, z (k_tp_z) ----
+        const int slz[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
+        tpr_partials_load<3>(opq(ta.partial), ta.T, R.st0, R.st1, slz, pl);
+        const double* ptq = opq(tp.ptq) + C0;
+        double vq[TP_NV];
+#pragma unroll
+        for (int m = 0; m < TP_NV; ++m) vq[m] = ld_ag(ptq + min(tid + WG * m, last));
+        tp_partials_sum<3>(ta, R.st0, R.st1, slz, pl, sc);
+        if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
+            if (lead) { S.rnorm2 = sc[2]; S.stalled = 1; }
+            break;
+        }
+        const double alpha = sc[0] / sc[1];
+#pragma unroll
+        for (int m = 0; m < TP_NV; ++m) {
+            const int i = tid + WG * m;
+            if (i < ldA) {
+                if constexpr (F64) rcd[i] = i < ncs ? vx[m] - alpha * vq[m] : 0.0;
+                else tp_rcs[i] = i < ncs ? (float)(vx[m] - alpha * vq[m]) : 0.f;
+            }
+        }
+        __syncthreads();
+        {
+            double sv[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                sv[k] = 0.0;
+                if constexpr (F64) {
+                    const double2* row = reinterpret_cast<const double2*>(tp.Ainv64 + tp.seg_aoff64[R.sg] +
+                                                                         (size_t)min(lc0 + k, last) * ldA);
+                    for (int j = 2 * tid; j < ldA; j += 2 * WG) {
+                        const double2 f = row[j / 2];
+                        sv[k] += f.x * rcd[j] + f.y * rcd[j + 1];
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < TP_Q4; ++m) {
+                        const int j = 8 * (tid + WG * m);
+                        if (j < ldA) {
+                            const uint4 ch = arow[k * lq + j / 8];
+                            const __half2* hp2 = reinterpret_cast<const __half2*>(&ch);
+                            const float4 ca = *reinterpret_cast<const float4*>(tp_rcs + j);
+                            const float4 cb = *reinterpret_cast<const float4*>(tp_rcs + j + 4);
+                            const float2 f0 = __half22float2(hp2[0]), f1 = __half22float2(hp2[1]);
+                            const float2 f2 = __half22float2(hp2[2]), f3 = __half22float2(hp2[3]);
+                            sv[k] += (double)f0.x * ca.x + (double)f0.y * ca.y + (double)f1.x * ca.z +
+                                     (double)f1.y * ca.w + (double)f2.x * cb.x + (double)f2.y * cb.y +
+                                     (double)f3.x * cb.z + (double)f3.y * cb.w;
+                        }
+                    }
+                }
+            }
+            const int w = tid / 64, l = tid % 64;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double t = wave_sum(sv[k]);
+                if (l == 0) zc_red[k][w] = t;
+            }
+            __syncthreads();
+            if (tid < 6) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < WG / 64; ++k) t += zc_red[tid][k];
+                zcs[tid] = tid < kr ? (F64 ? t : t * tp.ascale[2 * R.sg]) : 0.0;
+            }
+            __syncthreads();
+        }
+        double rz = 0.0, rr = 0.0, rc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (mine) {
+            xv = xv + alpha * pnew;
+            rv = rv - alpha * qv;
+        }
+        rl[tid] = rv;
+        __syncthreads();
+        {
+            const int sb = min(slot, CPP - 1) * D;
+            double zi = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) zi += (double)Dr[j] * rl[sb + j];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) zi += P6[j] * zcs[j];
+            if (mine) {
+                zv = zi;
+                rz += rv * zi;
+                rr += rv * rv;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) rc[j] += P6[j] * rv;
+                st_ag(opq(z) + d, zi);
+            }
+        }
+        {
+            double v[8] = {rz, rr, rc[0], rc[1], rc[2], rc[3], rc[4], rc[5]};
+            tp_reduce<8>(v, outz);
+            double* const pz = opq(ta.partial);
+            if (tid == 0) {
+                st_ag(pz + (size_t)(SL_RZ0 + (1 - P)) * ta.T + a, outz[0]);
+                st_ag(pz + (size_t)(SL_RR0 + (1 - P)) * ta.T + a, outz[1]);
+            }
+            if (tid < kr) st_ag(opq(tp.rcx) + (size_t)(1 - P) * nc + R.c0 + tid, outz[2 + tid]);
+        }
+        pold = pnew;
+        it += 1;
+        tpr_arrive(ra.bar, ++nbar);
+        if (!tpr_wait(ra.bar, nbar)) { aborted = true; break; }
+    }
+    if (mine) { x[d] = xv; r[d] = rv; }
+    if (lead) {
+        S.iters = it;
+        if (aborted) { S.rnorm2 = INFINITY; S.stalled = 1; }  // a barrier that gave up: the host falls back
+    }
+}
+
+__global__ void k_tpr_clear(unsigned* bar)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < TPR_WORDS) bar[i] = 0u;
+}
+
+static size_t tp_run_lds(int ldA, bool f64)
+{
+    return sizeof(double) * tp_run_share(ldA, f64) + (f64 ? 0 : (size_t)6 * ldA * sizeof(__half));
+}
+
 // the solve's vectors between node order and aggregate order
 __global__ __launch_bounds__(256) void k_tp_gather(const int32_t* __restrict__ agg_nodes, int n, const double* __restrict__ src,
                                                    double* __restrict__ dst)
@@ -2314,6 +2721,11 @@ void tl_release(tf_handle* h)
     if (tl.dcinv64) { (void)hipFree(tl.dcinv64); tl.dcinv64 = nullptr; }
     if (tl.Ainv64) { (void)hipFree(tl.Ainv64); tl.Ainv64 = nullptr; }
     if (tl.seg_aoff64) { (void)hipFree(tl.seg_aoff64); tl.seg_aoff64 = nullptr; }
+    for (int32_t** p : {&tl.pslot, &tl.halo_ptr, &tl.halo_pos})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (tl.bar) { (void)hipFree(tl.bar); tl.bar = nullptr; }
+    tl.run_ok = false;
+    for (int32_t& c : tl.run_cap) c = -1;
     tl.built = tl.fresh = tl.failed = false;
 }
 
@@ -2685,6 +3097,37 @@ static void twolevel_build(tf_handle* h)
         tl_upload(h, tl.wg_rec, rec4);
         tl_upload(h, tl.pcoef_p, pcoef_p);
         tl_upload(h, tl.tp_seg_ptr, seg_agg_ptr);
+        // the persistent solve (k_tp_run): every aggregate one pass with at most WG / 3L columns and
+        // TPR_MAXU own and neighbouring nodes; the column slot of every block; one tissue
+        {
+            bool ok = S == 1 && A >= TPR_NG && nc <= WG * TP_NV;
+            std::vector<int32_t> pslot(h->nb, 0), hptr(A + 1, 0), hpos;
+            for (int a = 0; a < A; ++a) {
+                const int n0 = agg_ptr[a], n1 = agg_ptr[a + 1], nown = n1 - n0;
+                if (pass_ptr[a + 1] - pass_ptr[a] != 1 || prow[n1] - prow[n0] > TP_PF * WG || nown / L > WG / (3 * L))
+                    ok = false;
+                std::vector<int32_t> halo;
+                for (int j = prow[n0]; j < prow[n1]; ++j)
+                    if (pcol[j] < n0 || pcol[j] >= n1) halo.push_back(pcol[j]);
+                std::sort(halo.begin(), halo.end());
+                halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+                if (nown + (int)halo.size() > TPR_MAXU) ok = false;
+                for (int j = prow[n0]; j < prow[n1]; ++j) {
+                    const int c = pcol[j];
+                    pslot[j] = c >= n0 && c < n1 ? c - n0
+                                                 : nown + (int)(std::lower_bound(halo.begin(), halo.end(), c) - halo.begin());
+                }
+                hpos.insert(hpos.end(), halo.begin(), halo.end());
+                hptr[a + 1] = (int32_t)hpos.size();
+            }
+            tl.run_ok = ok;
+            if (ok) {
+                tl_upload(h, tl.pslot, pslot);
+                tl_upload(h, tl.halo_ptr, hptr);
+                tl_upload(h, tl.halo_pos, hpos);
+                tl.bar = tl_alloc<unsigned>(TPR_WORDS);
+            }
+        }
         tl.tpart = tl_alloc<double>((size_t)TF_RED_SLOTS * A);
         // (a tissue without coarse unknowns reads one clamped entry at its C0, which may be nc)
         tl.ptq = tl_alloc<double>(nc + 8);
@@ -3087,6 +3530,61 @@ bool all_done(tf_handle* h)
     return true;
 }
 
+// The persistent two-level solve (k_tp_run) for the current options: the kernel instance, its LDS,
+// and whether the device holds n_agg of its workgroups at once (queried once per instance).
+template <typename VT>
+static const void* tp_run_fn(tf_handle* h, int& idx)
+{
+    const bool f64 = h->opts.precond_fp64 != 0;
+    const int L = h->tl.L;
+    idx = (std::is_same<VT, float>::value ? 8 : 0) + (f64 ? 4 : 0) + (L - 1);
+#define TF_TPR(LL)                                                                           \
+    if (L == LL) return f64 ? (const void*)k_tp_run<VT, LL, true> : (const void*)k_tp_run<VT, LL, false>
+    TF_TPR(1);
+    TF_TPR(2);
+    TF_TPR(3);
+    TF_TPR(4);
+#undef TF_TPR
+    return nullptr;
+}
+
+template <typename VT>
+static bool tp_run_fits(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    if (!tl.run_ok || tl.L < 1 || tl.L > TL_MAX_LEVELS) return false;
+    int idx = 0;
+    const void* fn = tp_run_fn<VT>(h, idx);
+    if (!fn) return false;
+    int& cap = tl.run_cap[idx];
+    if (cap < 0) {
+        int per_cu = 0;
+        const size_t lds = tp_run_lds(tl.max_ldA, h->opts.precond_fp64 != 0);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, lds) != hipSuccess) per_cu = 0;
+        cap = per_cu * h->n_cu;
+    }
+    return cap >= tl.n_agg;
+}
+
+template <typename VT>
+static void launch_tp_run(tf_handle* h)
+{
+    tf_handle::TwoLevel& tl = h->tl;
+    int idx = 0;
+    const void* fn = tp_run_fn<VT>(h, idx);
+    const size_t lds = tp_run_lds(tl.max_ldA, h->opts.precond_fp64 != 0);
+    hipLaunchKernelGGL(k_tpr_clear, dim3((TPR_WORDS + 255) / 256), dim3(256), 0, h->stream, tl.bar);
+    TF_HIP(hipGetLastError());
+    TileArgs ta = tp_tile_args(h);
+    TPArgs tp = tp_args(h);
+    const VT* val = tp_values<VT>(h);
+    int nb = h->nb;
+    TPRunArgs ra{tl.pslot, tl.halo_ptr, tl.halo_pos, tl.bar, h->opts.krylov_max_iter};
+    double *x = h->kx, *r = h->kr, *z = h->ks, *p0 = h->kp, *p1 = h->kph;
+    void* args[] = {&ta, &tp, &val, &nb, &ra, &x, &r, &z, &p0, &p1};
+    TF_HIP(hipLaunchCooperativeKernel(fn, dim3(tl.n_agg), dim3(WG), args, (unsigned)lds, h->stream));
+}
+
 // The graph of iters iterations of the given kind (sz 0: a chunk, 1: a small chunk), captured on
 // first use and kept per value type and size.  The captured launches carry the iteration cap as an
 // argument: a chunk captured under another cap (a timing probe's) is not reused.
@@ -3126,6 +3624,12 @@ hipGraphExec_t chunk_graph(tf_handle* h, const VT* val, int kind, int sz, int it
 void round_plan(tf_handle* h)
 {
     tf_handle::KrylovRun& R = h->krun;
+    if (R.persist && R.round == 0) {  // the whole solve in one launch
+        R.round = 1;
+        R.persist_next = true;
+        R.last_launched = -1;  // the launch's iterations are the solve's
+        return;
+    }
     if (R.graph && R.round > 0) {
         const bool fine = R.predicted && R.round <= 2;
         R.n_big = fine ? 0 : 1;
@@ -3145,7 +3649,12 @@ void round_launch(tf_handle* h, const VT* val, bool direct)
     tf_handle::KrylovRun& R = h->krun;
     KernelTimer& tm = h->t_spmv;
     if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
-    if (R.graph && !direct) {
+    if (R.persist_next) {
+        if (direct) throw std::logic_error("the persistent two-level solve cannot be captured");
+        R.persist_next = false;
+        launch_tp_run<VT>(h);
+        h->persist_solves += 1;
+    } else if (R.graph && !direct) {
         hipGraphExec_t big = R.n_big ? chunk_graph<VT>(h, val, R.kind, 0, R.chunk) : nullptr;
         hipGraphExec_t sm = R.n_small ? chunk_graph<VT>(h, val, R.kind, 1, R.small) : nullptr;
         for (int k = 0; k < R.n_big; ++k) TF_HIP(hipGraphLaunch(big, h->stream));
@@ -3240,11 +3749,15 @@ void solve_plan(tf_handle* h, int kind, bool fp32)
     // leaves the rest of its launches as no-ops.
     R.call = h->krylov_call;
     R.predicted = R.graph && R.call >= 0 && R.call < (int)h->krylov_pred.size() && h->krylov_pred[R.call] >= 0;
+    // the persistent launch runs the solve to its end: the round that follows needs no check
+    R.persist = kind == KIND_TL && h->opts.tl_persistent != 0 && !h->no_persist &&
+                (fp32 ? tp_run_fits<float>(h) : tp_run_fits<double>(h));
     if (R.predicted) {
         const int target = ((h->krylov_pred[R.call] + 1 + R.small - 1) / R.small + h->burst_slack) * R.small;
         R.n_big = target / R.chunk;
         R.n_small = (target - R.n_big * R.chunk + R.small - 1) / R.small;
     }
+    if (R.persist) R.predicted = true;
     KernelTimer& tm = h->t_spmv;
     if (R.timing && !tm.sample[0]) {
         TF_HIP(hipEventCreate(&tm.sample[0]));
@@ -3278,11 +3791,16 @@ int solve_end(tf_handle* h, const VT* val, bool seg_known)
             fin = all_done(h);
         }
         seg_known = false;
-        if (R.timing && R.last_launched > 0) {
+        if (R.timing && R.last_launched != 0) {
             float ms = 0.f;
             TF_HIP(hipEventElapsedTime(&ms, tm.sample[0], tm.sample[1]));
             tm.replay_ms += ms;
-            tm.replay_launches += R.last_launched;
+            int launched = R.last_launched;
+            if (launched < 0) {  // the persistent launch: the iterations it ran
+                launched = 0;
+                for (int s = 0; s < h->S; ++s) launched = std::max(launched, (int)h->h_seg[s].iters);
+            }
+            tm.replay_launches += launched;
             R.last_launched = 0;
         }
         if (fin) break;
