@@ -959,6 +959,26 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
 // aggregate's rows of Ac^-1 (16 bytes per lane and load), this lane's entries of rcx[P] and P^T q,
 // and the first pass's r, x, p, q, Dc^-1 rows and P rows; alpha is then formed, r_c = rcx[P] -
 // alpha P^T q staged in LDS (FP32: it only meets the FP16 coarse inverse), and z_c = Ac^-1 r_c.
+// Eight products of a chunk of a coarse-inverse row (FP16) with the chunk (ca, cb) of r_c (FP32),
+// summed by fused multiply-adds in FP32 in a fixed order; the row's sum over its chunks is FP64.
+// (The FP16 rounding of the inverse bounds the accuracy of the coarse correction far above the FP32
+// sum's.)
+__device__ inline double coarse_dot8(const uint4& ch, const float4& ca, const float4& cb)
+{
+    const __half2* hp = reinterpret_cast<const __half2*>(&ch);
+    const float2 f0 = __half22float2(hp[0]), f1 = __half22float2(hp[1]);
+    const float2 f2 = __half22float2(hp[2]), f3 = __half22float2(hp[3]);
+    float s = f0.x * ca.x;
+    s = fmaf(f0.y, ca.y, s);
+    s = fmaf(f1.x, ca.z, s);
+    s = fmaf(f1.y, ca.w, s);
+    s = fmaf(f2.x, cb.x, s);
+    s = fmaf(f2.y, cb.y, s);
+    s = fmaf(f3.x, cb.z, s);
+    s = fmaf(f3.y, cb.w, s);
+    return (double)s;
+}
+
 constexpr int TP_Q4 = 2;     // 16-byte loads (8 halves) per lane of every coarse row held in registers
 constexpr int TP_NV = 16;    // entries of rcx and P^T q per lane (nc_s <= 4096)
 
@@ -1039,31 +1059,26 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
     }
     __syncthreads();
     {
-        double sv[6];
+        double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if constexpr (F64) {  // the FP64 rows streamed here, 16 bytes per lane and load
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            sv[k] = 0.0;
-            if constexpr (F64) {  // the FP64 rows streamed here, 16 bytes per lane and load
+            for (int k = 0; k < 6; ++k) {
                 const double2* row = reinterpret_cast<const double2*>(tp.Ainv64 + tp.seg_aoff64[sg] +
                                                                      (size_t)min(lc0 + k, last) * ldA);
                 for (int j = 2 * tid; j < ldA; j += 2 * WG) {
                     const double2 f = row[j / 2];
                     sv[k] += f.x * rcd[j] + f.y * rcd[j + 1];
                 }
-            } else {
+            }
+        } else {
 #pragma unroll
-                for (int m = 0; m < TP_Q4; ++m) {
-                    const int j = 8 * (tid + WG * m);
-                    if (j < ldA) {  // chunks past ldA hold clamped copies: skipped
-                        const __half2* hp = reinterpret_cast<const __half2*>(&pre[k][m]);
-                        const float4 ca = *reinterpret_cast<const float4*>(tp_rcs + j);
-                        const float4 cb = *reinterpret_cast<const float4*>(tp_rcs + j + 4);
-                        const float2 f0 = __half22float2(hp[0]), f1 = __half22float2(hp[1]);
-                        const float2 f2 = __half22float2(hp[2]), f3 = __half22float2(hp[3]);
-                        sv[k] += (double)f0.x * ca.x + (double)f0.y * ca.y + (double)f1.x * ca.z +
-                                 (double)f1.y * ca.w + (double)f2.x * cb.x + (double)f2.y * cb.y +
-                                 (double)f3.x * cb.z + (double)f3.y * cb.w;
-                    }
+            for (int m = 0; m < TP_Q4; ++m) {
+                const int j = 8 * (tid + WG * m);
+                if (j < ldA) {  // chunks past ldA hold clamped copies: skipped
+                    const float4 ca = *reinterpret_cast<const float4*>(tp_rcs + j);
+                    const float4 cb = *reinterpret_cast<const float4*>(tp_rcs + j + 4);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) sv[k] += coarse_dot8(pre[k][m], ca, cb);
                 }
             }
         }
@@ -1225,7 +1240,9 @@ __device__ inline T opq(T v)
     return v;
 }
 
-// tp_partials_load through agent-scope loads (the partials change during the launch)
+// tp_partials_load through agent-scope loads (the partials change during the launch).  Lanes past the
+// tissue's aggregates load a wrapped index rather than one clamped entry: agent-scope loads are not
+// served by the L2, and hundreds of workgroups reading one line at once would queue at its channel.
 template <int K>
 __device__ inline void tpr_partials_load(const double* part, int T, int st0, int st1, const int (&sl)[K],
                                          double (&v)[K][TP_SU])
@@ -1233,7 +1250,7 @@ __device__ inline void tpr_partials_load(const double* part, int T, int st0, int
 #pragma unroll
     for (int u = 0; u < TP_SU; ++u) {
         const int k = st0 + (int)threadIdx.x + u * WG;
-        const int kk = k < st1 ? k : st0;
+        const int kk = k < st1 ? k : st0 + (k - st0) % max(1, st1 - st0);
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q][u] = ld_ag(part + (size_t)sl[q] * T + kk);
     }
@@ -1297,6 +1314,7 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         const int t = tid + u * WG;
         hd[u] = t < 3 * nh ? 3 * ra.halo_pos[h0 + t / 3] + t % 3 : -1;
     }
+    const int hdum = 3 * n0 + tid % max(1, 3 * nown);
     // the own rows' state
     double xv = mine ? x[d] : 0.0, rv = mine ? r[d] : 0.0, zv = mine ? z[d] : 0.0, pold = mine ? pb0[d] : 0.0;
     double pnew = 0.0, qv = 0.0;
@@ -1347,11 +1365,12 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         const int slq[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
         double pl[3][TP_SU];
         tpr_partials_load<3>(part, ta.T, R.st0, R.st1, slq, pl);
+        __builtin_amdgcn_sched_barrier(0);  // the partials' loads go out before the blocks' (they return in order)
         load_q();
         double hz[2], hp[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            const int q = hd[u] >= 0 ? hd[u] : 0;
+            const int q = hd[u] >= 0 ? hd[u] : hdum;  // lanes without a neighbour load an own row (spread)
             hz[u] = ld_ag(zb + q);
             hp[u] = ld_ag(pob + q);
         }
@@ -1415,15 +1434,16 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         const double* rcp = opq(tp.rcx) + (size_t)P * nc + C0;
         double vx[TP_NV];
 #pragma unroll
-        for (int m = 0; m < TP_NV; ++m) vx[m] = ld_ag(rcp + min(tid + WG * m, last));
+        for (int m = 0; m < TP_NV; ++m) vx[m] = ld_ag(rcp + (tid + WG * m) % (last + 1));  // wrapped, not clamped
         if (!tpr_wait(ra.bar, nbar)) { aborted = true; break; }
         // ---- alpha, the coarse correction, x, r, z (k_tp_z) ----
         const int slz[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
         tpr_partials_load<3>(opq(ta.partial), ta.T, R.st0, R.st1, slz, pl);
+        __builtin_amdgcn_sched_barrier(0);
         const double* ptq = opq(tp.ptq) + C0;
         double vq[TP_NV];
 #pragma unroll
-        for (int m = 0; m < TP_NV; ++m) vq[m] = ld_ag(ptq + min(tid + WG * m, last));
+        for (int m = 0; m < TP_NV; ++m) vq[m] = ld_ag(ptq + (tid + WG * m) % (last + 1));
         tp_partials_sum<3>(ta, R.st0, R.st1, slz, pl, sc);
         if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
             if (lead) { S.rnorm2 = sc[2]; S.stalled = 1; }
@@ -1440,32 +1460,26 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         }
         __syncthreads();
         {
-            double sv[6];
+            double sv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if constexpr (F64) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                sv[k] = 0.0;
-                if constexpr (F64) {
+                for (int k = 0; k < 6; ++k) {
                     const double2* row = reinterpret_cast<const double2*>(tp.Ainv64 + tp.seg_aoff64[R.sg] +
                                                                          (size_t)min(lc0 + k, last) * ldA);
                     for (int j = 2 * tid; j < ldA; j += 2 * WG) {
                         const double2 f = row[j / 2];
                         sv[k] += f.x * rcd[j] + f.y * rcd[j + 1];
                     }
-                } else {
+                }
+            } else {
 #pragma unroll
-                    for (int m = 0; m < TP_Q4; ++m) {
-                        const int j = 8 * (tid + WG * m);
-                        if (j < ldA) {
-                            const uint4 ch = arow[k * lq + j / 8];
-                            const __half2* hp2 = reinterpret_cast<const __half2*>(&ch);
-                            const float4 ca = *reinterpret_cast<const float4*>(tp_rcs + j);
-                            const float4 cb = *reinterpret_cast<const float4*>(tp_rcs + j + 4);
-                            const float2 f0 = __half22float2(hp2[0]), f1 = __half22float2(hp2[1]);
-                            const float2 f2 = __half22float2(hp2[2]), f3 = __half22float2(hp2[3]);
-                            sv[k] += (double)f0.x * ca.x + (double)f0.y * ca.y + (double)f1.x * ca.z +
-                                     (double)f1.y * ca.w + (double)f2.x * cb.x + (double)f2.y * cb.y +
-                                     (double)f3.x * cb.z + (double)f3.y * cb.w;
-                        }
+                for (int m = 0; m < TP_Q4; ++m) {
+                    const int j = 8 * (tid + WG * m);
+                    if (j < ldA) {
+                        const float4 ca = *reinterpret_cast<const float4*>(tp_rcs + j);
+                        const float4 cb = *reinterpret_cast<const float4*>(tp_rcs + j + 4);
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) sv[k] += coarse_dot8(arow[k * lq + j / 8], ca, cb);
                     }
                 }
             }
