@@ -369,15 +369,18 @@ int tf_autotune(tf_handle* h, double dt, int32_t n_iters, int32_t* n_variants, i
  * "assemble_gather", "spmv". */
 int tf_kernel_timing(tf_handle* h, const char* name, int32_t enable, double* avg_ms, int64_t* launches);
 /* Algorithmic bytes of one launch of the named kernel (unique minimum traffic): "spmv",
- * "coarse", "assemble_elements", "assemble_gather". */
+ * "coarse", "assemble_elements", "assemble_gather"; "persistent": one iteration of the persistent
+ * two-level solve. */
 int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes);
 /* Average device time (ms) of one launch of the kernel priced under "spmv" (the SpMV launch of
  * the active solver: k_tp_q for the two-level PCG, k_cg_q for the textbook CG, k_cg1, k_pipe),
  * n_launches replayed back to back in one graph on the current Jacobian and timed with HIP events
  * around the replay.  Leaves the Krylov work vectors undefined.  0 for BiCGSTAB. */
 int tf_kernel_probe(tf_handle* h, int32_t n_launches, double* avg_ms);
-/* The same probe for a named kernel: "spmv" (as tf_kernel_probe) or "coarse" (k_tp_z, the
- * two-level PCG's coarse-correction launch: 0 when the two-level preconditioner is not active). */
+/* The same probe for a named kernel: "spmv" (as tf_kernel_probe), "coarse" (k_tp_z, the
+ * two-level PCG's coarse-correction launch: 0 when the two-level preconditioner is not active) or
+ * "persistent" (ms per iteration of one persistent launch of n_launches iterations with tolerance 0:
+ * 0 when the solve does not run persistently). */
 int tf_kernel_probe_named(tf_handle* h, const char* name, int32_t n_launches, double* avg_ms);
 /* Krylov chunk replays timed with HIP events on the handle's stream since the last reset:
  * total device time of the replays (ms), the iterations they launched (no-op launches after a

@@ -500,7 +500,8 @@ double krylov_time_iteration(tf_handle* h, int n_iters);
 bool twolevel_active(const tf_handle* h);  // the CG solves of this handle use the two-level preconditioner
 // Average device time (ms) of one launch of the dominant Krylov kernel (the one tf_kernel_bytes
 // prices under "spmv"), n_launches replayed back to back on the current Jacobian; 0 for BiCGSTAB
-double krylov_probe(tf_handle* h, int n_launches, int which = 0);  // which: 0 the SpMV kernel, 1 k_tp_z
+// which: 0 the SpMV kernel, 1 k_tp_z, 2 ms per iteration of the persistent two-level solve (one launch)
+double krylov_probe(tf_handle* h, int n_launches, int which = 0);
 #define TF_TWO_LEVEL_MIN_NODES 4096       // TF_PRECOND_AUTO: two-level from this many nodes
 #ifndef TF_ONE_READ_SLACK
 #define TF_ONE_READ_SLACK 1               // small chunks a one-read iteration's first burst adds to its prediction

@@ -3963,6 +3963,12 @@ double krylov_time_iteration(tf_handle* h, int n_iters)
 double krylov_probe(tf_handle* h, int n_launches, int which)
 {
     if (which == 1 && !twolevel_active(h)) return 0.0;  // k_tp_z: the two-level PCG only
+    if (which == 2) {  // the persistent solve: ms per iteration of one launch of n_launches iterations
+        if (!twolevel_active(h) || !h->opts.tl_persistent) return 0.0;
+        if (!h->tl.built) twolevel_build(h);
+        const bool fits = fp32_values(h) ? tp_run_fits<float>(h) : tp_run_fits<double>(h);
+        return fits ? krylov_time_iteration(h, n_launches) : 0.0;
+    }
     int kind = h->opts.krylov;
     if (kind < 0) kind = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
     const bool tl = twolevel_active(h);

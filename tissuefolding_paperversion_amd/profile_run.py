@@ -38,6 +38,8 @@ def main(argv=None):
     wg, cap, tiles = s.tiling()
     print(json.dumps({"workload": t.name, "steps": a.steps, "nr_iterations": [st["nr_iterations"] for st in stats],
                       "krylov_iterations": [st["krylov_iterations"] for st in stats],
+                      "krylov_by_newton": [st["krylov_by_newton"][:st["nr_iterations"]] for st in stats],
+                      "persistent_solves": [st["krylov_persistent"] for st in stats],
                       "spmv_algorithmic_bytes": s.kernel_bytes("spmv"), "tiling": [wg, cap, tiles]}), flush=True)
     s.close()
     return 0
