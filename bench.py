@@ -6,20 +6,25 @@ surfaces, packing detection, and the full Newton-Raphson loop until ||dU||_2 <= 
 of forces and the Jacobian, fixed DoFs, the preconditioned Krylov solve, update).  Inputs are
 resident in HBM before the timed region.  Prints one JSON line.
 
-Roofline: the two-level PCG iteration is two launches, k_tp_q (p = z + beta p, the BSR3 SpMV
-q = M p, (p, q), P^T q) and k_tp_z (alpha, the coarse correction through the FP16 inverse, x, r, z
-updates, (r, z), (r, r), P^T r).  The timed steps carry no events.  A second run of the same
-steps on a fresh handle carries them: HIP events around every replay of the captured Krylov chunks
-give the device time per PCG iteration (both launches) as the solves ran it, and afterwards each
-launch is timed by HIP events around 64 back-to-back launches in one graph replay on the final
-Jacobian (tf_kernel_probe_named); the slower is the dominant kernel of the roofline line.
-Algorithmic bytes per launch (tf_kernel_bytes):
+Roofline: every CG solve of the two-level PCG runs as one cooperative launch, k_tp_run (one
+workgroup per aggregate, grid barriers between the halves of an iteration: p = z + beta p, the
+BSR3 SpMV q = M p, (p, q), P^T q; then alpha, the coarse correction through the FP16 inverse rows
+held in LDS, x, r, z updates, (r, z), (r, r), P^T r).  The roofline "launch" of k_tp_run is one
+PCG iteration.  The timed steps carry no events.  A second run of the same steps on a fresh handle
+carries them: HIP events around every persistent launch give the device time per PCG iteration as
+the solves ran it (each solve's start included), and afterwards one launch of 64 iterations with
+tolerance 0 on the final Jacobian gives the time per iteration (tf_kernel_probe_named
+"persistent").  The two launches of the same iteration (k_tp_q, k_tp_z, the path taken when the
+grid does not fit co-resident) are probed beside it by 64 back-to-back launches in one graph replay.
+Algorithmic bytes (tf_kernel_bytes):
+  k_tp_run, per iteration: nb (72 + 4) (blocks) + 2 N 18 x 8 (P, both halves)
+          + ncol (3L)^2 4 (column-block inverses) + 4 x 3N x 8 (z, p published and read) + 4 nc 8
   k_tp_q: nb (72 + 4) + (N + 1) 4 (matrix) + 4 x 3N x 8 (z, p_old read; p, q written) + N 18 x 8 (P)
   k_tp_z: nc ldA 2 (FP16 Ac^-1) + ncol (3L)^2 4 (column-block inverses) + N 18 x 8 (P)
           + 7 x 3N x 8 (x, r read and written; p, q read; z written) + 3 nc 8 (P^T r, P^T q)
 The achievable HBM rate is measured on the box by a 1 GiB device-to-device copy
-(tf_copy_bandwidth); whole-step algorithmic bytes (assembly kernels per NR iteration, both PCG
-launches per Krylov iteration) over ms_per_step give the step's own fraction of peak.
+(tf_copy_bandwidth); whole-step algorithmic bytes (assembly kernels per NR iteration, one PCG
+iteration's bytes per Krylov iteration) over ms_per_step give the step's own fraction of peak.
 A second leg runs the batched 256-disc sweep (config 4: one tissue segment per disc in one
 handle, autotuned tiling) and, for comparison, the same discs one handle at a time.
 A third leg times the discrete adjoint on the inverse-growth workload (config 5: 30 steps of
