@@ -130,11 +130,14 @@ typedef struct tf_solver_opts {
                                   launches replayed from one captured hipGraph; 0: the host checks
                                   the assembly status and every Krylov round                     */
     int32_t tl_persistent;     /* TF_PRECOND_TWO_LEVEL: 1 (default) runs a solve of one tissue whose
-                                  aggregates fit one workgroup each as one cooperative launch that
-                                  keeps the aggregate's blocks and coarse rows in registers, with a
-                                  grid barrier after each half of an iteration (same iterates as
-                                  the two launches per iteration it replaces); 0: two launches per
-                                  iteration, replayed from graphs                                */
+                                  aggregates fit on the device at once, one workgroup each, as one
+                                  launch that keeps the aggregate's coarse rows in LDS, with a grid
+                                  barrier after each half of an iteration (same iterates as the two
+                                  launches per iteration it replaces); a barrier wait that gives up
+                                  (workgroups not co-resident) falls back to BiCGSTAB; 2: the same
+                                  as a cooperative launch (co-residency guaranteed by the runtime,
+                                  slower to launch); 0: two launches per iteration, replayed from
+                                  graphs                                                         */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

@@ -113,7 +113,8 @@ def test_newton_iteration_graphs_replay_the_one_read_iterates(case):
     """nr_graph = 2 replays a Newton iteration (assembly, warm start, two-level set-up, the solve's
     first burst, update, convergence test) from one captured hipGraph per shape of its launches.
     The iterates equal those of the eagerly enqueued one-read path bit for bit (graphs captured on
-    first use, replayed afterwards), and the host reads what the eager path reads."""
+    first use, replayed afterwards), the host reads what the eager path reads, and the two-level
+    solves run as the same persistent launches."""
     t = _ellipse()
     opts = {"preconditioner": 0} if case == "block_jacobi" else {"preconditioner": 1}
     if case == "fp32_first_iteration":
@@ -127,6 +128,8 @@ def test_newton_iteration_graphs_replay_the_one_read_iterates(case):
         assert (a["nr_iterations"], a["krylov_iterations"]) == (b["nr_iterations"], b["krylov_iterations"])
         assert np.array_equal(eager.positions, graph.positions), step
         assert b["host_waits"] == a["host_waits"], step  # the same reads, from a graph or eager launches
+        # the persistent two-level solve runs inside the captured iterations too
+        assert b["krylov_persistent"] == a["krylov_persistent"], step
         builds.append(b["nr_graph_builds"])
         if step >= 2:
             waits += b["host_waits"]

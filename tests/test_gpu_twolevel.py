@@ -233,18 +233,20 @@ def test_coarse_inverse_matches_a_dense_inverse(make):
     assert np.array_equal(Ai, Ai.T)
 
 
-@pytest.mark.parametrize("case", ["ellipse", "ellipse_fp64_storage", "config2"])
+@pytest.mark.parametrize("case", ["ellipse", "ellipse_fp64_storage", "ellipse_cooperative", "config2"])
 def test_persistent_solve_equals_two_launches(case):
-    """tl_persistent runs a two-level solve as one cooperative launch (grid barriers between the
-    halves of an iteration, the coarse rows held in LDS) instead of two launches per iteration.  The
-    arithmetic is the same in the same order: positions, Newton and Krylov counts are equal bit for
-    bit over several timesteps."""
+    """tl_persistent runs a two-level solve as one launch (grid barriers between the halves of an
+    iteration, the coarse rows held in LDS) instead of two launches per iteration: an ordinary launch
+    of a grid the device holds at once (1), or a cooperative launch (2).  The arithmetic is the same
+    in the same order: positions, Newton and Krylov counts are equal bit for bit over several
+    timesteps."""
     t = configs.config2() if case == "config2" else _ellipse()
     opts = dict(TL)
     if case == "ellipse_fp64_storage":
         opts["precond_fp64"] = 1
     steps = 3 if case == "config2" else 5
-    two, one = _solver(t, tl_persistent=0, **opts), _solver(t, tl_persistent=1, **opts)
+    mode = 2 if case == "ellipse_cooperative" else 1
+    two, one = _solver(t, tl_persistent=0, **opts), _solver(t, tl_persistent=mode, **opts)
     for step in range(steps):
         a, b = two.runOneStep(t.dt), one.runOneStep(t.dt)
         assert a["converged"] == 1 and b["converged"] == 1

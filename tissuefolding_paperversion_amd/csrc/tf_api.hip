@@ -650,7 +650,7 @@ int tf_create(const tf_mesh_desc* d, tf_handle** out)
         h->seg_rtol2 = dev_alloc<double>(S);
         TF_HIP(hipHostMalloc((void**)&h->h_seg, sizeof(SegScalars) * S, hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->h_seg_norm, sizeof(double) * S, hipHostMallocDefault));
-        TF_HIP(hipHostMalloc((void**)&h->h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
+        TF_HIP(hipHostMalloc((void**)&h->h_status, 12 * sizeof(int32_t), hipHostMallocDefault));
         TF_HIP(hipHostMalloc((void**)&h->hp_rtol2, sizeof(double) * S, hipHostMallocDefault));
         for (double** v : {&h->kr, &h->kr0, &h->kp, &h->kph, &h->kv, &h->ks, &h->ksh, &h->kt, &h->kx, &h->km, &h->kq})
             *v = dev_alloc<double>((size_t)3 * N);
@@ -750,7 +750,7 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
         o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1) ||
         (o->precond_fp64 != 0 && o->precond_fp64 != 1) || o->nr_graph < 0 || o->nr_graph > 2 ||
-        (o->tl_persistent != 0 && o->tl_persistent != 1))
+        (o->tl_persistent < 0 || o->tl_persistent > 2))
         return fail(h, TF_INVALID, "invalid solver options");
     const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates ||
                                 o->column_blocks != h->opts.column_blocks || o->precond_fp64 != h->opts.precond_fp64;
@@ -1315,8 +1315,10 @@ static void nr_iteration_launch(tf_handle* h, double dt, int it, bool warm, bool
 static void nr_graph_run(tf_handle* h, double dt, int it, bool warm, bool tl_prep)
 {
     const tf_handle::KrylovRun& R = h->krun;
+    // a persistent solve is the same launch whatever its prediction: no chunk counts in its key
     const tf_handle::NrGraphKey key{it, warm ? h->warm_count[it] : 0, R.kind, R.fp32 ? 1 : 0, tangent_precision_now(h),
-                                    R.n_big, R.n_small, h->seg_rtol_on ? 1 : 0, dt, h->ablated};
+                                    R.persist ? -1 : R.n_big, R.persist ? -1 : R.n_small, h->seg_rtol_on ? 1 : 0, dt,
+                                    h->ablated};
     h->nr_graph_clock += 1;
     hipGraphExec_t exec = nullptr;
     for (tf_handle::NrGraph& g : h->nr_graphs)
@@ -1344,16 +1346,38 @@ static void nr_graph_run(tf_handle* h, double dt, int it, bool warm, bool tl_pre
         h->nr_graph_builds += 1;
     }
     TF_HIP(hipGraphLaunch(exec, h->stream));
+    if (R.persist) h->persist_solves += 1;
 }
 
-static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
+// The device side of a timestep's start: un <- uk, the tissues' Newton state reset, and (assemble)
+// the assembly of Newton iteration 0 enqueued ahead of the host's set-up of the loop.  Enqueued
+// again by a step that finds its input changed (packing pairs), it only repeats the same writes.
+static void nr_step_begin(tf_handle* h, double dt, bool assemble)
+{
+    launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
+    hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    TF_HIP(hipGetLastError());
+    if (assemble) {
+        h->nr_it = 0;  // the precision switch reads the iteration index
+        assemble_impl(h, dt);
+        h->nr_it = -1;
+    }
+}
+
+// begun: nr_step_begin(h, dt, true) has been enqueued for this step
+static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats, bool begun = false)
 {
     auto t0 = std::chrono::steady_clock::now();
     h->host_waits = 0;
     h->nr_graph_builds = 0;
     h->persist_solves = 0;
-    launch_copy(h, h->un, h->uk, 3 * (int64_t)h->N);
-    hipLaunchKernelGGL(k_seg_begin, dim3((h->S + 255) / 256), dim3(256), 0, h->stream, h->seg, h->S);
+    if (!begun) nr_step_begin(h, dt, false);
+    // The assembly of the iteration about to run is already enqueued: the host enqueues it as soon
+    // as the previous iteration's report says that one more follows, ahead of its own bookkeeping
+    // (warm-start history, forcing, tolerances), which then overlaps the assembly on the device.
+    // Not under nr_graph = 2, whose graphs hold the assembly.
+    bool asm_ahead = begun;
+    double next_asm = 0.0;
     std::fill(h->seg_nr_iters.begin(), h->seg_nr_iters.end(), 0);
     std::fill(h->seg_converged.begin(), h->seg_converged.end(), 0);
     std::fill(h->seg_failed.begin(), h->seg_failed.end(), 0);
@@ -1400,7 +1424,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
             h->krylov_call = it;
             bool burst = false;
             if (one_read) nr_report_alloc(h);
-            if (one_read && h->opts.nr_graph == 2 && nr_graph_ready(h, it)) {
+            if (!asm_ahead && one_read && h->opts.nr_graph == 2 && nr_graph_ready(h, it)) {
                 // the host's plan of the iteration (warm start, two-level prepare, the solve's first
                 // burst; no launches), then its launches from one graph
                 const bool w = warm && warm_start_begin(h, it, false);
@@ -1408,7 +1432,7 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
                 krylov_pre_solve(h, false);
                 const bool tl_prep = h->tl_pre_valid && h->tl_pre_ok;
                 if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
-                h->no_persist = true;  // a cooperative launch cannot be captured
+                h->no_persist = h->opts.tl_persistent == 2;  // a cooperative launch cannot be captured
                 try {
                     burst = krylov_solve_plan(h);
                 } catch (...) {
@@ -1420,9 +1444,14 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
                 nr_graph_run(h, dt, it, w, tl_prep);
                 nr_epilogue_read(h);
             } else {
-                auto ta = std::chrono::steady_clock::now();
-                assemble_impl(h, dt);
-                it_asm = ms_since(ta);
+                if (asm_ahead) {
+                    it_asm = next_asm;
+                } else {
+                    auto ta = std::chrono::steady_clock::now();
+                    assemble_impl(h, dt);
+                    it_asm = ms_since(ta);
+                }
+                asm_ahead = false;
                 h->warm_guess = warm && warm_start_begin(h, it) ? h->warm_x0 : nullptr;
                 krylov_pre_solve(h);
                 if (h->opts.krylov < 0) h->opts.krylov = h->any_viscous ? TF_KRYLOV_BICGSTAB : TF_KRYLOV_CG1;
@@ -1492,10 +1521,22 @@ static int nr_step_impl(tf_handle* h, double dt, tf_step_stats* stats)
                 TF_HIP(hipGetLastError());
             }
         }
-        if (warm && kst == TF_OK) warm_start_push(h, it);  // a guess for the next timestep's iteration it
         krylov_total += h->last_krylov_iters;
         if (stats && it < 4) stats->krylov_by_newton[it] = h->last_krylov_iters;
         int ust = applied ? update_uk_host(h, it, nr_report_norm(h->hp_nr_rep, h->S)) : update_uk_impl(h, it);
+        if (ust == TF_OK && it + 1 <= h->opts.nr_max_iter && h->opts.nr_graph != 2) {
+            bool all = true;
+            for (int s = 0; s < h->S; ++s) all &= (bool)(h->seg_converged[s] || h->seg_failed[s]);
+            if (!all) {  // another iteration follows: its assembly goes first
+                auto ta = std::chrono::steady_clock::now();
+                h->nr_it = it + 1;
+                assemble_impl(h, dt);
+                h->nr_it = it;
+                next_asm = ms_since(ta);
+                asm_ahead = true;
+            }
+        }
+        if (warm && kst == TF_OK) warm_start_push(h, it);  // a guess for the next timestep's iteration it
         if (h->opts.krylov_forcing) forcing_observe(h, it);
         ms_asm += it_asm;
         ms_sol += ms_since(ts) - it_asm;
@@ -1566,15 +1607,27 @@ int tf_timestep(tf_handle* h, double dt, tf_step_stats* stats)
     launch_grow(h, dt);
     h->time += dt;  // the growth update of this timestep is applied
     launch_surfaces(h);
-    int32_t st4[4];
-    d2h(h, st4, h->status, 4);
-    TF_HIP(hipStreamSynchronize(h->stream));
-    if (h->opts.implicit_packing) packing_detect(h);
-    else packing_clear(h);
+    // the growth update's counters, read once the step has ended
+    TF_HIP(hipMemcpyAsync(h->h_status + 4, h->status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    bool begun = false;
+    if (h->opts.implicit_packing) {
+        packing_detect_launch(h);
+        // With no pairs in force, the step's start and the assembly of Newton iteration 0 are
+        // enqueued behind the detection before the host waits for its pair count; pairs that
+        // formed send the step through its start again, with them.
+        if (h->pk.n == 0) {
+            nr_step_begin(h, dt, true);
+            begun = true;
+        }
+        if (packing_detect_finish(h) > 0) begun = false;
+    } else {
+        packing_clear(h);
+    }
     double ms_grow = ms_since(t0);
-    int st = nr_step_impl(h, dt, stats);
+    int st = nr_step_impl(h, dt, stats, begun);
     if (stats) {
-        stats->rotated_elements = st4[1];
+        TF_HIP(hipStreamSynchronize(h->stream));  // the counters' copy (the step has read its reports)
+        stats->rotated_elements = h->h_status[5];
         stats->ms_grow = ms_grow;
         stats->ms_total = ms_since(t0);
     }

@@ -136,6 +136,7 @@ struct PackingState {
     int32_t *nodes = nullptr, *nptr = nullptr, *nref = nullptr;  // their pairs: p (node a), ~p (node b)
     double *f = nullptr, *d = nullptr;   // [3N] packing force and Jacobian diagonal of every DoF
     std::vector<int64_t> h_blocks;       // pair blocks (min * N + max) kept in the BSR pattern
+    hipEvent_t ev_count = nullptr;       // recorded behind the pair count's copy to the host
 };
 
 // Everything a Krylov tile kernel reads about its tile, in one 32-byte record: its tissue, BSR
@@ -230,7 +231,8 @@ struct tf_handle {
     SegScalars *h_seg = nullptr;      // pinned host mirror
     double *seg_norm = nullptr;       // [S] per-segment ||dU||
     double *h_seg_norm = nullptr;     // pinned
-    int32_t *h_status = nullptr;      // pinned copy of status
+    int32_t *h_status = nullptr;      // pinned: [0, 4) copy of status, [4, 8) status after the growth
+                                      // update, [8] the packing detection's pair count
     NrReport* nr_rep = nullptr;       // device report of a Newton iteration (nr_report_bytes(S))
     NrReport* hp_nr_rep = nullptr;    // pinned host copy
     int32_t host_waits = 0;           // host waits for the device since the step began
@@ -302,7 +304,7 @@ struct tf_handle {
     // kind, chunk sizes, the rounds launched and the iterations they held.
     struct KrylovRun {
         bool pending = false, graph = false, predicted = false, timing = false, fp32 = false;
-        bool persist = false;         // the two-level solve runs as one cooperative launch (k_tp_run)
+        bool persist = false;         // the two-level solve runs as one persistent launch (k_tp_run)
         bool persist_next = false;    // the next round_launch is that launch
         bool cg = false;              // opts.krylov is a CG kind (a stall may fall back to BiCGSTAB)
         int32_t kind = 0, chunk = 0, small = 0, n_big = 1, n_small = 0, round = 0, done_iters = 0, call = -1;
@@ -386,7 +388,7 @@ struct tf_handle {
         double *tpart = nullptr, *ptq = nullptr, *rcx = nullptr;  // [SL_COUNT][A], [nc], [2][nc]
         int32_t* tp_seg_ptr = nullptr;                      // [S+1] the aggregates as every tissue's "tiles"
         int32_t tp_max_blocks = 0;                          // most blocks of a pass (LDS products)
-        // the persistent solve (one cooperative launch per solve): eligibility, the column slot of
+        // the persistent solve (one launch per solve): eligibility, the column slot of
         // every permuted block, every aggregate's neighbouring nodes, the grid-barrier words, and
         // how many workgroups of each kernel instance the device holds at once (-1: not queried)
         bool run_ok = false;
@@ -450,6 +452,8 @@ void growth_functions_free(tf_handle* h);
 void packing_prepare(tf_handle* h);             // detection tables (first use)
 void packing_reserve(tf_handle* h, int n);      // pair arrays for n pairs
 int packing_detect(tf_handle* h);               // masses + detection on uk; returns the pair count
+void packing_detect_launch(tf_handle* h);       // the same, enqueued up to the pair count's copy
+int packing_detect_finish(tf_handle* h);        // waits for the count (not for later work), installs
 void packing_install(tf_handle* h, const std::vector<int32_t>& a, const std::vector<int32_t>& b);
 void packing_clear(tf_handle* h);
 void launch_packing_mass(tf_handle* h);

@@ -1149,7 +1149,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
 static size_t tp_z_lds(int ldA, bool f64) { return (f64 ? sizeof(double) : sizeof(float)) * (size_t)ldA; }  // r_c
 
 // ---- the two-level PCG as one persistent launch --------------------------------------------------
-// One workgroup per aggregate, every workgroup resident at once (cooperative launch), runs the whole
+// One workgroup per aggregate, every workgroup resident at once, runs the whole
 // solve: the halves of the iteration that k_tp_q and k_tp_z launch, each followed by a grid barrier.
 // What a solve does not change and a workgroup can hold -- the aggregate's rows of the coarse inverse
 // (LDS), its column-block inverses and rows of P (registers) -- is loaded once; the own rows of x, r,
@@ -3596,7 +3596,15 @@ static void launch_tp_run(tf_handle* h)
     TPRunArgs ra{tl.pslot, tl.halo_ptr, tl.halo_pos, tl.bar, h->opts.krylov_max_iter};
     double *x = h->kx, *r = h->kr, *z = h->ks, *p0 = h->kp, *p1 = h->kph;
     void* args[] = {&ta, &tp, &val, &nb, &ra, &x, &r, &z, &p0, &p1};
-    TF_HIP(hipLaunchCooperativeKernel(fn, dim3(tl.n_agg), dim3(WG), args, (unsigned)lds, h->stream));
+    // The occupancy check of tp_run_fits puts every workgroup on the device at once when nothing
+    // else runs on it; the stream's order has the previous kernel finished.  A workgroup that still
+    // waits (another process's kernels holding compute units) ends its bounded barrier wait with the
+    // abort word and the solve falls back.  The cooperative launch has the runtime guarantee the
+    // co-residency instead, at about 30 us of launch overhead on each side of the kernel.
+    if (h->opts.tl_persistent == 2)
+        TF_HIP(hipLaunchCooperativeKernel(fn, dim3(tl.n_agg), dim3(WG), args, (unsigned)lds, h->stream));
+    else
+        TF_HIP(hipLaunchKernel(fn, dim3(tl.n_agg), dim3(WG), args, lds, h->stream));
 }
 
 // The graph of iters iterations of the given kind (sz 0: a chunk, 1: a small chunk), captured on
@@ -3664,10 +3672,13 @@ void round_launch(tf_handle* h, const VT* val, bool direct)
     KernelTimer& tm = h->t_spmv;
     if (R.timing) TF_HIP(hipEventRecord(tm.sample[0], h->stream));
     if (R.persist_next) {
-        if (direct) throw std::logic_error("the persistent two-level solve cannot be captured");
+        // a cooperative launch cannot be captured; the ordinary launch is a kernel node like any
+        // other (the Newton-iteration graph counts its replays)
+        if (direct && h->opts.tl_persistent == 2)
+            throw std::logic_error("the cooperative persistent solve cannot be captured");
         R.persist_next = false;
         launch_tp_run<VT>(h);
-        h->persist_solves += 1;
+        if (!direct) h->persist_solves += 1;
     } else if (R.graph && !direct) {
         hipGraphExec_t big = R.n_big ? chunk_graph<VT>(h, val, R.kind, 0, R.chunk) : nullptr;
         hipGraphExec_t sm = R.n_small ? chunk_graph<VT>(h, val, R.kind, 1, R.small) : nullptr;

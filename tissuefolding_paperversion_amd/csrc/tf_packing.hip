@@ -860,6 +860,7 @@ void packing_free(tf_handle* h)
     pk_free(pk.hstart); pk_free(pk.cell_nodes); pk_free(pk.node_slot); pk_free(pk.pcount); pk_free(pk.pstart);
     pk_free(pk.scan_tmp); pk_free(pk.a); pk_free(pk.b); pk_free(pk.w); pk_free(pk.slot); pk_free(pk.nodes);
     pk_free(pk.nptr); pk_free(pk.nref);
+    if (pk.ev_count) { (void)hipEventDestroy(pk.ev_count); pk.ev_count = nullptr; }
 }
 
 // Detection work arrays and node tables, allocated on first use.
@@ -1046,7 +1047,7 @@ void launch_reference_box(tf_handle* h, double* stats)
     TF_HIP(hipGetLastError());
 }
 
-int packing_detect(tf_handle* h)
+void packing_detect_launch(tf_handle* h)
 {
     packing_prepare(h);
     PackingState& pk = h->pk;
@@ -1073,9 +1074,18 @@ int packing_detect(tf_handle* h)
                        pk.cell_nodes, pk.adj_ptr, pk.adj_col, hmask, pk.pcount);
     TF_HIP(hipGetLastError());
     scan_i32(h, pk.pcount, N, pk.pstart);
-    int32_t total = 0;
-    TF_HIP(hipMemcpyAsync(&total, pk.pstart + N, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
-    TF_HIP(hipStreamSynchronize(h->stream));
+    if (!pk.ev_count) TF_HIP(hipEventCreateWithFlags(&pk.ev_count, hipEventDisableTiming));
+    TF_HIP(hipMemcpyAsync(h->h_status + 8, pk.pstart + N, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    TF_HIP(hipEventRecord(pk.ev_count, h->stream));
+}
+
+int packing_detect_finish(tf_handle* h)
+{
+    PackingState& pk = h->pk;
+    const int N = h->N;
+    const uint32_t hmask = (uint32_t)pk.H - 1u;
+    TF_HIP(hipEventSynchronize(pk.ev_count));
+    const int32_t total = h->h_status[8];
     std::vector<int32_t> a(total), b(total);
     if (total > 0) {
         packing_reserve(h, total);
@@ -1095,6 +1105,12 @@ int packing_detect(tf_handle* h)
     }
     packing_install(h, a, b);
     return total;
+}
+
+int packing_detect(tf_handle* h)
+{
+    packing_detect_launch(h);
+    return packing_detect_finish(h);
 }
 
 void packing_clear(tf_handle* h)
