@@ -459,7 +459,32 @@ __device__ inline void polar_rotation(const double* F, double* R)
     }
 }
 
-__global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, const double* __restrict__ xref,
+// The mean deformation gradient over the six Gauss points (ShapeBase.cpp:1676-1689),
+// Fm = 1/6 sum_g G_g G0_g^-1 with G_g = sum_a x_a dN_{g,a}^T, is linear in the positions:
+// Fm = sum_a x_a c_a^T with c_a = 1/6 sum_g G0_g^-T dN_{g,a}, fixed by the reference geometry.
+// One thread per prism, once per handle.
+__global__ __launch_bounds__(256) void k_mean_f_weights(const double* __restrict__ xref, int E, double* __restrict__ fmw)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double X[18], c[18];
+    for (int q = 0; q < 18; ++q) { X[q] = xref[(size_t)q * E + e]; c[q] = 0.0; }
+    for (int g = 0; g < 6; ++g) {
+        double G0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, IJ0[9], dN[6][3];
+        for (int a = 0; a < 6; ++a) {
+            tf::shape_deriv(g, a, dN[a]);
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) G0[3 * i + k] += X[3 * a + i] * dN[a][k];
+        }
+        tf::inv3(G0, IJ0);
+        for (int a = 0; a < 6; ++a)
+            for (int j = 0; j < 3; ++j)
+                c[3 * a + j] += (dN[a][0] * IJ0[j] + dN[a][1] * IJ0[3 + j] + dN[a][2] * IJ0[6 + j]) * (1.0 / 6.0);
+    }
+    for (int q = 0; q < 18; ++q) fmw[(size_t)q * E + e] = c[q];
+}
+
+__global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, const double* __restrict__ fmw,
                                               const double* __restrict__ pos, const double* __restrict__ rate,
                                               const double* __restrict__ ginc, double dt, int allow_rotation, int E,
                                               double* __restrict__ fg,
@@ -474,30 +499,17 @@ __global__ __launch_bounds__(256) void k_grow(const int32_t* __restrict__ conn, 
 #pragma unroll
         for (int q = 0; q < 9; ++q) inc[q] = ginc[(size_t)q * E + e];
     if (allow_rotation) {
-        // mean F at the current positions (ShapeBase.cpp:1676-1689)
-        double X[18], x[18];
+        // mean F at the current positions (ShapeBase.cpp:1676-1689): sum_a x_a c_a^T
+        double Fm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int a = 0; a < 6; ++a) {
             const int id = conn[a * E + e];
+            double x[3], c[3];
             for (int i = 0; i < 3; ++i) {
-                X[3 * a + i] = xref[(3 * a + i) * E + e];
-                x[3 * a + i] = pos[3 * id + i];
+                x[i] = pos[3 * id + i];
+                c[i] = fmw[(3 * a + i) * E + e];
             }
-        }
-        double Fm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int g = 0; g < 6; ++g) {
-            double G0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, IJ0[9], F[9];
-            for (int a = 0; a < 6; ++a) {
-                double d[3];
-                tf::shape_deriv(g, a, d);
-                for (int i = 0; i < 3; ++i)
-                    for (int k = 0; k < 3; ++k) {
-                        G0[3 * i + k] += X[3 * a + i] * d[k];
-                        G[3 * i + k] += x[3 * a + i] * d[k];
-                    }
-            }
-            tf::inv3(G0, IJ0);
-            tf::mm3(G, IJ0, F);
-            for (int q = 0; q < 9; ++q) Fm[q] += F[q] * (1.0 / 6.0);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) Fm[3 * i + j] += x[i] * c[j];
         }
         double R[9];
         polar_rotation(Fm, R);
@@ -718,8 +730,15 @@ double* scratch_addr(int device)
 void launch_grow(tf_handle* h, double dt)
 {
     if (h->gf.n_fn > 0) growth_functions_eval(h, dt);  // the step's increments into ginc
-    hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->xref, h->uk, h->rate,
+    hipLaunchKernelGGL(k_grow, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->conn, h->fmw, h->uk, h->rate,
                        h->ginc, dt, h->opts.growth_rotation, h->E, h->fg, h->invfg, h->rot_tz, h->status);
+    TF_HIP(hipGetLastError());
+}
+
+void launch_mean_f_weights(tf_handle* h)
+{
+    if (h->E <= 0) return;
+    hipLaunchKernelGGL(k_mean_f_weights, dim3((h->E + 255) / 256), dim3(256), 0, h->stream, h->xref, h->E, h->fmw);
     TF_HIP(hipGetLastError());
 }
 

@@ -137,6 +137,7 @@ struct PackingState {
     double *f = nullptr, *d = nullptr;   // [3N] packing force and Jacobian diagonal of every DoF
     std::vector<int64_t> h_blocks;       // pair blocks (min * N + max) kept in the BSR pattern
     hipEvent_t ev_count = nullptr;       // recorded behind the pair count's copy to the host
+    hipGraphExec_t graph = nullptr;      // the detection's kernels up to the pair count, captured once
 };
 
 // Everything a Krylov tile kernel reads about its tile, in one 32-byte record: its tissue, BSR
@@ -167,6 +168,7 @@ struct tf_handle {
     // element data
     int32_t *conn = nullptr;      // [6][E]
     double *xref = nullptr;       // [18][E]
+    double *fmw = nullptr;        // [18][E] c_a of the mean deformation gradient Fm = sum_a x_a c_a^T
     double *fg = nullptr, *invfg = nullptr;  // [9][E]
     double *lam = nullptr, *mu = nullptr, *eta = nullptr;  // [E]
     double *rate = nullptr;       // [3][E]
@@ -235,6 +237,8 @@ struct tf_handle {
                                       // update, [8] the packing detection's pair count
     NrReport* nr_rep = nullptr;       // device report of a Newton iteration (nr_report_bytes(S))
     NrReport* hp_nr_rep = nullptr;    // pinned host copy
+    double* hp_pos = nullptr;         // [3N] pinned, device-mapped staging of tf_get_positions
+    double* dp_pos = nullptr;         //      its device address
     int32_t host_waits = 0;           // host waits for the device since the step began
     // Whole Newton iterations captured as graphs (nr_graph = 2), one per shape of their launches:
     // Newton iteration, kept warm-start updates, solver kind, value and tangent precision, first
@@ -417,6 +421,7 @@ void launch_element_dump(tf_handle* h, int32_t e0, int32_t n, double dt, double*
 void scratch_init(int device);
 double* scratch_addr(int device);
 void launch_grow(tf_handle* h, double dt);
+void launch_mean_f_weights(tf_handle* h);     // fmw from xref (once, after the reference upload)
 void launch_surfaces(tf_handle* h);
 void launch_copy(tf_handle* h, double* dst, const double* src, int64_t n);
 void launch_fill(tf_handle* h, double* dst, double v, int64_t n);

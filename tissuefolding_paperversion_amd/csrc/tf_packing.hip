@@ -861,6 +861,7 @@ void packing_free(tf_handle* h)
     pk_free(pk.scan_tmp); pk_free(pk.a); pk_free(pk.b); pk_free(pk.w); pk_free(pk.slot); pk_free(pk.nodes);
     pk_free(pk.nptr); pk_free(pk.nref);
     if (pk.ev_count) { (void)hipEventDestroy(pk.ev_count); pk.ev_count = nullptr; }
+    if (pk.graph) { (void)hipGraphExecDestroy(pk.graph); pk.graph = nullptr; }
 }
 
 // Detection work arrays and node tables, allocated on first use.
@@ -1047,9 +1048,10 @@ void launch_reference_box(tf_handle* h, double* stats)
     TF_HIP(hipGetLastError());
 }
 
-void packing_detect_launch(tf_handle* h)
+// The detection's kernels up to the pair count (fixed arguments for the handle's lifetime: device
+// tables allocated once, sizes of the mesh)
+static void packing_detect_kernels(tf_handle* h)
 {
-    packing_prepare(h);
     PackingState& pk = h->pk;
     const int N = h->N, S = h->S;
     const uint32_t hmask = (uint32_t)pk.H - 1u;
@@ -1074,6 +1076,33 @@ void packing_detect_launch(tf_handle* h)
                        pk.cell_nodes, pk.adj_ptr, pk.adj_col, hmask, pk.pcount);
     TF_HIP(hipGetLastError());
     scan_i32(h, pk.pcount, N, pk.pstart);
+}
+
+void packing_detect_launch(tf_handle* h)
+{
+    packing_prepare(h);
+    PackingState& pk = h->pk;
+    const int N = h->N;
+    if (h->opts.use_graph) {  // the 17 small kernels from one graph: one launch for the host
+        if (!pk.graph) {
+            hipGraph_t gr;
+            TF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                packing_detect_kernels(h);
+            } catch (...) {
+                (void)hipStreamEndCapture(h->stream, &gr);
+                throw;
+            }
+            TF_HIP(hipStreamEndCapture(h->stream, &gr));
+            const hipError_t e = hipGraphInstantiate(&pk.graph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) pk.graph = nullptr;
+            TF_HIP(e);
+        }
+        TF_HIP(hipGraphLaunch(pk.graph, h->stream));
+    } else {
+        packing_detect_kernels(h);
+    }
     if (!pk.ev_count) TF_HIP(hipEventCreateWithFlags(&pk.ev_count, hipEventDisableTiming));
     TF_HIP(hipMemcpyAsync(h->h_status + 8, pk.pstart + N, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     TF_HIP(hipEventRecord(pk.ev_count, h->stream));
