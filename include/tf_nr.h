@@ -138,6 +138,9 @@ typedef struct tf_solver_opts {
                                   as a cooperative launch (co-residency guaranteed by the runtime,
                                   slower to launch); 0: two launches per iteration, replayed from
                                   graphs                                                         */
+    int32_t tl_barrier_polls;  /* polls of one grid-barrier wait of the persistent solve before it
+                                  gives up (0, the default: 2^22); a small value forces the
+                                  fallback that a grid without co-residency takes              */
 } tf_solver_opts;
 
 typedef struct tf_step_stats {

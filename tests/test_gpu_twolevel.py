@@ -255,3 +255,21 @@ def test_persistent_solve_equals_two_launches(case):
                (b["nr_iterations"], b["krylov_iterations"], b["krylov_by_newton"]), step
         assert np.array_equal(two.positions, one.positions), step
         assert a["krylov_persistent"] == 0 and b["krylov_persistent"] == b["nr_iterations"]
+
+
+def test_persistent_barrier_timeout_falls_back():
+    """A grid-barrier wait of the persistent solve that gives up (here forced from the host with a
+    limit of one poll, as a grid that is not co-resident would meet it) sets the abort word: every
+    workgroup leaves the launch, the solve is reported stalled and the step falls back to BiCGSTAB
+    from zero.  The steps still converge, to the positions of the persistent run."""
+    t = _ellipse()
+    ref, forced = _solver(t, **TL), _solver(t, tl_barrier_polls=1, **TL)
+    fell_back = 0
+    for step in range(3):
+        a, b = ref.runOneStep(t.dt), forced.runOneStep(t.dt)
+        assert a["converged"] == 1 and b["converged"] == 1, step
+        assert a["krylov_fallbacks"] == 0
+        fell_back += b["krylov_fallbacks"]
+        scale = np.max(np.abs(ref.positions))
+        assert np.max(np.abs(ref.positions - forced.positions)) <= 1e-8 * scale, step
+    assert fell_back > 0

@@ -57,7 +57,8 @@ class SolverOpts(C.Structure):
                 ("krylov_forcing", C.c_int32), ("xcd_tile_map", C.c_int32), ("krylov_warm_start", C.c_int32),
                 ("implicit_packing", C.c_int32), ("preconditioner", C.c_int32), ("coarse_aggregates", C.c_int32),
                 ("coarse_refresh", C.c_int32), ("column_blocks", C.c_int32), ("precond_fp64", C.c_int32),
-                ("mixed_fp64_from", C.c_int32), ("nr_graph", C.c_int32), ("tl_persistent", C.c_int32)]
+                ("mixed_fp64_from", C.c_int32), ("nr_graph", C.c_int32), ("tl_persistent", C.c_int32),
+                ("tl_barrier_polls", C.c_int32)]
 
 
 class GrowthFunction(C.Structure):

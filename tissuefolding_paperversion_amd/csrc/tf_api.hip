@@ -551,6 +551,7 @@ void tf_default_options(tf_solver_opts* o)
     o->mixed_fp64_from = 1;
     o->nr_graph = 1;
     o->tl_persistent = 1;
+    o->tl_barrier_polls = 0;
 }
 
 const char* tf_last_error(const tf_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -753,7 +754,7 @@ int tf_set_options(tf_handle* h, const tf_solver_opts* o)
         o->preconditioner < TF_PRECOND_AUTO || o->preconditioner > TF_PRECOND_TWO_LEVEL ||
         o->coarse_aggregates < 0 || o->coarse_refresh < 0 || (o->column_blocks != 0 && o->column_blocks != 1) ||
         (o->precond_fp64 != 0 && o->precond_fp64 != 1) || o->nr_graph < 0 || o->nr_graph > 2 ||
-        (o->tl_persistent < 0 || o->tl_persistent > 2))
+        (o->tl_persistent < 0 || o->tl_persistent > 2) || o->tl_barrier_polls < 0)
         return fail(h, TF_INVALID, "invalid solver options");
     const bool coarse_changed = o->coarse_aggregates != h->opts.coarse_aggregates ||
                                 o->column_blocks != h->opts.column_blocks || o->precond_fp64 != h->opts.precond_fp64;

@@ -288,6 +288,7 @@ struct tf_handle {
     // the solve).  Work arrays: per-tile and per-tissue Gram entries; the coefficients are solved
     // on the device by every tile of the tissue.
     double* warm_hist[TF_WARM_ITERS][TF_WARM_HIST] = {};
+    double* warm_block = nullptr;     // the buffers warm_hist points into (one allocation)
     int32_t warm_count[TF_WARM_ITERS] = {};
     double* warm_x0 = nullptr;
     double* warm_part = nullptr;      // [TF_WARM_DOTS][T]

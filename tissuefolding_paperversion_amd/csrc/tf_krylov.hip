@@ -1181,6 +1181,7 @@ struct TPRunArgs {
     const int32_t* halo_pos;  //      ... as positions
     unsigned* bar;            // [TPR_WORDS] barrier words, zero at the launch
     int maxit;
+    unsigned spin;            // polls of one barrier wait before it gives up
 };
 
 __device__ inline double ld_ag(const double* p)
@@ -1192,7 +1193,7 @@ __device__ inline void st_ag(double* p, double v) { __hip_atomic_store(p, v, __A
 // Grid barrier number k (1-based), in two halves so that loads that do not depend on the barrier can
 // be issued between them.  tpr_arrive: every store of the workgroup completes (s_waitcnt), the
 // workgroup arrives on its group's word, and the group's last arrival adds one to the global word.
-// tpr_wait: the workgroup waits for the global word to reach TPR_NG k.  A wait longer than TPR_SPIN
+// tpr_wait: the workgroup waits for the global word to reach TPR_NG k.  A wait longer than spin
 // polls sets the abort word; false when abort is set (the caller leaves its loop; every later wait
 // sees it).
 __device__ inline void tpr_arrive(unsigned* bar, unsigned k)
@@ -1208,7 +1209,7 @@ __device__ inline void tpr_arrive(unsigned* bar, unsigned k)
     }
 }
 
-__device__ inline bool tpr_wait(unsigned* bar, unsigned k)
+__device__ inline bool tpr_wait(unsigned* bar, unsigned k, unsigned spin)
 {
     __shared__ int pass;
     if (threadIdx.x == 0) {
@@ -1218,7 +1219,7 @@ __device__ inline bool tpr_wait(unsigned* bar, unsigned k)
         unsigned n = 0;
         while (__hip_atomic_load(glob, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)TPR_NG * k) {
             if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
-            if (++n > TPR_SPIN) {
+            if (++n > spin) {
                 __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -1435,7 +1436,7 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         double vx[TP_NV];
 #pragma unroll
         for (int m = 0; m < TP_NV; ++m) vx[m] = ld_ag(rcp + (tid + WG * m) % (last + 1));  // wrapped, not clamped
-        if (!tpr_wait(ra.bar, nbar)) { aborted = true; break; }
+        if (!tpr_wait(ra.bar, nbar, ra.spin)) { aborted = true; break; }
         // ---- alpha, the coarse correction, x, r, z (k_tp_z) ----
         const int slz[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
         tpr_partials_load<3>(opq(ta.partial), ta.T, R.st0, R.st1, slz, pl);
@@ -1534,7 +1535,7 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         pold = pnew;
         it += 1;
         tpr_arrive(ra.bar, ++nbar);
-        if (!tpr_wait(ra.bar, nbar)) { aborted = true; break; }
+        if (!tpr_wait(ra.bar, nbar, ra.spin)) { aborted = true; break; }
     }
     if (mine) { x[d] = xv; r[d] = rv; }
     if (lead) {
@@ -2659,13 +2660,15 @@ void warm_start_launch(tf_handle* h, int k)
 void warm_start_push(tf_handle* h, int k)
 {
     const size_t n3 = 3 * (size_t)h->N;
-    if (!h->warm_tab) {
+    if (!h->warm_tab) {  // every buffer of the history in one allocation, at the first push
         TF_HIP(hipMalloc((void**)&h->warm_tab, sizeof(double*) * TF_WARM_ITERS * TF_WARM_HIST));
         TF_HIP(hipHostMalloc((void**)&h->hp_warm_tab, sizeof(double*) * TF_WARM_ITERS * TF_WARM_HIST,
                              hipHostMallocDefault));
+        TF_HIP(hipMalloc((void**)&h->warm_block, sizeof(double) * n3 * TF_WARM_ITERS * TF_WARM_HIST));
+        for (int i = 0; i < TF_WARM_ITERS; ++i)
+            for (int j = 0; j < TF_WARM_HIST; ++j) h->warm_hist[i][j] = h->warm_block + n3 * (i * TF_WARM_HIST + j);
     }
     double* last = h->warm_hist[k][TF_WARM_HIST - 1];
-    if (!last) TF_HIP(hipMalloc((void**)&last, n3 * sizeof(double)));
     for (int j = TF_WARM_HIST - 1; j > 0; --j) h->warm_hist[k][j] = h->warm_hist[k][j - 1];
     h->warm_hist[k][0] = last;  // the oldest buffer is reused for the newest update
     launch_copy(h, last, h->du, (int64_t)n3);
@@ -2686,9 +2689,8 @@ void warm_start_clear(tf_handle* h)
 void warm_start_free(tf_handle* h)
 {
     for (int k = 0; k < TF_WARM_ITERS; ++k)
-        for (int j = 0; j < TF_WARM_HIST; ++j)
-            if (h->warm_hist[k][j]) { (void)hipFree(h->warm_hist[k][j]); h->warm_hist[k][j] = nullptr; }
-    for (double** p : {&h->warm_x0, &h->warm_part, &h->warm_seg})
+        for (int j = 0; j < TF_WARM_HIST; ++j) h->warm_hist[k][j] = nullptr;
+    for (double** p : {&h->warm_block, &h->warm_x0, &h->warm_part, &h->warm_seg})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (h->warm_tab) { (void)hipFree(h->warm_tab); h->warm_tab = nullptr; }
     if (h->hp_warm_tab) { (void)hipHostFree(h->hp_warm_tab); h->hp_warm_tab = nullptr; }
@@ -3593,7 +3595,8 @@ static void launch_tp_run(tf_handle* h)
     TPArgs tp = tp_args(h);
     const VT* val = tp_values<VT>(h);
     int nb = h->nb;
-    TPRunArgs ra{tl.pslot, tl.halo_ptr, tl.halo_pos, tl.bar, h->opts.krylov_max_iter};
+    TPRunArgs ra{tl.pslot, tl.halo_ptr, tl.halo_pos, tl.bar, h->opts.krylov_max_iter,
+                 h->opts.tl_barrier_polls > 0 ? (unsigned)h->opts.tl_barrier_polls : TPR_SPIN};
     double *x = h->kx, *r = h->kr, *z = h->ks, *p0 = h->kp, *p1 = h->kph;
     void* args[] = {&ta, &tp, &val, &nb, &ra, &x, &r, &z, &p0, &p1};
     // The occupancy check of tp_run_fits puts every workgroup on the device at once when nothing
