@@ -744,9 +744,9 @@ __device__ inline void tp_partials_load(const TileArgs& ta, int st0, int st1, co
 
 // Tissue totals from the loaded partials in the order of seg_sums_range (lane sums in index order,
 // then a fixed wave and tree order), identical in every lane and every workgroup of the tissue.
-template <int K>
+template <int K, int SU = TP_SU>
 __device__ inline void tp_partials_sum(const TileArgs& ta, int st0, int st1, const int (&sl)[K],
-                                       const double (&v)[K][TP_SU], double (&out)[K])
+                                       const double (&v)[K][SU], double (&out)[K])
 {
     constexpr int NW = WG / 64;
     __shared__ double red[K][NW];
@@ -754,11 +754,11 @@ __device__ inline void tp_partials_sum(const TileArgs& ta, int st0, int st1, con
 #pragma unroll
     for (int q = 0; q < K; ++q) s[q] = 0.0;
 #pragma unroll
-    for (int u = 0; u < TP_SU; ++u)
+    for (int u = 0; u < SU; ++u)
         if (st0 + (int)threadIdx.x + u * WG < st1)
 #pragma unroll
             for (int q = 0; q < K; ++q) s[q] += v[q][u];
-    for (int k = st0 + (int)threadIdx.x + TP_SU * WG; k < st1; k += WG)  // beyond TP_SU * WG: not built
+    for (int k = st0 + (int)threadIdx.x + SU * WG; k < st1; k += WG)  // beyond SU * WG: not built
 #pragma unroll
         for (int q = 0; q < K; ++q) s[q] += ta.partial[(size_t)sl[q] * ta.T + k];
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
@@ -1159,13 +1159,15 @@ static size_t tp_z_lds(int ldA, bool f64) { return (f64 ? sizeof(double) : sizeo
 // an XCD is not coherent with the others', and those accesses bypass it.  The arithmetic is that of
 // k_tp_q and k_tp_z, in the same order, so the iterates are the same bit for bit.
 // Eligible: one tissue; every aggregate one pass (<= WG / 3 rows, <= TP_PF WG blocks, <= WG / 3L
-// columns) with at most TPR_MAXU own and neighbouring nodes; at least TPR_NG aggregates; a grid of
-// n_agg workgroups that the device holds at once.
+// columns) with at most TPR_MAXU own and neighbouring nodes; TPR_NG to TPR_SU * WG aggregates; a
+// grid of n_agg workgroups that the device holds at once.
 constexpr int TPR_MAXU = 192;           // own + neighbouring nodes of an aggregate (LDS staging)
 constexpr int TPR_NG = 8;               // barrier arrival groups: workgroup b arrives on word b % TPR_NG
 constexpr int TPR_LINE = 32;            // the barrier words 128 bytes apart
 constexpr unsigned TPR_SPIN = 1u << 22; // polls of one barrier wait before it gives up (abort)
 constexpr int TPR_WORDS = (TPR_NG + 2) * TPR_LINE;  // group words, the global word, the abort word
+constexpr int TPR_SU = 2;               // partials per lane and slot: at most TPR_SU * WG aggregates
+constexpr int TPR_NV = 12;              // coarse values per lane: 6 TPR_SU WG coarse unknowns at most
 
 // doubles of k_tp_run's LDS that the products of q = M p and then r_c share
 __host__ __device__ inline int tp_run_share(int ldA, bool f64)
@@ -1246,10 +1248,10 @@ __device__ inline T opq(T v)
 // served by the L2, and hundreds of workgroups reading one line at once would queue at its channel.
 template <int K>
 __device__ inline void tpr_partials_load(const double* part, int T, int st0, int st1, const int (&sl)[K],
-                                         double (&v)[K][TP_SU])
+                                         double (&v)[K][TPR_SU])
 {
 #pragma unroll
-    for (int u = 0; u < TP_SU; ++u) {
+    for (int u = 0; u < TPR_SU; ++u) {
         const int k = st0 + (int)threadIdx.x + u * WG;
         const int kk = k < st1 ? k : st0 + (k - st0) % max(1, st1 - st0);
 #pragma unroll
@@ -1325,6 +1327,16 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
     const bool lead = a == R.st0 && tid == 0;
     unsigned nbar = 0;
     bool aborted = false;
+    // (r, z) of the last iteration (beta's denominator), carried from one iteration to the next;
+    // a launch that starts after iteration 0 sums it once
+    double rz_prev = 0.0;
+    if (it > 0) {
+        const int sl0[1] = {SL_RZ0 + (1 - (it & 1))};
+        double p0[1][TPR_SU], s0[1];
+        tpr_partials_load<1>(ta.partial, ta.T, R.st0, R.st1, sl0, p0);
+        tp_partials_sum<1, TPR_SU>(ta, R.st0, R.st1, sl0, p0, s0);
+        rz_prev = s0[0];
+    }
     // loads that do not depend on other workgroups: the blocks and P's row of q = M p with the
     // partials at the start of that half; the column-block row and P's row of z between the halves
     // of the barrier before it
@@ -1363,9 +1375,9 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         const double* pob = opq(P ? pb1 : pb0);  // p of the last iteration
         double* pnb = opq(P ? pb0 : pb1);        // p of this one
         // ---- q = M p (k_tp_q) ----
-        const int slq[3] = {SL_RR0 + P, SL_RZ0 + P, SL_RZ0 + (1 - P)};
-        double pl[3][TP_SU];
-        tpr_partials_load<3>(part, ta.T, R.st0, R.st1, slq, pl);
+        const int slq[2] = {SL_RR0 + P, SL_RZ0 + P};
+        double pl[2][TPR_SU];
+        tpr_partials_load<2>(part, ta.T, R.st0, R.st1, slq, pl);
         __builtin_amdgcn_sched_barrier(0);  // the partials' loads go out before the blocks' (they return in order)
         load_q();
         double hz[2], hp[2];
@@ -1375,8 +1387,8 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
             hz[u] = ld_ag(zb + q);
             hp[u] = ld_ag(pob + q);
         }
-        double sc[3];
-        tp_partials_sum<3>(ta, R.st0, R.st1, slq, pl, sc);
+        double sc[2];
+        tp_partials_sum<2, TPR_SU>(ta, R.st0, R.st1, slq, pl, sc);
         if (mine) { zst[tid] = zv; pst[tid] = pold; }
 #pragma unroll
         for (int u = 0; u < 2; ++u)
@@ -1395,7 +1407,11 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
             }
         }
         if (sc[0] < 0.5 * best) { best = sc[0]; best_it = it; }
-        const double beta = it > 0 ? sc[1] / sc[2] : 0.0;
+        const double beta = it > 0 ? sc[1] / rz_prev : 0.0;
+        // (r, r) and (r, z) of this iteration: the second half needs them again and the next
+        // iteration's beta (r, z) (the same sums of the same partials); only (p, q) is new there
+        const double rr_it = sc[0], rz_it = sc[1];
+        rz_prev = rz_it;
         __syncthreads();  // zst, pst staged
 #pragma unroll
         for (int k = 0; k < TP_PF; ++k) {
@@ -1433,26 +1449,28 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         // P^T r of the last iteration (written before the last barrier) is loaded while this one
         // completes; P^T q of this iteration after it
         const double* rcp = opq(tp.rcx) + (size_t)P * nc + C0;
-        double vx[TP_NV];
+        double vx[TPR_NV];
 #pragma unroll
-        for (int m = 0; m < TP_NV; ++m) vx[m] = ld_ag(rcp + (tid + WG * m) % (last + 1));  // wrapped, not clamped
+        for (int m = 0; m < TPR_NV; ++m) vx[m] = ld_ag(rcp + (tid + WG * m) % (last + 1));  // wrapped, not clamped
         if (!tpr_wait(ra.bar, nbar, ra.spin)) { aborted = true; break; }
         // ---- alpha, the coarse correction, x, r, z (k_tp_z) ----
-        const int slz[3] = {SL_RZ0 + P, SL_PQ, SL_RR0 + P};
-        tpr_partials_load<3>(opq(ta.partial), ta.T, R.st0, R.st1, slz, pl);
+        const int slz[1] = {SL_PQ};
+        double plz[1][TPR_SU];
+        tpr_partials_load<1>(opq(ta.partial), ta.T, R.st0, R.st1, slz, plz);
         __builtin_amdgcn_sched_barrier(0);
         const double* ptq = opq(tp.ptq) + C0;
-        double vq[TP_NV];
+        double vq[TPR_NV];
 #pragma unroll
-        for (int m = 0; m < TP_NV; ++m) vq[m] = ld_ag(ptq + (tid + WG * m) % (last + 1));
-        tp_partials_sum<3>(ta, R.st0, R.st1, slz, pl, sc);
-        if (!(sc[1] > 0.0) || !isfinite(sc[1]) || !(sc[0] > 0.0) || !isfinite(sc[0])) {
-            if (lead) { S.rnorm2 = sc[2]; S.stalled = 1; }
+        for (int m = 0; m < TPR_NV; ++m) vq[m] = ld_ag(ptq + (tid + WG * m) % (last + 1));
+        double pqs[1];
+        tp_partials_sum<1, TPR_SU>(ta, R.st0, R.st1, slz, plz, pqs);
+        if (!(pqs[0] > 0.0) || !isfinite(pqs[0]) || !(rz_it > 0.0) || !isfinite(rz_it)) {
+            if (lead) { S.rnorm2 = rr_it; S.stalled = 1; }
             break;
         }
-        const double alpha = sc[0] / sc[1];
+        const double alpha = rz_it / pqs[0];
 #pragma unroll
-        for (int m = 0; m < TP_NV; ++m) {
+        for (int m = 0; m < TPR_NV; ++m) {
             const int i = tid + WG * m;
             if (i < ldA) {
                 if constexpr (F64) rcd[i] = i < ncs ? vx[m] - alpha * vq[m] : 0.0;
@@ -3116,7 +3134,7 @@ static void twolevel_build(tf_handle* h)
         // the persistent solve (k_tp_run): every aggregate one pass with at most WG / 3L columns and
         // TPR_MAXU own and neighbouring nodes; the column slot of every block; one tissue
         {
-            bool ok = S == 1 && A >= TPR_NG && nc <= WG * TP_NV;
+            bool ok = S == 1 && A >= TPR_NG && A <= TPR_SU * WG && tl.max_ldA <= WG * TPR_NV;
             std::vector<int32_t> pslot(h->nb, 0), hptr(A + 1, 0), hpos;
             for (int a = 0; a < A; ++a) {
                 const int n0 = agg_ptr[a], n1 = agg_ptr[a + 1], nown = n1 - n0;
