@@ -1193,16 +1193,19 @@ __device__ inline double ld_ag(const double* p)
 __device__ inline void st_ag(double* p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Grid barrier number k (1-based), in two halves so that loads that do not depend on the barrier can
-// be issued between them.  tpr_arrive: every store of the workgroup completes (s_waitcnt), the
-// workgroup arrives on its group's word, and the group's last arrival adds one to the global word.
+// be issued between them.  tpr_arrive: every store of every wave of the workgroup completes
+// (s_waitcnt in each wave, then the workgroup barrier), the workgroup arrives on its group's word,
+// and the group's last arrival adds one to the global word.
 // tpr_wait: the workgroup waits for the global word to reach TPR_NG k.  A wait longer than spin
 // polls sets the abort word; false when abort is set (the caller leaves its loop; every later wait
 // sees it).
 __device__ inline void tpr_arrive(unsigned* bar, unsigned k)
 {
+    // every wave's stores (agent-scope: coherent once complete) are complete before the workgroup
+    // arrives; the workgroup barrier itself waits only for LDS
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_s_waitcnt(0);
         unsigned* glob = bar + TPR_NG * TPR_LINE;
         const int G = (int)gridDim.x, g = (int)blockIdx.x % TPR_NG;
         const unsigned Gg = (unsigned)((G - g + TPR_NG - 1) / TPR_NG);  // workgroups of group g (G >= TPR_NG)
