@@ -677,7 +677,17 @@ struct TPArgs {
     double* ptq;               // [nc] P^T q of the current iteration
     double* rcx;               // [2][nc] P^T r (ping-pong by parity)
     int nc, n_agg;
+    int c32;                   // ptq and rcx hold FP32 values (not under FP64 preconditioner storage):
+                               // every workgroup of the persistent solve gathers both, and the
+                               // coarse product takes its right-hand side in FP32 anyway
 };
+
+// Stores an entry of ptq / rcx in the storage TPArgs::c32 names
+__device__ inline void coarse_put(const TPArgs& tp, double* base, size_t i, double v)
+{
+    if (tp.c32) reinterpret_cast<float*>(base)[i] = (float)v;
+    else base[i] = v;
+}
 template <bool F64>
 __device__ inline std::conditional_t<F64, double, float> dc_row(const TPArgs& tp, size_t i)
 {
@@ -830,7 +840,7 @@ __global__ __launch_bounds__(WG) void k_tp_init(TileArgs ta, TPArgs tp, const VT
     __shared__ double out[7];
     tp_reduce<7>(v, out);
     if (threadIdx.x == 0) ta.partial[(size_t)SL_BB * ta.T + a] = out[0];
-    if ((int)threadIdx.x < R.kr) tp.rcx[tp.nc + R.c0 + threadIdx.x] = out[1 + threadIdx.x];
+    if ((int)threadIdx.x < R.kr) coarse_put(tp, tp.rcx, (size_t)tp.nc + R.c0 + threadIdx.x, out[1 + threadIdx.x]);
 }
 
 // beta, p = z + beta p_old, q = M p, (p, q) and P^T q; the convergence, cap and stagnation test.
@@ -950,7 +960,7 @@ __global__ __launch_bounds__(WG) void k_tp_q(TileArgs ta, TPArgs tp, const VT* _
     __shared__ double out[7];
     tp_reduce<7>(v, out);
     if (tid == 0) ta.partial[(size_t)SL_PQ * ta.T + R.a] = out[0];
-    if (tid < R.kr) tp.ptq[R.c0 + tid] = out[1 + tid];
+    if (tid < R.kr) coarse_put(tp, tp.ptq, (size_t)R.c0 + tid, out[1 + tid]);
 }
 
 // alpha, the coarse right-hand side, z_c, then x, r, z on the aggregate's rows, (r, z), (r, r) and
@@ -1015,14 +1025,17 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
             for (int m = 0; m < TP_Q4; ++m)
                 pre[k][m] = Ab[(size_t)min(lc0 + k, last) * lq + min(tid + WG * m, lq - 1)];
     }
-    const double* rcp = tp.rcx + (size_t)P * nc + C0;
-    const double* ptq = tp.ptq + C0;
     double vx[TP_NV], vq[TP_NV];
 #pragma unroll
     for (int m = 0; m < TP_NV; ++m) {
         const int ic = min(tid + WG * m, last);  // clamped: always a valid address
-        vx[m] = rcp[ic];
-        vq[m] = INIT ? 0.0 : ptq[ic];
+        if constexpr (F64) {
+            vx[m] = tp.rcx[(size_t)P * nc + C0 + ic];
+            vq[m] = INIT ? 0.0 : tp.ptq[C0 + ic];
+        } else {
+            vx[m] = reinterpret_cast<const float*>(tp.rcx)[(size_t)P * nc + C0 + ic];
+            vq[m] = INIT ? 0.0 : reinterpret_cast<const float*>(tp.ptq)[C0 + ic];
+        }
     }
     bool own = slot < CPP && slot < ncols;
     int g = n0 / L + (own ? slot : 0);  // global column; its positions gL .. gL + L - 1
@@ -1143,7 +1156,7 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
         ta.partial[(size_t)(SL_RR0 + (1 - P)) * ta.T + a] = out[1];
         if (!INIT && a == R.st0) S.iters += 1;
     }
-    if (tid < kr) tp.rcx[(size_t)(1 - P) * nc + c0 + tid] = out[2 + tid];
+    if (tid < kr) coarse_put(tp, tp.rcx, (size_t)(1 - P) * nc + c0 + tid, out[2 + tid]);
 }
 
 static size_t tp_z_lds(int ldA, bool f64) { return (f64 ? sizeof(double) : sizeof(float)) * (size_t)ldA; }  // r_c
@@ -1191,6 +1204,11 @@ __device__ inline double ld_ag(const double* p)
     return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_ag(double* p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline float ld_ag32(const float* p)
+{
+    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_ag32(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Grid barrier number k (1-based), in two halves so that loads that do not depend on the barrier can
 // be issued between them.  tpr_arrive: every store of every wave of the workgroup completes
@@ -1445,26 +1463,41 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
             double v[7] = {pq, pt[0], pt[1], pt[2], pt[3], pt[4], pt[5]};
             tp_reduce<7>(v, outq);
             if (tid == 0) st_ag(part + (size_t)SL_PQ * ta.T + a, outq[0]);
-            if (tid < kr) st_ag(opq(tp.ptq) + R.c0 + tid, outq[1 + tid]);
+            if (tid < kr) {
+                if constexpr (F64) st_ag(opq(tp.ptq) + R.c0 + tid, outq[1 + tid]);
+                else st_ag32(reinterpret_cast<float*>(opq(tp.ptq)) + R.c0 + tid, (float)outq[1 + tid]);
+            }
         }
         tpr_arrive(ra.bar, ++nbar);
         load_z();
         // P^T r of the last iteration (written before the last barrier) is loaded while this one
         // completes; P^T q of this iteration after it
-        const double* rcp = opq(tp.rcx) + (size_t)P * nc + C0;
         double vx[TPR_NV];
+        if constexpr (F64) {
+            const double* rcp = opq(tp.rcx) + (size_t)P * nc + C0;
 #pragma unroll
-        for (int m = 0; m < TPR_NV; ++m) vx[m] = ld_ag(rcp + (tid + WG * m) % (last + 1));  // wrapped, not clamped
+            for (int m = 0; m < TPR_NV; ++m) vx[m] = ld_ag(rcp + (tid + WG * m) % (last + 1));  // wrapped, not clamped
+        } else {
+            const float* rcp = reinterpret_cast<const float*>(opq(tp.rcx)) + (size_t)P * nc + C0;
+#pragma unroll
+            for (int m = 0; m < TPR_NV; ++m) vx[m] = (double)ld_ag32(rcp + (tid + WG * m) % (last + 1));
+        }
         if (!tpr_wait(ra.bar, nbar, ra.spin)) { aborted = true; break; }
         // ---- alpha, the coarse correction, x, r, z (k_tp_z) ----
         const int slz[1] = {SL_PQ};
         double plz[1][TPR_SU];
         tpr_partials_load<1>(opq(ta.partial), ta.T, R.st0, R.st1, slz, plz);
         __builtin_amdgcn_sched_barrier(0);
-        const double* ptq = opq(tp.ptq) + C0;
         double vq[TPR_NV];
+        if constexpr (F64) {
+            const double* ptq = opq(tp.ptq) + C0;
 #pragma unroll
-        for (int m = 0; m < TPR_NV; ++m) vq[m] = ld_ag(ptq + (tid + WG * m) % (last + 1));
+            for (int m = 0; m < TPR_NV; ++m) vq[m] = ld_ag(ptq + (tid + WG * m) % (last + 1));
+        } else {
+            const float* ptq = reinterpret_cast<const float*>(opq(tp.ptq)) + C0;
+#pragma unroll
+            for (int m = 0; m < TPR_NV; ++m) vq[m] = (double)ld_ag32(ptq + (tid + WG * m) % (last + 1));
+        }
         double pqs[1];
         tp_partials_sum<1, TPR_SU>(ta, R.st0, R.st1, slz, plz, pqs);
         if (!(pqs[0] > 0.0) || !isfinite(pqs[0]) || !(rz_it > 0.0) || !isfinite(rz_it)) {
@@ -1551,7 +1584,10 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
                 st_ag(pz + (size_t)(SL_RZ0 + (1 - P)) * ta.T + a, outz[0]);
                 st_ag(pz + (size_t)(SL_RR0 + (1 - P)) * ta.T + a, outz[1]);
             }
-            if (tid < kr) st_ag(opq(tp.rcx) + (size_t)(1 - P) * nc + R.c0 + tid, outz[2 + tid]);
+            if (tid < kr) {
+                if constexpr (F64) st_ag(opq(tp.rcx) + (size_t)(1 - P) * nc + R.c0 + tid, outz[2 + tid]);
+                else st_ag32(reinterpret_cast<float*>(opq(tp.rcx)) + (size_t)(1 - P) * nc + R.c0 + tid, (float)outz[2 + tid]);
+            }
         }
         pold = pnew;
         it += 1;
@@ -3233,7 +3269,8 @@ static TPArgs tp_args(tf_handle* h)
     tf_handle::TwoLevel& tl = h->tl;
     return TPArgs{tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_rows, reinterpret_cast<const TPRec*>(tl.wg_rec),
                   tl.pcoef_p, tl.dcinv, tl.dcinv64, reinterpret_cast<const __half2*>(tl.Ainv),
-                  tl.Ainv64, tl.seg_aoff64, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg};
+                  tl.Ainv64, tl.seg_aoff64, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg,
+                  h->opts.precond_fp64 ? 0 : 1};
 }
 
 template <int P, bool INIT>
