@@ -18,10 +18,10 @@ tolerance 0 on the final Jacobian gives the time per iteration (tf_kernel_probe_
 grid does not fit co-resident) are probed beside it by 64 back-to-back launches in one graph replay.
 Algorithmic bytes (tf_kernel_bytes):
   k_tp_run, per iteration: nb (72 + 4) (blocks) + 2 N 18 x 8 (P, both halves)
-          + ncol (3L)^2 4 (column-block inverses) + 4 x 3N x 8 (z, p published and read) + 4 nc 8
+          + ncol (3L)^2 4 (column-block inverses) + 4 x 3N x 8 (z, p published and read) + 4 nc 4
   k_tp_q: nb (72 + 4) + (N + 1) 4 (matrix) + 4 x 3N x 8 (z, p_old read; p, q written) + N 18 x 8 (P)
   k_tp_z: nc ldA 2 (FP16 Ac^-1) + ncol (3L)^2 4 (column-block inverses) + N 18 x 8 (P)
-          + 7 x 3N x 8 (x, r read and written; p, q read; z written) + 3 nc 8 (P^T r, P^T q)
+          + 7 x 3N x 8 (x, r read and written; p, q read; z written) + 3 nc 4 (P^T r, P^T q)
 The achievable HBM rate is measured on the box by a 1 GiB device-to-device copy
 (tf_copy_bandwidth); whole-step algorithmic bytes (assembly kernels per NR iteration, one PCG
 iteration's bytes per Krylov iteration) over ms_per_step give the step's own fraction of peak.

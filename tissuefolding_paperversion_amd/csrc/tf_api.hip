@@ -2219,8 +2219,9 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
         if (!twolevel_active(h) || !h->tl.built) { *bytes = 0.0; return TF_OK; }
         const tf_handle::TwoLevel& tl = h->tl;
         const double D = 3.0 * tl.L, eb = h->opts.precond_fp64 ? 8.0 : 2.0, db = h->opts.precond_fp64 ? 8.0 : 4.0;
+        // (P^T q and P^T r are FP32 unless the preconditioner is stored in FP64: db)
         *bytes = (double)tl.ainv_halves * eb + (double)tl.ncol * D * D * db + N * 18.0 * 8.0 + n * 8.0 * 7.0 +
-                 tl.nc * 8.0 * 3.0;
+                 tl.nc * db * 3.0;
     } else if (!std::strcmp(name, "persistent")) {
         // one iteration of k_tp_run: the blocks' values (read again every iteration), P's rows (both
         // halves), the column-block inverses; z and p published and their neighbours' read; P^T q and
@@ -2228,7 +2229,7 @@ int tf_kernel_bytes(tf_handle* h, const char* name, double* bytes)
         if (!twolevel_active(h) || !h->tl.built) { *bytes = 0.0; return TF_OK; }
         const tf_handle::TwoLevel& tl = h->tl;
         const double D = 3.0 * tl.L, db = h->opts.precond_fp64 ? 8.0 : 4.0;
-        *bytes = nb * vb + 2.0 * N * 18.0 * 8.0 + (double)tl.ncol * D * D * db + 4.0 * n * 8.0 + 4.0 * tl.nc * 8.0;
+        *bytes = nb * vb + 2.0 * N * 18.0 * 8.0 + (double)tl.ncol * D * D * db + 4.0 * n * 8.0 + 4.0 * tl.nc * db;
     } else if (!std::strcmp(name, "spmv")) {
         // the launch timed under "spmv": the dominant Krylov kernel of the active solver
         if (twolevel_active(h))  // k_tp_q: SpMV on p = z + beta p_old: z, p_old read; p_new, q written; P's rows

@@ -1741,55 +1741,114 @@ __device__ inline void sw_mfma_rows_cols(const SwQuarter& q, const double* a, in
     }
 }
 
-// In-place Gauss-Jordan inverse of a 64 x 64 tile held in registers by 256 threads: thread t holds
-// v[c] = T[t / 4][16 (t % 4) + c].  Each pivot p publishes its row and column through LDS (two
-// buffers of 68 + 64 doubles, alternated, so one barrier per pivot suffices: a buffer is written
-// again only after every thread passed the barrier behind its last reads).  A pivot that is not
-// positive and finite sets *bad.
-__device__ void sw_tile_inverse(double (&v)[16], double* buf, int* bad)
+// In-place Gauss-Jordan inverse of a 4 x 4 block in registers (no pivoting: a Schur complement of
+// a positive definite matrix); ok goes false on a pivot that is not positive and finite
+__device__ inline void sw_inv4(const double (&a)[16], double (&d)[16], bool& ok)
 {
-    const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16, pad = j0 >> 4;
-    bool ok = true;
-    for (int p = 0; p < SW_B; ++p) {
-        double* rowb = buf + (p & 1) * 132;  // pivot row, entry j at j + j / 16
-        double* colb = rowb + 68;            // pivot column
-        if (i == p) {
 #pragma unroll
-            for (int c = 0; c < 16; ++c) rowb[j0 + c + pad] = v[c];
-        }
-        if (p >= j0 && p < j0 + 16) {
-            double cv = 0.0;
+    for (int q = 0; q < 16; ++q) d[q] = a[q];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) cv = j0 + c == p ? v[c] : cv;
-            colb[i] = cv;
-        }
-        __syncthreads();
-        // every load unconditional and issued together (a load used in one arm of a select is sunk
-        // into a branch of its own and waited for alone); the pivot column is patched afterwards by
-        // register selects
-        const double piv = rowb[p + (p >> 4)];
-        const double f = colb[i];
-        double rp[16];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) rp[c] = rowb[j0 + c + pad];
-        // the row is needed here: its loads are issued with the pivot's, ahead of the reciprocal,
-        // instead of one by one after it
-        asm volatile("" ::"v"(rp[0]), "v"(rp[1]), "v"(rp[2]), "v"(rp[3]), "v"(rp[4]), "v"(rp[5]), "v"(rp[6]), "v"(rp[7]));
-        asm volatile("" ::"v"(rp[8]), "v"(rp[9]), "v"(rp[10]), "v"(rp[11]), "v"(rp[12]), "v"(rp[13]), "v"(rp[14]),
-                     "v"(rp[15]));
+    for (int p = 0; p < 4; ++p) {
+        const double piv = d[5 * p];
         ok = ok && piv > 0.0 && isfinite(piv);
         // 1 / piv: the hardware reciprocal refined by two Newton steps (within an ulp or two; the
         // inverse only preconditions)
         double ip = __builtin_amdgcn_rcp(piv);
         ip = ip * (2.0 - piv * ip);
         ip = ip * (2.0 - piv * ip);
-        const double fi = i == p ? 0.0 : f * ip;  // the pivot row is scaled, the others reduced
-        const double sc = i == p ? ip : 1.0;
+        double rp[4];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) v[c] = sc * (v[c] - fi * rp[c]);
-        const double pc = i == p ? ip : -fi;  // the new pivot-column entry
+        for (int c = 0; c < 4; ++c) rp[c] = d[4 * p + c];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) v[c] = j0 + c == p ? pc : v[c];
+        for (int r = 0; r < 4; ++r) {
+            if (r == p) continue;
+            const double fi = d[4 * r + p] * ip;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) d[4 * r + c] = c == p ? -fi : d[4 * r + c] - fi * rp[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[4 * p + c] = c == p ? ip : rp[c] * ip;
+    }
+}
+
+// In-place Gauss-Jordan inverse of a 64 x 64 tile held in registers by 256 threads: thread t holds
+// v[c] = T[t / 4][16 (t % 4) + c].  The pivots go in blocks of four (16 steps, one barrier each):
+// for the pivot block P, D = T_PP^-1 (every thread inverts it in registers), then T_Pc <- D T_Pc,
+// T_iP <- -T_iP D and T_ic <- T_ic - T_iP D T_Pc, T_PP <- D.  The block's rows and every row's
+// entries in the block's columns go through LDS (SW_INV_LDS doubles, two alternating buffers, so
+// one barrier per step suffices: a buffer is written again only after every thread passed the
+// barrier behind its last reads).  A pivot that is not positive and finite sets *bad.
+constexpr int SW_RS = 68;                            // row stride of the staged pivot rows
+constexpr int SW_INV_LDS = 2 * (4 * SW_RS + 64 * 4);  // doubles
+__device__ void sw_tile_inverse(double (&v)[16], double* buf, int* bad)
+{
+    const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16;
+    bool ok = true;
+    for (int s = 0; s < SW_B / 4; ++s) {
+        const int p0 = 4 * s;
+        double* rows = buf + (s & 1) * (4 * SW_RS + 64 * 4);  // [4][SW_RS]
+        double* cols = rows + 4 * SW_RS;                      // [64][4]
+        const int ri = i - p0;                                // the thread's row within the block
+        if (ri >= 0 && ri < 4) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) rows[ri * SW_RS + j0 + c] = v[c];
+        }
+        const int pc = p0 - j0;  // the block's columns within the thread's 16, if in [0, 16)
+        if (pc >= 0 && pc < 16) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                double cv = 0.0;
+#pragma unroll
+                for (int c = 0; c < 16; ++c) cv = c == pc + k ? v[c] : cv;
+                cols[i * 4 + k] = cv;
+            }
+        }
+        __syncthreads();
+        double a[16], d[16], aiP[4], rp[4][16];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a[4 * r + c] = rows[r * SW_RS + p0 + c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) aiP[k] = cols[i * 4 + k];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) rp[r][c] = rows[r * SW_RS + j0 + c];
+        sw_inv4(a, d, ok);
+        if (ri >= 0 && ri < 4) {  // a pivot row: D T_Pc, and D in the block's columns
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc += d[4 * ri + k] * rp[k][c];
+                v[c] = acc;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[c] = j0 + c == p0 + k ? d[4 * ri + k] : v[c];
+        } else {  // w = T_iP D; T_ic - w T_Pc, and -w in the block's columns
+            double w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acc += aiP[m] * d[4 * m + k];
+                w[k] = acc;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                double acc = v[c];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc -= w[k] * rp[k][c];
+                v[c] = acc;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[c] = j0 + c == p0 + k ? -w[k] : v[c];
+        }
     }
     if (!ok && t == 0) *bad = 1;
 }
@@ -1806,7 +1865,7 @@ __global__ __launch_bounds__(256) void k_sw_clear(double* __restrict__ A, int n,
 // D_0 = A_00^-1, and the failure flag cleared
 __global__ __launch_bounds__(256) void k_sw_first(const double* __restrict__ A, int ldc, double* __restrict__ D, int* bad)
 {
-    __shared__ double buf[2 * 132];
+    __shared__ double buf[SW_INV_LDS];
     const int t = threadIdx.x, i = t >> 2, j0 = (t & 3) * 16;
     if (t == 0) *bad = 0;
     double v[16];
@@ -1916,6 +1975,7 @@ __global__ __launch_bounds__(256) void k_sw_update(double* __restrict__ A, int l
     // A_IJ <- A_IJ - P_I O_J^T on the matrix cores, the inner dimension in two halves staged in LDS;
     // the next pivot tile is then inverted through the same buffer (33.8 KB)
     __shared__ double sbuf[2 * SW_B * SW_H];
+    static_assert(SW_INV_LDS <= 2 * SW_B * SW_H, "the pivot buffers of the tile inverse fit in the staging");
     double* sP = sbuf;
     double* sO = sbuf + SW_B * SW_H;
     const SwQuarter q;
