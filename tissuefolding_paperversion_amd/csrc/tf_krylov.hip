@@ -677,6 +677,8 @@ struct TPArgs {
     double* ptq;               // [nc] P^T q of the current iteration
     double* rcx;               // [2][nc] P^T r (ping-pong by parity)
     int nc, n_agg;
+    unsigned* bar;             // the persistent solve's barrier words (null without it), cleared by
+    int bar_words;             //   the first z of every solve (k_tp_z<P, true>)
     int c32;                   // ptq and rcx hold FP32 values (not under FP64 preconditioner storage):
                                // every workgroup of the persistent solve gathers both, and the
                                // coarse product takes its right-hand side in FP32 anyway
@@ -998,6 +1000,8 @@ __global__ __launch_bounds__(WG) void k_tp_z(TileArgs ta, TPArgs tp, const doubl
                                              double* __restrict__ r, double* __restrict__ z)
 {
     constexpr int D = 3 * L, CPP = WG / D;
+    if (INIT && tp.bar && blockIdx.x == 0)  // the words the persistent launch behind it starts from
+        for (int i = threadIdx.x; i < tp.bar_words; i += WG) tp.bar[i] = 0u;
     const TPRec R = tp.rec[blockIdx.x];
     const int a = R.a, sg = R.sg;
     SegScalars& S = ta.seg[sg];
@@ -1194,7 +1198,7 @@ struct TPRunArgs {
                               //      or the aggregate's own node count + its index among the neighbours
     const int32_t* halo_ptr;  // [A+1] neighbouring nodes of every aggregate ...
     const int32_t* halo_pos;  //      ... as positions
-    unsigned* bar;            // [TPR_WORDS] barrier words, zero at the launch
+    unsigned* bar;            // [TPR_WORDS] barrier words, zero at the launch (the solve's first z clears them)
     int maxit;
     unsigned spin;            // polls of one barrier wait before it gives up
 };
@@ -1599,12 +1603,6 @@ __global__ __launch_bounds__(WG, 2) void k_tp_run(TileArgs ta, TPArgs tp, const 
         S.iters = it;
         if (aborted) { S.rnorm2 = INFINITY; S.stalled = 1; }  // a barrier that gave up: the host falls back
     }
-}
-
-__global__ void k_tpr_clear(unsigned* bar)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < TPR_WORDS) bar[i] = 0u;
 }
 
 static size_t tp_run_lds(int ldA, bool f64)
@@ -3329,7 +3327,7 @@ static TPArgs tp_args(tf_handle* h)
     tf_handle::TwoLevel& tl = h->tl;
     return TPArgs{tl.agg_nodes, tl.prow_ptr, tl.pcol, tl.pass_rows, reinterpret_cast<const TPRec*>(tl.wg_rec),
                   tl.pcoef_p, tl.dcinv, tl.dcinv64, reinterpret_cast<const __half2*>(tl.Ainv),
-                  tl.Ainv64, tl.seg_aoff64, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg,
+                  tl.Ainv64, tl.seg_aoff64, tl.ascale, tl.ptq, tl.rcx, tl.nc, tl.n_agg, tl.bar, TPR_WORDS,
                   h->opts.precond_fp64 ? 0 : 1};
 }
 
@@ -3707,8 +3705,7 @@ static void launch_tp_run(tf_handle* h)
     int idx = 0;
     const void* fn = tp_run_fn<VT>(h, idx);
     const size_t lds = tp_run_lds(tl.max_ldA, h->opts.precond_fp64 != 0);
-    hipLaunchKernelGGL(k_tpr_clear, dim3((TPR_WORDS + 255) / 256), dim3(256), 0, h->stream, tl.bar);
-    TF_HIP(hipGetLastError());
+    // the barrier words are zero: the solve's first z (k_tp_z<P, true>, enqueued before) cleared them
     TileArgs ta = tp_tile_args(h);
     TPArgs tp = tp_args(h);
     const VT* val = tp_values<VT>(h);
