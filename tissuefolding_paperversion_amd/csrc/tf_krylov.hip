@@ -2571,12 +2571,14 @@ struct WarmArgs {
     const double* const* z;  // [TF_WARM_HIST] the kept updates (a device table, newest first)
     const double* g;
     int m;
+    int cap;                 // blocks of a tile (the one-pass Gram kernel's LDS stride)
 };
 
 // per tile, in one launch: M z_i on the tile's rows for every kept update z_i (one CSR-stream
 // pass per update; the row results stay in registers, at most 4 rows x components per lane),
 // then the partials of (Mz_i, Mz_j) for i <= j in row-major upper-triangle order, then (Mz_i, g)
-template <typename VT>
+// ONE: all kept updates' products in one pass over the blocks (LDS for TF_WARM_HIST of them)
+template <typename VT, bool ONE>
 __global__ __launch_bounds__(WG) void k_warm_gram(TileArgs ta, WarmArgs wa, const int32_t* __restrict__ row_ptr,
                                                   const int32_t* __restrict__ col, const VT* __restrict__ val, int nb,
                                                   double* __restrict__ part)
@@ -2588,19 +2590,53 @@ __global__ __launch_bounds__(WG) void k_warm_gram(TileArgs ta, WarmArgs wa, cons
     const int b0 = row_ptr[r0], b1 = row_ptr[r1];
     const int ntask = 3 * (r1 - r0);
     double mz[TF_WARM_HIST][MAXT];
+    if (ONE) {
+        // one pass over the tile's blocks: every block's values read once for all kept updates,
+        // products of update i at prod + i * 3 * cap (the same products and row sums as below)
+        const int cap = wa.cap;
+        const double* z[TF_WARM_HIST];
 #pragma unroll
-    for (int i = 0; i < TF_WARM_HIST; ++i) {
+        for (int i = 0; i < TF_WARM_HIST; ++i) z[i] = i < wa.m ? wa.z[i] : nullptr;
+        for (int j = b0 + threadIdx.x; j < b1; j += WG) {
+            const int c = col[j];
+            const double v0 = val[j], v1 = val[(size_t)nb + j], v2 = val[2 * (size_t)nb + j];
+            const double v3 = val[3 * (size_t)nb + j], v4 = val[4 * (size_t)nb + j], v5 = val[5 * (size_t)nb + j];
+            const double v6 = val[6 * (size_t)nb + j], v7 = val[7 * (size_t)nb + j], v8 = val[8 * (size_t)nb + j];
 #pragma unroll
-        for (int k = 0; k < MAXT; ++k) mz[i][k] = 0.0;
-        if (i < wa.m) {
-            const double* z = wa.z[i];
-            tile_products<VT>(col, val, nb, b0, b1, XLOAD(z), prod);
+            for (int i = 0; i < TF_WARM_HIST; ++i) {
+                if (i >= wa.m) continue;
+                const double x0 = z[i][3 * c], x1 = z[i][3 * c + 1], x2 = z[i][3 * c + 2];
+                double* p = prod + (size_t)i * 3 * cap + 3 * (j - b0);
+                p[0] = v0 * x0 + v1 * x1 + v2 * x2;
+                p[1] = v3 * x0 + v4 * x1 + v5 * x2;
+                p[2] = v6 * x0 + v7 * x1 + v8 * x2;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TF_WARM_HIST; ++i)
 #pragma unroll
             for (int k = 0; k < MAXT; ++k) {
                 const int task = threadIdx.x + k * WG;
-                if (task < ntask) mz[i][k] = row_sum(row_ptr, r0 + task / 3, task % 3, b0, prod);
+                mz[i][k] = i < wa.m && task < ntask
+                               ? row_sum(row_ptr, r0 + task / 3, task % 3, b0, prod + (size_t)i * 3 * cap)
+                               : 0.0;
             }
-            __syncthreads();  // prod is reused by the next update
+    } else {
+#pragma unroll
+        for (int i = 0; i < TF_WARM_HIST; ++i) {
+#pragma unroll
+            for (int k = 0; k < MAXT; ++k) mz[i][k] = 0.0;
+            if (i < wa.m) {
+                const double* z = wa.z[i];
+                tile_products<VT>(col, val, nb, b0, b1, XLOAD(z), prod);
+#pragma unroll
+                for (int k = 0; k < MAXT; ++k) {
+                    const int task = threadIdx.x + k * WG;
+                    if (task < ntask) mz[i][k] = row_sum(row_ptr, r0 + task / 3, task % 3, b0, prod);
+                }
+                __syncthreads();  // prod is reused by the next update
+            }
         }
     }
     double acc[TF_WARM_DOTS];
@@ -2760,13 +2796,22 @@ void warm_start_launch(tf_handle* h, int k)
     wa.z = h->warm_tab + (size_t)k * TF_WARM_HIST;
     wa.g = h->g;
     wa.m = h->warm_count[k];
-    const size_t lds = 3 * sizeof(double) * (size_t)h->tile_cap;
-    if (fp32_values(h))
-        hipLaunchKernelGGL(k_warm_gram<float>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val32,
-                           h->nb, h->warm_part);
-    else
-        hipLaunchKernelGGL(k_warm_gram<double>, dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr, h->col, h->val,
-                           h->nb, h->warm_part);
+    wa.cap = h->tile_cap;
+    // one pass over the blocks when the products of every kept update fit in 64 KB of LDS
+    const size_t lds1 = 3 * sizeof(double) * (size_t)h->tile_cap;
+    const bool one = TF_WARM_HIST * lds1 <= 65536;
+    const size_t lds = one ? TF_WARM_HIST * lds1 : lds1;
+    if (fp32_values(h)) {
+        if (one) hipLaunchKernelGGL((k_warm_gram<float, true>), dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr,
+                                    h->col, h->val32, h->nb, h->warm_part);
+        else hipLaunchKernelGGL((k_warm_gram<float, false>), dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr,
+                                h->col, h->val32, h->nb, h->warm_part);
+    } else {
+        if (one) hipLaunchKernelGGL((k_warm_gram<double, true>), dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr,
+                                    h->col, h->val, h->nb, h->warm_part);
+        else hipLaunchKernelGGL((k_warm_gram<double, false>), dim3(h->T), dim3(WG), lds, h->stream, ta, wa, h->row_ptr,
+                                h->col, h->val, h->nb, h->warm_part);
+    }
     hipLaunchKernelGGL(k_warm_seg, dim3(h->S, TF_WARM_DOTS), dim3(64), 0, h->stream, ta, h->warm_part, h->warm_seg);
     hipLaunchKernelGGL(k_warm_combine, dim3(h->T), dim3(WG), 0, h->stream, ta, wa, h->warm_seg, h->warm_x0);
     TF_HIP(hipGetLastError());
